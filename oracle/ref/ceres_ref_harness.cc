@@ -1,0 +1,283 @@
+// Harness around the reference's own Ceres Solver 1.10 (compiled from
+// 3rdparty/ceres, 3rdparty/cxsparse and 3rdparty/eigen3 of the reference by
+// oracle/ref/Makefile into oracle/_ref/).  TEST INFRASTRUCTURE ONLY: it is
+// used in this container to pin the FP64 oracle and to write the golden
+// fixtures under tests/golden/.  Nothing in the product links it.
+//
+// The libs/sfm residual functor (libs/sfm/include/mvg/sfm/
+// pinhole_brown_rt_ceres_functor.h:60-107) cannot be included directly: its
+// header chain needs the CMake-generated mvg/config.h.  Its arithmetic
+// (ceres::AngleAxisRotatePoint from the reference's Ceres headers + the
+// radial polynomial of brown_pinhole_camera.h:14-41) is therefore written
+// out below, operation for operation, as a Ceres autodiff functor.
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ceres/ceres.h"
+#include "ceres/rotation.h"
+#include "ceres/problem_impl.h"
+#include "ceres/program.h"
+#include "ceres/parameter_block.h"
+#include "ceres/residual_block.h"
+#include "ceres/reorder_program.h"
+#include "ceres/ordered_groups.h"
+#include "ceres/evaluator.h"
+#include "ceres/block_sparse_matrix.h"
+#include "ceres/linear_solver.h"
+#define private public
+#define protected public
+#include "ceres/schur_complement_solver.h"
+#include "ceres/block_random_access_sparse_matrix.h"
+#undef private
+#undef protected
+
+namespace {
+
+// [aa(3), t(3), f, k1, k2] camera; intrinsics [f, 0, 0, k1, k2, 0].
+struct Packed9Functor {
+  explicit Packed9Functor(const double* o) { obs[0] = o[0]; obs[1] = o[1]; }
+  template <typename T>
+  bool operator()(const T* const cam, const T* const X, T* r) const {
+    T x[3];
+    ceres::AngleAxisRotatePoint(cam, X, x);
+    x[0] += cam[3];
+    x[1] += cam[4];
+    x[2] += cam[5];
+    const T xe = x[0] / x[2];
+    const T ye = x[1] / x[2];
+    const T& f = cam[6];
+    const T ppx = T(0.0), ppy = T(0.0), k3 = T(0.0);
+    const T& k1 = cam[7];
+    const T& k2 = cam[8];
+    const T r2 = xe * xe + ye * ye;
+    const T r4 = r2 * r2;
+    const T r6 = r4 * r2;
+    const T rc = (T(1) + k1 * r2 + k2 * r4 + k3 * r6);
+    const T xd = xe * rc;
+    const T yd = ye * rc;
+    const T px = ppx + f * xd;
+    const T py = ppy + f * yd;
+    r[0] = px - T(obs[0]);
+    r[1] = py - T(obs[1]);
+    return true;
+  }
+  double obs[2];
+};
+
+// Intrinsic block [f, ppx, ppy, k1, k2, k3], extrinsic [aa, t], point.
+struct Intr6Ext6Functor {
+  explicit Intr6Ext6Functor(const double* o) { obs[0] = o[0]; obs[1] = o[1]; }
+  template <typename T>
+  bool operator()(const T* const in, const T* const rt, const T* const X, T* r) const {
+    T x[3];
+    ceres::AngleAxisRotatePoint(rt, X, x);
+    x[0] += rt[3];
+    x[1] += rt[4];
+    x[2] += rt[5];
+    const T xe = x[0] / x[2];
+    const T ye = x[1] / x[2];
+    const T r2 = xe * xe + ye * ye;
+    const T r4 = r2 * r2;
+    const T r6 = r4 * r2;
+    const T rc = (T(1) + in[3] * r2 + in[4] * r4 + in[5] * r6);
+    const T xd = xe * rc;
+    const T yd = ye * rc;
+    const T px = in[1] + in[0] * xd;
+    const T py = in[2] + in[0] * yd;
+    r[0] = px - T(obs[0]);
+    r[1] = py - T(obs[1]);
+    return true;
+  }
+  double obs[2];
+};
+
+struct Scene {
+  int layout, nc, ni, np;
+  long nobs;
+  double *cams, *intr, *pts;
+  const double* obs;
+  const int *cam_idx, *intr_idx, *pt_idx;
+  const unsigned char* cam_const;
+};
+
+ceres::CostFunction* MakeCost(const Scene& s, long i) {
+  if (s.layout == 0)
+    return new ceres::AutoDiffCostFunction<Packed9Functor, 2, 9, 3>(new Packed9Functor(s.obs + 2 * i));
+  return new ceres::AutoDiffCostFunction<Intr6Ext6Functor, 2, 6, 6, 3>(new Intr6Ext6Functor(s.obs + 2 * i));
+}
+
+// Block id of a user-state pointer: layout 0 cameras then points; layout 1
+// intrinsics, extrinsics, points.
+int BlockId(const Scene& s, const double* p) {
+  const int cs = s.layout == 0 ? 9 : 6;
+  if (s.layout == 1 && p >= s.intr && p < s.intr + 6 * s.ni) return (int)((p - s.intr) / 6);
+  const int ni = s.layout == 1 ? s.ni : 0;
+  if (p >= s.cams && p < s.cams + (long)cs * s.nc) return ni + (int)((p - s.cams) / cs);
+  return ni + s.nc + (int)((p - s.pts) / 3);
+}
+
+template <typename P>
+void AddAll(const Scene& s, P* problem, ceres::LossFunction* loss,
+            std::map<const ceres::CostFunction*, long>* ids) {
+  const int cs = s.layout == 0 ? 9 : 6;
+  for (long i = 0; i < s.nobs; ++i) {
+    ceres::CostFunction* cf = MakeCost(s, i);
+    if (ids) (*ids)[cf] = i;
+    double* cam = s.cams + (long)cs * s.cam_idx[i];
+    double* pt = s.pts + 3L * s.pt_idx[i];
+    if (s.layout == 0) {
+      problem->AddResidualBlock(cf, loss, cam, pt);
+    } else {
+      problem->AddResidualBlock(cf, loss, s.intr + 6L * s.intr_idx[i], cam, pt);
+    }
+  }
+  if (s.cam_const)
+    for (int c = 0; c < s.nc; ++c)
+      if (s.cam_const[c]) problem->SetParameterBlockConstant(s.cams + (long)cs * c);
+}
+
+}  // namespace
+
+extern "C" {
+
+// AutoDiffCostFunction::Evaluate for one observation (no loss).
+int ref_evaluate_obs(int layout, const double* intr, const double* cam, const double* pt,
+                     const double* obs, double* residual, double* j0, double* j1, double* j2) {
+  if (layout == 0) {
+    ceres::AutoDiffCostFunction<Packed9Functor, 2, 9, 3> cf(new Packed9Functor(obs));
+    const double* params[2] = {cam, pt};
+    double* jac[2] = {j0, j1};
+    return cf.Evaluate(params, residual, (j0 || j1) ? jac : NULL) ? 1 : 0;
+  }
+  ceres::AutoDiffCostFunction<Intr6Ext6Functor, 2, 6, 6, 3> cf(new Intr6Ext6Functor(obs));
+  const double* params[3] = {intr, cam, pt};
+  double* jac[3] = {j0, j1, j2};
+  return cf.Evaluate(params, residual, (j0 || j1 || j2) ? jac : NULL) ? 1 : 0;
+}
+
+// Schur ordering of the reduced program through Ceres's own preprocessing
+// calls (program.cc:269, reorder_program.cc:436-531).  Returns num E blocks.
+int ref_schur_ordering(int layout, int nc, int ni, int np, long nobs, double* cams, double* intr,
+                       double* pts, const double* obs, const int* cam_idx, const int* intr_idx,
+                       const int* pt_idx, const unsigned char* cam_const, int* param_order,
+                       int* num_blocks, long* residual_order, long* num_residuals,
+                       int* pairs, long pairs_cap, long* num_pairs) {
+  Scene s = {layout, nc, ni, np, nobs, cams, intr, pts, obs, cam_idx, intr_idx, pt_idx, cam_const};
+  ceres::Problem::Options po;
+  ceres::internal::ProblemImpl problem(po);
+  std::map<const ceres::CostFunction*, long> ids;
+  AddAll(s, &problem, NULL, &ids);
+  std::vector<double*> removed;
+  double fixed_cost = 0.0;
+  std::string error;
+  ceres::internal::scoped_ptr<ceres::internal::Program> reduced(
+      problem.mutable_program()->CreateReducedProgram(&removed, &fixed_cost, &error));
+  if (reduced.get() == NULL) return -1;
+  ceres::ParameterBlockOrdering ordering;
+  for (size_t i = 0; i < reduced->parameter_blocks().size(); ++i)
+    ordering.AddElementToGroup(reduced->parameter_blocks()[i]->mutable_user_state(), 0);
+  if (!ceres::internal::ReorderProgramForSchurTypeLinearSolver(
+          ceres::SPARSE_SCHUR, ceres::CX_SPARSE, problem.parameter_map(), &ordering,
+          reduced.get(), &error))
+    return -2;
+  const int ne = ordering.group_to_elements().begin()->second.size();
+  const std::vector<ceres::internal::ParameterBlock*>& pbs = reduced->parameter_blocks();
+  for (size_t i = 0; i < pbs.size(); ++i) param_order[i] = BlockId(s, pbs[i]->user_state());
+  *num_blocks = (int)pbs.size();
+  const std::vector<ceres::internal::ResidualBlock*>& rbs = reduced->residual_blocks();
+  for (size_t i = 0; i < rbs.size(); ++i) residual_order[i] = ids[rbs[i]->cost_function()];
+  *num_residuals = (long)rbs.size();
+
+  // S block pairs from SparseSchurComplementSolver::InitStorage itself.
+  if (pairs != NULL || num_pairs != NULL) {
+    ceres::internal::Evaluator::Options eo;
+    eo.linear_solver_type = ceres::SPARSE_SCHUR;
+    eo.num_eliminate_blocks = ne;
+    eo.num_threads = 1;
+    ceres::internal::scoped_ptr<ceres::internal::Evaluator> ev(
+        ceres::internal::Evaluator::Create(eo, reduced.get(), &error));
+    ceres::internal::scoped_ptr<ceres::internal::SparseMatrix> jac(ev->CreateJacobian());
+    ceres::internal::BlockSparseMatrix* bsm =
+        static_cast<ceres::internal::BlockSparseMatrix*>(jac.get());
+    ceres::internal::LinearSolver::Options lo;
+    lo.type = ceres::SPARSE_SCHUR;
+    lo.sparse_linear_algebra_library_type = ceres::CX_SPARSE;
+    lo.elimination_groups.push_back(ne);
+    lo.elimination_groups.push_back((int)pbs.size() - ne);
+    ceres::internal::SparseSchurComplementSolver solver(lo);
+    solver.InitStorage(bsm->block_structure());
+    const ceres::internal::BlockRandomAccessSparseMatrix* m =
+        static_cast<const ceres::internal::BlockRandomAccessSparseMatrix*>(solver.lhs());
+    const long np_ = (long)m->cell_values_.size();
+    if (num_pairs) *num_pairs = np_;
+    if (pairs) {
+      if (np_ > pairs_cap) return -3;
+      for (long i = 0; i < np_; ++i) {
+        pairs[2 * i] = m->cell_values_[i].first.first;
+        pairs[2 * i + 1] = m->cell_values_[i].first.second;
+      }
+    }
+  }
+  return ne;
+}
+
+// ceres::Solve as libs/sfm configures it (sfm_incremental_engine.cpp:1449-1472):
+// linear_solver 0 = SPARSE_SCHUR + CX_SPARSE, 1 = ITERATIVE_SCHUR + JACOBI,
+// 2 = DENSE_SCHUR.  summary[8] = {initial_cost, final_cost, successful,
+// unsuccessful, termination_type, iterations recorded, total_time,
+// linear_solver_time}.  iters[9 * k] = {iteration, cost, cost_change,
+// radius, successful, valid, linear iterations, relative_decrease,
+// gradient_max_norm}.
+int ref_solve(int layout, int nc, int ni, int np, long nobs, double* cams, double* intr,
+              double* pts, const double* obs, const int* cam_idx, const int* intr_idx,
+              const int* pt_idx, const unsigned char* cam_const, double huber_a,
+              int linear_solver, int max_iters, int num_threads, double* summary,
+              double* iters, int max_rec) {
+  Scene s = {layout, nc, ni, np, nobs, cams, intr, pts, obs, cam_idx, intr_idx, pt_idx, cam_const};
+  ceres::Problem problem;
+  ceres::LossFunction* loss = huber_a > 0.0 ? new ceres::HuberLoss(huber_a) : NULL;
+  AddAll(s, &problem, loss, NULL);
+  ceres::Solver::Options options;
+  options.preconditioner_type = ceres::JACOBI;
+  options.linear_solver_type = linear_solver == 1 ? ceres::ITERATIVE_SCHUR
+                               : linear_solver == 2 ? ceres::DENSE_SCHUR
+                                                    : ceres::SPARSE_SCHUR;
+  options.sparse_linear_algebra_library_type = ceres::CX_SPARSE;
+  options.minimizer_progress_to_stdout = false;
+  options.logging_type = ceres::SILENT;
+  options.max_num_iterations = max_iters;
+  options.num_threads = num_threads;
+  options.num_linear_solver_threads = num_threads;
+  ceres::Solver::Summary sum;
+  ceres::Solve(options, &problem, &sum);
+  summary[0] = sum.initial_cost;
+  summary[1] = sum.final_cost;
+  summary[2] = sum.num_successful_steps;
+  summary[3] = sum.num_unsuccessful_steps;
+  summary[4] = sum.termination_type;
+  summary[5] = (double)sum.iterations.size();
+  summary[6] = sum.total_time_in_seconds;
+  summary[7] = sum.linear_solver_time_in_seconds;
+  for (int k = 0; k < (int)sum.iterations.size() && k < max_rec; ++k) {
+    const ceres::IterationSummary& it = sum.iterations[k];
+    double* o = iters + 9 * k;
+    o[0] = it.iteration;
+    o[1] = it.cost;
+    o[2] = it.cost_change;
+    o[3] = it.trust_region_radius;
+    o[4] = it.step_is_successful;
+    o[5] = it.step_is_valid;
+    o[6] = it.linear_solver_iterations;
+    o[7] = it.relative_decrease;
+    o[8] = it.gradient_max_norm;
+  }
+  return sum.IsSolutionUsable() ? 1 : 0;
+}
+
+}  // extern "C"
