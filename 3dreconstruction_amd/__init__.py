@@ -1,0 +1,472 @@
+"""MI355X bundle-adjustment engine: Python host API over the C ABI (``include/mba.h``).
+
+The package name starts with a digit, so import it with
+``importlib.import_module("3dreconstruction_amd")``.
+
+Two layers:
+
+* ``BAProblem`` / ``solve`` / ``evaluate`` / ``schur_structure`` / ``Context``
+  -- array-level calls straight onto the C ABI;
+* ``Problem`` / ``SolverOptions`` / ``Solve`` / ``Summary`` / ``HuberLoss`` --
+  the shape of the Ceres calls libs/sfm makes for bundle adjustment
+  (libs/sfm/src/sfm_incremental_engine.cpp:1413-1483): add one residual block
+  per observation with a shared loss, mark camera blocks constant, set the
+  solver options, solve, read ``initial_cost`` / ``final_cost`` /
+  ``IsSolutionUsable()``; parameter blocks are numpy arrays updated in place
+  when the solution is usable.
+
+The engine runs only through ``libmba.so`` (HIP, gfx950).  If the library is
+missing every call raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmba.so")
+
+# ------------------------------------------------------------- constants --
+OK = 0
+ERR_INVALID_ARGUMENT, ERR_UNSUPPORTED, ERR_DEVICE, ERR_OUT_OF_MEMORY, ERR_NOT_FINITE = -1, -2, -3, -4, -5
+CONVERGENCE, NO_CONVERGENCE, FAILURE = 0, 1, 2
+LAYOUT_PACKED9 = 0
+EXPLICIT_SCHUR, IMPLICIT_SCHUR_PCG = 0, 1
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+# Ceres names for the linear solvers libs/sfm selects.
+SPARSE_SCHUR = EXPLICIT_SCHUR
+DENSE_SCHUR = EXPLICIT_SCHUR
+ITERATIVE_SCHUR = IMPLICIT_SCHUR_PCG
+
+_TERMINATION_NAMES = {CONVERGENCE: "CONVERGENCE", NO_CONVERGENCE: "NO_CONVERGENCE", FAILURE: "FAILURE"}
+
+_f64p = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class MbaError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"mba status {status}: {message}")
+        self.status = status
+
+
+class _Problem(C.Structure):
+    _fields_ = [("layout", C.c_int32), ("num_cameras", C.c_int32), ("num_points", C.c_int32),
+                ("reserved0", C.c_int32), ("num_observations", C.c_int64), ("cameras", _f64p),
+                ("points", _f64p), ("observations", _f64p), ("camera_index", _i32p),
+                ("point_index", _i32p), ("camera_constant", _u8p)]
+
+
+class Options(C.Structure):
+    _fields_ = [("max_num_iterations", C.c_int32), ("max_num_consecutive_invalid_steps", C.c_int32),
+                ("max_linear_solver_iterations", C.c_int32), ("min_linear_solver_iterations", C.c_int32),
+                ("jacobi_scaling", C.c_int32), ("linear_solver", C.c_int32), ("precision", C.c_int32),
+                ("device", C.c_int32), ("function_tolerance", C.c_double),
+                ("gradient_tolerance", C.c_double), ("parameter_tolerance", C.c_double),
+                ("initial_trust_region_radius", C.c_double), ("max_trust_region_radius", C.c_double),
+                ("min_trust_region_radius", C.c_double), ("min_relative_decrease", C.c_double),
+                ("min_lm_diagonal", C.c_double), ("max_lm_diagonal", C.c_double), ("eta", C.c_double),
+                ("pcg_r_tolerance", C.c_double), ("huber_a", C.c_double),
+                ("refinement_iterations", C.c_int32), ("reserved1", C.c_int32)]
+
+
+class Iteration(C.Structure):
+    _fields_ = [("iteration", C.c_int32), ("step_is_valid", C.c_int32), ("step_is_successful", C.c_int32),
+                ("linear_solver_iterations", C.c_int32), ("cost", C.c_double), ("cost_change", C.c_double),
+                ("gradient_max_norm", C.c_double), ("gradient_norm", C.c_double),
+                ("step_norm", C.c_double), ("relative_decrease", C.c_double),
+                ("trust_region_radius", C.c_double), ("cumulative_time_s", C.c_double)]
+
+
+class _Summary(C.Structure):
+    _fields_ = [("initial_cost", C.c_double), ("final_cost", C.c_double), ("fixed_cost", C.c_double),
+                ("preprocessor_time_s", C.c_double), ("jacobian_evaluation_time_s", C.c_double),
+                ("residual_evaluation_time_s", C.c_double), ("linear_solver_time_s", C.c_double),
+                ("total_time_s", C.c_double), ("num_successful_steps", C.c_int32),
+                ("num_unsuccessful_steps", C.c_int32), ("num_iterations", C.c_int32),
+                ("termination_type", C.c_int32), ("num_e_blocks", C.c_int32), ("num_f_blocks", C.c_int32),
+                ("num_linear_solver_iterations", C.c_int32), ("is_solution_usable", C.c_int32)]
+
+
+# ------------------------------------------------------------- library --
+_lib = None
+
+#: every exported symbol of include/mba.h
+EXPORTS = [
+    "mba_default_options", "mba_solve", "mba_evaluate", "mba_schur_structure",
+    "mba_context_create", "mba_context_destroy", "mba_context_jacobian_schur",
+    "mba_context_read_schur", "mba_context_read_scale", "mba_context_schur_matvec",
+    "mba_context_solve", "mba_context_download", "mba_context_synchronize", "mba_context_stream",
+    "mba_context_set_timing", "mba_context_last_timing", "mba_status_string", "mba_last_error",
+    "mba_abi_version", "mba_scene_generate",
+]
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load_library():
+    """Load libmba.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(python -m 3dreconstruction_amd.build is not importable; run build.py)")
+    lib = C.CDLL(LIB_PATH)
+    lib.mba_status_string.restype = C.c_char_p
+    lib.mba_last_error.restype = C.c_char_p
+    lib.mba_context_stream.restype = C.c_void_p
+    lib.mba_context_create.argtypes = [C.POINTER(_Problem), C.POINTER(Options), C.POINTER(C.c_void_p)]
+    for name in ["mba_context_destroy", "mba_context_synchronize"]:
+        getattr(lib, name).argtypes = [C.c_void_p]
+    lib.mba_context_jacobian_schur.argtypes = [C.c_void_p, C.c_double]
+    lib.mba_context_set_timing.argtypes = [C.c_void_p, C.c_int32]
+    lib.mba_context_last_timing.argtypes = [C.c_void_p, _f64p, C.c_int32]
+    lib.mba_context_read_schur.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p, _f64p]
+    lib.mba_context_read_scale.argtypes = [C.c_void_p, _f64p, _f64p]
+    lib.mba_context_schur_matvec.argtypes = [C.c_void_p, _f64p, _f64p]
+    lib.mba_context_solve.argtypes = [C.c_void_p, C.POINTER(_Summary), C.POINTER(Iteration), C.c_int32]
+    lib.mba_context_download.argtypes = [C.c_void_p, C.POINTER(_Problem)]
+    lib.mba_context_stream.argtypes = [C.c_void_p]
+    lib.mba_evaluate.argtypes = [C.POINTER(_Problem), C.c_int32, C.c_double, _f64p, _f64p, _f64p, _f64p, C.c_int32]
+    lib.mba_scene_generate.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _i64p, _f64p, _f64p, _f64p,
+                                       _f64p, _f64p, _i32p, _i32p]
+    _lib = lib
+    return lib
+
+
+def _check(status):
+    if status != OK:
+        lib = load_library()
+        raise MbaError(status, lib.mba_last_error().decode(errors="replace"))
+
+
+def _p(a, t=_f64p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    _check(load_library().mba_default_options(C.byref(o)))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
+
+
+# ------------------------------------------------------------ problems --
+class BAProblem:
+    """Arrays of a packed-9 bundle-adjustment problem (caller-owned)."""
+
+    def __init__(self, cameras, points, observations, camera_index, point_index, camera_constant=None):
+        self.cameras = np.ascontiguousarray(cameras, np.float64)
+        self.points = np.ascontiguousarray(points, np.float64)
+        self.observations = np.ascontiguousarray(observations, np.float64)
+        self.camera_index = np.ascontiguousarray(camera_index, np.int32)
+        self.point_index = np.ascontiguousarray(point_index, np.int32)
+        self.camera_constant = (None if camera_constant is None
+                                else np.ascontiguousarray(camera_constant, np.uint8))
+        if self.cameras.ndim != 2 or self.cameras.shape[1] != 9:
+            raise ValueError("cameras must be (Nc, 9)")
+        if self.points.ndim != 2 or self.points.shape[1] != 3:
+            raise ValueError("points must be (Np, 3)")
+        if self.observations.shape != (self.camera_index.shape[0], 2):
+            raise ValueError("observations must be (Nobs, 2) matching the index arrays")
+
+    @property
+    def num_cameras(self):
+        return self.cameras.shape[0]
+
+    @property
+    def num_points(self):
+        return self.points.shape[0]
+
+    @property
+    def num_observations(self):
+        return self.observations.shape[0]
+
+    def _struct(self) -> _Problem:
+        s = _Problem()
+        s.layout = LAYOUT_PACKED9
+        s.num_cameras, s.num_points, s.num_observations = self.num_cameras, self.num_points, self.num_observations
+        s.cameras, s.points, s.observations = _p(self.cameras), _p(self.points), _p(self.observations)
+        s.camera_index, s.point_index = _p(self.camera_index, _i32p), _p(self.point_index, _i32p)
+        s.camera_constant = _p(self.camera_constant, _u8p)
+        return s
+
+
+class Summary:
+    """Solver::Summary subset (include/ceres/solver.h:753-831)."""
+
+    def __init__(self, s: _Summary, iterations):
+        for name, _ in _Summary._fields_:
+            setattr(self, name, getattr(s, name))
+        self.iterations = iterations
+
+    def IsSolutionUsable(self) -> bool:
+        return self.termination_type in (CONVERGENCE, NO_CONVERGENCE)
+
+    def FullReport(self) -> str:
+        return (f"MI355X bundle adjustment: blocks E={self.num_e_blocks} F={self.num_f_blocks}\n"
+                f"  initial cost {self.initial_cost:.6e}  final cost {self.final_cost:.6e}\n"
+                f"  steps: {self.num_successful_steps} successful, {self.num_unsuccessful_steps} unsuccessful\n"
+                f"  linear solver iterations {self.num_linear_solver_iterations}\n"
+                f"  time: total {self.total_time_s:.4f}s, jacobian {self.jacobian_evaluation_time_s:.4f}s, "
+                f"residual {self.residual_evaluation_time_s:.4f}s, linear {self.linear_solver_time_s:.4f}s\n"
+                f"  termination: {_TERMINATION_NAMES.get(self.termination_type, self.termination_type)}")
+
+
+def solve(problem: BAProblem, options: Options | None = None, max_records: int = 1000) -> Summary:
+    """mba_solve: parameters updated in place when the solution is usable."""
+    lib = load_library()
+    o = options if options is not None else default_options()
+    s = problem._struct()
+    summ = _Summary()
+    its = (Iteration * max_records)()
+    _check(lib.mba_solve(C.byref(o), C.byref(s), C.byref(summ), its, C.c_int32(max_records)))
+    n = min(summ.num_iterations, max_records)
+    return Summary(summ, [its[i] for i in range(n)])
+
+
+def evaluate(problem: BAProblem, precision=PRECISION_FP64, huber_a=0.0, jacobians=True, device=0):
+    """CostFunction::Evaluate for every observation (input order) on the GPU."""
+    lib = load_library()
+    n = problem.num_observations
+    r = np.zeros((n, 2))
+    jc = np.zeros((n, 2, 9)) if jacobians else None
+    jp = np.zeros((n, 2, 3)) if jacobians else None
+    cost = C.c_double()
+    s = problem._struct()
+    _check(lib.mba_evaluate(C.byref(s), C.c_int32(precision), C.c_double(huber_a), _p(r), _p(jc), _p(jp),
+                            C.byref(cost), C.c_int32(device)))
+    return r, jc, jp, cost.value
+
+
+def schur_structure(problem: BAProblem, with_pairs=True):
+    """(num_e, column-block order, residual order, S block pairs)."""
+    lib = load_library()
+    s = problem._struct()
+    nb = problem.num_cameras + problem.num_points
+    order = np.zeros(nb, np.int32)
+    res = np.zeros(problem.num_observations, np.int64)
+    nblk, ne, npairs = C.c_int32(), C.c_int32(), C.c_int64()
+    _check(lib.mba_schur_structure(C.byref(s), _p(order, _i32p), C.byref(nblk), C.byref(ne), _p(res, _i64p),
+                                   None, C.c_int64(0), C.byref(npairs) if with_pairs else None))
+    pairs = None
+    if with_pairs:
+        pairs = np.zeros((npairs.value, 2), np.int32)
+        _check(lib.mba_schur_structure(C.byref(s), None, None, None, None, _p(pairs, _i32p),
+                                       C.c_int64(npairs.value), None))
+    return ne.value, order[:nblk.value], res, pairs
+
+
+class Context:
+    """A preprocessed, HBM-resident problem (mba_context_*)."""
+
+    def __init__(self, problem: BAProblem, options: Options | None = None):
+        self._lib = load_library()
+        self.problem = problem
+        self.options = options if options is not None else default_options()
+        self._s = problem._struct()
+        h = C.c_void_p()
+        _check(self._lib.mba_context_create(C.byref(self._s), C.byref(self.options), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self._lib.mba_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return self._lib.mba_context_stream(self.handle) or 0
+
+    def jacobian_schur(self, radius: float):
+        _check(self._lib.mba_context_jacobian_schur(self.handle, C.c_double(radius)))
+
+    def synchronize(self):
+        _check(self._lib.mba_context_synchronize(self.handle))
+
+    def set_timing(self, on: bool):
+        _check(self._lib.mba_context_set_timing(self.handle, 1 if on else 0))
+
+    def last_timing(self):
+        ms = np.zeros(3)
+        _check(self._lib.mba_context_last_timing(self.handle, _p(ms), 3))
+        return ms
+
+    def read_schur(self):
+        nc, np_ = self.problem.num_cameras, self.problem.num_points
+        out = dict(U=np.zeros((nc, 45)), gc=np.zeros((nc, 9)), rhs=np.zeros((nc, 9)),
+                   V=np.zeros((np_, 6)), gp=np.zeros((np_, 3)))
+        _check(self._lib.mba_context_read_schur(self.handle, _p(out["U"]), _p(out["gc"]), _p(out["rhs"]),
+                                                _p(out["V"]), _p(out["gp"])))
+        return out
+
+    def read_scale(self):
+        cs = np.zeros((self.problem.num_cameras, 9))
+        ps = np.zeros((self.problem.num_points, 3))
+        _check(self._lib.mba_context_read_scale(self.handle, _p(cs), _p(ps)))
+        return cs, ps
+
+    def schur_matvec(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        _check(self._lib.mba_context_schur_matvec(self.handle, _p(x), _p(y)))
+        return y
+
+    def solve(self, max_records=1000) -> Summary:
+        summ = _Summary()
+        its = (Iteration * max_records)()
+        _check(self._lib.mba_context_solve(self.handle, C.byref(summ), its, C.c_int32(max_records)))
+        n = min(summ.num_iterations, max_records)
+        return Summary(summ, [its[i] for i in range(n)])
+
+    def download(self):
+        _check(self._lib.mba_context_download(self.handle, C.byref(self._s)))
+
+
+def generate_scene(preset: int, window: int = 0) -> dict:
+    """Seeded benchmark scene (preset 1..5) generated by the library."""
+    lib = load_library()
+    nc, np_, no = C.c_int32(), C.c_int32(), C.c_int64()
+    _check(lib.mba_scene_generate(preset, window, C.byref(nc), C.byref(np_), C.byref(no),
+                                  None, None, None, None, None, None, None))
+    d = dict(cam_true=np.zeros((nc.value, 9)), cam_init=np.zeros((nc.value, 9)),
+             pt_true=np.zeros((np_.value, 3)), pt_init=np.zeros((np_.value, 3)),
+             obs=np.zeros((no.value, 2)), cam_idx=np.zeros(no.value, np.int32),
+             pt_idx=np.zeros(no.value, np.int32))
+    _check(lib.mba_scene_generate(preset, window, C.byref(nc), C.byref(np_), C.byref(no),
+                                  _p(d["cam_true"]), _p(d["cam_init"]), _p(d["pt_true"]), _p(d["pt_init"]),
+                                  _p(d["obs"]), _p(d["cam_idx"], _i32p), _p(d["pt_idx"], _i32p)))
+    return d
+
+
+# ------------------------------------------------ Ceres-shaped host API --
+class HuberLoss:
+    """ceres::HuberLoss(a): threshold a pixels (include/ceres/loss_function.h:175-184)."""
+
+    def __init__(self, a: float):
+        self.a = float(a)
+
+
+class ReprojectionError:
+    """The libs/sfm reprojection functor for one observation
+    (pinhole_brown_rt_ceres_functor.h:42-110), packed 9-parameter camera."""
+
+    def __init__(self, observed_xy):
+        self.observed = np.asarray(observed_xy, np.float64).reshape(2)
+
+
+class SolverOptions:
+    """Solver::Options fields libs/sfm sets (sfm_incremental_engine.cpp:1449-1468)."""
+
+    def __init__(self):
+        o = default_options()
+        self.linear_solver_type = SPARSE_SCHUR
+        self.precision = PRECISION_FP64
+        self.max_num_iterations = o.max_num_iterations
+        self.function_tolerance = o.function_tolerance
+        self.gradient_tolerance = o.gradient_tolerance
+        self.parameter_tolerance = o.parameter_tolerance
+        self.initial_trust_region_radius = o.initial_trust_region_radius
+        self.max_linear_solver_iterations = o.max_linear_solver_iterations
+        self.eta = o.eta
+        self.pcg_r_tolerance = o.pcg_r_tolerance
+        self.device = 0
+        self.minimizer_progress_to_stdout = False
+
+    def _to_c(self, huber_a) -> Options:
+        return default_options(
+            linear_solver=self.linear_solver_type, precision=self.precision,
+            max_num_iterations=self.max_num_iterations, function_tolerance=self.function_tolerance,
+            gradient_tolerance=self.gradient_tolerance, parameter_tolerance=self.parameter_tolerance,
+            initial_trust_region_radius=self.initial_trust_region_radius,
+            max_linear_solver_iterations=self.max_linear_solver_iterations, eta=self.eta,
+            pcg_r_tolerance=self.pcg_r_tolerance, device=self.device, huber_a=huber_a)
+
+
+class Problem:
+    """ceres::Problem for bundle adjustment: camera blocks of 9 and point
+    blocks of 3 (numpy float64 arrays, updated in place by Solve)."""
+
+    def __init__(self):
+        self._cams, self._pts = [], []
+        self._cam_id, self._pt_id = {}, {}
+        self._obs, self._ci, self._pi = [], [], []
+        self._loss = None
+        self._constant = set()
+
+    def _block(self, arr, size, table, ids):
+        if not isinstance(arr, np.ndarray) or arr.dtype != np.float64 or arr.size != size:
+            raise ValueError(f"parameter block must be a float64 array of {size}")
+        key = arr.__array_interface__["data"][0]
+        if key not in ids:
+            ids[key] = len(table)
+            table.append(arr)
+        return ids[key]
+
+    def AddResidualBlock(self, cost_function: ReprojectionError, loss_function, camera, point):
+        if loss_function is not None:
+            if self._loss is not None and loss_function is not self._loss and loss_function.a != self._loss.a:
+                raise ValueError("all residual blocks must share one loss function")
+            self._loss = loss_function
+        c = self._block(camera, 9, self._cams, self._cam_id)
+        p = self._block(point, 3, self._pts, self._pt_id)
+        self._obs.append(cost_function.observed)
+        self._ci.append(c)
+        self._pi.append(p)
+
+    def SetParameterBlockConstant(self, camera):
+        key = camera.__array_interface__["data"][0]
+        if key not in self._cam_id:
+            raise ValueError("unknown camera block")
+        self._constant.add(self._cam_id[key])
+
+    def NumResidualBlocks(self):
+        return len(self._obs)
+
+    def _arrays(self) -> BAProblem:
+        cams = np.array([c.reshape(9) for c in self._cams], np.float64).reshape(-1, 9)
+        pts = np.array([p.reshape(3) for p in self._pts], np.float64).reshape(-1, 3)
+        const = None
+        if self._constant:
+            const = np.zeros(len(self._cams), np.uint8)
+            const[list(self._constant)] = 1
+        return BAProblem(cams, pts, np.array(self._obs, np.float64).reshape(-1, 2),
+                         np.array(self._ci, np.int32), np.array(self._pi, np.int32), const)
+
+
+def Solve(options: SolverOptions, problem: Problem, summary_out: list | None = None) -> Summary:
+    """ceres::Solve: writes the parameter blocks back when usable."""
+    arrays = problem._arrays()
+    huber = problem._loss.a if problem._loss is not None else 0.0
+    summary = solve(arrays, options._to_c(huber))
+    if summary.IsSolutionUsable():
+        for i, c in enumerate(problem._cams):
+            c.reshape(9)[:] = arrays.cameras[i]
+        for i, p in enumerate(problem._pts):
+            p.reshape(3)[:] = arrays.points[i]
+    if summary_out is not None:
+        summary_out.append(summary)
+    return summary

@@ -1,0 +1,1403 @@
+// HIP kernels of the bundle-adjustment engine (gfx950, wave64).
+//
+// Precision: every per-observation kernel is templated on T (float for the
+// mixed path, double for the FP64 path).  Point-block inverses, the LM cost
+// and the model-cost change are always FP64.
+//
+// Data layout (all in HBM, built once per problem by the host runtime):
+//   rows      -- observations in the reference's residual order (point-major:
+//                E blocks in ordering order, each bucket reversed), so a
+//                point's rows are contiguous;
+//   cm rows   -- the same observations sorted by camera (F block) and cut
+//                into chunks of <= chunk_rows rows; one workgroup reduces one
+//                chunk, so per-camera 9x9 sums need no global atomics;
+//   camera table -- per camera R, dR/daa_k, t, f, k1, k2, recomputed from
+//                the FP64 parameters whenever they change; the Jacobian is
+//                recomputed from it in every pass instead of being stored.
+#ifndef MBA_DEVICE_H_
+#define MBA_DEVICE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cfloat>
+
+namespace mba {
+
+constexpr int kCamTab = 48;  // R 9 | dR 27 | t 3 | f k1 k2 | pad
+constexpr int kTabDR = 9, kTabT = 36, kTabF = 39;
+constexpr int kDirTab = 16;  // M 9 | dt 3 | dintr 3 | pad
+constexpr int kPartial = 64; // U 45 | g 9 | h 9 | pad
+constexpr int kBlock = 256;
+
+struct Scalars {
+  double sum;
+  double aux;
+  int32_t flag;
+  int32_t pad;
+};
+
+struct PcgState {
+  double rho, rho_last, Q0, alpha, beta, pq, norm_b, tol_r, zeta, norm_r;
+  int32_t iter, done, status, pad;  // status: 0 ok, 1 no convergence, -1 failure
+};
+
+template <typename T> __device__ __forceinline__ T tiny();
+template <> __device__ __forceinline__ float tiny<float>() { return FLT_MIN; }
+template <> __device__ __forceinline__ double tiny<double>() { return DBL_MIN; }
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum of N accumulators (blockDim.x == kBlock); result valid in
+// out[] for threads 0..N-1 (out is LDS of size N * kBlock/64).
+template <typename T, int N>
+__device__ __forceinline__ void block_sum(T (&acc)[N], T* lds) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const T v = wave_sum(acc[k]);
+    if (lane == 0) lds[wave * N + k] = v;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------ camera model
+// Rodrigues with the reference's branch point (rotation.h:565-625): first
+// order I + [w]x when theta^2 <= DBL_EPSILON.  Writes R and dR/dw_k.
+__device__ __forceinline__ void rodrigues_with_jacobian(const double w[3], double R[9],
+                                                        double dR[27]) {
+  const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  if (th2 > DBL_EPSILON) {
+    const double th = sqrt(th2), c = cos(th), s = sin(th), it = 1.0 / th;
+    const double u[3] = {w[0] * it, w[1] * it, w[2] * it};
+    const double oc = 1.0 - c;
+    // R = c I + s [u]x + (1 - c) u u'
+    R[0] = c + oc * u[0] * u[0];        R[1] = -s * u[2] + oc * u[0] * u[1]; R[2] = s * u[1] + oc * u[0] * u[2];
+    R[3] = s * u[2] + oc * u[1] * u[0]; R[4] = c + oc * u[1] * u[1];        R[5] = -s * u[0] + oc * u[1] * u[2];
+    R[6] = -s * u[1] + oc * u[2] * u[0]; R[7] = s * u[0] + oc * u[2] * u[1]; R[8] = c + oc * u[2] * u[2];
+    for (int k = 0; k < 3; ++k) {
+      // du_i/dw_k = (delta_ik - u_i u_k) / theta ; dtheta/dw_k = u_k
+      double du[3];
+      for (int i = 0; i < 3; ++i) du[i] = ((i == k ? 1.0 : 0.0) - u[i] * u[k]) * it;
+      const double dc = -s * u[k], ds = c * u[k], doc = s * u[k];
+      double* D = dR + 9 * k;
+      // d[u]x
+      const double X[9] = {0.0, -du[2], du[1], du[2], 0.0, -du[0], -du[1], du[0], 0.0};
+      const double Ux[9] = {0.0, -u[2], u[1], u[2], 0.0, -u[0], -u[1], u[0], 0.0};
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          const int ij = 3 * i + j;
+          D[ij] = (i == j ? dc : 0.0) + ds * Ux[ij] + s * X[ij] + doc * u[i] * u[j] +
+                  oc * (du[i] * u[j] + u[i] * du[j]);
+        }
+    }
+  } else {
+    R[0] = 1.0;   R[1] = -w[2]; R[2] = w[1];
+    R[3] = w[2];  R[4] = 1.0;   R[5] = -w[0];
+    R[6] = -w[1]; R[7] = w[0];  R[8] = 1.0;
+    for (int i = 0; i < 27; ++i) dR[i] = 0.0;
+    // d/dw_k [w]x = [e_k]x
+    dR[0 * 9 + 5] = -1.0; dR[0 * 9 + 7] = 1.0;
+    dR[1 * 9 + 2] = 1.0;  dR[1 * 9 + 6] = -1.0;
+    dR[2 * 9 + 1] = -1.0; dR[2 * 9 + 3] = 1.0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cam_table(int nc, const double* __restrict__ cams,
+                                                      T* __restrict__ tab) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const double* p = cams + 9 * (size_t)c;
+  double R[9], dR[27];
+  const double w[3] = {p[0], p[1], p[2]};
+  rodrigues_with_jacobian(w, R, dR);
+  T* o = tab + (size_t)kCamTab * c;
+  for (int i = 0; i < 9; ++i) o[i] = (T)R[i];
+  for (int i = 0; i < 27; ++i) o[kTabDR + i] = (T)dR[i];
+  for (int i = 0; i < 6; ++i) o[kTabT + i] = (T)p[3 + i];
+  for (int i = 42; i < kCamTab; ++i) o[i] = (T)0;
+}
+
+template <typename T>
+struct CamProj {
+  T R[9], t[3], f, k1, k2;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_proj(const T* __restrict__ tab, CamProj<T>& c) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) c.R[i] = tab[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) c.t[i] = tab[kTabT + i];
+  c.f = tab[kTabF];
+  c.k1 = tab[kTabF + 1];
+  c.k2 = tab[kTabF + 2];
+}
+
+template <typename T>
+struct Proj {
+  T rx, ry;          // residual, predicted - observed
+  T P[6];            // d r / d Xc (2 x 3)
+  T jf[2], jk1[2], jk2[2];
+};
+
+// Residual of libs/sfm's functor (pinhole_brown_rt_ceres_functor.h:60-107,
+// brown_pinhole_camera.h:14-41, principal point 0, k3 = 0) and its analytic
+// derivative with respect to the camera-frame point.
+template <typename T>
+__device__ __forceinline__ void project(const CamProj<T>& c, const T X[3], T ox, T oy, Proj<T>& o) {
+  const T xc = c.R[0] * X[0] + c.R[1] * X[1] + c.R[2] * X[2] + c.t[0];
+  const T yc = c.R[3] * X[0] + c.R[4] * X[1] + c.R[5] * X[2] + c.t[1];
+  const T zc = c.R[6] * X[0] + c.R[7] * X[1] + c.R[8] * X[2] + c.t[2];
+  const T iz = T(1) / zc;
+  const T xe = xc * iz, ye = yc * iz;
+  const T r2 = xe * xe + ye * ye;
+  const T d = T(1) + r2 * (c.k1 + c.k2 * r2);
+  const T fd = c.f * d;
+  o.rx = fd * xe - ox;
+  o.ry = fd * ye - oy;
+  const T fdd = c.f * T(2) * (c.k1 + T(2) * c.k2 * r2);
+  const T a11 = fd + fdd * xe * xe, a12 = fdd * xe * ye, a22 = fd + fdd * ye * ye;
+  o.P[0] = a11 * iz;
+  o.P[1] = a12 * iz;
+  o.P[2] = -(a11 * xe + a12 * ye) * iz;
+  o.P[3] = a12 * iz;
+  o.P[4] = a22 * iz;
+  o.P[5] = -(a12 * xe + a22 * ye) * iz;
+  o.jf[0] = d * xe;
+  o.jf[1] = d * ye;
+  o.jk1[0] = c.f * xe * r2;
+  o.jk1[1] = c.f * ye * r2;
+  o.jk2[0] = o.jk1[0] * r2;
+  o.jk2[1] = o.jk1[1] * r2;
+}
+
+// HuberLoss + Corrector for rho'' <= 0 (loss_function.cc:47-61,
+// corrector.cc:41-85): residual and Jacobian scale by sqrt(rho').
+template <typename T>
+__device__ __forceinline__ T huber_weight(T s, T a) {
+  if (a > T(0) && s > a * a) {
+    T rho1 = a / sqrt(s);
+    rho1 = rho1 > tiny<T>() ? rho1 : tiny<T>();
+    return sqrt(rho1);
+  }
+  return T(1);
+}
+
+template <typename T>
+__device__ __forceinline__ T huber_cost(T s, T a) {
+  if (a > T(0) && s > a * a) return T(0.5) * (T(2) * a * sqrt(s) - a * a);
+  return T(0.5) * s;
+}
+
+// Point Jacobian J_p = w * (P R) .* sp  (2 x 3, row-major).
+template <typename T>
+__device__ __forceinline__ void jac_point(const Proj<T>& o, const T R[9], T wgt, const T sp[3],
+                                          T Jp[6]) {
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      Jp[3 * r + j] = wgt * sp[j] *
+                      (o.P[3 * r] * R[j] + o.P[3 * r + 1] * R[3 + j] + o.P[3 * r + 2] * R[6 + j]);
+}
+
+// Camera Jacobian J_c (2 x 9) from the table's dR: columns aa (P dR_k X),
+// t (P), f, k1, k2; scaled by w and the camera scale.
+template <typename T>
+__device__ __forceinline__ void jac_cam(const Proj<T>& o, const T* dR, const T X[3], T wgt,
+                                        const T cs[9], T Jc[18]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const T* D = dR + 9 * k;
+    const T v0 = D[0] * X[0] + D[1] * X[1] + D[2] * X[2];
+    const T v1 = D[3] * X[0] + D[4] * X[1] + D[5] * X[2];
+    const T v2 = D[6] * X[0] + D[7] * X[1] + D[8] * X[2];
+    Jc[k] = wgt * cs[k] * (o.P[0] * v0 + o.P[1] * v1 + o.P[2] * v2);
+    Jc[9 + k] = wgt * cs[k] * (o.P[3] * v0 + o.P[4] * v1 + o.P[5] * v2);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    Jc[3 + k] = wgt * cs[3 + k] * o.P[k];
+    Jc[12 + k] = wgt * cs[3 + k] * o.P[3 + k];
+  }
+  Jc[6] = wgt * cs[6] * o.jf[0];
+  Jc[15] = wgt * cs[6] * o.jf[1];
+  Jc[7] = wgt * cs[7] * o.jk1[0];
+  Jc[16] = wgt * cs[7] * o.jk1[1];
+  Jc[8] = wgt * cs[8] * o.jk2[0];
+  Jc[17] = wgt * cs[8] * o.jk2[1];
+}
+
+// J_c x through a direction table (M X + dt, intrinsics), weight applied.
+template <typename T>
+__device__ __forceinline__ void jac_cam_dir(const Proj<T>& o, const T* __restrict__ dt, const T X[3],
+                                            T wgt, T out[2]) {
+  const T v0 = dt[0] * X[0] + dt[1] * X[1] + dt[2] * X[2] + dt[9];
+  const T v1 = dt[3] * X[0] + dt[4] * X[1] + dt[5] * X[2] + dt[10];
+  const T v2 = dt[6] * X[0] + dt[7] * X[1] + dt[8] * X[2] + dt[11];
+  out[0] = wgt * (o.P[0] * v0 + o.P[1] * v1 + o.P[2] * v2 + o.jf[0] * dt[12] + o.jk1[0] * dt[13] +
+                  o.jk2[0] * dt[14]);
+  out[1] = wgt * (o.P[3] * v0 + o.P[4] * v1 + o.P[5] * v2 + o.jf[1] * dt[12] + o.jk1[1] * dt[13] +
+                  o.jk2[1] * dt[14]);
+}
+
+// FP64 inverse of an SPD 3x3 (xx xy xz yy yz zz) via LLT.
+__device__ __forceinline__ bool inv3_spd(const double a[6], double o[6]) {
+  if (!(a[0] > 0.0)) return false;
+  const double l00 = sqrt(a[0]), l10 = a[1] / l00, l20 = a[2] / l00;
+  double t = a[3] - l10 * l10;
+  if (!(t > 0.0)) return false;
+  const double l11 = sqrt(t), l21 = (a[4] - l20 * l10) / l11;
+  t = a[5] - l20 * l20 - l21 * l21;
+  if (!(t > 0.0)) return false;
+  const double l22 = sqrt(t);
+  const double i00 = 1.0 / l00, i11 = 1.0 / l11, i22 = 1.0 / l22;
+  const double i10 = -l10 * i00 * i11, i21 = -l21 * i11 * i22, i20 = -(l20 * i00 + l21 * i10) * i22;
+  o[0] = i00 * i00 + i10 * i10 + i20 * i20;
+  o[1] = i10 * i11 + i20 * i21;
+  o[2] = i20 * i22;
+  o[3] = i11 * i11 + i21 * i21;
+  o[4] = i21 * i22;
+  o[5] = i22 * i22;
+  return true;
+}
+
+// FP64 inverse of an SPD n x n (row-major, n <= 9) via LLT; false if not PD.
+__device__ inline bool invn_spd(double* A, int n, double* inv) {
+  for (int j = 0; j < n; ++j) {
+    double d = A[j * n + j];
+    for (int k = 0; k < j; ++k) d -= A[j * n + k] * A[j * n + k];
+    if (!(d > 0.0)) return false;
+    d = sqrt(d);
+    A[j * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double s = A[i * n + j];
+      for (int k = 0; k < j; ++k) s -= A[i * n + k] * A[j * n + k];
+      A[i * n + j] = s / d;
+    }
+  }
+  for (int c = 0; c < n; ++c) {
+    double e[9];
+    for (int i = 0; i < n; ++i) {
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) s -= A[i * n + k] * e[k];
+      e[i] = s / A[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = e[i];
+      for (int k = i + 1; k < n; ++k) s -= A[k * n + i] * e[k];
+      e[i] = s / A[i * n + i];
+    }
+    for (int i = 0; i < n; ++i) inv[i * n + c] = e[i];
+  }
+  return true;
+}
+
+__device__ __forceinline__ int ut9(int i, int j) { return i * 9 - i * (i - 1) / 2 + (j - i); }
+
+// --------------------------------------------------------------- FP64 cost
+// One thread per E block; block partial sums of 1/2 rho(|r|^2).
+template <typename Tcam>
+__global__ __launch_bounds__(kBlock) void k_cost64(int ne, const int32_t* __restrict__ pt_start,
+                                                   const int32_t* __restrict__ row_cam,
+                                                   const double* __restrict__ row_obs,
+                                                   const double* __restrict__ pts,
+                                                   const double* __restrict__ ctab, double huber_a,
+                                                   double* __restrict__ partial,
+                                                   int32_t* __restrict__ flag) {
+  __shared__ double lds[kBlock / 64];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (e < ne) {
+    const double X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+    int bad = 0;
+    for (int r = pt_start[e]; r < pt_start[e + 1]; ++r) {
+      CamProj<double> c;
+      load_proj(ctab + (size_t)kCamTab * row_cam[r], c);
+      Proj<double> o;
+      project(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
+      const double s = o.rx * o.rx + o.ry * o.ry;
+      if (!isfinite(s)) bad = 1;
+      acc[0] += huber_cost(s, huber_a);
+    }
+    if (bad) *flag = 1;
+  }
+  block_sum<double, 1>(acc, lds);
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
+// Deterministic single-block sum of partials.
+__global__ __launch_bounds__(kBlock) void k_reduce(const double* __restrict__ partial, int n,
+                                                   double* __restrict__ out) {
+  __shared__ double lds[kBlock / 64];
+  double acc[1] = {0.0};
+  for (int i = threadIdx.x; i < n; i += kBlock) acc[0] += partial[i];
+  block_sum<double, 1>(acc, lds);
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w];
+    *out = s;
+  }
+}
+
+// Max-reduce of |g / scale| (unscaled gradient max norm) and sum of squares.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grad_partial(int64_t n, const T* __restrict__ g,
+                                                         const double* __restrict__ scale,
+                                                         double* __restrict__ pmax,
+                                                         double* __restrict__ psq) {
+  __shared__ double lds[2 * (kBlock / 64)];
+  double mx = 0.0, sq = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double v = (double)g[i] / scale[i];
+    mx = fmax(mx, fabs(v));
+    sq += v * v;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
+    sq += __shfl_xor(sq, o, 64);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) { lds[wave] = mx; lds[4 + wave] = sq; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) { a = fmax(a, lds[w]); b += lds[4 + w]; }
+    pmax[blockIdx.x] = a;
+    psq[blockIdx.x] = b;
+  }
+}
+
+__global__ void k_grad_final(const double* __restrict__ pmax, const double* __restrict__ psq, int n,
+                             double* __restrict__ out /* [max, sumsq] accumulate */) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a = out[0], b = out[1];
+  for (int i = 0; i < n; ++i) { a = fmax(a, pmax[i]); b += psq[i]; }
+  out[0] = a;
+  out[1] = b;
+}
+
+// ------------------------------------------------------------- point pass
+// V = sum Jp'Jp, g_p = sum Jp' r; LM diagonal of points; Vinv = (V + D^2)^-1
+// in FP64; w = Vinv g_p.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_point_pass(
+    int ne, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const T* __restrict__ row_obs, const T* __restrict__ pts, const T* __restrict__ pscale,
+    const T* __restrict__ ctab, T huber_a, double inv_radius, double min_diag, double max_diag,
+    double* __restrict__ diag_p, int recompute_diag, T* __restrict__ V, T* __restrict__ gp,
+    T* __restrict__ Vinv, T* __restrict__ wv, int32_t* __restrict__ flag) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+  const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
+  T v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+  for (int r = pt_start[e]; r < pt_start[e + 1]; ++r) {
+    CamProj<T> c;
+    load_proj(ctab + (size_t)kCamTab * row_cam[r], c);
+    Proj<T> o;
+    project(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
+    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    T Jp[6];
+    jac_point(o, c.R, wgt, sp, Jp);
+    const T rx = wgt * o.rx, ry = wgt * o.ry;
+    v[0] += Jp[0] * Jp[0] + Jp[3] * Jp[3];
+    v[1] += Jp[0] * Jp[1] + Jp[3] * Jp[4];
+    v[2] += Jp[0] * Jp[2] + Jp[3] * Jp[5];
+    v[3] += Jp[1] * Jp[1] + Jp[4] * Jp[4];
+    v[4] += Jp[1] * Jp[2] + Jp[4] * Jp[5];
+    v[5] += Jp[2] * Jp[2] + Jp[5] * Jp[5];
+    g[0] += Jp[0] * rx + Jp[3] * ry;
+    g[1] += Jp[1] * rx + Jp[4] * ry;
+    g[2] += Jp[2] * rx + Jp[5] * ry;
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) V[6 * (size_t)e + i] = v[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) gp[3 * (size_t)e + i] = g[i];
+  double dg[3];
+  const int di[3] = {0, 3, 5};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (recompute_diag) {
+      dg[i] = fmin(fmax((double)v[di[i]], min_diag), max_diag);
+      diag_p[3 * (size_t)e + i] = dg[i];
+    } else {
+      dg[i] = diag_p[3 * (size_t)e + i];
+    }
+  }
+  double a[6] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3], (double)v[4], (double)v[5]};
+  a[0] += dg[0] * inv_radius;
+  a[3] += dg[1] * inv_radius;
+  a[5] += dg[2] * inv_radius;
+  double iv[6];
+  if (!inv3_spd(a, iv)) {
+    *flag = 1;
+    for (int i = 0; i < 6; ++i) iv[i] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
+  const double g0 = g[0], g1 = g[1], g2 = g[2];
+  wv[3 * (size_t)e + 0] = (T)(iv[0] * g0 + iv[1] * g1 + iv[2] * g2);
+  wv[3 * (size_t)e + 1] = (T)(iv[1] * g0 + iv[3] * g1 + iv[4] * g2);
+  wv[3 * (size_t)e + 2] = (T)(iv[2] * g0 + iv[4] * g1 + iv[5] * g2);
+}
+
+// Re-damp after a rejected step: Vinv, w from stored V, g_p, diag_p.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restrict__ V,
+                                                       const T* __restrict__ gp,
+                                                       const double* __restrict__ diag_p,
+                                                       double inv_radius, T* __restrict__ Vinv,
+                                                       T* __restrict__ wv, int32_t* __restrict__ flag) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double a[6];
+  for (int i = 0; i < 6; ++i) a[i] = (double)V[6 * (size_t)e + i];
+  a[0] += diag_p[3 * (size_t)e] * inv_radius;
+  a[3] += diag_p[3 * (size_t)e + 1] * inv_radius;
+  a[5] += diag_p[3 * (size_t)e + 2] * inv_radius;
+  double iv[6];
+  if (!inv3_spd(a, iv)) {
+    *flag = 1;
+    for (int i = 0; i < 6; ++i) iv[i] = 0.0;
+  }
+  for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
+  const double g0 = gp[3 * (size_t)e], g1 = gp[3 * (size_t)e + 1], g2 = gp[3 * (size_t)e + 2];
+  wv[3 * (size_t)e + 0] = (T)(iv[0] * g0 + iv[1] * g1 + iv[2] * g2);
+  wv[3 * (size_t)e + 1] = (T)(iv[1] * g0 + iv[3] * g1 + iv[4] * g2);
+  wv[3 * (size_t)e + 2] = (T)(iv[2] * g0 + iv[4] * g1 + iv[5] * g2);
+}
+
+// ------------------------------------------------------------ camera pass
+// One workgroup per camera-major chunk (all rows of one camera): partial
+// U = sum Jc'Jc (45), g_c = sum Jc' r (9) and, with_h, h = sum Jc'(r - Jp w_e).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cam_pass(
+    const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
+    const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
+    const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pts,
+    const T* __restrict__ pscale, const T* __restrict__ cscale, const T* __restrict__ ctab,
+    const T* __restrict__ wv, T huber_a, int with_h, T* __restrict__ partial) {
+  __shared__ T lds[63 * (kBlock / 64)];
+  const int ch = blockIdx.x;
+  const int f = chunk_f[ch];
+  const T* tab = ctab + (size_t)kCamTab * f_to_cam[f];
+  CamProj<T> c;
+  load_proj(tab, c);
+  T dR[27];
+#pragma unroll
+  for (int i = 0; i < 27; ++i) dR[i] = tab[kTabDR + i];
+  T cs[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) cs[i] = cscale[9 * (size_t)f + i];
+  T acc[63];
+#pragma unroll
+  for (int i = 0; i < 63; ++i) acc[i] = T(0);
+  const int end = chunk_end[ch];
+  for (int i = chunk_begin[ch] + threadIdx.x; i < end; i += kBlock) {
+    const int e = cm_e[i];
+    const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+    Proj<T> o;
+    project(c, X, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], o);
+    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    T Jc[18];
+    jac_cam(o, dR, X, wgt, cs, Jc);
+    const T rx = wgt * o.rx, ry = wgt * o.ry;
+    T tx = rx, ty = ry;
+    if (with_h) {
+      const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
+      T Jp[6];
+      jac_point(o, c.R, wgt, sp, Jp);
+      const T w0 = wv[3 * (size_t)e], w1 = wv[3 * (size_t)e + 1], w2 = wv[3 * (size_t)e + 2];
+      tx -= Jp[0] * w0 + Jp[1] * w1 + Jp[2] * w2;
+      ty -= Jp[3] * w0 + Jp[4] * w1 + Jp[5] * w2;
+    }
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < 9; ++a)
+#pragma unroll
+      for (int b = a; b < 9; ++b) acc[k++] += Jc[a] * Jc[b] + Jc[9 + a] * Jc[9 + b];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      acc[45 + a] += Jc[a] * rx + Jc[9 + a] * ry;
+      acc[54 + a] += Jc[a] * tx + Jc[9 + a] * ty;
+    }
+  }
+  block_sum<T, 63>(acc, lds);
+  if (threadIdx.x < 63) {
+    T s = T(0);
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w * 63 + threadIdx.x];
+    partial[(size_t)kPartial * ch + threadIdx.x] = s;
+  }
+}
+
+// One thread per F block: sum its chunk partials; LM diagonal; FP64
+// inverse of U + D^2 (block-Jacobi preconditioner); rhs.
+template <typename T>
+__global__ __launch_bounds__(64) void k_cam_finalize(
+    int nf, const int32_t* __restrict__ f_chunk_start, const T* __restrict__ partial,
+    double inv_radius, double min_diag, double max_diag, double* __restrict__ diag_c,
+    int recompute_diag, int with_h, T* __restrict__ U, T* __restrict__ gc, T* __restrict__ rhs,
+    T* __restrict__ Minv, int32_t* __restrict__ flag) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  double s[63];
+  for (int k = 0; k < 63; ++k) s[k] = 0.0;
+  for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch)
+    for (int k = 0; k < 63; ++k) s[k] += (double)partial[(size_t)kPartial * ch + k];
+  for (int k = 0; k < 45; ++k) U[45 * (size_t)f + k] = (T)s[k];
+  for (int k = 0; k < 9; ++k) {
+    gc[9 * (size_t)f + k] = (T)s[45 + k];
+    if (with_h) rhs[9 * (size_t)f + k] = (T)s[54 + k];
+  }
+  double A[81], inv[81];
+  for (int i = 0; i < 9; ++i)
+    for (int j = i; j < 9; ++j) {
+      const double u = (double)(T)s[ut9(i, j)];
+      A[i * 9 + j] = u;
+      A[j * 9 + i] = u;
+    }
+  for (int i = 0; i < 9; ++i) {
+    double dg;
+    if (recompute_diag) {
+      dg = fmin(fmax(A[i * 10], min_diag), max_diag);
+      diag_c[9 * (size_t)f + i] = dg;
+    } else {
+      dg = diag_c[9 * (size_t)f + i];
+    }
+    A[i * 10] += dg * inv_radius;
+  }
+  if (!invn_spd(A, 9, inv)) {
+    *flag = 1;
+    for (int k = 0; k < 81; ++k) inv[k] = 0.0;
+  }
+  for (int k = 0; k < 81; ++k) Minv[81 * (size_t)f + k] = (T)inv[k];
+}
+
+// Re-damp the camera preconditioner after a rejected step.
+template <typename T>
+__global__ __launch_bounds__(64) void k_cam_damp(int nf, const T* __restrict__ U,
+                                                 const double* __restrict__ diag_c,
+                                                 double inv_radius, T* __restrict__ Minv,
+                                                 int32_t* __restrict__ flag) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  double A[81], inv[81];
+  for (int i = 0; i < 9; ++i)
+    for (int j = i; j < 9; ++j) {
+      const double u = (double)U[45 * (size_t)f + ut9(i, j)];
+      A[i * 9 + j] = u;
+      A[j * 9 + i] = u;
+    }
+  for (int i = 0; i < 9; ++i) A[i * 10] += diag_c[9 * (size_t)f + i] * inv_radius;
+  if (!invn_spd(A, 9, inv)) {
+    *flag = 1;
+    for (int k = 0; k < 81; ++k) inv[k] = 0.0;
+  }
+  for (int k = 0; k < 81; ++k) Minv[81 * (size_t)f + k] = (T)inv[k];
+}
+
+// ---------------------------------------------------------------- matvecs
+// Direction table for a camera vector x (F order, 9 per block, or NULL):
+// xs = sign * cscale .* x;  M = sum_k xs_k dR_k, dt = xs_t, dintr.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cam_dir(int nc, const int32_t* __restrict__ cam_to_f,
+                                                    const T* __restrict__ cscale,
+                                                    const T* __restrict__ ctab,
+                                                    const T* __restrict__ x, T sign,
+                                                    T* __restrict__ dtab, const PcgState* __restrict__ st) {
+  if (st && st->done) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  T* o = dtab + (size_t)kDirTab * c;
+  const int f = cam_to_f[c];
+  if (f < 0 || x == nullptr) {
+    for (int i = 0; i < kDirTab; ++i) o[i] = T(0);
+    return;
+  }
+  T xs[9];
+  for (int i = 0; i < 9; ++i) xs[i] = sign * cscale[9 * (size_t)f + i] * x[9 * (size_t)f + i];
+  const T* dR = ctab + (size_t)kCamTab * c + kTabDR;
+  for (int i = 0; i < 9; ++i) o[i] = xs[0] * dR[i] + xs[1] * dR[9 + i] + xs[2] * dR[18 + i];
+  for (int i = 0; i < 6; ++i) o[9 + i] = xs[3 + i];
+  o[15] = T(0);
+}
+
+// Per E block: q = sum_rows Jp'(beta r + Jc x) (x via dtab); z = Vinv q
+// (apply_vinv) or q.  With an optional point step `xp` (3 per E block) the
+// row term also adds Jp xp.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mv_point(
+    int ne, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const T* __restrict__ row_obs, const T* __restrict__ pts, const T* __restrict__ pscale,
+    const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ Vinv, T huber_a,
+    T beta, int apply_vinv, T* __restrict__ zout, const PcgState* __restrict__ st) {
+  if (st && st->done) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+  const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
+  T q[3] = {0, 0, 0};
+  for (int r = pt_start[e]; r < pt_start[e + 1]; ++r) {
+    const int cam = row_cam[r];
+    CamProj<T> c;
+    load_proj(ctab + (size_t)kCamTab * cam, c);
+    Proj<T> o;
+    project(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
+    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    T Jp[6];
+    jac_point(o, c.R, wgt, sp, Jp);
+    T v[2];
+    jac_cam_dir(o, dtab + (size_t)kDirTab * cam, X, wgt, v);
+    v[0] += beta * wgt * o.rx;
+    v[1] += beta * wgt * o.ry;
+    q[0] += Jp[0] * v[0] + Jp[3] * v[1];
+    q[1] += Jp[1] * v[0] + Jp[4] * v[1];
+    q[2] += Jp[2] * v[0] + Jp[5] * v[1];
+  }
+  if (apply_vinv) {
+    const T* iv = Vinv + 6 * (size_t)e;
+    zout[3 * (size_t)e + 0] = iv[0] * q[0] + iv[1] * q[1] + iv[2] * q[2];
+    zout[3 * (size_t)e + 1] = iv[1] * q[0] + iv[3] * q[1] + iv[4] * q[2];
+    zout[3 * (size_t)e + 2] = iv[2] * q[0] + iv[4] * q[1] + iv[5] * q[2];
+  } else {
+    zout[3 * (size_t)e + 0] = q[0];
+    zout[3 * (size_t)e + 1] = q[1];
+    zout[3 * (size_t)e + 2] = q[2];
+  }
+}
+
+// Per chunk: partial_c = sum_rows Jc'(Jp z_e) (9 values).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mv_cam(
+    const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
+    const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
+    const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pts,
+    const T* __restrict__ pscale, const T* __restrict__ cscale, const T* __restrict__ ctab,
+    const T* __restrict__ z, T huber_a, T* __restrict__ partial, const PcgState* __restrict__ st) {
+  __shared__ T lds[9 * (kBlock / 64)];
+  if (st && st->done) return;
+  const int ch = blockIdx.x;
+  const int f = chunk_f[ch];
+  const T* tab = ctab + (size_t)kCamTab * f_to_cam[f];
+  CamProj<T> c;
+  load_proj(tab, c);
+  T dR[27];
+#pragma unroll
+  for (int i = 0; i < 27; ++i) dR[i] = tab[kTabDR + i];
+  T cs[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) cs[i] = cscale[9 * (size_t)f + i];
+  T acc[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) acc[i] = T(0);
+  const int end = chunk_end[ch];
+  for (int i = chunk_begin[ch] + threadIdx.x; i < end; i += kBlock) {
+    const int e = cm_e[i];
+    const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+    const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
+    Proj<T> o;
+    project(c, X, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], o);
+    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    T Jp[6], Jc[18];
+    jac_point(o, c.R, wgt, sp, Jp);
+    jac_cam(o, dR, X, wgt, cs, Jc);
+    const T z0 = z[3 * (size_t)e], z1 = z[3 * (size_t)e + 1], z2 = z[3 * (size_t)e + 2];
+    const T u0 = Jp[0] * z0 + Jp[1] * z1 + Jp[2] * z2;
+    const T u1 = Jp[3] * z0 + Jp[4] * z1 + Jp[5] * z2;
+#pragma unroll
+    for (int a = 0; a < 9; ++a) acc[a] += Jc[a] * u0 + Jc[9 + a] * u1;
+  }
+  block_sum<T, 9>(acc, lds);
+  if (threadIdx.x < 9) {
+    T s = T(0);
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w * 9 + threadIdx.x];
+    partial[(size_t)kPartial * ch + threadIdx.x] = s;
+  }
+}
+
+// y_f = [u_term] (U + D^2) x_f - sum partials.  `partial_sign`: +1 / -1.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mv_finalize(
+    int nf, const int32_t* __restrict__ f_chunk_start, const T* __restrict__ partial,
+    const T* __restrict__ U, const double* __restrict__ diag_c, double inv_radius,
+    const T* __restrict__ x, int u_term, T* __restrict__ y, const PcgState* __restrict__ st) {
+  if (st && st->done) return;
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  T s[9];
+  for (int k = 0; k < 9; ++k) s[k] = T(0);
+  for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch)
+    for (int k = 0; k < 9; ++k) s[k] += partial[(size_t)kPartial * ch + k];
+  if (u_term) {
+    const T* u = U + 45 * (size_t)f;
+    const T* xf = x + 9 * (size_t)f;
+    for (int i = 0; i < 9; ++i) {
+      T a = (T)(diag_c[9 * (size_t)f + i] * inv_radius) * xf[i];
+      for (int j = 0; j < 9; ++j) a += u[i <= j ? ut9(i, j) : ut9(j, i)] * xf[j];
+      y[9 * (size_t)f + i] = a - s[i];
+    }
+  } else {
+    for (int i = 0; i < 9; ++i) y[9 * (size_t)f + i] = -s[i];
+  }
+}
+
+// rhs_f = g_f - sum partials.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rhs_finalize(int nf, const int32_t* __restrict__ f_chunk_start,
+                                                         const T* __restrict__ partial,
+                                                         const T* __restrict__ gc,
+                                                         T* __restrict__ rhs) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  T s[9];
+  for (int k = 0; k < 9; ++k) s[k] = T(0);
+  for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch)
+    for (int k = 0; k < 9; ++k) s[k] += partial[(size_t)kPartial * ch + k];
+  for (int k = 0; k < 9; ++k) rhs[9 * (size_t)f + k] = gc[9 * (size_t)f + k] - s[k];
+}
+
+// Model cost change terms: sum over rows of m.(r + m/2) with m = J step
+// (camera part via dtab, point part from step_p); FP64 accumulation.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_model_cost(
+    int ne, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const T* __restrict__ row_obs, const T* __restrict__ pts, const T* __restrict__ pscale,
+    const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ step_p,
+    T huber_a, double* __restrict__ partial) {
+  __shared__ double lds[kBlock / 64];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (e < ne) {
+    const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+    const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
+    const T dp[3] = {step_p[3 * (size_t)e], step_p[3 * (size_t)e + 1], step_p[3 * (size_t)e + 2]};
+    for (int r = pt_start[e]; r < pt_start[e + 1]; ++r) {
+      const int cam = row_cam[r];
+      CamProj<T> c;
+      load_proj(ctab + (size_t)kCamTab * cam, c);
+      Proj<T> o;
+      project(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
+      const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+      T Jp[6];
+      jac_point(o, c.R, wgt, sp, Jp);
+      T m[2];
+      jac_cam_dir(o, dtab + (size_t)kDirTab * cam, X, wgt, m);
+      m[0] += Jp[0] * dp[0] + Jp[1] * dp[1] + Jp[2] * dp[2];
+      m[1] += Jp[3] * dp[0] + Jp[4] * dp[1] + Jp[5] * dp[2];
+      const double m0 = m[0], m1 = m[1];
+      acc[0] += m0 * ((double)(wgt * o.rx) + 0.5 * m0) + m1 * ((double)(wgt * o.ry) + 0.5 * m1);
+    }
+  }
+  block_sum<double, 1>(acc, lds);
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
+// x_out = x + step .* scale (FP64), with partial sums of |x - x_out|^2.
+__global__ __launch_bounds__(kBlock) void k_plus(int64_t n, const double* __restrict__ x,
+                                                 const double* __restrict__ step,
+                                                 const double* __restrict__ scale,
+                                                 double* __restrict__ out,
+                                                 double* __restrict__ partial) {
+  __shared__ double lds[kBlock / 64];
+  double acc[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double d = step[i] * scale[i];
+    const double v = x[i] + d;
+    out[i] = v;
+    const double df = x[i] - v;
+    acc[0] += df * df;
+  }
+  block_sum<double, 1>(acc, lds);
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sumsq(int64_t n, const double* __restrict__ x,
+                                                  double* __restrict__ partial) {
+  __shared__ double lds[kBlock / 64];
+  double acc[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    acc[0] += x[i] * x[i];
+  block_sum<double, 1>(acc, lds);
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
+template <typename Ts, typename Td>
+__global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const Ts* __restrict__ src,
+                                                    Td* __restrict__ dst) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    dst[i] = (Td)src[i];
+}
+
+// Scatter FP64 values at `idx` positions (gather = 0) or gather.
+__global__ __launch_bounds__(kBlock) void k_gather_cams(int nf, const int32_t* __restrict__ f_to_cam,
+                                                        const double* __restrict__ cams,
+                                                        double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 9 * nf) return;
+  out[i] = cams[9 * (size_t)f_to_cam[i / 9] + i % 9];
+}
+__global__ __launch_bounds__(kBlock) void k_scatter_cams(int nf, const int32_t* __restrict__ f_to_cam,
+                                                         const double* __restrict__ src,
+                                                         double* __restrict__ cams) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 9 * nf) return;
+  cams[9 * (size_t)f_to_cam[i / 9] + i % 9] = src[i];
+}
+
+// -------------------------------------------------------------------- PCG
+// Conjugate gradients of conjugate_gradients_solver.cc:66-245 with the state
+// on the device.  F-space vectors, one thread per 9-entry F block; block
+// partial dot products reduced by single-thread scalar kernels.
+
+// p = z + beta p (or z on the first iteration); then the direction table.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_p(int nf, const PcgState* __restrict__ st,
+                                                  const T* __restrict__ z, T* __restrict__ p) {
+  if (st->done) return;
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const T beta = (T)st->beta;
+  for (int k = 0; k < 9; ++k) {
+    const size_t i = 9 * (size_t)f + k;
+    p[i] = (st->iter == 1) ? z[i] : z[i] + beta * p[i];
+  }
+}
+
+// Dot partials: out[b] = sum a.b over block (FP64 accumulation).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot_partial(int64_t n, const PcgState* __restrict__ st,
+                                                        const T* __restrict__ a,
+                                                        const T* __restrict__ b,
+                                                        double* __restrict__ partial) {
+  __shared__ double lds[kBlock / 64];
+  if (st && st->done) return;
+  double acc[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    acc[0] += (double)a[i] * (double)b[i];
+  block_sum<double, 1>(acc, lds);
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
+// alpha = rho / p.q, with the reference's failure tests.
+__global__ void k_pcg_alpha(const double* __restrict__ partial, int nb, PcgState* __restrict__ st) {
+  if (threadIdx.x != 0 || st->done) return;
+  double pq = 0.0;
+  for (int i = 0; i < nb; ++i) pq += partial[i];
+  st->pq = pq;
+  if (pq <= 0.0 || isinf(pq)) {  // indefinite: NO_CONVERGENCE, keep x
+    st->status = 1;
+    st->done = 1;
+    return;
+  }
+  st->alpha = st->rho / pq;
+  if (isinf(st->alpha)) {
+    st->status = -1;
+    st->done = 1;
+  }
+}
+
+// PCG vector update, one thread per F block.  mode 0: x += alpha p,
+// r -= alpha q;  1: x += alpha p only;  2: r = b - tmp;  3: r = b (x = 0).
+// Modes 0, 2, 3 then apply z = Minv r and write partials of r.z,
+// x.(b + r) and r.r.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int nf, const PcgState* __restrict__ st,
+                                                       int mode, const T* __restrict__ p,
+                                                       const T* __restrict__ q,
+                                                       const T* __restrict__ b,
+                                                       const T* __restrict__ tmp,
+                                                       const T* __restrict__ Minv, T* __restrict__ x,
+                                                       T* __restrict__ r, T* __restrict__ z,
+                                                       double* __restrict__ partial) {
+  __shared__ double lds[3 * (kBlock / 64)];
+  if (st->done) return;
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (f < nf) {
+    const T alpha = (T)st->alpha;
+    T rr[9];
+    for (int k = 0; k < 9; ++k) {
+      const size_t i = 9 * (size_t)f + k;
+      T xv;
+      if (mode == 3) {
+        xv = T(0);
+        x[i] = xv;
+      } else if (mode <= 1) {
+        xv = x[i] + alpha * p[i];
+        x[i] = xv;
+      } else {
+        xv = x[i];
+      }
+      if (mode == 1) continue;
+      const T rv = (mode == 0) ? r[i] - alpha * q[i] : (mode == 2 ? b[i] - tmp[i] : b[i]);
+      r[i] = rv;
+      rr[k] = rv;
+      acc[1] += (double)xv * ((double)b[i] + (double)rv);
+      acc[2] += (double)rv * (double)rv;
+    }
+    if (mode != 1) {
+      const T* M = Minv + 81 * (size_t)f;
+      for (int i = 0; i < 9; ++i) {
+        T a = T(0);
+        for (int j = 0; j < 9; ++j) a += M[9 * i + j] * rr[j];
+        z[9 * (size_t)f + i] = a;
+        acc[0] += (double)rr[i] * (double)a;
+      }
+    }
+  }
+  if (mode == 1) return;
+  block_sum<double, 3>(acc, lds);
+  if (threadIdx.x < 3) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w * 3 + threadIdx.x];
+    partial[3 * (size_t)blockIdx.x + threadIdx.x] = s;
+  }
+}
+
+// Termination tests and the next rho / beta.  `first`: the initial call
+// (x = 0, r = b) which only sets rho for iteration 1.
+__global__ void k_pcg_scalar(const double* __restrict__ partial, int nb, PcgState* __restrict__ st,
+                             double eta, int min_iters, int max_iters, int first) {
+  if (threadIdx.x != 0 || st->done) return;
+  double rz = 0.0, xbr = 0.0, rr = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    rz += partial[3 * i];
+    xbr += partial[3 * i + 1];
+    rr += partial[3 * i + 2];
+  }
+  st->norm_r = sqrt(rr);
+  if (first) {
+    st->norm_b = sqrt(rr);  // r = b at x = 0
+    st->tol_r = st->tol_r * st->norm_b;
+    if (st->norm_b == 0.0) {
+      st->done = 1;
+      st->status = 0;
+      return;
+    }
+    if (min_iters == 0 && st->norm_r <= st->tol_r) {
+      st->done = 1;
+      return;
+    }
+    st->Q0 = -xbr;
+    st->rho = 1.0;
+  } else {
+    const double Q1 = -xbr;
+    const double zeta = st->iter * (Q1 - st->Q0) / Q1;
+    st->zeta = zeta;
+    if (zeta < eta && st->iter >= min_iters) { st->done = 1; return; }
+    st->Q0 = Q1;
+    if (st->norm_r <= st->tol_r && st->iter >= min_iters) { st->done = 1; return; }
+    if (st->iter >= max_iters) { st->done = 1; st->status = 1; return; }
+    st->iter += 1;
+  }
+  // Next iteration's rho = r.z and beta.
+  const double last = st->rho;
+  st->rho_last = last;
+  st->rho = rz;
+  if (rz == 0.0 || isinf(rz)) { st->done = 1; st->status = -1; return; }
+  if (st->iter > 1) {
+    st->beta = rz / last;
+    if (st->beta == 0.0 || isinf(st->beta)) { st->done = 1; st->status = -1; return; }
+  }
+}
+
+// ----------------------------------------------------------- explicit S
+// Dense lower-triangular S (column-major, n = 9 Nf): diagonal blocks U + D^2.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_s_diag(int nf, const T* __restrict__ U,
+                                                   const double* __restrict__ diag_c,
+                                                   double inv_radius, int64_t n, T* __restrict__ S) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 81 * nf) return;
+  const int f = i / 81, a = (i % 81) / 9, b = i % 9;
+  if (a < b) return;
+  T v = U[45 * (size_t)f + ut9(b, a)];
+  if (a == b) v += (T)(diag_c[9 * (size_t)f + a] * inv_radius);
+  const int64_t row = 9 * (int64_t)f + a, col = 9 * (int64_t)f + b;
+  S[row + col * n] = v;
+}
+
+// One wave per E block: W_a = Jc_a' Jp_a (9x3) and Y_a = W_a Vinv in LDS,
+// then S(f_a, f_b) -= Y_a W_b' for f_a >= f_b via atomics.
+template <typename T>
+__global__ __launch_bounds__(64) void k_s_assemble(
+    int ne, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const int32_t* __restrict__ cam_to_f, const T* __restrict__ row_obs, const T* __restrict__ pts,
+    const T* __restrict__ pscale, const T* __restrict__ cscale, const T* __restrict__ ctab,
+    const T* __restrict__ Vinv, T huber_a, int64_t n, T* __restrict__ S) {
+  constexpr int kMaxT = 128;
+  __shared__ T W[kMaxT][27];
+  __shared__ T Y[kMaxT][27];
+  __shared__ int F[kMaxT];
+  const int e = blockIdx.x;
+  if (e >= ne) return;
+  const int r0 = pt_start[e], t = pt_start[e + 1] - r0;
+  const int lane = threadIdx.x;
+  const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+  const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
+  const T* iv = Vinv + 6 * (size_t)e;
+  const T Vi[9] = {iv[0], iv[1], iv[2], iv[1], iv[3], iv[4], iv[2], iv[4], iv[5]};
+  for (int base = 0; base < t; base += kMaxT) {
+    const int tb = min(kMaxT, t - base);
+    for (int a = lane; a < tb; a += 64) {
+      const int row = r0 + base + a;
+      const int cam = row_cam[row];
+      const int f = cam_to_f[cam];
+      F[a] = f;
+      if (f < 0) continue;
+      const T* tab = ctab + (size_t)kCamTab * cam;
+      CamProj<T> c;
+      load_proj(tab, c);
+      Proj<T> o;
+      project(c, X, row_obs[2 * (size_t)row], row_obs[2 * (size_t)row + 1], o);
+      const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+      T Jp[6], Jc[18], cs[9];
+      for (int k = 0; k < 9; ++k) cs[k] = cscale[9 * (size_t)f + k];
+      jac_point(o, c.R, wgt, sp, Jp);
+      jac_cam(o, tab + kTabDR, X, wgt, cs, Jc);
+      for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 3; ++j) W[a][3 * i + j] = Jc[i] * Jp[j] + Jc[9 + i] * Jp[3 + j];
+      for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 3; ++j)
+          Y[a][3 * i + j] = W[a][3 * i] * Vi[j] + W[a][3 * i + 1] * Vi[3 + j] + W[a][3 * i + 2] * Vi[6 + j];
+    }
+    __syncthreads();
+    // Pairs (a, b) within this slice, plus cross pairs with earlier slices
+    // are not needed when t <= kMaxT (the host enforces it).
+    const int npair = tb * tb;
+    for (int pidx = lane; pidx < npair; pidx += 64) {
+      const int a = pidx / tb, b = pidx % tb;
+      const int fa = F[a], fb = F[b];
+      if (fa < 0 || fb < 0 || fa < fb || (fa == fb && a < b)) continue;
+      // Block (fa, fb): rows 9 fa + i, cols 9 fb + j ; lower triangle only.
+      for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j) {
+          const int64_t row = 9 * (int64_t)fa + i, col = 9 * (int64_t)fb + j;
+          if (row < col) continue;
+          T v = Y[a][3 * i] * W[b][3 * j] + Y[a][3 * i + 1] * W[b][3 * j + 1] + Y[a][3 * i + 2] * W[b][3 * j + 2];
+          if (fa == fb && a != b)  // both orders of an in-camera pair
+            v += Y[b][3 * i] * W[a][3 * j] + Y[b][3 * i + 1] * W[a][3 * j + 1] + Y[b][3 * i + 2] * W[a][3 * j + 2];
+          atomicAdd(S + row + col * n, -v);
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------- dense Cholesky (lower)
+// Right-looking blocked LLT of a column-major SPD matrix, NB = 64.
+constexpr int kNB = 64;
+
+// Factor the kb x kb diagonal block at (k, k) in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ A, int64_t n, int k, int kb,
+                                                    int32_t* __restrict__ info) {
+  __shared__ T L[kNB][kNB + 1];
+  for (int idx = threadIdx.x; idx < kb * kb; idx += blockDim.x) {
+    const int i = idx % kb, j = idx / kb;
+    L[i][j] = A[(k + i) + (int64_t)(k + j) * n];
+  }
+  __syncthreads();
+  for (int j = 0; j < kb; ++j) {
+    if (threadIdx.x == 0) {
+      const T d = L[j][j];
+      if (!(d > T(0))) {
+        if (*info == 0) *info = k + j + 1;
+        L[j][j] = T(1);
+      } else {
+        L[j][j] = sqrt(d);
+      }
+    }
+    __syncthreads();
+    const T djj = L[j][j];
+    for (int i = j + 1 + threadIdx.x; i < kb; i += blockDim.x) L[i][j] /= djj;
+    __syncthreads();
+    // Trailing update of the block (lower part).
+    for (int idx = threadIdx.x; idx < (kb - j - 1) * (kb - j - 1); idx += blockDim.x) {
+      const int ii = j + 1 + idx % (kb - j - 1), jj = j + 1 + idx / (kb - j - 1);
+      if (ii >= jj) L[ii][jj] -= L[ii][j] * L[jj][j];
+    }
+    __syncthreads();
+  }
+  for (int idx = threadIdx.x; idx < kb * kb; idx += blockDim.x) {
+    const int i = idx % kb, j = idx / kb;
+    if (i >= j) A[(k + i) + (int64_t)(k + j) * n] = L[i][j];
+  }
+}
+
+// Panel: A[k+kb:n, k:k+kb] <- A[...] L11^-T; one 64-row slab per workgroup,
+// one thread per row.
+template <typename T>
+__global__ __launch_bounds__(64) void k_trsm_panel(T* __restrict__ A, int64_t n, int k, int kb) {
+  __shared__ T L[kNB][kNB + 1];
+  for (int idx = threadIdx.x; idx < kb * kb; idx += blockDim.x) {
+    const int i = idx % kb, j = idx / kb;
+    L[i][j] = (i >= j) ? A[(k + i) + (int64_t)(k + j) * n] : T(0);
+  }
+  __syncthreads();
+  const int64_t row = (int64_t)k + kb + (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (row >= n) return;
+  T x[kNB];
+  for (int j = 0; j < kb; ++j) x[j] = A[row + (int64_t)(k + j) * n];
+  for (int j = 0; j < kb; ++j) {
+    T s = x[j];
+    for (int l = 0; l < j; ++l) s -= x[l] * L[j][l];
+    x[j] = s / L[j][j];
+  }
+  for (int j = 0; j < kb; ++j) A[row + (int64_t)(k + j) * n] = x[j];
+}
+
+// Trailing update A22 -= L21 L21' on 64x64 lower tiles (register-tiled,
+// 256 threads, 4x4 outputs per thread).  Tiles strictly above the diagonal
+// are skipped.
+template <typename T>
+__global__ __launch_bounds__(256) void k_syrk_update(T* __restrict__ A, int64_t n, int k, int kb,
+                                                     int ntiles) {
+  __shared__ T As[kNB][kNB + 1];
+  __shared__ T Bs[kNB][kNB + 1];
+  // Map blockIdx.x to a lower-triangle tile (ti >= tj).
+  const int t = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  while (ti * (ti + 1) / 2 > t) --ti;
+  const int tj = t - ti * (ti + 1) / 2;
+  if (ti >= ntiles) return;
+  const int64_t base = (int64_t)k + kb;
+  const int64_t r0 = base + (int64_t)ti * kNB, c0 = base + (int64_t)tj * kNB;
+  for (int idx = threadIdx.x; idx < kNB * kb; idx += blockDim.x) {
+    const int i = idx % kNB, l = idx / kNB;
+    As[i][l] = (r0 + i < n) ? A[(r0 + i) + (int64_t)(k + l) * n] : T(0);
+    Bs[i][l] = (c0 + i < n) ? A[(c0 + i) + (int64_t)(k + l) * n] : T(0);
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  T acc[4][4];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+  for (int l = 0; l < kb; ++l) {
+    T av[4], bv[4];
+    for (int a = 0; a < 4; ++a) av[a] = As[ty * 4 + a][l];
+    for (int b = 0; b < 4; ++b) bv[b] = Bs[tx * 4 + b][l];
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+  }
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      const int64_t row = r0 + ty * 4 + a, col = c0 + tx * 4 + b;
+      if (row < n && col < n && row >= col) A[row + col * n] -= acc[a][b];
+    }
+}
+
+// Triangular solves with the factor (one workgroup; columns blocked by 64).
+// Forward: L y = b ; backward: L' x = y.  In place on b.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_lower(const T* __restrict__ L, int64_t n,
+                                                    T* __restrict__ b) {
+  __shared__ T xs[kNB];
+  for (int64_t k = 0; k < n; k += kNB) {
+    const int kb = (int)min((int64_t)kNB, n - k);
+    if (threadIdx.x == 0) {
+      for (int j = 0; j < kb; ++j) {
+        T s = b[k + j];
+        for (int l = 0; l < j; ++l) s -= L[(k + j) + (k + l) * n] * xs[l];
+        xs[j] = s / L[(k + j) + (k + j) * n];
+      }
+      for (int j = 0; j < kb; ++j) b[k + j] = xs[j];
+    }
+    __syncthreads();
+    for (int64_t i = k + kb + threadIdx.x; i < n; i += blockDim.x) {
+      T s = T(0);
+      for (int l = 0; l < kb; ++l) s += L[i + (k + l) * n] * xs[l];
+      b[i] -= s;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_upper(const T* __restrict__ L, int64_t n,
+                                                    T* __restrict__ b) {
+  __shared__ T xs[kNB];
+  __shared__ T red[4][kNB];
+  const int64_t nblk = (n + kNB - 1) / kNB;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t bi = nblk - 1; bi >= 0; --bi) {
+    const int64_t k = bi * kNB;
+    const int kb = (int)min((int64_t)kNB, n - k);
+    // b[k+j] -= sum_{i >= k+kb} L[i, k+j] b[i]: thread i-strided, 64 sums.
+    T acc[kNB];
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) acc[j] = T(0);
+    for (int64_t i = k + kb + threadIdx.x; i < n; i += blockDim.x) {
+      const T bv = b[i];
+#pragma unroll
+      for (int j = 0; j < kNB; ++j)
+        if (j < kb) acc[j] += L[i + (k + j) * n] * bv;
+    }
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) {
+      const T v = wave_sum(acc[j]);
+      if (lane == 0) red[wave][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int j = kb - 1; j >= 0; --j) {
+        T s = b[k + j] - (red[0][j] + red[1][j] + red[2][j] + red[3][j]);
+        for (int l = j + 1; l < kb; ++l) s -= L[(k + l) + (k + j) * n] * xs[l];
+        xs[j] = s / L[(k + j) + (k + j) * n];
+      }
+      for (int j = 0; j < kb; ++j) b[k + j] = xs[j];
+    }
+    __syncthreads();
+  }
+}
+
+// FP64 residual r = b - S x with S symmetric stored lower (column-major T).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sym_residual64(const T* __restrict__ S, int64_t n,
+                                                           const double* __restrict__ b,
+                                                           const double* __restrict__ x,
+                                                           double* __restrict__ r) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int64_t j = 0; j < n; ++j) {
+    const double a = (i >= j) ? (double)S[i + j * n] : (double)S[j + i * n];
+    s += a * x[j];
+  }
+  r[i] = b[i] - s;
+}
+
+// ------------------------------------------------------- small helpers
+__global__ __launch_bounds__(kBlock) void k_axpy_inplace(int64_t n, const double* __restrict__ dx,
+                                                         double* __restrict__ x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] += dx[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_negate(int64_t n, const T* __restrict__ src, T* __restrict__ dst) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = -src[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy64(int64_t n, const double* __restrict__ src,
+                                                   double* __restrict__ dst) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill64(int64_t n, double v, double* __restrict__ dst) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = v;
+}
+
+// Trial cameras: out[cam] = cams[cam] + step .* scale for active cameras
+// (F order step), with partial sums of |delta|^2.
+__global__ __launch_bounds__(kBlock) void k_plus_cams(int nf, const int32_t* __restrict__ f_to_cam,
+                                                      const double* __restrict__ cams,
+                                                      const double* __restrict__ step,
+                                                      const double* __restrict__ scale,
+                                                      double* __restrict__ out,
+                                                      double* __restrict__ partial) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < 9 * (int64_t)nf) {
+    const size_t j = 9 * (size_t)f_to_cam[i / 9] + i % 9;
+    const double v = cams[j] + step[i] * scale[i];
+    out[j] = v;
+    const double df = cams[j] - v;
+    acc[0] = df * df;
+  }
+  block_sum<double, 1>(acc, lds);
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
+// Jacobi scaling 1 / (1 + |column|) from an unscaled pass
+// (TrustRegionMinimizer::EstimateScale, trust_region_minimizer.cc:103-109).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_make_scale(int ne, int nf, const T* __restrict__ V,
+                                                       const T* __restrict__ U,
+                                                       double* __restrict__ pscale,
+                                                       double* __restrict__ cscale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ne) {
+    const int di[3] = {0, 3, 5};
+    for (int j = 0; j < 3; ++j) pscale[3 * (size_t)i + j] = 1.0 / (1.0 + sqrt((double)V[6 * (size_t)i + di[j]]));
+  }
+  if (i < nf)
+    for (int j = 0; j < 9; ++j) cscale[9 * (size_t)i + j] = 1.0 / (1.0 + sqrt((double)U[45 * (size_t)i + ut9(j, j)]));
+}
+
+// CostFunction::Evaluate per observation in input order (no loss, no
+// scaling): residual, row-major 2x9 camera and 2x3 point Jacobians.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_eval_obs(int64_t n, const int32_t* __restrict__ cam_idx,
+                                                     const int32_t* __restrict__ pt_idx,
+                                                     const T* __restrict__ obs, const T* __restrict__ ctab,
+                                                     const T* __restrict__ pts, double* __restrict__ res,
+                                                     double* __restrict__ jc, double* __restrict__ jp) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T* tab = ctab + (size_t)kCamTab * cam_idx[i];
+  CamProj<T> c;
+  load_proj(tab, c);
+  const int q = pt_idx[i];
+  const T X[3] = {pts[3 * (size_t)q], pts[3 * (size_t)q + 1], pts[3 * (size_t)q + 2]};
+  Proj<T> o;
+  project(c, X, obs[2 * i], obs[2 * i + 1], o);
+  if (res) {
+    res[2 * i] = (double)o.rx;
+    res[2 * i + 1] = (double)o.ry;
+  }
+  const T one[9] = {T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1)};
+  if (jc) {
+    T J[18];
+    jac_cam(o, tab + kTabDR, X, T(1), one, J);
+    for (int k = 0; k < 18; ++k) jc[18 * i + k] = (double)J[k];
+  }
+  if (jp) {
+    T J[6];
+    jac_point(o, c.R, T(1), one, J);
+    for (int k = 0; k < 6; ++k) jp[6 * i + k] = (double)J[k];
+  }
+}
+
+}  // namespace mba
+
+#endif  // MBA_DEVICE_H_

@@ -19,7 +19,9 @@
  *   aa ~ N(0, 0.1^2) per component, t ~ N(0, 1), f ~ U(450, 550),
  *   k1 ~ N(0, 0.01^2), k2 ~ N(0, 0.001^2);
  *   points uniform in [-2,2] x [-2,2] x [4,10];
- *   a (camera, point) pair whose depth is <= 0.5 is redrawn;
+ *   a (camera, point) pair whose depth is <= 0.5, or that falls outside a
+ *   90-degree field of view (|x/z| > 1 or |y/z| > 1: an image of +-f
+ *   pixels), is redrawn -- a camera records only points inside its image;
  *   pixel noise N(0, noise^2) per coordinate; a fraction of observations is
  *   shifted by outlier_px in a uniformly random direction;
  *   the initial guess multiplies every parameter by (1 + N(0, perturb^2)).
@@ -108,7 +110,8 @@ static inline void mba_scene_rotate(const double aa[3], const double p[3],
   }
 }
 
-/* Projection of a point by a packed 9-parameter camera; returns the depth. */
+/* Projection of a point by a packed 9-parameter camera; returns the depth
+ * (or -1 when the point lies outside the 90-degree field of view). */
 static inline double mba_scene_project(const double cam[9], const double X[3],
                                        double uv[2]) {
   double x[3];
@@ -121,6 +124,7 @@ static inline double mba_scene_project(const double cam[9], const double X[3],
   const double d = 1.0 + cam[7] * r2 + cam[8] * r2 * r2;
   uv[0] = cam[6] * xe * d;
   uv[1] = cam[6] * ye * d;
+  if (fabs(xe) > 1.0 || fabs(ye) > 1.0) return -1.0;
   return x[2];
 }
 

@@ -1,0 +1,218 @@
+"""Benchmark: Jacobian+Schur observations/s and LM time-to-tolerance on one MI355X.
+
+Workload (one "step"): one Jacobian + Schur pass of the bundle-adjustment
+engine over the 1,000-camera / 500,000-point / ~4.0M-observation scene with
+5% 20-px outliers under a 4-px Huber loss, mixed precision (FP32 residuals,
+Jacobians and block products; FP64 point-block inverses):
+residuals, analytic Jacobian (Huber-corrected, Jacobi-scaled), V_p, its FP64
+inverse, g_p, U_c, g_c and the reduced right-hand side.  Inputs are resident
+in HBM when the timed region starts.
+
+Also reported:
+  * the same pass in FP64;
+  * the LM solve (implicit-Schur PCG) in mixed precision and in FP64 to
+    tolerance: time until the FP64-evaluated cost is within 1e-6 of the FP64
+    path's final cost;
+  * the dominant kernel's roofline (HIP events on the engine's stream);
+  * the CPU baseline: the FP64 oracle's Jacobian + Schur pass on the host
+    cores (OpenMP), on the same scene.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+HUBER_A = 4.0
+RADIUS = 1e4
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cam_pass_bytes(nrows, ne, nf, nchunks, elem):
+    """Algorithmic bytes of one camera-pass launch: per camera-major row the
+    point index (4 B) and observation (2 elem); per point block its position,
+    Jacobi scale and V^-1 g_p (3 elem each) read once; per camera its table
+    row and scale; per chunk 63 partial sums written."""
+    return (nrows * (4 + 2 * elem) + ne * 9 * elem + nf * (48 + 9) * elem + nchunks * 63 * elem)
+
+
+def point_pass_bytes(nrows, ne, nc, elem):
+    """Per row camera id (4 B) + observation; per point start (4 B),
+    position, scale (read) and V, g_p, V^-1, w (write, 18 elem) plus the FP64
+    LM diagonal (24 B); camera tables once."""
+    return nrows * (4 + 2 * elem) + ne * (4 + 6 * elem + 18 * elem + 24) + nc * 48 * elem
+
+
+def run(args):
+    import torch
+
+    mba = importlib.import_module("3dreconstruction_amd")
+    mba.load_library()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log("bench: generating the 4M-observation scene")
+    d = mba.generate_scene(3)
+    nobs = d["obs"].shape[0]
+    nc, npt = d["cam_init"].shape[0], d["pt_init"].shape[0]
+
+    def problem():
+        return mba.BAProblem(d["cam_init"].copy(), d["pt_init"].copy(), d["obs"], d["cam_idx"], d["pt_idx"])
+
+    def time_pass(precision):
+        ctx = mba.Context(problem(), mba.default_options(precision=precision, huber_a=HUBER_A,
+                                                         linear_solver=mba.IMPLICIT_SCHUR_PCG))
+        stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
+        for _ in range(args.warmup):
+            ctx.jacobian_schur(RADIUS)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            ctx.jacobian_schur(RADIUS)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = e0.elapsed_time(e1)
+        # Per-kernel durations: HIP events recorded by the engine around
+        # each kernel on its stream.
+        ctx.set_timing(True)
+        kern = np.zeros(3)
+        for _ in range(args.steps):
+            ctx.jacobian_schur(RADIUS)
+            kern += ctx.last_timing()
+        ctx.set_timing(False)
+        kern /= args.steps
+        ctx.close()
+        return ms / args.steps, wall / args.steps, kern
+
+    log("bench: mixed-precision Jacobian+Schur pass")
+    ms_mixed, wall_mixed, kern_mixed = time_pass(mba.PRECISION_MIXED)
+    log(f"  {ms_mixed:.4f} ms/step (events), kernels point {kern_mixed[0]:.4f} camera {kern_mixed[1]:.4f} ms")
+    log("bench: FP64 Jacobian+Schur pass")
+    ms_fp64, _, kern_fp64 = time_pass(mba.PRECISION_FP64)
+    log(f"  {ms_fp64:.4f} ms/step")
+
+    # Structure sizes for the roofline (from the library's own ordering).
+    ne, order, _, _ = mba.schur_structure(problem(), with_pairs=False)
+    nf = order.shape[0] - ne
+    rows_per_chunk = max(64, min(2048, nobs // 2048))
+    rows_per_chunk = (rows_per_chunk + 63) // 64 * 64
+    nchunks = int(np.sum(np.ceil(np.bincount(d["cam_idx"], minlength=nc) / rows_per_chunk)))
+    if kern_mixed[1] >= kern_mixed[0]:
+        name, kms, nbytes = "k_cam_pass<float>", kern_mixed[1], cam_pass_bytes(nobs, ne, nf, nchunks, 4)
+    else:
+        name, kms, nbytes = "k_point_pass<float>", kern_mixed[0], point_pass_bytes(nobs, ne, nc, 4)
+    achieved = nbytes / (kms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_jacobian_schur_4M.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(name)
+
+    lm = None
+    if not args.no_lm:
+        log("bench: LM solves to tolerance (implicit-Schur PCG)")
+        res = {}
+        for label, prec in (("fp64", mba.PRECISION_FP64), ("mixed", mba.PRECISION_MIXED)):
+            p = problem()
+            opts = mba.default_options(precision=prec, huber_a=HUBER_A, linear_solver=mba.IMPLICIT_SCHUR_PCG,
+                                       max_num_iterations=args.lm_iters)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s = mba.solve(p, opts)
+            torch.cuda.synchronize()
+            res[label] = (s, time.perf_counter() - t0)
+            log(f"  {label}: final cost {s.final_cost:.10e} in {res[label][1]:.3f} s, "
+                f"{s.num_successful_steps}+{s.num_unsuccessful_steps} steps, {s.num_linear_solver_iterations} CG")
+        target = res["fp64"][0].final_cost * (1.0 + 1e-6)
+
+        def time_to(s):
+            for it in s.iterations:
+                if it.cost <= target:
+                    return s.preprocessor_time_s + it.cumulative_time_s
+            return None
+
+        t_mixed, t_fp64 = time_to(res["mixed"][0]), time_to(res["fp64"][0])
+        lm = {"solver": "implicit-Schur PCG (JACOBI), eta 0.1", "max_iterations": args.lm_iters,
+              "fp64_final_cost": res["fp64"][0].final_cost, "mixed_final_cost": res["mixed"][0].final_cost,
+              "final_cost_rel_diff": abs(res["mixed"][0].final_cost - res["fp64"][0].final_cost)
+              / res["fp64"][0].final_cost,
+              "time_to_tol_s": {"mixed": t_mixed, "fp64": t_fp64},
+              "solve_wall_s": {"mixed": res["mixed"][1], "fp64": res["fp64"][1]},
+              "speedup_mixed_vs_fp64": (t_fp64 / t_mixed) if (t_mixed and t_fp64) else None}
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # CPU baseline leg only
+
+        threads = max(1, min(16, os.cpu_count() or 1))
+        sc = O.Scene(dict(cam_init=d["cam_init"], pt_init=d["pt_init"], obs=d["obs"], cam_idx=d["cam_idx"],
+                          pt_idx=d["pt_idx"]))
+        log(f"bench: CPU baseline (FP64 oracle, {threads} threads)")
+        O.jacobian_schur(sc, huber_a=HUBER_A, num_threads=threads)
+        reps = args.cpu_reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            O.jacobian_schur(sc, huber_a=HUBER_A, num_threads=threads)
+        dt = (time.perf_counter() - t0) / reps
+        cpu = {"value": nobs / dt, "unit": "obs/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle Jacobian+Schur pass over the full {nobs}-observation scene, "
+                         f"{reps} passes after one warm-up, OpenMP {threads} threads"}
+
+    value = nobs / (ms_mixed * 1e-3)
+    out = {
+        "metric": "Jacobian+Schur obs/s; LM time-to-RMS-tol, mixed vs FP64 on GPU, 4M obs",
+        "value": value, "unit": "obs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_mixed, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (FP64 point-block inverses)", "data": "synthetic",
+        "config": {"workload": "jacobian_schur_pass", "cameras": nc, "points": npt, "observations": nobs,
+                   "huber_a_px": HUBER_A, "outliers": "5% x 20 px", "precision": "mixed",
+                   "parallelism": "single GPU"},
+        "fp64_pass": {"ms_per_step": ms_fp64, "value": nobs / (ms_fp64 * 1e-3), "unit": "obs/s",
+                      "mixed_speedup": ms_fp64 / ms_mixed},
+        "kernels_ms": {"point_pass": kern_mixed[0], "camera_pass": kern_mixed[1], "pass_total": kern_mixed[2]},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": nbytes, "kernel_ms": kms},
+        "lm": lm,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--lm-iters", type=int, default=50)
+    ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--no-lm", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the engine runs a problem on a single MI355X; "
+              "multi-GPU sharding is a design note only (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    run(args)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,266 @@
+/*
+ * mba.h -- C ABI of the MI355X bundle-adjustment engine.
+ *
+ * Drop-in for the Ceres call that libs/sfm makes for bundle adjustment
+ * (libs/sfm/src/sfm_incremental_engine.cpp:1413-1472,
+ *  libs/sfm/src/sfm_global_engine.cpp:1548-1594 and :1758-1802):
+ *
+ *   ceres::Problem problem;                                  -> mba_problem
+ *   problem.AddResidualBlock(cost, HuberLoss(a), cam, point) -> one observation
+ *       (include/ceres/problem.h:214-258)
+ *   problem.SetParameterBlockConstant(cam)                   -> camera_constant[]
+ *       (include/ceres/problem.h:307)
+ *   ceres::Solver::Options options                           -> mba_options
+ *       (include/ceres/solver.h:59-129)
+ *   ceres::Solve(options, &problem, &summary)                -> mba_solve
+ *       (include/ceres/solver.h:1020, internal/ceres/solver.cc:540-545)
+ *   summary.initial_cost / final_cost / IsSolutionUsable()   -> mba_summary
+ *       (include/ceres/solver.h:753-831, internal/ceres/solver_utils.h:41-56)
+ *   CostFunction::Evaluate(params, residuals, jacobians)     -> mba_evaluate
+ *       (include/ceres/cost_function.h:114-116)
+ *
+ * Conventions (same as the reference):
+ *   - the residual of an observation is predicted - observed (pixels) with
+ *     the libs/sfm angle-axis pinhole model and radial distortion
+ *     (libs/sfm/include/mvg/sfm/pinhole_brown_rt_ceres_functor.h:60-107);
+ *     a camera is packed as 9 doubles [aa(3), t(3), f, k1, k2] (principal
+ *     point 0, k3 = 0), a point as 3 doubles;
+ *   - cost = 1/2 sum rho(|r|^2) (HuberLoss with threshold huber_a pixels,
+ *     include/ceres/loss_function.h:175-184; huber_a <= 0 means no loss);
+ *   - all arrays are caller-owned; cameras and points are written back in
+ *     place only when the solution is usable (CONVERGENCE or
+ *     NO_CONVERGENCE), as Ceres does (internal/ceres/solver.cc:442-445);
+ *   - no entry point aborts the process: every failure is a status code,
+ *     with a message from mba_last_error() (thread-local).
+ *
+ * Threading: every call works on its own context and HIP stream; independent
+ * problems may be solved concurrently from several host threads (the global
+ * engine does this from an OpenMP loop, sfm_global_engine.cpp:985).
+ */
+#ifndef MBA_H_
+#define MBA_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBA_ABI_VERSION 1
+
+typedef enum mba_status {
+  MBA_OK = 0,
+  MBA_ERR_INVALID_ARGUMENT = -1, /* bad sizes, indices out of range, NULL */
+  MBA_ERR_UNSUPPORTED = -2,      /* structure the engine does not handle */
+  MBA_ERR_DEVICE = -3,           /* HIP runtime error / no device */
+  MBA_ERR_OUT_OF_MEMORY = -4,
+  MBA_ERR_NOT_FINITE = -5        /* non-finite input parameters */
+} mba_status;
+
+/* Ceres TerminationType (include/ceres/types.h:315-350); USER_* unused. */
+typedef enum mba_termination {
+  MBA_CONVERGENCE = 0,
+  MBA_NO_CONVERGENCE = 1,
+  MBA_FAILURE = 2
+} mba_termination;
+
+typedef enum mba_layout {
+  MBA_LAYOUT_PACKED9 = 0 /* camera [aa(3), t(3), f, k1, k2] + point(3) */
+} mba_layout;
+
+/* Linear solver of the LM step.  EXPLICIT_SCHUR forms the reduced camera
+ * matrix S and factors it (the SPARSE_SCHUR / DENSE_SCHUR role);
+ * IMPLICIT_SCHUR_PCG is ITERATIVE_SCHUR with the block-Jacobi (JACOBI)
+ * preconditioner. */
+typedef enum mba_linear_solver {
+  MBA_EXPLICIT_SCHUR = 0,
+  MBA_IMPLICIT_SCHUR_PCG = 1
+} mba_linear_solver;
+
+/* Arithmetic of the residual / Jacobian / Schur / PCG kernels.  The LM cost
+ * and gain ratio are always FP64; point-block inverses are always FP64. */
+typedef enum mba_precision {
+  MBA_PRECISION_FP64 = 0,
+  MBA_PRECISION_MIXED = 1 /* FP32 residuals, Jacobians, matvecs, S */
+} mba_precision;
+
+typedef struct mba_problem {
+  int32_t layout;       /* mba_layout */
+  int32_t num_cameras;
+  int32_t num_points;
+  int32_t reserved0;
+  int64_t num_observations;
+  double* cameras;              /* 9 * num_cameras, in/out */
+  double* points;               /* 3 * num_points, in/out */
+  const double* observations;   /* 2 * num_observations, pixels */
+  const int32_t* camera_index;  /* per observation */
+  const int32_t* point_index;   /* per observation */
+  const uint8_t* camera_constant; /* num_cameras flags, or NULL */
+} mba_problem;
+
+typedef struct mba_options {
+  /* Solver::Options fields (defaults of include/ceres/solver.h:62-129). */
+  int32_t max_num_iterations;                 /* 50 */
+  int32_t max_num_consecutive_invalid_steps;  /* 5 */
+  int32_t max_linear_solver_iterations;       /* 500 */
+  int32_t min_linear_solver_iterations;       /* 0 */
+  int32_t jacobi_scaling;                     /* 1 */
+  int32_t linear_solver;                      /* mba_linear_solver, default EXPLICIT */
+  int32_t precision;                          /* mba_precision, default FP64 */
+  int32_t device;                             /* HIP device ordinal, default 0 */
+  double function_tolerance;                  /* 1e-6 */
+  double gradient_tolerance;                  /* 1e-10 */
+  double parameter_tolerance;                 /* 1e-8 */
+  double initial_trust_region_radius;         /* 1e4 */
+  double max_trust_region_radius;             /* 1e16 */
+  double min_trust_region_radius;             /* 1e-32 */
+  double min_relative_decrease;               /* 1e-3 */
+  double min_lm_diagonal;                     /* 1e-6 */
+  double max_lm_diagonal;                     /* 1e32 */
+  double eta;                                 /* 0.1: CG q-tolerance */
+  /* PCG relative-residual tolerance |r| <= r_tolerance |b|.  The reference
+   * LM strategy disables it (-1, levenberg_marquardt_strategy.cc:91-96). */
+  double pcg_r_tolerance;                     /* -1 */
+  double huber_a;                             /* <= 0: no loss (NULL) */
+  /* Explicit Schur: FP64 iterative-refinement sweeps after the (FP32 in
+   * MIXED) Cholesky solve of S; 0 disables. */
+  int32_t refinement_iterations;              /* 2 */
+  int32_t reserved1;
+} mba_options;
+
+typedef struct mba_iteration {
+  int32_t iteration;
+  int32_t step_is_valid;
+  int32_t step_is_successful;
+  int32_t linear_solver_iterations;
+  double cost;
+  double cost_change;
+  double gradient_max_norm;
+  double gradient_norm;
+  double step_norm;
+  double relative_decrease;
+  double trust_region_radius;
+  double cumulative_time_s;
+} mba_iteration;
+
+typedef struct mba_summary {
+  double initial_cost;
+  double final_cost;
+  double fixed_cost;
+  double preprocessor_time_s;
+  double jacobian_evaluation_time_s;   /* residual + Jacobian + Schur blocks */
+  double residual_evaluation_time_s;   /* cost-only evaluations */
+  double linear_solver_time_s;
+  double total_time_s;
+  int32_t num_successful_steps;
+  int32_t num_unsuccessful_steps;
+  int32_t num_iterations;              /* iteration records produced */
+  int32_t termination_type;            /* mba_termination */
+  int32_t num_e_blocks;                /* eliminated (point) blocks */
+  int32_t num_f_blocks;                /* reduced (camera) blocks */
+  int32_t num_linear_solver_iterations;
+  int32_t is_solution_usable;
+} mba_summary;
+
+/* Options with the reference defaults. */
+mba_status mba_default_options(mba_options* options);
+
+/* ceres::Solve for a bundle-adjustment problem.  `iterations` (may be NULL)
+ * receives up to `max_iterations` IterationSummary records. */
+mba_status mba_solve(const mba_options* options, mba_problem* problem,
+                     mba_summary* summary, mba_iteration* iterations,
+                     int32_t max_iterations);
+
+/* CostFunction::Evaluate for every observation, input order, computed on the
+ * device in `precision`.  residuals: 2 per observation (no loss applied);
+ * jac_camera: row-major 2x9 per observation; jac_point: row-major 2x3; each
+ * may be NULL.  cost: robustified 1/2 sum rho (FP64), may be NULL. */
+mba_status mba_evaluate(const mba_problem* problem, int32_t precision,
+                        double huber_a, double* residuals, double* jac_camera,
+                        double* jac_point, double* cost, int32_t device);
+
+/* The bit-exact structure the reference's preprocessor builds
+ * (reorder_program.cc:436-531, schur_complement_solver.cc:246-313):
+ *   param_order: column blocks in order (cameras 0..Nc-1, points Nc..);
+ *   num_e_blocks: size of the eliminated group;
+ *   residual_order: input observation index of each row block;
+ *   pairs: upper-triangle block pairs of S (reduced indices), set order.
+ * Any output may be NULL; *num_pairs receives the pair count; returns
+ * MBA_ERR_INVALID_ARGUMENT if pairs_capacity is too small. */
+mba_status mba_schur_structure(const mba_problem* problem, int32_t* param_order,
+                               int32_t* num_param_blocks, int32_t* num_e_blocks,
+                               int64_t* residual_order, int32_t* pairs,
+                               int64_t pairs_capacity, int64_t* num_pairs);
+
+/* ------------------------------------------------- resident contexts ----
+ * A context owns the preprocessed problem on one device (ordering, device
+ * copies, stream).  It lets a caller (or a benchmark) run the engine's
+ * passes repeatedly on HBM-resident data. */
+typedef struct mba_context mba_context;
+
+mba_status mba_context_create(const mba_problem* problem,
+                              const mba_options* options, mba_context** out);
+mba_status mba_context_destroy(mba_context* ctx);
+
+/* One Jacobian + Schur pass at the context's current parameters: residuals,
+ * Jacobians (Huber-corrected, Jacobi-scaled), V_p = E'E + D_p^2 and its FP64
+ * inverse, g_p, U_c = F'F + D_c^2, g_c and the reduced right-hand side
+ * rhs_c = F'(r - E V^-1 g_p), for the LM diagonal at trust radius `radius`
+ * (radius <= 0: no damping).  Asynchronous on the context stream. */
+mba_status mba_context_jacobian_schur(mba_context* ctx, double radius);
+
+/* Copy the blocks of the last pass to the host, in INPUT ids: U (45 per
+ * camera, upper triangle row-major), g_c (9), rhs (9), V (6 per point:
+ * xx xy xz yy yz zz), g_p (3).  Any output may be NULL. */
+mba_status mba_context_read_schur(mba_context* ctx, double* U, double* gc,
+                                  double* rhs, double* V, double* gp);
+
+/* Jacobi column scaling in use (1 / (1 + |column|) of the first Jacobian,
+ * trust_region_minimizer.cc:103-109), input ids: 9 per camera, 3 per point
+ * (1 for blocks outside the reduced program).  Either output may be NULL. */
+mba_status mba_context_read_scale(mba_context* ctx, double* cam_scale, double* pt_scale);
+
+/* Implicit reduced-camera matvec y = S x of the last pass (S = U + D^2 -
+ * W V^-1 W'), x and y 9 per camera in input ids (host arrays). */
+mba_status mba_context_schur_matvec(mba_context* ctx, const double* x, double* y);
+
+/* Run the full LM solve on the context (parameters stay on the device until
+ * mba_context_download). */
+mba_status mba_context_solve(mba_context* ctx, mba_summary* summary,
+                             mba_iteration* iterations, int32_t max_iterations);
+
+/* Copy the context parameters back to the problem arrays. */
+mba_status mba_context_download(mba_context* ctx, mba_problem* problem);
+
+/* Block until the context stream is idle. */
+mba_status mba_context_synchronize(mba_context* ctx);
+
+/* The HIP stream (hipStream_t) the context launches on. */
+void* mba_context_stream(mba_context* ctx);
+
+/* Per-kernel timing of the last mba_context_jacobian_schur, measured with
+ * HIP events on the context stream: ms[0] point pass, ms[1] camera pass,
+ * ms[2] total.  Enabled with mba_context_set_timing(ctx, 1). */
+mba_status mba_context_set_timing(mba_context* ctx, int32_t enable);
+mba_status mba_context_last_timing(mba_context* ctx, double* ms, int32_t n);
+
+/* --------------------------------------------------------- utilities --- */
+const char* mba_status_string(mba_status status);
+const char* mba_last_error(void);
+int32_t mba_abi_version(void);
+
+/* Seeded synthetic scene (no counterpart in the reference; the reference
+ * test builds scenes with NRealisticCamerasRing,
+ * libs/multiview/src/nview_data_sets.cpp:23-64).  preset 1..5 are the
+ * benchmark scenes (4 = window `window` of the batched set).  Call with
+ * NULL arrays to get the sizes, then with arrays of those sizes. */
+mba_status mba_scene_generate(int32_t preset, int32_t window, int32_t* num_cameras,
+                              int32_t* num_points, int64_t* num_observations,
+                              double* cam_true, double* cam_init, double* pt_true,
+                              double* pt_init, double* observations,
+                              int32_t* camera_index, int32_t* point_index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MBA_H_ */

@@ -1,0 +1,100 @@
+"""Writes the golden fixtures of this directory from the reference's own Ceres
+Solver 1.10 (built from the reference sources by oracle/ref/Makefile into
+oracle/_ref/libceres_ref.so).  Runs only in a container that has the reference
+checkout; the fixtures it writes are plain numbers (inputs are regenerated
+from the seeded scene generator, so only seeds, indices and outputs are
+stored).
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle as O  # noqa: E402
+
+# Small scenes whose structure exercises the ordering beyond "all points E":
+# "tiny" has a camera of degree 2 that the independent set picks as an E
+# block; "const" fixes one camera (SetParameterBlockConstant).
+HUBER_SPEC = dict(num_cameras=30, num_points=3000, track_min=2, track_max=10, power_alpha=1.5,
+                  outlier_frac=0.05, seed=11)
+
+
+def tiny_scene():
+    """Hand-built: 4 cameras, 6 points; camera 3 sees only points 0 and 1."""
+    spec = O.make_spec(3, 6, 2, 3, seed=5)
+    d = O.generate(spec)
+    cams = np.vstack([d["cam_init"], d["cam_init"][0] * np.array([1, 1, 1, 1.1, 1, 1, 1, 1, 1])])
+    extra_obs = np.array([[10.0, -5.0], [3.0, 7.0]])
+    d2 = dict(cam_init=cams, pt_init=d["pt_init"],
+              obs=np.vstack([d["obs"], extra_obs]),
+              cam_idx=np.concatenate([d["cam_idx"], [3, 3]]).astype(np.int32),
+              pt_idx=np.concatenate([d["pt_idx"], [0, 1]]).astype(np.int32))
+    return d2
+
+
+def solve_fixture(d, huber, linear_solver, iters, const=None):
+    sc = O.Scene(d, cam_constant=const)
+    r = O.ref_solve(sc, huber_a=huber, linear_solver=linear_solver, max_iters=iters)
+    return dict(initial_cost=r["initial_cost"], final_cost=r["final_cost"], successful=r["successful"],
+                unsuccessful=r["unsuccessful"], termination=r["termination"], iterations=r["iterations"],
+                cameras=sc.cameras, points=sc.points)
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref/libceres_ref.so is not built (make -C oracle/ref)")
+    out = {}
+    # 1. LM trajectories (SPARSE_SCHUR + CX_SPARSE as libs/sfm sets it, and
+    #    ITERATIVE_SCHUR + JACOBI) on the 16-camera scene, 20 iterations.
+    d1 = O.generate(O.preset(1))
+    for name, ls in [("sparse", 0), ("iterative", 1)]:
+        f = solve_fixture(d1, 0.0, ls, 20)
+        for k, v in f.items():
+            out[f"c1_{name}_{k}"] = np.asarray(v)
+    # 2. Huber (a = 4 px) with 5% outliers, both solvers; a constant camera.
+    dh = O.generate(O.make_spec(**HUBER_SPEC))
+    for name, ls in [("sparse", 0), ("iterative", 1)]:
+        f = solve_fixture(dh, 4.0, ls, 20)
+        for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
+            out[f"huber_{name}_{k}"] = np.asarray(f[k])
+    const = np.zeros(30, np.uint8)
+    const[3] = 1
+    f = solve_fixture(dh, 4.0, 0, 20, const)
+    for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
+        out[f"const_{k}"] = np.asarray(f[k])
+    # 3. Structure: Schur ordering, residual order, S block pairs.
+    for name, d, cc in [("c1", d1, None), ("huber", dh, const), ("tiny", tiny_scene(), None)]:
+        ne, order, res, pairs = O.ref_schur_ordering(O.Scene(d, cam_constant=cc))
+        out[f"struct_{name}_num_e"] = np.asarray(ne)
+        out[f"struct_{name}_order"] = order
+        out[f"struct_{name}_residual_order"] = res.astype(np.int32)
+        out[f"struct_{name}_pairs"] = pairs
+    # 4. Ceres autodiff Jets on 400 seeded observations of the 700k scene.
+    d2 = O.generate(O.preset(2))
+    idx = np.sort(np.random.default_rng(2).choice(d2["num_obs"], 400, replace=False))
+    R, JC, JP = [], [], []
+    for i in idx:
+        c = d2["cam_init"][d2["cam_idx"][i]]
+        p = d2["pt_init"][d2["pt_idx"][i]]
+        r, jc, jp = O.ref_evaluate_obs(0, c, p, d2["obs"][i])
+        R.append(r)
+        JC.append(jc)
+        JP.append(jp)
+    out["jets_c2_index"] = idx
+    out["jets_c2_residual"] = np.array(R)
+    out["jets_c2_jac_camera"] = np.array(JC)
+    out["jets_c2_jac_point"] = np.array(JP)
+    path = os.path.join(HERE, "ceres_ref.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()

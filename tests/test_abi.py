@@ -1,0 +1,114 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, keeps the reference defaults, and its host-side
+preprocessing reproduces the reference's Schur structure bit for bit."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, ba_problem
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "mba.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:[\w\s\*]+?)\b(mba_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol(mba):
+    lib = mba.load_library()
+    syms = header_symbols()
+    assert len(syms) >= 18
+    for name in syms:
+        assert hasattr(lib, name), name
+    assert set(syms) == set(mba.EXPORTS)
+    assert lib.mba_abi_version() == 1
+
+
+def test_default_options_are_the_reference_defaults(mba):
+    """include/ceres/solver.h:62-129."""
+    o = mba.default_options()
+    assert o.max_num_iterations == 50
+    assert o.max_num_consecutive_invalid_steps == 5
+    assert o.function_tolerance == 1e-6 and o.gradient_tolerance == 1e-10 and o.parameter_tolerance == 1e-8
+    assert o.initial_trust_region_radius == 1e4 and o.max_trust_region_radius == 1e16
+    assert o.min_trust_region_radius == 1e-32 and o.min_relative_decrease == 1e-3
+    assert o.min_lm_diagonal == 1e-6 and o.max_lm_diagonal == 1e32
+    assert o.eta == 0.1 and o.pcg_r_tolerance == -1.0 and o.max_linear_solver_iterations == 500
+    assert o.jacobi_scaling == 1
+
+
+@pytest.mark.parametrize("name", ["c1", "huber", "tiny"])
+def test_schur_structure_bit_exact_with_reference(mba, golden, c1, huber_scene, tiny_scene, name):
+    d = {"c1": c1, "huber": huber_scene, "tiny": tiny_scene}[name]
+    const = None
+    if name == "huber":
+        const = np.zeros(30, np.uint8)
+        const[3] = 1
+    ne, order, res, pairs = mba.schur_structure(ba_problem(mba, d, const))
+    assert ne == int(golden[f"struct_{name}_num_e"])
+    np.testing.assert_array_equal(order, golden[f"struct_{name}_order"])
+    np.testing.assert_array_equal(res, golden[f"struct_{name}_residual_order"])
+    np.testing.assert_array_equal(pairs, golden[f"struct_{name}_pairs"])
+
+
+def test_schur_structure_matches_oracle_on_700k_scene(mba, O):
+    d = O.generate(O.preset(2))
+    ne, order, res, pairs = mba.schur_structure(ba_problem(mba, d))
+    ne2, order2, res2 = O.schur_ordering(O.Scene(d))
+    assert ne == ne2
+    np.testing.assert_array_equal(order, order2)
+    np.testing.assert_array_equal(res, res2)
+    np.testing.assert_array_equal(pairs, O.schur_block_pairs(O.Scene(d)))
+
+
+def test_scene_generator_matches_oracle_copy(mba, O):
+    for preset, window in [(1, 0), (4, 7)]:
+        a = mba.generate_scene(preset, window)
+        b = O.generate(O.preset(preset, window))
+        for k in ["cam_true", "cam_init", "pt_true", "pt_init", "obs", "cam_idx", "pt_idx"]:
+            np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_scene_statistics_follow_the_configs(O):
+    """Observation counts of the seeded configs (~8k, ~700k, ~4M)."""
+    L = O.lib()
+    for cfg, lo, hi in [(1, 7500, 8500), (2, 650_000, 750_000), (3, 3_800_000, 4_200_000)]:
+        sp = O.preset(cfg)
+        tl = np.zeros(sp.num_points, np.int32)
+        n = L.orc_scene_count(ctypes.byref(sp), tl.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        assert lo <= n <= hi, (cfg, n)
+
+
+def test_errors_are_status_codes_not_aborts(mba, c1):
+    p = ba_problem(mba, c1)
+    p.camera_index = p.camera_index.copy()
+    p.camera_index[7] = 99
+    with pytest.raises(mba.MbaError) as e:
+        mba.solve(p)
+    assert e.value.status == mba.ERR_INVALID_ARGUMENT
+    assert "out of range" in str(e.value)
+
+
+def test_non_point_elimination_is_reported_unsupported(mba, tiny_scene):
+    """The independent set puts a camera in E for this scene; the engine
+    declines rather than solving a different system."""
+    with pytest.raises(mba.MbaError) as e:
+        mba.solve(ba_problem(mba, tiny_scene))
+    assert e.value.status == mba.ERR_UNSUPPORTED
+
+
+def test_ceres_shaped_problem_builds_the_same_arrays(mba, c1):
+    prob = mba.Problem()
+    cams = [c1["cam_init"][i].copy() for i in range(c1["cam_init"].shape[0])]
+    pts = [c1["pt_init"][i].copy() for i in range(c1["pt_init"].shape[0])]
+    loss = mba.HuberLoss(4.0)
+    for i in range(200):
+        prob.AddResidualBlock(mba.ReprojectionError(c1["obs"][i]), loss, cams[c1["cam_idx"][i]],
+                              pts[c1["pt_idx"][i]])
+    prob.SetParameterBlockConstant(cams[c1["cam_idx"][0]])
+    arr = prob._arrays()
+    assert prob.NumResidualBlocks() == 200
+    assert arr.camera_constant is not None and arr.camera_constant.sum() == 1
+    np.testing.assert_array_equal(arr.observations, c1["obs"][:200])

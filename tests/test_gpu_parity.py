@@ -1,0 +1,240 @@
+"""GPU parity of the HIP engine against the FP64 oracle and the reference's
+Ceres fixtures.  Every call goes through the C ABI (libmba.so).
+
+Tolerances (stated per test):
+  FP64 path  -- Jets / blocks to 1e-10 relative, LM costs to 1e-9 relative;
+  mixed path -- per-entry |a - b| <= 1e-5 (|b| + column RMS) against FP64,
+                final cost within 1e-6 relative of the FP64 reference.
+"""
+import numpy as np
+import pytest
+
+from conftest import ba_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b, axis=0):
+    """max |a - b| / (|b| + RMS of b along `axis`)."""
+    rms = np.sqrt(np.mean(b * b, axis=axis, keepdims=True)) + 1e-30
+    return float(np.max(np.abs(a - b) / (np.abs(b) + rms)))
+
+
+def oracle_scale_D(O, d, scale, radius):
+    sc = O.Scene(d)
+    ref0 = O.jacobian_schur(sc, scale=scale)
+    ut = [i * 9 - i * (i - 1) // 2 for i in range(9)]
+    dc = np.clip(ref0["U"][:, ut], 1e-6, 1e32)
+    dp = np.clip(ref0["V"][:, [0, 3, 5]], 1e-6, 1e32)
+    return np.sqrt(np.concatenate([dc.ravel(), dp.ravel()]) / radius), ref0
+
+
+# --------------------------------------------------------------- evaluate --
+def test_evaluate_fp64_matches_oracle_jets(mba, O, c1):
+    r, jc, jp, cost = mba.evaluate(ba_problem(mba, c1), mba.PRECISION_FP64)
+    ro, jco, jpo, costo = O.evaluate(O.Scene(c1))
+    assert np.max(np.abs(r - ro)) < 1e-9
+    assert rel_err(jc, jco) < 1e-10 and rel_err(jp, jpo) < 1e-10
+    np.testing.assert_allclose(cost, costo, rtol=1e-12)
+
+
+def test_evaluate_fp64_matches_reference_autodiff(mba, O, golden):
+    d2 = O.generate(O.preset(2))
+    idx = golden["jets_c2_index"]
+    sub = dict(cam_init=d2["cam_init"], pt_init=d2["pt_init"], obs=d2["obs"][idx],
+               cam_idx=d2["cam_idx"][idx], pt_idx=d2["pt_idx"][idx])
+    r, jc, jp, _ = mba.evaluate(ba_problem(mba, sub), mba.PRECISION_FP64)
+    assert np.max(np.abs(r - golden["jets_c2_residual"])) < 1e-9
+    assert rel_err(jc, golden["jets_c2_jac_camera"]) < 1e-10
+    assert rel_err(jp, golden["jets_c2_jac_point"]) < 1e-10
+
+
+def test_adjoint_check_700k_mixed_vs_fp64_jets(mba, O):
+    """200 cameras, 100k points, ~700k observations: FP32 residual + analytic
+    Jacobian vs the FP64 Jets, per-entry 1e-5 (|b| + column RMS); residuals
+    within 1e-5 of the image-plane scale (|predicted| + f): an FP32 residual
+    is the difference of two pixel coordinates of size up to f |x/z|."""
+    d2 = O.generate(O.preset(2))
+    r, jc, jp, _ = mba.evaluate(ba_problem(mba, d2), mba.PRECISION_MIXED)
+    ro, jco, jpo, _ = O.evaluate(O.Scene(d2))
+    f = d2["cam_init"][d2["cam_idx"], 6][:, None]
+    pred = np.abs(ro + d2["obs"]) + f
+    assert np.max(np.abs(r - ro) / pred) < 1e-5
+    e_c, e_p = rel_err(jc, jco), rel_err(jp, jpo)
+    print(f"700k adjoint check: camera J {e_c:.2e}, point J {e_p:.2e}")
+    assert e_c < 1e-5 and e_p < 1e-5
+
+
+def test_jacobian_matches_central_differences(mba, O):
+    """Central differences, h = 1e-4 relative, on 300 observations (FP64 path)."""
+    d2 = O.generate(O.preset(2))
+    idx = np.random.default_rng(7).choice(d2["num_obs"], 300, replace=False)
+    sub = dict(cam_init=d2["cam_init"], pt_init=d2["pt_init"], obs=d2["obs"][idx],
+               cam_idx=d2["cam_idx"][idx], pt_idx=d2["pt_idx"][idx])
+    _, jc, jp, _ = mba.evaluate(ba_problem(mba, sub), mba.PRECISION_FP64)
+    for k in range(9):
+        h = 1e-4 * np.maximum(np.abs(sub["cam_init"][:, k]), 1e-2)
+        a, b = sub["cam_init"].copy(), sub["cam_init"].copy()
+        a[:, k] += h
+        b[:, k] -= h
+        ra = mba.evaluate(mba.BAProblem(a, sub["pt_init"], sub["obs"], sub["cam_idx"], sub["pt_idx"]), jacobians=False)[0]
+        rb = mba.evaluate(mba.BAProblem(b, sub["pt_init"], sub["obs"], sub["cam_idx"], sub["pt_idx"]), jacobians=False)[0]
+        fd = (ra - rb) / (2 * h[sub["cam_idx"]])[:, None]
+        assert rel_err(jc[:, :, k], fd) < 1e-5, k
+    for k in range(3):
+        h = 1e-4 * np.maximum(np.abs(sub["pt_init"][:, k]), 1e-2)
+        a, b = sub["pt_init"].copy(), sub["pt_init"].copy()
+        a[:, k] += h
+        b[:, k] -= h
+        ra = mba.evaluate(mba.BAProblem(sub["cam_init"], a, sub["obs"], sub["cam_idx"], sub["pt_idx"]), jacobians=False)[0]
+        rb = mba.evaluate(mba.BAProblem(sub["cam_init"], b, sub["obs"], sub["cam_idx"], sub["pt_idx"]), jacobians=False)[0]
+        fd = (ra - rb) / (2 * h[sub["pt_idx"]])[:, None]
+        assert rel_err(jp[:, :, k], fd) < 1e-5, k
+
+
+# -------------------------------------------------------- Jacobian + Schur --
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-5)])
+@pytest.mark.parametrize("cfg", [1, 2])
+def test_jacobian_schur_blocks_match_oracle(mba, O, prec, tol, cfg):
+    d = O.generate(O.preset(cfg))
+    huber = 4.0
+    with mba.Context(ba_problem(mba, d), mba.default_options(precision=prec, huber_a=huber)) as ctx:
+        ctx.jacobian_schur(1e4)
+        out = ctx.read_schur()
+        cs, ps = ctx.read_scale()
+    scale = np.concatenate([cs.ravel(), ps.ravel()])
+    sc = O.Scene(d)
+    ref0 = O.jacobian_schur(sc, huber_a=huber, scale=scale)
+    ut = [i * 9 - i * (i - 1) // 2 for i in range(9)]
+    D = np.sqrt(np.concatenate([np.clip(ref0["U"][:, ut], 1e-6, 1e32).ravel(),
+                                np.clip(ref0["V"][:, [0, 3, 5]], 1e-6, 1e32).ravel()]) / 1e4)
+    ref = O.jacobian_schur(sc, huber_a=huber, scale=scale, D=D)
+    for k, b in [("U", ref0["U"]), ("V", ref0["V"]), ("gc", ref["gc"]), ("gp", ref["gp"]), ("rhs", ref["rhs"])]:
+        e = rel_err(out[k], b)
+        assert e < tol, (k, e)
+
+
+def test_schur_matvec_matches_dense_schur_complement(mba, O):
+    d = O.generate(O.make_spec(8, 150, 2, 5, outlier_frac=0.1, seed=21))
+    with mba.Context(ba_problem(mba, d), mba.default_options(huber_a=4.0)) as ctx:
+        ctx.jacobian_schur(10.0)
+        cs, ps = ctx.read_scale()
+        x = np.random.default_rng(5).standard_normal((8, 9))
+        y = ctx.schur_matvec(x)
+    sc = O.Scene(d)
+    nc, npt, n = sc.num_cameras, sc.num_points, sc.num_obs
+    r, jc, jp, _ = O.evaluate(sc, huber_a=4.0)
+    scale = np.concatenate([cs.ravel(), ps.ravel()])
+    J = np.zeros((2 * n, 9 * nc + 3 * npt))
+    for i in range(n):
+        s = r[i] @ r[i]
+        w = np.sqrt(4.0 / np.sqrt(s)) if s > 16.0 else 1.0
+        J[2 * i:2 * i + 2, 9 * sc.cam_idx[i]:9 * sc.cam_idx[i] + 9] = w * jc[i]
+        J[2 * i:2 * i + 2, 9 * nc + 3 * sc.pt_idx[i]:9 * nc + 3 * sc.pt_idx[i] + 3] = w * jp[i]
+    J *= scale[None, :]
+    H = J.T @ J
+    diag = np.clip(np.diag(H), 1e-6, 1e32)
+    H = H + np.diag(diag / 10.0)
+    f, e = slice(0, 9 * nc), slice(9 * nc, None)
+    S = H[f, f] - H[f, e] @ np.linalg.solve(H[e, e], H[e, f])
+    np.testing.assert_allclose(y.ravel(), S @ x.ravel(), rtol=1e-9, atol=1e-9 * np.abs(S @ x.ravel()).max())
+
+
+# -------------------------------------------------------------- LM solves --
+@pytest.mark.parametrize("solver,name", [(0, "sparse"), (1, "iterative")])
+def test_fp64_solve_matches_reference_ceres(mba, golden, c1, solver, name):
+    p = ba_problem(mba, c1)
+    s = mba.solve(p, mba.default_options(max_num_iterations=20, linear_solver=solver))
+    ref = golden[f"c1_{name}_iterations"]
+    assert s.IsSolutionUsable()
+    assert s.num_successful_steps == int(golden[f"c1_{name}_successful"])
+    assert s.num_unsuccessful_steps == int(golden[f"c1_{name}_unsuccessful"])
+    assert s.termination_type == int(golden[f"c1_{name}_termination"])
+    np.testing.assert_allclose([it.cost for it in s.iterations], ref[:, 1], rtol=1e-9)
+    np.testing.assert_allclose(s.final_cost, float(golden[f"c1_{name}_final_cost"]), rtol=1e-10)
+    assert [it.linear_solver_iterations for it in s.iterations] == [int(v) for v in ref[:, 6]]
+    np.testing.assert_allclose(p.cameras, golden[f"c1_{name}_cameras"], rtol=1e-6, atol=1e-8)
+    np.testing.assert_allclose(p.points, golden[f"c1_{name}_points"], rtol=1e-6, atol=1e-8)
+
+
+@pytest.mark.parametrize("case", ["huber_sparse", "huber_iterative", "const"])
+def test_fp64_solve_huber_and_constant_camera(mba, golden, huber_scene, case):
+    const = None
+    if case == "const":
+        const = np.zeros(30, np.uint8)
+        const[3] = 1
+    solver = 1 if case == "huber_iterative" else 0
+    s = mba.solve(ba_problem(mba, huber_scene, const),
+                  mba.default_options(max_num_iterations=20, linear_solver=solver, huber_a=4.0))
+    assert s.num_successful_steps == int(golden[f"{case}_successful"])
+    np.testing.assert_allclose([it.cost for it in s.iterations], golden[f"{case}_iterations"][:, 1], rtol=1e-8)
+    np.testing.assert_allclose(s.final_cost, float(golden[f"{case}_final_cost"]), rtol=1e-9)
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_mixed_precision_reaches_fp64_final_cost(mba, golden, c1, solver):
+    s = mba.solve(ba_problem(mba, c1), mba.default_options(max_num_iterations=20, linear_solver=solver,
+                                                           precision=mba.PRECISION_MIXED))
+    ref = float(golden["c1_sparse_final_cost"])
+    assert s.IsSolutionUsable()
+    assert abs(s.final_cost - ref) / ref < 1e-6
+
+
+def test_solve_is_deterministic(mba, c1):
+    opts = mba.default_options(max_num_iterations=10, linear_solver=1, precision=mba.PRECISION_MIXED)
+    a, b = ba_problem(mba, c1), ba_problem(mba, c1)
+    sa, sb = mba.solve(a, opts), mba.solve(b, opts)
+    assert sa.final_cost == sb.final_cost
+    np.testing.assert_array_equal(a.cameras, b.cameras)
+    np.testing.assert_array_equal(a.points, b.points)
+
+
+def test_nonfinite_input_is_an_error(mba, c1):
+    p = ba_problem(mba, c1)
+    p.points[3, 1] = np.inf
+    with pytest.raises(mba.MbaError) as e:
+        mba.solve(p)
+    assert e.value.status == mba.ERR_NOT_FINITE
+
+
+def test_ceres_shaped_api_like_the_reference_unit_test(mba, O):
+    """bundle_adjustment_unittest.cpp:19-122 shape: per-observation residual
+    blocks, SPARSE_SCHUR, the solution is usable and the RMSE decreases; plus
+    the final cost agrees with the oracle."""
+    d = O.generate(O.make_spec(3, 6, 3, 3, seed=9))
+    cams = [d["cam_init"][i].copy() for i in range(3)]
+    pts = [d["pt_init"][i].copy() for i in range(6)]
+    prob = mba.Problem()
+    for i in range(d["num_obs"]):
+        prob.AddResidualBlock(mba.ReprojectionError(d["obs"][i]), None, cams[d["cam_idx"][i]], pts[d["pt_idx"][i]])
+    opts = mba.SolverOptions()
+    opts.linear_solver_type = mba.SPARSE_SCHUR
+    s = mba.Solve(opts, prob)
+    n = prob.NumResidualBlocks()
+    assert s.IsSolutionUsable()
+    assert np.sqrt(s.initial_cost / (2 * n)) > np.sqrt(s.final_cost / (2 * n))
+    so, _ = O.solve(O.Scene(d), O.default_options())
+    np.testing.assert_allclose(s.final_cost, so.final_cost, rtol=1e-8, atol=1e-12)
+    assert not np.allclose(cams[0], d["cam_init"][0])
+
+
+# ------------------------------------------- full-size (4M) properties --
+def test_full_size_jacobian_schur_properties(mba):
+    """1,000 cameras, 500k points, ~4M observations (Huber 4 px): the pass is
+    deterministic run to run, S is symmetric (x'Sy = y'Sx) and positive
+    definite along random directions."""
+    d = mba.generate_scene(3)
+    p = mba.BAProblem(d["cam_init"], d["pt_init"], d["obs"], d["cam_idx"], d["pt_idx"])
+    with mba.Context(p, mba.default_options(precision=mba.PRECISION_MIXED, huber_a=4.0)) as ctx:
+        ctx.jacobian_schur(1e4)
+        a = ctx.read_schur()
+        ctx.jacobian_schur(1e4)
+        b = ctx.read_schur()
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k])
+        assert all(np.all(np.isfinite(v)) for v in a.values())
+        rng = np.random.default_rng(0)
+        x, y = rng.standard_normal((1000, 9)), rng.standard_normal((1000, 9))
+        Sx, Sy = ctx.schur_matvec(x), ctx.schur_matvec(y)
+        assert abs(np.sum(y * Sx) - np.sum(x * Sy)) <= 1e-4 * np.sqrt(np.sum(x * Sx) * np.sum(y * Sy))
+        assert np.sum(x * Sx) > 0 and np.sum(y * Sy) > 0
