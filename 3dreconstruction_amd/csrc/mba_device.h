@@ -248,6 +248,42 @@ __device__ __forceinline__ void jac_cam_dir(const Proj<T>& o, const T* __restric
                   o.jk2[1] * dt[14]);
 }
 
+// Packed point record read by the camera-major kernels: [X(3), v(3), 0, 0]
+// with v = s_p .* (V^-1 g_p) or s_p .* z; one aligned 8-element gather per
+// observation instead of three scattered ones.
+constexpr int kPk = 8;
+
+__device__ __forceinline__ void load_pk(const float* __restrict__ pk, int e, float X[3], float v[3]) {
+  const float4* q = reinterpret_cast<const float4*>(pk + (size_t)kPk * e);
+  const float4 a = q[0], b = q[1];
+  X[0] = a.x; X[1] = a.y; X[2] = a.z; v[0] = a.w; v[1] = b.x; v[2] = b.y;
+}
+__device__ __forceinline__ void load_pk(const double* __restrict__ pk, int e, double X[3], double v[3]) {
+  const double2* q = reinterpret_cast<const double2*>(pk + (size_t)kPk * e);
+  const double2 a = q[0], b = q[1], c = q[2];
+  X[0] = a.x; X[1] = a.y; X[2] = b.x; v[0] = b.y; v[1] = c.x; v[2] = c.y;
+}
+template <typename T>
+__device__ __forceinline__ void store_pk(T* __restrict__ pk, int e, const T X[3], const T v[3]) {
+  T* o = pk + (size_t)kPk * e;
+  o[0] = X[0]; o[1] = X[1]; o[2] = X[2]; o[3] = v[0]; o[4] = v[1]; o[5] = v[2]; o[6] = T(0); o[7] = T(0);
+}
+template <typename T>
+__device__ __forceinline__ void store_pk_v(T* __restrict__ pk, int e, const T v[3]) {
+  T* o = pk + (size_t)kPk * e;
+  o[3] = v[0]; o[4] = v[1]; o[5] = v[2];
+}
+
+// u = P R v (2-vector): the point Jacobian (before weight) applied to v.
+template <typename T>
+__device__ __forceinline__ void jac_point_apply(const Proj<T>& o, const T R[9], const T v[3], T u[2]) {
+  const T a0 = R[0] * v[0] + R[1] * v[1] + R[2] * v[2];
+  const T a1 = R[3] * v[0] + R[4] * v[1] + R[5] * v[2];
+  const T a2 = R[6] * v[0] + R[7] * v[1] + R[8] * v[2];
+  u[0] = o.P[0] * a0 + o.P[1] * a1 + o.P[2] * a2;
+  u[1] = o.P[3] * a0 + o.P[4] * a1 + o.P[5] * a2;
+}
+
 // FP64 inverse of an SPD 3x3 (xx xy xz yy yz zz) via LLT.
 __device__ __forceinline__ bool inv3_spd(const double a[6], double o[6]) {
   if (!(a[0] > 0.0)) return false;
@@ -393,36 +429,51 @@ __global__ void k_grad_final(const double* __restrict__ pmax, const double* __re
 // V = sum Jp'Jp, g_p = sum Jp' r; LM diagonal of points; Vinv = (V + D^2)^-1
 // in FP64; w = Vinv g_p.
 template <typename T>
+__device__ __forceinline__ void point_row_accumulate(const T* __restrict__ ctab, int cam, T ox, T oy,
+                                                     const T X[3], const T sp[3], T huber_a, T v[6],
+                                                     T g[3]) {
+  CamProj<T> c;
+  load_proj(ctab + (size_t)kCamTab * cam, c);
+  Proj<T> o;
+  project(c, X, ox, oy, o);
+  const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+  T Jp[6];
+  jac_point(o, c.R, wgt, sp, Jp);
+  const T rx = wgt * o.rx, ry = wgt * o.ry;
+  v[0] += Jp[0] * Jp[0] + Jp[3] * Jp[3];
+  v[1] += Jp[0] * Jp[1] + Jp[3] * Jp[4];
+  v[2] += Jp[0] * Jp[2] + Jp[3] * Jp[5];
+  v[3] += Jp[1] * Jp[1] + Jp[4] * Jp[4];
+  v[4] += Jp[1] * Jp[2] + Jp[4] * Jp[5];
+  v[5] += Jp[2] * Jp[2] + Jp[5] * Jp[5];
+  g[0] += Jp[0] * rx + Jp[3] * ry;
+  g[1] += Jp[1] * rx + Jp[4] * ry;
+  g[2] += Jp[2] * rx + Jp[5] * ry;
+}
+
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_point_pass(
     int ne, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const T* __restrict__ pts, const T* __restrict__ pscale,
     const T* __restrict__ ctab, T huber_a, double inv_radius, double min_diag, double max_diag,
     double* __restrict__ diag_p, int recompute_diag, T* __restrict__ V, T* __restrict__ gp,
-    T* __restrict__ Vinv, T* __restrict__ wv, int32_t* __restrict__ flag) {
+    T* __restrict__ Vinv, T* __restrict__ wv, T* __restrict__ pk, int32_t* __restrict__ flag) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
   const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
   T v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
-  for (int r = pt_start[e]; r < pt_start[e + 1]; ++r) {
-    CamProj<T> c;
-    load_proj(ctab + (size_t)kCamTab * row_cam[r], c);
-    Proj<T> o;
-    project(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
-    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
-    T Jp[6];
-    jac_point(o, c.R, wgt, sp, Jp);
-    const T rx = wgt * o.rx, ry = wgt * o.ry;
-    v[0] += Jp[0] * Jp[0] + Jp[3] * Jp[3];
-    v[1] += Jp[0] * Jp[1] + Jp[3] * Jp[4];
-    v[2] += Jp[0] * Jp[2] + Jp[3] * Jp[5];
-    v[3] += Jp[1] * Jp[1] + Jp[4] * Jp[4];
-    v[4] += Jp[1] * Jp[2] + Jp[4] * Jp[5];
-    v[5] += Jp[2] * Jp[2] + Jp[5] * Jp[5];
-    g[0] += Jp[0] * rx + Jp[3] * ry;
-    g[1] += Jp[1] * rx + Jp[4] * ry;
-    g[2] += Jp[2] * rx + Jp[5] * ry;
+  const int r1 = pt_start[e + 1];
+  int r = pt_start[e];
+  // Two rows per iteration: both rows' loads are in flight together.
+  for (; r + 1 < r1; r += 2) {
+    const int c0 = row_cam[r], c1 = row_cam[r + 1];
+    const T o0x = row_obs[2 * (size_t)r], o0y = row_obs[2 * (size_t)r + 1];
+    const T o1x = row_obs[2 * (size_t)r + 2], o1y = row_obs[2 * (size_t)r + 3];
+    point_row_accumulate(ctab, c0, o0x, o0y, X, sp, huber_a, v, g);
+    point_row_accumulate(ctab, c1, o1x, o1y, X, sp, huber_a, v, g);
   }
+  if (r < r1) point_row_accumulate(ctab, row_cam[r], row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], X, sp, huber_a, v, g);
 #pragma unroll
   for (int i = 0; i < 6; ++i) V[6 * (size_t)e + i] = v[i];
 #pragma unroll
@@ -450,9 +501,13 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
 #pragma unroll
   for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
   const double g0 = g[0], g1 = g[1], g2 = g[2];
-  wv[3 * (size_t)e + 0] = (T)(iv[0] * g0 + iv[1] * g1 + iv[2] * g2);
-  wv[3 * (size_t)e + 1] = (T)(iv[1] * g0 + iv[3] * g1 + iv[4] * g2);
-  wv[3 * (size_t)e + 2] = (T)(iv[2] * g0 + iv[4] * g1 + iv[5] * g2);
+  const T w[3] = {(T)(iv[0] * g0 + iv[1] * g1 + iv[2] * g2), (T)(iv[1] * g0 + iv[3] * g1 + iv[4] * g2),
+                  (T)(iv[2] * g0 + iv[4] * g1 + iv[5] * g2)};
+  wv[3 * (size_t)e + 0] = w[0];
+  wv[3 * (size_t)e + 1] = w[1];
+  wv[3 * (size_t)e + 2] = w[2];
+  const T sw[3] = {sp[0] * w[0], sp[1] * w[1], sp[2] * w[2]};
+  store_pk(pk, e, X, sw);
 }
 
 // Re-damp after a rejected step: Vinv, w from stored V, g_p, diag_p.
@@ -460,8 +515,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restrict__ V,
                                                        const T* __restrict__ gp,
                                                        const double* __restrict__ diag_p,
-                                                       double inv_radius, T* __restrict__ Vinv,
-                                                       T* __restrict__ wv, int32_t* __restrict__ flag) {
+                                                       double inv_radius, const T* __restrict__ pscale,
+                                                       T* __restrict__ Vinv, T* __restrict__ wv,
+                                                       T* __restrict__ pk, int32_t* __restrict__ flag) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   double a[6];
@@ -476,9 +532,11 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restri
   }
   for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
   const double g0 = gp[3 * (size_t)e], g1 = gp[3 * (size_t)e + 1], g2 = gp[3 * (size_t)e + 2];
-  wv[3 * (size_t)e + 0] = (T)(iv[0] * g0 + iv[1] * g1 + iv[2] * g2);
-  wv[3 * (size_t)e + 1] = (T)(iv[1] * g0 + iv[3] * g1 + iv[4] * g2);
-  wv[3 * (size_t)e + 2] = (T)(iv[2] * g0 + iv[4] * g1 + iv[5] * g2);
+  const T w[3] = {(T)(iv[0] * g0 + iv[1] * g1 + iv[2] * g2), (T)(iv[1] * g0 + iv[3] * g1 + iv[4] * g2),
+                  (T)(iv[2] * g0 + iv[4] * g1 + iv[5] * g2)};
+  for (int i = 0; i < 3; ++i) wv[3 * (size_t)e + i] = w[i];
+  const T sw[3] = {pscale[3 * (size_t)e] * w[0], pscale[3 * (size_t)e + 1] * w[1], pscale[3 * (size_t)e + 2] * w[2]};
+  store_pk_v(pk, e, sw);
 }
 
 // ------------------------------------------------------------ camera pass
@@ -488,9 +546,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cam_pass(
     const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
-    const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pts,
-    const T* __restrict__ pscale, const T* __restrict__ cscale, const T* __restrict__ ctab,
-    const T* __restrict__ wv, T huber_a, int with_h, T* __restrict__ partial) {
+    const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
+    const T* __restrict__ cscale, const T* __restrict__ ctab, T huber_a, int with_h,
+    T* __restrict__ partial) {
   __shared__ T lds[63 * (kBlock / 64)];
   const int ch = blockIdx.x;
   const int f = chunk_f[ch];
@@ -509,7 +567,8 @@ __global__ __launch_bounds__(kBlock) void k_cam_pass(
   const int end = chunk_end[ch];
   for (int i = chunk_begin[ch] + threadIdx.x; i < end; i += kBlock) {
     const int e = cm_e[i];
-    const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+    T X[3], sw[3];
+    load_pk(pk, e, X, sw);
     Proj<T> o;
     project(c, X, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], o);
     const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
@@ -518,12 +577,10 @@ __global__ __launch_bounds__(kBlock) void k_cam_pass(
     const T rx = wgt * o.rx, ry = wgt * o.ry;
     T tx = rx, ty = ry;
     if (with_h) {
-      const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
-      T Jp[6];
-      jac_point(o, c.R, wgt, sp, Jp);
-      const T w0 = wv[3 * (size_t)e], w1 = wv[3 * (size_t)e + 1], w2 = wv[3 * (size_t)e + 2];
-      tx -= Jp[0] * w0 + Jp[1] * w1 + Jp[2] * w2;
-      ty -= Jp[3] * w0 + Jp[4] * w1 + Jp[5] * w2;
+      T u[2];
+      jac_point_apply(o, c.R, sw, u);
+      tx -= wgt * u[0];
+      ty -= wgt * u[1];
     }
     int k = 0;
 #pragma unroll
@@ -644,7 +701,7 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
     int ne, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const T* __restrict__ pts, const T* __restrict__ pscale,
     const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ Vinv, T huber_a,
-    T beta, int apply_vinv, T* __restrict__ zout, const PcgState* __restrict__ st) {
+    T beta, int apply_vinv, T* __restrict__ zout, T* __restrict__ pk, const PcgState* __restrict__ st) {
   if (st && st->done) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
@@ -668,15 +725,17 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
     q[1] += Jp[1] * v[0] + Jp[4] * v[1];
     q[2] += Jp[2] * v[0] + Jp[5] * v[1];
   }
+  T zv[3] = {q[0], q[1], q[2]};
   if (apply_vinv) {
     const T* iv = Vinv + 6 * (size_t)e;
-    zout[3 * (size_t)e + 0] = iv[0] * q[0] + iv[1] * q[1] + iv[2] * q[2];
-    zout[3 * (size_t)e + 1] = iv[1] * q[0] + iv[3] * q[1] + iv[4] * q[2];
-    zout[3 * (size_t)e + 2] = iv[2] * q[0] + iv[4] * q[1] + iv[5] * q[2];
-  } else {
-    zout[3 * (size_t)e + 0] = q[0];
-    zout[3 * (size_t)e + 1] = q[1];
-    zout[3 * (size_t)e + 2] = q[2];
+    zv[0] = iv[0] * q[0] + iv[1] * q[1] + iv[2] * q[2];
+    zv[1] = iv[1] * q[0] + iv[3] * q[1] + iv[4] * q[2];
+    zv[2] = iv[2] * q[0] + iv[4] * q[1] + iv[5] * q[2];
+  }
+  for (int i = 0; i < 3; ++i) zout[3 * (size_t)e + i] = zv[i];
+  if (pk) {
+    const T sz[3] = {sp[0] * zv[0], sp[1] * zv[1], sp[2] * zv[2]};
+    store_pk_v(pk, e, sz);
   }
 }
 
@@ -685,9 +744,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mv_cam(
     const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
-    const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pts,
-    const T* __restrict__ pscale, const T* __restrict__ cscale, const T* __restrict__ ctab,
-    const T* __restrict__ z, T huber_a, T* __restrict__ partial, const PcgState* __restrict__ st) {
+    const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
+    const T* __restrict__ cscale, const T* __restrict__ ctab, T huber_a, T* __restrict__ partial,
+    const PcgState* __restrict__ st) {
   __shared__ T lds[9 * (kBlock / 64)];
   if (st && st->done) return;
   const int ch = blockIdx.x;
@@ -707,19 +766,19 @@ __global__ __launch_bounds__(kBlock) void k_mv_cam(
   const int end = chunk_end[ch];
   for (int i = chunk_begin[ch] + threadIdx.x; i < end; i += kBlock) {
     const int e = cm_e[i];
-    const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
-    const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
+    T X[3], sz[3];
+    load_pk(pk, e, X, sz);
     Proj<T> o;
     project(c, X, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], o);
     const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
-    T Jp[6], Jc[18];
-    jac_point(o, c.R, wgt, sp, Jp);
+    T Jc[18];
     jac_cam(o, dR, X, wgt, cs, Jc);
-    const T z0 = z[3 * (size_t)e], z1 = z[3 * (size_t)e + 1], z2 = z[3 * (size_t)e + 2];
-    const T u0 = Jp[0] * z0 + Jp[1] * z1 + Jp[2] * z2;
-    const T u1 = Jp[3] * z0 + Jp[4] * z1 + Jp[5] * z2;
+    T u[2];
+    jac_point_apply(o, c.R, sz, u);
+    u[0] *= wgt;
+    u[1] *= wgt;
 #pragma unroll
-    for (int a = 0; a < 9; ++a) acc[a] += Jc[a] * u0 + Jc[9 + a] * u1;
+    for (int a = 0; a < 9; ++a) acc[a] += Jc[a] * u[0] + Jc[9 + a] * u[1];
   }
   block_sum<T, 9>(acc, lds);
   if (threadIdx.x < 9) {

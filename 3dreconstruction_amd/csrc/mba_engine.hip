@@ -308,7 +308,7 @@ struct Dev {
   // kernel-precision parameters
   T *pts = nullptr, *ctab = nullptr, *pscale = nullptr, *cscale = nullptr;
   // blocks
-  T *V = nullptr, *gp = nullptr, *Vinv = nullptr, *w = nullptr, *z = nullptr;
+  T *V = nullptr, *gp = nullptr, *Vinv = nullptr, *w = nullptr, *z = nullptr, *pk = nullptr;
   T *U = nullptr, *gc = nullptr, *rhs = nullptr, *Minv = nullptr, *dtab = nullptr, *partial = nullptr;
   // PCG / step
   T *x = nullptr, *r = nullptr, *zc = nullptr, *p = nullptr, *q = nullptr, *tmp = nullptr;
@@ -393,6 +393,7 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& row_cam,
   MBA_TRY(c->pool.alloc(&d.Vinv, 6 * ne));
   MBA_TRY(c->pool.alloc(&d.w, 3 * ne));
   MBA_TRY(c->pool.alloc(&d.z, 3 * ne));
+  MBA_TRY(c->pool.alloc(&d.pk, kPk * ne));
   MBA_TRY(c->pool.alloc(&d.U, 45 * nf));
   MBA_TRY(c->pool.alloc(&d.gc, 9 * nf));
   MBA_TRY(c->pool.alloc(&d.rhs, 9 * nf));
@@ -597,12 +598,12 @@ struct Engine {
     hipLaunchKernelGGL(k_point_pass<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.pt_start,
                        d.row_cam, d.row_obs, d.pts, d.pscale, d.ctab, huber, inv_r,
                        c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag,
-                       d.V, d.gp, d.Vinv, d.w, c->flag);
+                       d.V, d.gp, d.Vinv, d.w, d.pk, c->flag);
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
     if (c->nchunks > 0)
       hipLaunchKernelGGL(k_cam_pass<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
-                         d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pts, d.pscale, d.cscale, d.ctab,
-                         d.w, huber, 1, d.partial);
+                         d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber, 1,
+                         d.partial);
     if (c->timing) (void)hipEventRecord(c->ev[2], s);
     if (c->nf > 0)
       hipLaunchKernelGGL(k_cam_finalize<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf,
@@ -616,14 +617,14 @@ struct Engine {
   void damp(double radius) {
     const double inv_r = 1.0 / radius;
     hipLaunchKernelGGL(k_point_damp<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.V, d.gp,
-                       c->diag_p, inv_r, d.Vinv, d.w, c->flag);
+                       c->diag_p, inv_r, d.pscale, d.Vinv, d.w, d.pk, c->flag);
     if (c->nf > 0) {
       hipLaunchKernelGGL(k_cam_damp<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf, d.U, c->diag_c,
                          inv_r, d.Minv, c->flag);
       if (c->nchunks > 0)
         hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
-                           d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pts, d.pscale, d.cscale, d.ctab,
-                           d.w, huber, d.partial, (const PcgState*)nullptr);
+                           d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber,
+                           d.partial, (const PcgState*)nullptr);
       hipLaunchKernelGGL(k_rhs_finalize<T>, dim3(nblocks(c->nf)), dim3(kBlock), 0, s, c->nf,
                          d.f_chunk_start, d.partial, d.gc, d.rhs);
     }
@@ -636,11 +637,11 @@ struct Engine {
     hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
                        d.ctab, x, (T)1, d.dtab, st);
     hipLaunchKernelGGL(k_mv_point<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.pt_start, d.row_cam,
-                       d.row_obs, d.pts, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)0, 1, d.z, st);
+                       d.row_obs, d.pts, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)0, 1, d.z, d.pk, st);
     if (c->nchunks > 0)
       hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
-                         d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pts, d.pscale, d.cscale, d.ctab, d.z,
-                         huber, d.partial, st);
+                         d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber,
+                         d.partial, st);
     hipLaunchKernelGGL(k_mv_finalize<T>, dim3(nblocks(c->nf)), dim3(kBlock), 0, s, c->nf, d.f_chunk_start,
                        d.partial, d.U, c->diag_c, inv_r, x, 1, y, st);
   }
@@ -752,7 +753,7 @@ struct Engine {
     hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
                        d.ctab, d.x, (T)-1, d.dtab, (const PcgState*)nullptr);
     hipLaunchKernelGGL(k_mv_point<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.pt_start, d.row_cam,
-                       d.row_obs, d.pts, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)1, 1, d.z,
+                       d.row_obs, d.pts, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)1, 1, d.z, (T*)nullptr,
                        (const PcgState*)nullptr);
     hipLaunchKernelGGL(k_negate<T>, dim3(nblocks(3 * (int64_t)c->ne)), dim3(kBlock), 0, s,
                        (int64_t)3 * c->ne, d.z, d.step_p);
