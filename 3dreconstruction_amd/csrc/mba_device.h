@@ -42,6 +42,41 @@ struct PcgState {
   int32_t iter, done, status, pad;  // status: 0 ok, 1 no convergence, -1 failure
 };
 
+// Point-major launch layout: points (E blocks, device order) are grouped in
+// segments by track-length class; a segment gives each point `lanes` lanes
+// (1..64, a power of two), and its blocks follow the previous segment's.
+constexpr int kMaxSeg = 8;
+struct Seg {
+  int32_t n;
+  int32_t e0[kMaxSeg], e1[kMaxSeg], lanes[kMaxSeg], blk0[kMaxSeg + 1];
+};
+
+struct SegPos {
+  int e, sub, lanes;
+  bool valid;
+};
+
+__device__ __forceinline__ SegPos seg_locate(const Seg& sg) {
+  const int b = blockIdx.x;
+  int k = 0;
+  while (k + 1 < sg.n && b >= sg.blk0[k + 1]) ++k;
+  SegPos p;
+  p.lanes = sg.lanes[k];
+  const int t = (b - sg.blk0[k]) * blockDim.x + threadIdx.x;
+  p.e = sg.e0[k] + t / p.lanes;
+  p.sub = t % p.lanes;
+  p.valid = p.e < sg.e1[k];
+  return p;
+}
+
+// Sum over the `lanes` lanes of a point group (aligned, power of two).
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int lanes) {
+  for (int o = 32; o > 0; o >>= 1)
+    if (o < lanes) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 template <typename T> __device__ __forceinline__ T tiny();
 template <> __device__ __forceinline__ float tiny<float>() { return FLT_MIN; }
 template <> __device__ __forceinline__ double tiny<double>() { return DBL_MIN; }
@@ -340,8 +375,7 @@ __device__ __forceinline__ int ut9(int i, int j) { return i * 9 - i * (i - 1) / 
 
 // --------------------------------------------------------------- FP64 cost
 // One thread per E block; block partial sums of 1/2 rho(|r|^2).
-template <typename Tcam>
-__global__ __launch_bounds__(kBlock) void k_cost64(int ne, const int32_t* __restrict__ pt_start,
+__global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __restrict__ pt_start,
                                                    const int32_t* __restrict__ row_cam,
                                                    const double* __restrict__ row_obs,
                                                    const double* __restrict__ pts,
@@ -349,12 +383,13 @@ __global__ __launch_bounds__(kBlock) void k_cost64(int ne, const int32_t* __rest
                                                    double* __restrict__ partial,
                                                    int32_t* __restrict__ flag) {
   __shared__ double lds[kBlock / 64];
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const SegPos sp = seg_locate(sg);
   double acc[1] = {0.0};
-  if (e < ne) {
+  if (sp.valid) {
+    const int e = sp.e;
     const double X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
     int bad = 0;
-    for (int r = pt_start[e]; r < pt_start[e + 1]; ++r) {
+    for (int r = pt_start[e] + sp.sub; r < pt_start[e + 1]; r += sp.lanes) {
       CamProj<double> c;
       load_proj(ctab + (size_t)kCamTab * row_cam[r], c);
       Proj<double> o;
@@ -453,27 +488,35 @@ __device__ __forceinline__ void point_row_accumulate(const T* __restrict__ ctab,
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_point_pass(
-    int ne, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
-    const T* __restrict__ row_obs, const T* __restrict__ pts, const T* __restrict__ pscale,
+    Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const T* __restrict__ row_obs, const double* __restrict__ pts64, const T* __restrict__ pscale,
     const T* __restrict__ ctab, T huber_a, double inv_radius, double min_diag, double max_diag,
     double* __restrict__ diag_p, int recompute_diag, T* __restrict__ V, T* __restrict__ gp,
-    T* __restrict__ Vinv, T* __restrict__ wv, T* __restrict__ pk, int32_t* __restrict__ flag) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+    T* __restrict__ Vinv, T* __restrict__ pk, int32_t* __restrict__ flag) {
+  const SegPos q = seg_locate(sg);
+  const int e = q.valid ? q.e : sg.e0[0];
+  const T X[3] = {(T)pts64[3 * (size_t)e], (T)pts64[3 * (size_t)e + 1], (T)pts64[3 * (size_t)e + 2]};
   const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
   T v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
-  const int r1 = pt_start[e + 1];
-  int r = pt_start[e];
-  // Two rows per iteration: both rows' loads are in flight together.
-  for (; r + 1 < r1; r += 2) {
-    const int c0 = row_cam[r], c1 = row_cam[r + 1];
-    const T o0x = row_obs[2 * (size_t)r], o0y = row_obs[2 * (size_t)r + 1];
-    const T o1x = row_obs[2 * (size_t)r + 2], o1y = row_obs[2 * (size_t)r + 3];
-    point_row_accumulate(ctab, c0, o0x, o0y, X, sp, huber_a, v, g);
-    point_row_accumulate(ctab, c1, o1x, o1y, X, sp, huber_a, v, g);
+  if (q.valid) {
+    const int r1 = pt_start[e + 1];
+    const int L = q.lanes;
+    int r = pt_start[e] + q.sub;
+    // Two rows per iteration so both rows' loads are in flight together.
+    for (; r + L < r1; r += 2 * L) {
+      const int c0 = row_cam[r], c1 = row_cam[r + L];
+      const T o0x = row_obs[2 * (size_t)r], o0y = row_obs[2 * (size_t)r + 1];
+      const T o1x = row_obs[2 * (size_t)(r + L)], o1y = row_obs[2 * (size_t)(r + L) + 1];
+      point_row_accumulate(ctab, c0, o0x, o0y, X, sp, huber_a, v, g);
+      point_row_accumulate(ctab, c1, o1x, o1y, X, sp, huber_a, v, g);
+    }
+    if (r < r1) point_row_accumulate(ctab, row_cam[r], row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], X, sp, huber_a, v, g);
   }
-  if (r < r1) point_row_accumulate(ctab, row_cam[r], row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], X, sp, huber_a, v, g);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) v[i] = group_sum(v[i], q.lanes);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) g[i] = group_sum(g[i], q.lanes);
+  if (!q.valid || q.sub != 0) return;
 #pragma unroll
   for (int i = 0; i < 6; ++i) V[6 * (size_t)e + i] = v[i];
 #pragma unroll
@@ -501,12 +544,9 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
 #pragma unroll
   for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
   const double g0 = g[0], g1 = g[1], g2 = g[2];
-  const T w[3] = {(T)(iv[0] * g0 + iv[1] * g1 + iv[2] * g2), (T)(iv[1] * g0 + iv[3] * g1 + iv[4] * g2),
-                  (T)(iv[2] * g0 + iv[4] * g1 + iv[5] * g2)};
-  wv[3 * (size_t)e + 0] = w[0];
-  wv[3 * (size_t)e + 1] = w[1];
-  wv[3 * (size_t)e + 2] = w[2];
-  const T sw[3] = {sp[0] * w[0], sp[1] * w[1], sp[2] * w[2]};
+  const T sw[3] = {(T)((double)sp[0] * (iv[0] * g0 + iv[1] * g1 + iv[2] * g2)),
+                   (T)((double)sp[1] * (iv[1] * g0 + iv[3] * g1 + iv[4] * g2)),
+                   (T)((double)sp[2] * (iv[2] * g0 + iv[4] * g1 + iv[5] * g2))};
   store_pk(pk, e, X, sw);
 }
 
@@ -516,8 +556,8 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restri
                                                        const T* __restrict__ gp,
                                                        const double* __restrict__ diag_p,
                                                        double inv_radius, const T* __restrict__ pscale,
-                                                       T* __restrict__ Vinv, T* __restrict__ wv,
-                                                       T* __restrict__ pk, int32_t* __restrict__ flag) {
+                                                       T* __restrict__ Vinv, T* __restrict__ pk,
+                                                       int32_t* __restrict__ flag) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   double a[6];
@@ -534,7 +574,6 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restri
   const double g0 = gp[3 * (size_t)e], g1 = gp[3 * (size_t)e + 1], g2 = gp[3 * (size_t)e + 2];
   const T w[3] = {(T)(iv[0] * g0 + iv[1] * g1 + iv[2] * g2), (T)(iv[1] * g0 + iv[3] * g1 + iv[4] * g2),
                   (T)(iv[2] * g0 + iv[4] * g1 + iv[5] * g2)};
-  for (int i = 0; i < 3; ++i) wv[3 * (size_t)e + i] = w[i];
   const T sw[3] = {pscale[3 * (size_t)e] * w[0], pscale[3 * (size_t)e + 1] * w[1], pscale[3 * (size_t)e + 2] * w[2]};
   store_pk_v(pk, e, sw);
 }
@@ -542,13 +581,44 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restri
 // ------------------------------------------------------------ camera pass
 // One workgroup per camera-major chunk (all rows of one camera): partial
 // U = sum Jc'Jc (45), g_c = sum Jc' r (9) and, with_h, h = sum Jc'(r - Jp w_e).
+// Accumulate one camera-major row into U (45), g (9), h (9), with the
+// camera Jacobi scale left out (applied per camera in k_cam_finalize).
+template <typename T>
+__device__ __forceinline__ void cam_row_accumulate(const CamProj<T>& c, const T dR[27], const T X[3],
+                                                   const T sw[3], T ox, T oy, T huber_a, int with_h,
+                                                   T acc[63]) {
+  const T one[9] = {T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1)};
+  Proj<T> o;
+  project(c, X, ox, oy, o);
+  const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+  T Jc[18];
+  jac_cam(o, dR, X, wgt, one, Jc);
+  const T rx = wgt * o.rx, ry = wgt * o.ry;
+  T tx = rx, ty = ry;
+  if (with_h) {
+    T u[2];
+    jac_point_apply(o, c.R, sw, u);
+    tx -= wgt * u[0];
+    ty -= wgt * u[1];
+  }
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < 9; ++a)
+#pragma unroll
+    for (int b = a; b < 9; ++b) acc[k++] += Jc[a] * Jc[b] + Jc[9 + a] * Jc[9 + b];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) {
+    acc[45 + a] += Jc[a] * rx + Jc[9 + a] * ry;
+    acc[54 + a] += Jc[a] * tx + Jc[9 + a] * ty;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cam_pass(
     const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
-    const T* __restrict__ cscale, const T* __restrict__ ctab, T huber_a, int with_h,
-    T* __restrict__ partial) {
+    const T* __restrict__ ctab, T huber_a, int with_h, T* __restrict__ partial) {
   __shared__ T lds[63 * (kBlock / 64)];
   const int ch = blockIdx.x;
   const int f = chunk_f[ch];
@@ -558,40 +628,27 @@ __global__ __launch_bounds__(kBlock) void k_cam_pass(
   T dR[27];
 #pragma unroll
   for (int i = 0; i < 27; ++i) dR[i] = tab[kTabDR + i];
-  T cs[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) cs[i] = cscale[9 * (size_t)f + i];
   T acc[63];
 #pragma unroll
   for (int i = 0; i < 63; ++i) acc[i] = T(0);
   const int end = chunk_end[ch];
-  for (int i = chunk_begin[ch] + threadIdx.x; i < end; i += kBlock) {
-    const int e = cm_e[i];
-    T X[3], sw[3];
-    load_pk(pk, e, X, sw);
-    Proj<T> o;
-    project(c, X, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], o);
-    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
-    T Jc[18];
-    jac_cam(o, dR, X, wgt, cs, Jc);
-    const T rx = wgt * o.rx, ry = wgt * o.ry;
-    T tx = rx, ty = ry;
-    if (with_h) {
-      T u[2];
-      jac_point_apply(o, c.R, sw, u);
-      tx -= wgt * u[0];
-      ty -= wgt * u[1];
-    }
-    int k = 0;
-#pragma unroll
-    for (int a = 0; a < 9; ++a)
-#pragma unroll
-      for (int b = a; b < 9; ++b) acc[k++] += Jc[a] * Jc[b] + Jc[9 + a] * Jc[9 + b];
-#pragma unroll
-    for (int a = 0; a < 9; ++a) {
-      acc[45 + a] += Jc[a] * rx + Jc[9 + a] * ry;
-      acc[54 + a] += Jc[a] * tx + Jc[9 + a] * ty;
-    }
+  int i = chunk_begin[ch] + threadIdx.x;
+  // Two rows per iteration: both rows' index, observation and point-record
+  // loads are issued before either row is computed.
+  for (; i + kBlock < end; i += 2 * kBlock) {
+    const int e0 = cm_e[i], e1 = cm_e[i + kBlock];
+    const T o0x = cm_obs[2 * (size_t)i], o0y = cm_obs[2 * (size_t)i + 1];
+    const T o1x = cm_obs[2 * (size_t)(i + kBlock)], o1y = cm_obs[2 * (size_t)(i + kBlock) + 1];
+    T X0[3], s0[3], X1[3], s1[3];
+    load_pk(pk, e0, X0, s0);
+    load_pk(pk, e1, X1, s1);
+    cam_row_accumulate(c, dR, X0, s0, o0x, o0y, huber_a, with_h, acc);
+    cam_row_accumulate(c, dR, X1, s1, o1x, o1y, huber_a, with_h, acc);
+  }
+  if (i < end) {
+    T X0[3], s0[3];
+    load_pk(pk, cm_e[i], X0, s0);
+    cam_row_accumulate(c, dR, X0, s0, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], huber_a, with_h, acc);
   }
   block_sum<T, 63>(acc, lds);
   if (threadIdx.x < 63) {
@@ -601,29 +658,47 @@ __global__ __launch_bounds__(kBlock) void k_cam_pass(
   }
 }
 
-// One thread per F block: sum its chunk partials; LM diagonal; FP64
-// inverse of U + D^2 (block-Jacobi preconditioner); rhs.
+// One wave per F block: lanes sum the chunk partials (one value each), lane
+// 0 forms the LM diagonal and the FP64 inverse of U + D^2 (block-Jacobi
+// preconditioner); rhs = h.
 template <typename T>
 __global__ __launch_bounds__(64) void k_cam_finalize(
     int nf, const int32_t* __restrict__ f_chunk_start, const T* __restrict__ partial,
-    double inv_radius, double min_diag, double max_diag, double* __restrict__ diag_c,
-    int recompute_diag, int with_h, T* __restrict__ U, T* __restrict__ gc, T* __restrict__ rhs,
-    T* __restrict__ Minv, int32_t* __restrict__ flag) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  double s[63];
-  for (int k = 0; k < 63; ++k) s[k] = 0.0;
-  for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch)
-    for (int k = 0; k < 63; ++k) s[k] += (double)partial[(size_t)kPartial * ch + k];
-  for (int k = 0; k < 45; ++k) U[45 * (size_t)f + k] = (T)s[k];
-  for (int k = 0; k < 9; ++k) {
-    gc[9 * (size_t)f + k] = (T)s[45 + k];
-    if (with_h) rhs[9 * (size_t)f + k] = (T)s[54 + k];
+    const T* __restrict__ cscale, double inv_radius, double min_diag, double max_diag,
+    double* __restrict__ diag_c, int recompute_diag, int with_h, T* __restrict__ U, T* __restrict__ gc,
+    T* __restrict__ rhs, T* __restrict__ Minv, int32_t* __restrict__ flag) {
+  __shared__ double sum[64];
+  __shared__ int ij[64];
+  const int f = blockIdx.x;
+  const int k = threadIdx.x;
+  if (k == 0) {
+    int q = 0;
+    for (int a = 0; a < 9; ++a)
+      for (int b = a; b < 9; ++b) ij[q++] = a * 16 + b;
+    for (int a = 0; a < 9; ++a) ij[45 + a] = a * 16 + 15;
+    for (int a = 0; a < 9; ++a) ij[54 + a] = a * 16 + 15;
+    ij[63] = 0;
   }
+  __syncthreads();
+  double v = 0.0;
+  if (k < 63) {
+    for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch) v += (double)partial[(size_t)kPartial * ch + k];
+    // Partials carry the unscaled camera columns: scale entry (a, b) by
+    // s_a s_b, vectors by s_a.
+    const int a = ij[k] >> 4, b = ij[k] & 15;
+    v *= (double)cscale[9 * (size_t)f + a] * (b < 9 ? (double)cscale[9 * (size_t)f + b] : 1.0);
+  }
+  const T vt = (T)v;
+  if (k < 45) U[45 * (size_t)f + k] = vt;
+  else if (k < 54) gc[9 * (size_t)f + (k - 45)] = vt;
+  else if (k < 63 && with_h) rhs[9 * (size_t)f + (k - 54)] = vt;
+  sum[k] = (double)vt;
+  __syncthreads();
+  if (k != 0) return;
   double A[81], inv[81];
   for (int i = 0; i < 9; ++i)
     for (int j = i; j < 9; ++j) {
-      const double u = (double)(T)s[ut9(i, j)];
+      const double u = sum[ut9(i, j)];
       A[i * 9 + j] = u;
       A[j * 9 + i] = u;
     }
@@ -639,9 +714,9 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
   }
   if (!invn_spd(A, 9, inv)) {
     *flag = 1;
-    for (int k = 0; k < 81; ++k) inv[k] = 0.0;
+    for (int q = 0; q < 81; ++q) inv[q] = 0.0;
   }
-  for (int k = 0; k < 81; ++k) Minv[81 * (size_t)f + k] = (T)inv[k];
+  for (int q = 0; q < 81; ++q) Minv[81 * (size_t)f + q] = (T)inv[q];
 }
 
 // Re-damp the camera preconditioner after a rejected step.
@@ -694,37 +769,42 @@ __global__ __launch_bounds__(kBlock) void k_cam_dir(int nc, const int32_t* __res
 }
 
 // Per E block: q = sum_rows Jp'(beta r + Jc x) (x via dtab); z = Vinv q
-// (apply_vinv) or q.  With an optional point step `xp` (3 per E block) the
-// row term also adds Jp xp.
+// (apply_vinv) or q; pk's v part (if pk) becomes s_p .* z.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mv_point(
-    int ne, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
-    const T* __restrict__ row_obs, const T* __restrict__ pts, const T* __restrict__ pscale,
+    Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
     const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ Vinv, T huber_a,
     T beta, int apply_vinv, T* __restrict__ zout, T* __restrict__ pk, const PcgState* __restrict__ st) {
   if (st && st->done) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+  const SegPos qp = seg_locate(sg);
+  const int e = qp.valid ? qp.e : sg.e0[0];
+  T X[3], unused[3];
+  load_pk(pkX, e, X, unused);
   const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
   T q[3] = {0, 0, 0};
-  for (int r = pt_start[e]; r < pt_start[e + 1]; ++r) {
-    const int cam = row_cam[r];
-    CamProj<T> c;
-    load_proj(ctab + (size_t)kCamTab * cam, c);
-    Proj<T> o;
-    project(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
-    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
-    T Jp[6];
-    jac_point(o, c.R, wgt, sp, Jp);
-    T v[2];
-    jac_cam_dir(o, dtab + (size_t)kDirTab * cam, X, wgt, v);
-    v[0] += beta * wgt * o.rx;
-    v[1] += beta * wgt * o.ry;
-    q[0] += Jp[0] * v[0] + Jp[3] * v[1];
-    q[1] += Jp[1] * v[0] + Jp[4] * v[1];
-    q[2] += Jp[2] * v[0] + Jp[5] * v[1];
+  if (qp.valid) {
+    for (int r = pt_start[e] + qp.sub; r < pt_start[e + 1]; r += qp.lanes) {
+      const int cam = row_cam[r];
+      CamProj<T> c;
+      load_proj(ctab + (size_t)kCamTab * cam, c);
+      Proj<T> o;
+      project(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
+      const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+      T Jp[6];
+      jac_point(o, c.R, wgt, sp, Jp);
+      T v[2];
+      jac_cam_dir(o, dtab + (size_t)kDirTab * cam, X, wgt, v);
+      v[0] += beta * wgt * o.rx;
+      v[1] += beta * wgt * o.ry;
+      q[0] += Jp[0] * v[0] + Jp[3] * v[1];
+      q[1] += Jp[1] * v[0] + Jp[4] * v[1];
+      q[2] += Jp[2] * v[0] + Jp[5] * v[1];
+    }
   }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) q[i] = group_sum(q[i], qp.lanes);
+  if (!qp.valid || qp.sub != 0) return;
   T zv[3] = {q[0], q[1], q[2]};
   if (apply_vinv) {
     const T* iv = Vinv + 6 * (size_t)e;
@@ -833,18 +913,20 @@ __global__ __launch_bounds__(kBlock) void k_rhs_finalize(int nf, const int32_t* 
 // (camera part via dtab, point part from step_p); FP64 accumulation.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_model_cost(
-    int ne, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
-    const T* __restrict__ row_obs, const T* __restrict__ pts, const T* __restrict__ pscale,
+    Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
     const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ step_p,
     T huber_a, double* __restrict__ partial) {
   __shared__ double lds[kBlock / 64];
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const SegPos qp = seg_locate(sg);
   double acc[1] = {0.0};
-  if (e < ne) {
-    const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+  if (qp.valid) {
+    const int e = qp.e;
+    T X[3], unused[3];
+    load_pk(pkX, e, X, unused);
     const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
     const T dp[3] = {step_p[3 * (size_t)e], step_p[3 * (size_t)e + 1], step_p[3 * (size_t)e + 2]};
-    for (int r = pt_start[e]; r < pt_start[e + 1]; ++r) {
+    for (int r = pt_start[e] + qp.sub; r < pt_start[e + 1]; r += qp.lanes) {
       const int cam = row_cam[r];
       CamProj<T> c;
       load_proj(ctab + (size_t)kCamTab * cam, c);
@@ -1111,7 +1193,7 @@ __global__ __launch_bounds__(kBlock) void k_s_diag(int nf, const T* __restrict__
 template <typename T>
 __global__ __launch_bounds__(64) void k_s_assemble(
     int ne, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
-    const int32_t* __restrict__ cam_to_f, const T* __restrict__ row_obs, const T* __restrict__ pts,
+    const int32_t* __restrict__ cam_to_f, const T* __restrict__ row_obs, const T* __restrict__ pkX,
     const T* __restrict__ pscale, const T* __restrict__ cscale, const T* __restrict__ ctab,
     const T* __restrict__ Vinv, T huber_a, int64_t n, T* __restrict__ S) {
   constexpr int kMaxT = 128;
@@ -1122,7 +1204,8 @@ __global__ __launch_bounds__(64) void k_s_assemble(
   if (e >= ne) return;
   const int r0 = pt_start[e], t = pt_start[e + 1] - r0;
   const int lane = threadIdx.x;
-  const T X[3] = {pts[3 * (size_t)e], pts[3 * (size_t)e + 1], pts[3 * (size_t)e + 2]};
+  T X[3], unused[3];
+  load_pk(pkX, e, X, unused);
   const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
   const T* iv = Vinv + 6 * (size_t)e;
   const T Vi[9] = {iv[0], iv[1], iv[2], iv[1], iv[3], iv[4], iv[2], iv[4], iv[5]};

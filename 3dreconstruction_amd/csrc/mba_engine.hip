@@ -306,9 +306,9 @@ struct Dev {
   int32_t *f_chunk_start = nullptr, *f_to_cam = nullptr, *cam_to_f = nullptr;
   T* cm_obs = nullptr;
   // kernel-precision parameters
-  T *pts = nullptr, *ctab = nullptr, *pscale = nullptr, *cscale = nullptr;
+  T *ctab = nullptr, *pscale = nullptr, *cscale = nullptr;
   // blocks
-  T *V = nullptr, *gp = nullptr, *Vinv = nullptr, *w = nullptr, *z = nullptr, *pk = nullptr;
+  T *V = nullptr, *gp = nullptr, *Vinv = nullptr, *z = nullptr, *pk = nullptr;
   T *U = nullptr, *gc = nullptr, *rhs = nullptr, *Minv = nullptr, *dtab = nullptr, *partial = nullptr;
   // PCG / step
   T *x = nullptr, *r = nullptr, *zc = nullptr, *p = nullptr, *q = nullptr, *tmp = nullptr;
@@ -346,7 +346,11 @@ struct mba_context {
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_ms[3] = {0, 0, 0};
-  std::vector<double> host_pts_init;  // for restoring unused points
+  // Device point order (E blocks stably grouped by lane class) and the
+  // point-major launch layout.
+  std::vector<int32_t> dev_e_to_pt;
+  Seg seg;
+  int32_t seg_blocks = 0;
 };
 
 namespace mba {
@@ -365,7 +369,8 @@ mba_status upload(mba_context* c, X** out, const std::vector<X>& v) {
 }
 
 template <typename T>
-mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& row_cam,
+mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_start,
+                           const std::vector<int32_t>& row_cam,
                            const std::vector<double>& row_obs, const std::vector<int32_t>& cm_e,
                            const std::vector<double>& cm_obs, const std::vector<int32_t>& chunk_f,
                            const std::vector<int32_t>& chunk_begin, const std::vector<int32_t>& chunk_end,
@@ -374,7 +379,7 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& row_cam,
   std::vector<T> ro(row_obs.begin(), row_obs.end()), co(cm_obs.begin(), cm_obs.end());
   MBA_TRY(upload(c, &d.row_cam, row_cam));
   MBA_TRY(upload(c, &d.row_obs, ro));
-  MBA_TRY(upload(c, &d.pt_start, c->st.pt_start));
+  MBA_TRY(upload(c, &d.pt_start, dev_pt_start));
   MBA_TRY(upload(c, &d.cm_e, cm_e));
   MBA_TRY(upload(c, &d.cm_obs, co));
   MBA_TRY(upload(c, &d.chunk_f, chunk_f));
@@ -384,14 +389,12 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& row_cam,
   MBA_TRY(upload(c, &d.f_to_cam, c->st.f_to_cam));
   MBA_TRY(upload(c, &d.cam_to_f, c->st.cam_to_f));
   const size_t ne = c->ne, nf = c->nf, nc = c->nc;
-  MBA_TRY(c->pool.alloc(&d.pts, 3 * ne));
   MBA_TRY(c->pool.alloc(&d.ctab, kCamTab * nc));
   MBA_TRY(c->pool.alloc(&d.pscale, 3 * ne));
   MBA_TRY(c->pool.alloc(&d.cscale, 9 * nf));
   MBA_TRY(c->pool.alloc(&d.V, 6 * ne));
   MBA_TRY(c->pool.alloc(&d.gp, 3 * ne));
   MBA_TRY(c->pool.alloc(&d.Vinv, 6 * ne));
-  MBA_TRY(c->pool.alloc(&d.w, 3 * ne));
   MBA_TRY(c->pool.alloc(&d.z, 3 * ne));
   MBA_TRY(c->pool.alloc(&d.pk, kPk * ne));
   MBA_TRY(c->pool.alloc(&d.U, 45 * nf));
@@ -449,16 +452,64 @@ mba_status context_create(const mba_problem* p, const mba_options* o, mba_contex
   c->ne = st.num_e;
   c->nf = (int32_t)st.order.size() - st.num_e;
   c->nrows = (int32_t)st.nobs;
-  // Rows.
+  // Device point order: the reference's E order stably grouped by lane
+  // class (lanes_for), so each point-major launch segment is contiguous.
+  std::vector<int32_t> dperm(c->ne), dev_pt_start((size_t)c->ne + 1, 0);
+  {
+    std::vector<int32_t> cls(c->ne);
+    int32_t ncls[8] = {0};
+    for (int32_t e = 0; e < c->ne; ++e) {
+      const int t = st.pt_start[e + 1] - st.pt_start[e];
+      int k = 0;
+      while (k < 6 && t > 4 * (1 << k)) ++k;
+      cls[e] = k;
+      ncls[k]++;
+    }
+    int32_t start[8] = {0};
+    for (int k = 1; k < 8; ++k) start[k] = start[k - 1] + ncls[k - 1];
+    int32_t fill[8];
+    std::memcpy(fill, start, sizeof(fill));
+    for (int32_t e = 0; e < c->ne; ++e) dperm[fill[cls[e]]++] = e;
+    c->seg.n = 0;
+    int32_t blk = 0;
+    for (int k = 0; k < 7; ++k) {
+      if (ncls[k] == 0) continue;
+      const int j = c->seg.n++;
+      c->seg.e0[j] = start[k];
+      c->seg.e1[j] = start[k] + ncls[k];
+      c->seg.lanes[j] = 1 << k;
+      c->seg.blk0[j] = blk;
+      blk += (int32_t)(((int64_t)ncls[k] * (1 << k) + kBlock - 1) / kBlock);
+    }
+    c->seg.blk0[c->seg.n] = blk;
+    if (c->seg.n == 0) {  // no points: a harmless empty segment
+      c->seg.n = 1;
+      c->seg.e0[0] = c->seg.e1[0] = 0;
+      c->seg.lanes[0] = 1;
+      c->seg.blk0[0] = 0;
+      c->seg.blk0[1] = 0;
+    }
+    c->seg_blocks = blk;
+    c->dev_e_to_pt.resize(c->ne);
+    for (int32_t de = 0; de < c->ne; ++de) {
+      const int32_t e = dperm[de];
+      c->dev_e_to_pt[de] = st.e_to_pt[e];
+      dev_pt_start[de + 1] = dev_pt_start[de] + (st.pt_start[e + 1] - st.pt_start[e]);
+    }
+  }
+  // Rows in device order (each point's rows keep the reference's order).
   std::vector<int32_t> row_cam(c->nrows), row_e(c->nrows);
   std::vector<double> row_obs(2 * (size_t)c->nrows);
-  for (int32_t e = 0; e < c->ne; ++e)
-    for (int32_t r = st.pt_start[e]; r < st.pt_start[e + 1]; ++r) row_e[r] = e;
-  for (int32_t r = 0; r < c->nrows; ++r) {
-    const int64_t i = st.res_order[r];
-    row_cam[r] = p->camera_index[i];
-    row_obs[2 * (size_t)r] = p->observations[2 * i];
-    row_obs[2 * (size_t)r + 1] = p->observations[2 * i + 1];
+  for (int32_t de = 0; de < c->ne; ++de) {
+    const int32_t e = dperm[de];
+    int32_t r = dev_pt_start[de];
+    for (int32_t rr = st.pt_start[e]; rr < st.pt_start[e + 1]; ++rr, ++r) {
+      const int64_t i = st.res_order[rr];
+      row_e[r] = de;
+      row_cam[r] = p->camera_index[i];
+      row_obs[2 * (size_t)r] = p->observations[2 * i];
+      row_obs[2 * (size_t)r + 1] = p->observations[2 * i + 1];
+    }
   }
   // Camera-major rows (stable by row order), constant cameras excluded.
   std::vector<int64_t> fstart((size_t)c->nf + 1, 0);
@@ -496,7 +547,7 @@ mba_status context_create(const mba_problem* p, const mba_options* o, mba_contex
   // FP64 state.
   std::vector<double> pts(3 * (size_t)c->ne);
   for (int32_t e = 0; e < c->ne; ++e)
-    for (int k = 0; k < 3; ++k) pts[3 * (size_t)e + k] = p->points[3 * (size_t)st.e_to_pt[e] + k];
+    for (int k = 0; k < 3; ++k) pts[3 * (size_t)e + k] = p->points[3 * (size_t)c->dev_e_to_pt[e] + k];
   std::vector<double> cams(p->cameras, p->cameras + 9 * (size_t)c->nc);
   MBA_TRY(upload(c, &c->cams64, cams));
   MBA_TRY(upload(c, &c->cams64_t, cams));
@@ -522,9 +573,11 @@ mba_status context_create(const mba_problem* p, const mba_options* o, mba_contex
   MBA_TRY(c->pool.alloc(&c->x64, 9 * (size_t)c->nf));
   MBA_TRY(c->pool.alloc(&c->r64, 9 * (size_t)c->nf));
   if (c->mixed)
-    MBA_TRY(alloc_precision<float>(c, row_cam, row_obs, cm_e, cm_obs, chunk_f, chunk_begin, chunk_end, f_chunk_start));
+    MBA_TRY(alloc_precision<float>(c, dev_pt_start, row_cam, row_obs, cm_e, cm_obs, chunk_f, chunk_begin, chunk_end,
+                                   f_chunk_start));
   else
-    MBA_TRY(alloc_precision<double>(c, row_cam, row_obs, cm_e, cm_obs, chunk_f, chunk_begin, chunk_end, f_chunk_start));
+    MBA_TRY(alloc_precision<double>(c, dev_pt_start, row_cam, row_obs, cm_e, cm_obs, chunk_f, chunk_begin, chunk_end,
+                                    f_chunk_start));
   for (int i = 0; i < 4; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
   HIP_TRY(hipStreamSynchronize(c->stream));
   (void)t0;
@@ -571,19 +624,20 @@ struct Engine {
 
   // Kernel-precision tables from the FP64 state.
   void tables(const double* cams, const double* pts) {
+    (void)pts;  // points are read in FP64 by the point pass
     hipLaunchKernelGGL(k_cam_table<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, d.ctab);
-    hipLaunchKernelGGL((k_convert<double, T>), dim3(nblocks(3 * (int64_t)c->ne)), dim3(kBlock), 0, s,
-                       (int64_t)3 * c->ne, pts, d.pts);
   }
 
   // FP64 cost at (cams, pts) -> *cost; returns false on a non-finite value.
   mba_status cost64(const double* cams, const double* pts, double* cost, bool* finite) {
     MBA_TRY(clear_flag());
     hipLaunchKernelGGL(k_cam_table<double>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, c->ctab64);
-    const int nb = nblocks(c->ne);
-    hipLaunchKernelGGL(k_cost64<double>, dim3(nb), dim3(kBlock), 0, s, c->ne, d.pt_start, d.row_cam,
-                       c->row_obs64, pts, c->ctab64, c->opt.huber_a, c->red, c->flag);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, c->red, nb, &c->sc->sum);
+    const int nb = c->seg_blocks;
+    HIP_TRY(hipMemsetAsync(c->red, 0, sizeof(double), s));
+    if (nb > 0)
+      hipLaunchKernelGGL(k_cost64, dim3(nb), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam, c->row_obs64, pts,
+                         c->ctab64, c->opt.huber_a, c->red, c->flag);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, c->red, std::max(nb, 1), &c->sc->sum);
     MBA_TRY(read_scalar(&c->sc->sum, cost));
     int32_t f = 0;
     MBA_TRY(read_flag(&f));
@@ -595,19 +649,19 @@ struct Engine {
   void jacobian_pass(double radius, int recompute_diag) {
     const double inv_r = radius > 0.0 ? 1.0 / radius : 0.0;
     if (c->timing) (void)hipEventRecord(c->ev[0], s);
-    hipLaunchKernelGGL(k_point_pass<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.pt_start,
-                       d.row_cam, d.row_obs, d.pts, d.pscale, d.ctab, huber, inv_r,
-                       c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag,
-                       d.V, d.gp, d.Vinv, d.w, d.pk, c->flag);
+    if (c->seg_blocks > 0)
+      hipLaunchKernelGGL(k_point_pass<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start,
+                         d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, inv_r,
+                         c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag,
+                         d.V, d.gp, d.Vinv, d.pk, c->flag);
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
     if (c->nchunks > 0)
       hipLaunchKernelGGL(k_cam_pass<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
-                         d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber, 1,
-                         d.partial);
+                         d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial);
     if (c->timing) (void)hipEventRecord(c->ev[2], s);
     if (c->nf > 0)
-      hipLaunchKernelGGL(k_cam_finalize<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf,
-                         d.f_chunk_start, d.partial, inv_r, c->opt.min_lm_diagonal, c->opt.max_lm_diagonal,
+      hipLaunchKernelGGL(k_cam_finalize<T>, dim3(c->nf), dim3(64), 0, s, c->nf,
+                         d.f_chunk_start, d.partial, d.cscale, inv_r, c->opt.min_lm_diagonal, c->opt.max_lm_diagonal,
                          c->diag_c, recompute_diag, 1, d.U, d.gc, d.rhs, d.Minv, c->flag);
     if (c->timing) (void)hipEventRecord(c->ev[3], s);
     c->last_radius = radius;
@@ -617,7 +671,7 @@ struct Engine {
   void damp(double radius) {
     const double inv_r = 1.0 / radius;
     hipLaunchKernelGGL(k_point_damp<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.V, d.gp,
-                       c->diag_p, inv_r, d.pscale, d.Vinv, d.w, d.pk, c->flag);
+                       c->diag_p, inv_r, d.pscale, d.Vinv, d.pk, c->flag);
     if (c->nf > 0) {
       hipLaunchKernelGGL(k_cam_damp<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf, d.U, c->diag_c,
                          inv_r, d.Minv, c->flag);
@@ -636,8 +690,9 @@ struct Engine {
     const double inv_r = c->last_radius > 0.0 ? 1.0 / c->last_radius : 0.0;
     hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
                        d.ctab, x, (T)1, d.dtab, st);
-    hipLaunchKernelGGL(k_mv_point<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.pt_start, d.row_cam,
-                       d.row_obs, d.pts, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)0, 1, d.z, d.pk, st);
+    if (c->seg_blocks > 0)
+      hipLaunchKernelGGL(k_mv_point<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
+                         d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)0, 1, d.z, d.pk, st);
     if (c->nchunks > 0)
       hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
                          d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber,
@@ -706,7 +761,7 @@ struct Engine {
     hipLaunchKernelGGL(k_s_diag<T>, dim3(nblocks(81 * (int64_t)c->nf)), dim3(kBlock), 0, s, c->nf, d.U,
                        c->diag_c, inv_r, n, d.S);
     hipLaunchKernelGGL(k_s_assemble<T>, dim3(std::max(1, c->ne)), dim3(64), 0, s, c->ne, d.pt_start, d.row_cam,
-                       d.cam_to_f, d.row_obs, d.pts, d.pscale, d.cscale, d.ctab, d.Vinv, huber, n, d.S);
+                       d.cam_to_f, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab, d.Vinv, huber, n, d.S);
     HIP_TRY(hipMemcpyAsync(d.L, d.S, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(c->info, 0, sizeof(int32_t), s));
     for (int64_t k = 0; k < n; k += kNB) {
@@ -752,9 +807,10 @@ struct Engine {
   void back_substitute() {
     hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
                        d.ctab, d.x, (T)-1, d.dtab, (const PcgState*)nullptr);
-    hipLaunchKernelGGL(k_mv_point<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.pt_start, d.row_cam,
-                       d.row_obs, d.pts, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)1, 1, d.z, (T*)nullptr,
-                       (const PcgState*)nullptr);
+    if (c->seg_blocks > 0)
+      hipLaunchKernelGGL(k_mv_point<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
+                         d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)1, 1, d.z, (T*)nullptr,
+                         (const PcgState*)nullptr);
     hipLaunchKernelGGL(k_negate<T>, dim3(nblocks(3 * (int64_t)c->ne)), dim3(kBlock), 0, s,
                        (int64_t)3 * c->ne, d.z, d.step_p);
     hipLaunchKernelGGL(k_negate<T>, dim3(nblocks(9 * (int64_t)c->nf)), dim3(kBlock), 0, s,
@@ -765,10 +821,12 @@ struct Engine {
   mba_status model_cost_change(double* mcc) {
     hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
                        d.ctab, d.step_c, (T)1, d.dtab, (const PcgState*)nullptr);
-    const int nb = nblocks(c->ne);
-    hipLaunchKernelGGL(k_model_cost<T>, dim3(nb), dim3(kBlock), 0, s, c->ne, d.pt_start, d.row_cam, d.row_obs,
-                       d.pts, d.pscale, d.ctab, d.dtab, d.step_p, huber, c->red);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, c->red, nb, &c->sc->sum);
+    const int nb = c->seg_blocks;
+    HIP_TRY(hipMemsetAsync(c->red, 0, sizeof(double), s));
+    if (nb > 0)
+      hipLaunchKernelGGL(k_model_cost<T>, dim3(nb), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam, d.row_obs,
+                         d.pk, d.pscale, d.ctab, d.dtab, d.step_p, huber, c->red);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, c->red, std::max(nb, 1), &c->sc->sum);
     double v = 0.0;
     MBA_TRY(read_scalar(&c->sc->sum, &v));
     *mcc = -v;
@@ -1177,7 +1235,7 @@ mba_status mba_context_read_schur(mba_context* ctx, double* U, double* gc, doubl
       if (rhs) std::memcpy(rhs + 9 * (size_t)cam, hr.data() + 9 * (size_t)f, 9 * sizeof(double));
     }
     for (int32_t e = 0; e < ctx->ne; ++e) {
-      const int32_t q = st.e_to_pt[e];
+      const int32_t q = ctx->dev_e_to_pt[e];
       if (V) std::memcpy(V + 6 * (size_t)q, hV.data() + 6 * (size_t)e, 6 * sizeof(double));
       if (gp) std::memcpy(gp + 3 * (size_t)q, hgp.data() + 3 * (size_t)e, 3 * sizeof(double));
     }
@@ -1202,7 +1260,7 @@ mba_status mba_context_read_scale(mba_context* ctx, double* cam_scale, double* p
   if (pt_scale) {
     for (int64_t i = 0; i < 3 * (int64_t)ctx->np; ++i) pt_scale[i] = 1.0;
     for (int32_t e = 0; e < ctx->ne; ++e)
-      std::memcpy(pt_scale + 3 * (size_t)ctx->st.e_to_pt[e], ps.data() + 3 * (size_t)e, 24);
+      std::memcpy(pt_scale + 3 * (size_t)ctx->dev_e_to_pt[e], ps.data() + 3 * (size_t)e, 24);
   }
   return MBA_OK;
 }
@@ -1263,7 +1321,7 @@ mba_status mba_context_download(mba_context* ctx, mba_problem* p) {
     std::memcpy(p->cameras + 9 * (size_t)cam, cams.data() + 9 * (size_t)cam, 72);
   }
   for (int32_t e = 0; e < ctx->ne; ++e)
-    std::memcpy(p->points + 3 * (size_t)ctx->st.e_to_pt[e], pts.data() + 3 * (size_t)e, 24);
+    std::memcpy(p->points + 3 * (size_t)ctx->dev_e_to_pt[e], pts.data() + 3 * (size_t)e, 24);
   return MBA_OK;
 }
 
