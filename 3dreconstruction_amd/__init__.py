@@ -98,7 +98,7 @@ _lib = None
 
 #: every exported symbol of include/mba.h
 EXPORTS = [
-    "mba_default_options", "mba_solve", "mba_evaluate", "mba_schur_structure",
+    "mba_default_options", "mba_solve", "mba_solve_batched", "mba_evaluate", "mba_schur_structure",
     "mba_context_create", "mba_context_destroy", "mba_context_jacobian_schur",
     "mba_context_read_schur", "mba_context_read_scale", "mba_context_schur_matvec",
     "mba_context_solve", "mba_context_download", "mba_context_synchronize", "mba_context_stream",
@@ -124,6 +124,7 @@ def load_library():
     lib.mba_last_error.restype = C.c_char_p
     lib.mba_context_stream.restype = C.c_void_p
     lib.mba_context_create.argtypes = [C.POINTER(_Problem), C.POINTER(Options), C.POINTER(C.c_void_p)]
+    lib.mba_solve_batched.argtypes = [C.POINTER(Options), C.POINTER(_Problem), C.c_int32, C.POINTER(_Summary)]
     for name in ["mba_context_destroy", "mba_context_synchronize"]:
         getattr(lib, name).argtypes = [C.c_void_p]
     lib.mba_context_jacobian_schur.argtypes = [C.c_void_p, C.c_double]
@@ -234,6 +235,20 @@ def solve(problem: BAProblem, options: Options | None = None, max_records: int =
     _check(lib.mba_solve(C.byref(o), C.byref(s), C.byref(summ), its, C.c_int32(max_records)))
     n = min(summ.num_iterations, max_records)
     return Summary(summ, [its[i] for i in range(n)])
+
+
+def solve_batched(problems, options: Options | None = None) -> list:
+    """mba_solve_batched: independent windows solved together; each usable
+    window's parameters are updated in place.  Returns one Summary each."""
+    lib = load_library()
+    o = options if options is not None else default_options()
+    n = len(problems)
+    arr = (_Problem * max(n, 1))()
+    for i, p in enumerate(problems):
+        arr[i] = p._struct()
+    sums = (_Summary * max(n, 1))()
+    _check(lib.mba_solve_batched(C.byref(o), arr, C.c_int32(n), sums))
+    return [Summary(sums[i], []) for i in range(n)]
 
 
 def evaluate(problem: BAProblem, precision=PRECISION_FP64, huber_a=0.0, jacobians=True, device=0):

@@ -77,6 +77,44 @@ __device__ __forceinline__ T group_sum(T v, int lanes) {
   return v;
 }
 
+// Independent problems ("windows") solved together: the window of every
+// device point block and camera block, per-window 1 / radius, per-window
+// accumulators.  A single problem has of_e = of_f = nullptr (window 0) and
+// acc = nullptr (deterministic block partials instead of atomics).
+struct Win {
+  const int32_t* of_e;
+  const int32_t* of_f;
+  const double* inv_r;
+  double* acc;
+  int32_t* flag;  // per-window "non-finite / failed factorisation" flags
+};
+__device__ __forceinline__ int win_of_e(const Win& w, int e) { return w.of_e ? w.of_e[e] : 0; }
+__device__ __forceinline__ int win_of_f(const Win& w, int f) { return w.of_f ? w.of_f[f] : 0; }
+
+// acc[stride * window + k] += v for every lane (wi < 0: no contribution).
+// All lanes of the wave must call it.  A wave whose contributing lanes share
+// one window adds its wave sum with a single atomic.
+__device__ __forceinline__ void win_atomic_add(double* acc, int stride, int k, int wi, double v) {
+  const unsigned long long valid = __ballot(wi >= 0);
+  if (valid == 0ull) return;
+  const int leader = __ffsll((long long)valid) - 1;
+  const int wl = __shfl(wi, leader, 64);
+  const int wv = wi < 0 ? wl : wi;
+  if (__all(wv == wl)) {
+    double s = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((int)(threadIdx.x & 63) == leader) atomicAdd(acc + (size_t)stride * wl + k, s);
+  } else if (wi >= 0 && v != 0.0) {
+    atomicAdd(acc + (size_t)stride * wi + k, v);
+  }
+}
+
+// Max of a non-negative double via its ordered bit pattern.
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
 template <typename T> __device__ __forceinline__ T tiny();
 template <> __device__ __forceinline__ float tiny<float>() { return FLT_MIN; }
 template <> __device__ __forceinline__ double tiny<double>() { return DBL_MIN; }
@@ -380,8 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __rest
                                                    const double* __restrict__ row_obs,
                                                    const double* __restrict__ pts,
                                                    const double* __restrict__ ctab, double huber_a,
-                                                   double* __restrict__ partial,
-                                                   int32_t* __restrict__ flag) {
+                                                   double* __restrict__ partial, Win win) {
   __shared__ double lds[kBlock / 64];
   const SegPos sp = seg_locate(sg);
   double acc[1] = {0.0};
@@ -398,7 +435,11 @@ __global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __rest
       if (!isfinite(s)) bad = 1;
       acc[0] += huber_cost(s, huber_a);
     }
-    if (bad) *flag = 1;
+    if (bad) win.flag[win_of_e(win, e)] = 1;
+  }
+  if (win.acc) {
+    win_atomic_add(win.acc, 1, 0, sp.valid ? win_of_e(win, sp.e) : -1, acc[0]);
+    return;
   }
   block_sum<double, 1>(acc, lds);
   if (threadIdx.x == 0) {
@@ -451,6 +492,22 @@ __global__ __launch_bounds__(kBlock) void k_grad_partial(int64_t n, const T* __r
   }
 }
 
+// Per-window max |g / scale| and sum of squares into out[2 * window + {0, 1}].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grad_win(int64_t n, int per, int is_cam, const T* __restrict__ g,
+                                                     const double* __restrict__ scale, Win win,
+                                                     double* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  double v = 0.0;
+  int wi = -1;
+  if (i < n) {
+    v = (double)g[i] / scale[i];
+    wi = is_cam ? win_of_f(win, (int)(i / per)) : win_of_e(win, (int)(i / per));
+    atomic_max_nonneg(out + 2 * (size_t)wi, fabs(v));
+  }
+  win_atomic_add(out, 2, 1, wi, v * v);
+}
+
 __global__ void k_grad_final(const double* __restrict__ pmax, const double* __restrict__ psq, int n,
                              double* __restrict__ out /* [max, sumsq] accumulate */) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -490,9 +547,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_point_pass(
     Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const double* __restrict__ pts64, const T* __restrict__ pscale,
-    const T* __restrict__ ctab, T huber_a, double inv_radius, double min_diag, double max_diag,
+    const T* __restrict__ ctab, T huber_a, Win win, double min_diag, double max_diag,
     double* __restrict__ diag_p, int recompute_diag, T* __restrict__ V, T* __restrict__ gp,
-    T* __restrict__ Vinv, T* __restrict__ pk, int32_t* __restrict__ flag) {
+    T* __restrict__ Vinv, T* __restrict__ pk) {
   const SegPos q = seg_locate(sg);
   const int e = q.valid ? q.e : sg.e0[0];
   const T X[3] = {(T)pts64[3 * (size_t)e], (T)pts64[3 * (size_t)e + 1], (T)pts64[3 * (size_t)e + 2]};
@@ -532,13 +589,15 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
       dg[i] = diag_p[3 * (size_t)e + i];
     }
   }
+  const int wi = win_of_e(win, e);
+  const double inv_radius = win.inv_r[wi];
   double a[6] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3], (double)v[4], (double)v[5]};
   a[0] += dg[0] * inv_radius;
   a[3] += dg[1] * inv_radius;
   a[5] += dg[2] * inv_radius;
   double iv[6];
   if (!inv3_spd(a, iv)) {
-    *flag = 1;
+    win.flag[wi] = 1;
     for (int i = 0; i < 6; ++i) iv[i] = 0.0;
   }
 #pragma unroll
@@ -554,12 +613,13 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restrict__ V,
                                                        const T* __restrict__ gp,
-                                                       const double* __restrict__ diag_p,
-                                                       double inv_radius, const T* __restrict__ pscale,
-                                                       T* __restrict__ Vinv, T* __restrict__ pk,
-                                                       int32_t* __restrict__ flag) {
+                                                       const double* __restrict__ diag_p, Win win,
+                                                       const T* __restrict__ pscale,
+                                                       T* __restrict__ Vinv, T* __restrict__ pk) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
+  const int wi = win_of_e(win, e);
+  const double inv_radius = win.inv_r[wi];
   double a[6];
   for (int i = 0; i < 6; ++i) a[i] = (double)V[6 * (size_t)e + i];
   a[0] += diag_p[3 * (size_t)e] * inv_radius;
@@ -567,7 +627,7 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restri
   a[5] += diag_p[3 * (size_t)e + 2] * inv_radius;
   double iv[6];
   if (!inv3_spd(a, iv)) {
-    *flag = 1;
+    win.flag[wi] = 1;
     for (int i = 0; i < 6; ++i) iv[i] = 0.0;
   }
   for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
@@ -664,9 +724,9 @@ __global__ __launch_bounds__(kBlock) void k_cam_pass(
 template <typename T>
 __global__ __launch_bounds__(64) void k_cam_finalize(
     int nf, const int32_t* __restrict__ f_chunk_start, const T* __restrict__ partial,
-    const T* __restrict__ cscale, double inv_radius, double min_diag, double max_diag,
+    const T* __restrict__ cscale, Win win, double min_diag, double max_diag,
     double* __restrict__ diag_c, int recompute_diag, int with_h, T* __restrict__ U, T* __restrict__ gc,
-    T* __restrict__ rhs, T* __restrict__ Minv, int32_t* __restrict__ flag) {
+    T* __restrict__ rhs, T* __restrict__ Minv) {
   __shared__ double sum[64];
   __shared__ int ij[64];
   const int f = blockIdx.x;
@@ -695,6 +755,8 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
   sum[k] = (double)vt;
   __syncthreads();
   if (k != 0) return;
+  const int wi = win_of_f(win, f);
+  const double inv_radius = win.inv_r[wi];
   double A[81], inv[81];
   for (int i = 0; i < 9; ++i)
     for (int j = i; j < 9; ++j) {
@@ -713,7 +775,7 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
     A[i * 10] += dg * inv_radius;
   }
   if (!invn_spd(A, 9, inv)) {
-    *flag = 1;
+    win.flag[wi] = 1;
     for (int q = 0; q < 81; ++q) inv[q] = 0.0;
   }
   for (int q = 0; q < 81; ++q) Minv[81 * (size_t)f + q] = (T)inv[q];
@@ -722,11 +784,12 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
 // Re-damp the camera preconditioner after a rejected step.
 template <typename T>
 __global__ __launch_bounds__(64) void k_cam_damp(int nf, const T* __restrict__ U,
-                                                 const double* __restrict__ diag_c,
-                                                 double inv_radius, T* __restrict__ Minv,
-                                                 int32_t* __restrict__ flag) {
+                                                 const double* __restrict__ diag_c, Win win,
+                                                 T* __restrict__ Minv) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
+  const int wi = win_of_f(win, f);
+  const double inv_radius = win.inv_r[wi];
   double A[81], inv[81];
   for (int i = 0; i < 9; ++i)
     for (int j = i; j < 9; ++j) {
@@ -736,7 +799,7 @@ __global__ __launch_bounds__(64) void k_cam_damp(int nf, const T* __restrict__ U
     }
   for (int i = 0; i < 9; ++i) A[i * 10] += diag_c[9 * (size_t)f + i] * inv_radius;
   if (!invn_spd(A, 9, inv)) {
-    *flag = 1;
+    win.flag[wi] = 1;
     for (int k = 0; k < 81; ++k) inv[k] = 0.0;
   }
   for (int k = 0; k < 81; ++k) Minv[81 * (size_t)f + k] = (T)inv[k];
@@ -750,8 +813,8 @@ __global__ __launch_bounds__(kBlock) void k_cam_dir(int nc, const int32_t* __res
                                                     const T* __restrict__ cscale,
                                                     const T* __restrict__ ctab,
                                                     const T* __restrict__ x, T sign,
-                                                    T* __restrict__ dtab, const PcgState* __restrict__ st) {
-  if (st && st->done) return;
+                                                    T* __restrict__ dtab, const int32_t* __restrict__ all_done) {
+  if (all_done && *all_done) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   T* o = dtab + (size_t)kDirTab * c;
@@ -775,8 +838,8 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
     Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
     const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ Vinv, T huber_a,
-    T beta, int apply_vinv, T* __restrict__ zout, T* __restrict__ pk, const PcgState* __restrict__ st) {
-  if (st && st->done) return;
+    T beta, int apply_vinv, T* __restrict__ zout, T* __restrict__ pk, const int32_t* __restrict__ all_done) {
+  if (all_done && *all_done) return;
   const SegPos qp = seg_locate(sg);
   const int e = qp.valid ? qp.e : sg.e0[0];
   T X[3], unused[3];
@@ -826,9 +889,9 @@ __global__ __launch_bounds__(kBlock) void k_mv_cam(
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
     const T* __restrict__ cscale, const T* __restrict__ ctab, T huber_a, T* __restrict__ partial,
-    const PcgState* __restrict__ st) {
+    const int32_t* __restrict__ all_done) {
   __shared__ T lds[9 * (kBlock / 64)];
-  if (st && st->done) return;
+  if (all_done && *all_done) return;
   const int ch = blockIdx.x;
   const int f = chunk_f[ch];
   const T* tab = ctab + (size_t)kCamTab * f_to_cam[f];
@@ -872,11 +935,12 @@ __global__ __launch_bounds__(kBlock) void k_mv_cam(
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mv_finalize(
     int nf, const int32_t* __restrict__ f_chunk_start, const T* __restrict__ partial,
-    const T* __restrict__ U, const double* __restrict__ diag_c, double inv_radius,
-    const T* __restrict__ x, int u_term, T* __restrict__ y, const PcgState* __restrict__ st) {
-  if (st && st->done) return;
+    const T* __restrict__ U, const double* __restrict__ diag_c, Win win,
+    const T* __restrict__ x, int u_term, T* __restrict__ y, const int32_t* __restrict__ all_done) {
+  if (all_done && *all_done) return;
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
+  const double inv_radius = win.inv_r[win_of_f(win, f)];
   T s[9];
   for (int k = 0; k < 9; ++k) s[k] = T(0);
   for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch)
@@ -916,7 +980,7 @@ __global__ __launch_bounds__(kBlock) void k_model_cost(
     Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
     const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ step_p,
-    T huber_a, double* __restrict__ partial) {
+    T huber_a, double* __restrict__ partial, Win win) {
   __shared__ double lds[kBlock / 64];
   const SegPos qp = seg_locate(sg);
   double acc[1] = {0.0};
@@ -943,6 +1007,10 @@ __global__ __launch_bounds__(kBlock) void k_model_cost(
       acc[0] += m0 * ((double)(wgt * o.rx) + 0.5 * m0) + m1 * ((double)(wgt * o.ry) + 0.5 * m1);
     }
   }
+  if (win.acc) {
+    win_atomic_add(win.acc, 1, 0, qp.valid ? win_of_e(win, qp.e) : -1, acc[0]);
+    return;
+  }
   block_sum<double, 1>(acc, lds);
   if (threadIdx.x == 0) {
     double s = 0.0;
@@ -956,15 +1024,20 @@ __global__ __launch_bounds__(kBlock) void k_plus(int64_t n, const double* __rest
                                                  const double* __restrict__ step,
                                                  const double* __restrict__ scale,
                                                  double* __restrict__ out,
-                                                 double* __restrict__ partial) {
+                                                 double* __restrict__ partial, Win win) {
   __shared__ double lds[kBlock / 64];
   double acc[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (i < n) {
     const double d = step[i] * scale[i];
     const double v = x[i] + d;
     out[i] = v;
     const double df = x[i] - v;
-    acc[0] += df * df;
+    acc[0] = df * df;
+  }
+  if (win.acc) {
+    win_atomic_add(win.acc, 1, 0, i < n ? win_of_e(win, (int)(i / 3)) : -1, acc[0]);
+    return;
   }
   block_sum<double, 1>(acc, lds);
   if (threadIdx.x == 0) {
@@ -974,12 +1047,18 @@ __global__ __launch_bounds__(kBlock) void k_plus(int64_t n, const double* __rest
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_sumsq(int64_t n, const double* __restrict__ x,
-                                                  double* __restrict__ partial) {
+// Sum of squares of [points (3 per E block) | cameras (9 per F block)].
+__global__ __launch_bounds__(kBlock) void k_sumsq(int64_t n, int64_t npts3, const double* __restrict__ x,
+                                                  double* __restrict__ partial, Win win) {
   __shared__ double lds[kBlock / 64];
   double acc[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    acc[0] += x[i] * x[i];
+  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (i < n) acc[0] = x[i] * x[i];
+  if (win.acc) {
+    const int wi = i >= n ? -1 : (i < npts3 ? win_of_e(win, (int)(i / 3)) : win_of_f(win, (int)((i - npts3) / 9)));
+    win_atomic_add(win.acc, 1, 0, wi, acc[0]);
+    return;
+  }
   block_sum<double, 1>(acc, lds);
   if (threadIdx.x == 0) {
     double s = 0.0;
@@ -1016,31 +1095,38 @@ __global__ __launch_bounds__(kBlock) void k_scatter_cams(int nf, const int32_t* 
 // on the device.  F-space vectors, one thread per 9-entry F block; block
 // partial dot products reduced by single-thread scalar kernels.
 
-// p = z + beta p (or z on the first iteration); then the direction table.
+// p = z + beta p (or z on the first iteration).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_p(int nf, const PcgState* __restrict__ st,
-                                                  const T* __restrict__ z, T* __restrict__ p) {
-  if (st->done) return;
+__global__ __launch_bounds__(kBlock) void k_pcg_p(int nf, const PcgState* __restrict__ st, Win win,
+                                                  const T* __restrict__ z, T* __restrict__ p,
+                                                  const int32_t* __restrict__ all_done) {
+  if (*all_done) return;
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
-  const T beta = (T)st->beta;
+  const PcgState& s = st[win_of_f(win, f)];
+  if (s.done) return;
+  const T beta = (T)s.beta;
   for (int k = 0; k < 9; ++k) {
     const size_t i = 9 * (size_t)f + k;
-    p[i] = (st->iter == 1) ? z[i] : z[i] + beta * p[i];
+    p[i] = (s.iter == 1) ? z[i] : z[i] + beta * p[i];
   }
 }
 
-// Dot partials: out[b] = sum a.b over block (FP64 accumulation).
+// Dot partials over F-space entries: block partials (single problem) or
+// per-window accumulators.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dot_partial(int64_t n, const PcgState* __restrict__ st,
-                                                        const T* __restrict__ a,
-                                                        const T* __restrict__ b,
+__global__ __launch_bounds__(kBlock) void k_dot_partial(int64_t n, const int32_t* __restrict__ all_done, Win win,
+                                                        const T* __restrict__ a, const T* __restrict__ b,
                                                         double* __restrict__ partial) {
   __shared__ double lds[kBlock / 64];
-  if (st && st->done) return;
+  if (all_done && *all_done) return;
+  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
   double acc[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    acc[0] += (double)a[i] * (double)b[i];
+  if (i < n) acc[0] = (double)a[i] * (double)b[i];
+  if (win.acc) {
+    win_atomic_add(win.acc, 1, 0, i < n ? win_of_f(win, (int)(i / 9)) : -1, acc[0]);
+    return;
+  }
   block_sum<double, 1>(acc, lds);
   if (threadIdx.x == 0) {
     double s = 0.0;
@@ -1049,74 +1135,97 @@ __global__ __launch_bounds__(kBlock) void k_dot_partial(int64_t n, const PcgStat
   }
 }
 
-// alpha = rho / p.q, with the reference's failure tests.
-__global__ void k_pcg_alpha(const double* __restrict__ partial, int nb, PcgState* __restrict__ st) {
-  if (threadIdx.x != 0 || st->done) return;
-  double pq = 0.0;
-  for (int i = 0; i < nb; ++i) pq += partial[i];
-  st->pq = pq;
-  if (pq <= 0.0 || isinf(pq)) {  // indefinite: NO_CONVERGENCE, keep x
-    st->status = 1;
-    st->done = 1;
-    return;
-  }
-  st->alpha = st->rho / pq;
-  if (isinf(st->alpha)) {
-    st->status = -1;
-    st->done = 1;
+// alpha = rho / p.q per window, with the reference's failure tests.  The
+// per-window accumulators (windowed runs) are cleared as they are read.
+__global__ __launch_bounds__(256) void k_pcg_alpha(const double* __restrict__ partial, int nb, int nw,
+                                                   PcgState* __restrict__ st, double* __restrict__ wacc,
+                                                   const int32_t* __restrict__ all_done) {
+  if (*all_done) return;
+  for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+    PcgState& s = st[w];
+    if (s.done) continue;
+    double pq = 0.0;
+    if (wacc) {
+      pq = wacc[w];
+      wacc[w] = 0.0;  // ready for the next iteration's dot
+    } else {
+      for (int i = 0; i < nb; ++i) pq += partial[i];
+    }
+    s.pq = pq;
+    if (pq <= 0.0 || isinf(pq)) {  // indefinite: NO_CONVERGENCE, keep x
+      s.status = 1;
+      s.done = 1;
+      continue;
+    }
+    s.alpha = s.rho / pq;
+    if (isinf(s.alpha)) {
+      s.status = -1;
+      s.done = 1;
+    }
   }
 }
 
 // PCG vector update, one thread per F block.  mode 0: x += alpha p,
 // r -= alpha q;  1: x += alpha p only;  2: r = b - tmp;  3: r = b (x = 0).
-// Modes 0, 2, 3 then apply z = Minv r and write partials of r.z,
-// x.(b + r) and r.r.
+// Modes 0, 2, 3 then apply z = Minv r and reduce r.z, x.(b + r), r.r.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int nf, const PcgState* __restrict__ st,
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int nf, const PcgState* __restrict__ st, Win win,
                                                        int mode, const T* __restrict__ p,
                                                        const T* __restrict__ q,
                                                        const T* __restrict__ b,
                                                        const T* __restrict__ tmp,
                                                        const T* __restrict__ Minv, T* __restrict__ x,
                                                        T* __restrict__ r, T* __restrict__ z,
-                                                       double* __restrict__ partial) {
+                                                       double* __restrict__ partial,
+                                                       const int32_t* __restrict__ all_done) {
   __shared__ double lds[3 * (kBlock / 64)];
-  if (st->done) return;
+  if (*all_done) return;
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[3] = {0.0, 0.0, 0.0};
+  int wi = -1;
   if (f < nf) {
-    const T alpha = (T)st->alpha;
-    T rr[9];
-    for (int k = 0; k < 9; ++k) {
-      const size_t i = 9 * (size_t)f + k;
-      T xv;
-      if (mode == 3) {
-        xv = T(0);
-        x[i] = xv;
-      } else if (mode <= 1) {
-        xv = x[i] + alpha * p[i];
-        x[i] = xv;
-      } else {
-        xv = x[i];
+    wi = win_of_f(win, f);
+    const PcgState& s = st[wi];
+    if (s.done) {
+      wi = -1;
+    } else {
+      const T alpha = (T)s.alpha;
+      T rr[9];
+      for (int k = 0; k < 9; ++k) {
+        const size_t i = 9 * (size_t)f + k;
+        T xv;
+        if (mode == 3) {
+          xv = T(0);
+          x[i] = xv;
+        } else if (mode <= 1) {
+          xv = x[i] + alpha * p[i];
+          x[i] = xv;
+        } else {
+          xv = x[i];
+        }
+        if (mode == 1) continue;
+        const T rv = (mode == 0) ? r[i] - alpha * q[i] : (mode == 2 ? b[i] - tmp[i] : b[i]);
+        r[i] = rv;
+        rr[k] = rv;
+        acc[1] += (double)xv * ((double)b[i] + (double)rv);
+        acc[2] += (double)rv * (double)rv;
       }
-      if (mode == 1) continue;
-      const T rv = (mode == 0) ? r[i] - alpha * q[i] : (mode == 2 ? b[i] - tmp[i] : b[i]);
-      r[i] = rv;
-      rr[k] = rv;
-      acc[1] += (double)xv * ((double)b[i] + (double)rv);
-      acc[2] += (double)rv * (double)rv;
-    }
-    if (mode != 1) {
-      const T* M = Minv + 81 * (size_t)f;
-      for (int i = 0; i < 9; ++i) {
-        T a = T(0);
-        for (int j = 0; j < 9; ++j) a += M[9 * i + j] * rr[j];
-        z[9 * (size_t)f + i] = a;
-        acc[0] += (double)rr[i] * (double)a;
+      if (mode != 1) {
+        const T* M = Minv + 81 * (size_t)f;
+        for (int i = 0; i < 9; ++i) {
+          T a = T(0);
+          for (int j = 0; j < 9; ++j) a += M[9 * i + j] * rr[j];
+          z[9 * (size_t)f + i] = a;
+          acc[0] += (double)rr[i] * (double)a;
+        }
       }
     }
   }
   if (mode == 1) return;
+  if (win.acc) {
+    for (int k = 0; k < 3; ++k) win_atomic_add(win.acc, 3, k, wi, acc[k]);
+    return;
+  }
   block_sum<double, 3>(acc, lds);
   if (threadIdx.x < 3) {
     double s = 0.0;
@@ -1125,51 +1234,101 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int nf, const PcgState* _
   }
 }
 
-// Termination tests and the next rho / beta.  `first`: the initial call
-// (x = 0, r = b) which only sets rho for iteration 1.
-__global__ void k_pcg_scalar(const double* __restrict__ partial, int nb, PcgState* __restrict__ st,
-                             double eta, int min_iters, int max_iters, int first) {
-  if (threadIdx.x != 0 || st->done) return;
-  double rz = 0.0, xbr = 0.0, rr = 0.0;
-  for (int i = 0; i < nb; ++i) {
-    rz += partial[3 * i];
-    xbr += partial[3 * i + 1];
-    rr += partial[3 * i + 2];
-  }
-  st->norm_r = sqrt(rr);
-  if (first) {
-    st->norm_b = sqrt(rr);  // r = b at x = 0
-    st->tol_r = st->tol_r * st->norm_b;
-    if (st->norm_b == 0.0) {
-      st->done = 1;
-      st->status = 0;
-      return;
+// Per window: termination tests and the next rho / beta; then the
+// device-wide all-done flag.  `first`: the initial call (x = 0, r = b).
+__global__ __launch_bounds__(256) void k_pcg_scalar(const double* __restrict__ partial, int nb, int nw,
+                                                    PcgState* __restrict__ st, double* __restrict__ wacc,
+                                                    double eta, int min_iters, int max_iters, int first,
+                                                    int32_t* __restrict__ all_done) {
+  __shared__ int any_active;
+  if (threadIdx.x == 0) any_active = 0;
+  __syncthreads();
+  for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+    PcgState& s = st[w];
+    if (s.done) continue;
+    double rz = 0.0, xbr = 0.0, rr = 0.0;
+    if (wacc) {
+      rz = wacc[3 * (size_t)w];
+      xbr = wacc[3 * (size_t)w + 1];
+      rr = wacc[3 * (size_t)w + 2];
+      for (int k = 0; k < 3; ++k) wacc[3 * (size_t)w + k] = 0.0;
+    } else {
+      for (int i = 0; i < nb; ++i) {
+        rz += partial[3 * i];
+        xbr += partial[3 * i + 1];
+        rr += partial[3 * i + 2];
+      }
     }
-    if (min_iters == 0 && st->norm_r <= st->tol_r) {
-      st->done = 1;
-      return;
+    s.norm_r = sqrt(rr);
+    bool stop = false;
+    if (first) {
+      s.norm_b = sqrt(rr);  // r = b at x = 0
+      s.tol_r = s.tol_r * s.norm_b;
+      if (s.norm_b == 0.0) {
+        stop = true;
+      } else if (min_iters == 0 && s.norm_r <= s.tol_r) {
+        stop = true;
+      } else {
+        s.Q0 = -xbr;
+        s.rho = 1.0;
+      }
+    } else {
+      const double Q1 = -xbr;
+      const double zeta = s.iter * (Q1 - s.Q0) / Q1;
+      s.zeta = zeta;
+      if (zeta < eta && s.iter >= min_iters) {
+        stop = true;
+      } else {
+        s.Q0 = Q1;
+        if (s.norm_r <= s.tol_r && s.iter >= min_iters) {
+          stop = true;
+        } else if (s.iter >= max_iters) {
+          stop = true;
+          s.status = 1;
+        } else {
+          s.iter += 1;
+        }
+      }
     }
-    st->Q0 = -xbr;
-    st->rho = 1.0;
-  } else {
-    const double Q1 = -xbr;
-    const double zeta = st->iter * (Q1 - st->Q0) / Q1;
-    st->zeta = zeta;
-    if (zeta < eta && st->iter >= min_iters) { st->done = 1; return; }
-    st->Q0 = Q1;
-    if (st->norm_r <= st->tol_r && st->iter >= min_iters) { st->done = 1; return; }
-    if (st->iter >= max_iters) { st->done = 1; st->status = 1; return; }
-    st->iter += 1;
+    if (!stop) {
+      // Next iteration's rho = r.z and beta.
+      const double last = s.rho;
+      s.rho_last = last;
+      s.rho = rz;
+      if (rz == 0.0 || isinf(rz)) {
+        stop = true;
+        s.status = -1;
+      } else if (s.iter > 1) {
+        s.beta = rz / last;
+        if (s.beta == 0.0 || isinf(s.beta)) {
+          stop = true;
+          s.status = -1;
+        }
+      }
+    }
+    if (stop) s.done = 1;
+    else any_active = 1;
   }
-  // Next iteration's rho = r.z and beta.
-  const double last = st->rho;
-  st->rho_last = last;
-  st->rho = rz;
-  if (rz == 0.0 || isinf(rz)) { st->done = 1; st->status = -1; return; }
-  if (st->iter > 1) {
-    st->beta = rz / last;
-    if (st->beta == 0.0 || isinf(st->beta)) { st->done = 1; st->status = -1; return; }
-  }
+  __syncthreads();
+  if (threadIdx.x == 0) *all_done = any_active ? 0 : 1;
+}
+
+// Per-window flagged copy of trial parameters into the current ones.
+__global__ __launch_bounds__(kBlock) void k_accept_points(int ne, Win win, const uint8_t* __restrict__ accept,
+                                                          const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= 3 * (int64_t)ne) return;
+  if (accept[win_of_e(win, (int)(i / 3))]) dst[i] = src[i];
+}
+__global__ __launch_bounds__(kBlock) void k_accept_cams(int nf, Win win, const int32_t* __restrict__ f_to_cam,
+                                                        const uint8_t* __restrict__ accept,
+                                                        const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= 9 * (int64_t)nf) return;
+  const int f = (int)(i / 9);
+  if (!accept[win_of_f(win, f)]) return;
+  const size_t j = 9 * (size_t)f_to_cam[f] + i % 9;
+  dst[j] = src[j];
 }
 
 // ----------------------------------------------------------- explicit S
@@ -1319,62 +1478,120 @@ __global__ __launch_bounds__(64) void k_trsm_panel(T* __restrict__ A, int64_t n,
   for (int j = 0; j < kb; ++j) A[row + (int64_t)(k + j) * n] = x[j];
 }
 
-// Trailing update A22 -= L21 L21' on 64x64 lower tiles (register-tiled,
-// 256 threads, 4x4 outputs per thread).  Tiles strictly above the diagonal
-// are skipped.
-template <typename T>
-__global__ __launch_bounds__(256) void k_syrk_update(T* __restrict__ A, int64_t n, int k, int kb,
-                                                     int ntiles) {
-  __shared__ T As[kNB][kNB + 1];
-  __shared__ T Bs[kNB][kNB + 1];
-  // Map blockIdx.x to a lower-triangle tile (ti >= tj).
-  const int t = blockIdx.x;
-  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+// Trailing update A22 -= L21 L21' on MFMA: 64 x 64 lower tiles, four waves
+// per tile, one 32 x 32 quadrant per wave, K = kb (<= 64, zero-padded).
+//   FP32: v_mfma_f32_32x32x2_f32 -- A operand lane l = A[l % 32][k + l / 32],
+//         B operand lane l = B[k + l / 32][l % 32], result register r of lane
+//         l = D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+//   FP64: v_mfma_f64_16x16x4_f64 -- A lane l = A[l % 16][k + l / 16], B lane
+//         l = B[k + l / 16][l % 16], result r = D[l / 16 + 4 r][l % 16]
+//         (both result layouts checked on gfx950 with a lane probe).
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void syrk_tile_coords(int t, int& ti, int& tj) {
+  ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
   while (ti * (ti + 1) / 2 > t) --ti;
-  const int tj = t - ti * (ti + 1) / 2;
-  if (ti >= ntiles) return;
-  const int64_t base = (int64_t)k + kb;
-  const int64_t r0 = base + (int64_t)ti * kNB, c0 = base + (int64_t)tj * kNB;
-  for (int idx = threadIdx.x; idx < kNB * kb; idx += blockDim.x) {
-    const int i = idx % kNB, l = idx / kNB;
-    As[i][l] = (r0 + i < n) ? A[(r0 + i) + (int64_t)(k + l) * n] : T(0);
-    Bs[i][l] = (c0 + i < n) ? A[(c0 + i) + (int64_t)(k + l) * n] : T(0);
-  }
-  __syncthreads();
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  T acc[4][4];
-  for (int a = 0; a < 4; ++a)
-    for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
-  for (int l = 0; l < kb; ++l) {
-    T av[4], bv[4];
-    for (int a = 0; a < 4; ++a) av[a] = As[ty * 4 + a][l];
-    for (int b = 0; b < 4; ++b) bv[b] = Bs[tx * 4 + b][l];
-    for (int a = 0; a < 4; ++a)
-      for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
-  }
-  for (int a = 0; a < 4; ++a)
-    for (int b = 0; b < 4; ++b) {
-      const int64_t row = r0 + ty * 4 + a, col = c0 + tx * 4 + b;
-      if (row < n && col < n && row >= col) A[row + col * n] -= acc[a][b];
-    }
+  tj = t - ti * (ti + 1) / 2;
 }
 
-// Triangular solves with the factor (one workgroup; columns blocked by 64).
-// Forward: L y = b ; backward: L' x = y.  In place on b.
+template <typename T>
+__device__ __forceinline__ void syrk_load(T (*As)[kNB + 1], T (*Bs)[kNB + 1], const T* __restrict__ A,
+                                          int64_t n, int k, int kb, int64_t r0, int64_t c0) {
+  for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) {
+    const int i = idx % kNB, l = idx / kNB;
+    const bool kin = l < kb;
+    As[i][l] = (kin && r0 + i < n) ? A[(r0 + i) + (int64_t)(k + l) * n] : T(0);
+    Bs[i][l] = (kin && c0 + i < n) ? A[(c0 + i) + (int64_t)(k + l) * n] : T(0);
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_syrk_mfma_f32(float* __restrict__ A, int64_t n, int k, int kb) {
+  __shared__ float As[kNB][kNB + 1];
+  __shared__ float Bs[kNB][kNB + 1];
+  int ti, tj;
+  syrk_tile_coords(blockIdx.x, ti, tj);
+  const int64_t base = (int64_t)k + kb;
+  const int64_t r0 = base + (int64_t)ti * kNB, c0 = base + (int64_t)tj * kNB;
+  syrk_load(As, Bs, A, n, k, kb, r0, c0);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int qi = w >> 1, qj = w & 1;
+  floatx16 acc;
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  const int kpad = (kb + 1) & ~1;
+  for (int kk = 0; kk < kpad; kk += 2) {
+    const float a = As[qi * 32 + (lane & 31)][kk + (lane >> 5)];
+    const float b = Bs[qj * 32 + (lane & 31)][kk + (lane >> 5)];
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+  }
+  for (int r = 0; r < 16; ++r) {
+    const int64_t row = r0 + qi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int64_t col = c0 + qj * 32 + (lane & 31);
+    if (row < n && col < n && row >= col) A[row + col * n] -= acc[r];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_syrk_mfma_f64(double* __restrict__ A, int64_t n, int k, int kb) {
+  __shared__ double As[kNB][kNB + 1];
+  __shared__ double Bs[kNB][kNB + 1];
+  int ti, tj;
+  syrk_tile_coords(blockIdx.x, ti, tj);
+  const int64_t base = (int64_t)k + kb;
+  const int64_t r0 = base + (int64_t)ti * kNB, c0 = base + (int64_t)tj * kNB;
+  syrk_load(As, Bs, A, n, k, kb, r0, c0);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int qi = w >> 1, qj = w & 1;
+  doublex4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.0;
+  const int kpad = (kb + 3) & ~3;
+  for (int kk = 0; kk < kpad; kk += 4) {
+    double av[2], bv[2];
+    for (int q = 0; q < 2; ++q) {
+      av[q] = As[qi * 32 + q * 16 + (lane & 15)][kk + (lane >> 4)];
+      bv[q] = Bs[qj * 32 + q * 16 + (lane & 15)][kk + (lane >> 4)];
+    }
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+  }
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = r0 + qi * 32 + a * 16 + (lane >> 4) + 4 * r;
+        const int64_t col = c0 + qj * 32 + b * 16 + (lane & 15);
+        if (row < n && col < n && row >= col) A[row + col * n] -= acc[a][b][r];
+      }
+}
+
+// Triangular solves with the factor (one workgroup; columns blocked by 64,
+// each diagonal block solved by wave 0 from LDS).  Forward: L y = b.
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_lower(const T* __restrict__ L, int64_t n,
                                                     T* __restrict__ b) {
+  __shared__ T D[kNB][kNB + 1];
   __shared__ T xs[kNB];
+  const int lane = threadIdx.x & 63;
   for (int64_t k = 0; k < n; k += kNB) {
     const int kb = (int)min((int64_t)kNB, n - k);
-    if (threadIdx.x == 0) {
+    for (int idx = threadIdx.x; idx < kb * kb; idx += blockDim.x) {
+      const int i = idx % kb, j = idx / kb;
+      D[i][j] = L[(k + i) + (k + j) * n];
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      T bi = lane < kb ? b[k + lane] : T(0);
       for (int j = 0; j < kb; ++j) {
-        T s = b[k + j];
-        for (int l = 0; l < j; ++l) s -= L[(k + j) + (k + l) * n] * xs[l];
-        xs[j] = s / L[(k + j) + (k + j) * n];
+        const T xj = __shfl(bi, j, 64) / D[j][j];
+        if (lane == j) bi = xj;
+        if (lane > j && lane < kb) bi -= D[lane][j] * xj;
       }
-      for (int j = 0; j < kb; ++j) b[k + j] = xs[j];
+      if (lane < kb) {
+        xs[lane] = bi;
+        b[k + lane] = bi;
+      }
     }
     __syncthreads();
     for (int64_t i = k + kb + threadIdx.x; i < n; i += blockDim.x) {
@@ -1386,17 +1603,18 @@ __global__ __launch_bounds__(256) void k_trsv_lower(const T* __restrict__ L, int
   }
 }
 
+// Backward: L' x = y, in place.
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_upper(const T* __restrict__ L, int64_t n,
                                                     T* __restrict__ b) {
-  __shared__ T xs[kNB];
+  __shared__ T D[kNB][kNB + 1];
   __shared__ T red[4][kNB];
   const int64_t nblk = (n + kNB - 1) / kNB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int64_t bi = nblk - 1; bi >= 0; --bi) {
     const int64_t k = bi * kNB;
     const int kb = (int)min((int64_t)kNB, n - k);
-    // b[k+j] -= sum_{i >= k+kb} L[i, k+j] b[i]: thread i-strided, 64 sums.
+    // b[k+j] -= sum_{i >= k+kb} L[i, k+j] b[i]: thread i-strided, kb sums.
     T acc[kNB];
 #pragma unroll
     for (int j = 0; j < kNB; ++j) acc[j] = T(0);
@@ -1411,14 +1629,20 @@ __global__ __launch_bounds__(256) void k_trsv_upper(const T* __restrict__ L, int
       const T v = wave_sum(acc[j]);
       if (lane == 0) red[wave][j] = v;
     }
+    for (int idx = threadIdx.x; idx < kb * kb; idx += blockDim.x) {
+      const int i = idx % kb, j = idx / kb;
+      D[i][j] = L[(k + i) + (k + j) * n];
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+      T bj = lane < kb ? b[k + lane] - (red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]) : T(0);
+      // Column-oriented back substitution with L' (row j of L' = column j of L).
       for (int j = kb - 1; j >= 0; --j) {
-        T s = b[k + j] - (red[0][j] + red[1][j] + red[2][j] + red[3][j]);
-        for (int l = j + 1; l < kb; ++l) s -= L[(k + l) + (k + j) * n] * xs[l];
-        xs[j] = s / L[(k + j) + (k + j) * n];
+        const T xj = __shfl(bj, j, 64) / D[j][j];
+        if (lane == j) bj = xj;
+        if (lane < j) bj -= D[j][lane] * xj;
       }
-      for (int j = 0; j < kb; ++j) b[k + j] = xs[j];
+      if (lane < kb) b[k + lane] = bj;
     }
     __syncthreads();
   }
@@ -1471,7 +1695,7 @@ __global__ __launch_bounds__(kBlock) void k_plus_cams(int nf, const int32_t* __r
                                                       const double* __restrict__ step,
                                                       const double* __restrict__ scale,
                                                       double* __restrict__ out,
-                                                      double* __restrict__ partial) {
+                                                      double* __restrict__ partial, Win win) {
   __shared__ double lds[kBlock / 64];
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   double acc[1] = {0.0};
@@ -1481,6 +1705,10 @@ __global__ __launch_bounds__(kBlock) void k_plus_cams(int nf, const int32_t* __r
     out[j] = v;
     const double df = cams[j] - v;
     acc[0] = df * df;
+  }
+  if (win.acc) {
+    win_atomic_add(win.acc, 1, 0, i < 9 * (int64_t)nf ? win_of_f(win, (int)(i / 9)) : -1, acc[0]);
+    return;
   }
   block_sum<double, 1>(acc, lds);
   if (threadIdx.x == 0) {

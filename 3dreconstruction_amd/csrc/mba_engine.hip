@@ -324,7 +324,14 @@ using namespace mba;
 
 struct mba_context {
   mba_options opt;
-  Structure st;
+  // Independent problems ("windows") held together; a single problem is
+  // W = 1.  Windows are concatenated: cameras, points, E and F blocks of
+  // window w start at the offsets below (W + 1 prefix sums).
+  int32_t W = 1;
+  std::vector<Structure> wst;
+  std::vector<int32_t> cam_off, pt_off, e_off, f_off;
+  std::vector<int32_t> f_to_cam, cam_to_f;  // F block <-> global camera
+  int32_t max_track = 0;
   int device = 0;
   hipStream_t stream = nullptr;
   Pool pool;
@@ -334,21 +341,30 @@ struct mba_context {
   double *ctab64 = nullptr, *row_obs64 = nullptr, *pscale64 = nullptr, *cscale64 = nullptr;
   double *diag_p = nullptr, *diag_c = nullptr, *red = nullptr, *red2 = nullptr, *stepd = nullptr;
   double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr;
-  Scalars* sc = nullptr;      // device
-  PcgState* pcg = nullptr;    // device
-  int32_t* flag = nullptr;    // device
-  int32_t* info = nullptr;    // device
+  Scalars* sc = nullptr;        // device
+  PcgState* pcg = nullptr;      // device, one per window
+  int32_t* flag = nullptr;      // device, one per window
+  int32_t* info = nullptr;      // device
+  int32_t* all_done = nullptr;  // device: every window's PCG has stopped
+  // Window descriptors (device): window of each device E / F block (W > 1
+  // only), 1 / radius per window, per-window accumulators, accept flags.
+  int32_t *of_e = nullptr, *of_f = nullptr;
+  double *inv_r = nullptr, *wacc = nullptr, *wacc2 = nullptr;
+  uint8_t* accept = nullptr;
+  // Host copies of what was last uploaded (rewritten only while the stream
+  // is idle).
+  std::vector<double> h_inv_r;
+  std::vector<PcgState> h_pcg;
+  std::vector<uint8_t> h_accept;
   Dev<float> df;
   Dev<double> dd;
   bool mixed = false;
   bool scale_ready = false;
-  double last_radius = 0.0;
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  double last_ms[3] = {0, 0, 0};
-  // Device point order (E blocks stably grouped by lane class) and the
-  // point-major launch layout.
-  std::vector<int32_t> dev_e_to_pt;
+  // Device point order (E blocks of all windows stably grouped by lane
+  // class) and the point-major launch layout.
+  std::vector<int32_t> dev_e_to_pt;  // device E block -> global point
   Seg seg;
   int32_t seg_blocks = 0;
 };
@@ -386,8 +402,8 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_st
   MBA_TRY(upload(c, &d.chunk_begin, chunk_begin));
   MBA_TRY(upload(c, &d.chunk_end, chunk_end));
   MBA_TRY(upload(c, &d.f_chunk_start, f_chunk_start));
-  MBA_TRY(upload(c, &d.f_to_cam, c->st.f_to_cam));
-  MBA_TRY(upload(c, &d.cam_to_f, c->st.cam_to_f));
+  MBA_TRY(upload(c, &d.f_to_cam, c->f_to_cam));
+  MBA_TRY(upload(c, &d.cam_to_f, c->cam_to_f));
   const size_t ne = c->ne, nf = c->nf, nc = c->nc;
   MBA_TRY(c->pool.alloc(&d.ctab, kCamTab * nc));
   MBA_TRY(c->pool.alloc(&d.pscale, 3 * ne));
@@ -423,43 +439,101 @@ int chunk_rows_for(int64_t nrows) {
   return (int)((c + 63) / 64 * 64);
 }
 
-mba_status context_create(const mba_problem* p, const mba_options* o, mba_context** out) {
+void context_free(mba_context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int i = 0; i < 4; ++i)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  c->pool.release();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+struct ContextGuard {
+  mba_context* c;
+  ~ContextGuard() { context_free(c); }
+};
+
+std::string window_prefix(int32_t W, int32_t w) {
+  return W > 1 ? "window " + std::to_string(w) + ": " : std::string();
+}
+
+// Preprocess W independent problems into one device layout.
+mba_status context_create(const mba_problem* probs, int32_t W, const mba_options* o, mba_context** out) {
   if (!out) return fail(MBA_ERR_INVALID_ARGUMENT, "out is NULL");
   *out = nullptr;
-  const double t0 = now_s();
+  if (!probs || W < 1) return fail(MBA_ERR_INVALID_ARGUMENT, "no problem given");
   mba_context* c = new mba_context();
-  std::unique_ptr<mba_context> guard(c);
+  ContextGuard guard{c};
   if (o) c->opt = *o; else mba_default_options(&c->opt);
+  c->W = W;
   c->device = c->opt.device;
   c->mixed = (c->opt.precision == MBA_PRECISION_MIXED);
-  MBA_TRY(build_structure(p, &c->st));
-  if (!c->st.standard)
-    return fail(MBA_ERR_UNSUPPORTED, "the Schur ordering did not eliminate exactly the point blocks");
-  for (int32_t q = 0; q < p->num_points; ++q)
-    for (int k = 0; k < 3; ++k)
-      if (!std::isfinite(p->points[3 * (size_t)q + k])) return fail(MBA_ERR_NOT_FINITE, "non-finite point");
-  for (int32_t q = 0; q < p->num_cameras; ++q)
-    for (int k = 0; k < 9; ++k)
-      if (!std::isfinite(p->cameras[9 * (size_t)q + k])) return fail(MBA_ERR_NOT_FINITE, "non-finite camera");
+  c->wst.resize(W);
+  c->cam_off.assign((size_t)W + 1, 0);
+  c->pt_off.assign((size_t)W + 1, 0);
+  c->e_off.assign((size_t)W + 1, 0);
+  c->f_off.assign((size_t)W + 1, 0);
+  int64_t nrows = 0;
+  for (int32_t w = 0; w < W; ++w) {
+    const mba_problem* p = &probs[w];
+    Structure& st = c->wst[w];
+    {
+      const mba_status s = build_structure(p, &st);
+      if (s != MBA_OK) return fail(s, window_prefix(W, w) + g_last_error);
+    }
+    if (!st.standard)
+      return fail(MBA_ERR_UNSUPPORTED,
+                  window_prefix(W, w) + "the Schur ordering did not eliminate exactly the point blocks");
+    for (int32_t q = 0; q < p->num_points; ++q)
+      for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(p->points[3 * (size_t)q + k]))
+          return fail(MBA_ERR_NOT_FINITE, window_prefix(W, w) + "non-finite point");
+    for (int32_t q = 0; q < p->num_cameras; ++q)
+      for (int k = 0; k < 9; ++k)
+        if (!std::isfinite(p->cameras[9 * (size_t)q + k]))
+          return fail(MBA_ERR_NOT_FINITE, window_prefix(W, w) + "non-finite camera");
+    const int64_t nc_tot = (int64_t)c->cam_off[w] + st.nc, np_tot = (int64_t)c->pt_off[w] + st.np;
+    nrows += st.nobs;
+    if (nc_tot > 0x7fffffffLL || np_tot > 0x7fffffffLL || nrows > 0x7fffffffLL)
+      return fail(MBA_ERR_UNSUPPORTED, "more than 2^31-1 cameras, points or observations in one call");
+    c->cam_off[w + 1] = (int32_t)nc_tot;
+    c->pt_off[w + 1] = (int32_t)np_tot;
+    c->e_off[w + 1] = c->e_off[w] + st.num_e;
+    c->f_off[w + 1] = c->f_off[w] + ((int32_t)st.order.size() - st.num_e);
+    c->max_track = std::max(c->max_track, st.max_track);
+  }
   int ndev = 0;
   HIP_TRY(hipGetDeviceCount(&ndev));
   if (c->device < 0 || c->device >= ndev) return fail(MBA_ERR_DEVICE, "no such HIP device");
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  const Structure& st = c->st;
-  c->nc = st.nc;
-  c->np = st.np;
-  c->ne = st.num_e;
-  c->nf = (int32_t)st.order.size() - st.num_e;
-  c->nrows = (int32_t)st.nobs;
-  // Device point order: the reference's E order stably grouped by lane
-  // class (lanes_for), so each point-major launch segment is contiguous.
+  c->nc = c->cam_off[W];
+  c->np = c->pt_off[W];
+  c->ne = c->e_off[W];
+  c->nf = c->f_off[W];
+  c->nrows = (int32_t)nrows;
+  // Global E blocks in window order, each window in its reference E order.
+  std::vector<int32_t> ge_win(c->ne), ge_local(c->ne);
+  for (int32_t w = 0; w < W; ++w)
+    for (int32_t e = 0; e < c->wst[w].num_e; ++e) {
+      ge_win[c->e_off[w] + e] = w;
+      ge_local[c->e_off[w] + e] = e;
+    }
+  auto track_len = [&](int32_t ge) {
+    const Structure& st = c->wst[ge_win[ge]];
+    const int32_t e = ge_local[ge];
+    return st.pt_start[e + 1] - st.pt_start[e];
+  };
+  // Device point order: global E order stably grouped by lane class, so
+  // each point-major launch segment is contiguous.
   std::vector<int32_t> dperm(c->ne), dev_pt_start((size_t)c->ne + 1, 0);
   {
     std::vector<int32_t> cls(c->ne);
     int32_t ncls[8] = {0};
     for (int32_t e = 0; e < c->ne; ++e) {
-      const int t = st.pt_start[e + 1] - st.pt_start[e];
+      const int t = track_len(e);
       int k = 0;
       while (k < 6 && t > 4 * (1 << k)) ++k;
       cls[e] = k;
@@ -492,21 +566,33 @@ mba_status context_create(const mba_problem* p, const mba_options* o, mba_contex
     c->seg_blocks = blk;
     c->dev_e_to_pt.resize(c->ne);
     for (int32_t de = 0; de < c->ne; ++de) {
-      const int32_t e = dperm[de];
-      c->dev_e_to_pt[de] = st.e_to_pt[e];
-      dev_pt_start[de + 1] = dev_pt_start[de] + (st.pt_start[e + 1] - st.pt_start[e]);
+      const int32_t ge = dperm[de], w = ge_win[ge];
+      c->dev_e_to_pt[de] = c->pt_off[w] + c->wst[w].e_to_pt[ge_local[ge]];
+      dev_pt_start[de + 1] = dev_pt_start[de] + track_len(ge);
     }
   }
-  // Rows in device order (each point's rows keep the reference's order).
+  // F blocks: window-contiguous, each window in its reference F order.
+  c->f_to_cam.resize(c->nf);
+  c->cam_to_f.assign(c->nc, -1);
+  for (int32_t w = 0; w < W; ++w)
+    for (size_t f = 0; f < c->wst[w].f_to_cam.size(); ++f) {
+      const int32_t gf = c->f_off[w] + (int32_t)f, cam = c->cam_off[w] + c->wst[w].f_to_cam[f];
+      c->f_to_cam[gf] = cam;
+      c->cam_to_f[cam] = gf;
+    }
+  // Rows in device order (each point's rows keep its window's reference
+  // order); camera indices are global.
   std::vector<int32_t> row_cam(c->nrows), row_e(c->nrows);
   std::vector<double> row_obs(2 * (size_t)c->nrows);
   for (int32_t de = 0; de < c->ne; ++de) {
-    const int32_t e = dperm[de];
+    const int32_t ge = dperm[de], w = ge_win[ge], e = ge_local[ge];
+    const Structure& st = c->wst[w];
+    const mba_problem* p = &probs[w];
     int32_t r = dev_pt_start[de];
     for (int32_t rr = st.pt_start[e]; rr < st.pt_start[e + 1]; ++rr, ++r) {
       const int64_t i = st.res_order[rr];
       row_e[r] = de;
-      row_cam[r] = p->camera_index[i];
+      row_cam[r] = c->cam_off[w] + p->camera_index[i];
       row_obs[2 * (size_t)r] = p->observations[2 * i];
       row_obs[2 * (size_t)r + 1] = p->observations[2 * i + 1];
     }
@@ -514,7 +600,7 @@ mba_status context_create(const mba_problem* p, const mba_options* o, mba_contex
   // Camera-major rows (stable by row order), constant cameras excluded.
   std::vector<int64_t> fstart((size_t)c->nf + 1, 0);
   for (int32_t r = 0; r < c->nrows; ++r) {
-    const int32_t f = st.cam_to_f[row_cam[r]];
+    const int32_t f = c->cam_to_f[row_cam[r]];
     if (f >= 0) fstart[f + 1]++;
   }
   for (int32_t f = 0; f < c->nf; ++f) fstart[f + 1] += fstart[f];
@@ -524,7 +610,7 @@ mba_status context_create(const mba_problem* p, const mba_options* o, mba_contex
   {
     std::vector<int64_t> fill(fstart.begin(), fstart.end() - 1);
     for (int32_t r = 0; r < c->nrows; ++r) {
-      const int32_t f = st.cam_to_f[row_cam[r]];
+      const int32_t f = c->cam_to_f[row_cam[r]];
       if (f < 0) continue;
       const int64_t k = fill[f]++;
       cm_e[k] = row_e[r];
@@ -545,10 +631,16 @@ mba_status context_create(const mba_problem* p, const mba_options* o, mba_contex
   f_chunk_start[c->nf] = (int32_t)chunk_f.size();
   c->nchunks = (int32_t)chunk_f.size();
   // FP64 state.
-  std::vector<double> pts(3 * (size_t)c->ne);
-  for (int32_t e = 0; e < c->ne; ++e)
-    for (int k = 0; k < 3; ++k) pts[3 * (size_t)e + k] = p->points[3 * (size_t)c->dev_e_to_pt[e] + k];
-  std::vector<double> cams(p->cameras, p->cameras + 9 * (size_t)c->nc);
+  std::vector<double> pts(3 * (size_t)c->ne), cams(9 * (size_t)c->nc);
+  for (int32_t de = 0; de < c->ne; ++de) {
+    const int32_t ge = dperm[de], w = ge_win[ge];
+    const int32_t q = c->dev_e_to_pt[de] - c->pt_off[w];
+    for (int k = 0; k < 3; ++k) pts[3 * (size_t)de + k] = probs[w].points[3 * (size_t)q + k];
+  }
+  for (int32_t w = 0; w < W; ++w)
+    if (probs[w].num_cameras > 0)
+      std::memcpy(cams.data() + 9 * (size_t)c->cam_off[w], probs[w].cameras,
+                  72 * (size_t)probs[w].num_cameras);
   MBA_TRY(upload(c, &c->cams64, cams));
   MBA_TRY(upload(c, &c->cams64_t, cams));
   MBA_TRY(upload(c, &c->pts64, pts));
@@ -566,9 +658,23 @@ mba_status context_create(const mba_problem* p, const mba_options* o, mba_contex
   MBA_TRY(c->pool.alloc(&c->red, std::max<size_t>(4 * (size_t)maxb, 4096)));
   MBA_TRY(c->pool.alloc(&c->red2, std::max<size_t>(4 * (size_t)maxb, 4096)));
   MBA_TRY(c->pool.alloc(&c->sc, 4));
-  MBA_TRY(c->pool.alloc(&c->pcg, 1));
-  MBA_TRY(c->pool.alloc(&c->flag, 4));
+  MBA_TRY(c->pool.alloc(&c->pcg, (size_t)W));
+  MBA_TRY(c->pool.alloc(&c->flag, std::max<size_t>(4, (size_t)W)));
   MBA_TRY(c->pool.alloc(&c->info, 4));
+  MBA_TRY(c->pool.alloc(&c->all_done, 4));
+  MBA_TRY(c->pool.alloc(&c->wacc, 4 * (size_t)W));
+  MBA_TRY(c->pool.alloc(&c->wacc2, 2 * (size_t)W));
+  MBA_TRY(c->pool.alloc(&c->accept, (size_t)W));
+  c->h_inv_r.assign(W, 0.0);
+  MBA_TRY(upload(c, &c->inv_r, c->h_inv_r));
+  if (W > 1) {
+    std::vector<int32_t> of_e(c->ne), of_f(c->nf);
+    for (int32_t de = 0; de < c->ne; ++de) of_e[de] = ge_win[dperm[de]];
+    for (int32_t w = 0; w < W; ++w)
+      for (int32_t f = c->f_off[w]; f < c->f_off[w + 1]; ++f) of_f[f] = w;
+    MBA_TRY(upload(c, &c->of_e, of_e));
+    MBA_TRY(upload(c, &c->of_f, of_f));
+  }
   MBA_TRY(c->pool.alloc(&c->b64, 9 * (size_t)c->nf));
   MBA_TRY(c->pool.alloc(&c->x64, 9 * (size_t)c->nf));
   MBA_TRY(c->pool.alloc(&c->r64, 9 * (size_t)c->nf));
@@ -580,20 +686,22 @@ mba_status context_create(const mba_problem* p, const mba_options* o, mba_contex
                                     f_chunk_start));
   for (int i = 0; i < 4; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  (void)t0;
-  *out = guard.release();
+  *out = c;
+  guard.c = nullptr;
   return MBA_OK;
 }
 
-void context_free(mba_context* c) {
-  if (!c) return;
-  (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (int i = 0; i < 4; ++i)
-    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
-  c->pool.release();
-  if (c->stream) (void)hipStreamDestroy(c->stream);
-  delete c;
+// The window descriptor handed to kernels; `acc` is the per-window
+// accumulator array of a reduction (unused for a single problem).
+Win win_desc(mba_context* c, double* acc) {
+  Win w;
+  const bool multi = c->W > 1;
+  w.of_e = multi ? c->of_e : nullptr;
+  w.of_f = multi ? c->of_f : nullptr;
+  w.inv_r = c->inv_r;
+  w.acc = multi ? acc : nullptr;
+  w.flag = c->flag;
+  return w;
 }
 
 // ------------------------------------------------------------ LM passes
@@ -603,151 +711,207 @@ struct Engine {
   Dev<T>& d;
   hipStream_t s;
   T huber;
-  explicit Engine(mba_context* ctx) : c(ctx), d(dev<T>(ctx)), s(ctx->stream), huber((T)ctx->opt.huber_a) {}
+  int W;
+  explicit Engine(mba_context* ctx)
+      : c(ctx), d(dev<T>(ctx)), s(ctx->stream), huber((T)ctx->opt.huber_a), W(ctx->W) {}
 
   mba_status sync() {
     HIP_TRY(hipStreamSynchronize(s));
     return MBA_OK;
   }
-  mba_status read_scalar(const double* dptr, double* out) {
-    HIP_TRY(hipMemcpyAsync(out, dptr, sizeof(double), hipMemcpyDeviceToHost, s));
-    return sync();
-  }
-  mba_status clear_flag() {
-    HIP_TRY(hipMemsetAsync(c->flag, 0, sizeof(int32_t), s));
+  Win win(double* acc = nullptr) { return win_desc(c, acc); }
+
+  mba_status clear_flags() {
+    HIP_TRY(hipMemsetAsync(c->flag, 0, sizeof(int32_t) * W, s));
     return MBA_OK;
   }
-  mba_status read_flag(int32_t* f) {
-    HIP_TRY(hipMemcpyAsync(f, c->flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  mba_status read_flags(std::vector<int32_t>* f) {
+    f->assign(W, 0);
+    HIP_TRY(hipMemcpyAsync(f->data(), c->flag, sizeof(int32_t) * W, hipMemcpyDeviceToHost, s));
     return sync();
   }
 
-  // Kernel-precision tables from the FP64 state.
-  void tables(const double* cams, const double* pts) {
-    (void)pts;  // points are read in FP64 by the point pass
+  // Per-window sums.  A single problem reduces block partials in c->red
+  // with one deterministic k_reduce; several windows add into c->wacc.
+  mba_status begin_sum() {
+    if (W > 1) HIP_TRY(hipMemsetAsync(c->wacc, 0, sizeof(double) * W, s));
+    else HIP_TRY(hipMemsetAsync(c->red, 0, sizeof(double), s));
+    return MBA_OK;
+  }
+  mba_status end_sum(int npartials, std::vector<double>* out) {
+    out->assign(W, 0.0);
+    if (W == 1) {
+      hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, c->red, std::max(npartials, 1), &c->sc->sum);
+      HIP_TRY(hipMemcpyAsync(out->data(), &c->sc->sum, sizeof(double), hipMemcpyDeviceToHost, s));
+    } else {
+      HIP_TRY(hipMemcpyAsync(out->data(), c->wacc, sizeof(double) * W, hipMemcpyDeviceToHost, s));
+    }
+    return sync();
+  }
+
+  // 1 / radius per window (0 for radius 0: no damping).  Uploaded only
+  // when it changes, after the stream has drained.
+  mba_status set_radii(const std::vector<double>& radius) {
+    std::vector<double> inv(W);
+    for (int w = 0; w < W; ++w) inv[w] = radius[w] > 0.0 ? 1.0 / radius[w] : 0.0;
+    if (inv == c->h_inv_r) return MBA_OK;
+    MBA_TRY(sync());
+    c->h_inv_r = inv;
+    HIP_TRY(hipMemcpyAsync(c->inv_r, c->h_inv_r.data(), sizeof(double) * W, hipMemcpyHostToDevice, s));
+    return MBA_OK;
+  }
+  mba_status set_radius(double r) { return set_radii(std::vector<double>(W, r)); }
+
+  // Kernel-precision camera table from the FP64 state.
+  void tables(const double* cams) {
     hipLaunchKernelGGL(k_cam_table<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, d.ctab);
   }
 
-  // FP64 cost at (cams, pts) -> *cost; returns false on a non-finite value.
-  mba_status cost64(const double* cams, const double* pts, double* cost, bool* finite) {
-    MBA_TRY(clear_flag());
+  // FP64 cost per window at (cams, pts); finite[w] = 0 on a non-finite value.
+  mba_status cost64(const double* cams, const double* pts, std::vector<double>* cost, std::vector<uint8_t>* finite) {
+    MBA_TRY(clear_flags());
     hipLaunchKernelGGL(k_cam_table<double>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, c->ctab64);
     const int nb = c->seg_blocks;
-    HIP_TRY(hipMemsetAsync(c->red, 0, sizeof(double), s));
+    MBA_TRY(begin_sum());
     if (nb > 0)
       hipLaunchKernelGGL(k_cost64, dim3(nb), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam, c->row_obs64, pts,
-                         c->ctab64, c->opt.huber_a, c->red, c->flag);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, c->red, std::max(nb, 1), &c->sc->sum);
-    MBA_TRY(read_scalar(&c->sc->sum, cost));
-    int32_t f = 0;
-    MBA_TRY(read_flag(&f));
-    *finite = (f == 0) && std::isfinite(*cost);
+                         c->ctab64, c->opt.huber_a, c->red, win(c->wacc));
+    MBA_TRY(end_sum(nb, cost));
+    std::vector<int32_t> f;
+    MBA_TRY(read_flags(&f));
+    finite->assign(W, 1);
+    for (int w = 0; w < W; ++w) (*finite)[w] = (f[w] == 0) && std::isfinite((*cost)[w]);
     return MBA_OK;
   }
 
-  // Jacobian + Schur pass at the current x (tables must be current).
-  void jacobian_pass(double radius, int recompute_diag) {
-    const double inv_r = radius > 0.0 ? 1.0 / radius : 0.0;
+  // Jacobian + Schur pass at the current x (tables and radii current).
+  void jacobian_pass(int recompute_diag) {
     if (c->timing) (void)hipEventRecord(c->ev[0], s);
     if (c->seg_blocks > 0)
       hipLaunchKernelGGL(k_point_pass<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start,
-                         d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, inv_r,
+                         d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, win(),
                          c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag,
-                         d.V, d.gp, d.Vinv, d.pk, c->flag);
+                         d.V, d.gp, d.Vinv, d.pk);
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
     if (c->nchunks > 0)
       hipLaunchKernelGGL(k_cam_pass<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
                          d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial);
     if (c->timing) (void)hipEventRecord(c->ev[2], s);
     if (c->nf > 0)
-      hipLaunchKernelGGL(k_cam_finalize<T>, dim3(c->nf), dim3(64), 0, s, c->nf,
-                         d.f_chunk_start, d.partial, d.cscale, inv_r, c->opt.min_lm_diagonal, c->opt.max_lm_diagonal,
-                         c->diag_c, recompute_diag, 1, d.U, d.gc, d.rhs, d.Minv, c->flag);
+      hipLaunchKernelGGL(k_cam_finalize<T>, dim3(c->nf), dim3(64), 0, s, c->nf, d.f_chunk_start, d.partial,
+                         d.cscale, win(), c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_c,
+                         recompute_diag, 1, d.U, d.gc, d.rhs, d.Minv);
     if (c->timing) (void)hipEventRecord(c->ev[3], s);
-    c->last_radius = radius;
   }
 
-  // Re-damp after a rejected step and rebuild the reduced rhs.
-  void damp(double radius) {
-    const double inv_r = 1.0 / radius;
-    hipLaunchKernelGGL(k_point_damp<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.V, d.gp,
-                       c->diag_p, inv_r, d.pscale, d.Vinv, d.pk, c->flag);
+  // Re-damp after a rejected step (radii current) and rebuild the rhs.
+  void damp() {
+    hipLaunchKernelGGL(k_point_damp<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.V, d.gp, c->diag_p,
+                       win(), d.pscale, d.Vinv, d.pk);
     if (c->nf > 0) {
-      hipLaunchKernelGGL(k_cam_damp<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf, d.U, c->diag_c,
-                         inv_r, d.Minv, c->flag);
+      hipLaunchKernelGGL(k_cam_damp<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf, d.U, c->diag_c, win(),
+                         d.Minv);
       if (c->nchunks > 0)
         hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
                            d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber,
-                           d.partial, (const PcgState*)nullptr);
+                           d.partial, (const int32_t*)nullptr);
       hipLaunchKernelGGL(k_rhs_finalize<T>, dim3(nblocks(c->nf)), dim3(kBlock), 0, s, c->nf,
                          d.f_chunk_start, d.partial, d.gc, d.rhs);
     }
-    c->last_radius = radius;
   }
 
-  // y = S x (implicit), x and y F-order.  `st` may be null.
-  void matvec(const T* x, T* y, const PcgState* st) {
-    const double inv_r = c->last_radius > 0.0 ? 1.0 / c->last_radius : 0.0;
+  // y = S x (implicit), x and y F-order.  Skipped once `done` is set.
+  void matvec(const T* x, T* y, const int32_t* done) {
     hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
-                       d.ctab, x, (T)1, d.dtab, st);
+                       d.ctab, x, (T)1, d.dtab, done);
     if (c->seg_blocks > 0)
       hipLaunchKernelGGL(k_mv_point<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
-                         d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)0, 1, d.z, d.pk, st);
+                         d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)0, 1, d.z, d.pk, done);
     if (c->nchunks > 0)
       hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
                          d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber,
-                         d.partial, st);
+                         d.partial, done);
     hipLaunchKernelGGL(k_mv_finalize<T>, dim3(nblocks(c->nf)), dim3(kBlock), 0, s, c->nf, d.f_chunk_start,
-                       d.partial, d.U, c->diag_c, inv_r, x, 1, y, st);
+                       d.partial, d.U, c->diag_c, win(), x, 1, y, done);
   }
 
-  // PCG on S x = rhs (x in d.x).  Returns iterations and status.
-  mba_status pcg(int* iters, int* status) {
+  // PCG on S x = rhs (x in d.x) for every window with active[w]; each
+  // window stops on its own tests, the launches stop when all have.
+  mba_status pcg(const std::vector<uint8_t>& active, std::vector<int>* iters, std::vector<int>* status) {
     const int nf = c->nf;
-    const int nb = nblocks(nf);
-    PcgState h;
-    std::memset(&h, 0, sizeof(h));
-    h.iter = 1;
-    h.tol_r = c->opt.pcg_r_tolerance;
-    HIP_TRY(hipMemcpyAsync(c->pcg, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    const int nb = nblocks(nf), nbd = nblocks(9 * (int64_t)nf);
+    MBA_TRY(sync());
+    c->h_pcg.assign(W, PcgState());
+    for (int w = 0; w < W; ++w) {
+      PcgState& h = c->h_pcg[w];
+      std::memset(&h, 0, sizeof(h));
+      h.iter = 1;
+      h.tol_r = c->opt.pcg_r_tolerance;
+      h.done = active[w] ? 0 : 1;
+    }
+    HIP_TRY(hipMemcpyAsync(c->pcg, c->h_pcg.data(), sizeof(PcgState) * W, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(c->all_done, 0, sizeof(int32_t), s));
+    // Windowed accumulators: dot at wacc[0, W), update sums at wacc[W, 4W);
+    // the scalar kernels clear what they read.
+    double* dot_acc = c->W > 1 ? c->wacc : nullptr;
+    double* upd_acc = c->W > 1 ? c->wacc + W : nullptr;
+    if (c->W > 1) HIP_TRY(hipMemsetAsync(c->wacc, 0, sizeof(double) * 4 * W, s));
+    const Win wd = win(dot_acc), wu = win(upd_acc);
     const double eta = c->opt.eta;
     const int minit = c->opt.min_linear_solver_iterations, maxit = c->opt.max_linear_solver_iterations;
-    hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, 3, d.p, d.q, d.rhs, d.tmp,
-                       d.Minv, d.x, d.r, d.zc, c->red);
-    hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(1), 0, s, c->red, nb, c->pcg, eta, minit, maxit, 1);
+    const int nthr = std::min(256, std::max(64, (W + 63) / 64 * 64));
+    hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, wu, 3, d.p, d.q, d.rhs, d.tmp,
+                       d.Minv, d.x, d.r, d.zc, c->red, c->all_done);
+    hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(nthr), 0, s, c->red, nb, W, c->pcg, upd_acc, eta, minit, maxit,
+                       1, c->all_done);
     int check_every = 1;
     for (int it = 1; it <= maxit; ++it) {
-      hipLaunchKernelGGL(k_pcg_p<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, d.zc, d.p);
-      matvec(d.p, d.q, c->pcg);
-      hipLaunchKernelGGL(k_dot_partial<T>, dim3(nblocks(9 * (int64_t)nf)), dim3(kBlock), 0, s,
-                         (int64_t)9 * nf, c->pcg, d.p, d.q, c->red);
-      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(1), 0, s, c->red, nblocks(9 * (int64_t)nf), c->pcg);
+      hipLaunchKernelGGL(k_pcg_p<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, wu, d.zc, d.p, c->all_done);
+      matvec(d.p, d.q, c->all_done);
+      hipLaunchKernelGGL(k_dot_partial<T>, dim3(nbd), dim3(kBlock), 0, s, (int64_t)9 * nf, c->all_done, wd, d.p,
+                         d.q, c->red);
+      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(nthr), 0, s, c->red, nbd, W, c->pcg, dot_acc, c->all_done);
       if (it % 10 == 0) {
-        hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, 1, d.p, d.q, d.rhs,
-                           d.tmp, d.Minv, d.x, d.r, d.zc, c->red);
-        matvec(d.x, d.tmp, c->pcg);
-        hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, 2, d.p, d.q, d.rhs,
-                           d.tmp, d.Minv, d.x, d.r, d.zc, c->red);
+        hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, wu, 1, d.p, d.q, d.rhs,
+                           d.tmp, d.Minv, d.x, d.r, d.zc, c->red, c->all_done);
+        matvec(d.x, d.tmp, c->all_done);
+        hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, wu, 2, d.p, d.q, d.rhs,
+                           d.tmp, d.Minv, d.x, d.r, d.zc, c->red, c->all_done);
       } else {
-        hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, 0, d.p, d.q, d.rhs,
-                           d.tmp, d.Minv, d.x, d.r, d.zc, c->red);
+        hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, wu, 0, d.p, d.q, d.rhs,
+                           d.tmp, d.Minv, d.x, d.r, d.zc, c->red, c->all_done);
       }
-      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(1), 0, s, c->red, nb, c->pcg, eta, minit, maxit, 0);
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(nthr), 0, s, c->red, nb, W, c->pcg, upd_acc, eta, minit,
+                         maxit, 0, c->all_done);
       if (it % check_every == 0 || it == maxit) {
-        HIP_TRY(hipMemcpyAsync(&h, c->pcg, sizeof(h), hipMemcpyDeviceToHost, s));
+        int32_t done = 0;
+        HIP_TRY(hipMemcpyAsync(&done, c->all_done, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         MBA_TRY(sync());
-        if (h.done) break;
+        if (done) break;
         check_every = std::min(8, check_every * 2);
       }
     }
-    HIP_TRY(hipMemcpyAsync(&h, c->pcg, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(c->h_pcg.data(), c->pcg, sizeof(PcgState) * W, hipMemcpyDeviceToHost, s));
     MBA_TRY(sync());
-    *iters = h.iter;
-    *status = h.status;
+    iters->assign(W, 0);
+    status->assign(W, 0);
+    for (int w = 0; w < W; ++w) {
+      if (!active[w]) continue;
+      (*iters)[w] = c->h_pcg[w].iter;
+      (*status)[w] = c->h_pcg[w].status;
+    }
     return MBA_OK;
   }
 
-  // Explicit Schur: assemble dense S, Cholesky, solve, FP64 refinement.
+  void syrk(float* L, int64_t n, int k, int kb, unsigned tiles) {
+    hipLaunchKernelGGL(k_syrk_mfma_f32, dim3(tiles), dim3(256), 0, s, L, n, k, kb);
+  }
+  void syrk(double* L, int64_t n, int k, int kb, unsigned tiles) {
+    hipLaunchKernelGGL(k_syrk_mfma_f64, dim3(tiles), dim3(256), 0, s, L, n, k, kb);
+  }
+
+  // Explicit Schur (single problem): assemble dense S, Cholesky, solve,
+  // FP64 refinement.
   mba_status explicit_solve(bool* ok) {
     const int64_t n = 9 * (int64_t)c->nf;
     *ok = true;
@@ -756,7 +920,7 @@ struct Engine {
       MBA_TRY(c->pool.alloc(&d.S, (size_t)(n * n)));
       MBA_TRY(c->pool.alloc(&d.L, (size_t)(n * n)));
     }
-    const double inv_r = c->last_radius > 0.0 ? 1.0 / c->last_radius : 0.0;
+    const double inv_r = c->h_inv_r[0];
     HIP_TRY(hipMemsetAsync(d.S, 0, sizeof(T) * n * n, s));
     hipLaunchKernelGGL(k_s_diag<T>, dim3(nblocks(81 * (int64_t)c->nf)), dim3(kBlock), 0, s, c->nf, d.U,
                        c->diag_c, inv_r, n, d.S);
@@ -771,8 +935,7 @@ struct Engine {
       if (rest > 0) {
         hipLaunchKernelGGL(k_trsm_panel<T>, dim3((unsigned)((rest + 63) / 64)), dim3(64), 0, s, d.L, n, (int)k, kb);
         const int nt = (int)((rest + kNB - 1) / kNB);
-        hipLaunchKernelGGL(k_syrk_update<T>, dim3((unsigned)((int64_t)nt * (nt + 1) / 2)), dim3(256), 0, s,
-                           d.L, n, (int)k, kb, nt);
+        syrk(d.L, n, (int)k, kb, (unsigned)((int64_t)nt * (nt + 1) / 2));
       }
     }
     int32_t info = 0;
@@ -806,90 +969,104 @@ struct Engine {
   // Back substitution: y_e = Vinv E'(b - F z) into d.z; step = -y.
   void back_substitute() {
     hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
-                       d.ctab, d.x, (T)-1, d.dtab, (const PcgState*)nullptr);
+                       d.ctab, d.x, (T)-1, d.dtab, (const int32_t*)nullptr);
     if (c->seg_blocks > 0)
       hipLaunchKernelGGL(k_mv_point<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
                          d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)1, 1, d.z, (T*)nullptr,
-                         (const PcgState*)nullptr);
+                         (const int32_t*)nullptr);
     hipLaunchKernelGGL(k_negate<T>, dim3(nblocks(3 * (int64_t)c->ne)), dim3(kBlock), 0, s,
                        (int64_t)3 * c->ne, d.z, d.step_p);
     hipLaunchKernelGGL(k_negate<T>, dim3(nblocks(9 * (int64_t)c->nf)), dim3(kBlock), 0, s,
                        (int64_t)9 * c->nf, d.x, d.step_c);
   }
 
-  // Model cost change -J step . (r + J step / 2).
-  mba_status model_cost_change(double* mcc) {
+  // Model cost change -J step . (r + J step / 2) per window.
+  mba_status model_cost_change(std::vector<double>* mcc) {
     hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
-                       d.ctab, d.step_c, (T)1, d.dtab, (const PcgState*)nullptr);
+                       d.ctab, d.step_c, (T)1, d.dtab, (const int32_t*)nullptr);
     const int nb = c->seg_blocks;
-    HIP_TRY(hipMemsetAsync(c->red, 0, sizeof(double), s));
+    MBA_TRY(begin_sum());
     if (nb > 0)
       hipLaunchKernelGGL(k_model_cost<T>, dim3(nb), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam, d.row_obs,
-                         d.pk, d.pscale, d.ctab, d.dtab, d.step_p, huber, c->red);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, c->red, std::max(nb, 1), &c->sc->sum);
-    double v = 0.0;
-    MBA_TRY(read_scalar(&c->sc->sum, &v));
-    *mcc = -v;
+                         d.pk, d.pscale, d.ctab, d.dtab, d.step_p, huber, c->red, win(c->wacc));
+    MBA_TRY(end_sum(nb, mcc));
+    for (double& v : *mcc) v = -v;
     return MBA_OK;
   }
 
-  // Step finite check + x_trial = x + step .* scale, returns |delta|.
-  mba_status plus(double* step_norm, bool* finite) {
+  // x_trial = x + step .* scale, per window |delta| and its finiteness.
+  mba_status plus(std::vector<double>* step_norm, std::vector<uint8_t>* finite) {
     const int64_t np3 = 3 * (int64_t)c->ne, nf9 = 9 * (int64_t)c->nf;
     hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(np3)), dim3(kBlock), 0, s, np3, d.step_p, c->stepd);
     hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(nf9)), dim3(kBlock), 0, s, nf9, d.step_c,
                        c->stepd + np3);
     const int nb1 = nblocks(np3), nb2 = nblocks(nf9);
+    MBA_TRY(begin_sum());
     hipLaunchKernelGGL(k_plus, dim3(nb1), dim3(kBlock), 0, s, np3, c->pts64, c->stepd, c->pscale64, c->pts64_t,
-                       c->red);
+                       c->red, win(c->wacc));
     hipLaunchKernelGGL(k_plus_cams, dim3(nb2), dim3(kBlock), 0, s, c->nf, d.f_to_cam, c->cams64,
-                       c->stepd + np3, c->cscale64, c->cams64_t, c->red + nb1);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, c->red, nb1 + nb2, &c->sc->sum);
-    double v = 0.0;
-    MBA_TRY(read_scalar(&c->sc->sum, &v));
-    *step_norm = std::sqrt(v);
-    *finite = std::isfinite(v);
+                       c->stepd + np3, c->cscale64, c->cams64_t, c->red + nb1, win(c->wacc));
+    std::vector<double> v;
+    MBA_TRY(end_sum(nb1 + nb2, &v));
+    step_norm->assign(W, 0.0);
+    finite->assign(W, 1);
+    for (int w = 0; w < W; ++w) {
+      (*step_norm)[w] = std::sqrt(v[w]);
+      (*finite)[w] = std::isfinite(v[w]);
+    }
     return MBA_OK;
   }
 
-  mba_status x_norm(double* out) {
+  mba_status x_norm(std::vector<double>* out) {
     const int64_t np3 = 3 * (int64_t)c->ne, nf9 = 9 * (int64_t)c->nf;
     hipLaunchKernelGGL(k_gather_cams, dim3(nblocks(nf9)), dim3(kBlock), 0, s, c->nf, d.f_to_cam, c->cams64,
                        c->stepd + np3);
     hipLaunchKernelGGL(k_copy64, dim3(nblocks(np3)), dim3(kBlock), 0, s, np3, c->pts64, c->stepd);
     const int nb = nblocks(np3 + nf9);
-    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kBlock), 0, s, np3 + nf9, c->stepd, c->red);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, c->red, nb, &c->sc->sum);
-    double v = 0.0;
-    MBA_TRY(read_scalar(&c->sc->sum, &v));
-    *out = std::sqrt(v);
+    MBA_TRY(begin_sum());
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kBlock), 0, s, np3 + nf9, np3, c->stepd, c->red, win(c->wacc));
+    MBA_TRY(end_sum(nb, out));
+    for (double& v : *out) v = std::sqrt(v);
     return MBA_OK;
   }
 
-  // Max |g| and |g| of the unscaled gradient.
-  mba_status gradient_norms(double* gmax, double* gnorm) {
-    double init[2] = {0.0, 0.0};
-    HIP_TRY(hipMemcpyAsync(c->red2, init, sizeof(init), hipMemcpyHostToDevice, s));
+  // Max |g| and |g| of the unscaled gradient, per window.
+  mba_status gradient_norms(std::vector<double>* gmax, std::vector<double>* gnorm) {
     const int64_t np3 = 3 * (int64_t)c->ne, nf9 = 9 * (int64_t)c->nf;
-    int nb = std::min(1024, nblocks(np3));
-    hipLaunchKernelGGL(k_grad_partial<T>, dim3(nb), dim3(kBlock), 0, s, np3, d.gp, c->pscale64, c->red,
-                       c->red + 2048);
-    hipLaunchKernelGGL(k_grad_final, dim3(1), dim3(1), 0, s, c->red, c->red + 2048, nb, c->red2);
-    nb = std::min(1024, nblocks(nf9));
-    if (nf9 > 0) {
-      hipLaunchKernelGGL(k_grad_partial<T>, dim3(nb), dim3(kBlock), 0, s, nf9, d.gc, c->cscale64, c->red,
+    std::vector<double> v(2 * (size_t)W, 0.0);
+    if (W == 1) {
+      HIP_TRY(hipMemsetAsync(c->red2, 0, 2 * sizeof(double), s));
+      int nb = std::min(1024, nblocks(np3));
+      hipLaunchKernelGGL(k_grad_partial<T>, dim3(nb), dim3(kBlock), 0, s, np3, d.gp, c->pscale64, c->red,
                          c->red + 2048);
       hipLaunchKernelGGL(k_grad_final, dim3(1), dim3(1), 0, s, c->red, c->red + 2048, nb, c->red2);
+      nb = std::min(1024, nblocks(nf9));
+      if (nf9 > 0) {
+        hipLaunchKernelGGL(k_grad_partial<T>, dim3(nb), dim3(kBlock), 0, s, nf9, d.gc, c->cscale64, c->red,
+                           c->red + 2048);
+        hipLaunchKernelGGL(k_grad_final, dim3(1), dim3(1), 0, s, c->red, c->red + 2048, nb, c->red2);
+      }
+      HIP_TRY(hipMemcpyAsync(v.data(), c->red2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    } else {
+      HIP_TRY(hipMemsetAsync(c->wacc2, 0, 2 * sizeof(double) * W, s));
+      hipLaunchKernelGGL(k_grad_win<T>, dim3(nblocks(np3)), dim3(kBlock), 0, s, np3, 3, 0, d.gp, c->pscale64, win(),
+                         c->wacc2);
+      hipLaunchKernelGGL(k_grad_win<T>, dim3(nblocks(nf9)), dim3(kBlock), 0, s, nf9, 9, 1, d.gc, c->cscale64, win(),
+                         c->wacc2);
+      HIP_TRY(hipMemcpyAsync(v.data(), c->wacc2, 2 * sizeof(double) * W, hipMemcpyDeviceToHost, s));
     }
-    double v[2];
-    HIP_TRY(hipMemcpyAsync(v, c->red2, sizeof(v), hipMemcpyDeviceToHost, s));
     MBA_TRY(sync());
-    *gmax = v[0];
-    *gnorm = std::sqrt(v[1]);
+    gmax->assign(W, 0.0);
+    gnorm->assign(W, 0.0);
+    for (int w = 0; w < W; ++w) {
+      (*gmax)[w] = v[2 * (size_t)w];
+      (*gnorm)[w] = std::sqrt(v[2 * (size_t)w + 1]);
+    }
     return MBA_OK;
   }
 
-  // EstimateScale (trust_region_minimizer.cc:103-109) from an unscaled pass.
+  // EstimateScale (trust_region_minimizer.cc:103-109) from an unscaled,
+  // undamped pass.
   mba_status init_scale(bool jacobi) {
     hipLaunchKernelGGL(k_fill64, dim3(nblocks(3 * (int64_t)c->ne)), dim3(kBlock), 0, s, (int64_t)3 * c->ne,
                        1.0, c->pscale64);
@@ -900,7 +1077,8 @@ struct Engine {
       c->scale_ready = true;
       return MBA_OK;
     }
-    jacobian_pass(0.0, 1);
+    MBA_TRY(set_radius(0.0));
+    jacobian_pass(1);
     hipLaunchKernelGGL(k_make_scale<T>, dim3(nblocks(std::max(c->ne, c->nf))), dim3(kBlock), 0, s, c->ne,
                        c->nf, d.V, d.U, c->pscale64, c->cscale64);
     sync_scale_T();
@@ -914,192 +1092,306 @@ struct Engine {
                        (int64_t)9 * c->nf, c->cscale64, d.cscale);
   }
 
-  // Solve the LM system at the current radius; fills step_p / step_c.
-  mba_status compute_step(bool* solved, int* lin_iters) {
-    *solved = true;
-    *lin_iters = 0;
+  // Solve the LM systems at the current radii; fills step_p / step_c.
+  mba_status compute_step(const std::vector<uint8_t>& active, std::vector<uint8_t>* solved, std::vector<int>* lin) {
+    solved->assign(W, 1);
+    lin->assign(W, 0);
     if (c->opt.linear_solver == MBA_IMPLICIT_SCHUR_PCG) {
-      int status = 0;
       if (c->nf > 0) {
-        MBA_TRY(pcg(lin_iters, &status));
+        std::vector<int> status;
+        MBA_TRY(pcg(active, lin, &status));
+        for (int w = 0; w < W; ++w)
+          if (status[w] < 0) (*solved)[w] = 0;
       } else {
         HIP_TRY(hipMemsetAsync(d.x, 0, sizeof(T) * 9, s));
       }
-      if (status < 0) *solved = false;
     } else {
       bool ok = true;
       MBA_TRY(explicit_solve(&ok));
-      if (!ok) *solved = false;
+      if (!ok) (*solved)[0] = 0;
     }
-    if (*solved) back_substitute();
+    bool any = false;
+    for (int w = 0; w < W; ++w) any = any || (active[w] && (*solved)[w]);
+    if (any) back_substitute();
     return MBA_OK;
   }
 
-  // TrustRegionMinimizer::Minimize + LevenbergMarquardtStrategy.
-  mba_status solve(mba_summary* sum, mba_iteration* its, int32_t max_its) {
-    const mba_options& o = c->opt;
-    const double t_start = now_s();
-    std::memset(sum, 0, sizeof(*sum));
-    sum->num_e_blocks = c->ne;
-    sum->num_f_blocks = c->nf;
-    sum->termination_type = MBA_NO_CONVERGENCE;
+  // Make the trial point current for the windows with accept[w].
+  mba_status accept(const std::vector<uint8_t>& acc) {
+    if (W == 1) {
+      std::swap(c->cams64, c->cams64_t);
+      std::swap(c->pts64, c->pts64_t);
+      // Keep the constant cameras of the trial copy in sync.
+      HIP_TRY(hipMemcpyAsync(c->cams64_t, c->cams64, sizeof(double) * 9 * c->nc, hipMemcpyDeviceToDevice, s));
+      return MBA_OK;
+    }
+    MBA_TRY(sync());
+    c->h_accept = acc;
+    HIP_TRY(hipMemcpyAsync(c->accept, c->h_accept.data(), W, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_accept_points, dim3(nblocks(3 * (int64_t)c->ne)), dim3(kBlock), 0, s, c->ne, win(),
+                       c->accept, c->pts64_t, c->pts64);
+    hipLaunchKernelGGL(k_accept_cams, dim3(nblocks(9 * (int64_t)c->nf)), dim3(kBlock), 0, s, c->nf, win(),
+                       d.f_to_cam, c->accept, c->cams64_t, c->cams64);
+    return MBA_OK;
+  }
+
+  // Per-window trust-region state.
+  struct WinLM {
+    double radius = 0.0, decrease_factor = 2.0, cost = 0.0, x_norm = 0.0;
+    int num_invalid = 0;
+    bool active = true;
     int nrec = 0;
     double rec_min = std::numeric_limits<double>::max();
-    auto push = [&](const mba_iteration& it) {
-      if (its && nrec < max_its) its[nrec] = it;
-      ++nrec;
-      rec_min = std::min(rec_min, it.cost);
-    };
-    double radius = o.initial_trust_region_radius, decrease_factor = 2.0;
-    double t_jac = 0.0, t_res = 0.0, t_lin = 0.0;
     mba_iteration it;
-    std::memset(&it, 0, sizeof(it));
-    it.trust_region_radius = radius;
-    double cost = 0.0;
-    bool finite = true;
+  };
+
+  // TrustRegionMinimizer::Minimize + LevenbergMarquardtStrategy, run for
+  // every window at once: each window keeps its own radius, decrease
+  // factor, cost, records and termination; the passes cover all windows.
+  // `its` (W == 1 only) receives the iteration records.
+  mba_status solve(mba_summary* sums, mba_iteration* its, int32_t max_its) {
+    const mba_options& o = c->opt;
+    const double t_start = now_s();
+    std::vector<WinLM> L(W);
+    for (int w = 0; w < W; ++w) {
+      std::memset(&sums[w], 0, sizeof(mba_summary));
+      sums[w].num_e_blocks = c->e_off[w + 1] - c->e_off[w];
+      sums[w].num_f_blocks = c->f_off[w + 1] - c->f_off[w];
+      sums[w].termination_type = MBA_NO_CONVERGENCE;
+      std::memset(&L[w].it, 0, sizeof(mba_iteration));
+      L[w].radius = o.initial_trust_region_radius;
+      L[w].it.trust_region_radius = L[w].radius;
+    }
+    auto push = [&](int w) {
+      WinLM& l = L[w];
+      if (W == 1 && its && l.nrec < max_its) its[l.nrec] = l.it;
+      ++l.nrec;
+      l.rec_min = std::min(l.rec_min, l.it.cost);
+    };
+    auto finish = [&](int w, int32_t term) {
+      sums[w].termination_type = term;
+      L[w].active = false;
+    };
+    auto any_active = [&]() {
+      for (const WinLM& l : L)
+        if (l.active) return true;
+      return false;
+    };
+    auto radii = [&]() {
+      std::vector<double> r(W);
+      for (int w = 0; w < W; ++w) r[w] = L[w].radius;
+      return r;
+    };
+    double t_jac = 0.0, t_res = 0.0, t_lin = 0.0;
+    std::vector<double> cost, gmax, gnorm, xn;
+    std::vector<uint8_t> fin;
     double t0 = now_s();
-    MBA_TRY(cost64(c->cams64, c->pts64, &cost, &finite));
+    MBA_TRY(cost64(c->cams64, c->pts64, &cost, &fin));
     t_res += now_s() - t0;
-    int num_invalid = 0;
-    bool have_jac = false;
-    if (!finite) {
-      sum->termination_type = MBA_FAILURE;
-    } else {
-      sum->initial_cost = cost;
-      it.cost = cost;
-      t0 = now_s();
-      tables(c->cams64, c->pts64);
-      MBA_TRY(init_scale(o.jacobi_scaling != 0));
-      MBA_TRY(clear_flag());
-      jacobian_pass(radius, 1);
-      int32_t fl = 0;
-      MBA_TRY(read_flag(&fl));
-      t_jac += now_s() - t0;
-      have_jac = true;
-      MBA_TRY(gradient_norms(&it.gradient_max_norm, &it.gradient_norm));
-      if (it.gradient_max_norm <= o.gradient_tolerance) {
-        sum->termination_type = MBA_CONVERGENCE;
-        it.cumulative_time_s = now_s() - t_start;
-        push(it);
-        goto done;
+    for (int w = 0; w < W; ++w) {
+      if (!fin[w]) {
+        finish(w, MBA_FAILURE);
+        continue;
       }
-      it.cumulative_time_s = now_s() - t_start;
-      push(it);
-      double x_norm = 0.0;
-      MBA_TRY(x_norm_fn(&x_norm));
-      for (;;) {
-        if (it.iteration >= o.max_num_iterations) {
-          sum->termination_type = MBA_NO_CONVERGENCE;
-          break;
-        }
-        bool solved = true;
-        int lin = 0;
-        t0 = now_s();
-        MBA_TRY(clear_flag());
-        MBA_TRY(compute_step(&solved, &lin));
-        int32_t fl2 = 0;
-        MBA_TRY(read_flag(&fl2));
-        if (fl2) solved = false;
-        t_lin += now_s() - t0;
-        sum->num_linear_solver_iterations += lin;
+      sums[w].initial_cost = cost[w];
+      L[w].cost = cost[w];
+      L[w].it.cost = cost[w];
+    }
+    if (any_active()) {
+      t0 = now_s();
+      tables(c->cams64);
+      MBA_TRY(init_scale(o.jacobi_scaling != 0));
+      MBA_TRY(set_radii(radii()));
+      MBA_TRY(clear_flags());
+      jacobian_pass(1);
+      MBA_TRY(sync());
+      t_jac += now_s() - t0;
+      MBA_TRY(gradient_norms(&gmax, &gnorm));
+      for (int w = 0; w < W; ++w) {
+        if (!L[w].active) continue;
+        L[w].it.gradient_max_norm = gmax[w];
+        L[w].it.gradient_norm = gnorm[w];
+        L[w].it.cumulative_time_s = now_s() - t_start;
+        push(w);
+        if (gmax[w] <= o.gradient_tolerance) finish(w, MBA_CONVERGENCE);
+      }
+      MBA_TRY(x_norm(&xn));
+      for (int w = 0; w < W; ++w) L[w].x_norm = xn[w];
+    }
+    std::vector<uint8_t> act(W), solved, stepping(W), acc(W), sf, cf;
+    std::vector<int> lin;
+    std::vector<int32_t> fl;
+    std::vector<double> mcc(W, 0.0), pgm(W), pgn(W), sn, new_cost;
+    while (any_active()) {
+      for (int w = 0; w < W; ++w)
+        if (L[w].active && L[w].it.iteration >= o.max_num_iterations) finish(w, MBA_NO_CONVERGENCE);
+      if (!any_active()) break;
+      for (int w = 0; w < W; ++w) act[w] = L[w].active;
+      t0 = now_s();
+      MBA_TRY(clear_flags());
+      MBA_TRY(compute_step(act, &solved, &lin));
+      MBA_TRY(read_flags(&fl));
+      t_lin += now_s() - t0;
+      bool any_solved = false;
+      for (int w = 0; w < W; ++w) {
+        stepping[w] = 0;
+        acc[w] = 0;
+        if (!act[w]) continue;
+        if (fl[w]) solved[w] = 0;
+        sums[w].num_linear_solver_iterations += lin[w];
+        mba_iteration& it = L[w].it;
         const int prev = it.iteration;
-        const double pgm = it.gradient_max_norm, pgn = it.gradient_norm;
+        pgm[w] = it.gradient_max_norm;
+        pgn[w] = it.gradient_norm;
         std::memset(&it, 0, sizeof(it));
         it.iteration = prev + 1;
-        it.linear_solver_iterations = lin;
-        double mcc = 0.0;
-        if (solved) {
-          MBA_TRY(model_cost_change(&mcc));
-          it.step_is_valid = std::isfinite(mcc) && mcc >= 0.0;
-        }
+        it.linear_solver_iterations = lin[w];
+        any_solved = any_solved || solved[w];
+      }
+      if (any_solved) MBA_TRY(model_cost_change(&mcc));
+      bool any_valid = false;
+      for (int w = 0; w < W; ++w) {
+        if (!act[w]) continue;
+        mba_iteration& it = L[w].it;
+        it.step_is_valid = solved[w] && std::isfinite(mcc[w]) && mcc[w] >= 0.0;
+        stepping[w] = 1;
         if (!it.step_is_valid) {
-          if (++num_invalid >= o.max_num_consecutive_invalid_steps) {
-            sum->termination_type = MBA_FAILURE;
-            break;
+          if (++L[w].num_invalid >= o.max_num_consecutive_invalid_steps) {
+            finish(w, MBA_FAILURE);
+            stepping[w] = 0;
+            continue;
           }
-          it.cost = cost;
-          it.gradient_max_norm = pgm;
-          it.gradient_norm = pgn;
+          it.cost = L[w].cost;
+          it.gradient_max_norm = pgm[w];
+          it.gradient_norm = pgn[w];
         } else {
-          num_invalid = 0;
-          double step_norm = 0.0;
-          bool sf = true;
-          MBA_TRY(plus(&step_norm, &sf));
-          double new_cost = std::numeric_limits<double>::max();
-          bool cf = true;
-          t0 = now_s();
-          MBA_TRY(cost64(c->cams64_t, c->pts64_t, &new_cost, &cf));
-          t_res += now_s() - t0;
-          if (!cf || !sf) new_cost = std::numeric_limits<double>::max();
-          it.step_norm = step_norm;
-          if (step_norm <= o.parameter_tolerance * (x_norm + o.parameter_tolerance)) {
-            sum->termination_type = MBA_CONVERGENCE;
-            break;
+          L[w].num_invalid = 0;
+          any_valid = true;
+        }
+      }
+      if (any_valid) {
+        MBA_TRY(plus(&sn, &sf));
+        t0 = now_s();
+        MBA_TRY(cost64(c->cams64_t, c->pts64_t, &new_cost, &cf));
+        t_res += now_s() - t0;
+      }
+      bool any_acc = false, any_step = false;
+      for (int w = 0; w < W; ++w) {
+        if (!stepping[w]) continue;
+        WinLM& l = L[w];
+        mba_iteration& it = l.it;
+        if (it.step_is_valid) {
+          const double nc_w = (cf[w] && sf[w]) ? new_cost[w] : std::numeric_limits<double>::max();
+          it.step_norm = sn[w];
+          if (sn[w] <= o.parameter_tolerance * (l.x_norm + o.parameter_tolerance)) {
+            finish(w, MBA_CONVERGENCE);
+            stepping[w] = 0;
+            continue;
           }
-          it.cost_change = cost - new_cost;
-          if (std::fabs(it.cost_change) <= o.function_tolerance * cost) {
-            sum->termination_type = MBA_CONVERGENCE;
-            break;
+          it.cost_change = l.cost - nc_w;
+          if (std::fabs(it.cost_change) <= o.function_tolerance * l.cost) {
+            finish(w, MBA_CONVERGENCE);
+            stepping[w] = 0;
+            continue;
           }
-          it.relative_decrease = it.cost_change / mcc;
+          it.relative_decrease = it.cost_change / mcc[w];
           it.step_is_successful = it.relative_decrease > o.min_relative_decrease;
           if (it.step_is_successful) {
-            // Accept: radius update first (StepAccepted), then evaluate.
-            radius = radius / std::max(1.0 / 3.0, 1.0 - std::pow(2.0 * it.relative_decrease - 1.0, 3));
-            radius = std::min(o.max_trust_region_radius, radius);
-            decrease_factor = 2.0;
-            std::swap(c->cams64, c->cams64_t);
-            std::swap(c->pts64, c->pts64_t);
-            // Keep the constant cameras of the trial copy in sync.
-            HIP_TRY(hipMemcpyAsync(c->cams64_t, c->cams64, sizeof(double) * 9 * c->nc, hipMemcpyDeviceToDevice, s));
-            cost = new_cost;
-            MBA_TRY(x_norm_fn(&x_norm));
-            t0 = now_s();
-            tables(c->cams64, c->pts64);
-            jacobian_pass(radius, 1);
-            MBA_TRY(sync());
-            t_jac += now_s() - t0;
-            MBA_TRY(gradient_norms(&it.gradient_max_norm, &it.gradient_norm));
+            // StepAccepted: radius update, then the new point is current.
+            l.radius = l.radius / std::max(1.0 / 3.0, 1.0 - std::pow(2.0 * it.relative_decrease - 1.0, 3));
+            l.radius = std::min(o.max_trust_region_radius, l.radius);
+            l.decrease_factor = 2.0;
+            l.cost = nc_w;
+            acc[w] = 1;
+            any_acc = true;
           }
         }
         if (it.step_is_successful) {
-          ++sum->num_successful_steps;
+          ++sums[w].num_successful_steps;
         } else {
-          ++sum->num_unsuccessful_steps;
-          radius = radius / decrease_factor;  // StepRejected / StepIsInvalid
-          decrease_factor *= 2.0;
-          damp(radius);
+          ++sums[w].num_unsuccessful_steps;
+          l.radius = l.radius / l.decrease_factor;  // StepRejected / StepIsInvalid
+          l.decrease_factor *= 2.0;
         }
-        it.cost = cost;
-        it.trust_region_radius = radius;
-        it.cumulative_time_s = now_s() - t_start;
-        push(it);
-        if (it.step_is_successful) {
-          if (it.gradient_max_norm <= o.gradient_tolerance) {
-            sum->termination_type = MBA_CONVERGENCE;
-            break;
+        any_step = true;
+      }
+      if (any_acc) {
+        MBA_TRY(accept(acc));
+        MBA_TRY(x_norm(&xn));
+        for (int w = 0; w < W; ++w)
+          if (acc[w]) L[w].x_norm = xn[w];
+        t0 = now_s();
+        tables(c->cams64);
+        MBA_TRY(set_radii(radii()));
+        // A full pass also re-damps the windows whose step was rejected
+        // (their point is unchanged).
+        jacobian_pass(1);
+        MBA_TRY(sync());
+        t_jac += now_s() - t0;
+        MBA_TRY(gradient_norms(&gmax, &gnorm));
+        for (int w = 0; w < W; ++w)
+          if (acc[w]) {
+            L[w].it.gradient_max_norm = gmax[w];
+            L[w].it.gradient_norm = gnorm[w];
           }
-        } else if (radius < o.min_trust_region_radius) {
-          sum->termination_type = MBA_CONVERGENCE;
-          break;
+      } else if (any_step) {
+        MBA_TRY(set_radii(radii()));
+        damp();
+      }
+      for (int w = 0; w < W; ++w) {
+        if (!stepping[w]) continue;
+        WinLM& l = L[w];
+        l.it.cost = l.cost;
+        l.it.trust_region_radius = l.radius;
+        l.it.cumulative_time_s = now_s() - t_start;
+        push(w);
+        if (l.it.step_is_successful) {
+          if (l.it.gradient_max_norm <= o.gradient_tolerance) finish(w, MBA_CONVERGENCE);
+        } else if (l.radius < o.min_trust_region_radius) {
+          finish(w, MBA_CONVERGENCE);
         }
       }
     }
-  done:
-    (void)have_jac;
-    sum->final_cost = std::min(sum->initial_cost, rec_min);
-    if (sum->termination_type == MBA_FAILURE && nrec == 0) sum->final_cost = sum->initial_cost;
-    sum->num_iterations = nrec;
-    sum->is_solution_usable = sum->termination_type != MBA_FAILURE;
-    sum->jacobian_evaluation_time_s = t_jac;
-    sum->residual_evaluation_time_s = t_res;
-    sum->linear_solver_time_s = t_lin;
-    sum->total_time_s = now_s() - t_start;
+    const double total = now_s() - t_start;
+    for (int w = 0; w < W; ++w) {
+      mba_summary& sm = sums[w];
+      sm.final_cost = std::min(sm.initial_cost, L[w].rec_min);
+      if (sm.termination_type == MBA_FAILURE && L[w].nrec == 0) sm.final_cost = sm.initial_cost;
+      sm.num_iterations = L[w].nrec;
+      sm.is_solution_usable = sm.termination_type != MBA_FAILURE;
+      sm.jacobian_evaluation_time_s = t_jac;
+      sm.residual_evaluation_time_s = t_res;
+      sm.linear_solver_time_s = t_lin;
+      sm.total_time_s = total;
+    }
     return sync();
   }
-
-  mba_status x_norm_fn(double* out) { return x_norm(out); }
 };
+
+// Copy the current parameters of every window back to its problem (all
+// windows, or those whose summary says the solution is usable).
+mba_status download(mba_context* c, mba_problem* probs, const mba_summary* sums) {
+  std::vector<double> cams(9 * (size_t)c->nc), pts(3 * (size_t)c->ne);
+  if (!cams.empty()) HIP_TRY(hipMemcpyAsync(cams.data(), c->cams64, cams.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  if (!pts.empty()) HIP_TRY(hipMemcpyAsync(pts.data(), c->pts64, pts.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  auto wanted = [&](int32_t w) { return !sums || sums[w].is_solution_usable; };
+  int32_t w = 0;
+  for (int32_t f = 0; f < c->nf; ++f) {
+    while (f >= c->f_off[w + 1]) ++w;
+    if (!wanted(w)) continue;
+    const int32_t cam = c->f_to_cam[f];
+    std::memcpy(probs[w].cameras + 9 * (size_t)(cam - c->cam_off[w]), cams.data() + 9 * (size_t)cam, 72);
+  }
+  // Window of each global point by its offset.
+  for (int32_t e = 0; e < c->ne; ++e) {
+    const int32_t q = c->dev_e_to_pt[e];
+    const int32_t wq = (int32_t)(std::upper_bound(c->pt_off.begin(), c->pt_off.end(), q) - c->pt_off.begin()) - 1;
+    if (!wanted(wq)) continue;
+    std::memcpy(probs[wq].points + 3 * (size_t)(q - c->pt_off[wq]), pts.data() + 3 * (size_t)e, 24);
+  }
+  return MBA_OK;
+}
 
 }  // namespace
 }  // namespace mba
@@ -1151,7 +1443,7 @@ int32_t mba_abi_version(void) { return MBA_ABI_VERSION; }
 
 mba_status mba_context_create(const mba_problem* problem, const mba_options* options, mba_context** out) {
   g_last_error.clear();
-  return context_create(problem, options, out);
+  return context_create(problem, 1, options, out);
 }
 
 mba_status mba_context_destroy(mba_context* ctx) {
@@ -1192,11 +1484,12 @@ mba_status mba_context_jacobian_schur(mba_context* ctx, double radius) {
   HIP_TRY(hipSetDevice(ctx->device));
   auto run = [&](auto& eng) -> mba_status {
     if (!ctx->scale_ready) {
-      eng.tables(ctx->cams64, ctx->pts64);
+      eng.tables(ctx->cams64);
       MBA_TRY(eng.init_scale(ctx->opt.jacobi_scaling != 0));
     }
-    eng.tables(ctx->cams64, ctx->pts64);
-    eng.jacobian_pass(radius, 1);
+    eng.tables(ctx->cams64);
+    MBA_TRY(eng.set_radius(radius));
+    eng.jacobian_pass(1);
     return MBA_OK;
   };
   if (ctx->mixed) {
@@ -1212,7 +1505,6 @@ mba_status mba_context_read_scale(mba_context* ctx, double* cam_scale, double* p
 mba_status mba_context_read_schur(mba_context* ctx, double* U, double* gc, double* rhs, double* V, double* gp) {
   if (!ctx) return fail(MBA_ERR_INVALID_ARGUMENT, "ctx is NULL");
   HIP_TRY(hipSetDevice(ctx->device));
-  const Structure& st = ctx->st;
   auto rd = [&](auto* src, size_t n, std::vector<double>* out) -> mba_status {
     using X = std::remove_cv_t<std::remove_pointer_t<decltype(src)>>;
     std::vector<X> h(n);
@@ -1229,7 +1521,7 @@ mba_status mba_context_read_schur(mba_context* ctx, double* U, double* gc, doubl
     MBA_TRY(rd(d.V, 6 * (size_t)ctx->ne, &hV));
     MBA_TRY(rd(d.gp, 3 * (size_t)ctx->ne, &hgp));
     for (int32_t f = 0; f < ctx->nf; ++f) {
-      const int32_t cam = st.f_to_cam[f];
+      const int32_t cam = ctx->f_to_cam[f];
       if (U) std::memcpy(U + 45 * (size_t)cam, hU.data() + 45 * (size_t)f, 45 * sizeof(double));
       if (gc) std::memcpy(gc + 9 * (size_t)cam, hg.data() + 9 * (size_t)f, 9 * sizeof(double));
       if (rhs) std::memcpy(rhs + 9 * (size_t)cam, hr.data() + 9 * (size_t)f, 9 * sizeof(double));
@@ -1255,7 +1547,7 @@ mba_status mba_context_read_scale(mba_context* ctx, double* cam_scale, double* p
   if (cam_scale) {
     for (int64_t i = 0; i < 9 * (int64_t)ctx->nc; ++i) cam_scale[i] = 1.0;
     for (int32_t f = 0; f < ctx->nf; ++f)
-      std::memcpy(cam_scale + 9 * (size_t)ctx->st.f_to_cam[f], cs.data() + 9 * (size_t)f, 72);
+      std::memcpy(cam_scale + 9 * (size_t)ctx->f_to_cam[f], cs.data() + 9 * (size_t)f, 72);
   }
   if (pt_scale) {
     for (int64_t i = 0; i < 3 * (int64_t)ctx->np; ++i) pt_scale[i] = 1.0;
@@ -1272,14 +1564,14 @@ mba_status mba_context_schur_matvec(mba_context* ctx, const double* x, double* y
     using T = std::remove_pointer_t<decltype(dx)>;
     std::vector<T> hx(9 * (size_t)ctx->nf), hy(9 * (size_t)ctx->nf);
     for (int32_t f = 0; f < ctx->nf; ++f)
-      for (int k = 0; k < 9; ++k) hx[9 * (size_t)f + k] = (T)x[9 * (size_t)ctx->st.f_to_cam[f] + k];
+      for (int k = 0; k < 9; ++k) hx[9 * (size_t)f + k] = (T)x[9 * (size_t)ctx->f_to_cam[f] + k];
     if (!hx.empty()) HIP_TRY(hipMemcpyAsync(dx, hx.data(), hx.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
     eng.matvec(dx, dy, nullptr);
     if (!hy.empty()) HIP_TRY(hipMemcpyAsync(hy.data(), dy, hy.size() * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (int64_t i = 0; i < 9 * (int64_t)ctx->nc; ++i) y[i] = 0.0;
     for (int32_t f = 0; f < ctx->nf; ++f)
-      for (int k = 0; k < 9; ++k) y[9 * (size_t)ctx->st.f_to_cam[f] + k] = (double)hy[9 * (size_t)f + k];
+      for (int k = 0; k < 9; ++k) y[9 * (size_t)ctx->f_to_cam[f] + k] = (double)hy[9 * (size_t)f + k];
     return MBA_OK;
   };
   if (ctx->mixed) {
@@ -1298,7 +1590,7 @@ mba_status mba_context_solve(mba_context* ctx, mba_summary* summary, mba_iterati
     const double bytes = 2.0 * n * n * (ctx->mixed ? 4.0 : 8.0);
     if (bytes > 64.0 * (1ull << 30))
       return fail(MBA_ERR_UNSUPPORTED, "explicit Schur matrix too large; use MBA_IMPLICIT_SCHUR_PCG");
-    if (ctx->st.max_track > 128)
+    if (ctx->max_track > 128)
       return fail(MBA_ERR_UNSUPPORTED, "explicit Schur assembly supports tracks of at most 128 observations");
   }
   if (ctx->mixed) {
@@ -1312,17 +1604,7 @@ mba_status mba_context_solve(mba_context* ctx, mba_summary* summary, mba_iterati
 mba_status mba_context_download(mba_context* ctx, mba_problem* p) {
   if (!ctx || !p) return fail(MBA_ERR_INVALID_ARGUMENT, "NULL argument");
   HIP_TRY(hipSetDevice(ctx->device));
-  std::vector<double> cams(9 * (size_t)ctx->nc), pts(3 * (size_t)ctx->ne);
-  if (!cams.empty()) HIP_TRY(hipMemcpyAsync(cams.data(), ctx->cams64, cams.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-  if (!pts.empty()) HIP_TRY(hipMemcpyAsync(pts.data(), ctx->pts64, pts.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  for (int32_t f = 0; f < ctx->nf; ++f) {
-    const int32_t cam = ctx->st.f_to_cam[f];
-    std::memcpy(p->cameras + 9 * (size_t)cam, cams.data() + 9 * (size_t)cam, 72);
-  }
-  for (int32_t e = 0; e < ctx->ne; ++e)
-    std::memcpy(p->points + 3 * (size_t)ctx->dev_e_to_pt[e], pts.data() + 3 * (size_t)e, 24);
-  return MBA_OK;
+  return download(ctx, p, nullptr);
 }
 
 mba_status mba_solve(const mba_options* options, mba_problem* problem, mba_summary* summary,
@@ -1332,13 +1614,54 @@ mba_status mba_solve(const mba_options* options, mba_problem* problem, mba_summa
   std::memset(summary, 0, sizeof(*summary));
   const double t0 = now_s();
   mba_context* ctx = nullptr;
-  mba_status s = context_create(problem, options, &ctx);
+  mba_status s = context_create(problem, 1, options, &ctx);
   if (s != MBA_OK) return s;
   const double t_pre = now_s() - t0;
   s = mba_context_solve(ctx, summary, iterations, max_iterations);
   if (s == MBA_OK && summary->is_solution_usable) s = mba_context_download(ctx, problem);
   summary->preprocessor_time_s = t_pre;
   summary->total_time_s += t_pre;
+  context_free(ctx);
+  return s;
+}
+
+mba_status mba_solve_batched(const mba_options* options, mba_problem* problems, int32_t num_problems,
+                             mba_summary* summaries) {
+  g_last_error.clear();
+  if (num_problems < 0) return fail(MBA_ERR_INVALID_ARGUMENT, "num_problems < 0");
+  if (num_problems == 0) return MBA_OK;
+  if (!problems || !summaries) return fail(MBA_ERR_INVALID_ARGUMENT, "problems or summaries is NULL");
+  std::memset(summaries, 0, sizeof(mba_summary) * (size_t)num_problems);
+  mba_options o;
+  if (options) o = *options; else mba_default_options(&o);
+  if (o.linear_solver != MBA_IMPLICIT_SCHUR_PCG) {
+    // Dense reduced systems are factored one window at a time.
+    for (int32_t w = 0; w < num_problems; ++w) {
+      const mba_status s = mba_solve(&o, &problems[w], &summaries[w], nullptr, 0);
+      if (s != MBA_OK) return fail(s, "window " + std::to_string(w) + ": " + g_last_error);
+    }
+    return MBA_OK;
+  }
+  const double t0 = now_s();
+  mba_context* ctx = nullptr;
+  mba_status s = context_create(problems, num_problems, &o, &ctx);
+  if (s != MBA_OK) return s;
+  const double t_pre = now_s() - t0;
+  s = hipSetDevice(ctx->device) == hipSuccess ? MBA_OK : fail(MBA_ERR_DEVICE, "hipSetDevice failed");
+  if (s == MBA_OK) {
+    if (ctx->mixed) {
+      Engine<float> e(ctx);
+      s = e.solve(summaries, nullptr, 0);
+    } else {
+      Engine<double> e(ctx);
+      s = e.solve(summaries, nullptr, 0);
+    }
+  }
+  if (s == MBA_OK) s = download(ctx, problems, summaries);
+  for (int32_t w = 0; w < num_problems; ++w) {
+    summaries[w].preprocessor_time_s = t_pre;
+    summaries[w].total_time_s += t_pre;
+  }
   context_free(ctx);
   return s;
 }

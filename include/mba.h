@@ -171,6 +171,18 @@ mba_status mba_solve(const mba_options* options, mba_problem* problem,
                      mba_summary* summary, mba_iteration* iterations,
                      int32_t max_iterations);
 
+/* Many independent problems ("windows", e.g. local-BA windows of uneven
+ * size): the role of the global engine's concurrent ceres::Solve calls over
+ * image pairs (libs/sfm/src/sfm_global_engine.cpp:985-1001, one Solve per
+ * pair at :1168).  With MBA_IMPLICIT_SCHUR_PCG every window is solved in the
+ * same device launches: each window keeps its own trust region, PCG state
+ * and termination, exactly as its own mba_solve would decide them (sums are
+ * accumulated in another order, so values agree to rounding).  Other linear
+ * solvers solve the windows one after another.  summaries[i] is window i's;
+ * its time fields are the whole batch's.  Usable windows are written back. */
+mba_status mba_solve_batched(const mba_options* options, mba_problem* problems,
+                             int32_t num_problems, mba_summary* summaries);
+
 /* CostFunction::Evaluate for every observation, input order, computed on the
  * device in `precision`.  residuals: 2 per observation (no loss applied);
  * jac_camera: row-major 2x9 per observation; jac_point: row-major 2x3; each
