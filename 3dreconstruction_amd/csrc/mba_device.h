@@ -86,10 +86,18 @@ struct Win {
   const int32_t* of_f;
   const double* inv_r;
   double* acc;
-  int32_t* flag;  // per-window "non-finite / failed factorisation" flags
+  int32_t* flag;            // per-window "non-finite / failed factorisation" flags
+  const uint8_t* active;    // windows still iterating (nullptr: all)
+  const PcgState* pcg;      // inside PCG: windows whose solve has stopped are idle
 };
 __device__ __forceinline__ int win_of_e(const Win& w, int e) { return w.of_e ? w.of_e[e] : 0; }
 __device__ __forceinline__ int win_of_f(const Win& w, int f) { return w.of_f ? w.of_f[f] : 0; }
+__device__ __forceinline__ bool win_idle(const Win& w, int wi) {
+  return (w.active && !w.active[wi]) || (w.pcg && w.pcg[wi].done);
+}
+// Work of an idle window is skipped (a single problem is never idle).
+__device__ __forceinline__ bool win_idle_e(const Win& w, int e) { return w.of_e && win_idle(w, w.of_e[e]); }
+__device__ __forceinline__ bool win_idle_f(const Win& w, int f) { return w.of_f && win_idle(w, w.of_f[f]); }
 
 // acc[stride * window + k] += v for every lane (wi < 0: no contribution).
 // All lanes of the wave must call it.  A wave whose contributing lanes share
@@ -420,7 +428,8 @@ __global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __rest
                                                    const double* __restrict__ ctab, double huber_a,
                                                    double* __restrict__ partial, Win win) {
   __shared__ double lds[kBlock / 64];
-  const SegPos sp = seg_locate(sg);
+  SegPos sp = seg_locate(sg);
+  if (sp.valid && win_idle_e(win, sp.e)) sp.valid = false;
   double acc[1] = {0.0};
   if (sp.valid) {
     const int e = sp.e;
@@ -501,9 +510,12 @@ __global__ __launch_bounds__(kBlock) void k_grad_win(int64_t n, int per, int is_
   double v = 0.0;
   int wi = -1;
   if (i < n) {
-    v = (double)g[i] / scale[i];
-    wi = is_cam ? win_of_f(win, (int)(i / per)) : win_of_e(win, (int)(i / per));
-    atomic_max_nonneg(out + 2 * (size_t)wi, fabs(v));
+    const int blk = (int)(i / per);
+    if (!(is_cam ? win_idle_f(win, blk) : win_idle_e(win, blk))) {
+      v = (double)g[i] / scale[i];
+      wi = is_cam ? win_of_f(win, blk) : win_of_e(win, blk);
+      atomic_max_nonneg(out + 2 * (size_t)wi, fabs(v));
+    }
   }
   win_atomic_add(out, 2, 1, wi, v * v);
 }
@@ -550,7 +562,8 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
     const T* __restrict__ ctab, T huber_a, Win win, double min_diag, double max_diag,
     double* __restrict__ diag_p, int recompute_diag, T* __restrict__ V, T* __restrict__ gp,
     T* __restrict__ Vinv, T* __restrict__ pk) {
-  const SegPos q = seg_locate(sg);
+  SegPos q = seg_locate(sg);
+  if (q.valid && win_idle_e(win, q.e)) q.valid = false;
   const int e = q.valid ? q.e : sg.e0[0];
   const T X[3] = {(T)pts64[3 * (size_t)e], (T)pts64[3 * (size_t)e + 1], (T)pts64[3 * (size_t)e + 2]};
   const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
@@ -617,7 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restri
                                                        const T* __restrict__ pscale,
                                                        T* __restrict__ Vinv, T* __restrict__ pk) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
+  if (e >= ne || win_idle_e(win, e)) return;
   const int wi = win_of_e(win, e);
   const double inv_radius = win.inv_r[wi];
   double a[6];
@@ -678,10 +691,11 @@ __global__ __launch_bounds__(kBlock) void k_cam_pass(
     const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
-    const T* __restrict__ ctab, T huber_a, int with_h, T* __restrict__ partial) {
+    const T* __restrict__ ctab, T huber_a, int with_h, T* __restrict__ partial, Win win) {
   __shared__ T lds[63 * (kBlock / 64)];
   const int ch = blockIdx.x;
   const int f = chunk_f[ch];
+  if (win_idle_f(win, f)) return;
   const T* tab = ctab + (size_t)kCamTab * f_to_cam[f];
   CamProj<T> c;
   load_proj(tab, c);
@@ -731,6 +745,7 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
   __shared__ int ij[64];
   const int f = blockIdx.x;
   const int k = threadIdx.x;
+  if (win_idle_f(win, f)) return;
   if (k == 0) {
     int q = 0;
     for (int a = 0; a < 9; ++a)
@@ -787,7 +802,7 @@ __global__ __launch_bounds__(64) void k_cam_damp(int nf, const T* __restrict__ U
                                                  const double* __restrict__ diag_c, Win win,
                                                  T* __restrict__ Minv) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
+  if (f >= nf || win_idle_f(win, f)) return;
   const int wi = win_of_f(win, f);
   const double inv_radius = win.inv_r[wi];
   double A[81], inv[81];
@@ -813,12 +828,14 @@ __global__ __launch_bounds__(kBlock) void k_cam_dir(int nc, const int32_t* __res
                                                     const T* __restrict__ cscale,
                                                     const T* __restrict__ ctab,
                                                     const T* __restrict__ x, T sign,
-                                                    T* __restrict__ dtab, const int32_t* __restrict__ all_done) {
+                                                    T* __restrict__ dtab, const int32_t* __restrict__ all_done,
+                                                    Win win) {
   if (all_done && *all_done) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   T* o = dtab + (size_t)kDirTab * c;
   const int f = cam_to_f[c];
+  if (f >= 0 && win_idle_f(win, f)) return;
   if (f < 0 || x == nullptr) {
     for (int i = 0; i < kDirTab; ++i) o[i] = T(0);
     return;
@@ -838,9 +855,11 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
     Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
     const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ Vinv, T huber_a,
-    T beta, int apply_vinv, T* __restrict__ zout, T* __restrict__ pk, const int32_t* __restrict__ all_done) {
+    T beta, int apply_vinv, T* __restrict__ zout, T* __restrict__ pk, const int32_t* __restrict__ all_done,
+    Win win) {
   if (all_done && *all_done) return;
-  const SegPos qp = seg_locate(sg);
+  SegPos qp = seg_locate(sg);
+  if (qp.valid && win_idle_e(win, qp.e)) qp.valid = false;
   const int e = qp.valid ? qp.e : sg.e0[0];
   T X[3], unused[3];
   load_pk(pkX, e, X, unused);
@@ -889,11 +908,12 @@ __global__ __launch_bounds__(kBlock) void k_mv_cam(
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
     const T* __restrict__ cscale, const T* __restrict__ ctab, T huber_a, T* __restrict__ partial,
-    const int32_t* __restrict__ all_done) {
+    const int32_t* __restrict__ all_done, Win win) {
   __shared__ T lds[9 * (kBlock / 64)];
   if (all_done && *all_done) return;
   const int ch = blockIdx.x;
   const int f = chunk_f[ch];
+  if (win_idle_f(win, f)) return;
   const T* tab = ctab + (size_t)kCamTab * f_to_cam[f];
   CamProj<T> c;
   load_proj(tab, c);
@@ -939,7 +959,7 @@ __global__ __launch_bounds__(kBlock) void k_mv_finalize(
     const T* __restrict__ x, int u_term, T* __restrict__ y, const int32_t* __restrict__ all_done) {
   if (all_done && *all_done) return;
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
+  if (f >= nf || win_idle_f(win, f)) return;
   const double inv_radius = win.inv_r[win_of_f(win, f)];
   T s[9];
   for (int k = 0; k < 9; ++k) s[k] = T(0);
@@ -963,9 +983,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_rhs_finalize(int nf, const int32_t* __restrict__ f_chunk_start,
                                                          const T* __restrict__ partial,
                                                          const T* __restrict__ gc,
-                                                         T* __restrict__ rhs) {
+                                                         T* __restrict__ rhs, Win win) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
+  if (f >= nf || win_idle_f(win, f)) return;
   T s[9];
   for (int k = 0; k < 9; ++k) s[k] = T(0);
   for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch)
@@ -982,7 +1002,8 @@ __global__ __launch_bounds__(kBlock) void k_model_cost(
     const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ step_p,
     T huber_a, double* __restrict__ partial, Win win) {
   __shared__ double lds[kBlock / 64];
-  const SegPos qp = seg_locate(sg);
+  SegPos qp = seg_locate(sg);
+  if (qp.valid && win_idle_e(win, qp.e)) qp.valid = false;
   double acc[1] = {0.0};
   if (qp.valid) {
     const int e = qp.e;
@@ -1028,7 +1049,8 @@ __global__ __launch_bounds__(kBlock) void k_plus(int64_t n, const double* __rest
   __shared__ double lds[kBlock / 64];
   double acc[1] = {0.0};
   const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-  if (i < n) {
+  const bool on = i < n && !win_idle_e(win, (int)(i / 3));
+  if (on) {
     const double d = step[i] * scale[i];
     const double v = x[i] + d;
     out[i] = v;
@@ -1036,7 +1058,7 @@ __global__ __launch_bounds__(kBlock) void k_plus(int64_t n, const double* __rest
     acc[0] = df * df;
   }
   if (win.acc) {
-    win_atomic_add(win.acc, 1, 0, i < n ? win_of_e(win, (int)(i / 3)) : -1, acc[0]);
+    win_atomic_add(win.acc, 1, 0, on ? win_of_e(win, (int)(i / 3)) : -1, acc[0]);
     return;
   }
   block_sum<double, 1>(acc, lds);
@@ -1055,7 +1077,12 @@ __global__ __launch_bounds__(kBlock) void k_sumsq(int64_t n, int64_t npts3, cons
   const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
   if (i < n) acc[0] = x[i] * x[i];
   if (win.acc) {
-    const int wi = i >= n ? -1 : (i < npts3 ? win_of_e(win, (int)(i / 3)) : win_of_f(win, (int)((i - npts3) / 9)));
+    int wi = -1;
+    if (i < n) {
+      const bool pt = i < npts3;
+      const int blk = pt ? (int)(i / 3) : (int)((i - npts3) / 9);
+      if (!(pt ? win_idle_e(win, blk) : win_idle_f(win, blk))) wi = pt ? win_of_e(win, blk) : win_of_f(win, blk);
+    }
     win_atomic_add(win.acc, 1, 0, wi, acc[0]);
     return;
   }
@@ -1072,6 +1099,28 @@ __global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const Ts* __restr
                                                     Td* __restrict__ dst) {
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
     dst[i] = (Td)src[i];
+}
+
+// Layout set-up: the E block of every point-major row.
+__global__ __launch_bounds__(kBlock) void k_row_e(int ne, const int32_t* __restrict__ pt_start,
+                                                  int32_t* __restrict__ row_e) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  for (int r = pt_start[e]; r < pt_start[e + 1]; ++r) row_e[r] = e;
+}
+
+// Layout set-up: camera-major rows gathered from the point-major rows.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather_cm(int ncm, const int32_t* __restrict__ cm_src,
+                                                      const int32_t* __restrict__ row_e,
+                                                      const double* __restrict__ row_obs64,
+                                                      int32_t* __restrict__ cm_e, T* __restrict__ cm_obs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncm) return;
+  const int r = cm_src[k];
+  cm_e[k] = row_e[r];
+  cm_obs[2 * (size_t)k] = (T)row_obs64[2 * (size_t)r];
+  cm_obs[2 * (size_t)k + 1] = (T)row_obs64[2 * (size_t)r + 1];
 }
 
 // Scatter FP64 values at `idx` positions (gather = 0) or gather.
@@ -1122,9 +1171,10 @@ __global__ __launch_bounds__(kBlock) void k_dot_partial(int64_t n, const int32_t
   if (all_done && *all_done) return;
   const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
   double acc[1] = {0.0};
-  if (i < n) acc[0] = (double)a[i] * (double)b[i];
+  const bool on = i < n && !win_idle_f(win, (int)(i / 9));
+  if (on) acc[0] = (double)a[i] * (double)b[i];
   if (win.acc) {
-    win_atomic_add(win.acc, 1, 0, i < n ? win_of_f(win, (int)(i / 9)) : -1, acc[0]);
+    win_atomic_add(win.acc, 1, 0, on ? win_of_f(win, (int)(i / 9)) : -1, acc[0]);
     return;
   }
   block_sum<double, 1>(acc, lds);
@@ -1699,7 +1749,8 @@ __global__ __launch_bounds__(kBlock) void k_plus_cams(int nf, const int32_t* __r
   __shared__ double lds[kBlock / 64];
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   double acc[1] = {0.0};
-  if (i < 9 * (int64_t)nf) {
+  const bool on = i < 9 * (int64_t)nf && !win_idle_f(win, (int)(i / 9));
+  if (on) {
     const size_t j = 9 * (size_t)f_to_cam[i / 9] + i % 9;
     const double v = cams[j] + step[i] * scale[i];
     out[j] = v;
@@ -1707,7 +1758,7 @@ __global__ __launch_bounds__(kBlock) void k_plus_cams(int nf, const int32_t* __r
     acc[0] = df * df;
   }
   if (win.acc) {
-    win_atomic_add(win.acc, 1, 0, i < 9 * (int64_t)nf ? win_of_f(win, (int)(i / 9)) : -1, acc[0]);
+    win_atomic_add(win.acc, 1, 0, on ? win_of_f(win, (int)(i / 9)) : -1, acc[0]);
     return;
   }
   block_sum<double, 1>(acc, lds);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -13,6 +14,7 @@
 #include <memory>
 #include <type_traits>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mba.h"
@@ -351,11 +353,12 @@ struct mba_context {
   int32_t *of_e = nullptr, *of_f = nullptr;
   double *inv_r = nullptr, *wacc = nullptr, *wacc2 = nullptr;
   uint8_t* accept = nullptr;
+  uint8_t* wactive = nullptr;  // windows still iterating
   // Host copies of what was last uploaded (rewritten only while the stream
   // is idle).
   std::vector<double> h_inv_r;
   std::vector<PcgState> h_pcg;
-  std::vector<uint8_t> h_accept;
+  std::vector<uint8_t> h_accept, h_active;
   Dev<float> df;
   Dev<double> dd;
   bool mixed = false;
@@ -386,18 +389,30 @@ mba_status upload(mba_context* c, X** out, const std::vector<X>& v) {
 
 template <typename T>
 mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_start,
-                           const std::vector<int32_t>& row_cam,
-                           const std::vector<double>& row_obs, const std::vector<int32_t>& cm_e,
-                           const std::vector<double>& cm_obs, const std::vector<int32_t>& chunk_f,
-                           const std::vector<int32_t>& chunk_begin, const std::vector<int32_t>& chunk_end,
-                           const std::vector<int32_t>& f_chunk_start) {
+                           const std::vector<int32_t>& row_cam, const std::vector<int32_t>& cm_src,
+                           const std::vector<int32_t>& chunk_f, const std::vector<int32_t>& chunk_begin,
+                           const std::vector<int32_t>& chunk_end, const std::vector<int32_t>& f_chunk_start) {
   Dev<T>& d = dev<T>(c);
-  std::vector<T> ro(row_obs.begin(), row_obs.end()), co(cm_obs.begin(), cm_obs.end());
+  const hipStream_t s = c->stream;
   MBA_TRY(upload(c, &d.row_cam, row_cam));
-  MBA_TRY(upload(c, &d.row_obs, ro));
   MBA_TRY(upload(c, &d.pt_start, dev_pt_start));
-  MBA_TRY(upload(c, &d.cm_e, cm_e));
-  MBA_TRY(upload(c, &d.cm_obs, co));
+  // Kernel-precision observations and the camera-major copies are built
+  // on the device from the FP64 point-major rows.
+  if constexpr (std::is_same<T, double>::value) {
+    d.row_obs = c->row_obs64;
+  } else {
+    MBA_TRY(c->pool.alloc(&d.row_obs, 2 * (size_t)c->nrows));
+    hipLaunchKernelGGL((k_convert<double, T>), dim3(nblocks(2 * (int64_t)c->nrows)), dim3(kBlock), 0, s,
+                       2 * (int64_t)c->nrows, (const double*)c->row_obs64, d.row_obs);
+  }
+  int32_t *src = nullptr, *row_e = nullptr;
+  MBA_TRY(upload(c, &src, cm_src));
+  MBA_TRY(c->pool.alloc(&row_e, (size_t)c->nrows));
+  MBA_TRY(c->pool.alloc(&d.cm_e, (size_t)c->ncm));
+  MBA_TRY(c->pool.alloc(&d.cm_obs, 2 * (size_t)c->ncm));
+  hipLaunchKernelGGL(k_row_e, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, (const int32_t*)d.pt_start, row_e);
+  hipLaunchKernelGGL(k_gather_cm<T>, dim3(nblocks(c->ncm)), dim3(kBlock), 0, s, c->ncm, (const int32_t*)src,
+                     (const int32_t*)row_e, (const double*)c->row_obs64, d.cm_e, d.cm_obs);
   MBA_TRY(upload(c, &d.chunk_f, chunk_f));
   MBA_TRY(upload(c, &d.chunk_begin, chunk_begin));
   MBA_TRY(upload(c, &d.chunk_end, chunk_end));
@@ -439,6 +454,37 @@ int chunk_rows_for(int64_t nrows) {
   return (int)((c + 63) / 64 * 64);
 }
 
+// Host threads for n items (at most 16, about 64k items each).
+int host_threads(int64_t n, int64_t grain = 1 << 16) {
+  const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+  return (int)std::max<int64_t>(1, std::min<int64_t>({std::min<int64_t>(16, hw), (n + grain - 1) / grain}));
+}
+
+// fn(t) for t in [0, nt) on nt threads (inline when nt == 1).
+template <typename F>
+void run_threads(int nt, F&& fn) {
+  if (nt <= 1) {
+    fn(0);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; ++t) th.emplace_back([&fn, t]() { fn(t); });
+  for (std::thread& x : th) x.join();
+}
+
+// fn(begin, end) over equal ranges of [0, n).
+template <typename F>
+void parallel_for(int64_t n, F&& fn) {
+  if (n <= 0) return;
+  const int nt = host_threads(n);
+  const int64_t step = (n + nt - 1) / nt;
+  run_threads(nt, [&](int t) {
+    const int64_t b = t * step, e = std::min(n, b + step);
+    if (b < e) fn(b, e);
+  });
+}
+
 void context_free(mba_context* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
@@ -475,25 +521,54 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   c->pt_off.assign((size_t)W + 1, 0);
   c->e_off.assign((size_t)W + 1, 0);
   c->f_off.assign((size_t)W + 1, 0);
-  int64_t nrows = 0;
-  for (int32_t w = 0; w < W; ++w) {
+  // Per-window preprocessing (the reference's ordering), windows in
+  // parallel on host threads.
+  std::vector<mba_status> wstat(W, MBA_OK);
+  std::vector<std::string> werr(W);
+  auto prep = [&](int32_t w) {
     const mba_problem* p = &probs[w];
     Structure& st = c->wst[w];
-    {
-      const mba_status s = build_structure(p, &st);
-      if (s != MBA_OK) return fail(s, window_prefix(W, w) + g_last_error);
+    mba_status s = build_structure(p, &st);
+    if (s != MBA_OK) {
+      wstat[w] = s;
+      werr[w] = g_last_error;  // thread-local to this worker
+      return;
     }
-    if (!st.standard)
-      return fail(MBA_ERR_UNSUPPORTED,
-                  window_prefix(W, w) + "the Schur ordering did not eliminate exactly the point blocks");
-    for (int32_t q = 0; q < p->num_points; ++q)
-      for (int k = 0; k < 3; ++k)
-        if (!std::isfinite(p->points[3 * (size_t)q + k]))
-          return fail(MBA_ERR_NOT_FINITE, window_prefix(W, w) + "non-finite point");
-    for (int32_t q = 0; q < p->num_cameras; ++q)
-      for (int k = 0; k < 9; ++k)
-        if (!std::isfinite(p->cameras[9 * (size_t)q + k]))
-          return fail(MBA_ERR_NOT_FINITE, window_prefix(W, w) + "non-finite camera");
+    if (!st.standard) {
+      wstat[w] = MBA_ERR_UNSUPPORTED;
+      werr[w] = "the Schur ordering did not eliminate exactly the point blocks";
+      return;
+    }
+    for (int64_t i = 0; i < 3 * (int64_t)p->num_points; ++i)
+      if (!std::isfinite(p->points[i])) {
+        wstat[w] = MBA_ERR_NOT_FINITE;
+        werr[w] = "non-finite point";
+        return;
+      }
+    for (int64_t i = 0; i < 9 * (int64_t)p->num_cameras; ++i)
+      if (!std::isfinite(p->cameras[i])) {
+        wstat[w] = MBA_ERR_NOT_FINITE;
+        werr[w] = "non-finite camera";
+        return;
+      }
+  };
+  if (W == 1) {
+    prep(0);
+  } else {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<unsigned>((unsigned)W, std::min(16u, hw));
+    std::atomic<int32_t> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+      pool.emplace_back([&]() {
+        for (int32_t w = next.fetch_add(1); w < W; w = next.fetch_add(1)) prep(w);
+      });
+    for (std::thread& t : pool) t.join();
+  }
+  int64_t nrows = 0;
+  for (int32_t w = 0; w < W; ++w) {
+    if (wstat[w] != MBA_OK) return fail(wstat[w], window_prefix(W, w) + werr[w]);
+    const Structure& st = c->wst[w];
     const int64_t nc_tot = (int64_t)c->cam_off[w] + st.nc, np_tot = (int64_t)c->pt_off[w] + st.np;
     nrows += st.nobs;
     if (nc_tot > 0x7fffffffLL || np_tot > 0x7fffffffLL || nrows > 0x7fffffffLL)
@@ -581,42 +656,60 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       c->cam_to_f[cam] = gf;
     }
   // Rows in device order (each point's rows keep its window's reference
-  // order); camera indices are global.
-  std::vector<int32_t> row_cam(c->nrows), row_e(c->nrows);
+  // order); camera indices are global.  Points are independent.
+  std::vector<int32_t> row_cam(c->nrows);
   std::vector<double> row_obs(2 * (size_t)c->nrows);
-  for (int32_t de = 0; de < c->ne; ++de) {
-    const int32_t ge = dperm[de], w = ge_win[ge], e = ge_local[ge];
-    const Structure& st = c->wst[w];
-    const mba_problem* p = &probs[w];
-    int32_t r = dev_pt_start[de];
-    for (int32_t rr = st.pt_start[e]; rr < st.pt_start[e + 1]; ++rr, ++r) {
-      const int64_t i = st.res_order[rr];
-      row_e[r] = de;
-      row_cam[r] = c->cam_off[w] + p->camera_index[i];
-      row_obs[2 * (size_t)r] = p->observations[2 * i];
-      row_obs[2 * (size_t)r + 1] = p->observations[2 * i + 1];
+  parallel_for(c->ne, [&](int64_t b, int64_t end) {
+    for (int64_t de = b; de < end; ++de) {
+      const int32_t ge = dperm[de], w = ge_win[ge], e = ge_local[ge];
+      const Structure& st = c->wst[w];
+      const mba_problem* p = &probs[w];
+      int32_t r = dev_pt_start[de];
+      for (int32_t rr = st.pt_start[e]; rr < st.pt_start[e + 1]; ++rr, ++r) {
+        const int64_t i = st.res_order[rr];
+        row_cam[r] = c->cam_off[w] + p->camera_index[i];
+        row_obs[2 * (size_t)r] = p->observations[2 * i];
+        row_obs[2 * (size_t)r + 1] = p->observations[2 * i + 1];
+      }
     }
-  }
-  // Camera-major rows (stable by row order), constant cameras excluded.
+  });
+  // Camera-major rows: a stable counting sort of the rows by F block
+  // (constant cameras excluded) with per-thread histograms over row
+  // ranges; cm_src[k] is the point-major row of camera-major slot k.
   std::vector<int64_t> fstart((size_t)c->nf + 1, 0);
-  for (int32_t r = 0; r < c->nrows; ++r) {
-    const int32_t f = c->cam_to_f[row_cam[r]];
-    if (f >= 0) fstart[f + 1]++;
-  }
-  for (int32_t f = 0; f < c->nf; ++f) fstart[f + 1] += fstart[f];
-  c->ncm = (int32_t)fstart[c->nf];
-  std::vector<int32_t> cm_e(c->ncm);
-  std::vector<double> cm_obs(2 * (size_t)c->ncm);
+  std::vector<int32_t> cm_src;
   {
-    std::vector<int64_t> fill(fstart.begin(), fstart.end() - 1);
-    for (int32_t r = 0; r < c->nrows; ++r) {
-      const int32_t f = c->cam_to_f[row_cam[r]];
-      if (f < 0) continue;
-      const int64_t k = fill[f]++;
-      cm_e[k] = row_e[r];
-      cm_obs[2 * k] = row_obs[2 * (size_t)r];
-      cm_obs[2 * k + 1] = row_obs[2 * (size_t)r + 1];
+    const int nt = host_threads(c->nrows);
+    const int64_t step = ((int64_t)c->nrows + nt - 1) / nt;
+    std::vector<std::vector<int64_t>> cnt(nt, std::vector<int64_t>((size_t)c->nf, 0));
+    run_threads(nt, [&](int t) {
+      std::vector<int64_t>& h = cnt[t];
+      const int64_t end = std::min<int64_t>(c->nrows, (t + 1) * step);
+      for (int64_t r = t * step; r < end; ++r) {
+        const int32_t f = c->cam_to_f[row_cam[r]];
+        if (f >= 0) h[f]++;
+      }
+    });
+    int64_t run = 0;
+    for (int32_t f = 0; f < c->nf; ++f) {
+      fstart[f] = run;
+      for (int t = 0; t < nt; ++t) {
+        const int64_t n = cnt[t][f];
+        cnt[t][f] = run;
+        run += n;
+      }
     }
+    fstart[c->nf] = run;
+    c->ncm = (int32_t)run;
+    cm_src.resize((size_t)run);
+    run_threads(nt, [&](int t) {
+      std::vector<int64_t>& o = cnt[t];
+      const int64_t end = std::min<int64_t>(c->nrows, (t + 1) * step);
+      for (int64_t r = t * step; r < end; ++r) {
+        const int32_t f = c->cam_to_f[row_cam[r]];
+        if (f >= 0) cm_src[o[f]++] = (int32_t)r;
+      }
+    });
   }
   const int cr = chunk_rows_for(c->ncm);
   std::vector<int32_t> chunk_f, chunk_begin, chunk_end, f_chunk_start((size_t)c->nf + 1, 0);
@@ -632,11 +725,13 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   c->nchunks = (int32_t)chunk_f.size();
   // FP64 state.
   std::vector<double> pts(3 * (size_t)c->ne), cams(9 * (size_t)c->nc);
-  for (int32_t de = 0; de < c->ne; ++de) {
-    const int32_t ge = dperm[de], w = ge_win[ge];
-    const int32_t q = c->dev_e_to_pt[de] - c->pt_off[w];
-    for (int k = 0; k < 3; ++k) pts[3 * (size_t)de + k] = probs[w].points[3 * (size_t)q + k];
-  }
+  parallel_for(c->ne, [&](int64_t b, int64_t end) {
+    for (int64_t de = b; de < end; ++de) {
+      const int32_t w = ge_win[dperm[de]];
+      const int32_t q = c->dev_e_to_pt[de] - c->pt_off[w];
+      for (int k = 0; k < 3; ++k) pts[3 * (size_t)de + k] = probs[w].points[3 * (size_t)q + k];
+    }
+  });
   for (int32_t w = 0; w < W; ++w)
     if (probs[w].num_cameras > 0)
       std::memcpy(cams.data() + 9 * (size_t)c->cam_off[w], probs[w].cameras,
@@ -667,6 +762,8 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   MBA_TRY(c->pool.alloc(&c->accept, (size_t)W));
   c->h_inv_r.assign(W, 0.0);
   MBA_TRY(upload(c, &c->inv_r, c->h_inv_r));
+  c->h_active.assign(W, 1);
+  MBA_TRY(upload(c, &c->wactive, c->h_active));
   if (W > 1) {
     std::vector<int32_t> of_e(c->ne), of_f(c->nf);
     for (int32_t de = 0; de < c->ne; ++de) of_e[de] = ge_win[dperm[de]];
@@ -679,11 +776,9 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   MBA_TRY(c->pool.alloc(&c->x64, 9 * (size_t)c->nf));
   MBA_TRY(c->pool.alloc(&c->r64, 9 * (size_t)c->nf));
   if (c->mixed)
-    MBA_TRY(alloc_precision<float>(c, dev_pt_start, row_cam, row_obs, cm_e, cm_obs, chunk_f, chunk_begin, chunk_end,
-                                   f_chunk_start));
+    MBA_TRY(alloc_precision<float>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_chunk_start));
   else
-    MBA_TRY(alloc_precision<double>(c, dev_pt_start, row_cam, row_obs, cm_e, cm_obs, chunk_f, chunk_begin, chunk_end,
-                                    f_chunk_start));
+    MBA_TRY(alloc_precision<double>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_chunk_start));
   for (int i = 0; i < 4; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
   HIP_TRY(hipStreamSynchronize(c->stream));
   *out = c;
@@ -692,8 +787,10 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
 }
 
 // The window descriptor handed to kernels; `acc` is the per-window
-// accumulator array of a reduction (unused for a single problem).
-Win win_desc(mba_context* c, double* acc) {
+// accumulator array of a reduction (unused for a single problem).  Windows
+// that have stopped iterating (and, inside PCG, whose solve has stopped)
+// are skipped by the kernels.
+Win win_desc(mba_context* c, double* acc, bool in_pcg) {
   Win w;
   const bool multi = c->W > 1;
   w.of_e = multi ? c->of_e : nullptr;
@@ -701,6 +798,8 @@ Win win_desc(mba_context* c, double* acc) {
   w.inv_r = c->inv_r;
   w.acc = multi ? acc : nullptr;
   w.flag = c->flag;
+  w.active = multi ? c->wactive : nullptr;
+  w.pcg = multi && in_pcg ? c->pcg : nullptr;
   return w;
 }
 
@@ -719,7 +818,17 @@ struct Engine {
     HIP_TRY(hipStreamSynchronize(s));
     return MBA_OK;
   }
-  Win win(double* acc = nullptr) { return win_desc(c, acc); }
+  Win win(double* acc = nullptr, bool in_pcg = false) { return win_desc(c, acc, in_pcg); }
+
+  // Windows still iterating; uploaded when it changes, after the stream
+  // has drained.
+  mba_status set_active(const std::vector<uint8_t>& act) {
+    if (W == 1 || act == c->h_active) return MBA_OK;
+    MBA_TRY(sync());
+    c->h_active = act;
+    HIP_TRY(hipMemcpyAsync(c->wactive, c->h_active.data(), W, hipMemcpyHostToDevice, s));
+    return MBA_OK;
+  }
 
   mba_status clear_flags() {
     HIP_TRY(hipMemsetAsync(c->flag, 0, sizeof(int32_t) * W, s));
@@ -795,7 +904,7 @@ struct Engine {
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
     if (c->nchunks > 0)
       hipLaunchKernelGGL(k_cam_pass<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
-                         d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial);
+                         d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial, win());
     if (c->timing) (void)hipEventRecord(c->ev[2], s);
     if (c->nf > 0)
       hipLaunchKernelGGL(k_cam_finalize<T>, dim3(c->nf), dim3(64), 0, s, c->nf, d.f_chunk_start, d.partial,
@@ -814,25 +923,27 @@ struct Engine {
       if (c->nchunks > 0)
         hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
                            d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber,
-                           d.partial, (const int32_t*)nullptr);
+                           d.partial, (const int32_t*)nullptr, win());
       hipLaunchKernelGGL(k_rhs_finalize<T>, dim3(nblocks(c->nf)), dim3(kBlock), 0, s, c->nf,
-                         d.f_chunk_start, d.partial, d.gc, d.rhs);
+                         d.f_chunk_start, d.partial, d.gc, d.rhs, win());
     }
   }
 
-  // y = S x (implicit), x and y F-order.  Skipped once `done` is set.
+  // y = S x (implicit), x and y F-order.  Inside PCG (`done` set) the
+  // launches stop once every window's solve has stopped.
   void matvec(const T* x, T* y, const int32_t* done) {
+    const Win wv = win(nullptr, done != nullptr);
     hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
-                       d.ctab, x, (T)1, d.dtab, done);
+                       d.ctab, x, (T)1, d.dtab, done, wv);
     if (c->seg_blocks > 0)
       hipLaunchKernelGGL(k_mv_point<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
-                         d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)0, 1, d.z, d.pk, done);
+                         d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)0, 1, d.z, d.pk, done, wv);
     if (c->nchunks > 0)
       hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
                          d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber,
-                         d.partial, done);
+                         d.partial, done, wv);
     hipLaunchKernelGGL(k_mv_finalize<T>, dim3(nblocks(c->nf)), dim3(kBlock), 0, s, c->nf, d.f_chunk_start,
-                       d.partial, d.U, c->diag_c, win(), x, 1, y, done);
+                       d.partial, d.U, c->diag_c, wv, x, 1, y, done);
   }
 
   // PCG on S x = rhs (x in d.x) for every window with active[w]; each
@@ -856,7 +967,7 @@ struct Engine {
     double* dot_acc = c->W > 1 ? c->wacc : nullptr;
     double* upd_acc = c->W > 1 ? c->wacc + W : nullptr;
     if (c->W > 1) HIP_TRY(hipMemsetAsync(c->wacc, 0, sizeof(double) * 4 * W, s));
-    const Win wd = win(dot_acc), wu = win(upd_acc);
+    const Win wd = win(dot_acc, true), wu = win(upd_acc, true);
     const double eta = c->opt.eta;
     const int minit = c->opt.min_linear_solver_iterations, maxit = c->opt.max_linear_solver_iterations;
     const int nthr = std::min(256, std::max(64, (W + 63) / 64 * 64));
@@ -969,11 +1080,11 @@ struct Engine {
   // Back substitution: y_e = Vinv E'(b - F z) into d.z; step = -y.
   void back_substitute() {
     hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
-                       d.ctab, d.x, (T)-1, d.dtab, (const int32_t*)nullptr);
+                       d.ctab, d.x, (T)-1, d.dtab, (const int32_t*)nullptr, win());
     if (c->seg_blocks > 0)
       hipLaunchKernelGGL(k_mv_point<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
                          d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)1, 1, d.z, (T*)nullptr,
-                         (const int32_t*)nullptr);
+                         (const int32_t*)nullptr, win());
     hipLaunchKernelGGL(k_negate<T>, dim3(nblocks(3 * (int64_t)c->ne)), dim3(kBlock), 0, s,
                        (int64_t)3 * c->ne, d.z, d.step_p);
     hipLaunchKernelGGL(k_negate<T>, dim3(nblocks(9 * (int64_t)c->nf)), dim3(kBlock), 0, s,
@@ -983,7 +1094,7 @@ struct Engine {
   // Model cost change -J step . (r + J step / 2) per window.
   mba_status model_cost_change(std::vector<double>* mcc) {
     hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
-                       d.ctab, d.step_c, (T)1, d.dtab, (const int32_t*)nullptr);
+                       d.ctab, d.step_c, (T)1, d.dtab, (const int32_t*)nullptr, win());
     const int nb = c->seg_blocks;
     MBA_TRY(begin_sum());
     if (nb > 0)
@@ -1197,7 +1308,13 @@ struct Engine {
       L[w].cost = cost[w];
       L[w].it.cost = cost[w];
     }
+    auto activity = [&]() {
+      std::vector<uint8_t> a(W);
+      for (int w = 0; w < W; ++w) a[w] = L[w].active;
+      return a;
+    };
     if (any_active()) {
+      MBA_TRY(set_active(activity()));
       t0 = now_s();
       tables(c->cams64);
       MBA_TRY(init_scale(o.jacobi_scaling != 0));
@@ -1227,6 +1344,7 @@ struct Engine {
         if (L[w].active && L[w].it.iteration >= o.max_num_iterations) finish(w, MBA_NO_CONVERGENCE);
       if (!any_active()) break;
       for (int w = 0; w < W; ++w) act[w] = L[w].active;
+      MBA_TRY(set_active(act));
       t0 = now_s();
       MBA_TRY(clear_flags());
       MBA_TRY(compute_step(act, &solved, &lin));
@@ -1315,6 +1433,7 @@ struct Engine {
         }
         any_step = true;
       }
+      MBA_TRY(set_active(activity()));
       if (any_acc) {
         MBA_TRY(accept(acc));
         MBA_TRY(x_norm(&xn));

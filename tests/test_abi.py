@@ -99,6 +99,27 @@ def test_non_point_elimination_is_reported_unsupported(mba, tiny_scene):
     assert e.value.status == mba.ERR_UNSUPPORTED
 
 
+def test_batched_input_errors_name_the_window(mba, c1, tiny_scene):
+    """Windows are validated on the host before any device work; the first
+    bad window is named."""
+    opts = mba.default_options(linear_solver=1)
+    bad = ba_problem(mba, c1)
+    bad.cameras[0, 0] = np.nan
+    with pytest.raises(mba.MbaError) as e:
+        mba.solve_batched([ba_problem(mba, c1), bad, ba_problem(mba, tiny_scene)], opts)
+    assert e.value.status == mba.ERR_NOT_FINITE and "window 1" in str(e.value)
+    with pytest.raises(mba.MbaError) as e:
+        mba.solve_batched([ba_problem(mba, c1), ba_problem(mba, c1), ba_problem(mba, tiny_scene)], opts)
+    assert e.value.status == mba.ERR_UNSUPPORTED and "window 2" in str(e.value)
+    idx = ba_problem(mba, c1)
+    idx.point_index = idx.point_index.copy()
+    idx.point_index[3] = -1
+    with pytest.raises(mba.MbaError) as e:
+        mba.solve_batched([idx], opts)
+    assert e.value.status == mba.ERR_INVALID_ARGUMENT
+    assert mba.solve_batched([], opts) == []
+
+
 def test_ceres_shaped_problem_builds_the_same_arrays(mba, c1):
     prob = mba.Problem()
     cams = [c1["cam_init"][i].copy() for i in range(c1["cam_init"].shape[0])]

@@ -1,0 +1,110 @@
+"""Batched independent windows (mba_solve_batched) against the FP64 oracle,
+the reference's Ceres fixtures and independent mba_solve calls.  Every call
+goes through the C ABI.
+
+Tolerances:
+  FP64 batch   -- per-window final cost within 1e-9 relative of the oracle
+                  and of a lone mba_solve, same step counts and termination
+                  (the batch only reorders floating-point sums);
+  mixed batch  -- per-window final cost within 1e-6 relative of the FP64
+                  oracle (the north-star bar for the mixed pipeline).
+"""
+import numpy as np
+import pytest
+
+from conftest import ba_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def windows(mba, n, first=0):
+    return [mba.generate_scene(4, w) for w in range(first, first + n)]
+
+
+def oracle_solve(O, d, huber, const=None, **kw):
+    so, _ = O.solve(O.Scene(d, cam_constant=const), O.default_options(linear_solver=1, huber_a=huber, **kw))
+    return so
+
+
+@pytest.mark.parametrize("huber", [0.0, 4.0])
+def test_fp64_batch_matches_oracle_and_single_solves(mba, O, c1, huber_scene, huber):
+    """Uneven windows (16..40 cameras, one with a constant camera) solved
+    together agree window by window with the oracle and with lone solves."""
+    const = np.zeros(30, np.uint8)
+    const[3] = 1
+    scenes = [(c1, None), (huber_scene, const)] + [(d, None) for d in windows(mba, 4)]
+    opts = mba.default_options(linear_solver=1, huber_a=huber, max_num_iterations=20)
+    probs = [ba_problem(mba, d, k) for d, k in scenes]
+    sums = mba.solve_batched(probs, opts)
+    for w, ((d, k), s) in enumerate(zip(scenes, sums)):
+        lone = ba_problem(mba, d, k)
+        s1 = mba.solve(lone, opts)
+        so = oracle_solve(O, d, huber, k, max_num_iterations=20)
+        assert s.IsSolutionUsable(), w
+        assert abs(s.final_cost - so.final_cost) <= 1e-9 * so.final_cost, w
+        assert abs(s.final_cost - s1.final_cost) <= 1e-9 * s1.final_cost, w
+        assert (s.num_successful_steps, s.num_unsuccessful_steps, s.termination_type) == \
+               (so.num_successful_steps, so.num_unsuccessful_steps, so.termination_type), w
+        assert (s.num_e_blocks, s.num_f_blocks) == (so.num_e_blocks, so.num_f_blocks), w
+        np.testing.assert_allclose(probs[w].cameras, lone.cameras, rtol=1e-7, atol=1e-9)
+        np.testing.assert_allclose(probs[w].points, lone.points, rtol=1e-7, atol=1e-9)
+
+
+def test_fp64_batch_matches_reference_ceres_fixture(mba, golden, c1):
+    """The 2,000-point scene as one window among others reproduces the
+    reference Ceres ITERATIVE_SCHUR run's final cost and step counts."""
+    probs = [ba_problem(mba, d) for d in windows(mba, 2)] + [ba_problem(mba, c1)]
+    sums = mba.solve_batched(probs, mba.default_options(linear_solver=1, max_num_iterations=20))
+    s = sums[2]
+    assert s.num_successful_steps == int(golden["c1_iterative_successful"])
+    assert s.num_unsuccessful_steps == int(golden["c1_iterative_unsuccessful"])
+    assert s.termination_type == int(golden["c1_iterative_termination"])
+    np.testing.assert_allclose(s.final_cost, float(golden["c1_iterative_final_cost"]), rtol=1e-9)
+    np.testing.assert_allclose(probs[2].cameras, golden["c1_iterative_cameras"], rtol=1e-6, atol=1e-8)
+
+
+def test_mixed_batch_reaches_fp64_final_cost(mba, O):
+    ds = windows(mba, 12, first=100)
+    probs = [ba_problem(mba, d) for d in ds]
+    sums = mba.solve_batched(probs, mba.default_options(linear_solver=1, precision=mba.PRECISION_MIXED))
+    for w, (d, s) in enumerate(zip(ds, sums)):
+        so = oracle_solve(O, d, 0.0)
+        assert s.IsSolutionUsable(), w
+        assert abs(s.final_cost - so.final_cost) <= 1e-6 * so.final_cost, w
+
+
+def test_explicit_batch_solves_windows_in_turn(mba, O):
+    ds = windows(mba, 3, first=7)
+    probs = [ba_problem(mba, d) for d in ds]
+    sums = mba.solve_batched(probs, mba.default_options(linear_solver=0))
+    for d, s in zip(ds, sums):
+        so, _ = O.solve(O.Scene(d), O.default_options(linear_solver=0))
+        np.testing.assert_allclose(s.final_cost, so.final_cost, rtol=1e-9)
+
+
+def test_batch_with_empty_and_tiny_windows(mba, O):
+    """A window without observations converges at once; its neighbours are
+    unaffected."""
+    empty = mba.BAProblem(np.zeros((2, 9)), np.zeros((3, 3)), np.zeros((0, 2)), np.zeros(0, np.int32),
+                          np.zeros(0, np.int32))
+    d = O.generate(O.make_spec(3, 6, 3, 3, seed=9))
+    probs = [ba_problem(mba, d), empty, ba_problem(mba, d)]
+    sums = mba.solve_batched(probs, mba.default_options(linear_solver=1))
+    so, _ = O.solve(O.Scene(d), O.default_options(linear_solver=1))
+    assert sums[1].IsSolutionUsable() and sums[1].final_cost == 0.0
+    for w in (0, 2):  # this scene fits to ~1e-16: compare against the initial cost
+        np.testing.assert_allclose(sums[w].final_cost, so.final_cost, rtol=1e-9, atol=1e-12 * so.initial_cost)
+
+
+def test_full_batch_512_windows(mba):
+    """All 512 windows of uneven size (5-40 cameras, 200-5,000 points): every
+    window usable with a lower cost, and a sample agrees with lone solves."""
+    ds = windows(mba, 512)
+    opts = mba.default_options(linear_solver=1, precision=mba.PRECISION_MIXED)
+    probs = [ba_problem(mba, d) for d in ds]
+    sums = mba.solve_batched(probs, opts)
+    assert all(s.IsSolutionUsable() for s in sums)
+    assert all(s.final_cost < s.initial_cost for s in sums)
+    for w in (0, 255, 511):
+        s1 = mba.solve(ba_problem(mba, ds[w]), opts)
+        assert abs(sums[w].final_cost - s1.final_cost) <= 1e-6 * s1.final_cost, w
