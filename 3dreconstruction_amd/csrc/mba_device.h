@@ -1382,190 +1382,189 @@ __global__ __launch_bounds__(kBlock) void k_accept_cams(int nf, Win win, const i
 }
 
 // ----------------------------------------------------------- explicit S
-// Dense lower-triangular S (column-major, n = 9 Nf): diagonal blocks U + D^2.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_s_diag(int nf, const T* __restrict__ U,
-                                                   const double* __restrict__ diag_c,
-                                                   double inv_radius, int64_t n, T* __restrict__ S) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 81 * nf) return;
-  const int f = i / 81, a = (i % 81) / 9, b = i % 9;
-  if (a < b) return;
-  T v = U[45 * (size_t)f + ut9(b, a)];
-  if (a == b) v += (T)(diag_c[9 * (size_t)f + a] * inv_radius);
-  const int64_t row = 9 * (int64_t)f + a, col = 9 * (int64_t)f + b;
-  S[row + col * n] = v;
-}
+// Dense lower-triangular S (column-major, n = 9 Nf):
+//   S(fa, fb) = [fa == fb] (U + D^2) - sum over observation rows a, b of a
+//               common point with cameras fa >= fb of  W_a Vinv W_b',
+// W_r = Jc_r' Jp_r (9 x 3, Jacobi-scaled columns).
 
-// One wave per E block: W_a = Jc_a' Jp_a (9x3) and Y_a = W_a Vinv in LDS,
-// then S(f_a, f_b) -= Y_a W_b' for f_a >= f_b via atomics.
+// Per point-major row: W_r and Y_r = W_r Vinv_e, kWY values per row
+// (W at 0, Y at kWYY, row-major 9 x 3, padded for 16-byte loads).
+constexpr int kWY = 56, kWYY = 28;
 template <typename T>
-__global__ __launch_bounds__(64) void k_s_assemble(
-    int ne, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+__global__ __launch_bounds__(kBlock) void k_s_rows(
+    Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const int32_t* __restrict__ cam_to_f, const T* __restrict__ row_obs, const T* __restrict__ pkX,
     const T* __restrict__ pscale, const T* __restrict__ cscale, const T* __restrict__ ctab,
-    const T* __restrict__ Vinv, T huber_a, int64_t n, T* __restrict__ S) {
-  constexpr int kMaxT = 128;
-  __shared__ T W[kMaxT][27];
-  __shared__ T Y[kMaxT][27];
-  __shared__ int F[kMaxT];
-  const int e = blockIdx.x;
-  if (e >= ne) return;
-  const int r0 = pt_start[e], t = pt_start[e + 1] - r0;
-  const int lane = threadIdx.x;
+    const T* __restrict__ Vinv, T huber_a, T* __restrict__ WY) {
+  const SegPos q = seg_locate(sg);
+  if (!q.valid) return;
+  const int e = q.e;
   T X[3], unused[3];
   load_pk(pkX, e, X, unused);
   const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
   const T* iv = Vinv + 6 * (size_t)e;
   const T Vi[9] = {iv[0], iv[1], iv[2], iv[1], iv[3], iv[4], iv[2], iv[4], iv[5]};
-  for (int base = 0; base < t; base += kMaxT) {
-    const int tb = min(kMaxT, t - base);
-    for (int a = lane; a < tb; a += 64) {
-      const int row = r0 + base + a;
-      const int cam = row_cam[row];
-      const int f = cam_to_f[cam];
-      F[a] = f;
-      if (f < 0) continue;
-      const T* tab = ctab + (size_t)kCamTab * cam;
-      CamProj<T> c;
-      load_proj(tab, c);
-      Proj<T> o;
-      project(c, X, row_obs[2 * (size_t)row], row_obs[2 * (size_t)row + 1], o);
-      const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
-      T Jp[6], Jc[18], cs[9];
-      for (int k = 0; k < 9; ++k) cs[k] = cscale[9 * (size_t)f + k];
-      jac_point(o, c.R, wgt, sp, Jp);
-      jac_cam(o, tab + kTabDR, X, wgt, cs, Jc);
-      for (int i = 0; i < 9; ++i)
-        for (int j = 0; j < 3; ++j) W[a][3 * i + j] = Jc[i] * Jp[j] + Jc[9 + i] * Jp[3 + j];
-      for (int i = 0; i < 9; ++i)
-        for (int j = 0; j < 3; ++j)
-          Y[a][3 * i + j] = W[a][3 * i] * Vi[j] + W[a][3 * i + 1] * Vi[3 + j] + W[a][3 * i + 2] * Vi[6 + j];
-    }
-    __syncthreads();
-    // Pairs (a, b) within this slice, plus cross pairs with earlier slices
-    // are not needed when t <= kMaxT (the host enforces it).
-    const int npair = tb * tb;
-    for (int pidx = lane; pidx < npair; pidx += 64) {
-      const int a = pidx / tb, b = pidx % tb;
-      const int fa = F[a], fb = F[b];
-      if (fa < 0 || fb < 0 || fa < fb || (fa == fb && a < b)) continue;
-      // Block (fa, fb): rows 9 fa + i, cols 9 fb + j ; lower triangle only.
-      for (int i = 0; i < 9; ++i)
-        for (int j = 0; j < 9; ++j) {
-          const int64_t row = 9 * (int64_t)fa + i, col = 9 * (int64_t)fb + j;
-          if (row < col) continue;
-          T v = Y[a][3 * i] * W[b][3 * j] + Y[a][3 * i + 1] * W[b][3 * j + 1] + Y[a][3 * i + 2] * W[b][3 * j + 2];
-          if (fa == fb && a != b)  // both orders of an in-camera pair
-            v += Y[b][3 * i] * W[a][3 * j] + Y[b][3 * i + 1] * W[a][3 * j + 1] + Y[b][3 * i + 2] * W[a][3 * j + 2];
-          atomicAdd(S + row + col * n, -v);
-        }
-    }
-    __syncthreads();
+  for (int r = pt_start[e] + q.sub; r < pt_start[e + 1]; r += q.lanes) {
+    const int cam = row_cam[r];
+    const int f = cam_to_f[cam];
+    if (f < 0) continue;  // constant camera: no column in S
+    const T* tab = ctab + (size_t)kCamTab * cam;
+    CamProj<T> c;
+    load_proj(tab, c);
+    Proj<T> o;
+    project(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
+    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    T Jp[6], Jc[18], cs[9];
+    for (int k = 0; k < 9; ++k) cs[k] = cscale[9 * (size_t)f + k];
+    jac_point(o, c.R, wgt, sp, Jp);
+    jac_cam(o, tab + kTabDR, X, wgt, cs, Jc);
+    T* out = WY + (size_t)kWY * r;
+    T w[27];
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) w[3 * i + j] = Jc[i] * Jp[j] + Jc[9 + i] * Jp[3 + j];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) out[k] = w[k];
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        out[kWYY + 3 * i + j] = w[3 * i] * Vi[j] + w[3 * i + 1] * Vi[3 + j] + w[3 * i + 2] * Vi[6 + j];
   }
 }
 
-// ------------------------------------------------- dense Cholesky (lower)
-// Right-looking blocked LLT of a column-major SPD matrix, NB = 64.
-constexpr int kNB = 64;
-
-// Factor the kb x kb diagonal block at (k, k) in LDS.
+// One 64-lane workgroup per block (fa, fb), fa >= fb, that some point
+// couples: the block's row pairs (a, b) are staged 64 at a time in LDS and
+// lane l sums entries l and l + 64 of the 9 x 9 product; the block is then
+// written once (no atomics, fixed summation order) together with its
+// mirror above the diagonal, so S is stored whole.  Diagonal blocks add
+// U + D^2.
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ A, int64_t n, int k, int kb,
-                                                    int32_t* __restrict__ info) {
-  __shared__ T L[kNB][kNB + 1];
-  for (int idx = threadIdx.x; idx < kb * kb; idx += blockDim.x) {
-    const int i = idx % kb, j = idx / kb;
-    L[i][j] = A[(k + i) + (int64_t)(k + j) * n];
-  }
-  __syncthreads();
-  for (int j = 0; j < kb; ++j) {
-    if (threadIdx.x == 0) {
-      const T d = L[j][j];
-      if (!(d > T(0))) {
-        if (*info == 0) *info = k + j + 1;
-        L[j][j] = T(1);
-      } else {
-        L[j][j] = sqrt(d);
+__global__ __launch_bounds__(64) void k_s_blocks(
+    const int64_t* __restrict__ blk_start, const int32_t* __restrict__ blk_fa,
+    const int32_t* __restrict__ blk_fb, const int32_t* __restrict__ pair_a,
+    const int32_t* __restrict__ pair_b, const T* __restrict__ WY, const T* __restrict__ U,
+    const double* __restrict__ diag_c, double inv_radius, int64_t n, T* __restrict__ S) {
+  __shared__ T Ys[64][kWYY];
+  __shared__ T Ws[64][kWYY];
+  const int blk = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t p0 = blk_start[blk], p1 = blk_start[blk + 1];
+  const int fa = blk_fa[blk], fb = blk_fb[blk];
+  const int i0 = lane / 9, j0 = lane % 9;
+  const int e1 = lane + 64, i1 = e1 / 9, j1 = e1 % 9;
+  const bool two = e1 < 81;
+  T acc0 = T(0), acc1 = T(0);
+  for (int64_t base = p0; base < p1; base += 64) {
+    const int np = (int)min((int64_t)64, p1 - base);
+    if (lane < np) {
+      const T* ya = WY + (size_t)kWY * pair_a[base + lane] + kWYY;
+      const T* wb = WY + (size_t)kWY * pair_b[base + lane];
+#pragma unroll
+      for (int k = 0; k < 27; ++k) {
+        Ys[lane][k] = ya[k];
+        Ws[lane][k] = wb[k];
       }
     }
     __syncthreads();
-    const T djj = L[j][j];
-    for (int i = j + 1 + threadIdx.x; i < kb; i += blockDim.x) L[i][j] /= djj;
-    __syncthreads();
-    // Trailing update of the block (lower part).
-    for (int idx = threadIdx.x; idx < (kb - j - 1) * (kb - j - 1); idx += blockDim.x) {
-      const int ii = j + 1 + idx % (kb - j - 1), jj = j + 1 + idx / (kb - j - 1);
-      if (ii >= jj) L[ii][jj] -= L[ii][j] * L[jj][j];
+    for (int p = 0; p < np; ++p) {
+      acc0 += Ys[p][3 * i0] * Ws[p][3 * j0] + Ys[p][3 * i0 + 1] * Ws[p][3 * j0 + 1] +
+              Ys[p][3 * i0 + 2] * Ws[p][3 * j0 + 2];
+      if (two)
+        acc1 += Ys[p][3 * i1] * Ws[p][3 * j1] + Ys[p][3 * i1 + 1] * Ws[p][3 * j1 + 1] +
+                Ys[p][3 * i1 + 2] * Ws[p][3 * j1 + 2];
     }
     __syncthreads();
   }
-  for (int idx = threadIdx.x; idx < kb * kb; idx += blockDim.x) {
-    const int i = idx % kb, j = idx / kb;
-    if (i >= j) A[(k + i) + (int64_t)(k + j) * n] = L[i][j];
-  }
+  auto put = [&](int i, int j, T acc) {
+    T base = T(0);
+    if (fa == fb) {
+      base = U[45 * (size_t)fa + (i <= j ? ut9(i, j) : ut9(j, i))];
+      if (i == j) base += (T)(diag_c[9 * (size_t)fa + i] * inv_radius);
+    }
+    const int64_t row = 9 * (int64_t)fa + i, col = 9 * (int64_t)fb + j;
+    S[row + col * n] = base - acc;
+    if (fa != fb) S[col + row * n] = base - acc;
+  };
+  put(i0, j0, acc0);
+  if (two) put(i1, j1, acc1);
 }
 
-// Panel: A[k+kb:n, k:k+kb] <- A[...] L11^-T; one 64-row slab per workgroup,
-// one thread per row.
+// ------------------------------------------------- dense Cholesky (lower)
+// Right-looking blocked LLT of a column-major SPD matrix, NB = 64.  Step k:
+//   k_potrf_diag   factors the 64 x 64 diagonal block and inverts it;
+//   k_panel_mfma   L21 = A21 L11^-T as a GEMM with the inverted block;
+//   k_syrk_mfma_*  A22 -= L21 L21'.
+// The triangular solves with the factor run one launch per 64-row block.
+constexpr int kNB = 64;
+
+// One wave: factor the kb x kb diagonal block at (k, k) with lane i holding
+// row i in registers (columns exchanged by shuffles), write it back, and
+// write L11^-1 (64 x 64 column-major, zero-padded) from an LDS copy.  Rows
+// and columns past kb are padded with the identity.
 template <typename T>
-__global__ __launch_bounds__(64) void k_trsm_panel(T* __restrict__ A, int64_t n, int k, int kb) {
-  __shared__ T L[kNB][kNB + 1];
-  for (int idx = threadIdx.x; idx < kb * kb; idx += blockDim.x) {
-    const int i = idx % kb, j = idx / kb;
-    L[i][j] = (i >= j) ? A[(k + i) + (int64_t)(k + j) * n] : T(0);
+__global__ __launch_bounds__(64) void k_potrf_diag(T* __restrict__ A, int64_t n, int k, int kb,
+                                                   T* __restrict__ Linv, int32_t* __restrict__ info) {
+  __shared__ T D[kNB][kNB + 1];
+  const int lane = threadIdx.x;
+  T a[kNB];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c)
+    a[c] = (lane < kb && c < kb) ? A[(k + lane) + (int64_t)(k + c) * n] : (lane == c ? T(1) : T(0));
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    T d = __shfl(a[j], j, 64);
+    if (!(d > T(0))) {
+      bad = bad ? bad : j + 1;
+      d = T(1);
+    }
+    const T sj = sqrt(d);
+    if (lane == j) a[j] = sj;
+    if (lane > j) a[j] /= sj;
+    const T lij = lane > j ? a[j] : T(0);
+#pragma unroll
+    for (int c = j + 1; c < kNB; ++c) {
+      const T lcj = __shfl(a[j], c, 64);  // L[c][j]
+      if (c <= lane) a[c] -= lij * lcj;
+    }
+  }
+  if (lane == 0 && bad && bad <= kb && *info == 0) *info = k + bad;
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) {
+    D[lane][c] = c <= lane ? a[c] : T(0);
+    if (lane < kb && c < kb && c <= lane) A[(k + lane) + (int64_t)(k + c) * n] = a[c];
   }
   __syncthreads();
-  const int64_t row = (int64_t)k + kb + (int64_t)blockIdx.x * 64 + threadIdx.x;
-  if (row >= n) return;
+  // Column `lane` of L11^-1 by forward substitution (reads broadcast).
+  const int c = lane;
   T x[kNB];
-  for (int j = 0; j < kb; ++j) x[j] = A[row + (int64_t)(k + j) * n];
-  for (int j = 0; j < kb; ++j) {
-    T s = x[j];
-    for (int l = 0; l < j; ++l) s -= x[l] * L[j][l];
-    x[j] = s / L[j][j];
+#pragma unroll
+  for (int r = 0; r < kNB; ++r) {
+    T v = (r == c) ? T(1) : T(0);
+#pragma unroll
+    for (int q = 0; q < r; ++q) v -= D[r][q] * x[q];
+    x[r] = (r >= c) ? v / D[r][r] : T(0);
   }
-  for (int j = 0; j < kb; ++j) A[row + (int64_t)(k + j) * n] = x[j];
+#pragma unroll
+  for (int r = 0; r < kNB; ++r) Linv[r + kNB * c] = (r < kb && c < kb) ? x[r] : T(0);
 }
 
-// Trailing update A22 -= L21 L21' on MFMA: 64 x 64 lower tiles, four waves
-// per tile, one 32 x 32 quadrant per wave, K = kb (<= 64, zero-padded).
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// 64 x 64 tile product D = As Bs' (As, Bs: 64 x kpad in LDS, row-major) on
+// MFMA, four waves, one 32 x 32 quadrant each; `put(row, col, v)` receives
+// every element of the wave's quadrant.
 //   FP32: v_mfma_f32_32x32x2_f32 -- A operand lane l = A[l % 32][k + l / 32],
 //         B operand lane l = B[k + l / 32][l % 32], result register r of lane
 //         l = D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
 //   FP64: v_mfma_f64_16x16x4_f64 -- A lane l = A[l % 16][k + l / 16], B lane
 //         l = B[k + l / 16][l % 16], result r = D[l / 16 + 4 r][l % 16]
 //         (both result layouts checked on gfx950 with a lane probe).
-typedef float floatx16 __attribute__((ext_vector_type(16)));
-typedef double doublex4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ void syrk_tile_coords(int t, int& ti, int& tj) {
-  ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-  while (ti * (ti + 1) / 2 > t) --ti;
-  tj = t - ti * (ti + 1) / 2;
-}
-
-template <typename T>
-__device__ __forceinline__ void syrk_load(T (*As)[kNB + 1], T (*Bs)[kNB + 1], const T* __restrict__ A,
-                                          int64_t n, int k, int kb, int64_t r0, int64_t c0) {
-  for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) {
-    const int i = idx % kNB, l = idx / kNB;
-    const bool kin = l < kb;
-    As[i][l] = (kin && r0 + i < n) ? A[(r0 + i) + (int64_t)(k + l) * n] : T(0);
-    Bs[i][l] = (kin && c0 + i < n) ? A[(c0 + i) + (int64_t)(k + l) * n] : T(0);
-  }
-  __syncthreads();
-}
-
-__global__ __launch_bounds__(256) void k_syrk_mfma_f32(float* __restrict__ A, int64_t n, int k, int kb) {
-  __shared__ float As[kNB][kNB + 1];
-  __shared__ float Bs[kNB][kNB + 1];
-  int ti, tj;
-  syrk_tile_coords(blockIdx.x, ti, tj);
-  const int64_t base = (int64_t)k + kb;
-  const int64_t r0 = base + (int64_t)ti * kNB, c0 = base + (int64_t)tj * kNB;
-  syrk_load(As, Bs, A, n, k, kb, r0, c0);
+template <typename F>
+__device__ __forceinline__ void tile_nt(const float (*As)[kNB + 1], const float (*Bs)[kNB + 1], int kb, F&& put) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int qi = w >> 1, qj = w & 1;
   floatx16 acc;
@@ -1576,21 +1575,10 @@ __global__ __launch_bounds__(256) void k_syrk_mfma_f32(float* __restrict__ A, in
     const float b = Bs[qj * 32 + (lane & 31)][kk + (lane >> 5)];
     acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
   }
-  for (int r = 0; r < 16; ++r) {
-    const int64_t row = r0 + qi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    const int64_t col = c0 + qj * 32 + (lane & 31);
-    if (row < n && col < n && row >= col) A[row + col * n] -= acc[r];
-  }
+  for (int r = 0; r < 16; ++r) put(qi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), qj * 32 + (lane & 31), acc[r]);
 }
-
-__global__ __launch_bounds__(256) void k_syrk_mfma_f64(double* __restrict__ A, int64_t n, int k, int kb) {
-  __shared__ double As[kNB][kNB + 1];
-  __shared__ double Bs[kNB][kNB + 1];
-  int ti, tj;
-  syrk_tile_coords(blockIdx.x, ti, tj);
-  const int64_t base = (int64_t)k + kb;
-  const int64_t r0 = base + (int64_t)ti * kNB, c0 = base + (int64_t)tj * kNB;
-  syrk_load(As, Bs, A, n, k, kb, r0, c0);
+template <typename F>
+__device__ __forceinline__ void tile_nt(const double (*As)[kNB + 1], const double (*Bs)[kNB + 1], int kb, F&& put) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int qi = w >> 1, qj = w & 1;
   doublex4 acc[2][2];
@@ -1609,108 +1597,181 @@ __global__ __launch_bounds__(256) void k_syrk_mfma_f64(double* __restrict__ A, i
   }
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = r0 + qi * 32 + a * 16 + (lane >> 4) + 4 * r;
-        const int64_t col = c0 + qj * 32 + b * 16 + (lane & 15);
-        if (row < n && col < n && row >= col) A[row + col * n] -= acc[a][b][r];
-      }
+      for (int r = 0; r < 4; ++r)
+        put(qi * 32 + a * 16 + (lane >> 4) + 4 * r, qj * 32 + b * 16 + (lane & 15), acc[a][b][r]);
 }
 
-// Triangular solves with the factor (one workgroup; columns blocked by 64,
-// each diagonal block solved by wave 0 from LDS).  Forward: L y = b.
+// Rows r0.. of columns k..k+kb of A into a 64 x 64 LDS tile (zero-padded).
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_lower(const T* __restrict__ L, int64_t n,
-                                                    T* __restrict__ b) {
-  __shared__ T D[kNB][kNB + 1];
-  __shared__ T xs[kNB];
-  const int lane = threadIdx.x & 63;
-  for (int64_t k = 0; k < n; k += kNB) {
-    const int kb = (int)min((int64_t)kNB, n - k);
-    for (int idx = threadIdx.x; idx < kb * kb; idx += blockDim.x) {
-      const int i = idx % kb, j = idx / kb;
-      D[i][j] = L[(k + i) + (k + j) * n];
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      T bi = lane < kb ? b[k + lane] : T(0);
-      for (int j = 0; j < kb; ++j) {
-        const T xj = __shfl(bi, j, 64) / D[j][j];
-        if (lane == j) bi = xj;
-        if (lane > j && lane < kb) bi -= D[lane][j] * xj;
-      }
-      if (lane < kb) {
-        xs[lane] = bi;
-        b[k + lane] = bi;
-      }
-    }
-    __syncthreads();
-    for (int64_t i = k + kb + threadIdx.x; i < n; i += blockDim.x) {
-      T s = T(0);
-      for (int l = 0; l < kb; ++l) s += L[i + (k + l) * n] * xs[l];
-      b[i] -= s;
-    }
-    __syncthreads();
+__device__ __forceinline__ void load_cols(T (*Ts)[kNB + 1], const T* __restrict__ A, int64_t n, int k, int kb,
+                                          int64_t r0) {
+  // 256 threads: all 16 loads of a thread in flight before the LDS stores.
+  T v[kNB * kNB / 256];
+#pragma unroll
+  for (int q = 0; q < kNB * kNB / 256; ++q) {
+    const int idx = threadIdx.x + 256 * q, i = idx % kNB, l = idx / kNB;
+    v[q] = (l < kb && r0 + i < n) ? A[(r0 + i) + (int64_t)(k + l) * n] : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < kNB * kNB / 256; ++q) {
+    const int idx = threadIdx.x + 256 * q;
+    Ts[idx % kNB][idx / kNB] = v[q];
   }
 }
 
-// Backward: L' x = y, in place.
+// Panel: A[k+kb:n, k:k+kb] <- A[...] L11^-T, one 64-row slab per workgroup.
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_upper(const T* __restrict__ L, int64_t n,
-                                                    T* __restrict__ b) {
-  __shared__ T D[kNB][kNB + 1];
-  __shared__ T red[4][kNB];
-  const int64_t nblk = (n + kNB - 1) / kNB;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int64_t bi = nblk - 1; bi >= 0; --bi) {
-    const int64_t k = bi * kNB;
-    const int kb = (int)min((int64_t)kNB, n - k);
-    // b[k+j] -= sum_{i >= k+kb} L[i, k+j] b[i]: thread i-strided, kb sums.
-    T acc[kNB];
+__global__ __launch_bounds__(256) void k_panel_mfma(T* __restrict__ A, int64_t n, int k, int kb,
+                                                    const T* __restrict__ Linv) {
+  __shared__ T As[kNB][kNB + 1];
+  __shared__ T Bs[kNB][kNB + 1];
+  const int64_t r0 = (int64_t)k + kb + (int64_t)blockIdx.x * kNB;
+  load_cols(As, A, n, k, kb, r0);
+  load_cols(Bs, Linv, kNB, 0, kNB, 0);
+  __syncthreads();
+  tile_nt(As, Bs, kb, [&](int i, int j, T v) {
+    if (r0 + i < n && j < kb) A[(r0 + i) + (int64_t)(k + j) * n] = v;
+  });
+}
+
+__device__ __forceinline__ void syrk_tile_coords(int t, int& ti, int& tj) {
+  ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  while (ti * (ti + 1) / 2 > t) --ti;
+  tj = t - ti * (ti + 1) / 2;
+}
+
+// Trailing update A22 -= L21 L21' on 64 x 64 lower tiles (MFMA).
+template <typename T>
+__global__ __launch_bounds__(256) void k_syrk_mfma(T* __restrict__ A, int64_t n, int k, int kb) {
+  __shared__ T As[kNB][kNB + 1];
+  __shared__ T Bs[kNB][kNB + 1];
+  int ti, tj;
+  syrk_tile_coords(blockIdx.x, ti, tj);
+  const int64_t base = (int64_t)k + kb;
+  const int64_t r0 = base + (int64_t)ti * kNB, c0 = base + (int64_t)tj * kNB;
+  load_cols(As, A, n, k, kb, r0);
+  load_cols(Bs, A, n, k, kb, c0);
+  __syncthreads();
+  tile_nt(As, Bs, kb, [&](int i, int j, T v) {
+    const int64_t row = r0 + i, col = c0 + j;
+    if (row < n && col < n && row >= col) A[row + col * n] -= v;
+  });
+}
+
+// Forward solve L y = b in place, step k (k = -1 first): the workgroup of
+// 64-row block i > k subtracts L[i, k] y_k; the one for block k + 1 then
+// solves its diagonal block.  Launch over blocks k+1 .. nb-1.
+template <typename T>
+__global__ __launch_bounds__(64) void k_trsv_lower_step(const T* __restrict__ L, int64_t n, int k,
+                                                        T* __restrict__ b) {
+  __shared__ T xs[kNB];
+  const int lane = threadIdx.x;
+  const int i = k + 1 + blockIdx.x;
+  const int64_t i0 = (int64_t)i * kNB;
+  const int ib = (int)min((int64_t)kNB, n - i0);
+  const bool on = lane < ib;
+  T bi = on ? b[i0 + lane] : T(0);
+  if (k >= 0) {
+    const int64_t k0 = (int64_t)k * kNB;
+    xs[lane] = b[k0 + lane];  // block k is full (it precedes block i)
+    __syncthreads();
+    const T* col = L + (on ? i0 + lane : i0) + k0 * n;
+    T acc[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-    for (int j = 0; j < kNB; ++j) acc[j] = T(0);
-    for (int64_t i = k + kb + threadIdx.x; i < n; i += blockDim.x) {
-      const T bv = b[i];
+    for (int c = 0; c < kNB; ++c) acc[c & 3] += col[c * n] * xs[c];
+    if (on) bi -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  }
+  if (i == k + 1) {
+    // Diagonal block: lane r's row into registers, then the serial solve.
+    T l[kNB];
 #pragma unroll
-      for (int j = 0; j < kNB; ++j)
-        if (j < kb) acc[j] += L[i + (k + j) * n] * bv;
-    }
+    for (int j = 0; j < kNB; ++j) l[j] = (on && j < ib) ? L[(i0 + lane) + (i0 + j) * n] : T(1);
 #pragma unroll
     for (int j = 0; j < kNB; ++j) {
-      const T v = wave_sum(acc[j]);
-      if (lane == 0) red[wave][j] = v;
+      const T xj = __shfl(bi, j, 64) / __shfl(l[j], j, 64);
+      if (lane == j) bi = xj;
+      if (lane > j) bi -= l[j] * xj;
     }
-    for (int idx = threadIdx.x; idx < kb * kb; idx += blockDim.x) {
-      const int i = idx % kb, j = idx / kb;
-      D[i][j] = L[(k + i) + (k + j) * n];
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      T bj = lane < kb ? b[k + lane] - (red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]) : T(0);
-      // Column-oriented back substitution with L' (row j of L' = column j of L).
-      for (int j = kb - 1; j >= 0; --j) {
-        const T xj = __shfl(bj, j, 64) / D[j][j];
-        if (lane == j) bj = xj;
-        if (lane < j) bj -= D[j][lane] * xj;
-      }
-      if (lane < kb) b[k + lane] = bj;
-    }
-    __syncthreads();
   }
+  if (on) b[i0 + lane] = bi;
 }
 
-// FP64 residual r = b - S x with S symmetric stored lower (column-major T).
+// Backward solve L' x = y in place, step k (k = nb first): the workgroup of
+// block i < k subtracts L[k, i]' x_k (tile staged through LDS); the one for
+// block k - 1 then solves its diagonal block.  Launch over blocks 0 .. k-1.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sym_residual64(const T* __restrict__ S, int64_t n,
-                                                           const double* __restrict__ b,
-                                                           const double* __restrict__ x,
-                                                           double* __restrict__ r) {
+__global__ __launch_bounds__(64) void k_trsv_upper_step(const T* __restrict__ L, int64_t n, int k, int nb,
+                                                        int i_first, T* __restrict__ b) {
+  __shared__ T tile[kNB][kNB + 1];
+  __shared__ T xs[kNB];
+  const int lane = threadIdx.x;
+  const int i = i_first + blockIdx.x;
+  const int64_t i0 = (int64_t)i * kNB;
+  const int ib = (int)min((int64_t)kNB, n - i0);
+  T yi = lane < ib ? b[i0 + lane] : T(0);
+  // Stage the 64 x ib tile L[r0 .. r0+63, i0 .. i0+ib) (rows past n: 0,
+  // columns past ib: identity-padded) through registers into LDS.
+  auto stage = [&](int64_t r0, int rb) {
+    T v[kNB];
+    const bool on = lane < rb;
+    const T* row = L + (on ? r0 + lane : r0) + i0 * n;
+#pragma unroll
+    for (int c = 0; c < kNB; ++c) v[c] = (on && c < ib) ? row[c * n] : T(0);
+#pragma unroll
+    for (int c = 0; c < kNB; ++c) tile[lane][c] = v[c];
+  };
+  if (k < nb) {
+    const int64_t k0 = (int64_t)k * kNB;
+    const int kb = (int)min((int64_t)kNB, n - k0);
+    xs[lane] = lane < kb ? b[k0 + lane] : T(0);
+    stage(k0, kb);
+    __syncthreads();
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int r = 0; r < kNB; ++r) acc[r & 3] += tile[r][lane] * xs[r];
+    yi -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    __syncthreads();
+  }
+  if (i == k - 1) {
+    stage(i0, ib);
+    __syncthreads();
+    for (int j = ib - 1; j >= 0; --j) {
+      const T xj = __shfl(yi, j, 64) / tile[j][j];
+      if (lane == j) yi = xj;
+      if (lane < j) yi -= tile[j][lane] * xj;
+    }
+  }
+  if (lane < ib) b[i0 + lane] = yi;
+}
+
+// FP64 residual r = b - S x (S column-major, both triangles stored) in two
+// passes: partial row sums over kGemvSplit column ranges (coalesced down
+// the columns), then r = b - the partials summed in a fixed order.
+constexpr int kGemvSplit = 32;
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gemv_partial(const T* __restrict__ S, int64_t n,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ part) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t per = (n + kGemvSplit - 1) / kGemvSplit;
+  const int64_t j0 = blockIdx.y * per, j1 = min(n, j0 + per);
+  if (i >= n) return;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  int64_t j = j0;
+  for (; j + 4 <= j1; j += 4) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] += (double)S[i + (j + q) * n] * x[j + q];
+  }
+  for (; j < j1; ++j) acc[0] += (double)S[i + j * n] * x[j];
+  part[blockIdx.y * n + i] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+__global__ __launch_bounds__(kBlock) void k_gemv_final(int64_t n, const double* __restrict__ part,
+                                                       const double* __restrict__ b, double* __restrict__ r) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (int64_t j = 0; j < n; ++j) {
-    const double a = (i >= j) ? (double)S[i + j * n] : (double)S[j + i * n];
-    s += a * x[j];
-  }
+  for (int q = 0; q < kGemvSplit; ++q) s += part[q * n + i];
   r[i] = b[i] - s;
 }
 

@@ -316,7 +316,7 @@ struct Dev {
   T *x = nullptr, *r = nullptr, *zc = nullptr, *p = nullptr, *q = nullptr, *tmp = nullptr;
   T *step_p = nullptr, *step_c = nullptr;
   // explicit S
-  T *S = nullptr, *L = nullptr, *sv = nullptr;
+  T *S = nullptr, *L = nullptr, *sv = nullptr, *WY = nullptr, *Linv = nullptr;
 };
 
 }  // namespace
@@ -342,7 +342,7 @@ struct mba_context {
   double *cams64 = nullptr, *cams64_t = nullptr, *pts64 = nullptr, *pts64_t = nullptr;
   double *ctab64 = nullptr, *row_obs64 = nullptr, *pscale64 = nullptr, *cscale64 = nullptr;
   double *diag_p = nullptr, *diag_c = nullptr, *red = nullptr, *red2 = nullptr, *stepd = nullptr;
-  double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr;
+  double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr, *gemv_part = nullptr;
   Scalars* sc = nullptr;        // device
   PcgState* pcg = nullptr;      // device, one per window
   int32_t* flag = nullptr;      // device, one per window
@@ -354,6 +354,11 @@ struct mba_context {
   double *inv_r = nullptr, *wacc = nullptr, *wacc2 = nullptr;
   uint8_t* accept = nullptr;
   uint8_t* wactive = nullptr;  // windows still iterating
+  // Explicit Schur (single problem): the blocks of S that points couple and
+  // their observation-row pairs, sorted by block.
+  int64_t* s_blk_start = nullptr;
+  int32_t *s_blk_fa = nullptr, *s_blk_fb = nullptr, *s_pair_a = nullptr, *s_pair_b = nullptr;
+  int64_t s_nblk = 0, s_npairs = 0;
   // Host copies of what was last uploaded (rewritten only while the stream
   // is idle).
   std::vector<double> h_inv_r;
@@ -503,6 +508,107 @@ struct ContextGuard {
 
 std::string window_prefix(int32_t W, int32_t w) {
   return W > 1 ? "window " + std::to_string(w) + ": " : std::string();
+}
+
+// Blocks (fa, fb), fa >= fb, of the reduced camera matrix that points couple,
+// and for each the pairs (a, b) of point-major rows of a common point with
+// F blocks fa and fb (both orders when fa == fb), sorted by block and, in a
+// block, by point and row.  This is the block structure the reference sizes
+// its reduced matrix with (schur_complement_solver.cc:246-313); here it also
+// fixes who computes each block.
+mba_status build_s_pairs(mba_context* c, const std::vector<int32_t>& pt_start,
+                         const std::vector<int32_t>& row_cam) {
+  const int32_t nf = c->nf, ne = c->ne;
+  std::vector<int32_t> row_f(c->nrows);
+  parallel_for(c->nrows, [&](int64_t b, int64_t e) {
+    for (int64_t r = b; r < e; ++r) row_f[r] = c->cam_to_f[row_cam[r]];
+  });
+  const int nt = host_threads(ne, 4096);
+  const int64_t step = ((int64_t)ne + nt - 1) / nt;
+  // Pairs per fa, per thread (point ranges), then bucket offsets.
+  std::vector<std::vector<int64_t>> cnt(nt, std::vector<int64_t>((size_t)nf, 0));
+  auto for_pairs = [&](int t, auto&& fn) {
+    const int64_t end = std::min<int64_t>(ne, (t + 1) * step);
+    for (int64_t e = t * step; e < end; ++e)
+      for (int32_t a = pt_start[e]; a < pt_start[e + 1]; ++a) {
+        const int32_t fa = row_f[a];
+        if (fa < 0) continue;
+        for (int32_t b = pt_start[e]; b < pt_start[e + 1]; ++b) {
+          const int32_t fb = row_f[b];
+          if (fb >= 0 && fb <= fa) fn(fa, a, b);
+        }
+      }
+  };
+  run_threads(nt, [&](int t) { for_pairs(t, [&](int32_t fa, int32_t, int32_t) { cnt[t][fa]++; }); });
+  std::vector<int64_t> fa_start((size_t)nf + 1, 0);
+  int64_t run = 0;
+  for (int32_t fa = 0; fa < nf; ++fa) {
+    fa_start[fa] = run;
+    for (int t = 0; t < nt; ++t) {
+      const int64_t k = cnt[t][fa];
+      cnt[t][fa] = run;
+      run += k;
+    }
+  }
+  fa_start[nf] = run;
+  if (run > 0x7fffffffLL) return fail(MBA_ERR_UNSUPPORTED, "explicit Schur: more than 2^31-1 coupled row pairs");
+  c->s_npairs = run;
+  std::vector<int32_t> pa((size_t)run), pb((size_t)run);
+  run_threads(nt, [&](int t) {
+    for_pairs(t, [&](int32_t fa, int32_t a, int32_t b) {
+      const int64_t k = cnt[t][fa]++;
+      pa[k] = a;
+      pb[k] = b;
+    });
+  });
+  // Within each fa bucket: stable counting sort by fb, block list.
+  std::vector<int32_t> sa((size_t)run), sb((size_t)run);
+  std::vector<std::vector<int32_t>> bfb(nf);
+  std::vector<std::vector<int64_t>> bcnt(nf);
+  {
+    std::atomic<int32_t> next{0};
+    const int ntb = (int)std::min<int64_t>(std::max(1, host_threads(run, 1 << 14)), std::max(1, nf));
+    run_threads(ntb, [&](int) {
+      std::vector<int64_t> h((size_t)nf + 1, 0);
+      for (int32_t fa = next.fetch_add(1); fa < nf; fa = next.fetch_add(1)) {
+        const int64_t b0 = fa_start[fa], b1 = fa_start[fa + 1];
+        for (int64_t k = b0; k < b1; ++k) h[row_f[pb[k]]]++;
+        int64_t o = b0;
+        for (int32_t fb = 0; fb <= fa; ++fb) {
+          const int64_t m = h[fb];
+          if (m == 0) continue;
+          bfb[fa].push_back(fb);
+          bcnt[fa].push_back(m);
+          h[fb] = o;
+          o += m;
+        }
+        for (int64_t k = b0; k < b1; ++k) {
+          const int64_t dst = h[row_f[pb[k]]]++;
+          sa[dst] = pa[k];
+          sb[dst] = pb[k];
+        }
+        for (int32_t fb = 0; fb <= fa; ++fb) h[fb] = 0;
+      }
+    });
+  }
+  std::vector<int64_t> blk_start;
+  std::vector<int32_t> blk_fa, blk_fb;
+  int64_t o = 0;
+  for (int32_t fa = 0; fa < nf; ++fa)
+    for (size_t i = 0; i < bfb[fa].size(); ++i) {
+      blk_start.push_back(o);
+      blk_fa.push_back(fa);
+      blk_fb.push_back(bfb[fa][i]);
+      o += bcnt[fa][i];
+    }
+  blk_start.push_back(o);
+  c->s_nblk = (int64_t)blk_fa.size();
+  MBA_TRY(upload(c, &c->s_blk_start, blk_start));
+  MBA_TRY(upload(c, &c->s_blk_fa, blk_fa));
+  MBA_TRY(upload(c, &c->s_blk_fb, blk_fb));
+  MBA_TRY(upload(c, &c->s_pair_a, sa));
+  MBA_TRY(upload(c, &c->s_pair_b, sb));
+  return hipStreamSynchronize(c->stream) == hipSuccess ? MBA_OK : fail(MBA_ERR_DEVICE, "pair upload failed");
 }
 
 // Preprocess W independent problems into one device layout.
@@ -779,6 +885,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     MBA_TRY(alloc_precision<float>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_chunk_start));
   else
     MBA_TRY(alloc_precision<double>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_chunk_start));
+  if (W == 1 && c->opt.linear_solver == MBA_EXPLICIT_SCHUR) MBA_TRY(build_s_pairs(c, dev_pt_start, row_cam));
   for (int i = 0; i < 4; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
   HIP_TRY(hipStreamSynchronize(c->stream));
   *out = c;
@@ -1014,11 +1121,14 @@ struct Engine {
     return MBA_OK;
   }
 
-  void syrk(float* L, int64_t n, int k, int kb, unsigned tiles) {
-    hipLaunchKernelGGL(k_syrk_mfma_f32, dim3(tiles), dim3(256), 0, s, L, n, k, kb);
-  }
-  void syrk(double* L, int64_t n, int k, int kb, unsigned tiles) {
-    hipLaunchKernelGGL(k_syrk_mfma_f64, dim3(tiles), dim3(256), 0, s, L, n, k, kb);
+  // In-place solves with the factor: L y = v, then L' x = y.
+  void trsv(T* v, int64_t n) {
+    const int nb = (int)((n + kNB - 1) / kNB);
+    for (int k = -1; k < nb - 1; ++k)
+      hipLaunchKernelGGL(k_trsv_lower_step<T>, dim3(nb - 1 - k), dim3(64), 0, s, d.L, n, k, v);
+    hipLaunchKernelGGL(k_trsv_upper_step<T>, dim3(1), dim3(64), 0, s, d.L, n, nb, nb, nb - 1, v);
+    for (int k = nb - 1; k >= 1; --k)
+      hipLaunchKernelGGL(k_trsv_upper_step<T>, dim3(k), dim3(64), 0, s, d.L, n, k, nb, 0, v);
   }
 
   // Explicit Schur (single problem): assemble dense S, Cholesky, solve,
@@ -1030,23 +1140,30 @@ struct Engine {
     if (!d.S) {
       MBA_TRY(c->pool.alloc(&d.S, (size_t)(n * n)));
       MBA_TRY(c->pool.alloc(&d.L, (size_t)(n * n)));
+      MBA_TRY(c->pool.alloc(&d.WY, (size_t)kWY * c->nrows));
+      MBA_TRY(c->pool.alloc(&d.Linv, (size_t)kNB * kNB));
+      if (!c->gemv_part) MBA_TRY(c->pool.alloc(&c->gemv_part, (size_t)kGemvSplit * n));
     }
     const double inv_r = c->h_inv_r[0];
+    // Blocks no point couples stay zero.
     HIP_TRY(hipMemsetAsync(d.S, 0, sizeof(T) * n * n, s));
-    hipLaunchKernelGGL(k_s_diag<T>, dim3(nblocks(81 * (int64_t)c->nf)), dim3(kBlock), 0, s, c->nf, d.U,
-                       c->diag_c, inv_r, n, d.S);
-    hipLaunchKernelGGL(k_s_assemble<T>, dim3(std::max(1, c->ne)), dim3(64), 0, s, c->ne, d.pt_start, d.row_cam,
-                       d.cam_to_f, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab, d.Vinv, huber, n, d.S);
+    if (c->seg_blocks > 0)
+      hipLaunchKernelGGL(k_s_rows<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
+                         d.cam_to_f, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab, d.Vinv, huber, d.WY);
+    if (c->s_nblk > 0)
+      hipLaunchKernelGGL(k_s_blocks<T>, dim3((unsigned)c->s_nblk), dim3(64), 0, s, c->s_blk_start, c->s_blk_fa,
+                         c->s_blk_fb, c->s_pair_a, c->s_pair_b, d.WY, d.U, c->diag_c, inv_r, n, d.S);
     HIP_TRY(hipMemcpyAsync(d.L, d.S, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(c->info, 0, sizeof(int32_t), s));
     for (int64_t k = 0; k < n; k += kNB) {
       const int kb = (int)std::min<int64_t>(kNB, n - k);
-      hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, s, d.L, n, (int)k, kb, c->info);
+      hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(64), 0, s, d.L, n, (int)k, kb, d.Linv, c->info);
       const int64_t rest = n - k - kb;
       if (rest > 0) {
-        hipLaunchKernelGGL(k_trsm_panel<T>, dim3((unsigned)((rest + 63) / 64)), dim3(64), 0, s, d.L, n, (int)k, kb);
         const int nt = (int)((rest + kNB - 1) / kNB);
-        syrk(d.L, n, (int)k, kb, (unsigned)((int64_t)nt * (nt + 1) / 2));
+        hipLaunchKernelGGL(k_panel_mfma<T>, dim3(nt), dim3(256), 0, s, d.L, n, (int)k, kb, (const T*)d.Linv);
+        hipLaunchKernelGGL(k_syrk_mfma<T>, dim3((unsigned)((int64_t)nt * (nt + 1) / 2)), dim3(256), 0, s, d.L, n,
+                           (int)k, kb);
       }
     }
     int32_t info = 0;
@@ -1058,17 +1175,17 @@ struct Engine {
     }
     // x = S^-1 rhs with FP64 iterative refinement of the residual.
     HIP_TRY(hipMemcpyAsync(d.x, d.rhs, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_trsv_lower<T>, dim3(1), dim3(256), 0, s, d.L, n, d.x);
-    hipLaunchKernelGGL(k_trsv_upper<T>, dim3(1), dim3(256), 0, s, d.L, n, d.x);
+    trsv(d.x, n);
     if (c->opt.refinement_iterations > 0) {
       hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, d.rhs, c->b64);
       hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, d.x, c->x64);
       for (int it = 0; it < c->opt.refinement_iterations; ++it) {
-        hipLaunchKernelGGL(k_sym_residual64<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, d.S, n, c->b64, c->x64,
-                           c->r64);
+        hipLaunchKernelGGL(k_gemv_partial<T>, dim3(nblocks(n), kGemvSplit), dim3(kBlock), 0, s, d.S, n,
+                           (const double*)c->x64, c->gemv_part);
+        hipLaunchKernelGGL(k_gemv_final, dim3(nblocks(n)), dim3(kBlock), 0, s, n, (const double*)c->gemv_part,
+                           (const double*)c->b64, c->r64);
         hipLaunchKernelGGL((k_convert<double, T>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, c->r64, d.sv);
-        hipLaunchKernelGGL(k_trsv_lower<T>, dim3(1), dim3(256), 0, s, d.L, n, d.sv);
-        hipLaunchKernelGGL(k_trsv_upper<T>, dim3(1), dim3(256), 0, s, d.L, n, d.sv);
+        trsv(d.sv, n);
         hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, d.sv, c->r64);
         hipLaunchKernelGGL(k_axpy_inplace, dim3(nblocks(n)), dim3(kBlock), 0, s, n, c->r64, c->x64);
       }
@@ -1709,8 +1826,6 @@ mba_status mba_context_solve(mba_context* ctx, mba_summary* summary, mba_iterati
     const double bytes = 2.0 * n * n * (ctx->mixed ? 4.0 : 8.0);
     if (bytes > 64.0 * (1ull << 30))
       return fail(MBA_ERR_UNSUPPORTED, "explicit Schur matrix too large; use MBA_IMPLICIT_SCHUR_PCG");
-    if (ctx->max_track > 128)
-      return fail(MBA_ERR_UNSUPPORTED, "explicit Schur assembly supports tracks of at most 128 observations");
   }
   if (ctx->mixed) {
     Engine<float> e(ctx);
