@@ -743,7 +743,14 @@ static int schur_eliminate(const schur_ctx* s, const double* b, const double* D,
   memset(S, 0, sizeof(double) * nf * nf);
   memset(rhs, 0, sizeof(double) * nf);
   if (D) for (int64_t i = 0; i < nf; ++i) S[i * nf + i] = D[f0 + i] * D[f0 + i];
-  for (int32_t e = 0; e < g->num_e; ++e) {
+  /* Distinct F blocks of a chunk: at most 2 per row. */
+  int64_t max_rows = 1;
+  for (int32_t e = 0; e < g->num_e; ++e)
+    if (s->chunk_start[e + 1] - s->chunk_start[e] > max_rows) max_rows = s->chunk_start[e + 1] - s->chunk_start[e];
+  int32_t* fblk = (int32_t*)malloc(sizeof(int32_t) * 2 * (size_t)max_rows);
+  double (*buf)[27] = (double (*)[27])malloc(sizeof(double) * 27 * 2 * (size_t)max_rows);
+  int ok = 1;
+  for (int32_t e = 0; e < g->num_e && ok; ++e) {
     const int esz = g->bsize[g->order[e]];
     double ete[81], inv[81], gv[9] = {0};
     chunk_ete(s, e, D, ete);
@@ -769,7 +776,7 @@ static int schur_eliminate(const schur_ctx* s, const double* b, const double* D,
         }
       }
     }
-    if (!spd_inverse(ete, esz, inv)) return 0;
+    if (!spd_inverse(ete, esz, inv)) { ok = 0; break; }
     double ig[9];
     for (int i = 0; i < esz; ++i) {
       ig[i] = 0.0;
@@ -788,8 +795,6 @@ static int schur_eliminate(const schur_ctx* s, const double* b, const double* D,
       }
     }
     /* ChunkOuterProduct (:496-548): buffer_f = sum E'F per distinct F block. */
-    int32_t fblk[64];
-    double buf[64][27];
     int nbuf = 0;
     for (int64_t r = s->chunk_start[e]; r < s->chunk_start[e + 1]; ++r) {
       const double* E = cell_ptr(g, J, r, 0);
@@ -800,7 +805,6 @@ static int schur_eliminate(const schur_ctx* s, const double* b, const double* D,
         int k = 0;
         while (k < nbuf && fblk[k] != cb) ++k;
         if (k == nbuf) {
-          if (nbuf == 64) return 0;
           fblk[nbuf] = cb;
           memset(buf[nbuf], 0, sizeof(buf[0]));
           ++nbuf;
@@ -829,6 +833,9 @@ static int schur_eliminate(const schur_ctx* s, const double* b, const double* D,
       }
     }
   }
+  free(fblk);
+  free(buf);
+  if (!ok) return 0;
   /* NoEBlockRowsUpdate (:553-641): rows whose first cell is an F block. */
   for (int64_t r = s->chunk_start[g->num_e]; r < J->nrows; ++r) {
     for (int c1 = 0; c1 < J->ncell[r]; ++c1) {

@@ -224,6 +224,22 @@ def test_fp64_solve_huber_and_constant_camera(mba, golden, huber_scene, case):
     np.testing.assert_allclose(s.final_cost, float(golden[f"{case}_final_cost"]), rtol=1e-9)
 
 
+def test_explicit_solve_with_tracks_longer_than_128(mba, O):
+    """Points seen by up to 140 cameras (21 of them by more than 128): the
+    explicit FP64 trajectory matches the oracle's SPARSE_SCHUR run, the mixed
+    one reaches its final cost within 1e-6."""
+    d = O.generate(O.make_spec(num_cameras=150, num_points=800, track_min=2, track_max=140, power_alpha=1.0,
+                               seed=7))
+    assert np.bincount(d["pt_idx"]).max() > 128
+    so, its = O.solve(O.Scene(d), O.default_options(linear_solver=0, max_num_iterations=5, huber_a=4.0))
+    s = mba.solve(ba_problem(mba, d), mba.default_options(linear_solver=0, max_num_iterations=5, huber_a=4.0))
+    np.testing.assert_allclose([it.cost for it in s.iterations], [it.cost for it in its], rtol=1e-9)
+    assert s.num_successful_steps == so.num_successful_steps and s.termination_type == so.termination_type
+    sm = mba.solve(ba_problem(mba, d), mba.default_options(linear_solver=0, max_num_iterations=5, huber_a=4.0,
+                                                           precision=mba.PRECISION_MIXED))
+    assert abs(sm.final_cost - so.final_cost) <= 1e-6 * so.final_cost
+
+
 @pytest.mark.parametrize("solver", [0, 1])
 def test_mixed_precision_reaches_fp64_final_cost(mba, golden, c1, solver):
     s = mba.solve(ba_problem(mba, c1), mba.default_options(max_num_iterations=20, linear_solver=solver,
