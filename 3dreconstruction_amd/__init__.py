@@ -326,8 +326,11 @@ class Context:
         _check(self._lib.mba_context_set_timing(self.handle, 1 if on else 0))
 
     def last_timing(self):
-        ms = np.zeros(3)
-        _check(self._lib.mba_context_last_timing(self.handle, _p(ms), 3))
+        """[point pass, camera pass, pass total, S assembly, Cholesky,
+        triangular solves] in ms (the last three from the last explicit
+        solve, 0 if none ran)."""
+        ms = np.zeros(6)
+        _check(self._lib.mba_context_last_timing(self.handle, _p(ms), 6))
         return ms
 
     def read_schur(self):

@@ -387,37 +387,50 @@ __device__ __forceinline__ bool inv3_spd(const double a[6], double o[6]) {
 }
 
 // FP64 inverse of an SPD n x n (row-major, n <= 9) via LLT; false if not PD.
-__device__ inline bool invn_spd(double* A, int n, double* inv) {
-  for (int j = 0; j < n; ++j) {
-    double d = A[j * n + j];
-    for (int k = 0; k < j; ++k) d -= A[j * n + k] * A[j * n + k];
+
+__device__ __forceinline__ int ut9(int i, int j) { return i * 9 - i * (i - 1) / 2 + (j - i); }
+__device__ __forceinline__ int lt9(int i, int j) { return i * (i + 1) / 2 + j; }  // packed lower, j <= i
+
+// FP64 Cholesky of a 9 x 9 SPD matrix given by its upper triangle (ut9
+// order) into packed lower l[45]; false if not positive definite.  Fully
+// unrolled: every index is a constant, everything stays in registers.
+__device__ __forceinline__ bool chol9(const double a[45], double l[45]) {
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    double d = a[ut9(j, j)];
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= l[lt9(j, k)] * l[lt9(j, k)];
     if (!(d > 0.0)) return false;
     d = sqrt(d);
-    A[j * n + j] = d;
-    for (int i = j + 1; i < n; ++i) {
-      double s = A[i * n + j];
-      for (int k = 0; k < j; ++k) s -= A[i * n + k] * A[j * n + k];
-      A[i * n + j] = s / d;
+    l[lt9(j, j)] = d;
+#pragma unroll
+    for (int i = j + 1; i < 9; ++i) {
+      double s = a[ut9(j, i)];
+#pragma unroll
+      for (int k = 0; k < j; ++k) s -= l[lt9(i, k)] * l[lt9(j, k)];
+      l[lt9(i, j)] = s / d;
     }
-  }
-  for (int c = 0; c < n; ++c) {
-    double e[9];
-    for (int i = 0; i < n; ++i) {
-      double s = (i == c) ? 1.0 : 0.0;
-      for (int k = 0; k < i; ++k) s -= A[i * n + k] * e[k];
-      e[i] = s / A[i * n + i];
-    }
-    for (int i = n - 1; i >= 0; --i) {
-      double s = e[i];
-      for (int k = i + 1; k < n; ++k) s -= A[k * n + i] * e[k];
-      e[i] = s / A[i * n + i];
-    }
-    for (int i = 0; i < n; ++i) inv[i * n + c] = e[i];
   }
   return true;
 }
 
-__device__ __forceinline__ int ut9(int i, int j) { return i * 9 - i * (i - 1) / 2 + (j - i); }
+// Column c of (L L')^-1 from the packed factor.
+__device__ __forceinline__ void chol9_inv_col(const double l[45], int c, double e[9]) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= l[lt9(i, k)] * e[k];
+    e[i] = s / l[lt9(i, i)];
+  }
+#pragma unroll
+  for (int i = 8; i >= 0; --i) {
+    double s = e[i];
+#pragma unroll
+    for (int k = i + 1; k < 9; ++k) s -= l[lt9(k, i)] * e[k];
+    e[i] = s / l[lt9(i, i)];
+  }
+}
 
 // --------------------------------------------------------------- FP64 cost
 // One thread per E block; block partial sums of 1/2 rho(|r|^2).
@@ -769,31 +782,35 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
   else if (k < 63 && with_h) rhs[9 * (size_t)f + (k - 54)] = vt;
   sum[k] = (double)vt;
   __syncthreads();
-  if (k != 0) return;
+  // Lanes 0..8: each factors U + D^2 (FP64, redundantly) and writes one
+  // column of its inverse.
+  if (k >= 9) return;
   const int wi = win_of_f(win, f);
   const double inv_radius = win.inv_r[wi];
-  double A[81], inv[81];
-  for (int i = 0; i < 9; ++i)
-    for (int j = i; j < 9; ++j) {
-      const double u = sum[ut9(i, j)];
-      A[i * 9 + j] = u;
-      A[j * 9 + i] = u;
-    }
+  double a[45];
+#pragma unroll
+  for (int q = 0; q < 45; ++q) a[q] = sum[q];
+#pragma unroll
   for (int i = 0; i < 9; ++i) {
     double dg;
     if (recompute_diag) {
-      dg = fmin(fmax(A[i * 10], min_diag), max_diag);
-      diag_c[9 * (size_t)f + i] = dg;
+      dg = fmin(fmax(a[ut9(i, i)], min_diag), max_diag);
+      if (k == 0) diag_c[9 * (size_t)f + i] = dg;
     } else {
       dg = diag_c[9 * (size_t)f + i];
     }
-    A[i * 10] += dg * inv_radius;
+    a[ut9(i, i)] += dg * inv_radius;
   }
-  if (!invn_spd(A, 9, inv)) {
-    win.flag[wi] = 1;
-    for (int q = 0; q < 81; ++q) inv[q] = 0.0;
+  double l[45], e[9];
+  if (chol9(a, l)) {
+    chol9_inv_col(l, k, e);
+  } else {
+    if (k == 0) win.flag[wi] = 1;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) e[i] = 0.0;
   }
-  for (int q = 0; q < 81; ++q) Minv[81 * (size_t)f + q] = (T)inv[q];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Minv[81 * (size_t)f + 9 * i + k] = (T)e[i];
 }
 
 // Re-damp the camera preconditioner after a rejected step.
@@ -805,19 +822,26 @@ __global__ __launch_bounds__(64) void k_cam_damp(int nf, const T* __restrict__ U
   if (f >= nf || win_idle_f(win, f)) return;
   const int wi = win_of_f(win, f);
   const double inv_radius = win.inv_r[wi];
-  double A[81], inv[81];
-  for (int i = 0; i < 9; ++i)
-    for (int j = i; j < 9; ++j) {
-      const double u = (double)U[45 * (size_t)f + ut9(i, j)];
-      A[i * 9 + j] = u;
-      A[j * 9 + i] = u;
+  double a[45];
+#pragma unroll
+  for (int q = 0; q < 45; ++q) a[q] = (double)U[45 * (size_t)f + q];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) a[ut9(i, i)] += diag_c[9 * (size_t)f + i] * inv_radius;
+  double l[45];
+  const bool ok = chol9(a, l);
+  if (!ok) win.flag[wi] = 1;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    double e[9];
+    if (ok) {
+      chol9_inv_col(l, c, e);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) e[i] = 0.0;
     }
-  for (int i = 0; i < 9; ++i) A[i * 10] += diag_c[9 * (size_t)f + i] * inv_radius;
-  if (!invn_spd(A, 9, inv)) {
-    win.flag[wi] = 1;
-    for (int k = 0; k < 81; ++k) inv[k] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Minv[81 * (size_t)f + 9 * i + c] = (T)e[i];
   }
-  for (int k = 0; k < 81; ++k) Minv[81 * (size_t)f + k] = (T)inv[k];
 }
 
 // ---------------------------------------------------------------- matvecs

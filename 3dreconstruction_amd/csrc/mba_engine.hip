@@ -369,7 +369,8 @@ struct mba_context {
   bool mixed = false;
   bool scale_ready = false;
   bool timing = false;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool timed_explicit = false;
   // Device point order (E blocks of all windows stably grouped by lane
   // class) and the point-major launch layout.
   std::vector<int32_t> dev_e_to_pt;  // device E block -> global point
@@ -494,7 +495,7 @@ void context_free(mba_context* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 8; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   c->pool.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -886,7 +887,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   else
     MBA_TRY(alloc_precision<double>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_chunk_start));
   if (W == 1 && c->opt.linear_solver == MBA_EXPLICIT_SCHUR) MBA_TRY(build_s_pairs(c, dev_pt_start, row_cam));
-  for (int i = 0; i < 4; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
+  for (int i = 0; i < 8; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
   HIP_TRY(hipStreamSynchronize(c->stream));
   *out = c;
   guard.c = nullptr;
@@ -1145,6 +1146,7 @@ struct Engine {
       if (!c->gemv_part) MBA_TRY(c->pool.alloc(&c->gemv_part, (size_t)kGemvSplit * n));
     }
     const double inv_r = c->h_inv_r[0];
+    if (c->timing) (void)hipEventRecord(c->ev[4], s);
     // Blocks no point couples stay zero.
     HIP_TRY(hipMemsetAsync(d.S, 0, sizeof(T) * n * n, s));
     if (c->seg_blocks > 0)
@@ -1153,6 +1155,7 @@ struct Engine {
     if (c->s_nblk > 0)
       hipLaunchKernelGGL(k_s_blocks<T>, dim3((unsigned)c->s_nblk), dim3(64), 0, s, c->s_blk_start, c->s_blk_fa,
                          c->s_blk_fb, c->s_pair_a, c->s_pair_b, d.WY, d.U, c->diag_c, inv_r, n, d.S);
+    if (c->timing) (void)hipEventRecord(c->ev[5], s);
     HIP_TRY(hipMemcpyAsync(d.L, d.S, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(c->info, 0, sizeof(int32_t), s));
     for (int64_t k = 0; k < n; k += kNB) {
@@ -1166,6 +1169,7 @@ struct Engine {
                            (int)k, kb);
       }
     }
+    if (c->timing) (void)hipEventRecord(c->ev[6], s);
     int32_t info = 0;
     HIP_TRY(hipMemcpyAsync(&info, c->info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     MBA_TRY(sync());
@@ -1190,6 +1194,10 @@ struct Engine {
         hipLaunchKernelGGL(k_axpy_inplace, dim3(nblocks(n)), dim3(kBlock), 0, s, n, c->r64, c->x64);
       }
       hipLaunchKernelGGL((k_convert<double, T>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, c->x64, d.x);
+    }
+    if (c->timing) {
+      (void)hipEventRecord(c->ev[7], s);
+      c->timed_explicit = true;
     }
     return MBA_OK;
   }
@@ -1706,12 +1714,16 @@ mba_status mba_context_last_timing(mba_context* ctx, double* ms, int32_t n) {
   if (!ctx || !ms) return fail(MBA_ERR_INVALID_ARGUMENT, "NULL argument");
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  float a = 0.f, b = 0.f, t = 0.f;
-  HIP_TRY(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
-  HIP_TRY(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
-  HIP_TRY(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[3]));
-  const double v[3] = {a, b, t};
-  for (int i = 0; i < n && i < 3; ++i) ms[i] = v[i];
+  float e[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  HIP_TRY(hipEventElapsedTime(&e[0], ctx->ev[0], ctx->ev[1]));
+  HIP_TRY(hipEventElapsedTime(&e[1], ctx->ev[1], ctx->ev[2]));
+  HIP_TRY(hipEventElapsedTime(&e[2], ctx->ev[0], ctx->ev[3]));
+  if (ctx->timed_explicit) {
+    HIP_TRY(hipEventElapsedTime(&e[3], ctx->ev[4], ctx->ev[5]));
+    HIP_TRY(hipEventElapsedTime(&e[4], ctx->ev[5], ctx->ev[6]));
+    HIP_TRY(hipEventElapsedTime(&e[5], ctx->ev[6], ctx->ev[7]));
+  }
+  for (int i = 0; i < n && i < 6; ++i) ms[i] = e[i];
   return MBA_OK;
 }
 

@@ -250,9 +250,11 @@ mba_status mba_context_synchronize(mba_context* ctx);
 /* The HIP stream (hipStream_t) the context launches on. */
 void* mba_context_stream(mba_context* ctx);
 
-/* Per-kernel timing of the last mba_context_jacobian_schur, measured with
- * HIP events on the context stream: ms[0] point pass, ms[1] camera pass,
- * ms[2] total.  Enabled with mba_context_set_timing(ctx, 1). */
+/* Timing measured with HIP events on the context stream, enabled with
+ * mba_context_set_timing(ctx, 1).  Of the last Jacobian + Schur pass:
+ * ms[0] point pass, ms[1] camera pass, ms[2] total; of the last explicit
+ * Schur solve (0 if none ran): ms[3] assembly of S, ms[4] Cholesky
+ * factorisation, ms[5] triangular solves with refinement.  n <= 6 values. */
 mba_status mba_context_set_timing(mba_context* ctx, int32_t enable);
 mba_status mba_context_last_timing(mba_context* ctx, double* ms, int32_t n);
 
