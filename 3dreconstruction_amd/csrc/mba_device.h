@@ -24,8 +24,11 @@
 
 namespace mba {
 
-constexpr int kCamTab = 48;  // R 9 | dR 27 | t 3 | f k1 k2 | pad
-constexpr int kTabDR = 9, kTabT = 36, kTabF = 39;
+// Camera table row: the projection part first, so a row-wise gather of it
+// touches one 64-byte (FP32) / 128-byte (FP64) line:
+//   R 9 | t 3 | f k1 k2 | pad | dR 27 | pad 5.
+constexpr int kCamTab = 48;
+constexpr int kTabT = 9, kTabF = 12, kTabDR = 16;
 constexpr int kDirTab = 16;  // M 9 | dt 3 | dintr 3 | pad
 constexpr int kPartial = 64; // U 45 | g 9 | h 9 | pad
 constexpr int kBlock = 256;
@@ -75,6 +78,14 @@ __device__ __forceinline__ T group_sum(T v, int lanes) {
   for (int o = 32; o > 0; o >>= 1)
     if (o < lanes) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// acc += a b + c d as two chained FMAs (the sum form would round a b + c d
+// first and add it separately).
+template <typename T>
+__device__ __forceinline__ void fma2(T& acc, T a, T b, T c, T d) {
+  acc = fma(a, b, acc);
+  acc = fma(c, d, acc);
 }
 
 // Independent problems ("windows") solved together: the window of every
@@ -200,9 +211,10 @@ __global__ __launch_bounds__(kBlock) void k_cam_table(int nc, const double* __re
   rodrigues_with_jacobian(w, R, dR);
   T* o = tab + (size_t)kCamTab * c;
   for (int i = 0; i < 9; ++i) o[i] = (T)R[i];
-  for (int i = 0; i < 27; ++i) o[kTabDR + i] = (T)dR[i];
   for (int i = 0; i < 6; ++i) o[kTabT + i] = (T)p[3 + i];
-  for (int i = 42; i < kCamTab; ++i) o[i] = (T)0;
+  o[15] = (T)0;
+  for (int i = 0; i < 27; ++i) o[kTabDR + i] = (T)dR[i];
+  for (int i = kTabDR + 27; i < kCamTab; ++i) o[i] = (T)0;
 }
 
 template <typename T>
@@ -557,15 +569,15 @@ __device__ __forceinline__ void point_row_accumulate(const T* __restrict__ ctab,
   T Jp[6];
   jac_point(o, c.R, wgt, sp, Jp);
   const T rx = wgt * o.rx, ry = wgt * o.ry;
-  v[0] += Jp[0] * Jp[0] + Jp[3] * Jp[3];
-  v[1] += Jp[0] * Jp[1] + Jp[3] * Jp[4];
-  v[2] += Jp[0] * Jp[2] + Jp[3] * Jp[5];
-  v[3] += Jp[1] * Jp[1] + Jp[4] * Jp[4];
-  v[4] += Jp[1] * Jp[2] + Jp[4] * Jp[5];
-  v[5] += Jp[2] * Jp[2] + Jp[5] * Jp[5];
-  g[0] += Jp[0] * rx + Jp[3] * ry;
-  g[1] += Jp[1] * rx + Jp[4] * ry;
-  g[2] += Jp[2] * rx + Jp[5] * ry;
+  fma2(v[0], Jp[0], Jp[0], Jp[3], Jp[3]);
+  fma2(v[1], Jp[0], Jp[1], Jp[3], Jp[4]);
+  fma2(v[2], Jp[0], Jp[2], Jp[3], Jp[5]);
+  fma2(v[3], Jp[1], Jp[1], Jp[4], Jp[4]);
+  fma2(v[4], Jp[1], Jp[2], Jp[4], Jp[5]);
+  fma2(v[5], Jp[2], Jp[2], Jp[5], Jp[5]);
+  fma2(g[0], Jp[0], rx, Jp[3], ry);
+  fma2(g[1], Jp[1], rx, Jp[4], ry);
+  fma2(g[2], Jp[2], rx, Jp[5], ry);
 }
 
 template <typename T>
@@ -691,11 +703,11 @@ __device__ __forceinline__ void cam_row_accumulate(const CamProj<T>& c, const T 
 #pragma unroll
   for (int a = 0; a < 9; ++a)
 #pragma unroll
-    for (int b = a; b < 9; ++b) acc[k++] += Jc[a] * Jc[b] + Jc[9 + a] * Jc[9 + b];
+    for (int b = a; b < 9; ++b) fma2(acc[k++], Jc[a], Jc[b], Jc[9 + a], Jc[9 + b]);
 #pragma unroll
   for (int a = 0; a < 9; ++a) {
-    acc[45 + a] += Jc[a] * rx + Jc[9 + a] * ry;
-    acc[54 + a] += Jc[a] * tx + Jc[9 + a] * ty;
+    fma2(acc[45 + a], Jc[a], rx, Jc[9 + a], ry);
+    fma2(acc[54 + a], Jc[a], tx, Jc[9 + a], ty);
   }
 }
 
@@ -889,24 +901,34 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
   load_pk(pkX, e, X, unused);
   const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
   T q[3] = {0, 0, 0};
+  auto row = [&](int cam, T ox, T oy) {
+    CamProj<T> c;
+    load_proj(ctab + (size_t)kCamTab * cam, c);
+    Proj<T> o;
+    project(c, X, ox, oy, o);
+    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    T Jp[6];
+    jac_point(o, c.R, wgt, sp, Jp);
+    T v[2];
+    jac_cam_dir(o, dtab + (size_t)kDirTab * cam, X, wgt, v);
+    v[0] += beta * wgt * o.rx;
+    v[1] += beta * wgt * o.ry;
+    fma2(q[0], Jp[0], v[0], Jp[3], v[1]);
+    fma2(q[1], Jp[1], v[0], Jp[4], v[1]);
+    fma2(q[2], Jp[2], v[0], Jp[5], v[1]);
+  };
   if (qp.valid) {
-    for (int r = pt_start[e] + qp.sub; r < pt_start[e + 1]; r += qp.lanes) {
-      const int cam = row_cam[r];
-      CamProj<T> c;
-      load_proj(ctab + (size_t)kCamTab * cam, c);
-      Proj<T> o;
-      project(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
-      const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
-      T Jp[6];
-      jac_point(o, c.R, wgt, sp, Jp);
-      T v[2];
-      jac_cam_dir(o, dtab + (size_t)kDirTab * cam, X, wgt, v);
-      v[0] += beta * wgt * o.rx;
-      v[1] += beta * wgt * o.ry;
-      q[0] += Jp[0] * v[0] + Jp[3] * v[1];
-      q[1] += Jp[1] * v[0] + Jp[4] * v[1];
-      q[2] += Jp[2] * v[0] + Jp[5] * v[1];
+    const int r1 = pt_start[e + 1], L = qp.lanes;
+    int r = pt_start[e] + qp.sub;
+    // Two rows per iteration: both rows' loads are in flight together.
+    for (; r + L < r1; r += 2 * L) {
+      const int c0 = row_cam[r], c1 = row_cam[r + L];
+      const T o0x = row_obs[2 * (size_t)r], o0y = row_obs[2 * (size_t)r + 1];
+      const T o1x = row_obs[2 * (size_t)(r + L)], o1y = row_obs[2 * (size_t)(r + L) + 1];
+      row(c0, o0x, o0y);
+      row(c1, o1x, o1y);
     }
+    if (r < r1) row(row_cam[r], row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1]);
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i) q[i] = group_sum(q[i], qp.lanes);
@@ -951,12 +973,9 @@ __global__ __launch_bounds__(kBlock) void k_mv_cam(
 #pragma unroll
   for (int i = 0; i < 9; ++i) acc[i] = T(0);
   const int end = chunk_end[ch];
-  for (int i = chunk_begin[ch] + threadIdx.x; i < end; i += kBlock) {
-    const int e = cm_e[i];
-    T X[3], sz[3];
-    load_pk(pk, e, X, sz);
+  auto row = [&](const T X[3], const T sz[3], T ox, T oy) {
     Proj<T> o;
-    project(c, X, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], o);
+    project(c, X, ox, oy, o);
     const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
     T Jc[18];
     jac_cam(o, dR, X, wgt, cs, Jc);
@@ -965,7 +984,24 @@ __global__ __launch_bounds__(kBlock) void k_mv_cam(
     u[0] *= wgt;
     u[1] *= wgt;
 #pragma unroll
-    for (int a = 0; a < 9; ++a) acc[a] += Jc[a] * u[0] + Jc[9 + a] * u[1];
+    for (int a = 0; a < 9; ++a) fma2(acc[a], Jc[a], u[0], Jc[9 + a], u[1]);
+  };
+  int i = chunk_begin[ch] + threadIdx.x;
+  // Two rows per iteration: both rows' point-record gathers in flight.
+  for (; i + kBlock < end; i += 2 * kBlock) {
+    const int e0 = cm_e[i], e1 = cm_e[i + kBlock];
+    const T o0x = cm_obs[2 * (size_t)i], o0y = cm_obs[2 * (size_t)i + 1];
+    const T o1x = cm_obs[2 * (size_t)(i + kBlock)], o1y = cm_obs[2 * (size_t)(i + kBlock) + 1];
+    T X0[3], s0[3], X1[3], s1[3];
+    load_pk(pk, e0, X0, s0);
+    load_pk(pk, e1, X1, s1);
+    row(X0, s0, o0x, o0y);
+    row(X1, s1, o1x, o1y);
+  }
+  if (i < end) {
+    T X0[3], s0[3];
+    load_pk(pk, cm_e[i], X0, s0);
+    row(X0, s0, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1]);
   }
   block_sum<T, 9>(acc, lds);
   if (threadIdx.x < 9) {
@@ -1168,53 +1204,31 @@ __global__ __launch_bounds__(kBlock) void k_scatter_cams(int nf, const int32_t* 
 // on the device.  F-space vectors, one thread per 9-entry F block; block
 // partial dot products reduced by single-thread scalar kernels.
 
-// p = z + beta p (or z on the first iteration).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_p(int nf, const PcgState* __restrict__ st, Win win,
-                                                  const T* __restrict__ z, T* __restrict__ p,
-                                                  const int32_t* __restrict__ all_done) {
-  if (*all_done) return;
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  const PcgState& s = st[win_of_f(win, f)];
-  if (s.done) return;
-  const T beta = (T)s.beta;
-  for (int k = 0; k < 9; ++k) {
-    const size_t i = 9 * (size_t)f + k;
-    p[i] = (s.iter == 1) ? z[i] : z[i] + beta * p[i];
-  }
-}
+// PCG iteration, five launches: k_pcg_p_dir (p and its direction table),
+// k_mv_point, k_mv_cam, k_mv_finalize_alpha (q = S p, p.q, alpha) and
+// k_pcg_update_scalar (x, r, z = Minv r, the reductions, termination, rho,
+// beta).  The scalar steps run in whichever workgroup finishes last (a
+// counter the last one resets); single-problem partial sums are added in
+// block order, so results do not depend on scheduling.
 
-// Dot partials over F-space entries: block partials (single problem) or
-// per-window accumulators.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dot_partial(int64_t n, const int32_t* __restrict__ all_done, Win win,
-                                                        const T* __restrict__ a, const T* __restrict__ b,
-                                                        double* __restrict__ partial) {
-  __shared__ double lds[kBlock / 64];
-  if (all_done && *all_done) return;
-  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-  double acc[1] = {0.0};
-  const bool on = i < n && !win_idle_f(win, (int)(i / 9));
-  if (on) acc[0] = (double)a[i] * (double)b[i];
-  if (win.acc) {
-    win_atomic_add(win.acc, 1, 0, on ? win_of_f(win, (int)(i / 9)) : -1, acc[0]);
-    return;
-  }
-  block_sum<double, 1>(acc, lds);
+// Last-workgroup election once the block's partials are written: one
+// device-scope fence per workgroup (thread 0), not one per wave.
+__device__ __forceinline__ bool last_block(unsigned int* counter) {
+  __shared__ int is_last;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int w = 0; w < kBlock / 64; ++w) s += lds[w];
-    partial[blockIdx.x] = s;
+    __threadfence();
+    is_last = (atomicAdd(counter, 1u) == gridDim.x - 1);
+    if (is_last) __threadfence();
   }
+  __syncthreads();
+  return is_last != 0;
 }
 
 // alpha = rho / p.q per window, with the reference's failure tests.  The
 // per-window accumulators (windowed runs) are cleared as they are read.
-__global__ __launch_bounds__(256) void k_pcg_alpha(const double* __restrict__ partial, int nb, int nw,
-                                                   PcgState* __restrict__ st, double* __restrict__ wacc,
-                                                   const int32_t* __restrict__ all_done) {
-  if (*all_done) return;
+__device__ __forceinline__ void pcg_alpha(const double* __restrict__ partial, int nb, int nw,
+                                          PcgState* __restrict__ st, double* __restrict__ wacc) {
   for (int w = threadIdx.x; w < nw; w += blockDim.x) {
     PcgState& s = st[w];
     if (s.done) continue;
@@ -1239,81 +1253,11 @@ __global__ __launch_bounds__(256) void k_pcg_alpha(const double* __restrict__ pa
   }
 }
 
-// PCG vector update, one thread per F block.  mode 0: x += alpha p,
-// r -= alpha q;  1: x += alpha p only;  2: r = b - tmp;  3: r = b (x = 0).
-// Modes 0, 2, 3 then apply z = Minv r and reduce r.z, x.(b + r), r.r.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int nf, const PcgState* __restrict__ st, Win win,
-                                                       int mode, const T* __restrict__ p,
-                                                       const T* __restrict__ q,
-                                                       const T* __restrict__ b,
-                                                       const T* __restrict__ tmp,
-                                                       const T* __restrict__ Minv, T* __restrict__ x,
-                                                       T* __restrict__ r, T* __restrict__ z,
-                                                       double* __restrict__ partial,
-                                                       const int32_t* __restrict__ all_done) {
-  __shared__ double lds[3 * (kBlock / 64)];
-  if (*all_done) return;
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[3] = {0.0, 0.0, 0.0};
-  int wi = -1;
-  if (f < nf) {
-    wi = win_of_f(win, f);
-    const PcgState& s = st[wi];
-    if (s.done) {
-      wi = -1;
-    } else {
-      const T alpha = (T)s.alpha;
-      T rr[9];
-      for (int k = 0; k < 9; ++k) {
-        const size_t i = 9 * (size_t)f + k;
-        T xv;
-        if (mode == 3) {
-          xv = T(0);
-          x[i] = xv;
-        } else if (mode <= 1) {
-          xv = x[i] + alpha * p[i];
-          x[i] = xv;
-        } else {
-          xv = x[i];
-        }
-        if (mode == 1) continue;
-        const T rv = (mode == 0) ? r[i] - alpha * q[i] : (mode == 2 ? b[i] - tmp[i] : b[i]);
-        r[i] = rv;
-        rr[k] = rv;
-        acc[1] += (double)xv * ((double)b[i] + (double)rv);
-        acc[2] += (double)rv * (double)rv;
-      }
-      if (mode != 1) {
-        const T* M = Minv + 81 * (size_t)f;
-        for (int i = 0; i < 9; ++i) {
-          T a = T(0);
-          for (int j = 0; j < 9; ++j) a += M[9 * i + j] * rr[j];
-          z[9 * (size_t)f + i] = a;
-          acc[0] += (double)rr[i] * (double)a;
-        }
-      }
-    }
-  }
-  if (mode == 1) return;
-  if (win.acc) {
-    for (int k = 0; k < 3; ++k) win_atomic_add(win.acc, 3, k, wi, acc[k]);
-    return;
-  }
-  block_sum<double, 3>(acc, lds);
-  if (threadIdx.x < 3) {
-    double s = 0.0;
-    for (int w = 0; w < kBlock / 64; ++w) s += lds[w * 3 + threadIdx.x];
-    partial[3 * (size_t)blockIdx.x + threadIdx.x] = s;
-  }
-}
-
 // Per window: termination tests and the next rho / beta; then the
 // device-wide all-done flag.  `first`: the initial call (x = 0, r = b).
-__global__ __launch_bounds__(256) void k_pcg_scalar(const double* __restrict__ partial, int nb, int nw,
-                                                    PcgState* __restrict__ st, double* __restrict__ wacc,
-                                                    double eta, int min_iters, int max_iters, int first,
-                                                    int32_t* __restrict__ all_done) {
+__device__ __forceinline__ void pcg_scalar(const double* __restrict__ partial, int nb, int nw,
+                                           PcgState* __restrict__ st, double* __restrict__ wacc, double eta,
+                                           int min_iters, int max_iters, int first, int32_t* __restrict__ all_done) {
   __shared__ int any_active;
   if (threadIdx.x == 0) any_active = 0;
   __syncthreads();
@@ -1385,6 +1329,181 @@ __global__ __launch_bounds__(256) void k_pcg_scalar(const double* __restrict__ p
   }
   __syncthreads();
   if (threadIdx.x == 0) *all_done = any_active ? 0 : 1;
+}
+
+// The small PCG kernels give each F block (camera) a group of 16 lanes,
+// lane k < 9 owning component k; a group's values are exchanged with
+// shuffles inside the wave.
+constexpr int kGrp = 16;
+
+// p = z + beta p (p = z on iteration 1) for the camera's F block and its
+// direction-table row.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_p_dir(int nc, const int32_t* __restrict__ cam_to_f,
+                                                      const PcgState* __restrict__ st, Win win,
+                                                      const T* __restrict__ z, T* __restrict__ p,
+                                                      const T* __restrict__ cscale, const T* __restrict__ ctab,
+                                                      T* __restrict__ dtab, const int32_t* __restrict__ all_done) {
+  if (*all_done) return;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int c = (int)(t / kGrp), k = (int)(t % kGrp);
+  const int base = (threadIdx.x & 63) & ~(kGrp - 1);
+  const int f = c < nc ? cam_to_f[c] : -1;
+  bool on = f >= 0;
+  T beta = T(0);
+  int first = 0;
+  if (on) {
+    const PcgState& s = st[win_of_f(win, f)];
+    on = !s.done;
+    beta = (T)s.beta;
+    first = s.iter == 1;
+  }
+  T xs = T(0);
+  if (on && k < 9) {
+    const size_t i = 9 * (size_t)f + k;
+    const T pv = first ? z[i] : z[i] + beta * p[i];
+    p[i] = pv;
+    xs = cscale[i] * pv;
+  }
+  T x9[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) x9[j] = __shfl(xs, base + j, 64);
+  if (c >= nc) return;
+  T* o = dtab + (size_t)kDirTab * c;
+  if (f < 0) {  // constant camera: no direction
+    o[k] = T(0);
+    return;
+  }
+  if (!on) return;
+  T v = T(0);
+  if (k < 9) {
+    const T* dR = ctab + (size_t)kCamTab * c + kTabDR;
+    v = x9[0] * dR[k] + x9[1] * dR[9 + k] + x9[2] * dR[18 + k];
+  } else if (k < 15) {
+    v = x9[k - 6];
+  }
+  o[k] = v;
+}
+
+// q_f = (U + D^2) p_f - sum partials, p.q, then alpha (last workgroup).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mv_finalize_alpha(
+    int nf, const int32_t* __restrict__ f_chunk_start, const T* __restrict__ partial,
+    const T* __restrict__ U, const double* __restrict__ diag_c, Win win, const T* __restrict__ x,
+    T* __restrict__ y, const int32_t* __restrict__ all_done, PcgState* __restrict__ st, int nw,
+    double* __restrict__ bpart, unsigned int* __restrict__ counter) {
+  __shared__ double lds[kBlock / 64];
+  if (*all_done) return;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int f = (int)(t / kGrp), i = (int)(t % kGrp);
+  const int base = (threadIdx.x & 63) & ~(kGrp - 1);
+  const bool blk = f < nf && !win_idle_f(win, f);
+  const T xi = (blk && i < 9) ? x[9 * (size_t)f + i] : T(0);
+  T xv[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) xv[j] = __shfl(xi, base + j, 64);
+  double acc[1] = {0.0};
+  int wi = -1;
+  if (blk && i < 9) {
+    wi = win_of_f(win, f);
+    T sp = T(0);
+    for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch) sp += partial[(size_t)kPartial * ch + i];
+    const T* u = U + 45 * (size_t)f;
+    T a = (T)(diag_c[9 * (size_t)f + i] * win.inv_r[wi]) * xv[i];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) a += u[i <= j ? ut9(i, j) : ut9(j, i)] * xv[j];
+    const T yv = a - sp;
+    y[9 * (size_t)f + i] = yv;
+    acc[0] = (double)xi * (double)yv;
+  }
+  if (win.acc) {
+    win_atomic_add(win.acc, 1, 0, wi, acc[0]);
+  } else {
+    block_sum<double, 1>(acc, lds);
+    if (threadIdx.x == 0) {
+      double v = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) v += lds[w];
+      bpart[blockIdx.x] = v;
+    }
+  }
+  if (!last_block(counter)) return;
+  pcg_alpha(bpart, gridDim.x, nw, st, win.acc);
+  if (threadIdx.x == 0) *counter = 0u;
+}
+
+// PCG vector update.  mode 0: x += alpha p, r -= alpha q;  1: x += alpha p
+// only;  2: r = b - tmp;  3: r = b (x = 0).  Modes 0, 2, 3 then apply
+// z = Minv r, reduce r.z, x.(b + r), r.r, and the last workgroup runs the
+// scalar step (`first` with mode 3).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update_scalar(
+    int nf, PcgState* __restrict__ st, Win win, int mode, const T* __restrict__ p, const T* __restrict__ q,
+    const T* __restrict__ b, const T* __restrict__ tmp, const T* __restrict__ Minv, T* __restrict__ x,
+    T* __restrict__ r, T* __restrict__ z, double* __restrict__ bpart, int32_t* __restrict__ all_done, int nw,
+    double eta, int min_iters, int max_iters, unsigned int* __restrict__ counter) {
+  __shared__ double lds[3 * (kBlock / 64)];
+  if (*all_done) return;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int f = (int)(t / kGrp), k = (int)(t % kGrp);
+  const int base = (threadIdx.x & 63) & ~(kGrp - 1);
+  int wi = -1;
+  T alpha = T(0);
+  if (f < nf) {
+    const int w = win_of_f(win, f);
+    const PcgState& s = st[w];
+    if (!s.done) {
+      wi = w;
+      alpha = (T)s.alpha;
+    }
+  }
+  const bool on = wi >= 0 && k < 9;
+  const size_t idx = 9 * (size_t)f + k;
+  T xv = T(0), rv = T(0);
+  if (on) {
+    if (mode == 3) {
+      xv = T(0);
+      x[idx] = xv;
+    } else if (mode <= 1) {
+      xv = x[idx] + alpha * p[idx];
+      x[idx] = xv;
+    } else {
+      xv = x[idx];
+    }
+    if (mode != 1) {
+      rv = (mode == 0) ? r[idx] - alpha * q[idx] : (mode == 2 ? b[idx] - tmp[idx] : b[idx]);
+      r[idx] = rv;
+    }
+  }
+  if (mode == 1) return;
+  T r9[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) r9[j] = __shfl(rv, base + j, 64);
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (on) {
+    const T* M = Minv + 81 * (size_t)f + 9 * k;
+    T a = T(0);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) a += M[j] * r9[j];
+    z[idx] = a;
+    acc[0] = (double)rv * (double)a;
+    acc[1] = (double)xv * ((double)b[idx] + (double)rv);
+    acc[2] = (double)rv * (double)rv;
+  }
+  const int wl = on ? wi : -1;
+  if (win.acc) {
+    for (int j = 0; j < 3; ++j) win_atomic_add(win.acc, 3, j, wl, acc[j]);
+  } else {
+    block_sum<double, 3>(acc, lds);
+    if (threadIdx.x == 0)
+      for (int j = 0; j < 3; ++j) {
+        double v = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) v += lds[w * 3 + j];
+        bpart[3 * (size_t)blockIdx.x + j] = v;
+      }
+  }
+  if (!last_block(counter)) return;
+  pcg_scalar(bpart, gridDim.x, nw, st, win.acc, eta, min_iters, max_iters, mode == 3, all_done);
+  if (threadIdx.x == 0) *counter = 0u;
 }
 
 // Per-window flagged copy of trial parameters into the current ones.

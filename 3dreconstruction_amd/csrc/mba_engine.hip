@@ -348,6 +348,7 @@ struct mba_context {
   int32_t* flag = nullptr;      // device, one per window
   int32_t* info = nullptr;      // device
   int32_t* all_done = nullptr;  // device: every window's PCG has stopped
+  unsigned int* counter = nullptr;  // device: last-workgroup election (2)
   // Window descriptors (device): window of each device E / F block (W > 1
   // only), 1 / radius per window, per-window accumulators, accept flags.
   int32_t *of_e = nullptr, *of_f = nullptr;
@@ -857,13 +858,15 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   const int64_t maxb = nblocks(std::max<int64_t>({(int64_t)c->ne, 9 * (int64_t)c->nf, (int64_t)c->nrows, 1})) + 64;
   // Reduction scratch: block partials (<= 2 * maxb) plus the gradient
   // max/sumsq partials at offset 2048 (<= 1024 blocks each).
-  MBA_TRY(c->pool.alloc(&c->red, std::max<size_t>(4 * (size_t)maxb, 4096)));
+  const size_t nred = std::max<size_t>({4 * (size_t)maxb, 4096, 4 * (size_t)nblocks((int64_t)kGrp * c->nf)});
+  MBA_TRY(c->pool.alloc(&c->red, nred));
   MBA_TRY(c->pool.alloc(&c->red2, std::max<size_t>(4 * (size_t)maxb, 4096)));
   MBA_TRY(c->pool.alloc(&c->sc, 4));
   MBA_TRY(c->pool.alloc(&c->pcg, (size_t)W));
   MBA_TRY(c->pool.alloc(&c->flag, std::max<size_t>(4, (size_t)W)));
   MBA_TRY(c->pool.alloc(&c->info, 4));
   MBA_TRY(c->pool.alloc(&c->all_done, 4));
+  MBA_TRY(upload(c, &c->counter, std::vector<unsigned int>(4, 0u)));
   MBA_TRY(c->pool.alloc(&c->wacc, 4 * (size_t)W));
   MBA_TRY(c->pool.alloc(&c->wacc2, 2 * (size_t)W));
   MBA_TRY(c->pool.alloc(&c->accept, (size_t)W));
@@ -1058,7 +1061,7 @@ struct Engine {
   // window stops on its own tests, the launches stop when all have.
   mba_status pcg(const std::vector<uint8_t>& active, std::vector<int>* iters, std::vector<int>* status) {
     const int nf = c->nf;
-    const int nb = nblocks(nf), nbd = nblocks(9 * (int64_t)nf);
+    const int nbg = nblocks((int64_t)kGrp * nf);  // 16 lanes per F block
     MBA_TRY(sync());
     c->h_pcg.assign(W, PcgState());
     for (int w = 0; w < W; ++w) {
@@ -1078,30 +1081,33 @@ struct Engine {
     const Win wd = win(dot_acc, true), wu = win(upd_acc, true);
     const double eta = c->opt.eta;
     const int minit = c->opt.min_linear_solver_iterations, maxit = c->opt.max_linear_solver_iterations;
-    const int nthr = std::min(256, std::max(64, (W + 63) / 64 * 64));
-    hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, wu, 3, d.p, d.q, d.rhs, d.tmp,
-                       d.Minv, d.x, d.r, d.zc, c->red, c->all_done);
-    hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(nthr), 0, s, c->red, nb, W, c->pcg, upd_acc, eta, minit, maxit,
-                       1, c->all_done);
+    auto update = [&](int mode) {
+      hipLaunchKernelGGL(k_pcg_update_scalar<T>, dim3(nbg), dim3(kBlock), 0, s, nf, c->pcg, wu, mode, d.p, d.q, d.rhs,
+                         d.tmp, d.Minv, d.x, d.r, d.zc, c->red, c->all_done, W, eta, minit, maxit, c->counter + 1);
+    };
+    const Win wv = win(nullptr, true);
+    update(3);
     int check_every = 1;
     for (int it = 1; it <= maxit; ++it) {
-      hipLaunchKernelGGL(k_pcg_p<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, wu, d.zc, d.p, c->all_done);
-      matvec(d.p, d.q, c->all_done);
-      hipLaunchKernelGGL(k_dot_partial<T>, dim3(nbd), dim3(kBlock), 0, s, (int64_t)9 * nf, c->all_done, wd, d.p,
-                         d.q, c->red);
-      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(nthr), 0, s, c->red, nbd, W, c->pcg, dot_acc, c->all_done);
-      if (it % 10 == 0) {
-        hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, wu, 1, d.p, d.q, d.rhs,
-                           d.tmp, d.Minv, d.x, d.r, d.zc, c->red, c->all_done);
+      hipLaunchKernelGGL(k_pcg_p_dir<T>, dim3(nblocks((int64_t)kGrp * c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, c->pcg, wv,
+                         d.zc, d.p, d.cscale, d.ctab, d.dtab, c->all_done);
+      if (c->seg_blocks > 0)
+        hipLaunchKernelGGL(k_mv_point<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
+                           d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)0, 1, d.z, d.pk,
+                           c->all_done, wv);
+      if (c->nchunks > 0)
+        hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
+                           d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber, d.partial,
+                           c->all_done, wv);
+      hipLaunchKernelGGL(k_mv_finalize_alpha<T>, dim3(nbg), dim3(kBlock), 0, s, nf, d.f_chunk_start, d.partial, d.U,
+                         c->diag_c, wd, d.p, d.q, c->all_done, c->pcg, W, c->red, c->counter);
+      if (it % 10 == 0) {  // residual reset: r = b - S x
+        update(1);
         matvec(d.x, d.tmp, c->all_done);
-        hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, wu, 2, d.p, d.q, d.rhs,
-                           d.tmp, d.Minv, d.x, d.r, d.zc, c->red, c->all_done);
+        update(2);
       } else {
-        hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, s, nf, c->pcg, wu, 0, d.p, d.q, d.rhs,
-                           d.tmp, d.Minv, d.x, d.r, d.zc, c->red, c->all_done);
+        update(0);
       }
-      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(nthr), 0, s, c->red, nb, W, c->pcg, upd_acc, eta, minit,
-                         maxit, 0, c->all_done);
       if (it % check_every == 0 || it == maxit) {
         int32_t done = 0;
         HIP_TRY(hipMemcpyAsync(&done, c->all_done, sizeof(int32_t), hipMemcpyDeviceToHost, s));
