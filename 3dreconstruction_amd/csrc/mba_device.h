@@ -134,6 +134,24 @@ __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
   atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
 }
 
+// acc[stride * window + k] = max(., v >= 0) for every lane (wi < 0: none),
+// with a wave pre-reduction like win_atomic_add.
+__device__ __forceinline__ void win_atomic_max(double* acc, int stride, int k, int wi, double v) {
+  const unsigned long long valid = __ballot(wi >= 0);
+  if (valid == 0ull) return;
+  const int leader = __ffsll((long long)valid) - 1;
+  const int wl = __shfl(wi, leader, 64);
+  const int wv = wi < 0 ? wl : wi;
+  if (__all(wv == wl)) {
+    double m = wi >= 0 ? v : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((int)(threadIdx.x & 63) == leader) atomic_max_nonneg(acc + (size_t)stride * wl + k, m);
+  } else if (wi >= 0) {
+    atomic_max_nonneg(acc + (size_t)stride * wi + k, v);
+  }
+}
+
 template <typename T> __device__ __forceinline__ T tiny();
 template <> __device__ __forceinline__ float tiny<float>() { return FLT_MIN; }
 template <> __device__ __forceinline__ double tiny<double>() { return DBL_MIN; }
@@ -539,9 +557,9 @@ __global__ __launch_bounds__(kBlock) void k_grad_win(int64_t n, int per, int is_
     if (!(is_cam ? win_idle_f(win, blk) : win_idle_e(win, blk))) {
       v = (double)g[i] / scale[i];
       wi = is_cam ? win_of_f(win, blk) : win_of_e(win, blk);
-      atomic_max_nonneg(out + 2 * (size_t)wi, fabs(v));
     }
   }
+  win_atomic_max(out, 2, 0, wi, fabs(v));
   win_atomic_add(out, 2, 1, wi, v * v);
 }
 
