@@ -563,13 +563,35 @@ __global__ __launch_bounds__(kBlock) void k_grad_win(int64_t n, int per, int is_
   win_atomic_add(out, 2, 1, wi, v * v);
 }
 
-__global__ void k_grad_final(const double* __restrict__ pmax, const double* __restrict__ psq, int n,
-                             double* __restrict__ out /* [max, sumsq] accumulate */) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double a = out[0], b = out[1];
-  for (int i = 0; i < n; ++i) { a = fmax(a, pmax[i]); b += psq[i]; }
-  out[0] = a;
-  out[1] = b;
+__global__ __launch_bounds__(kBlock) void k_grad_final(const double* __restrict__ pmax,
+                                                       const double* __restrict__ psq, int n,
+                                                       double* __restrict__ out /* [max, sumsq] accumulate */) {
+  __shared__ double lds[2 * (kBlock / 64)];
+  double mx = 0.0, sq = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    mx = fmax(mx, pmax[i]);
+    sq += psq[i];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
+    sq += __shfl_xor(sq, o, 64);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[wave] = mx;
+    lds[kBlock / 64 + wave] = sq;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = out[0], b = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      a = fmax(a, lds[w]);
+      b += lds[kBlock / 64 + w];
+    }
+    out[0] = a;
+    out[1] += b;
+  }
 }
 
 // ------------------------------------------------------------- point pass

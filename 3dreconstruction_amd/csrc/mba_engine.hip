@@ -1281,12 +1281,12 @@ struct Engine {
       int nb = std::min(1024, nblocks(np3));
       hipLaunchKernelGGL(k_grad_partial<T>, dim3(nb), dim3(kBlock), 0, s, np3, d.gp, c->pscale64, c->red,
                          c->red + 2048);
-      hipLaunchKernelGGL(k_grad_final, dim3(1), dim3(1), 0, s, c->red, c->red + 2048, nb, c->red2);
+      hipLaunchKernelGGL(k_grad_final, dim3(1), dim3(kBlock), 0, s, c->red, c->red + 2048, nb, c->red2);
       nb = std::min(1024, nblocks(nf9));
       if (nf9 > 0) {
         hipLaunchKernelGGL(k_grad_partial<T>, dim3(nb), dim3(kBlock), 0, s, nf9, d.gc, c->cscale64, c->red,
                            c->red + 2048);
-        hipLaunchKernelGGL(k_grad_final, dim3(1), dim3(1), 0, s, c->red, c->red + 2048, nb, c->red2);
+        hipLaunchKernelGGL(k_grad_final, dim3(1), dim3(kBlock), 0, s, c->red, c->red + 2048, nb, c->red2);
       }
       HIP_TRY(hipMemcpyAsync(v.data(), c->red2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     } else {
