@@ -308,7 +308,14 @@ def run(args):
         "kernels_ms": {"point_pass": kern_mixed[0], "camera_pass": kern_mixed[1], "pass_total": kern_mixed[2]},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": nbytes, "kernel_ms": kms},
+                     "algorithmic_bytes": nbytes, "kernel_ms": kms,
+                     # The Jacobian+Schur unit as a stored-Jacobian design would move it (16 B read and
+                     # W = Jc'Jp 108 B written per observation, 60 B of point blocks per point), over the
+                     # whole pass; this engine recomputes instead of writing W.
+                     "stored_w_unit": {"bytes_per_obs": 16 + 108 + 60.0 * npt / nobs,
+                                       "achieved": (16 + 108 + 60.0 * npt / nobs) * nobs / (ms_mixed * 1e-3) / 1e9,
+                                       "frac": (16 + 108 + 60.0 * npt / nobs) * nobs / (ms_mixed * 1e-3) / 1e9
+                                       / HBM_PEAK_GBS}},
         "lm": lm,
         "explicit_schur": explicit,
         "batched_windows": batched,
