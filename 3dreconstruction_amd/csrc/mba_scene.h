@@ -25,6 +25,13 @@
  *   pixel noise N(0, noise^2) per coordinate; a fraction of observations is
  *   shifted by outlier_px in a uniformly random direction;
  *   the initial guess multiplies every parameter by (1 + N(0, perturb^2)).
+ *
+ * Grouped-intrinsics variant (the libs/sfm incremental engine's layout,
+ * libs/sfm/src/sfm_incremental_engine.cpp:1273-1393): cameras are [aa(3),
+ * t(3)] (the same draws as the packed scene of the same seed), camera c uses
+ * intrinsic group c mod G, a group is [f, ppx, ppy, k1, k2, k3] with
+ * f ~ U(450, 550), ppx, ppy ~ N(0, 10^2), k1 ~ N(0, 0.01^2), k2 ~ N(0,
+ * 0.001^2), k3 ~ N(0, 0.0001^2); the full Brown model projects the points.
  */
 #ifndef MBA_SCENE_H_
 #define MBA_SCENE_H_
@@ -129,7 +136,39 @@ static inline double mba_scene_project(const double cam[9], const double X[3],
 }
 
 /* ------------------------------------------------------ track lengths --- */
-enum { MBA_DOM_CAM = 1, MBA_DOM_PT = 2, MBA_DOM_TRACK = 3 };
+enum { MBA_DOM_CAM = 1, MBA_DOM_PT = 2, MBA_DOM_TRACK = 3, MBA_DOM_GROUP = 4 };
+
+/* Projection with the full libs/sfm Brown model: camera [aa(3), t(3)],
+ * intrinsics [f, ppx, ppy, k1, k2, k3]; returns the depth (or -1 outside the
+ * 90-degree field of view). */
+static inline double mba_scene_project_brown(const double cam[6], const double intr[6], const double X[3],
+                                             double uv[2]) {
+  double x[3];
+  mba_scene_rotate(cam, X, x);
+  x[0] += cam[3];
+  x[1] += cam[4];
+  x[2] += cam[5];
+  const double xe = x[0] / x[2], ye = x[1] / x[2];
+  const double r2 = xe * xe + ye * ye;
+  const double d = 1.0 + intr[3] * r2 + intr[4] * r2 * r2 + intr[5] * r2 * r2 * r2;
+  uv[0] = intr[1] + intr[0] * xe * d;
+  uv[1] = intr[2] + intr[0] * ye * d;
+  if (fabs(xe) > 1.0 || fabs(ye) > 1.0) return -1.0;
+  return x[2];
+}
+
+/* Camera models the fill core projects with. */
+typedef struct {
+  const double* cams;   /* 9 per camera (packed) or 6 (grouped) */
+  const double* intr;   /* 6 per group, NULL for packed */
+  int32_t groups;
+} mba_scene_model;
+
+static inline double mba_scene_project_model(const mba_scene_model* m, int32_t c, const double X[3],
+                                             double uv[2]) {
+  if (!m->intr) return mba_scene_project(m->cams + 9 * (size_t)c, X, uv);
+  return mba_scene_project_brown(m->cams + 6 * (size_t)c, m->intr + 6 * (size_t)(c % m->groups), X, uv);
+}
 
 /* Track length of every point (first pass; O(num_points)).  A power law is
  * drawn by inverse CDF over the truncated discrete distribution. */
@@ -158,33 +197,13 @@ static inline int64_t mba_scene_count(const mba_scene_spec* sp,
   return n;
 }
 
-/* Fill a scene.  Arrays (caller-owned):
- *   cam_true, cam_init : 9 * num_cameras
- *   pt_true,  pt_init  : 3 * num_points
- *   obs                : 2 * num_obs
- *   cam_idx, pt_idx    : num_obs
- *   obs_offset         : num_points + 1 (filled: CSR start of each point)
- *   track_len          : num_points (from mba_scene_count)
- * Any of cam_true / pt_true may be NULL.  Returns the observation count. */
-static inline int64_t mba_scene_fill(const mba_scene_spec* sp,
-                                     const int32_t* track_len,
-                                     double* cam_true, double* cam_init,
-                                     double* pt_true, double* pt_init,
-                                     double* obs, int32_t* cam_idx,
-                                     int32_t* pt_idx, int64_t* obs_offset) {
+/* Points, tracks and observations of a scene whose cameras are `m`.  The
+ * draws depend only on (seed, point), never on the camera model's layout. */
+static inline int64_t mba_scene_fill_points(const mba_scene_spec* sp, const int32_t* track_len,
+                                            const mba_scene_model* m, double* pt_true, double* pt_init,
+                                            double* obs, int32_t* cam_idx, int32_t* pt_idx,
+                                            int64_t* obs_offset) {
   const int32_t nc = sp->num_cameras, np = sp->num_points;
-  double* ct = cam_true ? cam_true : (double*)malloc(sizeof(double) * 9 * (size_t)nc);
-  for (int32_t c = 0; c < nc; ++c) {
-    mba_rng r = mba_rng_stream(sp->seed, MBA_DOM_CAM, (uint64_t)c);
-    double* cam = ct + 9 * (size_t)c;
-    for (int k = 0; k < 3; ++k) cam[k] = 0.1 * mba_normal(&r);
-    for (int k = 3; k < 6; ++k) cam[k] = mba_normal(&r);
-    cam[6] = 450.0 + 100.0 * mba_uniform(&r);
-    cam[7] = 0.01 * mba_normal(&r);
-    cam[8] = 0.001 * mba_normal(&r);
-    for (int k = 0; k < 9; ++k)
-      cam_init[9 * (size_t)c + k] = cam[k] * (1.0 + sp->perturb * mba_normal(&r));
-  }
   obs_offset[0] = 0;
   for (int32_t p = 0; p < np; ++p) obs_offset[p + 1] = obs_offset[p] + track_len[p];
 
@@ -207,7 +226,7 @@ static inline int64_t mba_scene_fill(const mba_scene_spec* sp,
       int dup = 0;
       for (int j = 0; j < n; ++j) dup |= (cam_idx[base + j] == c);
       double uv[2];
-      const double depth = mba_scene_project(ct + 9 * (size_t)c, X, uv);
+      const double depth = mba_scene_project_model(m, c, X, uv);
       if (!dup && (depth > 0.5 || ++guard > 1000)) cam_idx[base + n++] = c;
     }
     /* Ascending camera id within a track (insertion sort, t is small). */
@@ -223,7 +242,7 @@ static inline int64_t mba_scene_fill(const mba_scene_spec* sp,
     for (int i = 0; i < t; ++i) {
       const int64_t o = base + i;
       double uv[2];
-      mba_scene_project(ct + 9 * (size_t)cam_idx[o], X, uv);
+      mba_scene_project_model(m, cam_idx[o], X, uv);
       uv[0] += sp->noise_px * mba_normal(&r);
       uv[1] += sp->noise_px * mba_normal(&r);
       if (mba_uniform(&r) < sp->outlier_frac) {
@@ -236,8 +255,89 @@ static inline int64_t mba_scene_fill(const mba_scene_spec* sp,
       pt_idx[o] = p;
     }
   }
-  if (!cam_true) free(ct);
   return obs_offset[np];
+}
+
+/* Packed 9-parameter cameras (true and perturbed). */
+static inline void mba_scene_cameras(const mba_scene_spec* sp, double* ct, double* cam_init) {
+  for (int32_t c = 0; c < sp->num_cameras; ++c) {
+    mba_rng r = mba_rng_stream(sp->seed, MBA_DOM_CAM, (uint64_t)c);
+    double* cam = ct + 9 * (size_t)c;
+    for (int k = 0; k < 3; ++k) cam[k] = 0.1 * mba_normal(&r);
+    for (int k = 3; k < 6; ++k) cam[k] = mba_normal(&r);
+    cam[6] = 450.0 + 100.0 * mba_uniform(&r);
+    cam[7] = 0.01 * mba_normal(&r);
+    cam[8] = 0.001 * mba_normal(&r);
+    for (int k = 0; k < 9; ++k)
+      cam_init[9 * (size_t)c + k] = cam[k] * (1.0 + sp->perturb * mba_normal(&r));
+  }
+}
+
+/* Fill a scene.  Arrays (caller-owned):
+ *   cam_true, cam_init : 9 * num_cameras
+ *   pt_true,  pt_init  : 3 * num_points
+ *   obs                : 2 * num_obs
+ *   cam_idx, pt_idx    : num_obs
+ *   obs_offset         : num_points + 1 (filled: CSR start of each point)
+ *   track_len          : num_points (from mba_scene_count)
+ * Any of cam_true / pt_true may be NULL.  Returns the observation count. */
+static inline int64_t mba_scene_fill(const mba_scene_spec* sp,
+                                     const int32_t* track_len,
+                                     double* cam_true, double* cam_init,
+                                     double* pt_true, double* pt_init,
+                                     double* obs, int32_t* cam_idx,
+                                     int32_t* pt_idx, int64_t* obs_offset) {
+  const int32_t nc = sp->num_cameras;
+  double* ct = cam_true ? cam_true : (double*)malloc(sizeof(double) * 9 * (size_t)nc);
+  mba_scene_cameras(sp, ct, cam_init);
+  const mba_scene_model m = {ct, NULL, 0};
+  const int64_t n = mba_scene_fill_points(sp, track_len, &m, pt_true, pt_init, obs, cam_idx, pt_idx, obs_offset);
+  if (!cam_true) free(ct);
+  return n;
+}
+
+/* Grouped-intrinsics scene.  Arrays (caller-owned):
+ *   cam_true, cam_init   : 6 * num_cameras ([aa, t])
+ *   intr_true, intr_init : 6 * groups ([f, ppx, ppy, k1, k2, k3])
+ *   intr_idx             : num_obs (group of the observing camera)
+ *   the rest as in mba_scene_fill.  cam_true / intr_true / pt_true may be
+ *   NULL.  Returns the observation count. */
+static inline int64_t mba_scene_fill_groups(const mba_scene_spec* sp, int32_t groups,
+                                            const int32_t* track_len, double* cam_true,
+                                            double* cam_init, double* intr_true, double* intr_init,
+                                            double* pt_true, double* pt_init, double* obs,
+                                            int32_t* cam_idx, int32_t* intr_idx, int32_t* pt_idx,
+                                            int64_t* obs_offset) {
+  const int32_t nc = sp->num_cameras;
+  double* c9 = (double*)malloc(sizeof(double) * 9 * (size_t)nc);
+  double* c9i = (double*)malloc(sizeof(double) * 9 * (size_t)nc);
+  mba_scene_cameras(sp, c9, c9i);
+  double* ct = cam_true ? cam_true : (double*)malloc(sizeof(double) * 6 * (size_t)nc);
+  for (int32_t c = 0; c < nc; ++c)
+    for (int k = 0; k < 6; ++k) {
+      ct[6 * (size_t)c + k] = c9[9 * (size_t)c + k];
+      cam_init[6 * (size_t)c + k] = c9i[9 * (size_t)c + k];
+    }
+  double* it = intr_true ? intr_true : (double*)malloc(sizeof(double) * 6 * (size_t)groups);
+  for (int32_t g = 0; g < groups; ++g) {
+    mba_rng r = mba_rng_stream(sp->seed, MBA_DOM_GROUP, (uint64_t)g);
+    double* q = it + 6 * (size_t)g;
+    q[0] = 450.0 + 100.0 * mba_uniform(&r);
+    q[1] = 10.0 * mba_normal(&r);
+    q[2] = 10.0 * mba_normal(&r);
+    q[3] = 0.01 * mba_normal(&r);
+    q[4] = 0.001 * mba_normal(&r);
+    q[5] = 0.0001 * mba_normal(&r);
+    for (int k = 0; k < 6; ++k) intr_init[6 * (size_t)g + k] = q[k] * (1.0 + sp->perturb * mba_normal(&r));
+  }
+  const mba_scene_model m = {ct, it, groups};
+  const int64_t n = mba_scene_fill_points(sp, track_len, &m, pt_true, pt_init, obs, cam_idx, pt_idx, obs_offset);
+  for (int64_t o = 0; o < n; ++o) intr_idx[o] = cam_idx[o] % groups;
+  free(c9);
+  free(c9i);
+  if (!cam_true) free(ct);
+  if (!intr_true) free(it);
+  return n;
 }
 
 /* The five benchmark scenes (index 1..5; 4 is one window of the batched

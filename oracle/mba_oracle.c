@@ -1531,3 +1531,10 @@ int64_t orc_scene_fill(const mba_scene_spec* sp, const int32_t* track_len, doubl
                         pt_idx, obs_offset);
 }
 mba_scene_spec orc_scene_preset(int config, int window) { return mba_scene_preset(config, window); }
+int64_t orc_scene_fill_groups(const mba_scene_spec* sp, int32_t groups, const int32_t* track_len,
+                              double* cam_true, double* cam_init, double* intr_true, double* intr_init,
+                              double* pt_true, double* pt_init, double* obs, int32_t* cam_idx,
+                              int32_t* intr_idx, int32_t* pt_idx, int64_t* obs_offset) {
+  return mba_scene_fill_groups(sp, groups, track_len, cam_true, cam_init, intr_true, intr_init, pt_true,
+                               pt_init, obs, cam_idx, intr_idx, pt_idx, obs_offset);
+}

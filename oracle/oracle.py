@@ -97,6 +97,7 @@ def lib():
         _orc.orc_scene_preset.restype = SceneSpec
         _orc.orc_scene_count.restype = C.c_int64
         _orc.orc_scene_fill.restype = C.c_int64
+        _orc.orc_scene_fill_groups.restype = C.c_int64
         _orc.orc_schur_ordering.restype = C.c_int32
         _orc.orc_schur_block_pairs.restype = C.c_int64
     return _orc
@@ -153,6 +154,34 @@ def generate(spec: SceneSpec) -> dict:
                      ptr(d["offsets"], _i64p))
     d["num_cameras"], d["num_points"], d["num_obs"] = spec.num_cameras, spec.num_points, int(nobs)
     return d
+
+
+def generate_groups(spec: SceneSpec, groups: int) -> dict:
+    """Grouped-intrinsics scene (the incremental engine's layout): cameras
+    [aa, t], intrinsic groups [f, ppx, ppy, k1, k2, k3], camera c in group
+    c mod groups."""
+    L = lib()
+    tl = np.zeros(spec.num_points, np.int32)
+    nobs = L.orc_scene_count(C.byref(spec), ptr(tl, _i32p))
+    d = dict(
+        cam_true=np.zeros((spec.num_cameras, 6)), cam_init=np.zeros((spec.num_cameras, 6)),
+        intr_true=np.zeros((groups, 6)), intr_init=np.zeros((groups, 6)),
+        pt_true=np.zeros((spec.num_points, 3)), pt_init=np.zeros((spec.num_points, 3)),
+        obs=np.zeros((nobs, 2)), cam_idx=np.zeros(nobs, np.int32), intr_idx=np.zeros(nobs, np.int32),
+        pt_idx=np.zeros(nobs, np.int32), offsets=np.zeros(spec.num_points + 1, np.int64))
+    L.orc_scene_fill_groups(C.byref(spec), C.c_int32(groups), ptr(tl, _i32p), ptr(d["cam_true"]),
+                            ptr(d["cam_init"]), ptr(d["intr_true"]), ptr(d["intr_init"]),
+                            ptr(d["pt_true"]), ptr(d["pt_init"]), ptr(d["obs"]), ptr(d["cam_idx"], _i32p),
+                            ptr(d["intr_idx"], _i32p), ptr(d["pt_idx"], _i32p), ptr(d["offsets"], _i64p))
+    d["num_cameras"], d["num_points"], d["num_obs"] = spec.num_cameras, spec.num_points, int(nobs)
+    d["num_intrinsics"] = groups
+    return d
+
+
+def group_scene(d, cam_constant=None, intr=None) -> "Scene":
+    """Oracle Scene (layout 1) of a grouped-intrinsics scene."""
+    return Scene(d, cam_constant=cam_constant, layout=1,
+                 intrinsics=d["intr_init"] if intr is None else intr, intr_idx=d["intr_idx"])
 
 
 class Scene:

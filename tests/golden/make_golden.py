@@ -26,6 +26,24 @@ HUBER_SPEC = dict(num_cameras=30, num_points=3000, track_min=2, track_max=10, po
                   outlier_frac=0.05, seed=11)
 
 
+# The incremental engine's layout: cameras [aa, t], three shared intrinsic
+# groups [f, ppx, ppy, k1, k2, k3], residual blocks (intrinsics, camera,
+# point) (libs/sfm/src/sfm_incremental_engine.cpp:1413-1446).
+GROUP_SPEC = dict(num_cameras=24, num_points=1500, track_min=2, track_max=8, outlier_frac=0.05, seed=31)
+GROUPS = 3
+GROUP_CONST = 2  # SetParameterBlockConstant on this camera's extrinsic block
+
+
+def group_scene():
+    return O.generate_groups(O.make_spec(**GROUP_SPEC), GROUPS)
+
+
+def group_const():
+    c = np.zeros(GROUP_SPEC["num_cameras"], np.uint8)
+    c[GROUP_CONST] = 1
+    return c
+
+
 def tiny_scene():
     """Hand-built: 4 cameras, 6 points; camera 3 sees only points 0 and 1."""
     spec = O.make_spec(3, 6, 2, 3, seed=5)
@@ -91,6 +109,36 @@ def main():
     out["jets_c2_residual"] = np.array(R)
     out["jets_c2_jac_camera"] = np.array(JC)
     out["jets_c2_jac_point"] = np.array(JP)
+    # 5. Grouped intrinsics (layout of the incremental engine).
+    dg = group_scene()
+    ne, order, res, pairs = O.ref_schur_ordering(O.group_scene(dg, cam_constant=group_const()))
+    out["grp_struct_num_e"] = np.asarray(ne)
+    out["grp_struct_order"] = order
+    out["grp_struct_residual_order"] = res.astype(np.int32)
+    out["grp_struct_pairs"] = pairs
+    gidx = np.sort(np.random.default_rng(3).choice(dg["num_obs"], 200, replace=False))
+    R, JI, JC, JP = [], [], [], []
+    for i in gidx:
+        r, ji, jc, jp = O.ref_evaluate_obs(1, dg["cam_init"][dg["cam_idx"][i]], dg["pt_init"][dg["pt_idx"][i]],
+                                           dg["obs"][i], dg["intr_init"][dg["intr_idx"][i]])
+        R.append(r)
+        JI.append(ji)
+        JC.append(jc)
+        JP.append(jp)
+    out["grp_jets_index"] = gidx
+    out["grp_jets_residual"] = np.array(R)
+    out["grp_jets_jac_intrinsics"] = np.array(JI)
+    out["grp_jets_jac_camera"] = np.array(JC)
+    out["grp_jets_jac_point"] = np.array(JP)
+    for name, huber, ls, cc in [("sparse", 4.0, 0, group_const()), ("iterative", 0.0, 1, None),
+                                ("iterative_const", 4.0, 1, group_const())]:
+        sc = O.group_scene(dg, cam_constant=cc)
+        r = O.ref_solve(sc, huber_a=huber, linear_solver=ls, max_iters=20)
+        for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
+            out[f"grp_{name}_{k}"] = np.asarray(r[k])
+        out[f"grp_{name}_cameras"] = sc.cameras
+        out[f"grp_{name}_intrinsics"] = sc.intrinsics
+        out[f"grp_{name}_points"] = sc.points
     path = os.path.join(HERE, "ceres_ref.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")

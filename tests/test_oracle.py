@@ -9,9 +9,14 @@ oracle's own formula:
   - dense normal equations in numpy for the Schur complement;
   - a hand-worked ordering example.
 """
+import os
+import sys
+
 import numpy as np
 import pytest
 from scipy.spatial.transform import Rotation
+
+from conftest import ROOT
 
 C1_ITERS = 20
 
@@ -232,3 +237,68 @@ def test_lm_nonfinite_input_fails(O, c1):
     pts[5, 2] = np.nan
     s, _ = O.solve(O.Scene(c1, cameras=cams, points=pts), O.default_options(max_num_iterations=5))
     assert s.termination_type == O.FAILURE
+
+
+# ------------------------------------- grouped intrinsics (layout 1) -----
+@pytest.fixture(scope="module")
+def group_scene():
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_golden
+    return make_golden.group_scene(), make_golden.group_const()
+
+
+def test_grouped_jets_match_reference_ceres_autodiff(O, golden, group_scene):
+    """ErrorFunc_Refine_Camera_3DPoints (intrinsics, extrinsics, point) against
+    the reference Ceres autodiff on 200 observations."""
+    d, _ = group_scene
+    for k, i in enumerate(golden["grp_jets_index"]):
+        r, ji, jc, jp = O.evaluate_obs(1, d["cam_init"][d["cam_idx"][i]], d["pt_init"][d["pt_idx"][i]], d["obs"][i],
+                                       d["intr_init"][d["intr_idx"][i]])
+        np.testing.assert_allclose(r, golden["grp_jets_residual"][k], rtol=1e-12, atol=1e-10)
+        np.testing.assert_allclose(ji, golden["grp_jets_jac_intrinsics"][k], rtol=1e-12, atol=1e-10)
+        np.testing.assert_allclose(jc, golden["grp_jets_jac_camera"][k], rtol=1e-12, atol=1e-9)
+        np.testing.assert_allclose(jp, golden["grp_jets_jac_point"][k], rtol=1e-12, atol=1e-10)
+
+
+def test_grouped_ordering_and_pairs_match_reference(O, golden, group_scene):
+    d, const = group_scene
+    sc = O.group_scene(d, cam_constant=const)
+    ne, order, res = O.schur_ordering(sc)
+    assert ne == int(golden["grp_struct_num_e"])
+    np.testing.assert_array_equal(order, golden["grp_struct_order"])
+    np.testing.assert_array_equal(res, golden["grp_struct_residual_order"])
+    np.testing.assert_array_equal(O.schur_block_pairs(sc), golden["grp_struct_pairs"])
+
+
+@pytest.mark.parametrize("case,huber,solver,const", [("sparse", 4.0, 0, True), ("iterative", 0.0, 1, False)])
+def test_grouped_trajectory_matches_reference(O, golden, group_scene, case, huber, solver, const):
+    d, cc = group_scene
+    sc = O.group_scene(d, cam_constant=cc if const else None)
+    s, its = O.solve(sc, O.default_options(max_num_iterations=20, linear_solver=solver, huber_a=huber))
+    ref = golden[f"grp_{case}_iterations"]
+    assert len(its) == ref.shape[0]
+    assert (s.num_successful_steps, s.num_unsuccessful_steps, s.termination_type) == (
+        int(golden[f"grp_{case}_successful"]), int(golden[f"grp_{case}_unsuccessful"]),
+        int(golden[f"grp_{case}_termination"]))
+    for it, row in zip(its, ref):
+        np.testing.assert_allclose(it.cost, row[1], rtol=1e-10)
+        assert it.linear_solver_iterations == int(row[6])
+    # The run stops at 20 iterations while still stepping (|step| ~ 0.02) along
+    # weakly determined directions, where rounding moves parameters by ~1e-6
+    # at equal cost: absolute tolerances for the poses and points.
+    np.testing.assert_allclose(sc.cameras, golden[f"grp_{case}_cameras"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(sc.intrinsics, golden[f"grp_{case}_intrinsics"], rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(sc.points, golden[f"grp_{case}_points"], rtol=0, atol=1e-5)
+
+
+def test_grouped_iterative_with_constant_camera_matches_reference(O, golden, group_scene):
+    """With a constant camera the 39-iteration CG solves amplify summation-order
+    rounding: the first iterations agree to 1e-10, the final cost to 1e-6,
+    CG counts exactly."""
+    d, cc = group_scene
+    s, its = O.solve(O.group_scene(d, cam_constant=cc), O.default_options(max_num_iterations=20, linear_solver=1,
+                                                                           huber_a=4.0))
+    ref = golden["grp_iterative_const_iterations"]
+    np.testing.assert_allclose([it.cost for it in its[:4]], ref[:4, 1], rtol=1e-10)
+    assert [it.linear_solver_iterations for it in its] == [int(v) for v in ref[:, 6]]
+    np.testing.assert_allclose(s.final_cost, float(golden["grp_iterative_const_final_cost"]), rtol=1e-6)
