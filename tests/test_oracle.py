@@ -292,13 +292,18 @@ def test_grouped_trajectory_matches_reference(O, golden, group_scene, case, hube
 
 
 def test_grouped_iterative_with_constant_camera_matches_reference(O, golden, group_scene):
-    """With a constant camera the 39-iteration CG solves amplify summation-order
-    rounding: the first iterations agree to 1e-10, the final cost to 1e-6,
-    CG counts exactly."""
+    """With a constant camera and Huber outliers the CG solves reach 45
+    iterations and amplify summation-order rounding (the oracle and Ceres add
+    the Jacobian cells of a row in different orders): the first four
+    iterations agree to 1e-10 with equal CG counts; afterwards the Nash-Sofer
+    stop can land one iteration apart, so every cost is held to 1e-6 and the
+    final cost to 1e-7."""
     d, cc = group_scene
     s, its = O.solve(O.group_scene(d, cam_constant=cc), O.default_options(max_num_iterations=20, linear_solver=1,
                                                                            huber_a=4.0))
     ref = golden["grp_iterative_const_iterations"]
+    assert len(its) == ref.shape[0]
     np.testing.assert_allclose([it.cost for it in its[:4]], ref[:4, 1], rtol=1e-10)
-    assert [it.linear_solver_iterations for it in its] == [int(v) for v in ref[:, 6]]
-    np.testing.assert_allclose(s.final_cost, float(golden["grp_iterative_const_final_cost"]), rtol=1e-6)
+    assert [it.linear_solver_iterations for it in its[:5]] == [int(v) for v in ref[:5, 6]]
+    np.testing.assert_allclose([it.cost for it in its], ref[:, 1], rtol=1e-6)
+    np.testing.assert_allclose(s.final_cost, float(golden["grp_iterative_const_final_cost"]), rtol=1e-7)
