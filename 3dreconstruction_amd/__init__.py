@@ -33,6 +33,7 @@ OK = 0
 ERR_INVALID_ARGUMENT, ERR_UNSUPPORTED, ERR_DEVICE, ERR_OUT_OF_MEMORY, ERR_NOT_FINITE = -1, -2, -3, -4, -5
 CONVERGENCE, NO_CONVERGENCE, FAILURE = 0, 1, 2
 LAYOUT_PACKED9 = 0
+LAYOUT_GROUPED = 1
 EXPLICIT_SCHUR, IMPLICIT_SCHUR_PCG = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
@@ -57,9 +58,10 @@ class MbaError(RuntimeError):
 
 class _Problem(C.Structure):
     _fields_ = [("layout", C.c_int32), ("num_cameras", C.c_int32), ("num_points", C.c_int32),
-                ("reserved0", C.c_int32), ("num_observations", C.c_int64), ("cameras", _f64p),
+                ("num_intrinsics", C.c_int32), ("num_observations", C.c_int64), ("cameras", _f64p),
                 ("points", _f64p), ("observations", _f64p), ("camera_index", _i32p),
-                ("point_index", _i32p), ("camera_constant", _u8p)]
+                ("point_index", _i32p), ("camera_constant", _u8p), ("intrinsics", _f64p),
+                ("intrinsic_index", _i32p), ("intrinsic_constant", _u8p), ("intrinsic_subset", _u8p)]
 
 
 class Options(C.Structure):
@@ -98,7 +100,8 @@ _lib = None
 
 #: every exported symbol of include/mba.h
 EXPORTS = [
-    "mba_default_options", "mba_solve", "mba_solve_batched", "mba_evaluate", "mba_schur_structure",
+    "mba_default_options", "mba_solve", "mba_solve_batched", "mba_evaluate", "mba_evaluate_blocks",
+    "mba_schur_structure",
     "mba_context_create", "mba_context_destroy", "mba_context_jacobian_schur",
     "mba_context_read_schur", "mba_context_read_scale", "mba_context_schur_matvec",
     "mba_context_solve", "mba_context_download", "mba_context_synchronize", "mba_context_stream",
@@ -137,6 +140,8 @@ def load_library():
     lib.mba_context_download.argtypes = [C.c_void_p, C.POINTER(_Problem)]
     lib.mba_context_stream.argtypes = [C.c_void_p]
     lib.mba_evaluate.argtypes = [C.POINTER(_Problem), C.c_int32, C.c_double, _f64p, _f64p, _f64p, _f64p, C.c_int32]
+    lib.mba_evaluate_blocks.argtypes = [C.POINTER(_Problem), C.c_int32, C.c_double, _f64p, _f64p, _f64p, _f64p,
+                                        _f64p, C.c_int32]
     lib.mba_scene_generate.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _i64p, _f64p, _f64p, _f64p,
                                        _f64p, _f64p, _i32p, _i32p]
     _lib = lib
@@ -165,9 +170,13 @@ def default_options(**kw) -> Options:
 
 # ------------------------------------------------------------ problems --
 class BAProblem:
-    """Arrays of a packed-9 bundle-adjustment problem (caller-owned)."""
+    """Arrays of a bundle-adjustment problem (caller-owned).  Packed layout:
+    cameras (Nc, 9).  Grouped layout (give `intrinsics`): cameras (Nc, 6)
+    [aa, t], intrinsics (Ni, 6) [f, ppx, ppy, k1, k2, k3] indexed per
+    observation by `intrinsic_index`."""
 
-    def __init__(self, cameras, points, observations, camera_index, point_index, camera_constant=None):
+    def __init__(self, cameras, points, observations, camera_index, point_index, camera_constant=None,
+                 intrinsics=None, intrinsic_index=None, intrinsic_constant=None, intrinsic_subset=None):
         self.cameras = np.ascontiguousarray(cameras, np.float64)
         self.points = np.ascontiguousarray(points, np.float64)
         self.observations = np.ascontiguousarray(observations, np.float64)
@@ -175,8 +184,22 @@ class BAProblem:
         self.point_index = np.ascontiguousarray(point_index, np.int32)
         self.camera_constant = (None if camera_constant is None
                                 else np.ascontiguousarray(camera_constant, np.uint8))
-        if self.cameras.ndim != 2 or self.cameras.shape[1] != 9:
-            raise ValueError("cameras must be (Nc, 9)")
+        self.layout = LAYOUT_PACKED9 if intrinsics is None else LAYOUT_GROUPED
+        self.intrinsics = None if intrinsics is None else np.ascontiguousarray(intrinsics, np.float64)
+        self.intrinsic_index = (None if intrinsic_index is None
+                                else np.ascontiguousarray(intrinsic_index, np.int32))
+        self.intrinsic_constant = (None if intrinsic_constant is None
+                                   else np.ascontiguousarray(intrinsic_constant, np.uint8))
+        self.intrinsic_subset = (None if intrinsic_subset is None
+                                 else np.ascontiguousarray(intrinsic_subset, np.uint8))
+        width = 9 if self.layout == LAYOUT_PACKED9 else 6
+        if self.cameras.ndim != 2 or self.cameras.shape[1] != width:
+            raise ValueError(f"cameras must be (Nc, {width})")
+        if self.layout == LAYOUT_GROUPED:
+            if self.intrinsics.ndim != 2 or self.intrinsics.shape[1] != 6:
+                raise ValueError("intrinsics must be (Ni, 6)")
+            if self.intrinsic_index is None or self.intrinsic_index.shape != self.camera_index.shape:
+                raise ValueError("intrinsic_index must give a group per observation")
         if self.points.ndim != 2 or self.points.shape[1] != 3:
             raise ValueError("points must be (Np, 3)")
         if self.observations.shape != (self.camera_index.shape[0], 2):
@@ -194,13 +217,22 @@ class BAProblem:
     def num_observations(self):
         return self.observations.shape[0]
 
+    @property
+    def num_intrinsics(self):
+        return 0 if self.intrinsics is None else self.intrinsics.shape[0]
+
     def _struct(self) -> _Problem:
         s = _Problem()
-        s.layout = LAYOUT_PACKED9
+        s.layout = self.layout
         s.num_cameras, s.num_points, s.num_observations = self.num_cameras, self.num_points, self.num_observations
+        s.num_intrinsics = self.num_intrinsics
         s.cameras, s.points, s.observations = _p(self.cameras), _p(self.points), _p(self.observations)
         s.camera_index, s.point_index = _p(self.camera_index, _i32p), _p(self.point_index, _i32p)
         s.camera_constant = _p(self.camera_constant, _u8p)
+        s.intrinsics = _p(self.intrinsics)
+        s.intrinsic_index = _p(self.intrinsic_index, _i32p)
+        s.intrinsic_constant = _p(self.intrinsic_constant, _u8p)
+        s.intrinsic_subset = _p(self.intrinsic_subset, _u8p)
         return s
 
 
@@ -265,11 +297,28 @@ def evaluate(problem: BAProblem, precision=PRECISION_FP64, huber_a=0.0, jacobian
     return r, jc, jp, cost.value
 
 
+def evaluate_blocks(problem: BAProblem, precision=PRECISION_FP64, huber_a=0.0, device=0):
+    """Residuals and per-block Jacobians on the GPU: (r, J_intrinsics or None,
+    J_camera, J_point, cost)."""
+    lib = load_library()
+    n = problem.num_observations
+    r = np.zeros((n, 2))
+    grouped = problem.layout == LAYOUT_GROUPED
+    ji = np.zeros((n, 2, 6)) if grouped else None
+    jc = np.zeros((n, 2, 6 if grouped else 9))
+    jp = np.zeros((n, 2, 3))
+    cost = C.c_double()
+    s = problem._struct()
+    _check(lib.mba_evaluate_blocks(C.byref(s), C.c_int32(precision), C.c_double(huber_a), _p(r), _p(ji), _p(jc),
+                                   _p(jp), C.byref(cost), C.c_int32(device)))
+    return r, ji, jc, jp, cost.value
+
+
 def schur_structure(problem: BAProblem, with_pairs=True):
     """(num_e, column-block order, residual order, S block pairs)."""
     lib = load_library()
     s = problem._struct()
-    nb = problem.num_cameras + problem.num_points
+    nb = problem.num_intrinsics + problem.num_cameras + problem.num_points
     order = np.zeros(nb, np.int32)
     res = np.zeros(problem.num_observations, np.int64)
     nblk, ne, npairs = C.c_int32(), C.c_int32(), C.c_int64()

@@ -11,9 +11,17 @@
 //   cm rows   -- the same observations sorted by camera (F block) and cut
 //                into chunks of <= chunk_rows rows; one workgroup reduces one
 //                chunk, so per-camera 9x9 sums need no global atomics;
-//   camera table -- per camera R, dR/daa_k, t, f, k1, k2, recomputed from
-//                the FP64 parameters whenever they change; the Jacobian is
-//                recomputed from it in every pass instead of being stored.
+//   camera table -- per camera R, dR/daa_k, t and the intrinsics,
+//                recomputed from the FP64 parameters whenever they change;
+//                the Jacobian is recomputed from it in every pass instead of
+//                being stored.
+//
+// Layouts (template parameter L of the per-observation kernels): L = 0 is
+// the packed camera [aa, t, f, k1, k2] (principal point 0, k3 = 0); L = 1 is
+// the grouped layout, camera [aa, t] plus a shared intrinsic group [f, ppx,
+// ppy, k1, k2, k3].  Every F block (a camera, or a group in L = 1) keeps 9
+// slots in F-space vectors; a grouped camera uses slots [aa, t] and a group
+// [f, ppx, ppy, k1, k2, k3], the rest have zero Jacobian columns.
 #ifndef MBA_DEVICE_H_
 #define MBA_DEVICE_H_
 
@@ -24,12 +32,16 @@
 
 namespace mba {
 
-// Camera table row: the projection part first, so a row-wise gather of it
-// touches one 64-byte (FP32) / 128-byte (FP64) line:
-//   R 9 | t 3 | f k1 k2 | pad | dR 27 | pad 5.
+// Camera table row: the packed projection part first, so a row-wise gather
+// of it touches one 64-byte (FP32) / 128-byte (FP64) line:
+//   R 9 | t 3 | f k1 k2 k3 | dR 27 | ppx ppy | pad 3
+// (k3, ppx, ppy are zero for the packed layout, whose kernels never read them).
 constexpr int kCamTab = 48;
-constexpr int kTabT = 9, kTabF = 12, kTabDR = 16;
-constexpr int kDirTab = 16;  // M 9 | dt 3 | dintr 3 | pad
+constexpr int kTabT = 9, kTabF = 12, kTabK3 = 15, kTabDR = 16, kTabPP = 43;
+constexpr int kDirTab = 16;   // M 9 | dt 3 | df dk1 dk2 | pad
+constexpr int kDirTabG = 24;  // M 9 | dt 3 | df dppx dppy dk1 dk2 dk3 | pad
+template <int L>
+__host__ __device__ constexpr int dir_stride() { return L == 0 ? kDirTab : kDirTabG; }
 constexpr int kPartial = 64; // U 45 | g 9 | h 9 | pad
 constexpr int kBlock = 256;
 
@@ -230,17 +242,17 @@ __global__ __launch_bounds__(kBlock) void k_cam_table(int nc, const double* __re
   T* o = tab + (size_t)kCamTab * c;
   for (int i = 0; i < 9; ++i) o[i] = (T)R[i];
   for (int i = 0; i < 6; ++i) o[kTabT + i] = (T)p[3 + i];
-  o[15] = (T)0;
+  o[kTabK3] = (T)0;
   for (int i = 0; i < 27; ++i) o[kTabDR + i] = (T)dR[i];
   for (int i = kTabDR + 27; i < kCamTab; ++i) o[i] = (T)0;
 }
 
 template <typename T>
 struct CamProj {
-  T R[9], t[3], f, k1, k2;
+  T R[9], t[3], f, k1, k2, k3, ppx, ppy;
 };
 
-template <typename T>
+template <typename T, int L = 0>
 __device__ __forceinline__ void load_proj(const T* __restrict__ tab, CamProj<T>& c) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) c.R[i] = tab[i];
@@ -249,19 +261,25 @@ __device__ __forceinline__ void load_proj(const T* __restrict__ tab, CamProj<T>&
   c.f = tab[kTabF];
   c.k1 = tab[kTabF + 1];
   c.k2 = tab[kTabF + 2];
+  if constexpr (L == 1) {
+    c.k3 = tab[kTabK3];
+    c.ppx = tab[kTabPP];
+    c.ppy = tab[kTabPP + 1];
+  }
 }
 
 template <typename T>
 struct Proj {
   T rx, ry;          // residual, predicted - observed
   T P[6];            // d r / d Xc (2 x 3)
-  T jf[2], jk1[2], jk2[2];
+  T jf[2], jk1[2], jk2[2], jk3[2];
 };
 
 // Residual of libs/sfm's functor (pinhole_brown_rt_ceres_functor.h:60-107,
-// brown_pinhole_camera.h:14-41, principal point 0, k3 = 0) and its analytic
-// derivative with respect to the camera-frame point.
-template <typename T>
+// brown_pinhole_camera.h:14-41) and its analytic derivative with respect to
+// the camera-frame point: pred = pp + f d(r^2) (x/z, y/z) with
+// d = 1 + k1 r^2 + k2 r^4 + k3 r^6 (L = 0: principal point 0, k3 = 0).
+template <typename T, int L = 0>
 __device__ __forceinline__ void project(const CamProj<T>& c, const T X[3], T ox, T oy, Proj<T>& o) {
   const T xc = c.R[0] * X[0] + c.R[1] * X[1] + c.R[2] * X[2] + c.t[0];
   const T yc = c.R[3] * X[0] + c.R[4] * X[1] + c.R[5] * X[2] + c.t[1];
@@ -269,11 +287,22 @@ __device__ __forceinline__ void project(const CamProj<T>& c, const T X[3], T ox,
   const T iz = T(1) / zc;
   const T xe = xc * iz, ye = yc * iz;
   const T r2 = xe * xe + ye * ye;
-  const T d = T(1) + r2 * (c.k1 + c.k2 * r2);
+  T d, fdd;
+  if constexpr (L == 0) {
+    d = T(1) + r2 * (c.k1 + c.k2 * r2);
+    fdd = c.f * T(2) * (c.k1 + T(2) * c.k2 * r2);
+  } else {
+    d = T(1) + r2 * (c.k1 + r2 * (c.k2 + r2 * c.k3));
+    fdd = c.f * T(2) * (c.k1 + r2 * (T(2) * c.k2 + T(3) * c.k3 * r2));
+  }
   const T fd = c.f * d;
-  o.rx = fd * xe - ox;
-  o.ry = fd * ye - oy;
-  const T fdd = c.f * T(2) * (c.k1 + T(2) * c.k2 * r2);
+  if constexpr (L == 0) {
+    o.rx = fd * xe - ox;
+    o.ry = fd * ye - oy;
+  } else {
+    o.rx = fd * xe + c.ppx - ox;
+    o.ry = fd * ye + c.ppy - oy;
+  }
   const T a11 = fd + fdd * xe * xe, a12 = fdd * xe * ye, a22 = fd + fdd * ye * ye;
   o.P[0] = a11 * iz;
   o.P[1] = a12 * iz;
@@ -287,6 +316,10 @@ __device__ __forceinline__ void project(const CamProj<T>& c, const T X[3], T ox,
   o.jk1[1] = c.f * ye * r2;
   o.jk2[0] = o.jk1[0] * r2;
   o.jk2[1] = o.jk1[1] * r2;
+  if constexpr (L == 1) {
+    o.jk3[0] = o.jk2[0] * r2;
+    o.jk3[1] = o.jk2[1] * r2;
+  }
 }
 
 // HuberLoss + Corrector for rho'' <= 0 (loss_function.cc:47-61,
@@ -347,16 +380,25 @@ __device__ __forceinline__ void jac_cam(const Proj<T>& o, const T* dR, const T X
 }
 
 // J_c x through a direction table (M X + dt, intrinsics), weight applied.
-template <typename T>
+// L = 1: the intrinsic part is the row's group direction
+// [df, dppx, dppy, dk1, dk2, dk3].
+template <typename T, int L = 0>
 __device__ __forceinline__ void jac_cam_dir(const Proj<T>& o, const T* __restrict__ dt, const T X[3],
                                             T wgt, T out[2]) {
   const T v0 = dt[0] * X[0] + dt[1] * X[1] + dt[2] * X[2] + dt[9];
   const T v1 = dt[3] * X[0] + dt[4] * X[1] + dt[5] * X[2] + dt[10];
   const T v2 = dt[6] * X[0] + dt[7] * X[1] + dt[8] * X[2] + dt[11];
-  out[0] = wgt * (o.P[0] * v0 + o.P[1] * v1 + o.P[2] * v2 + o.jf[0] * dt[12] + o.jk1[0] * dt[13] +
-                  o.jk2[0] * dt[14]);
-  out[1] = wgt * (o.P[3] * v0 + o.P[4] * v1 + o.P[5] * v2 + o.jf[1] * dt[12] + o.jk1[1] * dt[13] +
-                  o.jk2[1] * dt[14]);
+  if constexpr (L == 0) {
+    out[0] = wgt * (o.P[0] * v0 + o.P[1] * v1 + o.P[2] * v2 + o.jf[0] * dt[12] + o.jk1[0] * dt[13] +
+                    o.jk2[0] * dt[14]);
+    out[1] = wgt * (o.P[3] * v0 + o.P[4] * v1 + o.P[5] * v2 + o.jf[1] * dt[12] + o.jk1[1] * dt[13] +
+                    o.jk2[1] * dt[14]);
+  } else {
+    out[0] = wgt * (o.P[0] * v0 + o.P[1] * v1 + o.P[2] * v2 + o.jf[0] * dt[12] + dt[13] + o.jk1[0] * dt[15] +
+                    o.jk2[0] * dt[16] + o.jk3[0] * dt[17]);
+    out[1] = wgt * (o.P[3] * v0 + o.P[4] * v1 + o.P[5] * v2 + o.jf[1] * dt[12] + dt[14] + o.jk1[1] * dt[15] +
+                    o.jk2[1] * dt[16] + o.jk3[1] * dt[17]);
+  }
 }
 
 // Packed point record read by the camera-major kernels: [X(3), v(3), 0, 0]
@@ -464,6 +506,7 @@ __device__ __forceinline__ void chol9_inv_col(const double l[45], int c, double 
 
 // --------------------------------------------------------------- FP64 cost
 // One thread per E block; block partial sums of 1/2 rho(|r|^2).
+template <int L>
 __global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __restrict__ pt_start,
                                                    const int32_t* __restrict__ row_cam,
                                                    const double* __restrict__ row_obs,
@@ -480,9 +523,9 @@ __global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __rest
     int bad = 0;
     for (int r = pt_start[e] + sp.sub; r < pt_start[e + 1]; r += sp.lanes) {
       CamProj<double> c;
-      load_proj(ctab + (size_t)kCamTab * row_cam[r], c);
+      load_proj<double, L>(ctab + (size_t)kCamTab * row_cam[r], c);
       Proj<double> o;
-      project(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
+      project<double, L>(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
       const double s = o.rx * o.rx + o.ry * o.ry;
       if (!isfinite(s)) bad = 1;
       acc[0] += huber_cost(s, huber_a);
@@ -597,14 +640,14 @@ __global__ __launch_bounds__(kBlock) void k_grad_final(const double* __restrict_
 // ------------------------------------------------------------- point pass
 // V = sum Jp'Jp, g_p = sum Jp' r; LM diagonal of points; Vinv = (V + D^2)^-1
 // in FP64; w = Vinv g_p.
-template <typename T>
+template <typename T, int L>
 __device__ __forceinline__ void point_row_accumulate(const T* __restrict__ ctab, int cam, T ox, T oy,
                                                      const T X[3], const T sp[3], T huber_a, T v[6],
                                                      T g[3]) {
   CamProj<T> c;
-  load_proj(ctab + (size_t)kCamTab * cam, c);
+  load_proj<T, L>(ctab + (size_t)kCamTab * cam, c);
   Proj<T> o;
-  project(c, X, ox, oy, o);
+  project<T, L>(c, X, ox, oy, o);
   const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
   T Jp[6];
   jac_point(o, c.R, wgt, sp, Jp);
@@ -620,7 +663,7 @@ __device__ __forceinline__ void point_row_accumulate(const T* __restrict__ ctab,
   fma2(g[2], Jp[2], rx, Jp[5], ry);
 }
 
-template <typename T>
+template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_point_pass(
     Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const double* __restrict__ pts64, const T* __restrict__ pscale,
@@ -635,17 +678,19 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
   T v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
   if (q.valid) {
     const int r1 = pt_start[e + 1];
-    const int L = q.lanes;
+    const int ln = q.lanes;
     int r = pt_start[e] + q.sub;
     // Two rows per iteration so both rows' loads are in flight together.
-    for (; r + L < r1; r += 2 * L) {
-      const int c0 = row_cam[r], c1 = row_cam[r + L];
+    for (; r + ln < r1; r += 2 * ln) {
+      const int c0 = row_cam[r], c1 = row_cam[r + ln];
       const T o0x = row_obs[2 * (size_t)r], o0y = row_obs[2 * (size_t)r + 1];
-      const T o1x = row_obs[2 * (size_t)(r + L)], o1y = row_obs[2 * (size_t)(r + L) + 1];
-      point_row_accumulate(ctab, c0, o0x, o0y, X, sp, huber_a, v, g);
-      point_row_accumulate(ctab, c1, o1x, o1y, X, sp, huber_a, v, g);
+      const T o1x = row_obs[2 * (size_t)(r + ln)], o1y = row_obs[2 * (size_t)(r + ln) + 1];
+      point_row_accumulate<T, L>(ctab, c0, o0x, o0y, X, sp, huber_a, v, g);
+      point_row_accumulate<T, L>(ctab, c1, o1x, o1y, X, sp, huber_a, v, g);
     }
-    if (r < r1) point_row_accumulate(ctab, row_cam[r], row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], X, sp, huber_a, v, g);
+    if (r < r1)
+      point_row_accumulate<T, L>(ctab, row_cam[r], row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], X, sp,
+                                 huber_a, v, g);
   }
 #pragma unroll
   for (int i = 0; i < 6; ++i) v[i] = group_sum(v[i], q.lanes);
@@ -800,9 +845,10 @@ __global__ __launch_bounds__(kBlock) void k_cam_pass(
 // One wave per F block: lanes sum the chunk partials (one value each), lane
 // 0 forms the LM diagonal and the FP64 inverse of U + D^2 (block-Jacobi
 // preconditioner); rhs = h.
+// The partials of F block f are rows [f_lo[f], f_hi[f]) of `partial`.
 template <typename T>
 __global__ __launch_bounds__(64) void k_cam_finalize(
-    int nf, const int32_t* __restrict__ f_chunk_start, const T* __restrict__ partial,
+    int nf, const int32_t* __restrict__ f_lo, const int32_t* __restrict__ f_hi, const T* __restrict__ partial,
     const T* __restrict__ cscale, Win win, double min_diag, double max_diag,
     double* __restrict__ diag_c, int recompute_diag, int with_h, T* __restrict__ U, T* __restrict__ gc,
     T* __restrict__ rhs, T* __restrict__ Minv) {
@@ -822,7 +868,7 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
   __syncthreads();
   double v = 0.0;
   if (k < 63) {
-    for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch) v += (double)partial[(size_t)kPartial * ch + k];
+    for (int ch = f_lo[f]; ch < f_hi[f]; ++ch) v += (double)partial[(size_t)kPartial * ch + k];
     // Partials carry the unscaled camera columns: scale entry (a, b) by
     // s_a s_b, vectors by s_a.
     const int a = ij[k] >> 4, b = ij[k] & 15;
@@ -926,7 +972,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_dir(int nc, const int32_t* __res
 
 // Per E block: q = sum_rows Jp'(beta r + Jc x) (x via dtab); z = Vinv q
 // (apply_vinv) or q; pk's v part (if pk) becomes s_p .* z.
-template <typename T>
+template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_mv_point(
     Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
@@ -943,14 +989,14 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
   T q[3] = {0, 0, 0};
   auto row = [&](int cam, T ox, T oy) {
     CamProj<T> c;
-    load_proj(ctab + (size_t)kCamTab * cam, c);
+    load_proj<T, L>(ctab + (size_t)kCamTab * cam, c);
     Proj<T> o;
-    project(c, X, ox, oy, o);
+    project<T, L>(c, X, ox, oy, o);
     const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
     T Jp[6];
     jac_point(o, c.R, wgt, sp, Jp);
     T v[2];
-    jac_cam_dir(o, dtab + (size_t)kDirTab * cam, X, wgt, v);
+    jac_cam_dir<T, L>(o, dtab + (size_t)dir_stride<L>() * cam, X, wgt, v);
     v[0] += beta * wgt * o.rx;
     v[1] += beta * wgt * o.ry;
     fma2(q[0], Jp[0], v[0], Jp[3], v[1]);
@@ -958,13 +1004,13 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
     fma2(q[2], Jp[2], v[0], Jp[5], v[1]);
   };
   if (qp.valid) {
-    const int r1 = pt_start[e + 1], L = qp.lanes;
+    const int r1 = pt_start[e + 1], ln = qp.lanes;
     int r = pt_start[e] + qp.sub;
     // Two rows per iteration: both rows' loads are in flight together.
-    for (; r + L < r1; r += 2 * L) {
-      const int c0 = row_cam[r], c1 = row_cam[r + L];
+    for (; r + ln < r1; r += 2 * ln) {
+      const int c0 = row_cam[r], c1 = row_cam[r + ln];
       const T o0x = row_obs[2 * (size_t)r], o0y = row_obs[2 * (size_t)r + 1];
-      const T o1x = row_obs[2 * (size_t)(r + L)], o1y = row_obs[2 * (size_t)(r + L) + 1];
+      const T o1x = row_obs[2 * (size_t)(r + ln)], o1y = row_obs[2 * (size_t)(r + ln) + 1];
       row(c0, o0x, o0y);
       row(c1, o1x, o1y);
     }
@@ -1051,10 +1097,11 @@ __global__ __launch_bounds__(kBlock) void k_mv_cam(
   }
 }
 
-// y_f = [u_term] (U + D^2) x_f - sum partials.  `partial_sign`: +1 / -1.
-template <typename T>
+// y_f = [u_term] (U + D^2) x_f - sum partials (L = 0), or D^2 x_f + sum
+// partials (L = 1, partials of F'(F x - E z)).
+template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_mv_finalize(
-    int nf, const int32_t* __restrict__ f_chunk_start, const T* __restrict__ partial,
+    int nf, const int32_t* __restrict__ f_lo, const int32_t* __restrict__ f_hi, const T* __restrict__ partial,
     const T* __restrict__ U, const double* __restrict__ diag_c, Win win,
     const T* __restrict__ x, int u_term, T* __restrict__ y, const int32_t* __restrict__ all_done) {
   if (all_done && *all_done) return;
@@ -1063,11 +1110,13 @@ __global__ __launch_bounds__(kBlock) void k_mv_finalize(
   const double inv_radius = win.inv_r[win_of_f(win, f)];
   T s[9];
   for (int k = 0; k < 9; ++k) s[k] = T(0);
-  for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch)
+  for (int ch = f_lo[f]; ch < f_hi[f]; ++ch)
     for (int k = 0; k < 9; ++k) s[k] += partial[(size_t)kPartial * ch + k];
-  if (u_term) {
+  const T* xf = x + 9 * (size_t)f;
+  if (L == 1) {
+    for (int i = 0; i < 9; ++i) y[9 * (size_t)f + i] = (T)(diag_c[9 * (size_t)f + i] * inv_radius) * xf[i] + s[i];
+  } else if (u_term) {
     const T* u = U + 45 * (size_t)f;
-    const T* xf = x + 9 * (size_t)f;
     for (int i = 0; i < 9; ++i) {
       T a = (T)(diag_c[9 * (size_t)f + i] * inv_radius) * xf[i];
       for (int j = 0; j < 9; ++j) a += u[i <= j ? ut9(i, j) : ut9(j, i)] * xf[j];
@@ -1078,9 +1127,11 @@ __global__ __launch_bounds__(kBlock) void k_mv_finalize(
   }
 }
 
-// rhs_f = g_f - sum partials.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_rhs_finalize(int nf, const int32_t* __restrict__ f_chunk_start,
+// rhs_f = g_f - sum partials (L = 0: partials of Jc'Jp w; L = 1: partials
+// already carry the sign, -F'E w).
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_rhs_finalize(int nf, const int32_t* __restrict__ f_lo,
+                                                         const int32_t* __restrict__ f_hi,
                                                          const T* __restrict__ partial,
                                                          const T* __restrict__ gc,
                                                          T* __restrict__ rhs, Win win) {
@@ -1088,14 +1139,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs_finalize(int nf, const int32_t* 
   if (f >= nf || win_idle_f(win, f)) return;
   T s[9];
   for (int k = 0; k < 9; ++k) s[k] = T(0);
-  for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch)
+  for (int ch = f_lo[f]; ch < f_hi[f]; ++ch)
     for (int k = 0; k < 9; ++k) s[k] += partial[(size_t)kPartial * ch + k];
-  for (int k = 0; k < 9; ++k) rhs[9 * (size_t)f + k] = gc[9 * (size_t)f + k] - s[k];
+  for (int k = 0; k < 9; ++k) rhs[9 * (size_t)f + k] = L == 0 ? gc[9 * (size_t)f + k] - s[k] : gc[9 * (size_t)f + k] + s[k];
 }
 
 // Model cost change terms: sum over rows of m.(r + m/2) with m = J step
 // (camera part via dtab, point part from step_p); FP64 accumulation.
-template <typename T>
+template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_model_cost(
     Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
@@ -1114,14 +1165,14 @@ __global__ __launch_bounds__(kBlock) void k_model_cost(
     for (int r = pt_start[e] + qp.sub; r < pt_start[e + 1]; r += qp.lanes) {
       const int cam = row_cam[r];
       CamProj<T> c;
-      load_proj(ctab + (size_t)kCamTab * cam, c);
+      load_proj<T, L>(ctab + (size_t)kCamTab * cam, c);
       Proj<T> o;
-      project(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
+      project<T, L>(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
       const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
       T Jp[6];
       jac_point(o, c.R, wgt, sp, Jp);
       T m[2];
-      jac_cam_dir(o, dtab + (size_t)kDirTab * cam, X, wgt, m);
+      jac_cam_dir<T, L>(o, dtab + (size_t)dir_stride<L>() * cam, X, wgt, m);
       m[0] += Jp[0] * dp[0] + Jp[1] * dp[1] + Jp[2] * dp[2];
       m[1] += Jp[3] * dp[0] + Jp[4] * dp[1] + Jp[5] * dp[2];
       const double m0 = m[0], m1 = m[1];
@@ -1425,10 +1476,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p_dir(int nc, const int32_t* __r
   o[k] = v;
 }
 
-// q_f = (U + D^2) p_f - sum partials, p.q, then alpha (last workgroup).
-template <typename T>
+// q_f = (U + D^2) p_f - sum partials (L = 0), or D^2 p_f + sum partials
+// (L = 1: the partials hold F'(F p - E z) in full), p.q, then alpha (last
+// workgroup).
+template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_mv_finalize_alpha(
-    int nf, const int32_t* __restrict__ f_chunk_start, const T* __restrict__ partial,
+    int nf, const int32_t* __restrict__ f_lo, const int32_t* __restrict__ f_hi, const T* __restrict__ partial,
     const T* __restrict__ U, const double* __restrict__ diag_c, Win win, const T* __restrict__ x,
     T* __restrict__ y, const int32_t* __restrict__ all_done, PcgState* __restrict__ st, int nw,
     double* __restrict__ bpart, unsigned int* __restrict__ counter) {
@@ -1447,12 +1500,17 @@ __global__ __launch_bounds__(kBlock) void k_mv_finalize_alpha(
   if (blk && i < 9) {
     wi = win_of_f(win, f);
     T sp = T(0);
-    for (int ch = f_chunk_start[f]; ch < f_chunk_start[f + 1]; ++ch) sp += partial[(size_t)kPartial * ch + i];
-    const T* u = U + 45 * (size_t)f;
+    for (int ch = f_lo[f]; ch < f_hi[f]; ++ch) sp += partial[(size_t)kPartial * ch + i];
     T a = (T)(diag_c[9 * (size_t)f + i] * win.inv_r[wi]) * xv[i];
+    T yv;
+    if constexpr (L == 0) {
+      const T* u = U + 45 * (size_t)f;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) a += u[i <= j ? ut9(i, j) : ut9(j, i)] * xv[j];
-    const T yv = a - sp;
+      for (int j = 0; j < 9; ++j) a += u[i <= j ? ut9(i, j) : ut9(j, i)] * xv[j];
+      yv = a - sp;
+    } else {
+      yv = a + sp;
+    }
     y[9 * (size_t)f + i] = yv;
     acc[0] = (double)xi * (double)yv;
   }

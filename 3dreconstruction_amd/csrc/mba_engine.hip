@@ -19,6 +19,7 @@
 
 #include "../../include/mba.h"
 #include "mba_device.h"
+#include "mba_device_grouped.h"
 #include "mba_scene.h"
 
 namespace mba {
@@ -64,22 +65,39 @@ inline int nblocks(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1,
 //   LexicographicallyOrderResidualBlocks (reorder_program.cc:249-328): rows
 //     bucketed by their E block, each bucket filled back to front.
 struct Structure {
-  int32_t nc = 0, np = 0;
+  int32_t layout = MBA_LAYOUT_PACKED9;
+  int32_t ng = 0, nc = 0, np = 0;  // ng: intrinsic groups (grouped layout)
   int64_t nobs = 0;
-  std::vector<int32_t> order;      // column blocks: cameras c, points nc + p
+  // Column blocks in order.  Packed: cameras c, points nc + p.  Grouped:
+  // groups g, cameras ng + c, points ng + nc + p.
+  std::vector<int32_t> order;
   int32_t num_e = 0;
   std::vector<int64_t> res_order;  // row -> input observation
-  bool standard = false;           // E == active points, F == active cameras
-  // Derived (standard structure only).
+  bool standard = false;           // E == active points
+  // Derived (standard structure only).  f_to_cam: the F block's parameter
+  // slot block (packed: its camera; grouped: camera c at c, group g at
+  // nc + g); cam_to_f is its inverse (-1: constant or unused).
   std::vector<int32_t> e_to_pt, f_to_cam, cam_to_f, pt_start;
+  std::vector<int32_t> cam_grp;    // grouped: the group of each camera (-1: none)
   int32_t max_track = 0;
 };
 
 mba_status validate(const mba_problem* p) {
   if (!p) return fail(MBA_ERR_INVALID_ARGUMENT, "problem is NULL");
-  if (p->layout != MBA_LAYOUT_PACKED9) return fail(MBA_ERR_UNSUPPORTED, "only the packed 9-parameter camera layout is supported");
+  if (p->layout != MBA_LAYOUT_PACKED9 && p->layout != MBA_LAYOUT_GROUPED)
+    return fail(MBA_ERR_INVALID_ARGUMENT, "unknown camera layout " + std::to_string(p->layout));
   if (p->num_cameras < 0 || p->num_points < 0 || p->num_observations < 0)
     return fail(MBA_ERR_INVALID_ARGUMENT, "negative sizes");
+  if (p->layout == MBA_LAYOUT_GROUPED) {
+    if (p->num_intrinsics < 0) return fail(MBA_ERR_INVALID_ARGUMENT, "negative num_intrinsics");
+    if (p->num_intrinsics > 0 && !p->intrinsics) return fail(MBA_ERR_INVALID_ARGUMENT, "intrinsics is NULL");
+    if (p->num_observations > 0 && !p->intrinsic_index)
+      return fail(MBA_ERR_INVALID_ARGUMENT, "intrinsic_index is NULL");
+    for (int64_t i = 0; i < p->num_observations; ++i)
+      if (p->intrinsic_index[i] < 0 || p->intrinsic_index[i] >= p->num_intrinsics)
+        return fail(MBA_ERR_INVALID_ARGUMENT,
+                    "observation " + std::to_string(i) + " has an intrinsic index out of range");
+  }
   if (p->num_observations > 0x7fffffffLL) return fail(MBA_ERR_UNSUPPORTED, "more than 2^31-1 observations");
   if (p->num_observations > 0 && (!p->observations || !p->camera_index || !p->point_index))
     return fail(MBA_ERR_INVALID_ARGUMENT, "observation arrays are NULL");
@@ -93,8 +111,11 @@ mba_status validate(const mba_problem* p) {
   return MBA_OK;
 }
 
+mba_status build_structure_grouped(const mba_problem* p, Structure* st);
+
 mba_status build_structure(const mba_problem* p, Structure* st) {
   MBA_TRY(validate(p));
+  if (p->layout == MBA_LAYOUT_GROUPED) return build_structure_grouped(p, st);
   const int32_t nc = p->num_cameras, np = p->num_points;
   const int64_t no = p->num_observations;
   st->nc = nc;
@@ -227,29 +248,181 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
   return MBA_OK;
 }
 
+// Parameter blocks of observation i in AddResidualBlock argument order, as
+// vertex ids of the structure; returns their count.
+inline int residual_blocks(const mba_problem* p, int64_t i, int32_t out[3]) {
+  if (p->layout == MBA_LAYOUT_GROUPED) {
+    const int32_t ng = p->num_intrinsics, nc = p->num_cameras;
+    out[0] = p->intrinsic_index[i];
+    out[1] = ng + p->camera_index[i];
+    out[2] = ng + nc + p->point_index[i];
+    return 3;
+  }
+  out[0] = p->camera_index[i];
+  out[1] = p->num_cameras + p->point_index[i];
+  return 2;
+}
+
+// SetParameterBlockConstant of a vertex (points are never constant).
+inline bool block_constant(const mba_problem* p, int32_t b) {
+  if (p->layout == MBA_LAYOUT_GROUPED) {
+    const int32_t ng = p->num_intrinsics, nc = p->num_cameras;
+    if (b < ng) return p->intrinsic_constant && p->intrinsic_constant[b];
+    if (b < ng + nc) return p->camera_constant && p->camera_constant[b - ng];
+    return false;
+  }
+  return b < p->num_cameras && p->camera_constant && p->camera_constant[b];
+}
+
+// The same preprocessing for the grouped layout's residual blocks
+// (intrinsics, camera, point): Hessian-graph edges join every pair of
+// non-constant blocks of a residual (parameter_block_ordering.cc:132-151), so
+// a shared group is adjacent to its cameras and to every point they see.
+// Every observation of a camera must use one group (as the incremental
+// engine builds them, sfm_incremental_engine.cpp:1273-1393).
+mba_status build_structure_grouped(const mba_problem* p, Structure* st) {
+  const int32_t ng = p->num_intrinsics, nc = p->num_cameras, np = p->num_points;
+  const int64_t no = p->num_observations;
+  st->layout = MBA_LAYOUT_GROUPED;
+  st->ng = ng;
+  st->nc = nc;
+  st->np = np;
+  st->nobs = no;
+  if ((int64_t)ng + nc + np > 0x7fffffffLL) return fail(MBA_ERR_UNSUPPORTED, "more than 2^31-1 parameter blocks");
+  const int32_t nb = ng + nc + np;
+  st->cam_grp.assign(nc, -1);
+  for (int64_t i = 0; i < no; ++i) {
+    const int32_t c = p->camera_index[i], g = p->intrinsic_index[i];
+    if (st->cam_grp[c] < 0) st->cam_grp[c] = g;
+    else if (st->cam_grp[c] != g)
+      return fail(MBA_ERR_UNSUPPORTED, "camera " + std::to_string(c) + " is observed with two intrinsic groups");
+  }
+  auto is_const = [&](int32_t b) { return block_constant(p, b); };
+  std::vector<int32_t> prog;
+  prog.reserve(nb);
+  std::vector<uint8_t> seen(nb, 0);
+  int32_t rb[3];
+  for (int64_t i = 0; i < no; ++i) {
+    const int n = residual_blocks(p, i, rb);
+    for (int k = 0; k < n; ++k)
+      if (!seen[rb[k]]) { seen[rb[k]] = 1; prog.push_back(rb[k]); }
+  }
+  // Distinct edges between non-constant blocks, then adjacency lists.
+  std::vector<int64_t> keys;
+  keys.reserve(3 * (size_t)no);
+  for (int64_t i = 0; i < no; ++i) {
+    const int n = residual_blocks(p, i, rb);
+    for (int a = 0; a < n; ++a) {
+      if (is_const(rb[a])) continue;
+      for (int b = a + 1; b < n; ++b) {
+        if (is_const(rb[b]) || rb[a] == rb[b]) continue;
+        const int32_t lo = std::min(rb[a], rb[b]), hi = std::max(rb[a], rb[b]);
+        keys.push_back((int64_t)lo * nb + hi);
+      }
+    }
+  }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  std::vector<int64_t> adj_start((size_t)nb + 1, 0);
+  for (int64_t k : keys) {
+    adj_start[k / nb + 1]++;
+    adj_start[k % nb + 1]++;
+  }
+  for (int32_t v = 0; v < nb; ++v) adj_start[v + 1] += adj_start[v];
+  std::vector<int32_t> adj((size_t)adj_start[nb]);
+  {
+    std::vector<int64_t> fill(adj_start.begin(), adj_start.end() - 1);
+    for (int64_t k : keys) {
+      const int32_t a = (int32_t)(k / nb), b = (int32_t)(k % nb);
+      adj[fill[a]++] = b;
+      adj[fill[b]++] = a;
+    }
+  }
+  std::vector<int32_t> active;
+  active.reserve(prog.size());
+  for (int32_t b : prog)
+    if (!is_const(b)) active.push_back(b);
+  int32_t maxdeg = 0;
+  for (int32_t b : active) maxdeg = std::max<int32_t>(maxdeg, (int32_t)(adj_start[b + 1] - adj_start[b]));
+  std::vector<int64_t> cnt((size_t)maxdeg + 2, 0);
+  for (int32_t b : active) cnt[adj_start[b + 1] - adj_start[b] + 1]++;
+  for (int32_t d = 0; d <= maxdeg; ++d) cnt[d + 1] += cnt[d];
+  std::vector<int32_t> queue(active.size());
+  for (int32_t b : active) queue[cnt[adj_start[b + 1] - adj_start[b]]++] = b;
+  std::vector<uint8_t> color(nb, 0);
+  st->order.clear();
+  st->order.reserve(active.size());
+  for (int32_t b : queue) {
+    if (color[b] != 0) continue;
+    st->order.push_back(b);
+    color[b] = 2;
+    for (int64_t k = adj_start[b]; k < adj_start[b + 1]; ++k) color[adj[k]] = 1;
+  }
+  st->num_e = (int32_t)st->order.size();
+  for (int32_t b : queue)
+    if (color[b] != 2) st->order.push_back(b);
+  std::vector<int32_t> col(nb, -1);
+  for (size_t i = 0; i < st->order.size(); ++i) col[st->order[i]] = (int32_t)i;
+  std::vector<int64_t> hist((size_t)st->num_e + 1, 0);
+  std::vector<int32_t> bucket(no);
+  for (int64_t i = 0; i < no; ++i) {
+    const int n = residual_blocks(p, i, rb);
+    int32_t m = st->num_e;
+    for (int k = 0; k < n; ++k)
+      if (!is_const(rb[k])) m = std::min(m, col[rb[k]]);
+    bucket[i] = m;
+    hist[m]++;
+  }
+  for (int32_t e = 1; e <= st->num_e; ++e) hist[e] += hist[e - 1];
+  st->res_order.assign(no, 0);
+  for (int64_t i = 0; i < no; ++i) st->res_order[--hist[bucket[i]]] = i;
+  int32_t n_act_pts = 0;
+  for (int32_t b : active) n_act_pts += b >= ng + nc;
+  bool ok = st->num_e == n_act_pts;
+  for (int32_t i = 0; ok && i < st->num_e; ++i) ok = st->order[i] >= ng + nc;
+  st->standard = ok;
+  if (!ok) return MBA_OK;
+  st->e_to_pt.resize(st->num_e);
+  for (int32_t i = 0; i < st->num_e; ++i) st->e_to_pt[i] = st->order[i] - ng - nc;
+  const int32_t nf = (int32_t)st->order.size() - st->num_e;
+  st->f_to_cam.resize(nf);
+  st->cam_to_f.assign((size_t)nc + ng, -1);
+  for (int32_t i = 0; i < nf; ++i) {
+    const int32_t v = st->order[st->num_e + i];
+    st->f_to_cam[i] = v < ng ? nc + v : v - ng;
+    st->cam_to_f[st->f_to_cam[i]] = i;
+  }
+  st->pt_start.assign((size_t)st->num_e + 1, 0);
+  for (int64_t r = 0; r < no; ++r) st->pt_start[bucket[st->res_order[r]] + 1]++;
+  for (int32_t e = 0; e < st->num_e; ++e) st->pt_start[e + 1] += st->pt_start[e];
+  st->max_track = 0;
+  for (int32_t e = 0; e < st->num_e; ++e)
+    st->max_track = std::max(st->max_track, st->pt_start[e + 1] - st->pt_start[e]);
+  return MBA_OK;
+}
+
 // Upper-triangle block pairs of the reduced camera matrix
 // (SparseSchurComplementSolver::InitStorage, schur_complement_solver.cc:246-313).
 std::vector<int64_t> block_pairs(const mba_problem* p, const Structure& st) {
-  const int32_t nc = st.nc;
   const int32_t nf = (int32_t)st.order.size() - st.num_e;
-  std::vector<int32_t> col(nc + st.np, -1);
+  std::vector<int32_t> col((size_t)st.ng + st.nc + st.np, -1);
   for (size_t i = 0; i < st.order.size(); ++i) col[st.order[i]] = (int32_t)i;
   std::vector<int64_t> keys;
   keys.reserve(nf);
   for (int32_t i = 0; i < nf; ++i) keys.push_back((int64_t)i * nf + i);
-  auto is_const = [&](int32_t b) { return b < nc && p->camera_constant && p->camera_constant[b]; };
   auto cells = [&](int64_t obs, int32_t* out) {
+    int32_t rb[3];
+    const int nrb = residual_blocks(p, obs, rb);
     int n = 0;
-    const int32_t a = p->camera_index[obs], b = nc + p->point_index[obs];
-    if (!is_const(a)) out[n++] = col[a];
-    out[n++] = col[b];
+    for (int k = 0; k < nrb; ++k)
+      if (!block_constant(p, rb[k])) out[n++] = col[rb[k]];
     std::sort(out, out + n);
     return n;
   };
   int64_t r = 0;
   const int64_t nr = (int64_t)st.res_order.size();
   std::vector<int32_t> fb;
-  int32_t c2[2];
+  int32_t c2[3];
   while (r < nr) {
     cells(st.res_order[r], c2);
     if (c2[0] >= st.num_e) break;
@@ -303,9 +476,10 @@ struct Dev {
   int32_t* row_cam = nullptr;
   T* row_obs = nullptr;
   int32_t* pt_start = nullptr;
-  // camera-major chunks
+  // camera-major chunks; chunk_f is the chunk's F block (packed) or camera
+  // (grouped); an F block's partial rows are [f_lo, f_hi)
   int32_t *cm_e = nullptr, *chunk_f = nullptr, *chunk_begin = nullptr, *chunk_end = nullptr;
-  int32_t *f_chunk_start = nullptr, *f_to_cam = nullptr, *cam_to_f = nullptr;
+  int32_t *f_lo = nullptr, *f_hi = nullptr, *f_to_cam = nullptr, *cam_to_f = nullptr;
   T* cm_obs = nullptr;
   // kernel-precision parameters
   T *ctab = nullptr, *pscale = nullptr, *cscale = nullptr;
@@ -332,8 +506,15 @@ struct mba_context {
   int32_t W = 1;
   std::vector<Structure> wst;
   std::vector<int32_t> cam_off, pt_off, e_off, f_off;
-  std::vector<int32_t> f_to_cam, cam_to_f;  // F block <-> global camera
+  std::vector<int32_t> f_to_cam, cam_to_f;  // F block <-> global camera (slot block)
   int32_t max_track = 0;
+  // Grouped layout (single problem): intrinsic groups, the FP64 slot blocks
+  // (cameras, then groups), each camera's group and group F block.
+  bool grouped = false;
+  int32_t ng = 0, nslot = 0;
+  std::vector<int32_t> cam_grp, cam_gf;
+  int32_t *d_cam_grp = nullptr, *d_cam_gf = nullptr;
+  uint8_t* fmask = nullptr;  // 9 per F block: 1 for a free slot
   int device = 0;
   hipStream_t stream = nullptr;
   Pool pool;
@@ -398,7 +579,8 @@ template <typename T>
 mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_start,
                            const std::vector<int32_t>& row_cam, const std::vector<int32_t>& cm_src,
                            const std::vector<int32_t>& chunk_f, const std::vector<int32_t>& chunk_begin,
-                           const std::vector<int32_t>& chunk_end, const std::vector<int32_t>& f_chunk_start) {
+                           const std::vector<int32_t>& chunk_end, const std::vector<int32_t>& f_lo,
+                           const std::vector<int32_t>& f_hi) {
   Dev<T>& d = dev<T>(c);
   const hipStream_t s = c->stream;
   MBA_TRY(upload(c, &d.row_cam, row_cam));
@@ -423,7 +605,8 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_st
   MBA_TRY(upload(c, &d.chunk_f, chunk_f));
   MBA_TRY(upload(c, &d.chunk_begin, chunk_begin));
   MBA_TRY(upload(c, &d.chunk_end, chunk_end));
-  MBA_TRY(upload(c, &d.f_chunk_start, f_chunk_start));
+  MBA_TRY(upload(c, &d.f_lo, f_lo));
+  MBA_TRY(upload(c, &d.f_hi, f_hi));
   MBA_TRY(upload(c, &d.f_to_cam, c->f_to_cam));
   MBA_TRY(upload(c, &d.cam_to_f, c->cam_to_f));
   const size_t ne = c->ne, nf = c->nf, nc = c->nc;
@@ -439,8 +622,8 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_st
   MBA_TRY(c->pool.alloc(&d.gc, 9 * nf));
   MBA_TRY(c->pool.alloc(&d.rhs, 9 * nf));
   MBA_TRY(c->pool.alloc(&d.Minv, 81 * nf));
-  MBA_TRY(c->pool.alloc(&d.dtab, kDirTab * nc));
-  MBA_TRY(c->pool.alloc(&d.partial, (size_t)kPartial * c->nchunks));
+  MBA_TRY(c->pool.alloc(&d.dtab, (c->grouped ? kDirTabG : kDirTab) * nc));
+  MBA_TRY(c->pool.alloc(&d.partial, (size_t)kPartial * c->nchunks * (c->grouped ? 2 : 1)));
   MBA_TRY(c->pool.alloc(&d.x, 9 * nf));
   MBA_TRY(c->pool.alloc(&d.r, 9 * nf));
   MBA_TRY(c->pool.alloc(&d.zc, 9 * nf));
@@ -633,6 +816,9 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   // parallel on host threads.
   std::vector<mba_status> wstat(W, MBA_OK);
   std::vector<std::string> werr(W);
+  for (int32_t w = 0; w < W; ++w)
+    if (probs[w].layout == MBA_LAYOUT_GROUPED && W > 1)
+      return fail(MBA_ERR_UNSUPPORTED, "grouped-intrinsics problems are solved one context each");
   auto prep = [&](int32_t w) {
     const mba_problem* p = &probs[w];
     Structure& st = c->wst[w];
@@ -653,12 +839,20 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
         werr[w] = "non-finite point";
         return;
       }
-    for (int64_t i = 0; i < 9 * (int64_t)p->num_cameras; ++i)
+    const int cw = p->layout == MBA_LAYOUT_GROUPED ? 6 : 9;
+    for (int64_t i = 0; i < cw * (int64_t)p->num_cameras; ++i)
       if (!std::isfinite(p->cameras[i])) {
         wstat[w] = MBA_ERR_NOT_FINITE;
         werr[w] = "non-finite camera";
         return;
       }
+    if (p->layout == MBA_LAYOUT_GROUPED)
+      for (int64_t i = 0; i < 6 * (int64_t)p->num_intrinsics; ++i)
+        if (!std::isfinite(p->intrinsics[i])) {
+          wstat[w] = MBA_ERR_NOT_FINITE;
+          werr[w] = "non-finite intrinsics";
+          return;
+        }
   };
   if (W == 1) {
     prep(0);
@@ -697,6 +891,9 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   c->ne = c->e_off[W];
   c->nf = c->f_off[W];
   c->nrows = (int32_t)nrows;
+  c->grouped = probs[0].layout == MBA_LAYOUT_GROUPED;
+  c->ng = c->grouped ? c->wst[0].ng : 0;
+  c->nslot = c->nc + c->ng;
   // Global E blocks in window order, each window in its reference E order.
   std::vector<int32_t> ge_win(c->ne), ge_local(c->ne);
   for (int32_t w = 0; w < W; ++w)
@@ -756,13 +953,19 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   }
   // F blocks: window-contiguous, each window in its reference F order.
   c->f_to_cam.resize(c->nf);
-  c->cam_to_f.assign(c->nc, -1);
+  c->cam_to_f.assign(c->nslot, -1);
   for (int32_t w = 0; w < W; ++w)
     for (size_t f = 0; f < c->wst[w].f_to_cam.size(); ++f) {
       const int32_t gf = c->f_off[w] + (int32_t)f, cam = c->cam_off[w] + c->wst[w].f_to_cam[f];
       c->f_to_cam[gf] = cam;
       c->cam_to_f[cam] = gf;
     }
+  if (c->grouped) {
+    c->cam_grp = c->wst[0].cam_grp;
+    c->cam_gf.assign(c->nc, -1);
+    for (int32_t cam = 0; cam < c->nc; ++cam)
+      if (c->cam_grp[cam] >= 0) c->cam_gf[cam] = c->cam_to_f[c->nc + c->cam_grp[cam]];
+  }
   // Rows in device order (each point's rows keep its window's reference
   // order); camera indices are global.  Points are independent.
   std::vector<int32_t> row_cam(c->nrows);
@@ -781,25 +984,42 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       }
     }
   });
-  // Camera-major rows: a stable counting sort of the rows by F block
-  // (constant cameras excluded) with per-thread histograms over row
-  // ranges; cm_src[k] is the point-major row of camera-major slot k.
-  std::vector<int64_t> fstart((size_t)c->nf + 1, 0);
+  // Camera-major rows: a stable counting sort of the rows by key with
+  // per-thread histograms over row ranges; cm_src[k] is the point-major row
+  // of camera-major slot k.  Packed: key = F block (constant cameras
+  // excluded).  Grouped: key = the camera's rank in (group, camera) order,
+  // over cameras whose block or group block is free, so each group's
+  // cameras are contiguous.
+  std::vector<int32_t> key_of_cam, key_cam;
+  int32_t nkeys = c->nf;
+  if (c->grouped) {
+    key_of_cam.assign(c->nc, -1);
+    std::vector<int32_t> cams_sorted;
+    for (int32_t cam = 0; cam < c->nc; ++cam)
+      if (c->cam_to_f[cam] >= 0 || c->cam_gf[cam] >= 0) cams_sorted.push_back(cam);
+    std::stable_sort(cams_sorted.begin(), cams_sorted.end(),
+                     [&](int32_t a, int32_t b) { return c->cam_grp[a] < c->cam_grp[b]; });
+    for (size_t k = 0; k < cams_sorted.size(); ++k) key_of_cam[cams_sorted[k]] = (int32_t)k;
+    key_cam = cams_sorted;
+    nkeys = (int32_t)cams_sorted.size();
+  }
+  auto row_key = [&](int32_t cam) { return c->grouped ? key_of_cam[cam] : c->cam_to_f[cam]; };
+  std::vector<int64_t> fstart((size_t)nkeys + 1, 0);
   std::vector<int32_t> cm_src;
   {
     const int nt = host_threads(c->nrows);
     const int64_t step = ((int64_t)c->nrows + nt - 1) / nt;
-    std::vector<std::vector<int64_t>> cnt(nt, std::vector<int64_t>((size_t)c->nf, 0));
+    std::vector<std::vector<int64_t>> cnt(nt, std::vector<int64_t>((size_t)nkeys, 0));
     run_threads(nt, [&](int t) {
       std::vector<int64_t>& h = cnt[t];
       const int64_t end = std::min<int64_t>(c->nrows, (t + 1) * step);
       for (int64_t r = t * step; r < end; ++r) {
-        const int32_t f = c->cam_to_f[row_cam[r]];
+        const int32_t f = row_key(row_cam[r]);
         if (f >= 0) h[f]++;
       }
     });
     int64_t run = 0;
-    for (int32_t f = 0; f < c->nf; ++f) {
+    for (int32_t f = 0; f < nkeys; ++f) {
       fstart[f] = run;
       for (int t = 0; t < nt; ++t) {
         const int64_t n = cnt[t][f];
@@ -807,32 +1027,57 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
         run += n;
       }
     }
-    fstart[c->nf] = run;
+    fstart[nkeys] = run;
     c->ncm = (int32_t)run;
     cm_src.resize((size_t)run);
     run_threads(nt, [&](int t) {
       std::vector<int64_t>& o = cnt[t];
       const int64_t end = std::min<int64_t>(c->nrows, (t + 1) * step);
       for (int64_t r = t * step; r < end; ++r) {
-        const int32_t f = c->cam_to_f[row_cam[r]];
+        const int32_t f = row_key(row_cam[r]);
         if (f >= 0) cm_src[o[f]++] = (int32_t)r;
       }
     });
   }
   const int cr = chunk_rows_for(c->ncm);
-  std::vector<int32_t> chunk_f, chunk_begin, chunk_end, f_chunk_start((size_t)c->nf + 1, 0);
-  for (int32_t f = 0; f < c->nf; ++f) {
-    f_chunk_start[f] = (int32_t)chunk_f.size();
+  std::vector<int32_t> chunk_f, chunk_begin, chunk_end, key_chunk((size_t)nkeys + 1, 0);
+  for (int32_t f = 0; f < nkeys; ++f) {
+    key_chunk[f] = (int32_t)chunk_f.size();
     for (int64_t b = fstart[f]; b < fstart[f + 1]; b += cr) {
-      chunk_f.push_back(f);
+      chunk_f.push_back(c->grouped ? key_cam[f] : f);
       chunk_begin.push_back((int32_t)b);
       chunk_end.push_back((int32_t)std::min<int64_t>(b + cr, fstart[f + 1]));
     }
   }
-  f_chunk_start[c->nf] = (int32_t)chunk_f.size();
+  key_chunk[nkeys] = (int32_t)chunk_f.size();
   c->nchunks = (int32_t)chunk_f.size();
-  // FP64 state.
-  std::vector<double> pts(3 * (size_t)c->ne), cams(9 * (size_t)c->nc);
+  // Partial rows of each F block: its chunks (packed), or its camera's
+  // chunks / its cameras' group rows after the nchunks camera rows.
+  std::vector<int32_t> f_lo(c->nf), f_hi(c->nf);
+  if (!c->grouped) {
+    for (int32_t f = 0; f < c->nf; ++f) {
+      f_lo[f] = key_chunk[f];
+      f_hi[f] = key_chunk[f + 1];
+    }
+  } else {
+    for (int32_t f = 0; f < c->nf; ++f) {
+      f_lo[f] = 0;
+      f_hi[f] = 0;
+    }
+    for (int32_t k = 0; k < nkeys; ++k) {
+      const int32_t cam = key_cam[k], fc = c->cam_to_f[cam], fg = c->cam_gf[cam];
+      if (fc >= 0) {
+        f_lo[fc] = key_chunk[k];
+        f_hi[fc] = key_chunk[k + 1];
+      }
+      if (fg >= 0) {
+        if (f_hi[fg] == 0) f_lo[fg] = c->nchunks + key_chunk[k];
+        f_hi[fg] = c->nchunks + key_chunk[k + 1];
+      }
+    }
+  }
+  // FP64 state: cameras (slot blocks), then intrinsic groups (grouped).
+  std::vector<double> pts(3 * (size_t)c->ne), cams(9 * (size_t)c->nslot, 0.0);
   parallel_for(c->ne, [&](int64_t b, int64_t end) {
     for (int64_t de = b; de < end; ++de) {
       const int32_t w = ge_win[dperm[de]];
@@ -840,10 +1085,28 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       for (int k = 0; k < 3; ++k) pts[3 * (size_t)de + k] = probs[w].points[3 * (size_t)q + k];
     }
   });
-  for (int32_t w = 0; w < W; ++w)
-    if (probs[w].num_cameras > 0)
-      std::memcpy(cams.data() + 9 * (size_t)c->cam_off[w], probs[w].cameras,
-                  72 * (size_t)probs[w].num_cameras);
+  if (c->grouped) {
+    for (int32_t cam = 0; cam < c->nc; ++cam)
+      for (int k = 0; k < 6; ++k) cams[9 * (size_t)cam + k] = probs[0].cameras[6 * (size_t)cam + k];
+    for (int32_t g = 0; g < c->ng; ++g)
+      for (int k = 0; k < 6; ++k) cams[9 * ((size_t)c->nc + g) + k] = probs[0].intrinsics[6 * (size_t)g + k];
+    MBA_TRY(upload(c, &c->d_cam_grp, c->cam_grp));
+    MBA_TRY(upload(c, &c->d_cam_gf, c->cam_gf));
+    // Free slots: a camera's [aa, t]; a group's entries outside its
+    // SubsetParameterization constants.
+    std::vector<uint8_t> mask(9 * (size_t)c->nf, 0);
+    for (int32_t f = 0; f < c->nf; ++f) {
+      const int32_t b = c->f_to_cam[f];
+      const uint8_t sub = (b >= c->nc && probs[0].intrinsic_subset) ? probs[0].intrinsic_subset[b - c->nc] : 0;
+      for (int k = 0; k < 6; ++k) mask[9 * (size_t)f + k] = !((sub >> k) & 1);
+    }
+    MBA_TRY(upload(c, &c->fmask, mask));
+  } else {
+    for (int32_t w = 0; w < W; ++w)
+      if (probs[w].num_cameras > 0)
+        std::memcpy(cams.data() + 9 * (size_t)c->cam_off[w], probs[w].cameras,
+                    72 * (size_t)probs[w].num_cameras);
+  }
   MBA_TRY(upload(c, &c->cams64, cams));
   MBA_TRY(upload(c, &c->cams64_t, cams));
   MBA_TRY(upload(c, &c->pts64, pts));
@@ -886,10 +1149,11 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   MBA_TRY(c->pool.alloc(&c->x64, 9 * (size_t)c->nf));
   MBA_TRY(c->pool.alloc(&c->r64, 9 * (size_t)c->nf));
   if (c->mixed)
-    MBA_TRY(alloc_precision<float>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_chunk_start));
+    MBA_TRY(alloc_precision<float>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_lo, f_hi));
   else
-    MBA_TRY(alloc_precision<double>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_chunk_start));
-  if (W == 1 && c->opt.linear_solver == MBA_EXPLICIT_SCHUR) MBA_TRY(build_s_pairs(c, dev_pt_start, row_cam));
+    MBA_TRY(alloc_precision<double>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_lo, f_hi));
+  if (W == 1 && c->opt.linear_solver == MBA_EXPLICIT_SCHUR && !c->grouped)
+    MBA_TRY(build_s_pairs(c, dev_pt_start, row_cam));
   for (int i = 0; i < 8; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
   HIP_TRY(hipStreamSynchronize(c->stream));
   *out = c;
@@ -983,19 +1247,25 @@ struct Engine {
   mba_status set_radius(double r) { return set_radii(std::vector<double>(W, r)); }
 
   // Kernel-precision camera table from the FP64 state.
-  void tables(const double* cams) {
-    hipLaunchKernelGGL(k_cam_table<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, d.ctab);
+  template <typename X>
+  void table_into(const double* cams, X* tab) {
+    if (c->grouped)
+      hipLaunchKernelGGL(k_cam_table_g<X>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams,
+                         (const int32_t*)c->d_cam_grp, tab);
+    else
+      hipLaunchKernelGGL(k_cam_table<X>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, tab);
   }
+  void tables(const double* cams) { table_into<T>(cams, d.ctab); }
 
   // FP64 cost per window at (cams, pts); finite[w] = 0 on a non-finite value.
   mba_status cost64(const double* cams, const double* pts, std::vector<double>* cost, std::vector<uint8_t>* finite) {
     MBA_TRY(clear_flags());
-    hipLaunchKernelGGL(k_cam_table<double>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, c->ctab64);
+    table_into<double>(cams, c->ctab64);
     const int nb = c->seg_blocks;
     MBA_TRY(begin_sum());
     if (nb > 0)
-      hipLaunchKernelGGL(k_cost64, dim3(nb), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam, c->row_obs64, pts,
-                         c->ctab64, c->opt.huber_a, c->red, win(c->wacc));
+      hipLaunchKernelGGL((c->grouped ? k_cost64<1> : k_cost64<0>), dim3(nb), dim3(kBlock), 0, s, c->seg, d.pt_start,
+                         d.row_cam, c->row_obs64, pts, c->ctab64, c->opt.huber_a, c->red, win(c->wacc));
     MBA_TRY(end_sum(nb, cost));
     std::vector<int32_t> f;
     MBA_TRY(read_flags(&f));
@@ -1008,17 +1278,22 @@ struct Engine {
   void jacobian_pass(int recompute_diag) {
     if (c->timing) (void)hipEventRecord(c->ev[0], s);
     if (c->seg_blocks > 0)
-      hipLaunchKernelGGL(k_point_pass<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start,
-                         d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, win(),
+      hipLaunchKernelGGL((c->grouped ? k_point_pass<T, 1> : k_point_pass<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0,
+                         s, c->seg, d.pt_start, d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, win(),
                          c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag,
                          d.V, d.gp, d.Vinv, d.pk);
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
-    if (c->nchunks > 0)
-      hipLaunchKernelGGL(k_cam_pass<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
-                         d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial, win());
+    if (c->nchunks > 0) {
+      if (c->grouped)
+        hipLaunchKernelGGL(k_cam_pass_g<T>, dim3(c->nchunks), dim3(kBlock), 0, s, c->nchunks, d.chunk_f,
+                           d.chunk_begin, d.chunk_end, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, d.partial);
+      else
+        hipLaunchKernelGGL(k_cam_pass<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
+                           d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial, win());
+    }
     if (c->timing) (void)hipEventRecord(c->ev[2], s);
     if (c->nf > 0)
-      hipLaunchKernelGGL(k_cam_finalize<T>, dim3(c->nf), dim3(64), 0, s, c->nf, d.f_chunk_start, d.partial,
+      hipLaunchKernelGGL(k_cam_finalize<T>, dim3(c->nf), dim3(64), 0, s, c->nf, d.f_lo, d.f_hi, d.partial,
                          d.cscale, win(), c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_c,
                          recompute_diag, 1, d.U, d.gc, d.rhs, d.Minv);
     if (c->timing) (void)hipEventRecord(c->ev[3], s);
@@ -1031,30 +1306,52 @@ struct Engine {
     if (c->nf > 0) {
       hipLaunchKernelGGL(k_cam_damp<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf, d.U, c->diag_c, win(),
                          d.Minv);
-      if (c->nchunks > 0)
-        hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
-                           d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber,
-                           d.partial, (const int32_t*)nullptr, win());
-      hipLaunchKernelGGL(k_rhs_finalize<T>, dim3(nblocks(c->nf)), dim3(kBlock), 0, s, c->nf,
-                         d.f_chunk_start, d.partial, d.gc, d.rhs, win());
+      mv_cam(nullptr, nullptr);
+      hipLaunchKernelGGL((c->grouped ? k_rhs_finalize<T, 1> : k_rhs_finalize<T, 0>), dim3(nblocks(c->nf)),
+                         dim3(kBlock), 0, s, c->nf, d.f_lo, d.f_hi, d.partial, d.gc, d.rhs, win());
     }
+  }
+
+  // Direction table of an F-space vector x (sign * scale .* x) for the
+  // point-major kernels.
+  void cam_dir(const T* x, T sign, const int32_t* done, const Win& wv) {
+    if (c->grouped)
+      hipLaunchKernelGGL(k_cam_dir_g<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f,
+                         (const int32_t*)c->d_cam_gf, d.cscale, d.ctab, x, sign, d.dtab, done);
+    else
+      hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
+                         d.ctab, x, sign, d.dtab, done, wv);
+  }
+  // Point-major: z_e = Vinv E'(beta r + F x) (x through d.dtab).
+  void mv_point(T beta, T* pk_out, const int32_t* done, const Win& wv) {
+    if (c->seg_blocks == 0) return;
+    hipLaunchKernelGGL((c->grouped ? k_mv_point<T, 1> : k_mv_point<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0, s,
+                       c->seg, d.pt_start, d.row_cam, d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber,
+                       beta, 1, d.z, pk_out, done, wv);
+  }
+  // Camera-major partials: packed, sum Jc'(Jp z); grouped, sum F'(F x - E z)
+  // (dtab: the direction table of x, or NULL for x = 0).
+  void mv_cam(const T* dtab, const int32_t* done, const Win* wv = nullptr) {
+    if (c->nchunks == 0) return;
+    if (c->grouped)
+      hipLaunchKernelGGL(k_mv_cam_g<T>, dim3(c->nchunks), dim3(kBlock), 0, s, c->nchunks, d.chunk_f, d.chunk_begin,
+                         d.chunk_end, d.cam_to_f, (const int32_t*)c->d_cam_gf, d.cm_e, d.cm_obs, d.pk, d.cscale,
+                         d.ctab, dtab, huber, d.partial, done);
+    else
+      hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
+                         d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber, d.partial,
+                         done, wv ? *wv : win());
   }
 
   // y = S x (implicit), x and y F-order.  Inside PCG (`done` set) the
   // launches stop once every window's solve has stopped.
   void matvec(const T* x, T* y, const int32_t* done) {
     const Win wv = win(nullptr, done != nullptr);
-    hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
-                       d.ctab, x, (T)1, d.dtab, done, wv);
-    if (c->seg_blocks > 0)
-      hipLaunchKernelGGL(k_mv_point<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
-                         d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)0, 1, d.z, d.pk, done, wv);
-    if (c->nchunks > 0)
-      hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
-                         d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber,
-                         d.partial, done, wv);
-    hipLaunchKernelGGL(k_mv_finalize<T>, dim3(nblocks(c->nf)), dim3(kBlock), 0, s, c->nf, d.f_chunk_start,
-                       d.partial, d.U, c->diag_c, wv, x, 1, y, done);
+    cam_dir(x, (T)1, done, wv);
+    mv_point((T)0, d.pk, done, wv);
+    mv_cam(d.dtab, done, &wv);
+    hipLaunchKernelGGL((c->grouped ? k_mv_finalize<T, 1> : k_mv_finalize<T, 0>), dim3(nblocks(c->nf)), dim3(kBlock),
+                       0, s, c->nf, d.f_lo, d.f_hi, d.partial, d.U, c->diag_c, wv, x, 1, y, done);
   }
 
   // PCG on S x = rhs (x in d.x) for every window with active[w]; each
@@ -1089,18 +1386,20 @@ struct Engine {
     update(3);
     int check_every = 1;
     for (int it = 1; it <= maxit; ++it) {
-      hipLaunchKernelGGL(k_pcg_p_dir<T>, dim3(nblocks((int64_t)kGrp * c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, c->pcg, wv,
-                         d.zc, d.p, d.cscale, d.ctab, d.dtab, c->all_done);
-      if (c->seg_blocks > 0)
-        hipLaunchKernelGGL(k_mv_point<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
-                           d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)0, 1, d.z, d.pk,
-                           c->all_done, wv);
-      if (c->nchunks > 0)
-        hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
-                           d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber, d.partial,
-                           c->all_done, wv);
-      hipLaunchKernelGGL(k_mv_finalize_alpha<T>, dim3(nbg), dim3(kBlock), 0, s, nf, d.f_chunk_start, d.partial, d.U,
-                         c->diag_c, wd, d.p, d.q, c->all_done, c->pcg, W, c->red, c->counter);
+      if (c->grouped) {
+        // A group's direction feeds all its cameras' rows: update p first.
+        hipLaunchKernelGGL(k_pcg_p<T>, dim3(nblocks(9 * (int64_t)nf)), dim3(kBlock), 0, s, nf, (const PcgState*)c->pcg,
+                           (const T*)d.zc, d.p, (const int32_t*)c->all_done);
+        cam_dir(d.p, (T)1, c->all_done, wv);
+      } else {
+        hipLaunchKernelGGL(k_pcg_p_dir<T>, dim3(nblocks((int64_t)kGrp * c->nc)), dim3(kBlock), 0, s, c->nc,
+                           d.cam_to_f, c->pcg, wv, d.zc, d.p, d.cscale, d.ctab, d.dtab, c->all_done);
+      }
+      mv_point((T)0, d.pk, c->all_done, wv);
+      mv_cam(d.dtab, c->all_done, &wv);
+      hipLaunchKernelGGL((c->grouped ? k_mv_finalize_alpha<T, 1> : k_mv_finalize_alpha<T, 0>), dim3(nbg), dim3(kBlock),
+                         0, s, nf, d.f_lo, d.f_hi, d.partial, d.U, c->diag_c, wd, d.p, d.q, c->all_done, c->pcg, W,
+                         c->red, c->counter);
       if (it % 10 == 0) {  // residual reset: r = b - S x
         update(1);
         matvec(d.x, d.tmp, c->all_done);
@@ -1210,12 +1509,8 @@ struct Engine {
 
   // Back substitution: y_e = Vinv E'(b - F z) into d.z; step = -y.
   void back_substitute() {
-    hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
-                       d.ctab, d.x, (T)-1, d.dtab, (const int32_t*)nullptr, win());
-    if (c->seg_blocks > 0)
-      hipLaunchKernelGGL(k_mv_point<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
-                         d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, (T)1, 1, d.z, (T*)nullptr,
-                         (const int32_t*)nullptr, win());
+    cam_dir(d.x, (T)-1, nullptr, win());
+    mv_point((T)1, nullptr, nullptr, win());
     hipLaunchKernelGGL(k_negate<T>, dim3(nblocks(3 * (int64_t)c->ne)), dim3(kBlock), 0, s,
                        (int64_t)3 * c->ne, d.z, d.step_p);
     hipLaunchKernelGGL(k_negate<T>, dim3(nblocks(9 * (int64_t)c->nf)), dim3(kBlock), 0, s,
@@ -1224,13 +1519,13 @@ struct Engine {
 
   // Model cost change -J step . (r + J step / 2) per window.
   mba_status model_cost_change(std::vector<double>* mcc) {
-    hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
-                       d.ctab, d.step_c, (T)1, d.dtab, (const int32_t*)nullptr, win());
+    cam_dir(d.step_c, (T)1, nullptr, win());
     const int nb = c->seg_blocks;
     MBA_TRY(begin_sum());
     if (nb > 0)
-      hipLaunchKernelGGL(k_model_cost<T>, dim3(nb), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam, d.row_obs,
-                         d.pk, d.pscale, d.ctab, d.dtab, d.step_p, huber, c->red, win(c->wacc));
+      hipLaunchKernelGGL((c->grouped ? k_model_cost<T, 1> : k_model_cost<T, 0>), dim3(nb), dim3(kBlock), 0, s, c->seg,
+                         d.pt_start, d.row_cam, d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.step_p, huber, c->red,
+                         win(c->wacc));
     MBA_TRY(end_sum(nb, mcc));
     for (double& v : *mcc) v = -v;
     return MBA_OK;
@@ -1332,6 +1627,9 @@ struct Engine {
                        (int64_t)3 * c->ne, c->pscale64, d.pscale);
     hipLaunchKernelGGL((k_convert<double, T>), dim3(nblocks(9 * (int64_t)c->nf)), dim3(kBlock), 0, s,
                        (int64_t)9 * c->nf, c->cscale64, d.cscale);
+    if (c->grouped && c->nf > 0)
+      hipLaunchKernelGGL(k_scale_mask<T>, dim3(nblocks(9 * (int64_t)c->nf)), dim3(kBlock), 0, s, (int64_t)9 * c->nf,
+                         (const uint8_t*)c->fmask, d.cscale);
   }
 
   // Solve the LM systems at the current radii; fills step_p / step_c.
@@ -1363,8 +1661,8 @@ struct Engine {
     if (W == 1) {
       std::swap(c->cams64, c->cams64_t);
       std::swap(c->pts64, c->pts64_t);
-      // Keep the constant cameras of the trial copy in sync.
-      HIP_TRY(hipMemcpyAsync(c->cams64_t, c->cams64, sizeof(double) * 9 * c->nc, hipMemcpyDeviceToDevice, s));
+      // Keep the constant blocks of the trial copy in sync.
+      HIP_TRY(hipMemcpyAsync(c->cams64_t, c->cams64, sizeof(double) * 9 * c->nslot, hipMemcpyDeviceToDevice, s));
       return MBA_OK;
     }
     MBA_TRY(sync());
@@ -1621,11 +1919,22 @@ struct Engine {
 // Copy the current parameters of every window back to its problem (all
 // windows, or those whose summary says the solution is usable).
 mba_status download(mba_context* c, mba_problem* probs, const mba_summary* sums) {
-  std::vector<double> cams(9 * (size_t)c->nc), pts(3 * (size_t)c->ne);
+  std::vector<double> cams(9 * (size_t)c->nslot), pts(3 * (size_t)c->ne);
   if (!cams.empty()) HIP_TRY(hipMemcpyAsync(cams.data(), c->cams64, cams.size() * 8, hipMemcpyDeviceToHost, c->stream));
   if (!pts.empty()) HIP_TRY(hipMemcpyAsync(pts.data(), c->pts64, pts.size() * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   auto wanted = [&](int32_t w) { return !sums || sums[w].is_solution_usable; };
+  if (c->grouped) {
+    if (!wanted(0)) return MBA_OK;
+    for (int32_t f = 0; f < c->nf; ++f) {
+      const int32_t b = c->f_to_cam[f];
+      double* dst = b < c->nc ? probs[0].cameras + 6 * (size_t)b : probs[0].intrinsics + 6 * (size_t)(b - c->nc);
+      std::memcpy(dst, cams.data() + 9 * (size_t)b, 48);
+    }
+    for (int32_t e = 0; e < c->ne; ++e)
+      std::memcpy(probs[0].points + 3 * (size_t)c->dev_e_to_pt[e], pts.data() + 3 * (size_t)e, 24);
+    return MBA_OK;
+  }
   int32_t w = 0;
   for (int32_t f = 0; f < c->nf; ++f) {
     while (f >= c->f_off[w + 1]) ++w;
@@ -1758,6 +2067,7 @@ mba_status mba_context_read_scale(mba_context* ctx, double* cam_scale, double* p
 
 mba_status mba_context_read_schur(mba_context* ctx, double* U, double* gc, double* rhs, double* V, double* gp) {
   if (!ctx) return fail(MBA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  if (ctx->grouped) return fail(MBA_ERR_UNSUPPORTED, "read_schur reads packed-camera blocks");
   HIP_TRY(hipSetDevice(ctx->device));
   auto rd = [&](auto* src, size_t n, std::vector<double>* out) -> mba_status {
     using X = std::remove_cv_t<std::remove_pointer_t<decltype(src)>>;
@@ -1793,6 +2103,7 @@ mba_status mba_context_read_schur(mba_context* ctx, double* U, double* gc, doubl
 
 mba_status mba_context_read_scale(mba_context* ctx, double* cam_scale, double* pt_scale) {
   if (!ctx) return fail(MBA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  if (ctx->grouped) return fail(MBA_ERR_UNSUPPORTED, "read_scale reads packed-camera blocks");
   HIP_TRY(hipSetDevice(ctx->device));
   std::vector<double> cs(9 * (size_t)ctx->nf), ps(3 * (size_t)ctx->ne);
   if (!cs.empty()) HIP_TRY(hipMemcpyAsync(cs.data(), ctx->cscale64, cs.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1813,6 +2124,7 @@ mba_status mba_context_read_scale(mba_context* ctx, double* cam_scale, double* p
 
 mba_status mba_context_schur_matvec(mba_context* ctx, const double* x, double* y) {
   if (!ctx || !x || !y) return fail(MBA_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (ctx->grouped) return fail(MBA_ERR_UNSUPPORTED, "schur_matvec takes packed-camera vectors");
   HIP_TRY(hipSetDevice(ctx->device));
   auto go = [&](auto& eng, auto* dx, auto* dy) -> mba_status {
     using T = std::remove_pointer_t<decltype(dx)>;
@@ -1840,6 +2152,8 @@ mba_status mba_context_solve(mba_context* ctx, mba_summary* summary, mba_iterati
   if (!ctx || !summary) return fail(MBA_ERR_INVALID_ARGUMENT, "NULL argument");
   HIP_TRY(hipSetDevice(ctx->device));
   if (ctx->opt.linear_solver == MBA_EXPLICIT_SCHUR) {
+    if (ctx->grouped)
+      return fail(MBA_ERR_UNSUPPORTED, "grouped-intrinsics problems use MBA_IMPLICIT_SCHUR_PCG");
     const double n = 9.0 * ctx->nf;
     const double bytes = 2.0 * n * n * (ctx->mixed ? 4.0 : 8.0);
     if (bytes > 64.0 * (1ull << 30))
@@ -1886,8 +2200,11 @@ mba_status mba_solve_batched(const mba_options* options, mba_problem* problems, 
   std::memset(summaries, 0, sizeof(mba_summary) * (size_t)num_problems);
   mba_options o;
   if (options) o = *options; else mba_default_options(&o);
-  if (o.linear_solver != MBA_IMPLICIT_SCHUR_PCG) {
-    // Dense reduced systems are factored one window at a time.
+  bool any_grouped = false;
+  for (int32_t w = 0; w < num_problems; ++w) any_grouped = any_grouped || problems[w].layout == MBA_LAYOUT_GROUPED;
+  if (o.linear_solver != MBA_IMPLICIT_SCHUR_PCG || any_grouped) {
+    // Dense reduced systems (and grouped-intrinsics problems) are solved one
+    // window at a time.
     for (int32_t w = 0; w < num_problems; ++w) {
       const mba_status s = mba_solve(&o, &problems[w], &summaries[w], nullptr, 0);
       if (s != MBA_OK) return fail(s, "window " + std::to_string(w) + ": " + g_last_error);
@@ -1946,9 +2263,31 @@ mba_status mba_schur_structure(const mba_problem* problem, int32_t* param_order,
 }  // extern "C"
 
 template <typename T>
-static mba_status evaluate_impl(const mba_problem* p, double huber_a, double* residuals, double* jac_camera,
-                                double* jac_point, double* cost, int32_t device) {
+static mba_status evaluate_impl(const mba_problem* p, double huber_a, double* residuals, double* jac_intr,
+                                double* jac_camera, double* jac_point, double* cost, int32_t device) {
   MBA_TRY(validate(p));
+  const bool grouped = p->layout == MBA_LAYOUT_GROUPED;
+  const int64_t n = p->num_observations;
+  const int32_t nc = p->num_cameras, np = p->num_points, ng = grouped ? p->num_intrinsics : 0;
+  // FP64 slot blocks as the solver holds them: cameras, then groups.
+  std::vector<double> slots(9 * ((size_t)nc + ng), 0.0);
+  std::vector<int32_t> cam_grp;
+  if (grouped) {
+    cam_grp.assign(nc, -1);
+    for (int64_t i = 0; i < n; ++i) {
+      const int32_t cc = p->camera_index[i], g = p->intrinsic_index[i];
+      if (cam_grp[cc] < 0) cam_grp[cc] = g;
+      else if (cam_grp[cc] != g)
+        return fail(MBA_ERR_UNSUPPORTED, "camera " + std::to_string(cc) + " is observed with two intrinsic groups");
+    }
+    for (int32_t cc = 0; cc < nc; ++cc)
+      for (int k = 0; k < 6; ++k) slots[9 * (size_t)cc + k] = p->cameras[6 * (size_t)cc + k];
+    for (int32_t g = 0; g < ng; ++g)
+      for (int k = 0; k < 6; ++k) slots[9 * ((size_t)nc + g) + k] = p->intrinsics[6 * (size_t)g + k];
+  } else if (nc) {
+    std::memcpy(slots.data(), p->cameras, 72 * (size_t)nc);
+  }
+  const int jcw = grouped ? 12 : 18;
   int ndev = 0;
   HIP_TRY(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(MBA_ERR_DEVICE, "no such HIP device");
@@ -1956,36 +2295,50 @@ static mba_status evaluate_impl(const mba_problem* p, double huber_a, double* re
   hipStream_t s;
   HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   Pool pool;
-  const int64_t n = p->num_observations;
-  const int32_t nc = p->num_cameras, np = p->num_points;
   double* cams = nullptr;
-  double *res = nullptr, *jc = nullptr, *jp = nullptr;
+  double *res = nullptr, *ji = nullptr, *jc = nullptr, *jp = nullptr;
   T *tab = nullptr, *pts = nullptr, *obs = nullptr;
-  int32_t *ci = nullptr, *pi = nullptr;
+  int32_t *ci = nullptr, *pi = nullptr, *cg = nullptr;
   auto run = [&]() -> mba_status {
-    MBA_TRY(pool.alloc(&cams, 9 * (size_t)nc));
+    MBA_TRY(pool.alloc(&cams, slots.size()));
     MBA_TRY(pool.alloc(&tab, kCamTab * (size_t)nc));
     MBA_TRY(pool.alloc(&pts, 3 * (size_t)np));
     MBA_TRY(pool.alloc(&obs, 2 * (size_t)n));
     MBA_TRY(pool.alloc(&ci, (size_t)n));
     MBA_TRY(pool.alloc(&pi, (size_t)n));
     MBA_TRY(pool.alloc(&res, 2 * (size_t)n));
-    if (jac_camera) MBA_TRY(pool.alloc(&jc, 18 * (size_t)n));
+    if (grouped) MBA_TRY(pool.alloc(&cg, (size_t)nc));
+    if (jac_intr && grouped) MBA_TRY(pool.alloc(&ji, 12 * (size_t)n));
+    if (jac_camera) MBA_TRY(pool.alloc(&jc, jcw * (size_t)n));
     if (jac_point) MBA_TRY(pool.alloc(&jp, 6 * (size_t)n));
     std::vector<T> hp(p->points, p->points + 3 * (size_t)np), ho(p->observations, p->observations + 2 * n);
-    if (nc) HIP_TRY(hipMemcpyAsync(cams, p->cameras, 72 * (size_t)nc, hipMemcpyHostToDevice, s));
+    if (!slots.empty()) HIP_TRY(hipMemcpyAsync(cams, slots.data(), slots.size() * 8, hipMemcpyHostToDevice, s));
+    if (grouped && nc) HIP_TRY(hipMemcpyAsync(cg, cam_grp.data(), 4 * (size_t)nc, hipMemcpyHostToDevice, s));
     if (np) HIP_TRY(hipMemcpyAsync(pts, hp.data(), hp.size() * sizeof(T), hipMemcpyHostToDevice, s));
     if (n) {
       HIP_TRY(hipMemcpyAsync(obs, ho.data(), ho.size() * sizeof(T), hipMemcpyHostToDevice, s));
       HIP_TRY(hipMemcpyAsync(ci, p->camera_index, 4 * (size_t)n, hipMemcpyHostToDevice, s));
       HIP_TRY(hipMemcpyAsync(pi, p->point_index, 4 * (size_t)n, hipMemcpyHostToDevice, s));
     }
-    if (nc) hipLaunchKernelGGL(k_cam_table<T>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cams, tab);
-    if (n) hipLaunchKernelGGL(k_eval_obs<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ci, pi, obs, tab, pts,
-                              res, jc, jp);
+    if (nc) {
+      if (grouped)
+        hipLaunchKernelGGL(k_cam_table_g<T>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, (const double*)cams,
+                           (const int32_t*)cg, tab);
+      else
+        hipLaunchKernelGGL(k_cam_table<T>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cams, tab);
+    }
+    if (n) {
+      if (grouped)
+        hipLaunchKernelGGL(k_eval_obs_g<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ci, pi, obs, tab, pts, res, ji,
+                           jc, jp);
+      else
+        hipLaunchKernelGGL(k_eval_obs<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ci, pi, obs, tab, pts, res, jc,
+                           jp);
+    }
     std::vector<double> hr(2 * (size_t)n);
     if (n) HIP_TRY(hipMemcpyAsync(hr.data(), res, hr.size() * 8, hipMemcpyDeviceToHost, s));
-    if (jc && n) HIP_TRY(hipMemcpyAsync(jac_camera, jc, 18 * 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+    if (ji && n) HIP_TRY(hipMemcpyAsync(jac_intr, ji, 12 * 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+    if (jc && n) HIP_TRY(hipMemcpyAsync(jac_camera, jc, jcw * 8 * (size_t)n, hipMemcpyDeviceToHost, s));
     if (jp && n) HIP_TRY(hipMemcpyAsync(jac_point, jp, 6 * 8 * (size_t)n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     HIP_TRY(hipGetLastError());
@@ -2012,10 +2365,16 @@ extern "C" {
 
 mba_status mba_evaluate(const mba_problem* p, int32_t precision, double huber_a, double* residuals,
                         double* jac_camera, double* jac_point, double* cost, int32_t device) {
+  return mba_evaluate_blocks(p, precision, huber_a, residuals, nullptr, jac_camera, jac_point, cost, device);
+}
+
+mba_status mba_evaluate_blocks(const mba_problem* p, int32_t precision, double huber_a, double* residuals,
+                               double* jac_intrinsics, double* jac_camera, double* jac_point, double* cost,
+                               int32_t device) {
   g_last_error.clear();
   if (precision == MBA_PRECISION_MIXED)
-    return evaluate_impl<float>(p, huber_a, residuals, jac_camera, jac_point, cost, device);
-  return evaluate_impl<double>(p, huber_a, residuals, jac_camera, jac_point, cost, device);
+    return evaluate_impl<float>(p, huber_a, residuals, jac_intrinsics, jac_camera, jac_point, cost, device);
+  return evaluate_impl<double>(p, huber_a, residuals, jac_intrinsics, jac_camera, jac_point, cost, device);
 }
 
 mba_status mba_scene_generate(int32_t preset, int32_t window, int32_t* num_cameras, int32_t* num_points,

@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define MBA_ABI_VERSION 1
+#define MBA_ABI_VERSION 2
 
 typedef enum mba_status {
   MBA_OK = 0,
@@ -65,7 +65,16 @@ typedef enum mba_termination {
 } mba_termination;
 
 typedef enum mba_layout {
-  MBA_LAYOUT_PACKED9 = 0 /* camera [aa(3), t(3), f, k1, k2] + point(3) */
+  /* camera [aa(3), t(3), f, k1, k2] + point(3): the camera of the
+   * benchmark scenes and of the functor of bundle_adjustment_unittest.cpp
+   * (principal point 0, k3 0). */
+  MBA_LAYOUT_PACKED9 = 0,
+  /* The incremental engine's layout (sfm_incremental_engine.cpp:1273-1446,
+   * ErrorFunc_Refine_Camera_3DPoints in
+   * libs/sfm/include/mvg/sfm/pinhole_brown_rt_ceres_functor.h:42-110):
+   * residual blocks (intrinsics, camera, point) with camera [aa(3), t(3)]
+   * and a shared intrinsic group [f, ppx, ppy, k1, k2, k3] per camera. */
+  MBA_LAYOUT_GROUPED = 1
 } mba_layout;
 
 /* Linear solver of the LM step.  EXPLICIT_SCHUR forms the reduced camera
@@ -88,14 +97,27 @@ typedef struct mba_problem {
   int32_t layout;       /* mba_layout */
   int32_t num_cameras;
   int32_t num_points;
-  int32_t reserved0;
+  int32_t num_intrinsics;       /* GROUPED: intrinsic groups (else ignored) */
   int64_t num_observations;
-  double* cameras;              /* 9 * num_cameras, in/out */
+  double* cameras;              /* 9 (PACKED9) or 6 (GROUPED) per camera, in/out */
   double* points;               /* 3 * num_points, in/out */
   const double* observations;   /* 2 * num_observations, pixels */
   const int32_t* camera_index;  /* per observation */
   const int32_t* point_index;   /* per observation */
   const uint8_t* camera_constant; /* num_cameras flags, or NULL */
+  /* GROUPED only: 6 * num_intrinsics [f, ppx, ppy, k1, k2, k3], in/out; the
+   * group of each observation (every observation of a camera must use the
+   * same group, as the incremental engine builds them); per-group constant
+   * flags (SetParameterBlockConstant) or NULL. */
+  double* intrinsics;
+  const int32_t* intrinsic_index;
+  const uint8_t* intrinsic_constant;
+  /* GROUPED only: per-group SubsetParameterization (include/ceres/
+   * local_parameterization.h:173), bit k set = entry k of
+   * [f, ppx, ppy, k1, k2, k3] held constant (the incremental engine's
+   * focal-only refinement is 0x3E, sfm_incremental_engine.cpp:1396-1409);
+   * NULL or 0: all six free. */
+  const uint8_t* intrinsic_subset;
 } mba_problem;
 
 typedef struct mba_options {
@@ -185,15 +207,27 @@ mba_status mba_solve_batched(const mba_options* options, mba_problem* problems,
 
 /* CostFunction::Evaluate for every observation, input order, computed on the
  * device in `precision`.  residuals: 2 per observation (no loss applied);
- * jac_camera: row-major 2x9 per observation; jac_point: row-major 2x3; each
- * may be NULL.  cost: robustified 1/2 sum rho (FP64), may be NULL. */
+ * jac_camera: row-major 2x9 per observation (2x6 for GROUPED); jac_point:
+ * row-major 2x3; each may be NULL.  cost: robustified 1/2 sum rho (FP64),
+ * may be NULL.  Same as mba_evaluate_blocks with jac_intrinsics NULL. */
 mba_status mba_evaluate(const mba_problem* problem, int32_t precision,
                         double huber_a, double* residuals, double* jac_camera,
                         double* jac_point, double* cost, int32_t device);
 
+/* mba_evaluate for either layout with the intrinsics block too: GROUPED
+ * gives jac_intrinsics 2x6 ([f, ppx, ppy, k1, k2, k3]) and jac_camera 2x6
+ * ([aa, t]) per observation; PACKED9 gives jac_camera 2x9 and leaves
+ * jac_intrinsics untouched.  Any output may be NULL. */
+mba_status mba_evaluate_blocks(const mba_problem* problem, int32_t precision,
+                               double huber_a, double* residuals,
+                               double* jac_intrinsics, double* jac_camera,
+                               double* jac_point, double* cost, int32_t device);
+
 /* The bit-exact structure the reference's preprocessor builds
  * (reorder_program.cc:436-531, schur_complement_solver.cc:246-313):
- *   param_order: column blocks in order (cameras 0..Nc-1, points Nc..);
+ *   param_order: column blocks in order (PACKED9: cameras 0..Nc-1, points
+ *     Nc..; GROUPED: intrinsic groups 0..Ni-1, cameras Ni.., points Ni+Nc..),
+ *     room for every block (Ni + Nc + Np);
  *   num_e_blocks: size of the eliminated group;
  *   residual_order: input observation index of each row block;
  *   pairs: upper-triangle block pairs of S (reduced indices), set order.

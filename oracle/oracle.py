@@ -240,7 +240,7 @@ def solve(scene: Scene, opts: Options, max_records=1000):
 def evaluate(scene: Scene, huber_a=0.0, jacobians=True):
     n = scene.num_obs
     r = np.zeros((n, 2))
-    cs = 9 if scene.layout == 0 else 12
+    cs = 9 if scene.layout == 0 else 6  # layout 1: the extrinsic block
     jc = np.zeros((n, 2, cs)) if jacobians else None
     jp = np.zeros((n, 2, 3)) if jacobians else None
     cost = C.c_double()

@@ -54,3 +54,10 @@ def tiny_scene(O):
 def ba_problem(mba, d, const=None):
     """Fresh (copied) parameter arrays so a solve never touches the fixture."""
     return mba.BAProblem(d["cam_init"].copy(), d["pt_init"].copy(), d["obs"], d["cam_idx"], d["pt_idx"], const)
+
+
+def grouped_problem(mba, d, const=None, intr_const=None, subset=None):
+    """BAProblem of a grouped-intrinsics scene (fresh parameter copies)."""
+    return mba.BAProblem(d["cam_init"].copy(), d["pt_init"].copy(), d["obs"], d["cam_idx"], d["pt_idx"], const,
+                         intrinsics=d["intr_init"].copy(), intrinsic_index=d["intr_idx"],
+                         intrinsic_constant=intr_const, intrinsic_subset=subset)

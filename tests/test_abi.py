@@ -8,7 +8,7 @@ import re
 import numpy as np
 import pytest
 
-from conftest import ROOT, ba_problem
+from conftest import ROOT, ba_problem, grouped_problem
 
 
 def header_symbols():
@@ -23,7 +23,9 @@ def test_library_exports_every_declared_symbol(mba):
     for name in syms:
         assert hasattr(lib, name), name
     assert set(syms) == set(mba.EXPORTS)
-    assert lib.mba_abi_version() == 1
+    assert lib.mba_abi_version() == 2
+    # mba_problem: 4 int32, 1 int64, 10 pointers (x86-64 layout, no padding).
+    assert ctypes.sizeof(mba._Problem) == 4 * 4 + 8 + 10 * 8
 
 
 def test_default_options_are_the_reference_defaults(mba):
@@ -51,6 +53,46 @@ def test_schur_structure_bit_exact_with_reference(mba, golden, c1, huber_scene, 
     np.testing.assert_array_equal(order, golden[f"struct_{name}_order"])
     np.testing.assert_array_equal(res, golden[f"struct_{name}_residual_order"])
     np.testing.assert_array_equal(pairs, golden[f"struct_{name}_pairs"])
+
+
+@pytest.fixture(scope="module")
+def group_scene(O):
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_golden import group_const, group_scene as gs
+    return gs(), group_const()
+
+
+def test_grouped_schur_structure_bit_exact_with_reference(mba, golden, group_scene):
+    """Residual blocks (intrinsics, camera, point) with shared groups and a
+    constant camera: E/F order, residual order and S pairs are the
+    reference's (block ids: groups, then cameras, then points)."""
+    d, const = group_scene
+    ne, order, res, pairs = mba.schur_structure(grouped_problem(mba, d, const))
+    assert ne == int(golden["grp_struct_num_e"])
+    np.testing.assert_array_equal(order, golden["grp_struct_order"])
+    np.testing.assert_array_equal(res, golden["grp_struct_residual_order"])
+    np.testing.assert_array_equal(pairs, golden["grp_struct_pairs"])
+
+
+def test_grouped_problem_checks(mba, group_scene):
+    d, _ = group_scene
+    p = grouped_problem(mba, d)
+    p.intrinsic_index = p.intrinsic_index.copy()
+    p.intrinsic_index[5] = 7
+    with pytest.raises(mba.MbaError) as e:
+        mba.schur_structure(p)
+    assert e.value.status == mba.ERR_INVALID_ARGUMENT and "intrinsic index" in str(e.value)
+    # A camera observed through two groups is not the incremental engine's
+    # structure.
+    p = grouped_problem(mba, d)
+    cam = int(p.camera_index[0])
+    p.intrinsic_index = p.intrinsic_index.copy()
+    rows = np.flatnonzero(p.camera_index == cam)
+    p.intrinsic_index[rows[1]] = (p.intrinsic_index[rows[0]] + 1) % 3
+    with pytest.raises(mba.MbaError) as e:
+        mba.schur_structure(p)
+    assert e.value.status == mba.ERR_UNSUPPORTED and "two intrinsic groups" in str(e.value)
 
 
 def test_schur_structure_matches_oracle_on_700k_scene(mba, O):

@@ -1,0 +1,156 @@
+"""GPU parity of the grouped-intrinsics layout: the incremental engine's
+residual blocks (intrinsics, camera, point) with intrinsic groups shared by
+cameras (sfm_incremental_engine.cpp:1413-1446).  Checked against the
+reference's Ceres fixtures and the FP64 oracle; every call goes through the
+C ABI (libmba.so).
+
+Tolerances (stated per test):
+  FP64 evaluate -- residuals and Jacobians 1e-10 relative (absolute 1e-9 px,
+                   1e-8 for extrinsic columns, whose entries reach ~1e4);
+  FP64 LM       -- costs 1e-9 relative, CG iteration counts equal, as for the
+                   packed layout; where the oracle itself meets Ceres only to
+                   1e-6 (Huber + constant camera, long CG solves) the same
+                   bars as the oracle's;
+  mixed         -- per-entry |a - b| <= 1e-5 (|b| + column RMS) against FP64,
+                   final cost within 1e-6 relative of the FP64 reference.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, grouped_problem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def group_scene(O):
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_golden import group_const, group_scene as gs
+    return gs(), group_const()
+
+
+def rel_err(a, b, axis=0):
+    rms = np.sqrt(np.mean(b * b, axis=axis, keepdims=True)) + 1e-30
+    return float(np.max(np.abs(a - b) / (np.abs(b) + rms)))
+
+
+def test_grouped_evaluate_matches_reference_jets(mba, golden, group_scene):
+    d, _ = group_scene
+    r, ji, jc, jp, _ = mba.evaluate_blocks(grouped_problem(mba, d), mba.PRECISION_FP64)
+    idx = golden["grp_jets_index"]
+    np.testing.assert_allclose(r[idx], golden["grp_jets_residual"], rtol=1e-10, atol=1e-9)
+    np.testing.assert_allclose(ji[idx], golden["grp_jets_jac_intrinsics"], rtol=1e-10, atol=1e-9)
+    np.testing.assert_allclose(jc[idx], golden["grp_jets_jac_camera"], rtol=1e-10, atol=1e-8)
+    np.testing.assert_allclose(jp[idx], golden["grp_jets_jac_point"], rtol=1e-10, atol=1e-9)
+
+
+def test_grouped_evaluate_matches_oracle_and_closed_forms(mba, O, group_scene):
+    d, _ = group_scene
+    r, ji, jc, jp, cost = mba.evaluate_blocks(grouped_problem(mba, d), mba.PRECISION_FP64, huber_a=4.0)
+    ro, jco, jpo, costo = O.evaluate(O.group_scene(d), huber_a=4.0)
+    np.testing.assert_allclose(r, ro, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(jc, jco, rtol=1e-10, atol=1e-8)
+    np.testing.assert_allclose(jp, jpo, rtol=1e-10, atol=1e-9)
+    np.testing.assert_allclose(cost, costo, rtol=1e-12)
+    # Intrinsic columns [f, ppx, ppy, k1, k2, k3] from the model's closed
+    # form: f dr/df = pred - pp; dr/dpp = I; dr/dk_{j+1} = r^2 dr/dk_j, so
+    # (dr/dk2)^2 = dr/dk1 dr/dk3.
+    intr = d["intr_init"][d["intr_idx"]]
+    pred_minus_pp = r + d["obs"] - intr[:, 1:3]
+    np.testing.assert_allclose(intr[:, :1] * ji[:, :, 0], pred_minus_pp, rtol=1e-12, atol=1e-9)
+    np.testing.assert_array_equal(ji[:, :, 1], np.tile([1.0, 0.0], (len(r), 1)))
+    np.testing.assert_array_equal(ji[:, :, 2], np.tile([0.0, 1.0], (len(r), 1)))
+    np.testing.assert_allclose(ji[:, :, 4] ** 2, ji[:, :, 3] * ji[:, :, 5], rtol=1e-10, atol=1e-12)
+
+
+def test_grouped_evaluate_mixed_within_fp32_rounding(mba, group_scene):
+    d, _ = group_scene
+    p = grouped_problem(mba, d)
+    r, ji, jc, jp, _ = mba.evaluate_blocks(p, mba.PRECISION_FP64)
+    r32, ji32, jc32, jp32, _ = mba.evaluate_blocks(p, mba.PRECISION_MIXED)
+    # A residual is a difference of ~500 px values: its FP32 error is
+    # relative to (|pred| + f), not to the residual.
+    scale = np.abs(r + d["obs"]) + d["intr_init"][d["intr_idx"], :1]
+    assert np.max(np.abs(r32 - r) / scale) <= 1e-5
+    for a, b in [(ji32, ji), (jc32, jc), (jp32, jp)]:
+        assert rel_err(a.reshape(len(r), -1), b.reshape(len(r), -1)) <= 1e-5
+
+
+def test_grouped_fp64_pcg_matches_reference_ceres(mba, golden, group_scene):
+    """ITERATIVE_SCHUR + JACOBI on the grouped scene, 20 iterations: the
+    reference's trajectory, CG counts and result."""
+    d, _ = group_scene
+    p = grouped_problem(mba, d)
+    s = mba.solve(p, mba.default_options(max_num_iterations=20, linear_solver=1))
+    ref = golden["grp_iterative_iterations"]
+    assert s.IsSolutionUsable()
+    assert (s.num_successful_steps, s.num_unsuccessful_steps, s.termination_type) == (
+        int(golden["grp_iterative_successful"]), int(golden["grp_iterative_unsuccessful"]),
+        int(golden["grp_iterative_termination"]))
+    np.testing.assert_allclose([it.cost for it in s.iterations], ref[:, 1], rtol=1e-9)
+    assert [it.linear_solver_iterations for it in s.iterations] == [int(v) for v in ref[:, 6]]
+    np.testing.assert_allclose(s.final_cost, float(golden["grp_iterative_final_cost"]), rtol=1e-10)
+    # Weakly determined directions still moving at iteration 20: absolute
+    # tolerances for poses and points (as the oracle's own check).
+    np.testing.assert_allclose(p.cameras, golden["grp_iterative_cameras"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(p.intrinsics, golden["grp_iterative_intrinsics"], rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(p.points, golden["grp_iterative_points"], rtol=0, atol=1e-5)
+
+
+def test_grouped_fp64_pcg_huber_constant_camera(mba, golden, group_scene):
+    """Huber (4 px) with 5% outliers and a constant camera: long CG solves
+    amplify summation-order rounding, so (as for the oracle against Ceres)
+    the first four iterations are held to 1e-9 with equal CG counts for the
+    first five, every cost to 1e-6 and the final cost to 1e-7."""
+    d, cc = group_scene
+    s = mba.solve(grouped_problem(mba, d, cc),
+                  mba.default_options(max_num_iterations=20, linear_solver=1, huber_a=4.0))
+    ref = golden["grp_iterative_const_iterations"]
+    its = s.iterations
+    assert len(its) == ref.shape[0]
+    np.testing.assert_allclose([it.cost for it in its[:4]], ref[:4, 1], rtol=1e-9)
+    assert [it.linear_solver_iterations for it in its[:5]] == [int(v) for v in ref[:5, 6]]
+    np.testing.assert_allclose([it.cost for it in its], ref[:, 1], rtol=1e-6)
+    np.testing.assert_allclose(s.final_cost, float(golden["grp_iterative_const_final_cost"]), rtol=1e-7)
+
+
+def test_grouped_constant_blocks_stay_put(mba, group_scene):
+    """SetParameterBlockConstant on a camera and on a group: their values are
+    returned bit for bit."""
+    d, cc = group_scene
+    ic = np.zeros(3, np.uint8)
+    ic[1] = 1
+    p = grouped_problem(mba, d, cc, ic)
+    s = mba.solve(p, mba.default_options(max_num_iterations=5, linear_solver=1, huber_a=4.0))
+    assert s.IsSolutionUsable() and s.final_cost < s.initial_cost
+    np.testing.assert_array_equal(p.cameras[cc == 1], d["cam_init"][cc == 1])
+    np.testing.assert_array_equal(p.intrinsics[1], d["intr_init"][1])
+    assert np.all(p.intrinsics[[0, 2]] != d["intr_init"][[0, 2]])
+
+
+def test_grouped_single_shared_group_matches_oracle(mba, O):
+    """One intrinsic group shared by all 40 cameras (a single-camera
+    sequence): the shared block couples every camera and every point; the
+    trajectory follows the oracle's."""
+    d = O.generate_groups(O.make_spec(num_cameras=40, num_points=3000, track_min=2, track_max=12,
+                                      outlier_frac=0.05, seed=77), 1)
+    opts = dict(max_num_iterations=10, linear_solver=1, huber_a=4.0)
+    so, its = O.solve(O.group_scene(d), O.default_options(**opts))
+    s = mba.solve(mba.BAProblem(d["cam_init"].copy(), d["pt_init"].copy(), d["obs"], d["cam_idx"], d["pt_idx"],
+                                intrinsics=d["intr_init"].copy(), intrinsic_index=d["intr_idx"]),
+                  mba.default_options(**opts))
+    n = min(len(its), 4)
+    np.testing.assert_allclose([it.cost for it in s.iterations[:n]], [it.cost for it in its[:n]], rtol=1e-9)
+    np.testing.assert_allclose(s.final_cost, so.final_cost, rtol=1e-7)
+
+
+def test_grouped_mixed_reaches_fp64_final_cost(mba, golden, group_scene):
+    d, _ = group_scene
+    s = mba.solve(grouped_problem(mba, d), mba.default_options(max_num_iterations=20, linear_solver=1,
+                                                               precision=mba.PRECISION_MIXED))
+    ref = float(golden["grp_iterative_final_cost"])
+    assert s.IsSolutionUsable()
+    assert abs(s.final_cost - ref) / ref < 1e-6
