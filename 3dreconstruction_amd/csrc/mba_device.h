@@ -1623,10 +1623,11 @@ __global__ __launch_bounds__(kBlock) void k_accept_cams(int nf, Win win, const i
 }
 
 // ----------------------------------------------------------- explicit S
-// Dense lower-triangular S (column-major, n = 9 Nf):
+// Dense S (column-major, n = 9 Nf, both triangles):
 //   S(fa, fb) = [fa == fb] (U + D^2) - sum over observation rows a, b of a
 //               common point with cameras fa >= fb of  W_a Vinv W_b',
-// W_r = Jc_r' Jp_r (9 x 3, Jacobi-scaled columns).
+// W_r = Jc_r' Jp_r (9 x 3, Jacobi-scaled columns).  In the grouped layout
+// the rows are virtual rows (one per observation and F block it touches).
 
 // Per point-major row: W_r and Y_r = W_r Vinv_e, kWY values per row
 // (W at 0, Y at kWYY, row-major 9 x 3, padded for 16-byte loads).
@@ -1686,7 +1687,8 @@ __global__ __launch_bounds__(64) void k_s_blocks(
     const int64_t* __restrict__ blk_start, const int32_t* __restrict__ blk_fa,
     const int32_t* __restrict__ blk_fb, const int32_t* __restrict__ pair_a,
     const int32_t* __restrict__ pair_b, const T* __restrict__ WY, const T* __restrict__ U,
-    const double* __restrict__ diag_c, double inv_radius, int64_t n, T* __restrict__ S) {
+    const double* __restrict__ diag_c, double inv_radius, int64_t n, T* __restrict__ S,
+    const int32_t* __restrict__ blk_cross, const T* __restrict__ cross) {
   __shared__ T Ys[64][kWYY];
   __shared__ T Ws[64][kWYY];
   const int blk = blockIdx.x;
@@ -1718,11 +1720,18 @@ __global__ __launch_bounds__(64) void k_s_blocks(
     }
     __syncthreads();
   }
+  // Grouped layout: a camera / group block pair also carries the F'F
+  // coupling of their common rows, cross[36 c + 6 a + b] (camera slot a,
+  // group slot b); blk_cross = 2 c + (1 if the group block is fa).
+  const int xc = blk_cross ? blk_cross[blk] : -1;
   auto put = [&](int i, int j, T acc) {
     T base = T(0);
     if (fa == fb) {
       base = U[45 * (size_t)fa + (i <= j ? ut9(i, j) : ut9(j, i))];
       if (i == j) base += (T)(diag_c[9 * (size_t)fa + i] * inv_radius);
+    } else if (xc >= 0 && i < 6 && j < 6) {
+      const T* x = cross + 36 * (size_t)(xc >> 1);
+      base = (xc & 1) ? x[6 * j + i] : x[6 * i + j];
     }
     const int64_t row = 9 * (int64_t)fa + i, col = 9 * (int64_t)fb + j;
     S[row + col * n] = base - acc;

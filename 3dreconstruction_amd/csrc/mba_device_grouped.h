@@ -306,6 +306,130 @@ __global__ __launch_bounds__(kBlock) void k_mv_cam_g(
   }
 }
 
+// ------------------------------------------------------------ explicit S
+// Each observation row r has two virtual rows of S: its camera's (v = r)
+// and its group's (v = nrows + r), with W_v = J_v' J_p (9-slot rows, kWY
+// layout of the packed kernels) and Y_v = W_v Vinv; the packed block
+// assembly then runs over virtual-row pairs.  The camera-group block of F'F
+// (same observation, two F blocks) comes from k_cross_g / k_cross_final.
+template <typename T>
+__device__ __forceinline__ void write_wy(const T J[12], const T Jp[6], const T Vi[9], T* __restrict__ out) {
+  T w[18];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) w[3 * i + j] = J[i] * Jp[j] + J[6 + i] * Jp[3 + j];
+#pragma unroll
+  for (int k = 0; k < 18; ++k) out[k] = w[k];
+#pragma unroll
+  for (int k = 18; k < 27; ++k) out[k] = T(0);
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      out[kWYY + 3 * i + j] = w[3 * i] * Vi[j] + w[3 * i + 1] * Vi[3 + j] + w[3 * i + 2] * Vi[6 + j];
+#pragma unroll
+  for (int k = 18; k < 27; ++k) out[kWYY + k] = T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_s_rows_g(
+    Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const int32_t* __restrict__ cam_to_f, const int32_t* __restrict__ cam_gf, const T* __restrict__ row_obs,
+    const T* __restrict__ pkX, const T* __restrict__ pscale, const T* __restrict__ cscale,
+    const T* __restrict__ ctab, const T* __restrict__ Vinv, T huber_a, int nrows, T* __restrict__ WY) {
+  const SegPos q = seg_locate(sg);
+  if (!q.valid) return;
+  const int e = q.e;
+  T X[3], unused[3];
+  load_pk(pkX, e, X, unused);
+  const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
+  const T* iv = Vinv + 6 * (size_t)e;
+  const T Vi[9] = {iv[0], iv[1], iv[2], iv[1], iv[3], iv[4], iv[2], iv[4], iv[5]};
+  for (int r = pt_start[e] + q.sub; r < pt_start[e + 1]; r += q.lanes) {
+    const int cam = row_cam[r];
+    const int f = cam_to_f[cam], g = cam_gf[cam];
+    if (f < 0 && g < 0) continue;
+    const T* tab = ctab + (size_t)kCamTab * cam;
+    CamProj<T> c;
+    load_proj<T, 1>(tab, c);
+    Proj<T> o;
+    project<T, 1>(c, X, row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], o);
+    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    T Jp[6], J[12], cs[6];
+    jac_point(o, c.R, wgt, sp, Jp);
+    if (f >= 0) {
+      for (int k = 0; k < 6; ++k) cs[k] = cscale[9 * (size_t)f + k];
+      jac_ext(o, tab + kTabDR, X, wgt, cs, J);
+      write_wy(J, Jp, Vi, WY + (size_t)kWY * r);
+    }
+    if (g >= 0) {
+      for (int k = 0; k < 6; ++k) cs[k] = cscale[9 * (size_t)g + k];
+      jac_intr(o, wgt, cs, J);
+      write_wy(J, Jp, Vi, WY + (size_t)kWY * ((size_t)nrows + r));
+    }
+  }
+}
+
+// Per camera-major chunk: sum over its rows of J_e' J_i (6 x 6, unscaled).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cross_g(
+    const int32_t* __restrict__ chunk_cam, const int32_t* __restrict__ chunk_begin,
+    const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs,
+    const T* __restrict__ pk, const T* __restrict__ ctab, T huber_a, T* __restrict__ partial) {
+  __shared__ T lds[36 * (kBlock / 64)];
+  const int ch = blockIdx.x;
+  const T* tab = ctab + (size_t)kCamTab * chunk_cam[ch];
+  CamProj<T> c;
+  load_proj<T, 1>(tab, c);
+  T dR[27];
+#pragma unroll
+  for (int i = 0; i < 27; ++i) dR[i] = tab[kTabDR + i];
+  const T one[6] = {T(1), T(1), T(1), T(1), T(1), T(1)};
+  T acc[36];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) acc[i] = T(0);
+  for (int i = chunk_begin[ch] + threadIdx.x; i < chunk_end[ch]; i += kBlock) {
+    T X[3], unused[3];
+    load_pk(pk, cm_e[i], X, unused);
+    Proj<T> o;
+    project<T, 1>(c, X, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], o);
+    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    T Je[12], Ji[12];
+    jac_ext(o, dR, X, wgt, one, Je);
+    jac_intr(o, wgt, one, Ji);
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = 0; b < 6; ++b) fma2(acc[6 * a + b], Je[a], Ji[b], Je[6 + a], Ji[6 + b]);
+  }
+  block_sum<T, 36>(acc, lds);
+  if (threadIdx.x < 36) {
+    T s = T(0);
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w * 36 + threadIdx.x];
+    partial[(size_t)kPartial * ch + threadIdx.x] = s;
+  }
+}
+
+// cross[36 c + 6 a + b] = s_e[a] s_i[b] (sum of camera c's chunk partials)
+// for cameras whose block and group block are both free.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cross_final(int nc, const int32_t* __restrict__ cam_to_f,
+                                                        const int32_t* __restrict__ cam_gf,
+                                                        const int32_t* __restrict__ f_lo,
+                                                        const int32_t* __restrict__ f_hi,
+                                                        const T* __restrict__ partial, const T* __restrict__ cscale,
+                                                        T* __restrict__ cross) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int c = (int)(t / 36), k = (int)(t % 36);
+  if (c >= nc) return;
+  const int f = cam_to_f[c], g = cam_gf[c];
+  if (f < 0 || g < 0) return;
+  T s = T(0);
+  for (int ch = f_lo[f]; ch < f_hi[f]; ++ch) s += partial[(size_t)kPartial * ch + k];
+  cross[36 * (size_t)c + k] = s * cscale[9 * (size_t)f + k / 6] * cscale[9 * (size_t)g + k % 6];
+}
+
 // Zero the kernel-precision scale of slots outside a block's local
 // parameterisation (unused slots and SubsetParameterization constants), so
 // their Jacobian columns vanish; mask[i] is 1 for a free slot.

@@ -97,6 +97,13 @@ mba_status validate(const mba_problem* p) {
       if (p->intrinsic_index[i] < 0 || p->intrinsic_index[i] >= p->num_intrinsics)
         return fail(MBA_ERR_INVALID_ARGUMENT,
                     "observation " + std::to_string(i) + " has an intrinsic index out of range");
+    // SubsetParameterization needs at least one constant and one free entry
+    // (local_parameterization.cc:93-109); a fully held block is constant.
+    if (p->intrinsic_subset)
+      for (int32_t g = 0; g < p->num_intrinsics; ++g)
+        if (p->intrinsic_subset[g] > 0x3E && p->intrinsic_subset[g] != 0)
+          return fail(MBA_ERR_INVALID_ARGUMENT, "intrinsic_subset of group " + std::to_string(g) +
+                                                    " must hold 1..5 of the 6 entries (use intrinsic_constant)");
   }
   if (p->num_observations > 0x7fffffffLL) return fail(MBA_ERR_UNSUPPORTED, "more than 2^31-1 observations");
   if (p->num_observations > 0 && (!p->observations || !p->camera_index || !p->point_index))
@@ -490,7 +497,7 @@ struct Dev {
   T *x = nullptr, *r = nullptr, *zc = nullptr, *p = nullptr, *q = nullptr, *tmp = nullptr;
   T *step_p = nullptr, *step_c = nullptr;
   // explicit S
-  T *S = nullptr, *L = nullptr, *sv = nullptr, *WY = nullptr, *Linv = nullptr;
+  T *S = nullptr, *L = nullptr, *sv = nullptr, *WY = nullptr, *Linv = nullptr, *cross = nullptr;
 };
 
 }  // namespace
@@ -538,8 +545,11 @@ struct mba_context {
   uint8_t* wactive = nullptr;  // windows still iterating
   // Explicit Schur (single problem): the blocks of S that points couple and
   // their observation-row pairs, sorted by block.
+  // Grouped layout: rows are virtual rows (camera part r, group part
+  // nrows + r); s_blk_cross marks camera / group block pairs.
   int64_t* s_blk_start = nullptr;
   int32_t *s_blk_fa = nullptr, *s_blk_fb = nullptr, *s_pair_a = nullptr, *s_pair_b = nullptr;
+  int32_t* s_blk_cross = nullptr;
   int64_t s_nblk = 0, s_npairs = 0;
   // Host copies of what was last uploaded (rewritten only while the stream
   // is idle).
@@ -704,10 +714,18 @@ std::string window_prefix(int32_t W, int32_t w) {
 mba_status build_s_pairs(mba_context* c, const std::vector<int32_t>& pt_start,
                          const std::vector<int32_t>& row_cam) {
   const int32_t nf = c->nf, ne = c->ne;
-  std::vector<int32_t> row_f(c->nrows);
+  // Virtual rows: one per observation (packed), two in the grouped layout
+  // (its camera block at r, its group block at nrows + r).
+  const int nv = c->grouped ? 2 : 1;
+  if ((int64_t)nv * c->nrows > 0x7fffffffLL) return fail(MBA_ERR_UNSUPPORTED, "explicit Schur: too many rows");
+  std::vector<int32_t> row_f((size_t)nv * c->nrows);
   parallel_for(c->nrows, [&](int64_t b, int64_t e) {
-    for (int64_t r = b; r < e; ++r) row_f[r] = c->cam_to_f[row_cam[r]];
+    for (int64_t r = b; r < e; ++r) {
+      row_f[r] = c->cam_to_f[row_cam[r]];
+      if (nv == 2) row_f[c->nrows + r] = c->cam_gf[row_cam[r]];
+    }
   });
+  const int32_t nrows = c->nrows;
   const int nt = host_threads(ne, 4096);
   const int64_t step = ((int64_t)ne + nt - 1) / nt;
   // Pairs per fa, per thread (point ranges), then bucket offsets.
@@ -715,14 +733,16 @@ mba_status build_s_pairs(mba_context* c, const std::vector<int32_t>& pt_start,
   auto for_pairs = [&](int t, auto&& fn) {
     const int64_t end = std::min<int64_t>(ne, (t + 1) * step);
     for (int64_t e = t * step; e < end; ++e)
-      for (int32_t a = pt_start[e]; a < pt_start[e + 1]; ++a) {
-        const int32_t fa = row_f[a];
-        if (fa < 0) continue;
-        for (int32_t b = pt_start[e]; b < pt_start[e + 1]; ++b) {
-          const int32_t fb = row_f[b];
-          if (fb >= 0 && fb <= fa) fn(fa, a, b);
+      for (int ka = 0; ka < nv; ++ka)
+        for (int32_t ra = pt_start[e]; ra < pt_start[e + 1]; ++ra) {
+          const int32_t a = ka * nrows + ra, fa = row_f[a];
+          if (fa < 0) continue;
+          for (int kb = 0; kb < nv; ++kb)
+            for (int32_t rb = pt_start[e]; rb < pt_start[e + 1]; ++rb) {
+              const int32_t b = kb * nrows + rb, fb = row_f[b];
+              if (fb >= 0 && fb <= fa) fn(fa, a, b);
+            }
         }
-      }
   };
   run_threads(nt, [&](int t) { for_pairs(t, [&](int32_t fa, int32_t, int32_t) { cnt[t][fa]++; }); });
   std::vector<int64_t> fa_start((size_t)nf + 1, 0);
@@ -788,6 +808,16 @@ mba_status build_s_pairs(mba_context* c, const std::vector<int32_t>& pt_start,
     }
   blk_start.push_back(o);
   c->s_nblk = (int64_t)blk_fa.size();
+  if (c->grouped) {
+    // Camera / group pairs: 2 c (+1 when the group block is fa).
+    std::vector<int32_t> cross(blk_fa.size(), -1);
+    for (size_t k = 0; k < blk_fa.size(); ++k) {
+      const int32_t ba = c->f_to_cam[blk_fa[k]], bb = c->f_to_cam[blk_fb[k]];
+      if (ba < c->nc && bb >= c->nc && c->cam_gf[ba] == blk_fb[k]) cross[k] = 2 * ba;
+      else if (bb < c->nc && ba >= c->nc && c->cam_gf[bb] == blk_fa[k]) cross[k] = 2 * bb + 1;
+    }
+    MBA_TRY(upload(c, &c->s_blk_cross, cross));
+  }
   MBA_TRY(upload(c, &c->s_blk_start, blk_start));
   MBA_TRY(upload(c, &c->s_blk_fa, blk_fa));
   MBA_TRY(upload(c, &c->s_blk_fb, blk_fb));
@@ -1152,8 +1182,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     MBA_TRY(alloc_precision<float>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_lo, f_hi));
   else
     MBA_TRY(alloc_precision<double>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_lo, f_hi));
-  if (W == 1 && c->opt.linear_solver == MBA_EXPLICIT_SCHUR && !c->grouped)
-    MBA_TRY(build_s_pairs(c, dev_pt_start, row_cam));
+  if (W == 1 && c->opt.linear_solver == MBA_EXPLICIT_SCHUR) MBA_TRY(build_s_pairs(c, dev_pt_start, row_cam));
   for (int i = 0; i < 8; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
   HIP_TRY(hipStreamSynchronize(c->stream));
   *out = c;
@@ -1446,7 +1475,8 @@ struct Engine {
     if (!d.S) {
       MBA_TRY(c->pool.alloc(&d.S, (size_t)(n * n)));
       MBA_TRY(c->pool.alloc(&d.L, (size_t)(n * n)));
-      MBA_TRY(c->pool.alloc(&d.WY, (size_t)kWY * c->nrows));
+      MBA_TRY(c->pool.alloc(&d.WY, (size_t)kWY * c->nrows * (c->grouped ? 2 : 1)));
+      if (c->grouped) MBA_TRY(c->pool.alloc(&d.cross, 36 * (size_t)std::max(1, c->nc)));
       MBA_TRY(c->pool.alloc(&d.Linv, (size_t)kNB * kNB));
       if (!c->gemv_part) MBA_TRY(c->pool.alloc(&c->gemv_part, (size_t)kGemvSplit * n));
     }
@@ -1454,12 +1484,26 @@ struct Engine {
     if (c->timing) (void)hipEventRecord(c->ev[4], s);
     // Blocks no point couples stay zero.
     HIP_TRY(hipMemsetAsync(d.S, 0, sizeof(T) * n * n, s));
-    if (c->seg_blocks > 0)
+    if (c->grouped) {
+      if (c->seg_blocks > 0)
+        hipLaunchKernelGGL(k_s_rows_g<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
+                           d.cam_to_f, (const int32_t*)c->d_cam_gf, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab,
+                           d.Vinv, huber, c->nrows, d.WY);
+      // The camera-group blocks of F'F (partials reuse the pass's scratch).
+      if (c->nchunks > 0) {
+        hipLaunchKernelGGL(k_cross_g<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin, d.chunk_end,
+                           d.cm_e, d.cm_obs, d.pk, d.ctab, huber, d.partial);
+        hipLaunchKernelGGL(k_cross_final<T>, dim3(nblocks(36 * (int64_t)c->nc)), dim3(kBlock), 0, s, c->nc,
+                           d.cam_to_f, (const int32_t*)c->d_cam_gf, d.f_lo, d.f_hi, d.partial, d.cscale, d.cross);
+      }
+    } else if (c->seg_blocks > 0) {
       hipLaunchKernelGGL(k_s_rows<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
                          d.cam_to_f, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab, d.Vinv, huber, d.WY);
+    }
     if (c->s_nblk > 0)
       hipLaunchKernelGGL(k_s_blocks<T>, dim3((unsigned)c->s_nblk), dim3(64), 0, s, c->s_blk_start, c->s_blk_fa,
-                         c->s_blk_fb, c->s_pair_a, c->s_pair_b, d.WY, d.U, c->diag_c, inv_r, n, d.S);
+                         c->s_blk_fb, c->s_pair_a, c->s_pair_b, d.WY, d.U, c->diag_c, inv_r, n, d.S,
+                         (const int32_t*)c->s_blk_cross, (const T*)d.cross);
     if (c->timing) (void)hipEventRecord(c->ev[5], s);
     HIP_TRY(hipMemcpyAsync(d.L, d.S, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(c->info, 0, sizeof(int32_t), s));
@@ -2152,8 +2196,6 @@ mba_status mba_context_solve(mba_context* ctx, mba_summary* summary, mba_iterati
   if (!ctx || !summary) return fail(MBA_ERR_INVALID_ARGUMENT, "NULL argument");
   HIP_TRY(hipSetDevice(ctx->device));
   if (ctx->opt.linear_solver == MBA_EXPLICIT_SCHUR) {
-    if (ctx->grouped)
-      return fail(MBA_ERR_UNSUPPORTED, "grouped-intrinsics problems use MBA_IMPLICIT_SCHUR_PCG");
     const double n = 9.0 * ctx->nf;
     const double bytes = 2.0 * n * n * (ctx->mixed ? 4.0 : 8.0);
     if (bytes > 64.0 * (1ull << 30))

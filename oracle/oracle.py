@@ -178,10 +178,15 @@ def generate_groups(spec: SceneSpec, groups: int) -> dict:
     return d
 
 
-def group_scene(d, cam_constant=None, intr=None) -> "Scene":
-    """Oracle Scene (layout 1) of a grouped-intrinsics scene."""
-    return Scene(d, cam_constant=cam_constant, layout=1,
-                 intrinsics=d["intr_init"] if intr is None else intr, intr_idx=d["intr_idx"])
+def group_scene(d, cam_constant=None, intr=None, intr_constant=None, intr_subset=None) -> "Scene":
+    """Oracle Scene (layout 1) of a grouped-intrinsics scene.  intr_constant /
+    intr_subset (per-group constant flags, SubsetParameterization masks) are
+    honoured by the reference harness only."""
+    sc = Scene(d, cam_constant=cam_constant, layout=1,
+               intrinsics=d["intr_init"] if intr is None else intr, intr_idx=d["intr_idx"])
+    sc.intr_constant = None if intr_constant is None else np.ascontiguousarray(intr_constant, np.uint8)
+    sc.intr_subset = None if intr_subset is None else np.ascontiguousarray(intr_subset, np.uint8)
+    return sc
 
 
 class Scene:
@@ -198,6 +203,8 @@ class Scene:
         self.cam_constant = None if cam_constant is None else np.ascontiguousarray(cam_constant, np.uint8)
         self.intrinsics = None if intrinsics is None else np.ascontiguousarray(intrinsics, np.float64).copy()
         self.intr_idx = None if intr_idx is None else np.ascontiguousarray(intr_idx, np.int32)
+        self.intr_constant = None
+        self.intr_subset = None
         self.num_cameras = self.cameras.shape[0]
         self.num_points = self.points.shape[0]
         self.num_obs = self.obs.shape[0]
@@ -322,7 +329,8 @@ def ref_schur_ordering(scene: Scene, with_pairs=True):
 def ref_solve(scene: Scene, huber_a=0.0, linear_solver=0, max_iters=50, num_threads=1, max_rec=1000):
     summ = np.zeros(8)
     its = np.zeros((max_rec, 9))
-    usable = ref_lib().ref_solve(*_ref_args(scene), C.c_double(huber_a), C.c_int(linear_solver),
+    usable = ref_lib().ref_solve(*_ref_args(scene), ptr(scene.intr_constant, _u8p), ptr(scene.intr_subset, _u8p),
+                                 C.c_double(huber_a), C.c_int(linear_solver),
                                  C.c_int(max_iters), C.c_int(num_threads), ptr(summ), ptr(its),
                                  C.c_int(max_rec))
     n = int(summ[5])

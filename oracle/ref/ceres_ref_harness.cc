@@ -104,6 +104,10 @@ struct Scene {
   const double* obs;
   const int *cam_idx, *intr_idx, *pt_idx;
   const unsigned char* cam_const;
+  // Layout 1: per-group SetParameterBlockConstant flags and
+  // SubsetParameterization masks (bit k: entry k constant); may be NULL.
+  const unsigned char* intr_const;
+  const unsigned char* intr_subset;
 };
 
 ceres::CostFunction* MakeCost(const Scene& s, long i) {
@@ -140,6 +144,26 @@ void AddAll(const Scene& s, P* problem, ceres::LossFunction* loss,
   if (s.cam_const)
     for (int c = 0; c < s.nc; ++c)
       if (s.cam_const[c]) problem->SetParameterBlockConstant(s.cams + (long)cs * c);
+  if (s.layout == 1) {
+    // One SubsetParameterization per distinct mask, shared by the groups
+    // that use it (the incremental engine shares one, :1396-1436).
+    std::map<int, ceres::LocalParameterization*> param;
+    for (int g = 0; g < s.ni; ++g) {
+      double* blk = s.intr + 6L * g;
+      if (s.intr_const && s.intr_const[g]) {
+        problem->SetParameterBlockConstant(blk);
+      } else if (s.intr_subset && s.intr_subset[g]) {
+        const int m = s.intr_subset[g];
+        if (!param.count(m)) {
+          std::vector<int> held;
+          for (int k = 0; k < 6; ++k)
+            if ((m >> k) & 1) held.push_back(k);
+          param[m] = new ceres::SubsetParameterization(6, held);
+        }
+        problem->SetParameterization(blk, param[m]);
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -236,10 +260,11 @@ int ref_schur_ordering(int layout, int nc, int ni, int np, long nobs, double* ca
 // gradient_max_norm}.
 int ref_solve(int layout, int nc, int ni, int np, long nobs, double* cams, double* intr,
               double* pts, const double* obs, const int* cam_idx, const int* intr_idx,
-              const int* pt_idx, const unsigned char* cam_const, double huber_a,
-              int linear_solver, int max_iters, int num_threads, double* summary,
-              double* iters, int max_rec) {
-  Scene s = {layout, nc, ni, np, nobs, cams, intr, pts, obs, cam_idx, intr_idx, pt_idx, cam_const};
+              const int* pt_idx, const unsigned char* cam_const, const unsigned char* intr_const,
+              const unsigned char* intr_subset, double huber_a, int linear_solver, int max_iters,
+              int num_threads, double* summary, double* iters, int max_rec) {
+  Scene s = {layout, nc, ni, np, nobs, cams, intr, pts, obs, cam_idx, intr_idx, pt_idx, cam_const,
+             intr_const, intr_subset};
   ceres::Problem problem;
   ceres::LossFunction* loss = huber_a > 0.0 ? new ceres::HuberLoss(huber_a) : NULL;
   AddAll(s, &problem, loss, NULL);

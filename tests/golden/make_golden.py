@@ -32,6 +32,10 @@ HUBER_SPEC = dict(num_cameras=30, num_points=3000, track_min=2, track_max=10, po
 GROUP_SPEC = dict(num_cameras=24, num_points=1500, track_min=2, track_max=8, outlier_frac=0.05, seed=31)
 GROUPS = 3
 GROUP_CONST = 2  # SetParameterBlockConstant on this camera's extrinsic block
+# SubsetParameterization masks per group (bit k = entry k of [f, ppx, ppy,
+# k1, k2, k3] held constant) and per-group constant flags.
+GROUP_SUBSET = np.array([0x3E, 0x38, 0x00], np.uint8)
+GROUP_INTR_CONST = np.array([0, 1, 0], np.uint8)
 
 
 def group_scene():
@@ -134,6 +138,20 @@ def main():
                                 ("iterative_const", 4.0, 1, group_const())]:
         sc = O.group_scene(dg, cam_constant=cc)
         r = O.ref_solve(sc, huber_a=huber, linear_solver=ls, max_iters=20)
+        for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
+            out[f"grp_{name}_{k}"] = np.asarray(r[k])
+        out[f"grp_{name}_cameras"] = sc.cameras
+        out[f"grp_{name}_intrinsics"] = sc.intrinsics
+        out[f"grp_{name}_points"] = sc.points
+    # 6. The incremental engine's intrinsic options on the grouped scene:
+    #    SubsetParameterization masks (0x3E = focal only, its default;
+    #    0x38 = k1..k3 held) and a constant group, Huber 4 px, the constant
+    #    camera, 20 iterations.
+    for name, ls, sub, ic in [("subset_sparse", 0, GROUP_SUBSET, None),
+                              ("subset_iterative", 1, np.full(GROUPS, 0x3E, np.uint8), None),
+                              ("const_group_sparse", 0, None, GROUP_INTR_CONST)]:
+        sc = O.group_scene(dg, cam_constant=group_const(), intr_constant=ic, intr_subset=sub)
+        r = O.ref_solve(sc, huber_a=4.0, linear_solver=ls, max_iters=20)
         for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
             out[f"grp_{name}_{k}"] = np.asarray(r[k])
         out[f"grp_{name}_cameras"] = sc.cameras

@@ -95,6 +95,17 @@ def test_grouped_problem_checks(mba, group_scene):
     assert e.value.status == mba.ERR_UNSUPPORTED and "two intrinsic groups" in str(e.value)
 
 
+def test_grouped_subset_mask_checks(mba, group_scene):
+    """SubsetParameterization must hold 1..5 of the 6 entries
+    (local_parameterization.cc:93-109): a full mask is an argument error."""
+    d, _ = group_scene
+    for bad in [0x3F, 0x40]:
+        with pytest.raises(mba.MbaError) as e:
+            mba.schur_structure(grouped_problem(mba, d, subset=np.array([bad, 0, 0], np.uint8)))
+        assert e.value.status == mba.ERR_INVALID_ARGUMENT
+    mba.schur_structure(grouped_problem(mba, d, subset=np.array([0x3E, 0x01, 0], np.uint8)))
+
+
 def test_schur_structure_matches_oracle_on_700k_scene(mba, O):
     d = O.generate(O.preset(2))
     ne, order, res, pairs = mba.schur_structure(ba_problem(mba, d))

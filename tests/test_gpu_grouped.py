@@ -100,6 +100,29 @@ def test_grouped_fp64_pcg_matches_reference_ceres(mba, golden, group_scene):
     np.testing.assert_allclose(p.points, golden["grp_iterative_points"], rtol=0, atol=1e-5)
 
 
+def test_grouped_fp64_explicit_matches_reference_sparse_schur(mba, golden, group_scene):
+    """The incremental engine's own call: SPARSE_SCHUR, Huber 4 px, a
+    constant camera (explicit reduced system, dense Cholesky on the device)."""
+    d, cc = group_scene
+    p = grouped_problem(mba, d, cc)
+    s = mba.solve(p, mba.default_options(max_num_iterations=20, linear_solver=0, huber_a=4.0))
+    ref = golden["grp_sparse_iterations"]
+    assert s.IsSolutionUsable()
+    assert (s.num_successful_steps, s.num_unsuccessful_steps, s.termination_type) == (
+        int(golden["grp_sparse_successful"]), int(golden["grp_sparse_unsuccessful"]),
+        int(golden["grp_sparse_termination"]))
+    np.testing.assert_allclose([it.cost for it in s.iterations], ref[:, 1], rtol=1e-9)
+    np.testing.assert_allclose(s.final_cost, float(golden["grp_sparse_final_cost"]), rtol=1e-10)
+    np.testing.assert_allclose(p.cameras, golden["grp_sparse_cameras"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(p.intrinsics, golden["grp_sparse_intrinsics"], rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(p.points, golden["grp_sparse_points"], rtol=0, atol=1e-5)
+    # Mixed: FP32 S with FP64 refinement reaches the same final cost.
+    sm = mba.solve(grouped_problem(mba, d, cc), mba.default_options(max_num_iterations=20, linear_solver=0,
+                                                                    huber_a=4.0, precision=mba.PRECISION_MIXED))
+    ref_final = float(golden["grp_sparse_final_cost"])
+    assert abs(sm.final_cost - ref_final) / ref_final < 1e-6
+
+
 def test_grouped_fp64_pcg_huber_constant_camera(mba, golden, group_scene):
     """Huber (4 px) with 5% outliers and a constant camera: long CG solves
     amplify summation-order rounding, so (as for the oracle against Ceres)
@@ -115,6 +138,43 @@ def test_grouped_fp64_pcg_huber_constant_camera(mba, golden, group_scene):
     assert [it.linear_solver_iterations for it in its[:5]] == [int(v) for v in ref[:5, 6]]
     np.testing.assert_allclose([it.cost for it in its], ref[:, 1], rtol=1e-6)
     np.testing.assert_allclose(s.final_cost, float(golden["grp_iterative_const_final_cost"]), rtol=1e-7)
+
+
+@pytest.mark.parametrize("case,solver", [("subset_sparse", 0), ("subset_iterative", 1),
+                                         ("const_group_sparse", 0)])
+def test_grouped_subset_and_constant_group_match_reference(mba, golden, group_scene, case, solver):
+    """The incremental engine's intrinsic options against the reference's
+    Ceres: SubsetParameterization masks (0x3E focal-only, its default; 0x38
+    k1..k3 held) and a constant group, Huber 4 px, a constant camera.  Held
+    entries come back bit for bit.  SPARSE_SCHUR trajectories to 1e-9; the
+    ITERATIVE_SCHUR one with the bars of the other long-CG case."""
+    import make_golden
+    d, cc = group_scene
+    sub = ic = None
+    if case == "subset_sparse":
+        sub = make_golden.GROUP_SUBSET
+    elif case == "subset_iterative":
+        sub = np.full(3, 0x3E, np.uint8)
+    else:
+        ic = make_golden.GROUP_INTR_CONST
+    p = grouped_problem(mba, d, cc, ic, sub)
+    s = mba.solve(p, mba.default_options(max_num_iterations=20, linear_solver=solver, huber_a=4.0))
+    ref = golden[f"grp_{case}_iterations"]
+    its = s.iterations
+    assert s.IsSolutionUsable() and len(its) == ref.shape[0]
+    if solver == 0:
+        np.testing.assert_allclose([it.cost for it in its], ref[:, 1], rtol=1e-9)
+        np.testing.assert_allclose(s.final_cost, float(golden[f"grp_{case}_final_cost"]), rtol=1e-10)
+        np.testing.assert_allclose(p.intrinsics, golden[f"grp_{case}_intrinsics"], rtol=1e-8, atol=1e-8)
+    else:
+        np.testing.assert_allclose([it.cost for it in its[:4]], ref[:4, 1], rtol=1e-9)
+        assert [it.linear_solver_iterations for it in its[:4]] == [int(v) for v in ref[:4, 6]]
+        np.testing.assert_allclose([it.cost for it in its], ref[:, 1], rtol=1e-6)
+        np.testing.assert_allclose(s.final_cost, float(golden[f"grp_{case}_final_cost"]), rtol=1e-7)
+    for g in range(3):
+        held = [k for k in range(6) if (sub is not None and (int(sub[g]) >> k) & 1) or (ic is not None and ic[g])]
+        np.testing.assert_array_equal(p.intrinsics[g, held], d["intr_init"][g, held])
+        np.testing.assert_array_equal(golden[f"grp_{case}_intrinsics"][g, held], d["intr_init"][g, held])
 
 
 def test_grouped_constant_blocks_stay_put(mba, group_scene):
