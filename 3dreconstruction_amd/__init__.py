@@ -439,10 +439,35 @@ class HuberLoss:
 
 class ReprojectionError:
     """The libs/sfm reprojection functor for one observation
-    (pinhole_brown_rt_ceres_functor.h:42-110), packed 9-parameter camera."""
+    (pinhole_brown_rt_ceres_functor.h:42-110), packed 9-parameter camera:
+    AddResidualBlock(cost, loss, camera[9], point[3])."""
 
     def __init__(self, observed_xy):
         self.observed = np.asarray(observed_xy, np.float64).reshape(2)
+
+
+class ErrorFunc_Refine_Camera_3DPoints(ReprojectionError):  # noqa: N801 (reference name)
+    """The incremental engine's functor (pinhole_brown_rt_ceres_functor.h:42-110):
+    AddResidualBlock(cost, loss, intrinsics[6], camera[6], point[3]) with
+    intrinsics [f, ppx, ppy, k1, k2, k3] and camera [aa, t]."""
+
+
+class SubsetParameterization:
+    """ceres::SubsetParameterization(size, constant_parameters)
+    (include/ceres/local_parameterization.h:173; checks of
+    internal/ceres/local_parameterization.cc:88-113)."""
+
+    def __init__(self, size, constant_parameters):
+        held = sorted(int(k) for k in constant_parameters)
+        if size != 6:
+            raise ValueError("SubsetParameterization applies to 6-entry intrinsic blocks")
+        if not held:
+            raise ValueError("the set of constant parameters should contain at least one element")
+        if len(set(held)) != len(held):
+            raise ValueError("the set of constant parameters cannot contain duplicates")
+        if len(held) >= size or held[0] < 0 or held[-1] >= size:
+            raise ValueError("hold between 1 and size - 1 entries in [0, size)")
+        self.mask = sum(1 << k for k in held)
 
 
 class SolverOptions:
@@ -474,15 +499,20 @@ class SolverOptions:
 
 
 class Problem:
-    """ceres::Problem for bundle adjustment: camera blocks of 9 and point
-    blocks of 3 (numpy float64 arrays, updated in place by Solve)."""
+    """ceres::Problem for bundle adjustment, in either layout (numpy float64
+    arrays as parameter blocks, updated in place by Solve):
+      AddResidualBlock(ReprojectionError, loss, camera[9], point[3]), or
+      AddResidualBlock(ErrorFunc_Refine_Camera_3DPoints, loss, intrinsics[6],
+                       camera[6], point[3]) (the incremental engine's call)."""
 
     def __init__(self):
-        self._cams, self._pts = [], []
-        self._cam_id, self._pt_id = {}, {}
-        self._obs, self._ci, self._pi = [], [], []
+        self._cams, self._pts, self._intr = [], [], []
+        self._cam_id, self._pt_id, self._intr_id = {}, {}, {}
+        self._obs, self._ci, self._pi, self._ii = [], [], [], []
         self._loss = None
-        self._constant = set()
+        self._constant, self._intr_constant = set(), set()
+        self._subset = {}
+        self._layout = None
 
     def _block(self, arr, size, table, ids):
         if not isinstance(arr, np.ndarray) or arr.dtype != np.float64 or arr.size != size:
@@ -493,35 +523,69 @@ class Problem:
             table.append(arr)
         return ids[key]
 
-    def AddResidualBlock(self, cost_function: ReprojectionError, loss_function, camera, point):
+    def AddResidualBlock(self, cost_function: ReprojectionError, loss_function, *blocks):
+        grouped = len(blocks) == 3
+        if len(blocks) not in (2, 3) or grouped != isinstance(cost_function, ErrorFunc_Refine_Camera_3DPoints):
+            raise ValueError("(camera[9], point[3]) with ReprojectionError, or (intrinsics[6], camera[6], "
+                             "point[3]) with ErrorFunc_Refine_Camera_3DPoints")
+        layout = LAYOUT_GROUPED if grouped else LAYOUT_PACKED9
+        if self._layout is not None and layout != self._layout:
+            raise ValueError("all residual blocks of a problem use one camera layout")
+        self._layout = layout
         if loss_function is not None:
             if self._loss is not None and loss_function is not self._loss and loss_function.a != self._loss.a:
                 raise ValueError("all residual blocks must share one loss function")
             self._loss = loss_function
-        c = self._block(camera, 9, self._cams, self._cam_id)
-        p = self._block(point, 3, self._pts, self._pt_id)
+        if grouped:
+            self._ii.append(self._block(blocks[0], 6, self._intr, self._intr_id))
+            blocks = blocks[1:]
+        c = self._block(blocks[0], 6 if grouped else 9, self._cams, self._cam_id)
+        p = self._block(blocks[1], 3, self._pts, self._pt_id)
         self._obs.append(cost_function.observed)
         self._ci.append(c)
         self._pi.append(p)
 
-    def SetParameterBlockConstant(self, camera):
-        key = camera.__array_interface__["data"][0]
-        if key not in self._cam_id:
-            raise ValueError("unknown camera block")
-        self._constant.add(self._cam_id[key])
+    def _key(self, block):
+        return block.__array_interface__["data"][0]
+
+    def SetParameterBlockConstant(self, block):
+        key = self._key(block)
+        if key in self._cam_id:
+            self._constant.add(self._cam_id[key])
+        elif key in self._intr_id:
+            self._intr_constant.add(self._intr_id[key])
+        else:
+            raise ValueError("unknown camera or intrinsics block")
+
+    def SetParameterization(self, block, parameterization: SubsetParameterization):
+        key = self._key(block)
+        if key not in self._intr_id:
+            raise ValueError("parameterizations apply to intrinsics blocks")
+        self._subset[self._intr_id[key]] = parameterization.mask
 
     def NumResidualBlocks(self):
         return len(self._obs)
 
     def _arrays(self) -> BAProblem:
-        cams = np.array([c.reshape(9) for c in self._cams], np.float64).reshape(-1, 9)
+        w = 6 if self._layout == LAYOUT_GROUPED else 9
+        cams = np.array([c.reshape(w) for c in self._cams], np.float64).reshape(-1, w)
         pts = np.array([p.reshape(3) for p in self._pts], np.float64).reshape(-1, 3)
         const = None
         if self._constant:
             const = np.zeros(len(self._cams), np.uint8)
             const[list(self._constant)] = 1
+        kw = {}
+        if self._layout == LAYOUT_GROUPED:
+            ni = len(self._intr)
+            ic = np.zeros(ni, np.uint8)
+            ic[list(self._intr_constant)] = 1
+            sub = np.zeros(ni, np.uint8)
+            for g, m in self._subset.items():
+                sub[g] = m
+            kw = dict(intrinsics=np.array([b.reshape(6) for b in self._intr], np.float64).reshape(-1, 6),
+                      intrinsic_index=np.array(self._ii, np.int32), intrinsic_constant=ic, intrinsic_subset=sub)
         return BAProblem(cams, pts, np.array(self._obs, np.float64).reshape(-1, 2),
-                         np.array(self._ci, np.int32), np.array(self._pi, np.int32), const)
+                         np.array(self._ci, np.int32), np.array(self._pi, np.int32), const, **kw)
 
 
 def Solve(options: SolverOptions, problem: Problem, summary_out: list | None = None) -> Summary:
@@ -530,10 +594,13 @@ def Solve(options: SolverOptions, problem: Problem, summary_out: list | None = N
     huber = problem._loss.a if problem._loss is not None else 0.0
     summary = solve(arrays, options._to_c(huber))
     if summary.IsSolutionUsable():
+        w = arrays.cameras.shape[1]
         for i, c in enumerate(problem._cams):
-            c.reshape(9)[:] = arrays.cameras[i]
+            c.reshape(w)[:] = arrays.cameras[i]
         for i, p in enumerate(problem._pts):
             p.reshape(3)[:] = arrays.points[i]
+        for i, b in enumerate(problem._intr):
+            b.reshape(6)[:] = arrays.intrinsics[i]
     if summary_out is not None:
         summary_out.append(summary)
     return summary

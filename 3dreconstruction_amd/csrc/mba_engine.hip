@@ -922,6 +922,11 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   c->nf = c->f_off[W];
   c->nrows = (int32_t)nrows;
   c->grouped = probs[0].layout == MBA_LAYOUT_GROUPED;
+  // An explicit S assembled in FP32 loses the weakly determined directions
+  // of a grouped problem (shared intrinsics with nearly collinear k1..k3
+  // columns) to cancellation in U - W Vinv W', which FP64 refinement against
+  // the same S cannot restore: grouped explicit solves run in FP64.
+  if (c->grouped && c->opt.linear_solver == MBA_EXPLICIT_SCHUR) c->mixed = false;
   c->ng = c->grouped ? c->wst[0].ng : 0;
   c->nslot = c->nc + c->ng;
   // Global E blocks in window order, each window in its reference E order.

@@ -87,7 +87,9 @@ typedef enum mba_linear_solver {
 } mba_linear_solver;
 
 /* Arithmetic of the residual / Jacobian / Schur / PCG kernels.  The LM cost
- * and gain ratio are always FP64; point-block inverses are always FP64. */
+ * and gain ratio are always FP64; point-block inverses are always FP64.
+ * GROUPED problems with MBA_EXPLICIT_SCHUR always run in FP64: an FP32
+ * reduced matrix loses their weakly determined intrinsic directions. */
 typedef enum mba_precision {
   MBA_PRECISION_FP64 = 0,
   MBA_PRECISION_MIXED = 1 /* FP32 residuals, Jacobians, matvecs, S */

@@ -186,3 +186,30 @@ def test_ceres_shaped_problem_builds_the_same_arrays(mba, c1):
     assert prob.NumResidualBlocks() == 200
     assert arr.camera_constant is not None and arr.camera_constant.sum() == 1
     np.testing.assert_array_equal(arr.observations, c1["obs"][:200])
+
+
+def test_ceres_shaped_grouped_problem_builds_the_same_arrays(mba, group_scene):
+    """The incremental engine's calls (sfm_incremental_engine.cpp:1396-1446):
+    AddResidualBlock(intrinsics, extrinsics, point), SetParameterization with
+    a SubsetParameterization, SetParameterBlockConstant on a camera."""
+    d, _ = group_scene
+    intr = [d["intr_init"][g].copy() for g in range(3)]
+    cams = [d["cam_init"][c].copy() for c in range(d["cam_init"].shape[0])]
+    pts = [d["pt_init"][q].copy() for q in range(d["pt_init"].shape[0])]
+    prob = mba.Problem()
+    focal_only = mba.SubsetParameterization(6, [1, 2, 3, 4, 5])
+    for i in range(300):
+        g = d["intr_idx"][i]
+        prob.AddResidualBlock(mba.ErrorFunc_Refine_Camera_3DPoints(d["obs"][i]), mba.HuberLoss(4.0), intr[g],
+                              cams[d["cam_idx"][i]], pts[d["pt_idx"][i]])
+        prob.SetParameterization(intr[g], focal_only)
+    prob.SetParameterBlockConstant(cams[d["cam_idx"][0]])
+    arr = prob._arrays()
+    assert arr.layout == mba.LAYOUT_GROUPED and arr.cameras.shape[1] == 6
+    assert set(arr.intrinsic_subset.tolist()) == {0x3E} and arr.intrinsic_constant.sum() == 0
+    assert arr.camera_constant.sum() == 1
+    np.testing.assert_array_equal(arr.intrinsics[arr.intrinsic_index], d["intr_init"][d["intr_idx"][:300]])
+    with pytest.raises(ValueError):
+        mba.SubsetParameterization(6, [0, 1, 2, 3, 4, 5])
+    with pytest.raises(ValueError):
+        prob.AddResidualBlock(mba.ReprojectionError(d["obs"][0]), None, np.zeros(9), pts[0])
