@@ -106,7 +106,7 @@ EXPORTS = [
     "mba_context_read_schur", "mba_context_read_scale", "mba_context_schur_matvec",
     "mba_context_solve", "mba_context_download", "mba_context_synchronize", "mba_context_stream",
     "mba_context_set_timing", "mba_context_last_timing", "mba_status_string", "mba_last_error",
-    "mba_abi_version", "mba_scene_generate",
+    "mba_abi_version", "mba_scene_generate", "mba_scene_generate_grouped",
 ]
 
 
@@ -144,6 +144,8 @@ def load_library():
                                         _f64p, C.c_int32]
     lib.mba_scene_generate.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _i64p, _f64p, _f64p, _f64p,
                                        _f64p, _f64p, _i32p, _i32p]
+    lib.mba_scene_generate_grouped.argtypes = [C.c_int32, _i32p, _i32p, _i32p, _i64p, _f64p, _f64p, _f64p, _f64p,
+                                               _i32p, _i32p, _i32p]
     _lib = lib
     return lib
 
@@ -426,6 +428,22 @@ def generate_scene(preset: int, window: int = 0) -> dict:
     _check(lib.mba_scene_generate(preset, window, C.byref(nc), C.byref(np_), C.byref(no),
                                   _p(d["cam_true"]), _p(d["cam_init"]), _p(d["pt_true"]), _p(d["pt_init"]),
                                   _p(d["obs"]), _p(d["cam_idx"], _i32p), _p(d["pt_idx"], _i32p)))
+    return d
+
+
+def generate_grouped_scene(preset: int = 6) -> dict:
+    """Seeded grouped-intrinsics scene (incremental-engine scale)."""
+    lib = load_library()
+    nc, np_, ni, no = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64()
+    _check(lib.mba_scene_generate_grouped(preset, C.byref(nc), C.byref(np_), C.byref(ni), C.byref(no),
+                                          None, None, None, None, None, None, None))
+    d = dict(cam_init=np.zeros((nc.value, 6)), intr_init=np.zeros((ni.value, 6)), pt_init=np.zeros((np_.value, 3)),
+             obs=np.zeros((no.value, 2)), cam_idx=np.zeros(no.value, np.int32),
+             intr_idx=np.zeros(no.value, np.int32), pt_idx=np.zeros(no.value, np.int32))
+    _check(lib.mba_scene_generate_grouped(preset, C.byref(nc), C.byref(np_), C.byref(ni), C.byref(no),
+                                          _p(d["cam_init"]), _p(d["intr_init"]), _p(d["pt_init"]), _p(d["obs"]),
+                                          _p(d["cam_idx"], _i32p), _p(d["intr_idx"], _i32p),
+                                          _p(d["pt_idx"], _i32p)))
     return d
 
 

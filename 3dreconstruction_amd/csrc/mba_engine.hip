@@ -2442,4 +2442,24 @@ mba_status mba_scene_generate(int32_t preset, int32_t window, int32_t* num_camer
   return MBA_OK;
 }
 
+mba_status mba_scene_generate_grouped(int32_t preset, int32_t* num_cameras, int32_t* num_points,
+                                      int32_t* num_intrinsics, int64_t* num_observations, double* cam_init,
+                                      double* intr_init, double* pt_init, double* observations,
+                                      int32_t* camera_index, int32_t* intrinsic_index, int32_t* point_index) {
+  if (preset != 6) return fail(MBA_ERR_INVALID_ARGUMENT, "the grouped scene preset is 6");
+  const mba_scene_spec sp = mba_scene_preset(preset, 0);
+  std::vector<int32_t> tl(sp.num_points);
+  const int64_t n = mba_scene_count(&sp, tl.data());
+  if (num_cameras) *num_cameras = sp.num_cameras;
+  if (num_points) *num_points = sp.num_points;
+  if (num_intrinsics) *num_intrinsics = MBA_SCENE_GROUPS;
+  if (num_observations) *num_observations = n;
+  if (!cam_init || !intr_init || !pt_init || !observations || !camera_index || !intrinsic_index || !point_index)
+    return MBA_OK;
+  std::vector<int64_t> off((size_t)sp.num_points + 1);
+  mba_scene_fill_groups(&sp, MBA_SCENE_GROUPS, tl.data(), nullptr, cam_init, nullptr, intr_init, nullptr, pt_init,
+                        observations, camera_index, intrinsic_index, point_index, off.data());
+  return MBA_OK;
+}
+
 }  // extern "C"

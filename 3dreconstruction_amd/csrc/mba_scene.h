@@ -340,8 +340,10 @@ static inline int64_t mba_scene_fill_groups(const mba_scene_spec* sp, int32_t gr
   return n;
 }
 
-/* The five benchmark scenes (index 1..5; 4 is one window of the batched
- * set, selected by `window`). */
+/* The benchmark scenes (index 1..6; 4 is one window of the batched set,
+ * selected by `window`; 6 is a grouped-intrinsics scene with
+ * MBA_SCENE_GROUPS groups, built by mba_scene_fill_groups). */
+#define MBA_SCENE_GROUPS 2
 static inline mba_scene_spec mba_scene_preset(int config, int window) {
   mba_scene_spec s;
   memset(&s, 0, sizeof(s));
@@ -376,6 +378,13 @@ static inline mba_scene_spec mba_scene_preset(int config, int window) {
       s.num_cameras = 10000; s.num_points = 4000000;
       s.track_kind = MBA_TRACK_POWER_LAW; s.track_min = 2; s.track_max = 100;
       s.alpha = 2.0482;
+      break;
+    case 6:
+      /* Incremental-engine scale, grouped intrinsics (MBA_SCENE_GROUPS
+       * groups): a few hundred images of one or two camera models. */
+      s.num_cameras = 300; s.num_points = 100000;
+      s.track_kind = MBA_TRACK_UNIFORM; s.track_min = 2; s.track_max = 10;
+      s.outlier_frac = 0.05; s.outlier_px = 20.0;
       break;
     default:
       break;

@@ -265,6 +265,61 @@ def run(args):
         log(f"  pass {ms_l:.3f} ms ({nol / (ms_l * 1e-3):.3e} obs/s); LM {s.total_time_s:.2f} s")
         del dl
 
+    grouped = None
+    if not args.no_grouped:
+        log("bench: grouped-intrinsics scene (the incremental engine's residual blocks)")
+        dg = mba.generate_grouped_scene(6)
+        ngo = dg["obs"].shape[0]
+        gconst = np.zeros(dg["cam_init"].shape[0], np.uint8)
+        gconst[0] = 1  # the incremental engine holds its first camera
+
+        def gproblem(subset):
+            return mba.BAProblem(dg["cam_init"].copy(), dg["pt_init"].copy(), dg["obs"], dg["cam_idx"],
+                                 dg["pt_idx"], gconst, intrinsics=dg["intr_init"].copy(),
+                                 intrinsic_index=dg["intr_idx"],
+                                 intrinsic_subset=np.full(dg["intr_init"].shape[0], subset, np.uint8))
+        gpass = {}
+        for label, prec in (("mixed", mba.PRECISION_MIXED), ("fp64", mba.PRECISION_FP64)):
+            ctx = mba.Context(gproblem(0), mba.default_options(precision=prec, huber_a=HUBER_A,
+                                                               linear_solver=mba.IMPLICIT_SCHUR_PCG))
+            stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
+            for _ in range(args.warmup):
+                ctx.jacobian_schur(RADIUS)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.steps):
+                ctx.jacobian_schur(RADIUS)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms_g = e0.elapsed_time(e1) / args.steps
+            ctx.close()
+            gpass[label] = {"ms_per_step": ms_g, "obs_per_s": ngo / (ms_g * 1e-3)}
+        gsolves = {}
+        for label, ls, prec, subset in (("sparse_schur_role_fp64_focal_only", mba.EXPLICIT_SCHUR, mba.PRECISION_FP64,
+                                         0x3E),
+                                        ("sparse_schur_role_fp64_all_intrinsics", mba.EXPLICIT_SCHUR,
+                                         mba.PRECISION_FP64, 0),
+                                        ("iterative_schur_mixed_all_intrinsics", mba.IMPLICIT_SCHUR_PCG,
+                                         mba.PRECISION_MIXED, 0)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s = mba.solve(gproblem(subset), mba.default_options(linear_solver=ls, precision=prec, huber_a=HUBER_A))
+            wall = time.perf_counter() - t0
+            gsolves[label] = {"iterations": s.num_iterations - 1, "successful": s.num_successful_steps,
+                              "termination": s.termination_type, "total_s": s.total_time_s,
+                              "preprocess_s": s.preprocessor_time_s, "linear_solver_s": s.linear_solver_time_s,
+                              "wall_s": wall, "initial_rms_px": float(np.sqrt(2 * s.initial_cost / ngo)),
+                              "final_rms_px": float(np.sqrt(2 * s.final_cost / ngo))}
+            log(f"  {label}: {s.num_iterations - 1} iterations in {s.total_time_s:.3f} s, RMS "
+                f"{gsolves[label]['initial_rms_px']:.3f} -> {gsolves[label]['final_rms_px']:.4f} px")
+        grouped = {"cameras": dg["cam_init"].shape[0], "intrinsic_groups": dg["intr_init"].shape[0],
+                   "points": dg["pt_init"].shape[0], "observations": ngo,
+                   "options": "Huber 4 px, first camera constant, Ceres defaults (50 iterations)",
+                   "pass": gpass, "lm": gsolves}
+        log(f"  pass mixed {gpass['mixed']['ms_per_step']:.3f} ms, fp64 {gpass['fp64']['ms_per_step']:.3f} ms")
+        del dg
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -320,6 +375,7 @@ def run(args):
         "explicit_schur": explicit,
         "batched_windows": batched,
         "large_scene": large,
+        "grouped_intrinsics": grouped,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -362,6 +418,7 @@ def main():
     ap.add_argument("--no-explicit", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--explicit-iters", type=int, default=5)
     ap.add_argument("--pmc-target", nargs="?", const="pass", choices=["pass", "explicit"],
                     help="target run for rocprofv3 --pmc: a 256 MiB device copy (known bytes, the "

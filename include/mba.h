@@ -302,13 +302,23 @@ int32_t mba_abi_version(void);
 /* Seeded synthetic scene (no counterpart in the reference; the reference
  * test builds scenes with NRealisticCamerasRing,
  * libs/multiview/src/nview_data_sets.cpp:23-64).  preset 1..5 are the
- * benchmark scenes (4 = window `window` of the batched set).  Call with
+ * packed-camera benchmark scenes (4 = window `window` of the batched set).  Call with
  * NULL arrays to get the sizes, then with arrays of those sizes. */
 mba_status mba_scene_generate(int32_t preset, int32_t window, int32_t* num_cameras,
                               int32_t* num_points, int64_t* num_observations,
                               double* cam_true, double* cam_init, double* pt_true,
                               double* pt_init, double* observations,
                               int32_t* camera_index, int32_t* point_index);
+
+/* Seeded grouped-intrinsics scene (preset 6: 300 cameras in 2 intrinsic
+ * groups, 100,000 points, tracks of 2..10, 5% outliers of 20 px; no
+ * counterpart in the reference): cameras 6 per camera [aa, t], intrinsics
+ * 6 per group.  Call with NULL arrays to get the sizes. */
+mba_status mba_scene_generate_grouped(int32_t preset, int32_t* num_cameras, int32_t* num_points,
+                                      int32_t* num_intrinsics, int64_t* num_observations,
+                                      double* cam_init, double* intr_init, double* pt_init,
+                                      double* observations, int32_t* camera_index,
+                                      int32_t* intrinsic_index, int32_t* point_index);
 
 #ifdef __cplusplus
 }

@@ -124,6 +124,18 @@ def test_scene_generator_matches_oracle_copy(mba, O):
             np.testing.assert_array_equal(a[k], b[k])
 
 
+def test_grouped_scene_generator_matches_oracle_copy(mba, O):
+    """Same seeded scene from the library and the oracle's copy of the
+    generator; projections may differ in the last bit between the two
+    compilers' floating-point contraction, so observations are held to
+    1e-12 px."""
+    a = mba.generate_grouped_scene(6)
+    b = O.generate_groups(O.preset(6), 2)
+    for k in ["cam_init", "intr_init", "pt_init", "cam_idx", "intr_idx", "pt_idx"]:
+        np.testing.assert_array_equal(a[k], b[k])
+    np.testing.assert_allclose(a["obs"], b["obs"], rtol=0, atol=1e-12)
+
+
 def test_scene_statistics_follow_the_configs(O):
     """Observation counts of the seeded configs (~8k, ~700k, ~4M)."""
     L = O.lib()
