@@ -1676,24 +1676,53 @@ __global__ __launch_bounds__(kBlock) void k_s_rows(
   }
 }
 
-// One 64-lane workgroup per block (fa, fb), fa >= fb, that some point
-// couples: the block's row pairs (a, b) are staged 64 at a time in LDS and
-// lane l sums entries l and l + 64 of the 9 x 9 product; the block is then
-// written once (no atomics, fixed summation order) together with its
-// mirror above the diagonal, so S is stored whole.  Diagonal blocks add
-// U + D^2.
+// Entry (i, j) of block (fa, fb) of S from the accumulated sum of its row
+// pairs: written with its mirror above the diagonal, so S is stored whole.
+// Diagonal blocks add U + D^2.  Grouped layout: a camera / group block pair
+// also carries the F'F coupling of their common rows, cross[36 c + 6 a + b]
+// (camera slot a, group slot b); blk_cross = 2 c (+1 if the group block is
+// fa), or -1.
+template <typename T>
+__device__ __forceinline__ void s_put(int blk, int fa, int fb, int i, int j, T acc, const T* __restrict__ U,
+                                      const double* __restrict__ diag_c, double inv_radius, int64_t n,
+                                      T* __restrict__ S, const int32_t* __restrict__ blk_cross,
+                                      const T* __restrict__ cross) {
+  T base = T(0);
+  if (fa == fb) {
+    base = U[45 * (size_t)fa + (i <= j ? ut9(i, j) : ut9(j, i))];
+    if (i == j) base += (T)(diag_c[9 * (size_t)fa + i] * inv_radius);
+  } else if (blk_cross && i < 6 && j < 6) {
+    const int xc = blk_cross[blk];
+    if (xc >= 0) {
+      const T* x = cross + 36 * (size_t)(xc >> 1);
+      base = (xc & 1) ? x[6 * j + i] : x[6 * i + j];
+    }
+  }
+  const int64_t row = 9 * (int64_t)fa + i, col = 9 * (int64_t)fb + j;
+  S[row + col * n] = base - acc;
+  if (fa != fb) S[col + row * n] = base - acc;
+}
+
+// One 64-lane workgroup per work item: a block (fa, fb), fa >= fb, that
+// some point couples, or a slice of at most kSplitPairs of its row pairs.
+// The pairs (a, b) are staged 64 at a time in LDS and lane l sums entries l
+// and l + 64 of the 9 x 9 product.  A whole block is written at once (no
+// atomics, fixed summation order); a slice writes its 81 sums to scratch
+// slot item_slot, and k_s_split_final adds a split block's slots in order.
+constexpr int64_t kSplitPairs = 4096;
 template <typename T>
 __global__ __launch_bounds__(64) void k_s_blocks(
-    const int64_t* __restrict__ blk_start, const int32_t* __restrict__ blk_fa,
-    const int32_t* __restrict__ blk_fb, const int32_t* __restrict__ pair_a,
-    const int32_t* __restrict__ pair_b, const T* __restrict__ WY, const T* __restrict__ U,
-    const double* __restrict__ diag_c, double inv_radius, int64_t n, T* __restrict__ S,
-    const int32_t* __restrict__ blk_cross, const T* __restrict__ cross) {
+    const int32_t* __restrict__ item_blk, const int64_t* __restrict__ item_p0, const int64_t* __restrict__ item_p1,
+    const int32_t* __restrict__ item_slot, const int32_t* __restrict__ blk_fa, const int32_t* __restrict__ blk_fb,
+    const int32_t* __restrict__ pair_a, const int32_t* __restrict__ pair_b, const T* __restrict__ WY,
+    const T* __restrict__ U, const double* __restrict__ diag_c, double inv_radius, int64_t n, T* __restrict__ S,
+    const int32_t* __restrict__ blk_cross, const T* __restrict__ cross, T* __restrict__ scratch) {
   __shared__ T Ys[64][kWYY];
   __shared__ T Ws[64][kWYY];
-  const int blk = blockIdx.x;
+  const int item = blockIdx.x;
   const int lane = threadIdx.x;
-  const int64_t p0 = blk_start[blk], p1 = blk_start[blk + 1];
+  const int blk = item_blk[item];
+  const int64_t p0 = item_p0[item], p1 = item_p1[item];
   const int fa = blk_fa[blk], fb = blk_fb[blk];
   const int i0 = lane / 9, j0 = lane % 9;
   const int e1 = lane + 64, i1 = e1 / 9, j1 = e1 % 9;
@@ -1720,25 +1749,35 @@ __global__ __launch_bounds__(64) void k_s_blocks(
     }
     __syncthreads();
   }
-  // Grouped layout: a camera / group block pair also carries the F'F
-  // coupling of their common rows, cross[36 c + 6 a + b] (camera slot a,
-  // group slot b); blk_cross = 2 c + (1 if the group block is fa).
-  const int xc = blk_cross ? blk_cross[blk] : -1;
-  auto put = [&](int i, int j, T acc) {
-    T base = T(0);
-    if (fa == fb) {
-      base = U[45 * (size_t)fa + (i <= j ? ut9(i, j) : ut9(j, i))];
-      if (i == j) base += (T)(diag_c[9 * (size_t)fa + i] * inv_radius);
-    } else if (xc >= 0 && i < 6 && j < 6) {
-      const T* x = cross + 36 * (size_t)(xc >> 1);
-      base = (xc & 1) ? x[6 * j + i] : x[6 * i + j];
-    }
-    const int64_t row = 9 * (int64_t)fa + i, col = 9 * (int64_t)fb + j;
-    S[row + col * n] = base - acc;
-    if (fa != fb) S[col + row * n] = base - acc;
-  };
-  put(i0, j0, acc0);
-  if (two) put(i1, j1, acc1);
+  const int slot = item_slot[item];
+  if (slot >= 0) {
+    scratch[81 * (size_t)slot + lane] = acc0;
+    if (two) scratch[81 * (size_t)slot + e1] = acc1;
+    return;
+  }
+  s_put(blk, fa, fb, i0, j0, acc0, U, diag_c, inv_radius, n, S, blk_cross, cross);
+  if (two) s_put(blk, fa, fb, i1, j1, acc1, U, diag_c, inv_radius, n, S, blk_cross, cross);
+}
+
+// Split blocks: sum slots [split_lo, split_hi) in order, then write.
+template <typename T>
+__global__ __launch_bounds__(64) void k_s_split_final(
+    const int32_t* __restrict__ split_blk, const int32_t* __restrict__ split_lo,
+    const int32_t* __restrict__ split_hi, const int32_t* __restrict__ blk_fa, const int32_t* __restrict__ blk_fb,
+    const T* __restrict__ U, const double* __restrict__ diag_c, double inv_radius, int64_t n, T* __restrict__ S,
+    const int32_t* __restrict__ blk_cross, const T* __restrict__ cross, const T* __restrict__ scratch) {
+  const int k = blockIdx.x, lane = threadIdx.x;
+  const int blk = split_blk[k];
+  const int fa = blk_fa[blk], fb = blk_fb[blk];
+  const int e1 = lane + 64;
+  const bool two = e1 < 81;
+  T acc0 = T(0), acc1 = T(0);
+  for (int sl = split_lo[k]; sl < split_hi[k]; ++sl) {
+    acc0 += scratch[81 * (size_t)sl + lane];
+    if (two) acc1 += scratch[81 * (size_t)sl + e1];
+  }
+  s_put(blk, fa, fb, lane / 9, lane % 9, acc0, U, diag_c, inv_radius, n, S, blk_cross, cross);
+  if (two) s_put(blk, fa, fb, e1 / 9, e1 % 9, acc1, U, diag_c, inv_radius, n, S, blk_cross, cross);
 }
 
 // ------------------------------------------------- dense Cholesky (lower)

@@ -184,6 +184,33 @@ __global__ __launch_bounds__(kBlock) void k_cam_pass_g(
   }
 }
 
+// A group block's partial rows span all chunks of its cameras (hundreds to
+// thousands of rows): one 1024-thread workgroup per group block sums rows
+// [g_lo, g_hi) (first `width` values) in a fixed order into row out_row[k],
+// so the finalize kernels read one row per group.
+constexpr int kGroupReduceThreads = 1024;
+template <typename T>
+__global__ __launch_bounds__(kGroupReduceThreads) void k_group_rows(const int32_t* __restrict__ g_lo,
+                                                                   const int32_t* __restrict__ g_hi,
+                                                                   const int32_t* __restrict__ out_row, int width,
+                                                                   T* __restrict__ partial,
+                                                                   const int32_t* __restrict__ all_done) {
+  __shared__ T lds[kGroupReduceThreads];
+  if (all_done && *all_done) return;
+  const int k = blockIdx.x;
+  const int v = threadIdx.x & 63, stripe = threadIdx.x >> 6, nstripe = kGroupReduceThreads / 64;
+  T s = T(0);
+  if (v < width)
+    for (int r = g_lo[k] + stripe; r < g_hi[k]; r += nstripe) s += partial[(size_t)kPartial * r + v];
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  if (stripe == 0 && v < width) {
+    T t = T(0);
+    for (int w = 0; w < nstripe; ++w) t += lds[64 * w + v];
+    partial[(size_t)kPartial * out_row[k] + v] = t;
+  }
+}
+
 // Direction table of a grouped camera for an F-space vector x (or zeros):
 // M = sum_k xs_k dR_k, dt = xs_t from its camera block, and its group
 // block's [df, dppx, dppy, dk1, dk2, dk3]; xs = sign * cscale .* x.

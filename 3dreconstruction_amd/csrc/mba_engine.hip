@@ -498,6 +498,7 @@ struct Dev {
   T *step_p = nullptr, *step_c = nullptr;
   // explicit S
   T *S = nullptr, *L = nullptr, *sv = nullptr, *WY = nullptr, *Linv = nullptr, *cross = nullptr;
+  T* s_scratch = nullptr;
 };
 
 }  // namespace
@@ -522,6 +523,10 @@ struct mba_context {
   std::vector<int32_t> cam_grp, cam_gf;
   int32_t *d_cam_grp = nullptr, *d_cam_gf = nullptr;
   uint8_t* fmask = nullptr;  // 9 per F block: 1 for a free slot
+  // Group F blocks: their chunk partial rows [g_lo, g_hi) are summed into
+  // row g_out (after the 2 nchunks chunk rows) before any finalize.
+  int32_t ngf = 0;
+  int32_t *g_lo = nullptr, *g_hi = nullptr, *g_out = nullptr;
   int device = 0;
   hipStream_t stream = nullptr;
   Pool pool;
@@ -547,9 +552,14 @@ struct mba_context {
   // their observation-row pairs, sorted by block.
   // Grouped layout: rows are virtual rows (camera part r, group part
   // nrows + r); s_blk_cross marks camera / group block pairs.
-  int64_t* s_blk_start = nullptr;
   int32_t *s_blk_fa = nullptr, *s_blk_fb = nullptr, *s_pair_a = nullptr, *s_pair_b = nullptr;
   int32_t* s_blk_cross = nullptr;
+  // Work items of the block assembly (a block, or a slice of a block with
+  // more than kSplitPairs pairs whose sums go to scratch slots).
+  int32_t *s_item_blk = nullptr, *s_item_slot = nullptr;
+  int64_t *s_item_p0 = nullptr, *s_item_p1 = nullptr;
+  int32_t *s_split_blk = nullptr, *s_split_lo = nullptr, *s_split_hi = nullptr;
+  int64_t s_nitems = 0, s_nsplit = 0, s_nslots = 0;
   int64_t s_nblk = 0, s_npairs = 0;
   // Host copies of what was last uploaded (rewritten only while the stream
   // is idle).
@@ -633,7 +643,8 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_st
   MBA_TRY(c->pool.alloc(&d.rhs, 9 * nf));
   MBA_TRY(c->pool.alloc(&d.Minv, 81 * nf));
   MBA_TRY(c->pool.alloc(&d.dtab, (c->grouped ? kDirTabG : kDirTab) * nc));
-  MBA_TRY(c->pool.alloc(&d.partial, (size_t)kPartial * c->nchunks * (c->grouped ? 2 : 1)));
+  MBA_TRY(c->pool.alloc(&d.partial, (size_t)kPartial * (c->grouped ? 2 * (size_t)c->nchunks + c->ngf
+                                                                    : (size_t)c->nchunks)));
   MBA_TRY(c->pool.alloc(&d.x, 9 * nf));
   MBA_TRY(c->pool.alloc(&d.r, 9 * nf));
   MBA_TRY(c->pool.alloc(&d.zc, 9 * nf));
@@ -808,6 +819,40 @@ mba_status build_s_pairs(mba_context* c, const std::vector<int32_t>& pt_start,
     }
   blk_start.push_back(o);
   c->s_nblk = (int64_t)blk_fa.size();
+  {
+    std::vector<int32_t> iblk, islot, sblk, slo, shi;
+    std::vector<int64_t> ip0, ip1;
+    int32_t nslots = 0;
+    for (int64_t b = 0; b < c->s_nblk; ++b) {
+      const int64_t p0 = blk_start[b], p1 = blk_start[b + 1];
+      if (p1 - p0 <= kSplitPairs) {
+        iblk.push_back((int32_t)b);
+        ip0.push_back(p0);
+        ip1.push_back(p1);
+        islot.push_back(-1);
+        continue;
+      }
+      sblk.push_back((int32_t)b);
+      slo.push_back(nslots);
+      for (int64_t q = p0; q < p1; q += kSplitPairs) {
+        iblk.push_back((int32_t)b);
+        ip0.push_back(q);
+        ip1.push_back(std::min(p1, q + kSplitPairs));
+        islot.push_back(nslots++);
+      }
+      shi.push_back(nslots);
+    }
+    c->s_nitems = (int64_t)iblk.size();
+    c->s_nsplit = (int64_t)sblk.size();
+    c->s_nslots = nslots;
+    MBA_TRY(upload(c, &c->s_item_blk, iblk));
+    MBA_TRY(upload(c, &c->s_item_p0, ip0));
+    MBA_TRY(upload(c, &c->s_item_p1, ip1));
+    MBA_TRY(upload(c, &c->s_item_slot, islot));
+    MBA_TRY(upload(c, &c->s_split_blk, sblk));
+    MBA_TRY(upload(c, &c->s_split_lo, slo));
+    MBA_TRY(upload(c, &c->s_split_hi, shi));
+  }
   if (c->grouped) {
     // Camera / group pairs: 2 c (+1 when the group block is fa).
     std::vector<int32_t> cross(blk_fa.size(), -1);
@@ -818,7 +863,6 @@ mba_status build_s_pairs(mba_context* c, const std::vector<int32_t>& pt_start,
     }
     MBA_TRY(upload(c, &c->s_blk_cross, cross));
   }
-  MBA_TRY(upload(c, &c->s_blk_start, blk_start));
   MBA_TRY(upload(c, &c->s_blk_fa, blk_fa));
   MBA_TRY(upload(c, &c->s_blk_fb, blk_fb));
   MBA_TRY(upload(c, &c->s_pair_a, sa));
@@ -1110,6 +1154,19 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
         f_hi[fg] = c->nchunks + key_chunk[k + 1];
       }
     }
+    std::vector<int32_t> glo, ghi, gout;
+    for (int32_t f = 0; f < c->nf; ++f) {
+      if (c->f_to_cam[f] < c->nc) continue;
+      glo.push_back(f_lo[f]);
+      ghi.push_back(f_hi[f]);
+      gout.push_back(2 * c->nchunks + (int32_t)gout.size());
+      f_lo[f] = gout.back();
+      f_hi[f] = gout.back() + 1;
+    }
+    c->ngf = (int32_t)gout.size();
+    MBA_TRY(upload(c, &c->g_lo, glo));
+    MBA_TRY(upload(c, &c->g_hi, ghi));
+    MBA_TRY(upload(c, &c->g_out, gout));
   }
   // FP64 state: cameras (slot blocks), then intrinsic groups (grouped).
   std::vector<double> pts(3 * (size_t)c->ne), cams(9 * (size_t)c->nslot, 0.0);
@@ -1318,10 +1375,11 @@ struct Engine {
                          d.V, d.gp, d.Vinv, d.pk);
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
     if (c->nchunks > 0) {
-      if (c->grouped)
+      if (c->grouped) {
         hipLaunchKernelGGL(k_cam_pass_g<T>, dim3(c->nchunks), dim3(kBlock), 0, s, c->nchunks, d.chunk_f,
                            d.chunk_begin, d.chunk_end, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, d.partial);
-      else
+        group_rows(63, nullptr);
+      } else
         hipLaunchKernelGGL(k_cam_pass<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
                            d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial, win());
     }
@@ -1365,13 +1423,20 @@ struct Engine {
   }
   // Camera-major partials: packed, sum Jc'(Jp z); grouped, sum F'(F x - E z)
   // (dtab: the direction table of x, or NULL for x = 0).
+  // Grouped layout: sum each group block's chunk partials into its row.
+  void group_rows(int width, const int32_t* done) {
+    if (c->ngf > 0)
+      hipLaunchKernelGGL(k_group_rows<T>, dim3(c->ngf), dim3(kGroupReduceThreads), 0, s, (const int32_t*)c->g_lo,
+                         (const int32_t*)c->g_hi, (const int32_t*)c->g_out, width, d.partial, done);
+  }
   void mv_cam(const T* dtab, const int32_t* done, const Win* wv = nullptr) {
     if (c->nchunks == 0) return;
-    if (c->grouped)
+    if (c->grouped) {
       hipLaunchKernelGGL(k_mv_cam_g<T>, dim3(c->nchunks), dim3(kBlock), 0, s, c->nchunks, d.chunk_f, d.chunk_begin,
                          d.chunk_end, d.cam_to_f, (const int32_t*)c->d_cam_gf, d.cm_e, d.cm_obs, d.pk, d.cscale,
                          d.ctab, dtab, huber, d.partial, done);
-    else
+      group_rows(9, done);
+    } else
       hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
                          d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber, d.partial,
                          done, wv ? *wv : win());
@@ -1482,6 +1547,7 @@ struct Engine {
       MBA_TRY(c->pool.alloc(&d.L, (size_t)(n * n)));
       MBA_TRY(c->pool.alloc(&d.WY, (size_t)kWY * c->nrows * (c->grouped ? 2 : 1)));
       if (c->grouped) MBA_TRY(c->pool.alloc(&d.cross, 36 * (size_t)std::max(1, c->nc)));
+      MBA_TRY(c->pool.alloc(&d.s_scratch, 81 * (size_t)std::max<int64_t>(1, c->s_nslots)));
       MBA_TRY(c->pool.alloc(&d.Linv, (size_t)kNB * kNB));
       if (!c->gemv_part) MBA_TRY(c->pool.alloc(&c->gemv_part, (size_t)kGemvSplit * n));
     }
@@ -1505,10 +1571,18 @@ struct Engine {
       hipLaunchKernelGGL(k_s_rows<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
                          d.cam_to_f, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab, d.Vinv, huber, d.WY);
     }
-    if (c->s_nblk > 0)
-      hipLaunchKernelGGL(k_s_blocks<T>, dim3((unsigned)c->s_nblk), dim3(64), 0, s, c->s_blk_start, c->s_blk_fa,
-                         c->s_blk_fb, c->s_pair_a, c->s_pair_b, d.WY, d.U, c->diag_c, inv_r, n, d.S,
-                         (const int32_t*)c->s_blk_cross, (const T*)d.cross);
+    if (c->s_nitems > 0)
+      hipLaunchKernelGGL(k_s_blocks<T>, dim3((unsigned)c->s_nitems), dim3(64), 0, s, (const int32_t*)c->s_item_blk,
+                         (const int64_t*)c->s_item_p0, (const int64_t*)c->s_item_p1, (const int32_t*)c->s_item_slot,
+                         (const int32_t*)c->s_blk_fa, (const int32_t*)c->s_blk_fb, (const int32_t*)c->s_pair_a,
+                         (const int32_t*)c->s_pair_b, (const T*)d.WY, (const T*)d.U, (const double*)c->diag_c, inv_r,
+                         n, d.S, (const int32_t*)c->s_blk_cross, (const T*)d.cross, d.s_scratch);
+    if (c->s_nsplit > 0)
+      hipLaunchKernelGGL(k_s_split_final<T>, dim3((unsigned)c->s_nsplit), dim3(64), 0, s,
+                         (const int32_t*)c->s_split_blk, (const int32_t*)c->s_split_lo,
+                         (const int32_t*)c->s_split_hi, (const int32_t*)c->s_blk_fa, (const int32_t*)c->s_blk_fb,
+                         (const T*)d.U, (const double*)c->diag_c, inv_r, n, d.S, (const int32_t*)c->s_blk_cross,
+                         (const T*)d.cross, (const T*)d.s_scratch);
     if (c->timing) (void)hipEventRecord(c->ev[5], s);
     HIP_TRY(hipMemcpyAsync(d.L, d.S, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(c->info, 0, sizeof(int32_t), s));
