@@ -61,7 +61,8 @@ class _Problem(C.Structure):
                 ("num_intrinsics", C.c_int32), ("num_observations", C.c_int64), ("cameras", _f64p),
                 ("points", _f64p), ("observations", _f64p), ("camera_index", _i32p),
                 ("point_index", _i32p), ("camera_constant", _u8p), ("intrinsics", _f64p),
-                ("intrinsic_index", _i32p), ("intrinsic_constant", _u8p), ("intrinsic_subset", _u8p)]
+                ("intrinsic_index", _i32p), ("intrinsic_constant", _u8p), ("intrinsic_subset", _u8p),
+                ("camera_params", C.c_int32), ("reserved2", C.c_int32)]
 
 
 class Options(C.Structure):
@@ -178,7 +179,8 @@ class BAProblem:
     observation by `intrinsic_index`."""
 
     def __init__(self, cameras, points, observations, camera_index, point_index, camera_constant=None,
-                 intrinsics=None, intrinsic_index=None, intrinsic_constant=None, intrinsic_subset=None):
+                 intrinsics=None, intrinsic_index=None, intrinsic_constant=None, intrinsic_subset=None,
+                 camera_params=0):
         self.cameras = np.ascontiguousarray(cameras, np.float64)
         self.points = np.ascontiguousarray(points, np.float64)
         self.observations = np.ascontiguousarray(observations, np.float64)
@@ -194,6 +196,7 @@ class BAProblem:
                                    else np.ascontiguousarray(intrinsic_constant, np.uint8))
         self.intrinsic_subset = (None if intrinsic_subset is None
                                  else np.ascontiguousarray(intrinsic_subset, np.uint8))
+        self.camera_params = int(camera_params)
         width = 9 if self.layout == LAYOUT_PACKED9 else 6
         if self.cameras.ndim != 2 or self.cameras.shape[1] != width:
             raise ValueError(f"cameras must be (Nc, {width})")
@@ -235,6 +238,7 @@ class BAProblem:
         s.intrinsic_index = _p(self.intrinsic_index, _i32p)
         s.intrinsic_constant = _p(self.intrinsic_constant, _u8p)
         s.intrinsic_subset = _p(self.intrinsic_subset, _u8p)
+        s.camera_params = self.camera_params
         return s
 
 

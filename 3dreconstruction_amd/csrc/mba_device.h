@@ -1275,12 +1275,14 @@ __global__ __launch_bounds__(kBlock) void k_gather_cm(int ncm, const int32_t* __
 }
 
 // Scatter FP64 values at `idx` positions (gather = 0) or gather.
+// (xmask, if given: 0 for slots outside the parameter block -> 0).
 __global__ __launch_bounds__(kBlock) void k_gather_cams(int nf, const int32_t* __restrict__ f_to_cam,
                                                         const double* __restrict__ cams,
-                                                        double* __restrict__ out) {
+                                                        double* __restrict__ out,
+                                                        const uint8_t* __restrict__ xmask) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 9 * nf) return;
-  out[i] = cams[9 * (size_t)f_to_cam[i / 9] + i % 9];
+  out[i] = (xmask && !xmask[i]) ? 0.0 : cams[9 * (size_t)f_to_cam[i / 9] + i % 9];
 }
 __global__ __launch_bounds__(kBlock) void k_scatter_cams(int nf, const int32_t* __restrict__ f_to_cam,
                                                          const double* __restrict__ src,

@@ -88,6 +88,11 @@ mba_status validate(const mba_problem* p) {
     return fail(MBA_ERR_INVALID_ARGUMENT, "unknown camera layout " + std::to_string(p->layout));
   if (p->num_cameras < 0 || p->num_points < 0 || p->num_observations < 0)
     return fail(MBA_ERR_INVALID_ARGUMENT, "negative sizes");
+  {
+    const int32_t all = p->layout == MBA_LAYOUT_GROUPED ? 0x3F : 0x1FF;
+    if (p->camera_params < 0 || (p->camera_params & ~all) != 0)
+      return fail(MBA_ERR_INVALID_ARGUMENT, "camera_params has bits outside the camera vector");
+  }
   if (p->layout == MBA_LAYOUT_GROUPED) {
     if (p->num_intrinsics < 0) return fail(MBA_ERR_INVALID_ARGUMENT, "negative num_intrinsics");
     if (p->num_intrinsics > 0 && !p->intrinsics) return fail(MBA_ERR_INVALID_ARGUMENT, "intrinsics is NULL");
@@ -522,7 +527,10 @@ struct mba_context {
   int32_t ng = 0, nslot = 0;
   std::vector<int32_t> cam_grp, cam_gf;
   int32_t *d_cam_grp = nullptr, *d_cam_gf = nullptr;
-  uint8_t* fmask = nullptr;  // 9 per F block: 1 for a free slot
+  // 9 per F block (NULL when every slot is a free parameter): fmask = 1 for
+  // a slot that takes steps, xmask = 1 for a slot of the parameter block
+  // (counted in |x|; a SubsetParameterization entry is held but counted).
+  uint8_t *fmask = nullptr, *xmask = nullptr;
   // Group F blocks: their chunk partial rows [g_lo, g_hi) are summed into
   // row g_out (after the 2 nchunks chunk rows) before any finalize.
   int32_t ngf = 0;
@@ -1184,20 +1192,42 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       for (int k = 0; k < 6; ++k) cams[9 * ((size_t)c->nc + g) + k] = probs[0].intrinsics[6 * (size_t)g + k];
     MBA_TRY(upload(c, &c->d_cam_grp, c->cam_grp));
     MBA_TRY(upload(c, &c->d_cam_gf, c->cam_gf));
-    // Free slots: a camera's [aa, t]; a group's entries outside its
-    // SubsetParameterization constants.
-    std::vector<uint8_t> mask(9 * (size_t)c->nf, 0);
-    for (int32_t f = 0; f < c->nf; ++f) {
-      const int32_t b = c->f_to_cam[f];
-      const uint8_t sub = (b >= c->nc && probs[0].intrinsic_subset) ? probs[0].intrinsic_subset[b - c->nc] : 0;
-      for (int k = 0; k < 6; ++k) mask[9 * (size_t)f + k] = !((sub >> k) & 1);
-    }
-    MBA_TRY(upload(c, &c->fmask, mask));
   } else {
     for (int32_t w = 0; w < W; ++w)
       if (probs[w].num_cameras > 0)
         std::memcpy(cams.data() + 9 * (size_t)c->cam_off[w], probs[w].cameras,
                     72 * (size_t)probs[w].num_cameras);
+  }
+  {
+    // Slot masks: a camera's parameter entries (camera_params); a group's
+    // entries outside its SubsetParameterization constants (still counted
+    // in |x|); unused slots of the 9 neither.
+    std::vector<uint8_t> fm(9 * (size_t)c->nf, 0), xm(9 * (size_t)c->nf, 0);
+    bool trivial = true;
+    int32_t w = 0;
+    for (int32_t f = 0; f < c->nf; ++f) {
+      while (f >= c->f_off[w + 1]) ++w;
+      const mba_problem& pw = probs[w];
+      const int32_t b = c->f_to_cam[f];
+      int32_t free_bits, in_bits;
+      if (c->grouped && b >= c->nc) {
+        const int32_t sub = pw.intrinsic_subset ? pw.intrinsic_subset[b - c->nc] : 0;
+        in_bits = 0x3F;
+        free_bits = 0x3F & ~sub;
+      } else {
+        const int32_t all = c->grouped ? 0x3F : 0x1FF;
+        in_bits = free_bits = pw.camera_params ? pw.camera_params : all;
+      }
+      for (int k = 0; k < 9; ++k) {
+        fm[9 * (size_t)f + k] = (free_bits >> k) & 1;
+        xm[9 * (size_t)f + k] = (in_bits >> k) & 1;
+        trivial = trivial && fm[9 * (size_t)f + k] && xm[9 * (size_t)f + k];
+      }
+    }
+    if (!trivial) {
+      MBA_TRY(upload(c, &c->fmask, fm));
+      MBA_TRY(upload(c, &c->xmask, xm));
+    }
   }
   MBA_TRY(upload(c, &c->cams64, cams));
   MBA_TRY(upload(c, &c->cams64_t, cams));
@@ -1680,7 +1710,7 @@ struct Engine {
   mba_status x_norm(std::vector<double>* out) {
     const int64_t np3 = 3 * (int64_t)c->ne, nf9 = 9 * (int64_t)c->nf;
     hipLaunchKernelGGL(k_gather_cams, dim3(nblocks(nf9)), dim3(kBlock), 0, s, c->nf, d.f_to_cam, c->cams64,
-                       c->stepd + np3);
+                       c->stepd + np3, (const uint8_t*)c->xmask);
     hipLaunchKernelGGL(k_copy64, dim3(nblocks(np3)), dim3(kBlock), 0, s, np3, c->pts64, c->stepd);
     const int nb = nblocks(np3 + nf9);
     MBA_TRY(begin_sum());
@@ -1750,7 +1780,7 @@ struct Engine {
                        (int64_t)3 * c->ne, c->pscale64, d.pscale);
     hipLaunchKernelGGL((k_convert<double, T>), dim3(nblocks(9 * (int64_t)c->nf)), dim3(kBlock), 0, s,
                        (int64_t)9 * c->nf, c->cscale64, d.cscale);
-    if (c->grouped && c->nf > 0)
+    if (c->fmask && c->nf > 0)
       hipLaunchKernelGGL(k_scale_mask<T>, dim3(nblocks(9 * (int64_t)c->nf)), dim3(kBlock), 0, s, (int64_t)9 * c->nf,
                          (const uint8_t*)c->fmask, d.cscale);
   }

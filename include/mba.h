@@ -120,6 +120,17 @@ typedef struct mba_problem {
    * focal-only refinement is 0x3E, sfm_incremental_engine.cpp:1396-1409);
    * NULL or 0: all six free. */
   const uint8_t* intrinsic_subset;
+  /* Which entries of every camera vector are parameters (bit k = entry k);
+   * the others are fixed data of the residual, not part of the parameter
+   * block (they take no step and do not count in |x|).  0 = all (PACKED9
+   * 0x1FF, GROUPED 0x3F).  The global engine's functors
+   * (libs/sfm/include/mvg/sfm/sfm_bundle_adjustment_helper_tonly.h):
+   *   PinholeReprojectionError_Rt <2,6,3>: PACKED9, 0x3F, focal in entry 6,
+   *     k1 = k2 = 0 (sfm_global_engine.cpp:1144-1174, 1758-1802);
+   *   PinholeReprojectionError_t <2,3,3>: PACKED9, 0x38 (translation only;
+   *     rotation and focal are data, sfm_global_engine.cpp:1548-1594). */
+  int32_t camera_params;
+  int32_t reserved2;
 } mba_problem;
 
 typedef struct mba_options {

@@ -9,7 +9,9 @@
 // header chain needs the CMake-generated mvg/config.h.  Its arithmetic
 // (ceres::AngleAxisRotatePoint from the reference's Ceres headers + the
 // radial polynomial of brown_pinhole_camera.h:14-41) is therefore written
-// out below, operation for operation, as a Ceres autodiff functor.
+// out below, operation for operation, as a Ceres autodiff functor.  The
+// global engine's functors (PinholeReprojectionError_Rt / _t) need only the
+// Ceres headers and are included from the reference as they are.
 
 #include <algorithm>
 #include <cstring>
@@ -21,6 +23,7 @@
 
 #include "ceres/ceres.h"
 #include "ceres/rotation.h"
+#include "mvg/sfm/sfm_bundle_adjustment_helper_tonly.h"
 #include "ceres/problem_impl.h"
 #include "ceres/program.h"
 #include "ceres/parameter_block.h"
@@ -110,16 +113,28 @@ struct Scene {
   const unsigned char* intr_subset;
 };
 
+// Layouts: 0 packed [aa, t, f, k1, k2]; 1 (intrinsics, [aa, t], point);
+// 2 PinholeReprojectionError_Rt ([aa, t], focal = intr[0]); 3
+// PinholeReprojectionError_t ([t], focal = intr[0], rotation of camera c at
+// intr + 1 + 3 c).
+int CamStride(int layout) { return layout == 0 ? 9 : layout == 3 ? 3 : 6; }
+
 ceres::CostFunction* MakeCost(const Scene& s, long i) {
   if (s.layout == 0)
     return new ceres::AutoDiffCostFunction<Packed9Functor, 2, 9, 3>(new Packed9Functor(s.obs + 2 * i));
+  if (s.layout == 2)
+    return new ceres::AutoDiffCostFunction<mvg::sfm::PinholeReprojectionError_Rt, 2, 6, 3>(
+        new mvg::sfm::PinholeReprojectionError_Rt(s.obs + 2 * i, s.intr[0]));
+  if (s.layout == 3)
+    return new ceres::AutoDiffCostFunction<mvg::sfm::PinholeReprojectionError_t, 2, 3, 3>(
+        new mvg::sfm::PinholeReprojectionError_t(s.obs + 2 * i, s.intr[0], s.intr + 1 + 3L * s.cam_idx[i]));
   return new ceres::AutoDiffCostFunction<Intr6Ext6Functor, 2, 6, 6, 3>(new Intr6Ext6Functor(s.obs + 2 * i));
 }
 
 // Block id of a user-state pointer: layout 0 cameras then points; layout 1
 // intrinsics, extrinsics, points.
 int BlockId(const Scene& s, const double* p) {
-  const int cs = s.layout == 0 ? 9 : 6;
+  const int cs = CamStride(s.layout);
   if (s.layout == 1 && p >= s.intr && p < s.intr + 6 * s.ni) return (int)((p - s.intr) / 6);
   const int ni = s.layout == 1 ? s.ni : 0;
   if (p >= s.cams && p < s.cams + (long)cs * s.nc) return ni + (int)((p - s.cams) / cs);
@@ -129,13 +144,13 @@ int BlockId(const Scene& s, const double* p) {
 template <typename P>
 void AddAll(const Scene& s, P* problem, ceres::LossFunction* loss,
             std::map<const ceres::CostFunction*, long>* ids) {
-  const int cs = s.layout == 0 ? 9 : 6;
+  const int cs = CamStride(s.layout);
   for (long i = 0; i < s.nobs; ++i) {
     ceres::CostFunction* cf = MakeCost(s, i);
     if (ids) (*ids)[cf] = i;
     double* cam = s.cams + (long)cs * s.cam_idx[i];
     double* pt = s.pts + 3L * s.pt_idx[i];
-    if (s.layout == 0) {
+    if (s.layout != 1) {
       problem->AddResidualBlock(cf, loss, cam, pt);
     } else {
       problem->AddResidualBlock(cf, loss, s.intr + 6L * s.intr_idx[i], cam, pt);

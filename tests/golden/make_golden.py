@@ -48,6 +48,42 @@ def group_const():
     return c
 
 
+GLOBAL_FOCAL = 500.0
+TWO_VIEW_PAIRS = [(0, 1), (2, 3), (4, 5), (6, 7), (8, 9), (10, 11), (12, 13), (14, 15)]
+
+
+def global_scene():
+    """The global engine's BA problems (ComputeResidual: pinhole, one fixed
+    focal, no distortion; libs/sfm/include/mvg/sfm/pinhole_ceres_functor.h)
+    on the 16-camera scene's geometry: observations are that pinhole's
+    projections of the true points plus the scene's own noise and outliers."""
+    d = O.generate(O.preset(1))
+    r_true = O.evaluate(O.Scene(d, cameras=d["cam_true"], points=d["pt_true"]), jacobians=False)[0]
+    pin = d["cam_true"].copy()
+    pin[:, 6] = GLOBAL_FOCAL
+    pin[:, 7:] = 0.0
+    r_pin = O.evaluate(O.Scene(d, cameras=pin, points=d["pt_true"]), jacobians=False)[0]
+    obs = (r_pin + d["obs"]) - r_true  # pinhole prediction + (observed - packed prediction)
+    return dict(obs=obs, cam_idx=d["cam_idx"], pt_idx=d["pt_idx"], pt_init=d["pt_init"],
+                cam_true=d["cam_true"][:, :6].copy(), cam_init=d["cam_init"][:, :6].copy(), focal=GLOBAL_FOCAL)
+
+
+def two_view_problems(g):
+    """One 2-view problem per camera pair (the global engine's pair loop,
+    sfm_global_engine.cpp:985-1174): the points both cameras see and their
+    observations in both, cameras renumbered 0 and 1."""
+    out = []
+    for a, b in TWO_VIEW_PAIRS:
+        ia, ib = g["cam_idx"] == a, g["cam_idx"] == b
+        common = np.intersect1d(g["pt_idx"][ia], g["pt_idx"][ib])
+        rows = np.flatnonzero((ia | ib) & np.isin(g["pt_idx"], common))
+        remap = {int(q): k for k, q in enumerate(common)}
+        out.append(dict(obs=g["obs"][rows].copy(), cam_idx=(g["cam_idx"][rows] == b).astype(np.int32),
+                        pt_idx=np.array([remap[int(q)] for q in g["pt_idx"][rows]], np.int32),
+                        pt_init=g["pt_init"][common].copy(), cam_init=g["cam_init"][[a, b]].copy()))
+    return out
+
+
 def tiny_scene():
     """Hand-built: 4 cameras, 6 points; camera 3 sees only points 0 and 1."""
     spec = O.make_spec(3, 6, 2, 3, seed=5)
@@ -157,6 +193,38 @@ def main():
         out[f"grp_{name}_cameras"] = sc.cameras
         out[f"grp_{name}_intrinsics"] = sc.intrinsics
         out[f"grp_{name}_points"] = sc.points
+    # 7. The global engine's functors: PinholeReprojectionError_Rt <2,6,3>
+    #    (final BA, HuberLoss(Square(2.0)), SPARSE_SCHUR), PinholeReprojectionError_t
+    #    <2,3,3> (translation only, rotations fixed at the true ones,
+    #    HuberLoss(Square(4.0))), and the 2-view DENSE_SCHUR problems.
+    g = global_scene()
+    foc = np.array([g["focal"]])
+    sc = O.Scene(dict(cam_init=g["cam_init"], pt_init=g["pt_init"], obs=g["obs"], cam_idx=g["cam_idx"],
+                      pt_idx=g["pt_idx"]), layout=2, intrinsics=foc)
+    r = O.ref_solve(sc, huber_a=4.0, linear_solver=0, max_iters=20)
+    for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
+        out[f"rt_sparse_{k}"] = np.asarray(r[k])
+    out["rt_sparse_cameras"] = sc.cameras
+    sc = O.Scene(dict(cam_init=g["cam_init"][:, 3:].copy(), pt_init=g["pt_init"], obs=g["obs"],
+                      cam_idx=g["cam_idx"], pt_idx=g["pt_idx"]), layout=3,
+                 intrinsics=np.concatenate([foc, g["cam_true"][:, :3].ravel()]))
+    r = O.ref_solve(sc, huber_a=16.0, linear_solver=0, max_iters=20)
+    for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
+        out[f"tonly_sparse_{k}"] = np.asarray(r[k])
+    out["tonly_sparse_translations"] = sc.cameras
+    costs, succ, term, nit = [], [], [], []
+    for k, tv in enumerate(two_view_problems(g)):
+        sc = O.Scene(tv, layout=2, intrinsics=foc)
+        r = O.ref_solve(sc, huber_a=4.0, linear_solver=2, max_iters=50)
+        costs.append([r["initial_cost"], r["final_cost"]])
+        succ.append([r["successful"], r["unsuccessful"]])
+        term.append(r["termination"])
+        nit.append(len(r["iterations"]))
+        out[f"twoview_{k}_iteration_costs"] = r["iterations"][:, 1].copy()
+        out[f"twoview_{k}_cameras"] = sc.cameras
+    out["twoview_costs"] = np.array(costs)
+    out["twoview_steps"] = np.array(succ)
+    out["twoview_termination"] = np.array(term)
     path = os.path.join(HERE, "ceres_ref.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")

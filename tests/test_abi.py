@@ -24,8 +24,8 @@ def test_library_exports_every_declared_symbol(mba):
         assert hasattr(lib, name), name
     assert set(syms) == set(mba.EXPORTS)
     assert lib.mba_abi_version() == 2
-    # mba_problem: 4 int32, 1 int64, 10 pointers (x86-64 layout, no padding).
-    assert ctypes.sizeof(mba._Problem) == 4 * 4 + 8 + 10 * 8
+    # mba_problem: 4 int32, 1 int64, 10 pointers, 2 int32 (x86-64 layout).
+    assert ctypes.sizeof(mba._Problem) == 4 * 4 + 8 + 10 * 8 + 2 * 4
 
 
 def test_default_options_are_the_reference_defaults(mba):
