@@ -34,6 +34,7 @@ ERR_INVALID_ARGUMENT, ERR_UNSUPPORTED, ERR_DEVICE, ERR_OUT_OF_MEMORY, ERR_NOT_FI
 CONVERGENCE, NO_CONVERGENCE, FAILURE = 0, 1, 2
 LAYOUT_PACKED9 = 0
 LAYOUT_GROUPED = 1
+ORDER_AUTOMATIC, ORDER_POINTS_FIRST = 0, 1
 EXPLICIT_SCHUR, IMPLICIT_SCHUR_PCG = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
@@ -62,7 +63,7 @@ class _Problem(C.Structure):
                 ("points", _f64p), ("observations", _f64p), ("camera_index", _i32p),
                 ("point_index", _i32p), ("camera_constant", _u8p), ("intrinsics", _f64p),
                 ("intrinsic_index", _i32p), ("intrinsic_constant", _u8p), ("intrinsic_subset", _u8p),
-                ("camera_params", C.c_int32), ("reserved2", C.c_int32)]
+                ("camera_params", C.c_int32), ("ordering", C.c_int32)]
 
 
 class Options(C.Structure):
@@ -108,6 +109,8 @@ EXPORTS = [
     "mba_context_solve", "mba_context_download", "mba_context_synchronize", "mba_context_stream",
     "mba_context_set_timing", "mba_context_last_timing", "mba_status_string", "mba_last_error",
     "mba_abi_version", "mba_scene_generate", "mba_scene_generate_grouped",
+    "mba_rotation_matrix_to_angle_axis", "mba_angle_axis_to_rotation_matrix", "mba_read_libmv",
+    "mba_read_bal", "mba_write_bal",
 ]
 
 
@@ -147,6 +150,12 @@ def load_library():
                                        _f64p, _f64p, _i32p, _i32p]
     lib.mba_scene_generate_grouped.argtypes = [C.c_int32, _i32p, _i32p, _i32p, _i64p, _f64p, _f64p, _f64p, _f64p,
                                                _i32p, _i32p, _i32p]
+    lib.mba_rotation_matrix_to_angle_axis.argtypes = [_f64p, _f64p, C.c_int64]
+    lib.mba_angle_axis_to_rotation_matrix.argtypes = [_f64p, _f64p, C.c_int64]
+    lib.mba_read_libmv.argtypes = [C.c_char_p, _i32p, _i32p, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p, _i32p,
+                                   _i32p, _i32p, _i32p]
+    lib.mba_read_bal.argtypes = [C.c_char_p, _i32p, _i32p, _i64p, _f64p, _f64p, _f64p, _i32p, _i32p]
+    lib.mba_write_bal.argtypes = [C.c_char_p, C.POINTER(_Problem)]
     _lib = lib
     return lib
 
@@ -180,7 +189,7 @@ class BAProblem:
 
     def __init__(self, cameras, points, observations, camera_index, point_index, camera_constant=None,
                  intrinsics=None, intrinsic_index=None, intrinsic_constant=None, intrinsic_subset=None,
-                 camera_params=0):
+                 camera_params=0, ordering=0):
         self.cameras = np.ascontiguousarray(cameras, np.float64)
         self.points = np.ascontiguousarray(points, np.float64)
         self.observations = np.ascontiguousarray(observations, np.float64)
@@ -197,6 +206,7 @@ class BAProblem:
         self.intrinsic_subset = (None if intrinsic_subset is None
                                  else np.ascontiguousarray(intrinsic_subset, np.uint8))
         self.camera_params = int(camera_params)
+        self.ordering = int(ordering)
         width = 9 if self.layout == LAYOUT_PACKED9 else 6
         if self.cameras.ndim != 2 or self.cameras.shape[1] != width:
             raise ValueError(f"cameras must be (Nc, {width})")
@@ -239,6 +249,7 @@ class BAProblem:
         s.intrinsic_constant = _p(self.intrinsic_constant, _u8p)
         s.intrinsic_subset = _p(self.intrinsic_subset, _u8p)
         s.camera_params = self.camera_params
+        s.ordering = self.ordering
         return s
 
 
@@ -449,6 +460,62 @@ def generate_grouped_scene(preset: int = 6) -> dict:
                                           _p(d["cam_idx"], _i32p), _p(d["intr_idx"], _i32p),
                                           _p(d["pt_idx"], _i32p)))
     return d
+
+
+# ------------------------------------------------ conversions and files --
+def rotation_matrix_to_angle_axis(R):
+    """RotationMatrixToAngleAxis of (n, 3, 3) matrices (row-major numpy
+    arrays; the C ABI takes column-major)."""
+    R = np.asarray(R, np.float64).reshape(-1, 3, 3)
+    Rc = np.ascontiguousarray(np.transpose(R, (0, 2, 1)))
+    aa = np.zeros((R.shape[0], 3))
+    _check(load_library().mba_rotation_matrix_to_angle_axis(_p(Rc), _p(aa), R.shape[0]))
+    return aa
+
+
+def angle_axis_to_rotation_matrix(aa):
+    aa = np.ascontiguousarray(np.asarray(aa, np.float64).reshape(-1, 3))
+    Rc = np.zeros((aa.shape[0], 3, 3))
+    _check(load_library().mba_angle_axis_to_rotation_matrix(_p(aa), _p(Rc), aa.shape[0]))
+    return np.transpose(Rc, (0, 2, 1)).copy()
+
+
+def read_libmv(path) -> dict:
+    """A libmv problem file as grouped-layout arrays (one intrinsic group)."""
+    lib = load_library()
+    b = os.fsencode(path)
+    nc, np_, no, sp = C.c_int32(), C.c_int32(), C.c_int64(), C.c_int32()
+    _check(lib.mba_read_libmv(b, C.byref(nc), C.byref(np_), C.byref(no), C.byref(sp), None, None, None, None,
+                              None, None, None, None))
+    d = dict(intrinsics=np.zeros((1, 6)), cameras=np.zeros((nc.value, 6)), points=np.zeros((np_.value, 3)),
+             obs=np.zeros((no.value, 2)), cam_idx=np.zeros(no.value, np.int32), pt_idx=np.zeros(no.value, np.int32),
+             image_of_camera=np.zeros(nc.value, np.int32), track_of_point=np.zeros(np_.value, np.int32))
+    _check(lib.mba_read_libmv(b, C.byref(nc), C.byref(np_), C.byref(no), C.byref(sp), _p(d["intrinsics"]),
+                              _p(d["cameras"]), _p(d["points"]), _p(d["obs"]), _p(d["cam_idx"], _i32p),
+                              _p(d["pt_idx"], _i32p), _p(d["image_of_camera"], _i32p),
+                              _p(d["track_of_point"], _i32p)))
+    d["image_space"] = bool(sp.value)
+    d["intr_idx"] = np.zeros(no.value, np.int32)
+    return d
+
+
+def read_bal(path) -> dict:
+    """A BAL text problem as packed-layout arrays (observations negated: see
+    mba_read_bal)."""
+    lib = load_library()
+    b = os.fsencode(path)
+    nc, np_, no = C.c_int32(), C.c_int32(), C.c_int64()
+    _check(lib.mba_read_bal(b, C.byref(nc), C.byref(np_), C.byref(no), None, None, None, None, None))
+    d = dict(cameras=np.zeros((nc.value, 9)), points=np.zeros((np_.value, 3)), obs=np.zeros((no.value, 2)),
+             cam_idx=np.zeros(no.value, np.int32), pt_idx=np.zeros(no.value, np.int32))
+    _check(lib.mba_read_bal(b, C.byref(nc), C.byref(np_), C.byref(no), _p(d["cameras"]), _p(d["points"]),
+                            _p(d["obs"]), _p(d["cam_idx"], _i32p), _p(d["pt_idx"], _i32p)))
+    return d
+
+
+def write_bal(path, problem: BAProblem):
+    s = problem._struct()
+    _check(load_library().mba_write_bal(os.fsencode(path), C.byref(s)))
 
 
 # ------------------------------------------------ Ceres-shaped host API --

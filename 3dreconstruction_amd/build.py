@@ -18,9 +18,9 @@ FLAGS = [
 
 
 def build(verbose: bool = False) -> str:
-    src = os.path.join(CSRC, "mba_engine.hip")
+    srcs = [os.path.join(CSRC, "mba_engine.hip"), os.path.join(CSRC, "mba_io.cpp")]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, src]
+    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -32,6 +32,10 @@ mba_status fail(mba_status s, const std::string& msg) {
   return s;
 }
 
+}  // namespace
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+namespace {
+
 #define HIP_TRY(expr)                                                              \
   do {                                                                             \
     hipError_t e_ = (expr);                                                        \
@@ -88,6 +92,8 @@ mba_status validate(const mba_problem* p) {
     return fail(MBA_ERR_INVALID_ARGUMENT, "unknown camera layout " + std::to_string(p->layout));
   if (p->num_cameras < 0 || p->num_points < 0 || p->num_observations < 0)
     return fail(MBA_ERR_INVALID_ARGUMENT, "negative sizes");
+  if (p->ordering != MBA_ORDER_AUTOMATIC && p->ordering != MBA_ORDER_POINTS_FIRST)
+    return fail(MBA_ERR_INVALID_ARGUMENT, "unknown ordering " + std::to_string(p->ordering));
   {
     const int32_t all = p->layout == MBA_LAYOUT_GROUPED ? 0x3F : 0x1FF;
     if (p->camera_params < 0 || (p->camera_params & ~all) != 0)
@@ -124,6 +130,26 @@ mba_status validate(const mba_problem* p) {
 }
 
 mba_status build_structure_grouped(const mba_problem* p, Structure* st);
+
+// MBA_ORDER_POINTS_FIRST: group 0 = every active point block, group 1 = the
+// rest, each group in address order of the libs/sfm containers (cameras,
+// intrinsics, points): vertex ids in the structure are already in that
+// order within each group (points are the ids >= `first_point`).
+void user_points_first(const std::vector<int32_t>& active, int32_t first_point, Structure* st) {
+  std::vector<int32_t> pts, rest;
+  for (int32_t b : active) (b >= first_point ? pts : rest).push_back(b);
+  std::sort(pts.begin(), pts.end());
+  std::sort(rest.begin(), rest.end(), [&](int32_t a, int32_t b) {
+    // Grouped ids: groups 0..ng-1 precede cameras, but in the containers
+    // the cameras come first; packed ids are cameras only.
+    const bool ga = a < st->ng, gb = b < st->ng;
+    if (ga != gb) return gb;
+    return a < b;
+  });
+  st->order = pts;
+  st->num_e = (int32_t)pts.size();
+  st->order.insert(st->order.end(), rest.begin(), rest.end());
+}
 
 mba_status build_structure(const mba_problem* p, Structure* st) {
   MBA_TRY(validate(p));
@@ -197,24 +223,28 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
   for (int32_t d = 0; d <= maxdeg; ++d) cnt[d + 1] += cnt[d];
   std::vector<int32_t> queue(active.size());
   for (int32_t b : active) queue[cnt[deg[b]]++] = b;
-  // Greedy independent set.
-  std::vector<uint8_t> color(nb, 0);  // 0 white, 1 grey, 2 black
   st->order.clear();
   st->order.reserve(active.size());
-  for (int32_t b : queue) {
-    if (color[b] != 0) continue;
-    st->order.push_back(b);
-    color[b] = 2;
-    if (b < nc) {
-      for (int64_t k = cadj_start[b]; k < cadj_start[b + 1]; ++k) color[nc + cadj[k]] = 1;
-    } else {
-      const int32_t q = b - nc;
-      for (int64_t k = padj_start[q]; k < padj_start[q + 1]; ++k) color[padj[k]] = 1;
+  if (p->ordering == MBA_ORDER_POINTS_FIRST) {
+    user_points_first(active, nc, st);
+  } else {
+    // Greedy independent set.
+    std::vector<uint8_t> color(nb, 0);  // 0 white, 1 grey, 2 black
+    for (int32_t b : queue) {
+      if (color[b] != 0) continue;
+      st->order.push_back(b);
+      color[b] = 2;
+      if (b < nc) {
+        for (int64_t k = cadj_start[b]; k < cadj_start[b + 1]; ++k) color[nc + cadj[k]] = 1;
+      } else {
+        const int32_t q = b - nc;
+        for (int64_t k = padj_start[q]; k < padj_start[q + 1]; ++k) color[padj[k]] = 1;
+      }
     }
+    st->num_e = (int32_t)st->order.size();
+    for (int32_t b : queue)
+      if (color[b] != 2) st->order.push_back(b);
   }
-  st->num_e = (int32_t)st->order.size();
-  for (int32_t b : queue)
-    if (color[b] != 2) st->order.push_back(b);
   // Column-block index of every block.
   std::vector<int32_t> col(nb, -1);
   for (size_t i = 0; i < st->order.size(); ++i) col[st->order[i]] = (int32_t)i;
@@ -361,18 +391,22 @@ mba_status build_structure_grouped(const mba_problem* p, Structure* st) {
   for (int32_t d = 0; d <= maxdeg; ++d) cnt[d + 1] += cnt[d];
   std::vector<int32_t> queue(active.size());
   for (int32_t b : active) queue[cnt[adj_start[b + 1] - adj_start[b]]++] = b;
-  std::vector<uint8_t> color(nb, 0);
   st->order.clear();
   st->order.reserve(active.size());
-  for (int32_t b : queue) {
-    if (color[b] != 0) continue;
-    st->order.push_back(b);
-    color[b] = 2;
-    for (int64_t k = adj_start[b]; k < adj_start[b + 1]; ++k) color[adj[k]] = 1;
+  if (p->ordering == MBA_ORDER_POINTS_FIRST) {
+    user_points_first(active, ng + nc, st);
+  } else {
+    std::vector<uint8_t> color(nb, 0);
+    for (int32_t b : queue) {
+      if (color[b] != 0) continue;
+      st->order.push_back(b);
+      color[b] = 2;
+      for (int64_t k = adj_start[b]; k < adj_start[b + 1]; ++k) color[adj[k]] = 1;
+    }
+    st->num_e = (int32_t)st->order.size();
+    for (int32_t b : queue)
+      if (color[b] != 2) st->order.push_back(b);
   }
-  st->num_e = (int32_t)st->order.size();
-  for (int32_t b : queue)
-    if (color[b] != 2) st->order.push_back(b);
   std::vector<int32_t> col(nb, -1);
   for (size_t i = 0; i < st->order.size(); ++i) col[st->order[i]] = (int32_t)i;
   std::vector<int64_t> hist((size_t)st->num_e + 1, 0);

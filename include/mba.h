@@ -77,6 +77,20 @@ typedef enum mba_layout {
   MBA_LAYOUT_GROUPED = 1
 } mba_layout;
 
+typedef enum mba_ordering {
+  /* ComputeStableSchurOrdering (parameter_block_ordering.cc:50-79): the
+   * greedy independent set, which must come out as the point blocks. */
+  MBA_ORDER_AUTOMATIC = 0,
+  /* A user ordering with every point block in group 0 and every other block
+   * in group 1 (examples/bundle_adjuster.cc's "user" ordering).  Ceres keeps
+   * a group's blocks in address order (ordered_groups.h), which for the
+   * libs/sfm containers ([cameras | intrinsics | points]) is: points by
+   * index, then cameras by index, then intrinsic groups by index.  Needed
+   * when the automatic set would eliminate cameras (few points, many
+   * cameras, e.g. the libmv camera-refinement problems). */
+  MBA_ORDER_POINTS_FIRST = 1
+} mba_ordering;
+
 /* Linear solver of the LM step.  EXPLICIT_SCHUR forms the reduced camera
  * matrix S and factors it (the SPARSE_SCHUR / DENSE_SCHUR role);
  * IMPLICIT_SCHUR_PCG is ITERATIVE_SCHUR with the block-Jacobi (JACOBI)
@@ -130,7 +144,9 @@ typedef struct mba_problem {
    *   PinholeReprojectionError_t <2,3,3>: PACKED9, 0x38 (translation only;
    *     rotation and focal are data, sfm_global_engine.cpp:1548-1594). */
   int32_t camera_params;
-  int32_t reserved2;
+  /* mba_ordering: the Schur elimination ordering (Solver::Options::
+   * linear_solver_ordering, include/ceres/solver.h:432-520). */
+  int32_t ordering;
 } mba_problem;
 
 typedef struct mba_options {
@@ -304,6 +320,43 @@ void* mba_context_stream(mba_context* ctx);
  * factorisation, ms[5] triangular solves with refinement.  n <= 6 values. */
 mba_status mba_context_set_timing(mba_context* ctx, int32_t enable);
 mba_status mba_context_last_timing(mba_context* ctx, double* ms, int32_t n);
+
+/* ----------------------------------------- conversions and file formats --
+ * Host-only (no device needed). */
+
+/* RotationMatrixToAngleAxis / AngleAxisToRotationMatrix
+ * (include/ceres/rotation.h:357-415, the conversions the incremental engine
+ * applies on entry and exit, sfm_incremental_engine.cpp:1306,1505): n
+ * column-major 3x3 matrices <-> n angle-axis vectors. */
+mba_status mba_rotation_matrix_to_angle_axis(const double* R, double* angle_axis, int64_t n);
+mba_status mba_angle_axis_to_rotation_matrix(const double* angle_axis, double* R, int64_t n);
+
+/* A libmv bundle-adjustment problem file (the reader of
+ * 3rdparty/ceres/examples/libmv_bundle_adjuster.cc:255-437; the files of
+ * 3rdparty/ceres/data/libmv-ba-problems) as a GROUPED problem with one
+ * intrinsic group: cameras [aa, t] (RotationMatrixToAngleAxis of the file's
+ * R), points, and the markers whose image and track exist, in file order;
+ * cameras and points numbered in ascending image / track id
+ * (image_of_camera, track_of_point).  intrinsics: 6 values [f, ppx, ppy,
+ * k1, k2, k3]; image_space: 1 for pixel markers.  Tangential distortion
+ * (p1, p2 != 0) is MBA_ERR_UNSUPPORTED.  Call with NULL arrays for the
+ * sizes, then with arrays of those sizes. */
+mba_status mba_read_libmv(const char* path, int32_t* num_cameras, int32_t* num_points,
+                          int64_t* num_observations, int32_t* image_space, double* intrinsics,
+                          double* cameras, double* points, double* observations,
+                          int32_t* camera_index, int32_t* point_index,
+                          int32_t* image_of_camera, int32_t* track_of_point);
+
+/* BAL text problems (examples/bal_problem.cc:72-175) as PACKED9 problems.
+ * A BAL camera looks down -z (Snavely's model); the libs/sfm model predicts
+ * the negated image point with the same camera vector, so observations are
+ * negated on reading and writing (cost and normal equations unchanged).
+ * mba_read_bal: NULL arrays give the sizes.  mba_write_bal: the text of
+ * BALProblem::WriteToFile. */
+mba_status mba_read_bal(const char* path, int32_t* num_cameras, int32_t* num_points,
+                        int64_t* num_observations, double* cameras, double* points,
+                        double* observations, int32_t* camera_index, int32_t* point_index);
+mba_status mba_write_bal(const char* path, const mba_problem* problem);
 
 /* --------------------------------------------------------- utilities --- */
 const char* mba_status_string(mba_status status);

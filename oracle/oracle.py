@@ -348,3 +348,75 @@ def ref_evaluate_obs(layout, cam, pt, obs, intr=None):
     j0, j1, j2 = np.zeros((2, 6)), np.zeros((2, 6)), np.zeros((2, 3))
     ref_lib().ref_evaluate_obs(1, ptr(intr), ptr(cam), ptr(pt), ptr(obs), ptr(r), ptr(j0), ptr(j1), ptr(j2))
     return r, j0, j1, j2
+
+
+# ------------------------------------------ reference formats and rotations --
+def ref_set_points_first(on: bool):
+    """Solver::Options::linear_solver_ordering with the points in group 0."""
+    ref_lib().ref_set_points_first(C.c_int(1 if on else 0))
+
+
+def ref_rotation_matrix_to_angle_axis(R):
+    """ceres::RotationMatrixToAngleAxis of (n, 3, 3) row-major matrices."""
+    R = np.asarray(R, np.float64).reshape(-1, 3, 3)
+    Rc = np.ascontiguousarray(np.transpose(R, (0, 2, 1)))
+    aa = np.zeros((R.shape[0], 3))
+    ref_lib().ref_rotation_matrix_to_angle_axis(ptr(Rc), ptr(aa), C.c_long(R.shape[0]))
+    return aa
+
+
+def ref_bal_roundtrip(path_in, path_out):
+    ref_lib().ref_bal_roundtrip(os.fsencode(path_in), os.fsencode(path_out))
+
+
+def ref_bal_arrays(path):
+    sizes = np.zeros(3, np.int32)
+    ref_lib().ref_bal_arrays(os.fsencode(path), ptr(sizes, _i32p), None, None, None, None)
+    nc, np_, no = (int(v) for v in sizes)
+    d = dict(params=np.zeros(9 * nc + 3 * np_), obs=np.zeros((no, 2)), cam_idx=np.zeros(no, np.int32),
+             pt_idx=np.zeros(no, np.int32))
+    ref_lib().ref_bal_arrays(os.fsencode(path), ptr(sizes, _i32p), ptr(d["params"]), ptr(d["obs"]),
+                             ptr(d["cam_idx"], _i32p), ptr(d["pt_idx"], _i32p))
+    d["cameras"] = d["params"][:9 * nc].reshape(nc, 9)
+    d["points"] = d["params"][9 * nc:].reshape(np_, 3)
+    return d
+
+
+def ref_bal_solve(path, huber_a=0.0, linear_solver=0, max_iters=20, max_rec=1000):
+    summ = np.zeros(8)
+    its = np.zeros((max_rec, 9))
+    d = ref_bal_arrays(path)
+    params = np.zeros_like(d["params"])
+    ref_lib().ref_bal_solve(os.fsencode(path), C.c_double(huber_a), C.c_int(linear_solver), C.c_int(max_iters),
+                            ptr(summ), ptr(its), C.c_int(max_rec), ptr(params))
+    n = int(summ[5])
+    nc = d["cameras"].shape[0]
+    return dict(initial_cost=summ[0], final_cost=summ[1], successful=int(summ[2]), unsuccessful=int(summ[3]),
+                termination=int(summ[4]), iterations=its[:min(n, max_rec)].copy(),
+                cameras=params[:9 * nc].reshape(nc, 9), points=params[9 * nc:].reshape(-1, 3))
+
+
+def parse_libmv(path):
+    """The libmv problem format read independently of the library
+    (libmv_bundle_adjuster.cc:255-437): returns the file's raw contents."""
+    import struct
+    b = open(path, "rb").read()
+    e = {ord("v"): "<", ord("V"): ">"}[b[0]]
+    out = dict(image_space=b[1] == ord("P"))
+    o = 2
+    out["intr8"] = np.array(struct.unpack_from(e + "8f", b, o), np.float64)
+    o += 32
+    nc, = struct.unpack_from(e + "i", b, o)
+    o += 4
+    cams = np.frombuffer(b, np.dtype([("image", e + "i4"), ("R", e + "f4", 9), ("t", e + "f4", 3)]), nc, o)
+    o += cams.nbytes
+    npt, = struct.unpack_from(e + "i", b, o)
+    o += 4
+    pts = np.frombuffer(b, np.dtype([("track", e + "i4"), ("X", e + "f4", 3)]), npt, o)
+    o += pts.nbytes
+    nm, = struct.unpack_from(e + "i", b, o)
+    o += 4
+    mk = np.frombuffer(b, np.dtype([("image", e + "i4"), ("track", e + "i4"), ("x", e + "f4"), ("y", e + "f4")]),
+                       nm, o)
+    out.update(cams=cams, pts=pts, markers=mk)
+    return out

@@ -24,6 +24,8 @@
 #include "ceres/ceres.h"
 #include "ceres/rotation.h"
 #include "mvg/sfm/sfm_bundle_adjustment_helper_tonly.h"
+#include "bal_problem.h"                 // 3rdparty/ceres/examples
+#include "snavely_reprojection_error.h"  // 3rdparty/ceres/examples
 #include "ceres/problem_impl.h"
 #include "ceres/program.h"
 #include "ceres/parameter_block.h"
@@ -181,9 +183,105 @@ void AddAll(const Scene& s, P* problem, ceres::LossFunction* loss,
   }
 }
 
+// User ordering "points first" (Solver::Options::linear_solver_ordering
+// with every point block in group 0, the rest in group 1); set by
+// ref_set_points_first.  Ceres keeps each group in address order, so for
+// layout 1 the cameras and intrinsics are moved into one buffer
+// [cameras | intrinsics] (the libs/sfm container order) while solving.
+int g_points_first = 0;
+
+void AddPointsFirst(const Scene& s, const std::vector<double*>& blocks, ceres::ParameterBlockOrdering* o) {
+  for (double* b : blocks) o->AddElementToGroup(b, (b >= s.pts && b < s.pts + 3L * s.np) ? 0 : 1);
+}
+
+struct ContiguousCamsIntr {
+  Scene* s;
+  std::vector<double> buf;
+  double *cams0, *intr0;
+  explicit ContiguousCamsIntr(Scene* sc) : s(sc), cams0(sc->cams), intr0(sc->intr) {
+    if (!g_points_first || s->layout != 1) return;
+    buf.assign(cams0, cams0 + 6L * s->nc);
+    buf.insert(buf.end(), intr0, intr0 + 6L * s->ni);
+    s->cams = buf.data();
+    s->intr = buf.data() + 6L * s->nc;
+  }
+  ~ContiguousCamsIntr() {
+    if (buf.empty()) return;
+    std::copy(buf.begin(), buf.begin() + 6L * s->nc, cams0);
+    std::copy(buf.begin() + 6L * s->nc, buf.end(), intr0);
+    s->cams = cams0;
+    s->intr = intr0;
+  }
+};
+
 }  // namespace
 
 extern "C" {
+
+void ref_set_points_first(int on) { g_points_first = on; }
+
+// ceres::RotationMatrixToAngleAxis on n column-major matrices.
+void ref_rotation_matrix_to_angle_axis(const double* R, double* aa, long n) {
+  for (long i = 0; i < n; ++i) ceres::RotationMatrixToAngleAxis(R + 9 * i, aa + 3 * i);
+}
+
+// BALProblem (examples/bal_problem.cc): read `in`, write it back to `out`.
+int ref_bal_roundtrip(const char* in, const char* out) {
+  ceres::examples::BALProblem bal(in, false);
+  bal.WriteToFile(out);
+  return 1;
+}
+
+// BALProblem's arrays: sizes[3] = {Nc, Np, Nobs}; params 9 Nc + 3 Np.
+int ref_bal_arrays(const char* in, int* sizes, double* params, double* obs, int* cam_idx, int* pt_idx) {
+  ceres::examples::BALProblem bal(in, false);
+  sizes[0] = bal.num_cameras();
+  sizes[1] = bal.num_points();
+  sizes[2] = bal.num_observations();
+  if (params) std::copy(bal.parameters(), bal.parameters() + bal.num_parameters(), params);
+  if (obs) std::copy(bal.observations(), bal.observations() + 2 * bal.num_observations(), obs);
+  if (cam_idx) std::copy(bal.camera_index(), bal.camera_index() + bal.num_observations(), cam_idx);
+  if (pt_idx) std::copy(bal.point_index(), bal.point_index() + bal.num_observations(), pt_idx);
+  return 1;
+}
+
+// A BAL file solved with the examples' SnavelyReprojectionError (the
+// bundle_adjuster example's functor), options as ref_solve; summary and
+// iteration records as ref_solve, final parameters into params_out.
+int ref_bal_solve(const char* in, double huber_a, int linear_solver, int max_iters, double* summary,
+                  double* iters, int max_rec, double* params_out) {
+  ceres::examples::BALProblem bal(in, false);
+  ceres::Problem problem;
+  ceres::LossFunction* loss = huber_a > 0.0 ? new ceres::HuberLoss(huber_a) : NULL;
+  const double* o = bal.observations();
+  for (int i = 0; i < bal.num_observations(); ++i)
+    problem.AddResidualBlock(ceres::examples::SnavelyReprojectionError::Create(o[2 * i], o[2 * i + 1]), loss,
+                             bal.mutable_cameras() + 9 * bal.camera_index()[i],
+                             bal.mutable_points() + 3 * bal.point_index()[i]);
+  ceres::Solver::Options options;
+  options.preconditioner_type = ceres::JACOBI;
+  options.linear_solver_type = linear_solver == 1 ? ceres::ITERATIVE_SCHUR : ceres::SPARSE_SCHUR;
+  options.sparse_linear_algebra_library_type = ceres::CX_SPARSE;
+  options.logging_type = ceres::SILENT;
+  options.max_num_iterations = max_iters;
+  ceres::Solver::Summary sum;
+  ceres::Solve(options, &problem, &sum);
+  summary[0] = sum.initial_cost;
+  summary[1] = sum.final_cost;
+  summary[2] = sum.num_successful_steps;
+  summary[3] = sum.num_unsuccessful_steps;
+  summary[4] = sum.termination_type;
+  summary[5] = (double)sum.iterations.size();
+  for (int k = 0; k < (int)sum.iterations.size() && k < max_rec; ++k) {
+    const ceres::IterationSummary& it = sum.iterations[k];
+    double* r = iters + 9 * k;
+    r[0] = it.iteration; r[1] = it.cost; r[2] = it.cost_change; r[3] = it.trust_region_radius;
+    r[4] = it.step_is_successful; r[5] = it.step_is_valid; r[6] = it.linear_solver_iterations;
+    r[7] = it.relative_decrease; r[8] = it.gradient_max_norm;
+  }
+  if (params_out) std::copy(bal.parameters(), bal.parameters() + bal.num_parameters(), params_out);
+  return sum.IsSolutionUsable() ? 1 : 0;
+}
 
 // AutoDiffCostFunction::Evaluate for one observation (no loss).
 int ref_evaluate_obs(int layout, const double* intr, const double* cam, const double* pt,
@@ -208,6 +306,7 @@ int ref_schur_ordering(int layout, int nc, int ni, int np, long nobs, double* ca
                        int* num_blocks, long* residual_order, long* num_residuals,
                        int* pairs, long pairs_cap, long* num_pairs) {
   Scene s = {layout, nc, ni, np, nobs, cams, intr, pts, obs, cam_idx, intr_idx, pt_idx, cam_const};
+  ContiguousCamsIntr contiguous(&s);
   ceres::Problem::Options po;
   ceres::internal::ProblemImpl problem(po);
   std::map<const ceres::CostFunction*, long> ids;
@@ -219,8 +318,14 @@ int ref_schur_ordering(int layout, int nc, int ni, int np, long nobs, double* ca
       problem.mutable_program()->CreateReducedProgram(&removed, &fixed_cost, &error));
   if (reduced.get() == NULL) return -1;
   ceres::ParameterBlockOrdering ordering;
+  std::vector<double*> blocks;
   for (size_t i = 0; i < reduced->parameter_blocks().size(); ++i)
-    ordering.AddElementToGroup(reduced->parameter_blocks()[i]->mutable_user_state(), 0);
+    blocks.push_back(reduced->parameter_blocks()[i]->mutable_user_state());
+  if (g_points_first) {
+    AddPointsFirst(s, blocks, &ordering);
+  } else {
+    for (double* b : blocks) ordering.AddElementToGroup(b, 0);
+  }
   if (!ceres::internal::ReorderProgramForSchurTypeLinearSolver(
           ceres::SPARSE_SCHUR, ceres::CX_SPARSE, problem.parameter_map(), &ordering,
           reduced.get(), &error))
@@ -280,10 +385,18 @@ int ref_solve(int layout, int nc, int ni, int np, long nobs, double* cams, doubl
               int num_threads, double* summary, double* iters, int max_rec) {
   Scene s = {layout, nc, ni, np, nobs, cams, intr, pts, obs, cam_idx, intr_idx, pt_idx, cam_const,
              intr_const, intr_subset};
+  ContiguousCamsIntr contiguous(&s);
   ceres::Problem problem;
   ceres::LossFunction* loss = huber_a > 0.0 ? new ceres::HuberLoss(huber_a) : NULL;
   AddAll(s, &problem, loss, NULL);
   ceres::Solver::Options options;
+  if (g_points_first) {
+    std::vector<double*> blocks;
+    problem.GetParameterBlocks(&blocks);
+    ceres::ParameterBlockOrdering* o = new ceres::ParameterBlockOrdering;
+    AddPointsFirst(s, blocks, o);
+    options.linear_solver_ordering.reset(o);
+  }
   options.preconditioner_type = ceres::JACOBI;
   options.linear_solver_type = linear_solver == 1 ? ceres::ITERATIVE_SCHUR
                                : linear_solver == 2 ? ceres::DENSE_SCHUR

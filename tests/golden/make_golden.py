@@ -84,6 +84,74 @@ def two_view_problems(g):
     return out
 
 
+LIBMV_DIR = os.path.join(HERE, "libmv")
+LIBMV_FILES = ["problem_01.bin", "problem_02.bin", "problem_03.bin"]
+# libmv_bundle_adjuster.cc:817-836 with pixel markers refines f, k1, k2:
+# ppx, ppy, k3 (and p1, p2, zero here) held -> SubsetParameterization mask.
+LIBMV_SUBSET = 0x26
+BAL_FILE = os.path.join(HERE, "bal_small.txt")
+
+
+def libmv_problem(path):
+    """A libmv problem file as grouped arrays, built from an independent
+    parse of the file; rotations converted by the reference's
+    RotationMatrixToAngleAxis.  Cameras by ascending image, points by
+    ascending track (the address order of libmv's vectors), markers in file
+    order; the first marker's camera is held constant
+    (libmv_bundle_adjuster.cc:700-725)."""
+    f = O.parse_libmv(path)
+    cam = {int(c["image"]): c for c in f["cams"] if c["image"] >= 0}
+    pt = {int(p["track"]): p for p in f["pts"] if p["track"] >= 0}
+    images, tracks = sorted(cam), sorted(pt)
+    ci = {im: k for k, im in enumerate(images)}
+    pi = {tr: k for k, tr in enumerate(tracks)}
+    keep = [m for m in f["markers"] if int(m["image"]) in ci and int(m["track"]) in pi]
+    R = np.array([np.asarray(cam[im]["R"], np.float64).reshape(3, 3).T for im in images])  # column-major file
+    aa = O.ref_rotation_matrix_to_angle_axis(R)
+    t = np.array([np.asarray(cam[im]["t"], np.float64) for im in images])
+    const = np.zeros(len(images), np.uint8)
+    const[ci[int(keep[0]["image"])]] = 1
+    return dict(cam_init=np.concatenate([aa, t], axis=1), intr_init=f["intr8"][None, :6].copy(),
+                pt_init=np.array([np.asarray(pt[tr]["X"], np.float64) for tr in tracks]),
+                obs=np.array([[m["x"], m["y"]] for m in keep], np.float64),
+                cam_idx=np.array([ci[int(m["image"])] for m in keep], np.int32),
+                pt_idx=np.array([pi[int(m["track"])] for m in keep], np.int32),
+                intr_idx=np.zeros(len(keep), np.int32), const=const, R=R)
+
+
+def rotation_cases():
+    """Rotation matrices for the angle-axis conversion: random, identity,
+    tiny angles, exactly and nearly pi about axes (the branches of the
+    quaternion extraction)."""
+    rng = np.random.default_rng(8)
+    def rodrigues(w):
+        th = np.linalg.norm(w)
+        if th == 0:
+            return np.eye(3)
+        k = w / th
+        K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+        return np.eye(3) + np.sin(th) * K + (1 - np.cos(th)) * K @ K
+    ws = [rng.normal(size=3) * s for s in (0.1, 1.0, 2.0, 3.0) for _ in range(8)]
+    ws += [np.zeros(3), np.array([1e-9, -2e-9, 3e-9]), np.array([np.pi, 0, 0]), np.array([0, np.pi, 0]),
+           np.array([0, 0, np.pi]), np.array([0, 0, np.pi - 1e-7]), np.array([np.pi / 2, 0, 0])]
+    ws += [rng.normal(size=3) / np.linalg.norm(rng.normal(size=3)) for _ in range(2)]
+    axis = rng.normal(size=3)
+    ws.append(axis / np.linalg.norm(axis) * (np.pi - 1e-9))
+    return np.array([rodrigues(w) for w in ws])
+
+
+def write_bal_scene():
+    """A small packed scene as a BAL file (BAL cameras look down -z, so the
+    BAL observations are the negated libs/sfm ones)."""
+    d = O.generate(O.make_spec(8, 300, 2, 5, outlier_frac=0.05, seed=21))
+    with open(BAL_FILE, "w") as fo:
+        fo.write(f"{d['num_cameras']} {d['num_points']} {d['num_obs']}\n")
+        for i in range(d["num_obs"]):
+            fo.write("%d %d %.17g %.17g\n" % (d["cam_idx"][i], d["pt_idx"][i], -d["obs"][i, 0], -d["obs"][i, 1]))
+        for v in np.concatenate([d["cam_init"].ravel(), d["pt_init"].ravel()]):
+            fo.write("%.17g\n" % v)
+
+
 def tiny_scene():
     """Hand-built: 4 cameras, 6 points; camera 3 sees only points 0 and 1."""
     spec = O.make_spec(3, 6, 2, 3, seed=5)
@@ -225,6 +293,45 @@ def main():
     out["twoview_costs"] = np.array(costs)
     out["twoview_steps"] = np.array(succ)
     out["twoview_termination"] = np.array(term)
+    # 8. Boundary conversions and file formats.
+    Rs = rotation_cases()
+    out["rot_R"] = Rs
+    out["rot_aa"] = O.ref_rotation_matrix_to_angle_axis(Rs)
+    write_bal_scene()
+    import hashlib
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        tmp = os.path.join(td, "rt.txt")
+        O.ref_bal_roundtrip(BAL_FILE, tmp)
+        out["bal_write_sha256"] = np.array(hashlib.sha256(open(tmp, "rb").read()).hexdigest())
+    r = O.ref_bal_solve(BAL_FILE, huber_a=4.0, linear_solver=0, max_iters=20)
+    for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations", "cameras",
+              "points"]:
+        out[f"bal_sparse_{k}"] = np.asarray(r[k])
+    O.ref_set_points_first(True)
+    try:
+        for k, fn in enumerate(LIBMV_FILES):
+            lp = libmv_problem(os.path.join(LIBMV_DIR, fn))
+            out[f"libmv_{k}_aa"] = lp["cam_init"][:, :3].copy()
+            sc = O.group_scene(lp, cam_constant=lp["const"], intr_subset=np.array([LIBMV_SUBSET], np.uint8))
+            ne, order, res, pairs = O.ref_schur_ordering(sc)
+            out[f"libmv_{k}_struct_num_e"] = np.asarray(ne)
+            out[f"libmv_{k}_struct_order"] = order
+            out[f"libmv_{k}_struct_residual_order"] = res.astype(np.int32)
+            # ~10^5 pairs (every camera couples through the few points):
+            # their count and digest instead of the list.
+            out[f"libmv_{k}_struct_num_pairs"] = np.asarray(pairs.shape[0])
+            out[f"libmv_{k}_struct_pairs_sha256"] = np.array(
+                hashlib.sha256(np.ascontiguousarray(pairs, np.int32).tobytes()).hexdigest())
+            sc = O.group_scene(lp, cam_constant=lp["const"], intr_subset=np.array([LIBMV_SUBSET], np.uint8))
+            r = O.ref_solve(sc, huber_a=0.0, linear_solver=0, max_iters=20)
+            for key in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
+                out[f"libmv_{k}_{key}"] = np.asarray(r[key])
+            out[f"libmv_{k}_cameras"] = sc.cameras
+            out[f"libmv_{k}_intrinsics"] = sc.intrinsics
+            out[f"libmv_{k}_points"] = sc.points
+    finally:
+        O.ref_set_points_first(False)
     path = os.path.join(HERE, "ceres_ref.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")
