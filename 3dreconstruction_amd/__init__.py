@@ -35,6 +35,7 @@ CONVERGENCE, NO_CONVERGENCE, FAILURE = 0, 1, 2
 LAYOUT_PACKED9 = 0
 LAYOUT_GROUPED = 1
 ORDER_AUTOMATIC, ORDER_POINTS_FIRST = 0, 1
+PRECOND_JACOBI, PRECOND_SCHUR_JACOBI = 0, 1
 EXPLICIT_SCHUR, IMPLICIT_SCHUR_PCG = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
@@ -76,7 +77,7 @@ class Options(C.Structure):
                 ("min_trust_region_radius", C.c_double), ("min_relative_decrease", C.c_double),
                 ("min_lm_diagonal", C.c_double), ("max_lm_diagonal", C.c_double), ("eta", C.c_double),
                 ("pcg_r_tolerance", C.c_double), ("huber_a", C.c_double),
-                ("refinement_iterations", C.c_int32), ("reserved1", C.c_int32)]
+                ("refinement_iterations", C.c_int32), ("preconditioner", C.c_int32)]
 
 
 class Iteration(C.Structure):

@@ -851,7 +851,7 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
     int nf, const int32_t* __restrict__ f_lo, const int32_t* __restrict__ f_hi, const T* __restrict__ partial,
     const T* __restrict__ cscale, Win win, double min_diag, double max_diag,
     double* __restrict__ diag_c, int recompute_diag, int with_h, T* __restrict__ U, T* __restrict__ gc,
-    T* __restrict__ rhs, T* __restrict__ Minv) {
+    T* __restrict__ rhs, T* __restrict__ Minv, const T* __restrict__ Tsj) {
   __shared__ double sum[64];
   __shared__ int ij[64];
   const int f = blockIdx.x;
@@ -892,12 +892,18 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
   for (int i = 0; i < 9; ++i) {
     double dg;
     if (recompute_diag) {
+      // D from the Jacobian's columns (U's diagonal), before any
+      // SCHUR_JACOBI term.
       dg = fmin(fmax(a[ut9(i, i)], min_diag), max_diag);
       if (k == 0) diag_c[9 * (size_t)f + i] = dg;
     } else {
       dg = diag_c[9 * (size_t)f + i];
     }
     a[ut9(i, i)] += dg * inv_radius;
+  }
+  if (Tsj) {
+#pragma unroll
+    for (int q = 0; q < 45; ++q) a[q] -= (double)Tsj[45 * (size_t)f + q];
   }
   double l[45], e[9];
   if (chol9(a, l)) {
@@ -911,18 +917,104 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
   for (int i = 0; i < 9; ++i) Minv[81 * (size_t)f + 9 * i + k] = (T)e[i];
 }
 
+// SCHUR_JACOBI (schur_jacobi_preconditioner.cc:83-103): per camera-major
+// chunk, the sum over its rows of Jc' (Jp Vinv Jp') Jc (45 values; camera
+// columns unscaled like the camera pass, point columns scaled, Huber weight
+// applied) -- each row's share of W Vinv W' in the camera's diagonal block
+// of S when a camera observes a point once.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sj_pass(
+    const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
+    const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
+    const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
+    const T* __restrict__ pscale, const T* __restrict__ Vinv, const T* __restrict__ ctab, T huber_a,
+    T* __restrict__ partial, Win win) {
+  __shared__ T lds[45 * (kBlock / 64)];
+  const int ch = blockIdx.x;
+  const int f = chunk_f[ch];
+  if (win_idle_f(win, f)) return;
+  const T* tab = ctab + (size_t)kCamTab * f_to_cam[f];
+  CamProj<T> c;
+  load_proj(tab, c);
+  T dR[27];
+#pragma unroll
+  for (int i = 0; i < 27; ++i) dR[i] = tab[kTabDR + i];
+  const T one[9] = {T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1)};
+  T acc[45];
+#pragma unroll
+  for (int i = 0; i < 45; ++i) acc[i] = T(0);
+  for (int i = chunk_begin[ch] + threadIdx.x; i < chunk_end[ch]; i += kBlock) {
+    const int e = cm_e[i];
+    T X[3], unused[3];
+    load_pk(pk, e, X, unused);
+    const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
+    const T* iv = Vinv + 6 * (size_t)e;
+    const T Vi[9] = {iv[0], iv[1], iv[2], iv[1], iv[3], iv[4], iv[2], iv[4], iv[5]};
+    Proj<T> o;
+    project(c, X, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], o);
+    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    T Jc[18], Jp[6];
+    jac_cam(o, dR, X, wgt, one, Jc);
+    jac_point(o, c.R, wgt, sp, Jp);
+    // A = Jp Vinv Jp' (2 x 2), then B = A Jc.
+    T G[6];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        G[3 * r + j] = Jp[3 * r] * Vi[j] + Jp[3 * r + 1] * Vi[3 + j] + Jp[3 * r + 2] * Vi[6 + j];
+    const T a00 = G[0] * Jp[0] + G[1] * Jp[1] + G[2] * Jp[2];
+    const T a01 = G[0] * Jp[3] + G[1] * Jp[4] + G[2] * Jp[5];
+    const T a11 = G[3] * Jp[3] + G[4] * Jp[4] + G[5] * Jp[5];
+    T B0[9], B1[9];
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+      B0[b] = a00 * Jc[b] + a01 * Jc[9 + b];
+      B1[b] = a01 * Jc[b] + a11 * Jc[9 + b];
+    }
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < 9; ++a)
+#pragma unroll
+      for (int b = a; b < 9; ++b) fma2(acc[k++], Jc[a], B0[b], Jc[9 + a], B1[b]);
+  }
+  block_sum<T, 45>(acc, lds);
+  if (threadIdx.x < 45) {
+    T sum = T(0);
+    for (int w = 0; w < kBlock / 64; ++w) sum += lds[w * 45 + threadIdx.x];
+    partial[(size_t)kPartial * ch + threadIdx.x] = sum;
+  }
+}
+
+// Tsj_f = s_a s_b (sum of the F block's SCHUR_JACOBI partials), 45 per block.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sj_finalize(int nf, const int32_t* __restrict__ f_lo,
+                                                        const int32_t* __restrict__ f_hi,
+                                                        const T* __restrict__ partial, const T* __restrict__ cscale,
+                                                        Win win, T* __restrict__ Tsj) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int f = (int)(t / 45), k = (int)(t % 45);
+  if (f >= nf || win_idle_f(win, f)) return;
+  int a = 0;
+  while (a < 8 && k >= ut9(a + 1, a + 1)) ++a;
+  const int b = a + (k - ut9(a, a));
+  double v = 0.0;
+  for (int ch = f_lo[f]; ch < f_hi[f]; ++ch) v += (double)partial[(size_t)kPartial * ch + k];
+  Tsj[45 * (size_t)f + k] = (T)(v * (double)cscale[9 * (size_t)f + a] * (double)cscale[9 * (size_t)f + b]);
+}
+
 // Re-damp the camera preconditioner after a rejected step.
 template <typename T>
 __global__ __launch_bounds__(64) void k_cam_damp(int nf, const T* __restrict__ U,
                                                  const double* __restrict__ diag_c, Win win,
-                                                 T* __restrict__ Minv) {
+                                                 T* __restrict__ Minv, const T* __restrict__ Tsj) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf || win_idle_f(win, f)) return;
   const int wi = win_of_f(win, f);
   const double inv_radius = win.inv_r[wi];
   double a[45];
 #pragma unroll
-  for (int q = 0; q < 45; ++q) a[q] = (double)U[45 * (size_t)f + q];
+  for (int q = 0; q < 45; ++q) a[q] = (double)U[45 * (size_t)f + q] - (Tsj ? (double)Tsj[45 * (size_t)f + q] : 0.0);
 #pragma unroll
   for (int i = 0; i < 9; ++i) a[ut9(i, i)] += diag_c[9 * (size_t)f + i] * inv_radius;
   double l[45];

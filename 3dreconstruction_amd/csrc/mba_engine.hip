@@ -538,6 +538,8 @@ struct Dev {
   // explicit S
   T *S = nullptr, *L = nullptr, *sv = nullptr, *WY = nullptr, *Linv = nullptr, *cross = nullptr;
   T* s_scratch = nullptr;
+  // SCHUR_JACOBI preconditioner terms (IMPLICIT_SCHUR_PCG only)
+  T *partial_sj = nullptr, *Tsj = nullptr;
 };
 
 }  // namespace
@@ -696,6 +698,10 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_st
   MBA_TRY(c->pool.alloc(&d.step_p, 3 * ne));
   MBA_TRY(c->pool.alloc(&d.step_c, 9 * nf));
   MBA_TRY(c->pool.alloc(&d.sv, 9 * nf));
+  if (c->opt.preconditioner == MBA_PRECOND_SCHUR_JACOBI && c->opt.linear_solver == MBA_IMPLICIT_SCHUR_PCG) {
+    MBA_TRY(c->pool.alloc(&d.partial_sj, (size_t)kPartial * std::max(1, c->nchunks)));
+    MBA_TRY(c->pool.alloc(&d.Tsj, 45 * std::max<size_t>(1, nf)));
+  }
   return MBA_OK;
 }
 
@@ -935,6 +941,12 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   for (int32_t w = 0; w < W; ++w)
     if (probs[w].layout == MBA_LAYOUT_GROUPED && W > 1)
       return fail(MBA_ERR_UNSUPPORTED, "grouped-intrinsics problems are solved one context each");
+  if (c->opt.preconditioner != MBA_PRECOND_JACOBI && c->opt.preconditioner != MBA_PRECOND_SCHUR_JACOBI)
+    return fail(MBA_ERR_INVALID_ARGUMENT, "unknown preconditioner " + std::to_string(c->opt.preconditioner));
+  const bool schur_jacobi =
+      c->opt.preconditioner == MBA_PRECOND_SCHUR_JACOBI && c->opt.linear_solver == MBA_IMPLICIT_SCHUR_PCG;
+  if (schur_jacobi && probs[0].layout == MBA_LAYOUT_GROUPED)
+    return fail(MBA_ERR_UNSUPPORTED, "SCHUR_JACOBI is implemented for packed cameras");
   auto prep = [&](int32_t w) {
     const mba_problem* p = &probs[w];
     Structure& st = c->wst[w];
@@ -997,11 +1009,6 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     c->f_off[w + 1] = c->f_off[w] + ((int32_t)st.order.size() - st.num_e);
     c->max_track = std::max(c->max_track, st.max_track);
   }
-  int ndev = 0;
-  HIP_TRY(hipGetDeviceCount(&ndev));
-  if (c->device < 0 || c->device >= ndev) return fail(MBA_ERR_DEVICE, "no such HIP device");
-  HIP_TRY(hipSetDevice(c->device));
-  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->nc = c->cam_off[W];
   c->np = c->pt_off[W];
   c->ne = c->e_off[W];
@@ -1105,6 +1112,20 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       }
     }
   });
+  if (schur_jacobi) {
+    // The per-row W Vinv W' terms are the camera's diagonal block only if
+    // no camera observes a point twice.
+    std::atomic<bool> dup{false};
+    parallel_for(c->ne, [&](int64_t b, int64_t end) {
+      std::vector<int32_t> cams;
+      for (int64_t de = b; de < end && !dup; ++de) {
+        cams.assign(row_cam.begin() + dev_pt_start[de], row_cam.begin() + dev_pt_start[de + 1]);
+        std::sort(cams.begin(), cams.end());
+        if (std::adjacent_find(cams.begin(), cams.end()) != cams.end()) dup = true;
+      }
+    });
+    if (dup) return fail(MBA_ERR_UNSUPPORTED, "SCHUR_JACOBI: a camera observes a point more than once");
+  }
   // Camera-major rows: a stable counting sort of the rows by key with
   // per-thread histograms over row ranges; cm_src[k] is the point-major row
   // of camera-major slot k.  Packed: key = F block (constant cameras
@@ -1172,6 +1193,12 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   }
   key_chunk[nkeys] = (int32_t)chunk_f.size();
   c->nchunks = (int32_t)chunk_f.size();
+  // Host-side checks are done: the device from here on.
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (c->device < 0 || c->device >= ndev) return fail(MBA_ERR_DEVICE, "no such HIP device");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   // Partial rows of each F block: its chunks (packed), or its camera's
   // chunks / its cameras' group rows after the nchunks camera rows.
   std::vector<int32_t> f_lo(c->nf), f_hi(c->nf);
@@ -1447,12 +1474,24 @@ struct Engine {
         hipLaunchKernelGGL(k_cam_pass<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
                            d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial, win());
     }
+    sj_terms();
     if (c->timing) (void)hipEventRecord(c->ev[2], s);
     if (c->nf > 0)
       hipLaunchKernelGGL(k_cam_finalize<T>, dim3(c->nf), dim3(64), 0, s, c->nf, d.f_lo, d.f_hi, d.partial,
                          d.cscale, win(), c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_c,
-                         recompute_diag, 1, d.U, d.gc, d.rhs, d.Minv);
+                         recompute_diag, 1, d.U, d.gc, d.rhs, d.Minv, (const T*)d.Tsj);
     if (c->timing) (void)hipEventRecord(c->ev[3], s);
+  }
+
+  // SCHUR_JACOBI: the W Vinv W' terms of the cameras' diagonal blocks of S
+  // at the current point damping (d.Tsj; NULL with JACOBI).
+  void sj_terms() {
+    if (!d.Tsj || c->nf == 0) return;
+    if (c->nchunks > 0)
+      hipLaunchKernelGGL(k_sj_pass<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin, d.chunk_end,
+                         d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.pscale, d.Vinv, d.ctab, huber, d.partial_sj, win());
+    hipLaunchKernelGGL(k_sj_finalize<T>, dim3(nblocks(45 * (int64_t)c->nf)), dim3(kBlock), 0, s, c->nf, d.f_lo,
+                       d.f_hi, (const T*)d.partial_sj, (const T*)d.cscale, win(), d.Tsj);
   }
 
   // Re-damp after a rejected step (radii current) and rebuild the rhs.
@@ -1460,8 +1499,9 @@ struct Engine {
     hipLaunchKernelGGL(k_point_damp<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.V, d.gp, c->diag_p,
                        win(), d.pscale, d.Vinv, d.pk);
     if (c->nf > 0) {
+      sj_terms();
       hipLaunchKernelGGL(k_cam_damp<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf, d.U, c->diag_c, win(),
-                         d.Minv);
+                         d.Minv, (const T*)d.Tsj);
       mv_cam(nullptr, nullptr);
       hipLaunchKernelGGL((c->grouped ? k_rhs_finalize<T, 1> : k_rhs_finalize<T, 0>), dim3(nblocks(c->nf)),
                          dim3(kBlock), 0, s, c->nf, d.f_lo, d.f_hi, d.partial, d.gc, d.rhs, win());

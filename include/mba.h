@@ -100,6 +100,17 @@ typedef enum mba_linear_solver {
   MBA_IMPLICIT_SCHUR_PCG = 1
 } mba_linear_solver;
 
+/* Preconditioner of IMPLICIT_SCHUR_PCG (Solver::Options::preconditioner_type,
+ * iterative_schur_complement_solver.cc:126-146). */
+typedef enum mba_preconditioner {
+  /* Block diagonal of F'F + D^2 (implicit_schur_complement.cc:71-93). */
+  MBA_PRECOND_JACOBI = 0,
+  /* Exact diagonal blocks of S (schur_jacobi_preconditioner.cc:83-103):
+   * U + D^2 - sum over the camera's rows of W V^-1 W'.  Packed cameras that
+   * observe each point at most once. */
+  MBA_PRECOND_SCHUR_JACOBI = 1
+} mba_preconditioner;
+
 /* Arithmetic of the residual / Jacobian / Schur / PCG kernels.  The LM cost
  * and gain ratio are always FP64; point-block inverses are always FP64.
  * GROUPED problems with MBA_EXPLICIT_SCHUR always run in FP64: an FP32
@@ -176,7 +187,7 @@ typedef struct mba_options {
   /* Explicit Schur: FP64 iterative-refinement sweeps after the (FP32 in
    * MIXED) Cholesky solve of S; 0 disables. */
   int32_t refinement_iterations;              /* 2 */
-  int32_t reserved1;
+  int32_t preconditioner;                     /* mba_preconditioner, JACOBI */
 } mba_options;
 
 typedef struct mba_iteration {

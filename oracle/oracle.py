@@ -356,6 +356,11 @@ def ref_set_points_first(on: bool):
     ref_lib().ref_set_points_first(C.c_int(1 if on else 0))
 
 
+def ref_set_schur_jacobi(on: bool):
+    """Solver::Options::preconditioner_type = SCHUR_JACOBI in ref_solve."""
+    ref_lib().ref_set_schur_jacobi(C.c_int(1 if on else 0))
+
+
 def ref_rotation_matrix_to_angle_axis(R):
     """ceres::RotationMatrixToAngleAxis of (n, 3, 3) row-major matrices."""
     R = np.asarray(R, np.float64).reshape(-1, 3, 3)

@@ -189,6 +189,8 @@ void AddAll(const Scene& s, P* problem, ceres::LossFunction* loss,
 // layout 1 the cameras and intrinsics are moved into one buffer
 // [cameras | intrinsics] (the libs/sfm container order) while solving.
 int g_points_first = 0;
+// Preconditioner of ITERATIVE_SCHUR in ref_solve: 0 JACOBI, 1 SCHUR_JACOBI.
+int g_schur_jacobi = 0;
 
 void AddPointsFirst(const Scene& s, const std::vector<double*>& blocks, ceres::ParameterBlockOrdering* o) {
   for (double* b : blocks) o->AddElementToGroup(b, (b >= s.pts && b < s.pts + 3L * s.np) ? 0 : 1);
@@ -219,6 +221,7 @@ struct ContiguousCamsIntr {
 extern "C" {
 
 void ref_set_points_first(int on) { g_points_first = on; }
+void ref_set_schur_jacobi(int on) { g_schur_jacobi = on; }
 
 // ceres::RotationMatrixToAngleAxis on n column-major matrices.
 void ref_rotation_matrix_to_angle_axis(const double* R, double* aa, long n) {
@@ -390,6 +393,7 @@ int ref_solve(int layout, int nc, int ni, int np, long nobs, double* cams, doubl
   ceres::LossFunction* loss = huber_a > 0.0 ? new ceres::HuberLoss(huber_a) : NULL;
   AddAll(s, &problem, loss, NULL);
   ceres::Solver::Options options;
+  if (g_schur_jacobi) options.preconditioner_type = ceres::SCHUR_JACOBI;
   if (g_points_first) {
     std::vector<double*> blocks;
     problem.GetParameterBlocks(&blocks);
@@ -397,7 +401,7 @@ int ref_solve(int layout, int nc, int ni, int np, long nobs, double* cams, doubl
     AddPointsFirst(s, blocks, o);
     options.linear_solver_ordering.reset(o);
   }
-  options.preconditioner_type = ceres::JACOBI;
+  options.preconditioner_type = g_schur_jacobi ? ceres::SCHUR_JACOBI : ceres::JACOBI;
   options.linear_solver_type = linear_solver == 1 ? ceres::ITERATIVE_SCHUR
                                : linear_solver == 2 ? ceres::DENSE_SCHUR
                                                     : ceres::SPARSE_SCHUR;

@@ -332,6 +332,17 @@ def main():
             out[f"libmv_{k}_points"] = sc.points
     finally:
         O.ref_set_points_first(False)
+    # 9. ITERATIVE_SCHUR with SCHUR_JACOBI (exact diagonal blocks of S).
+    O.ref_set_schur_jacobi(True)
+    try:
+        for name, d, huber, cc in [("c1_schur_jacobi", d1, 0.0, None), ("huber_schur_jacobi", dh, 4.0, const)]:
+            sc = O.Scene(d, cam_constant=cc)
+            r = O.ref_solve(sc, huber_a=huber, linear_solver=1, max_iters=20)
+            for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
+                out[f"{name}_{k}"] = np.asarray(r[k])
+            out[f"{name}_cameras"] = sc.cameras
+    finally:
+        O.ref_set_schur_jacobi(False)
     path = os.path.join(HERE, "ceres_ref.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")

@@ -225,3 +225,23 @@ def test_ceres_shaped_grouped_problem_builds_the_same_arrays(mba, group_scene):
         mba.SubsetParameterization(6, [0, 1, 2, 3, 4, 5])
     with pytest.raises(ValueError):
         prob.AddResidualBlock(mba.ReprojectionError(d["obs"][0]), None, np.zeros(9), pts[0])
+
+
+def test_schur_jacobi_requirements(mba, c1, group_scene):
+    """SCHUR_JACOBI is for packed cameras observing each point once; both
+    are checked on the host before any device work."""
+    d, _ = group_scene
+    opts = mba.default_options(linear_solver=mba.IMPLICIT_SCHUR_PCG, preconditioner=mba.PRECOND_SCHUR_JACOBI)
+    with pytest.raises(mba.MbaError) as e:
+        mba.solve(grouped_problem(mba, d), opts)
+    assert e.value.status == mba.ERR_UNSUPPORTED
+    p = ba_problem(mba, c1)
+    obs = np.vstack([p.observations, p.observations[:1]])
+    dup = mba.BAProblem(p.cameras, p.points, obs, np.append(p.camera_index, p.camera_index[0]),
+                        np.append(p.point_index, p.point_index[0]))
+    with pytest.raises(mba.MbaError) as e:
+        mba.solve(dup, opts)
+    assert e.value.status == mba.ERR_UNSUPPORTED and "more than once" in str(e.value)
+    with pytest.raises(mba.MbaError) as e:
+        mba.solve(ba_problem(mba, c1), mba.default_options(preconditioner=7))
+    assert e.value.status == mba.ERR_INVALID_ARGUMENT
