@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -57,6 +58,60 @@ double now_s() {
 
 inline int nblocks(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
 
+// Host buffers that are fully written before they are read: default-
+// initialised (no zero fill of tens of MB on the preprocessing path).
+template <typename T>
+struct NoInit : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInit<U>;
+  };
+  NoInit() = default;
+  template <typename U>
+  NoInit(const NoInit<U>&) noexcept {}
+  template <typename U>
+  void construct(U* q) noexcept {
+    ::new ((void*)q) U;
+  }
+  template <typename U, typename... A>
+  void construct(U* q, A&&... a) {
+    ::new ((void*)q) U(std::forward<A>(a)...);
+  }
+};
+template <typename T>
+using Buf = std::vector<T, NoInit<T>>;
+
+// Host threads for n items (at most 16, about 64k items each).
+int host_threads(int64_t n, int64_t grain = 1 << 16) {
+  const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+  return (int)std::max<int64_t>(1, std::min<int64_t>({std::min<int64_t>(16, hw), (n + grain - 1) / grain}));
+}
+
+// fn(t) for t in [0, nt) on nt threads (inline when nt == 1).
+template <typename F>
+void run_threads(int nt, F&& fn) {
+  if (nt <= 1) {
+    fn(0);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; ++t) th.emplace_back([&fn, t]() { fn(t); });
+  for (std::thread& x : th) x.join();
+}
+
+// fn(begin, end) over equal ranges of [0, n).
+template <typename F>
+void parallel_for(int64_t n, F&& fn) {
+  if (n <= 0) return;
+  const int nt = host_threads(n);
+  const int64_t step = (n + nt - 1) / nt;
+  run_threads(nt, [&](int t) {
+    const int64_t b = t * step, e = std::min(n, b + step);
+    if (b < e) fn(b, e);
+  });
+}
+
 // ---------------------------------------------------------------- ordering
 // The reference's preprocessing for a (camera, point) bundle problem:
 //   program order = first appearance in AddResidualBlock argument lists
@@ -76,7 +131,7 @@ struct Structure {
   // groups g, cameras ng + c, points ng + nc + p.
   std::vector<int32_t> order;
   int32_t num_e = 0;
-  std::vector<int64_t> res_order;  // row -> input observation
+  Buf<int64_t> res_order;          // row -> input observation
   bool standard = false;           // E == active points
   // Derived (standard structure only).  f_to_cam: the F block's parameter
   // slot block (packed: its camera; grouped: camera c at c, group g at
@@ -84,6 +139,24 @@ struct Structure {
   std::vector<int32_t> e_to_pt, f_to_cam, cam_to_f, pt_start;
   std::vector<int32_t> cam_grp;    // grouped: the group of each camera (-1: none)
   int32_t max_track = 0;
+};
+
+// Phase timer of the host preprocessing: with MBA_TRACE_PREP=1 in the
+// environment, context_create prints each phase's wall time to stderr.
+struct PrepTrace {
+  bool on = false;
+  double t = 0.0;
+  PrepTrace() {
+    const char* e = std::getenv("MBA_TRACE_PREP");
+    on = e && e[0] == '1';
+    t = now_s();
+  }
+  void mark(const char* what) {
+    if (!on) return;
+    const double n = now_s();
+    std::fprintf(stderr, "mba prep: %-28s %8.2f ms\n", what, 1e3 * (n - t));
+    t = n;
+  }
 };
 
 mba_status validate(const mba_problem* p) {
@@ -151,6 +224,9 @@ void user_points_first(const std::vector<int32_t>& active, int32_t first_point, 
   st->order.insert(st->order.end(), rest.begin(), rest.end());
 }
 
+// Relaxed atomic add on a plain int (host threads).
+inline int32_t atomic_inc(int32_t* v) { return __atomic_fetch_add(v, 1, __ATOMIC_RELAXED); }
+
 mba_status build_structure(const mba_problem* p, Structure* st) {
   MBA_TRY(validate(p));
   if (p->layout == MBA_LAYOUT_GROUPED) return build_structure_grouped(p, st);
@@ -163,7 +239,8 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
   auto is_const = [&](int32_t b) -> bool {
     return b < nc && p->camera_constant != nullptr && p->camera_constant[b] != 0;
   };
-  // First appearance.
+  PrepTrace tr;
+  // First appearance (program order, one pass).
   std::vector<int32_t> prog;
   prog.reserve(nb);
   std::vector<uint8_t> seen(nb, 0);
@@ -172,70 +249,107 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
     if (!seen[a]) { seen[a] = 1; prog.push_back(a); }
     if (!seen[b]) { seen[b] = 1; prog.push_back(b); }
   }
-  // Edges of the Hessian graph: distinct (camera, point) pairs between
-  // non-constant blocks.  CSR of observations by camera, marker per point.
-  std::vector<int64_t> cstart(nc + 1, 0);
-  for (int64_t i = 0; i < no; ++i) cstart[p->camera_index[i] + 1]++;
-  for (int32_t c = 0; c < nc; ++c) cstart[c + 1] += cstart[c];
-  std::vector<int32_t> cobs_pt(no);
+  // CSR of observations by camera: per-thread histograms over observation
+  // ranges (the cameras are few), then each thread fills its own slots.
+  std::vector<int64_t> cstart((size_t)nc + 1, 0);
+  Buf<int32_t> cobs_pt(no);
   {
-    std::vector<int64_t> fill(cstart.begin(), cstart.end() - 1);
-    for (int64_t i = 0; i < no; ++i) cobs_pt[fill[p->camera_index[i]]++] = p->point_index[i];
-  }
-  std::vector<int32_t> deg(nb, 0);
-  std::vector<int32_t> mark(np, -1);
-  // Adjacency CSR for both sides (deduplicated).
-  std::vector<int64_t> cadj_start(nc + 1, 0);
-  std::vector<int32_t> cadj;
-  cadj.reserve(no);
-  for (int32_t c = 0; c < nc; ++c) {
-    cadj_start[c] = (int64_t)cadj.size();
-    if (is_const(c)) continue;
-    for (int64_t k = cstart[c]; k < cstart[c + 1]; ++k) {
-      const int32_t q = cobs_pt[k];
-      if (mark[q] == c) continue;
-      mark[q] = c;
-      cadj.push_back(q);
+    const int nt = host_threads(no);
+    const int64_t step = (no + nt - 1) / std::max(1, nt);
+    std::vector<std::vector<int64_t>> h(nt, std::vector<int64_t>((size_t)nc, 0));
+    run_threads(nt, [&](int t) {
+      const int64_t e = std::min<int64_t>(no, (t + 1) * step);
+      for (int64_t i = t * step; i < e; ++i) h[t][p->camera_index[i]]++;
+    });
+    int64_t run = 0;
+    for (int32_t c = 0; c < nc; ++c) {
+      cstart[c] = run;
+      for (int t = 0; t < nt; ++t) {
+        const int64_t k = h[t][c];
+        h[t][c] = run;
+        run += k;
+      }
     }
+    cstart[nc] = run;
+    run_threads(nt, [&](int t) {
+      const int64_t e = std::min<int64_t>(no, (t + 1) * step);
+      for (int64_t i = t * step; i < e; ++i) cobs_pt[h[t][p->camera_index[i]]++] = p->point_index[i];
+    });
   }
-  cadj_start[nc] = (int64_t)cadj.size();
-  std::vector<int64_t> padj_start(np + 1, 0);
-  for (int32_t c = 0; c < nc; ++c)
-    for (int64_t k = cadj_start[c]; k < cadj_start[c + 1]; ++k) padj_start[cadj[k] + 1]++;
-  for (int32_t q = 0; q < np; ++q) padj_start[q + 1] += padj_start[q];
-  std::vector<int32_t> padj(cadj.size());
+  tr.mark("  first appearance + CSR");
+  // Hessian-graph edges: the distinct points of every non-constant camera
+  // (cameras in parallel, a point marker per thread), compacted into cadj;
+  // a point's degree counts the cameras that list it.
+  // cadj: camera c's distinct points at [cstart[c], cstart[c] + deg[c]).
+  std::vector<int32_t> deg(nb, 0);
+  Buf<int32_t> cadj(no);
   {
-    std::vector<int64_t> fill(padj_start.begin(), padj_start.end() - 1);
-    for (int32_t c = 0; c < nc; ++c)
-      for (int64_t k = cadj_start[c]; k < cadj_start[c + 1]; ++k) padj[fill[cadj[k]]++] = c;
+    const int nt = host_threads(no);
+    std::atomic<int32_t> next{0};
+    run_threads(nt, [&](int) {
+      std::vector<int32_t> mark(np, -1);
+      for (int32_t c = next.fetch_add(1); c < nc; c = next.fetch_add(1)) {
+        if (is_const(c)) continue;
+        int64_t o = cstart[c];
+        for (int64_t k = cstart[c]; k < cstart[c + 1]; ++k) {
+          const int32_t q = cobs_pt[k];
+          if (mark[q] == c) continue;
+          mark[q] = c;
+          cadj[o++] = q;
+          atomic_inc(&deg[nc + q]);
+        }
+        deg[c] = (int32_t)(o - cstart[c]);
+      }
+    });
   }
-  for (int32_t c = 0; c < nc; ++c) deg[c] = (int32_t)(cadj_start[c + 1] - cadj_start[c]);
-  for (int32_t q = 0; q < np; ++q) deg[nc + q] = (int32_t)(padj_start[q + 1] - padj_start[q]);
+  tr.mark("  adjacency");
   // Active vertices in program order, stable counting sort by degree.
   std::vector<int32_t> active;
   active.reserve(prog.size());
   for (int32_t b : prog)
     if (!is_const(b)) active.push_back(b);
-  int32_t maxdeg = 0;
-  for (int32_t b : active) maxdeg = std::max(maxdeg, deg[b]);
+  int32_t maxdeg = 0, max_pt = 0, min_cam = std::numeric_limits<int32_t>::max();
+  for (int32_t b : active) {
+    maxdeg = std::max(maxdeg, deg[b]);
+    if (b < nc) min_cam = std::min(min_cam, deg[b]);
+    else max_pt = std::max(max_pt, deg[b]);
+  }
   std::vector<int64_t> cnt((size_t)maxdeg + 2, 0);
   for (int32_t b : active) cnt[deg[b] + 1]++;
   for (int32_t d = 0; d <= maxdeg; ++d) cnt[d + 1] += cnt[d];
   std::vector<int32_t> queue(active.size());
   for (int32_t b : active) queue[cnt[deg[b]]++] = b;
+  tr.mark("  degree sort");
   st->order.clear();
   st->order.reserve(active.size());
   if (p->ordering == MBA_ORDER_POINTS_FIRST) {
     user_points_first(active, nc, st);
+  } else if (max_pt < min_cam) {
+    // Every point precedes every camera in the queue: each point is still
+    // white when reached (only cameras neighbour it) and every camera is
+    // then grey, so the greedy set is the points in queue order.
+    for (int32_t b : queue)
+      if (b >= nc) st->order.push_back(b);
+    st->num_e = (int32_t)st->order.size();
+    for (int32_t b : queue)
+      if (b < nc) st->order.push_back(b);
   } else {
-    // Greedy independent set.
+    // Greedy independent set over both sides of the adjacency.
+    std::vector<int64_t> padj_start((size_t)np + 1, 0);
+    for (int32_t q = 0; q < np; ++q) padj_start[q + 1] = padj_start[q] + deg[nc + q];
+    std::vector<int32_t> padj((size_t)padj_start[np]);
+    {
+      std::vector<int64_t> fill(padj_start.begin(), padj_start.end() - 1);
+      for (int32_t c = 0; c < nc; ++c)
+        for (int64_t k = cstart[c]; k < cstart[c] + deg[c]; ++k) padj[fill[cadj[k]]++] = c;
+    }
     std::vector<uint8_t> color(nb, 0);  // 0 white, 1 grey, 2 black
     for (int32_t b : queue) {
       if (color[b] != 0) continue;
       st->order.push_back(b);
       color[b] = 2;
       if (b < nc) {
-        for (int64_t k = cadj_start[b]; k < cadj_start[b + 1]; ++k) color[nc + cadj[k]] = 1;
+        for (int64_t k = cstart[b]; k < cstart[b] + deg[b]; ++k) color[nc + cadj[k]] = 1;
       } else {
         const int32_t q = b - nc;
         for (int64_t k = padj_start[q]; k < padj_start[q + 1]; ++k) color[padj[k]] = 1;
@@ -245,28 +359,59 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
     for (int32_t b : queue)
       if (color[b] != 2) st->order.push_back(b);
   }
+  tr.mark("  independent set");
   // Column-block index of every block.
   std::vector<int32_t> col(nb, -1);
   for (size_t i = 0; i < st->order.size(); ++i) col[st->order[i]] = (int32_t)i;
   // Residual order: all rows have a non-constant block (points are never
-  // constant); bucket = smallest column block index.
-  std::vector<int64_t> hist((size_t)st->num_e + 1, 0);
-  std::vector<int32_t> bucket(no);
-  for (int64_t i = 0; i < no; ++i) {
-    const int32_t a = p->camera_index[i], b = nc + p->point_index[i];
-    int32_t m = st->num_e;
-    if (!is_const(a)) m = std::min(m, col[a]);
-    m = std::min(m, col[b]);
-    bucket[i] = m;
-    hist[m]++;
+  // constant); bucket = smallest column block index; buckets in column
+  // order, each filled back to front (descending observation index).  The
+  // buckets are counted and filled in parallel, then each bucket is sorted.
+  const int32_t ne = st->num_e;
+  Buf<int32_t> bucket(no);
+  std::vector<int32_t> hist((size_t)ne + 1, 0);
+  parallel_for(no, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const int32_t a = p->camera_index[i], q = nc + p->point_index[i];
+      int32_t m = ne;
+      if (!is_const(a)) m = std::min(m, col[a]);
+      m = std::min(m, col[q]);
+      bucket[i] = m;
+    }
+  });
+  // Each thread owns a range of buckets: it counts them over all
+  // observations, then fills them scanning the observations downwards.
+  std::vector<int64_t> bstart((size_t)ne + 2, 0);
+  st->res_order.resize(no);
+  {
+    const int nt = host_threads(no);
+    const int64_t span = ((int64_t)ne + 1 + nt - 1) / std::max(1, nt);
+    run_threads(nt, [&](int t) {
+      const int32_t b0 = (int32_t)std::min<int64_t>(ne + 1, t * span), b1 = (int32_t)std::min<int64_t>(ne + 1, (t + 1) * span);
+      // Branch-free: buckets outside the range count into a spare slot.
+      const uint32_t w = (uint32_t)(b1 - b0);
+      std::vector<int32_t> local((size_t)w + 1, 0);
+      for (int64_t i = 0; i < no; ++i) {
+        const uint32_t k = (uint32_t)(bucket[i] - b0);
+        local[k < w ? k : w]++;
+      }
+      std::copy(local.begin(), local.begin() + w, hist.begin() + b0);
+    });
+    for (int32_t e = 0; e <= ne; ++e) bstart[e + 1] = bstart[e] + hist[e];
+    run_threads(nt, [&](int t) {
+      const int32_t b0 = (int32_t)std::min<int64_t>(ne + 1, t * span), b1 = (int32_t)std::min<int64_t>(ne + 1, (t + 1) * span);
+      if (b0 >= b1) return;
+      std::vector<int64_t> pos(bstart.begin() + b0, bstart.begin() + b1);
+      for (int64_t i = no - 1; i >= 0; --i) {
+        const int32_t m = bucket[i];
+        if (m >= b0 && m < b1) st->res_order[pos[m - b0]++] = i;
+      }
+    });
   }
-  for (int32_t e = 1; e <= st->num_e; ++e) hist[e] += hist[e - 1];
-  st->res_order.assign(no, 0);
-  for (int64_t i = 0; i < no; ++i) st->res_order[--hist[bucket[i]]] = i;
-
+  tr.mark("  residual order");
   // Standard BA structure: E = every active point, F = every active camera.
-  int32_t n_act_pts = 0, n_act_cams = 0;
-  for (int32_t b : active) (b < nc ? n_act_cams : n_act_pts)++;
+  int32_t n_act_pts = 0;
+  for (int32_t b : active) n_act_pts += b >= nc;
   bool ok = (st->num_e == n_act_pts);
   for (int32_t i = 0; ok && i < st->num_e; ++i) ok = st->order[i] >= nc;
   st->standard = ok;
@@ -280,13 +425,13 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
     st->f_to_cam[i] = st->order[st->num_e + i];
     st->cam_to_f[st->f_to_cam[i]] = i;
   }
+  // Every row's bucket is its point's E block, so the point-major row
+  // ranges are the buckets.
   st->pt_start.assign((size_t)st->num_e + 1, 0);
-  for (int64_t r = 0; r < no; ++r) st->pt_start[bucket[st->res_order[r]] + 1]++;
-  for (int32_t e = 0; e < st->num_e; ++e) st->pt_start[e + 1] += st->pt_start[e];
+  for (int32_t e = 0; e <= st->num_e; ++e) st->pt_start[e] = (int32_t)bstart[e];
   st->max_track = 0;
-  for (int32_t e = 0; e < st->num_e; ++e)
-    st->max_track = std::max(st->max_track, st->pt_start[e + 1] - st->pt_start[e]);
-  (void)n_act_cams;
+  for (int32_t e = 0; e < st->num_e; ++e) st->max_track = std::max(st->max_track, hist[e]);
+  tr.mark("  E/F maps");
   return MBA_OK;
 }
 
@@ -632,8 +777,8 @@ template <> Dev<float>& dev<float>(mba_context* c) { return c->df; }
 template <> Dev<double>& dev<double>(mba_context* c) { return c->dd; }
 
 // Copy a host vector to a new device allocation.
-template <typename X>
-mba_status upload(mba_context* c, X** out, const std::vector<X>& v) {
+template <typename X, typename A>
+mba_status upload(mba_context* c, X** out, const std::vector<X, A>& v) {
   MBA_TRY(c->pool.alloc(out, v.size()));
   if (!v.empty()) HIP_TRY(hipMemcpyAsync(*out, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice, c->stream));
   return MBA_OK;
@@ -641,7 +786,7 @@ mba_status upload(mba_context* c, X** out, const std::vector<X>& v) {
 
 template <typename T>
 mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_start,
-                           const std::vector<int32_t>& row_cam, const std::vector<int32_t>& cm_src,
+                           const Buf<int32_t>& row_cam, const Buf<int32_t>& cm_src,
                            const std::vector<int32_t>& chunk_f, const std::vector<int32_t>& chunk_begin,
                            const std::vector<int32_t>& chunk_end, const std::vector<int32_t>& f_lo,
                            const std::vector<int32_t>& f_hi) {
@@ -713,37 +858,6 @@ int chunk_rows_for(int64_t nrows) {
   return (int)((c + 63) / 64 * 64);
 }
 
-// Host threads for n items (at most 16, about 64k items each).
-int host_threads(int64_t n, int64_t grain = 1 << 16) {
-  const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
-  return (int)std::max<int64_t>(1, std::min<int64_t>({std::min<int64_t>(16, hw), (n + grain - 1) / grain}));
-}
-
-// fn(t) for t in [0, nt) on nt threads (inline when nt == 1).
-template <typename F>
-void run_threads(int nt, F&& fn) {
-  if (nt <= 1) {
-    fn(0);
-    return;
-  }
-  std::vector<std::thread> th;
-  th.reserve(nt);
-  for (int t = 0; t < nt; ++t) th.emplace_back([&fn, t]() { fn(t); });
-  for (std::thread& x : th) x.join();
-}
-
-// fn(begin, end) over equal ranges of [0, n).
-template <typename F>
-void parallel_for(int64_t n, F&& fn) {
-  if (n <= 0) return;
-  const int nt = host_threads(n);
-  const int64_t step = (n + nt - 1) / nt;
-  run_threads(nt, [&](int t) {
-    const int64_t b = t * step, e = std::min(n, b + step);
-    if (b < e) fn(b, e);
-  });
-}
-
 void context_free(mba_context* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
@@ -771,7 +885,7 @@ std::string window_prefix(int32_t W, int32_t w) {
 // its reduced matrix with (schur_complement_solver.cc:246-313); here it also
 // fixes who computes each block.
 mba_status build_s_pairs(mba_context* c, const std::vector<int32_t>& pt_start,
-                         const std::vector<int32_t>& row_cam) {
+                         const Buf<int32_t>& row_cam) {
   const int32_t nf = c->nf, ne = c->ne;
   // Virtual rows: one per observation (packed), two in the grouped layout
   // (its camera block at r, its group block at nrows + r).
@@ -947,6 +1061,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       c->opt.preconditioner == MBA_PRECOND_SCHUR_JACOBI && c->opt.linear_solver == MBA_IMPLICIT_SCHUR_PCG;
   if (schur_jacobi && probs[0].layout == MBA_LAYOUT_GROUPED)
     return fail(MBA_ERR_UNSUPPORTED, "SCHUR_JACOBI is implemented for packed cameras");
+  PrepTrace trace;
   auto prep = [&](int32_t w) {
     const mba_problem* p = &probs[w];
     Structure& st = c->wst[w];
@@ -995,6 +1110,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       });
     for (std::thread& t : pool) t.join();
   }
+  trace.mark("ordering (per window)");
   int64_t nrows = 0;
   for (int32_t w = 0; w < W; ++w) {
     if (wstat[w] != MBA_OK) return fail(wstat[w], window_prefix(W, w) + werr[w]);
@@ -1079,6 +1195,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       dev_pt_start[de + 1] = dev_pt_start[de] + track_len(ge);
     }
   }
+  trace.mark("device point order");
   // F blocks: window-contiguous, each window in its reference F order.
   c->f_to_cam.resize(c->nf);
   c->cam_to_f.assign(c->nslot, -1);
@@ -1096,8 +1213,8 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   }
   // Rows in device order (each point's rows keep its window's reference
   // order); camera indices are global.  Points are independent.
-  std::vector<int32_t> row_cam(c->nrows);
-  std::vector<double> row_obs(2 * (size_t)c->nrows);
+  Buf<int32_t> row_cam(c->nrows);
+  Buf<double> row_obs(2 * (size_t)c->nrows);
   parallel_for(c->ne, [&](int64_t b, int64_t end) {
     for (int64_t de = b; de < end; ++de) {
       const int32_t ge = dperm[de], w = ge_win[ge], e = ge_local[ge];
@@ -1112,6 +1229,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       }
     }
   });
+  trace.mark("rows");
   if (schur_jacobi) {
     // The per-row W Vinv W' terms are the camera's diagonal block only if
     // no camera observes a point twice.
@@ -1147,7 +1265,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   }
   auto row_key = [&](int32_t cam) { return c->grouped ? key_of_cam[cam] : c->cam_to_f[cam]; };
   std::vector<int64_t> fstart((size_t)nkeys + 1, 0);
-  std::vector<int32_t> cm_src;
+  Buf<int32_t> cm_src;
   {
     const int nt = host_threads(c->nrows);
     const int64_t step = ((int64_t)c->nrows + nt - 1) / nt;
@@ -1181,6 +1299,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       }
     });
   }
+  trace.mark("camera-major sort");
   const int cr = chunk_rows_for(c->ncm);
   std::vector<int32_t> chunk_f, chunk_begin, chunk_end, key_chunk((size_t)nkeys + 1, 0);
   for (int32_t f = 0; f < nkeys; ++f) {
@@ -1237,6 +1356,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     MBA_TRY(upload(c, &c->g_hi, ghi));
     MBA_TRY(upload(c, &c->g_out, gout));
   }
+  trace.mark("chunks + device setup");
   // FP64 state: cameras (slot blocks), then intrinsic groups (grouped).
   std::vector<double> pts(3 * (size_t)c->ne), cams(9 * (size_t)c->nslot, 0.0);
   parallel_for(c->ne, [&](int64_t b, int64_t end) {
@@ -1331,13 +1451,16 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   MBA_TRY(c->pool.alloc(&c->b64, 9 * (size_t)c->nf));
   MBA_TRY(c->pool.alloc(&c->x64, 9 * (size_t)c->nf));
   MBA_TRY(c->pool.alloc(&c->r64, 9 * (size_t)c->nf));
+  trace.mark("state + uploads");
   if (c->mixed)
     MBA_TRY(alloc_precision<float>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_lo, f_hi));
   else
     MBA_TRY(alloc_precision<double>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_lo, f_hi));
   if (W == 1 && c->opt.linear_solver == MBA_EXPLICIT_SCHUR) MBA_TRY(build_s_pairs(c, dev_pt_start, row_cam));
+  trace.mark("precision layout + S pairs");
   for (int i = 0; i < 8; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  trace.mark("drain");
   *out = c;
   guard.c = nullptr;
   return MBA_OK;
