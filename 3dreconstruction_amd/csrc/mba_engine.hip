@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -102,9 +103,9 @@ void run_threads(int nt, F&& fn) {
 
 // fn(begin, end) over equal ranges of [0, n).
 template <typename F>
-void parallel_for(int64_t n, F&& fn) {
+void parallel_for(int64_t n, F&& fn, int64_t grain = 1 << 16) {
   if (n <= 0) return;
-  const int nt = host_threads(n);
+  const int nt = host_threads(n, grain);
   const int64_t step = (n + nt - 1) / nt;
   run_threads(nt, [&](int t) {
     const int64_t b = t * step, e = std::min(n, b + step);
@@ -1154,20 +1155,40 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   // each point-major launch segment is contiguous.
   std::vector<int32_t> dperm(c->ne), dev_pt_start((size_t)c->ne + 1, 0);
   {
-    std::vector<int32_t> cls(c->ne);
-    int32_t ncls[8] = {0};
-    for (int32_t e = 0; e < c->ne; ++e) {
-      const int t = track_len(e);
-      int k = 0;
-      while (k < 6 && t > 4 * (1 << k)) ++k;
-      cls[e] = k;
-      ncls[k]++;
-    }
-    int32_t start[8] = {0};
+    // Stable counting sort by class, per-thread counts over E ranges.
+    Buf<uint8_t> cls(c->ne);
+    const int nt = host_threads(c->ne, 1 << 14);
+    const int64_t step = ((int64_t)c->ne + nt - 1) / nt;
+    std::vector<std::array<int32_t, 8>> tc(nt);
+    run_threads(nt, [&](int t) {
+      tc[t].fill(0);
+      const int64_t end = std::min<int64_t>(c->ne, (t + 1) * step);
+      for (int64_t e = t * step; e < end; ++e) {
+        const int tl = track_len((int32_t)e);
+        int k = 0;
+        while (k < 6 && tl > 4 * (1 << k)) ++k;
+        cls[e] = (uint8_t)k;
+        tc[t][k]++;
+      }
+    });
+    int32_t ncls[8] = {0}, start[8] = {0};
+    for (int t = 0; t < nt; ++t)
+      for (int k = 0; k < 8; ++k) ncls[k] += tc[t][k];
     for (int k = 1; k < 8; ++k) start[k] = start[k - 1] + ncls[k - 1];
-    int32_t fill[8];
-    std::memcpy(fill, start, sizeof(fill));
-    for (int32_t e = 0; e < c->ne; ++e) dperm[fill[cls[e]]++] = e;
+    {
+      int32_t run[8];
+      std::memcpy(run, start, sizeof(run));
+      for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < 8; ++k) {
+          const int32_t n = tc[t][k];
+          tc[t][k] = run[k];
+          run[k] += n;
+        }
+    }
+    run_threads(nt, [&](int t) {
+      const int64_t end = std::min<int64_t>(c->ne, (t + 1) * step);
+      for (int64_t e = t * step; e < end; ++e) dperm[tc[t][cls[e]]++] = (int32_t)e;
+    });
     c->seg.n = 0;
     int32_t blk = 0;
     for (int k = 0; k < 7; ++k) {
@@ -1189,11 +1210,14 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     }
     c->seg_blocks = blk;
     c->dev_e_to_pt.resize(c->ne);
-    for (int32_t de = 0; de < c->ne; ++de) {
-      const int32_t ge = dperm[de], w = ge_win[ge];
-      c->dev_e_to_pt[de] = c->pt_off[w] + c->wst[w].e_to_pt[ge_local[ge]];
-      dev_pt_start[de + 1] = dev_pt_start[de] + track_len(ge);
-    }
+    parallel_for(c->ne, [&](int64_t b, int64_t end) {
+      for (int64_t de = b; de < end; ++de) {
+        const int32_t ge = dperm[de], w = ge_win[ge];
+        c->dev_e_to_pt[de] = c->pt_off[w] + c->wst[w].e_to_pt[ge_local[ge]];
+        dev_pt_start[de + 1] = track_len(ge);
+      }
+    }, 1 << 14);
+    for (int32_t de = 0; de < c->ne; ++de) dev_pt_start[de + 1] += dev_pt_start[de];
   }
   trace.mark("device point order");
   // F blocks: window-contiguous, each window in its reference F order.
@@ -1228,7 +1252,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
         row_obs[2 * (size_t)r + 1] = p->observations[2 * i + 1];
       }
     }
-  });
+  }, 1 << 14);
   trace.mark("rows");
   if (schur_jacobi) {
     // The per-row W Vinv W' terms are the camera's diagonal block only if
