@@ -107,12 +107,14 @@ __device__ __forceinline__ void fma2(T& acc, T a, T b, T c, T d) {
 struct Win {
   const int32_t* of_e;
   const int32_t* of_f;
-  const double* inv_r;
+  const double* inv_r;      // 1 / radius per window, or NULL: inv_r0 (single problem)
+  double inv_r0;
   double* acc;
   int32_t* flag;            // per-window "non-finite / failed factorisation" flags
   const uint8_t* active;    // windows still iterating (nullptr: all)
   const PcgState* pcg;      // inside PCG: windows whose solve has stopped are idle
 };
+__device__ __forceinline__ double win_inv_r(const Win& w, int wi) { return w.inv_r ? w.inv_r[wi] : w.inv_r0; }
 __device__ __forceinline__ int win_of_e(const Win& w, int e) { return w.of_e ? w.of_e[e] : 0; }
 __device__ __forceinline__ int win_of_f(const Win& w, int f) { return w.of_f ? w.of_f[f] : 0; }
 __device__ __forceinline__ bool win_idle(const Win& w, int wi) {
@@ -713,7 +715,7 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
     }
   }
   const int wi = win_of_e(win, e);
-  const double inv_radius = win.inv_r[wi];
+  const double inv_radius = win_inv_r(win, wi);
   double a[6] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3], (double)v[4], (double)v[5]};
   a[0] += dg[0] * inv_radius;
   a[3] += dg[1] * inv_radius;
@@ -742,7 +744,7 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restri
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne || win_idle_e(win, e)) return;
   const int wi = win_of_e(win, e);
-  const double inv_radius = win.inv_r[wi];
+  const double inv_radius = win_inv_r(win, wi);
   double a[6];
   for (int i = 0; i < 6; ++i) a[i] = (double)V[6 * (size_t)e + i];
   a[0] += diag_p[3 * (size_t)e] * inv_radius;
@@ -884,7 +886,7 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
   // column of its inverse.
   if (k >= 9) return;
   const int wi = win_of_f(win, f);
-  const double inv_radius = win.inv_r[wi];
+  const double inv_radius = win_inv_r(win, wi);
   double a[45];
 #pragma unroll
   for (int q = 0; q < 45; ++q) a[q] = sum[q];
@@ -1011,7 +1013,7 @@ __global__ __launch_bounds__(64) void k_cam_damp(int nf, const T* __restrict__ U
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf || win_idle_f(win, f)) return;
   const int wi = win_of_f(win, f);
-  const double inv_radius = win.inv_r[wi];
+  const double inv_radius = win_inv_r(win, wi);
   double a[45];
 #pragma unroll
   for (int q = 0; q < 45; ++q) a[q] = (double)U[45 * (size_t)f + q] - (Tsj ? (double)Tsj[45 * (size_t)f + q] : 0.0);
@@ -1199,7 +1201,7 @@ __global__ __launch_bounds__(kBlock) void k_mv_finalize(
   if (all_done && *all_done) return;
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf || win_idle_f(win, f)) return;
-  const double inv_radius = win.inv_r[win_of_f(win, f)];
+  const double inv_radius = win_inv_r(win, win_of_f(win, f));
   T s[9];
   for (int k = 0; k < 9; ++k) s[k] = T(0);
   for (int ch = f_lo[f]; ch < f_hi[f]; ++ch)
@@ -1442,7 +1444,8 @@ __device__ __forceinline__ void pcg_alpha(const double* __restrict__ partial, in
 // device-wide all-done flag.  `first`: the initial call (x = 0, r = b).
 __device__ __forceinline__ void pcg_scalar(const double* __restrict__ partial, int nb, int nw,
                                            PcgState* __restrict__ st, double* __restrict__ wacc, double eta,
-                                           int min_iters, int max_iters, int first, int32_t* __restrict__ all_done) {
+                                           int min_iters, int max_iters, int first, int32_t* __restrict__ all_done,
+                                           int32_t* __restrict__ host_done) {
   __shared__ int any_active;
   if (threadIdx.x == 0) any_active = 0;
   __syncthreads();
@@ -1513,7 +1516,29 @@ __device__ __forceinline__ void pcg_scalar(const double* __restrict__ partial, i
     else any_active = 1;
   }
   __syncthreads();
-  if (threadIdx.x == 0) *all_done = any_active ? 0 : 1;
+  if (threadIdx.x == 0) {
+    *all_done = any_active ? 0 : 1;
+    // Host-visible copy (pinned host memory) the host polls between launches.
+    if (!any_active && host_done) *host_done = 1;
+  }
+}
+
+// PCG start: every window's state cleared, windows not iterating
+// (active[w] == 0) marked stopped; all_done cleared.
+__global__ __launch_bounds__(kBlock) void k_pcg_init(int nw, PcgState* __restrict__ st,
+                                                     const uint8_t* __restrict__ active, double tol_r,
+                                                     int32_t* __restrict__ all_done) {
+  for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+    PcgState z;
+    z.rho = z.rho_last = z.Q0 = z.alpha = z.beta = z.pq = z.norm_b = z.zeta = z.norm_r = 0.0;
+    z.tol_r = tol_r;
+    z.iter = 1;
+    z.done = (active && !active[w]) ? 1 : 0;
+    z.status = 0;
+    z.pad = 0;
+    st[w] = z;
+  }
+  if (threadIdx.x == 0) *all_done = 0;
 }
 
 // The small PCG kernels give each F block (camera) a group of 16 lanes,
@@ -1595,7 +1620,7 @@ __global__ __launch_bounds__(kBlock) void k_mv_finalize_alpha(
     wi = win_of_f(win, f);
     T sp = T(0);
     for (int ch = f_lo[f]; ch < f_hi[f]; ++ch) sp += partial[(size_t)kPartial * ch + i];
-    T a = (T)(diag_c[9 * (size_t)f + i] * win.inv_r[wi]) * xv[i];
+    T a = (T)(diag_c[9 * (size_t)f + i] * win_inv_r(win, wi)) * xv[i];
     T yv;
     if constexpr (L == 0) {
       const T* u = U + 45 * (size_t)f;
@@ -1632,7 +1657,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_scalar(
     int nf, PcgState* __restrict__ st, Win win, int mode, const T* __restrict__ p, const T* __restrict__ q,
     const T* __restrict__ b, const T* __restrict__ tmp, const T* __restrict__ Minv, T* __restrict__ x,
     T* __restrict__ r, T* __restrict__ z, double* __restrict__ bpart, int32_t* __restrict__ all_done, int nw,
-    double eta, int min_iters, int max_iters, unsigned int* __restrict__ counter) {
+    double eta, int min_iters, int max_iters, unsigned int* __restrict__ counter, int32_t* __restrict__ host_done) {
   __shared__ double lds[3 * (kBlock / 64)];
   if (*all_done) return;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1694,7 +1719,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_scalar(
       }
   }
   if (!last_block(counter)) return;
-  pcg_scalar(bpart, gridDim.x, nw, st, win.acc, eta, min_iters, max_iters, mode == 3, all_done);
+  pcg_scalar(bpart, gridDim.x, nw, st, win.acc, eta, min_iters, max_iters, mode == 3, all_done, host_done);
   if (threadIdx.x == 0) *counter = 0u;
 }
 

@@ -688,6 +688,14 @@ struct Dev {
   T *partial_sj = nullptr, *Tsj = nullptr;
 };
 
+constexpr int kSumSlots = 8;  // per-window sums read back per LM iteration
+// Sum slots: model cost change, |step|^2, trial cost, |x|^2, gradient max
+// and |g|^2 (two slots), a slot for sums read at once.
+constexpr int kSlotModel = 0, kSlotPlus = 1, kSlotCost = 2, kSlotXnorm = 3, kSlotGrad = 4, kSlotSync = 7;
+constexpr int kStage = 8;     // pinned upload slots in rotation
+constexpr int kLook = 4;      // PCG iterations the host may run ahead
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
 }  // namespace
 }  // namespace mba
 
@@ -751,11 +759,9 @@ struct mba_context {
   int32_t *s_split_blk = nullptr, *s_split_lo = nullptr, *s_split_hi = nullptr;
   int64_t s_nitems = 0, s_nsplit = 0, s_nslots = 0;
   int64_t s_nblk = 0, s_npairs = 0;
-  // Host copies of what was last uploaded (rewritten only while the stream
-  // is idle).
+  // Host copies of what was last uploaded.
   std::vector<double> h_inv_r;
-  std::vector<PcgState> h_pcg;
-  std::vector<uint8_t> h_accept, h_active;
+  std::vector<uint8_t> h_active;
   Dev<float> df;
   Dev<double> dd;
   bool mixed = false;
@@ -763,6 +769,25 @@ struct mba_context {
   bool timing = false;
   hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   bool timed_explicit = false;
+  // Pinned host memory (hipHostMalloc) for the LM loop's readbacks and
+  // small uploads: kSumSlots x W sums, 2 x W flags, W PCG states, kStage
+  // upload slots of 8 W bytes.  An upload slot is rewritten only after the
+  // event recorded behind its copy has completed.  The PCG "all stopped"
+  // word is written by the device into coherent pinned memory (d_done is
+  // its device address).
+  char* pin = nullptr;
+  double* h_sums = nullptr;
+  int32_t* h_flags = nullptr;
+  PcgState* h_pcgs = nullptr;
+  int32_t* pin_done = nullptr;
+  volatile int32_t* h_done = nullptr;
+  int32_t* d_done = nullptr;
+  char* h_stage = nullptr;
+  size_t stage_bytes = 0;
+  int stage_next = 0;
+  hipEvent_t stage_ev[kStage] = {};
+  hipEvent_t look_ev[kLook] = {};  // PCG look-ahead: one per in-flight iteration
+  hipEvent_t tev[3] = {};          // LM phase timing
   // Device point order (E blocks of all windows stably grouped by lane
   // class) and the point-major launch layout.
   std::vector<int32_t> dev_e_to_pt;  // device E block -> global point
@@ -865,6 +890,14 @@ void context_free(mba_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (int i = 0; i < 8; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  for (hipEvent_t e : c->stage_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->look_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->tev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->pin) (void)hipHostFree(c->pin);
+  if (c->pin_done) (void)hipHostFree(c->pin_done);
   c->pool.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1483,6 +1516,26 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   if (W == 1 && c->opt.linear_solver == MBA_EXPLICIT_SCHUR) MBA_TRY(build_s_pairs(c, dev_pt_start, row_cam));
   trace.mark("precision layout + S pairs");
   for (int i = 0; i < 8; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
+  for (hipEvent_t& e : c->stage_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (hipEvent_t& e : c->look_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (hipEvent_t& e : c->tev) HIP_TRY(hipEventCreate(&e));
+  {
+    const size_t Wz = (size_t)W;
+    const size_t o_flags = sizeof(double) * kSumSlots * Wz;
+    const size_t o_pcg = align_up(o_flags + sizeof(int32_t) * 2 * Wz, 64);
+    const size_t o_stage = align_up(o_pcg + sizeof(PcgState) * Wz, 64);
+    c->stage_bytes = align_up(8 * Wz, 64);
+    HIP_TRY(hipHostMalloc((void**)&c->pin, o_stage + kStage * c->stage_bytes, hipHostMallocDefault));
+    std::memset(c->pin, 0, o_stage + kStage * c->stage_bytes);
+    c->h_sums = (double*)c->pin;
+    c->h_flags = (int32_t*)(c->pin + o_flags);
+    c->h_pcgs = (PcgState*)(c->pin + o_pcg);
+    c->h_stage = c->pin + o_stage;
+    HIP_TRY(hipHostMalloc((void**)&c->pin_done, 64, hipHostMallocCoherent));
+    c->pin_done[0] = 0;
+    c->h_done = c->pin_done;
+    HIP_TRY(hipHostGetDevicePointer((void**)&c->d_done, c->pin_done, 0));
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   trace.mark("drain");
   *out = c;
@@ -1499,7 +1552,9 @@ Win win_desc(mba_context* c, double* acc, bool in_pcg) {
   const bool multi = c->W > 1;
   w.of_e = multi ? c->of_e : nullptr;
   w.of_f = multi ? c->of_f : nullptr;
-  w.inv_r = c->inv_r;
+  // A single problem takes 1 / radius by value (no upload, no sync).
+  w.inv_r = multi ? c->inv_r : nullptr;
+  w.inv_r0 = c->h_inv_r.empty() ? 0.0 : c->h_inv_r[0];
   w.acc = multi ? acc : nullptr;
   w.flag = c->flag;
   w.active = multi ? c->wactive : nullptr;
@@ -1522,56 +1577,84 @@ struct Engine {
     HIP_TRY(hipStreamSynchronize(s));
     return MBA_OK;
   }
+  // Seconds between two completed events.
+  static double elapsed_s(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    return hipEventElapsedTime(&ms, a, b) == hipSuccess ? 1e-3 * ms : 0.0;
+  }
   Win win(double* acc = nullptr, bool in_pcg = false) { return win_desc(c, acc, in_pcg); }
 
-  // Windows still iterating; uploaded when it changes, after the stream
-  // has drained.
+  // Host -> device upload of n <= 8 W bytes through the next pinned slot,
+  // in stream order and without a sync: the slot is rewritten only after
+  // the event recorded behind its last copy has completed.
+  mba_status stage_upload(void* dst, const void* src, size_t n) {
+    if (n > c->stage_bytes) return fail(MBA_ERR_INVALID_ARGUMENT, "staged upload larger than its slot");
+    const int k = c->stage_next;
+    c->stage_next = (k + 1) % kStage;
+    HIP_TRY(hipEventSynchronize(c->stage_ev[k]));
+    char* h = c->h_stage + (size_t)k * c->stage_bytes;
+    std::memcpy(h, src, n);
+    HIP_TRY(hipMemcpyAsync(dst, h, n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(c->stage_ev[k], s));
+    return MBA_OK;
+  }
+
+  // Windows still iterating; uploaded when it changes.
   mba_status set_active(const std::vector<uint8_t>& act) {
     if (W == 1 || act == c->h_active) return MBA_OK;
-    MBA_TRY(sync());
     c->h_active = act;
-    HIP_TRY(hipMemcpyAsync(c->wactive, c->h_active.data(), W, hipMemcpyHostToDevice, s));
-    return MBA_OK;
+    return stage_upload(c->wactive, act.data(), (size_t)W);
   }
 
   mba_status clear_flags() {
     HIP_TRY(hipMemsetAsync(c->flag, 0, sizeof(int32_t) * W, s));
     return MBA_OK;
   }
-  mba_status read_flags(std::vector<int32_t>* f) {
-    f->assign(W, 0);
-    HIP_TRY(hipMemcpyAsync(f->data(), c->flag, sizeof(int32_t) * W, hipMemcpyDeviceToHost, s));
-    return sync();
+  // Flags into pinned slot `slot` (0 or 1), valid after the next sync.
+  mba_status flags_async(int slot) {
+    HIP_TRY(hipMemcpyAsync(c->h_flags + (size_t)slot * W, c->flag, sizeof(int32_t) * W, hipMemcpyDeviceToHost, s));
+    return MBA_OK;
   }
+  int32_t flag_at(int slot, int w) const { return c->h_flags[(size_t)slot * W + w]; }
 
   // Per-window sums.  A single problem reduces block partials in c->red
   // with one deterministic k_reduce; several windows add into c->wacc.
+  // sum_async leaves the result in pinned sum slot `slot`, valid after
+  // the next sync; end_sum waits for it.
   mba_status begin_sum() {
     if (W > 1) HIP_TRY(hipMemsetAsync(c->wacc, 0, sizeof(double) * W, s));
     else HIP_TRY(hipMemsetAsync(c->red, 0, sizeof(double), s));
     return MBA_OK;
   }
-  mba_status end_sum(int npartials, std::vector<double>* out) {
-    out->assign(W, 0.0);
+  mba_status sum_async(int npartials, int slot) {
+    double* h = c->h_sums + (size_t)slot * W;
     if (W == 1) {
       hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, c->red, std::max(npartials, 1), &c->sc->sum);
-      HIP_TRY(hipMemcpyAsync(out->data(), &c->sc->sum, sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(h, &c->sc->sum, sizeof(double), hipMemcpyDeviceToHost, s));
     } else {
-      HIP_TRY(hipMemcpyAsync(out->data(), c->wacc, sizeof(double) * W, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(h, c->wacc, sizeof(double) * W, hipMemcpyDeviceToHost, s));
     }
-    return sync();
+    return MBA_OK;
+  }
+  double sum_at(int slot, int w) const { return c->h_sums[(size_t)slot * W + w]; }
+  mba_status end_sum(int npartials, std::vector<double>* out) {
+    MBA_TRY(sum_async(npartials, kSlotSync));
+    MBA_TRY(sync());
+    out->assign(W, 0.0);
+    for (int w = 0; w < W; ++w) (*out)[w] = sum_at(kSlotSync, w);
+    return MBA_OK;
   }
 
-  // 1 / radius per window (0 for radius 0: no damping).  Uploaded only
-  // when it changes, after the stream has drained.
+  // 1 / radius per window (0 for radius 0: no damping).  A single problem
+  // passes it by value in the window descriptor; several windows upload
+  // it when it changes.
   mba_status set_radii(const std::vector<double>& radius) {
     std::vector<double> inv(W);
     for (int w = 0; w < W; ++w) inv[w] = radius[w] > 0.0 ? 1.0 / radius[w] : 0.0;
     if (inv == c->h_inv_r) return MBA_OK;
-    MBA_TRY(sync());
     c->h_inv_r = inv;
-    HIP_TRY(hipMemcpyAsync(c->inv_r, c->h_inv_r.data(), sizeof(double) * W, hipMemcpyHostToDevice, s));
-    return MBA_OK;
+    if (W == 1) return MBA_OK;
+    return stage_upload(c->inv_r, c->h_inv_r.data(), sizeof(double) * W);
   }
   mba_status set_radius(double r) { return set_radii(std::vector<double>(W, r)); }
 
@@ -1586,8 +1669,9 @@ struct Engine {
   }
   void tables(const double* cams) { table_into<T>(cams, d.ctab); }
 
-  // FP64 cost per window at (cams, pts); finite[w] = 0 on a non-finite value.
-  mba_status cost64(const double* cams, const double* pts, std::vector<double>* cost, std::vector<uint8_t>* finite) {
+  // FP64 cost per window at (cams, pts) into sum slot kSlotCost and its
+  // non-finite flags into flag slot 1; cost_result reads them after a sync.
+  mba_status cost64_async(const double* cams, const double* pts) {
     MBA_TRY(clear_flags());
     table_into<double>(cams, c->ctab64);
     const int nb = c->seg_blocks;
@@ -1595,11 +1679,21 @@ struct Engine {
     if (nb > 0)
       hipLaunchKernelGGL((c->grouped ? k_cost64<1> : k_cost64<0>), dim3(nb), dim3(kBlock), 0, s, c->seg, d.pt_start,
                          d.row_cam, c->row_obs64, pts, c->ctab64, c->opt.huber_a, c->red, win(c->wacc));
-    MBA_TRY(end_sum(nb, cost));
-    std::vector<int32_t> f;
-    MBA_TRY(read_flags(&f));
+    MBA_TRY(sum_async(nb, kSlotCost));
+    return flags_async(1);
+  }
+  void cost_result(std::vector<double>* cost, std::vector<uint8_t>* finite) const {
+    cost->assign(W, 0.0);
     finite->assign(W, 1);
-    for (int w = 0; w < W; ++w) (*finite)[w] = (f[w] == 0) && std::isfinite((*cost)[w]);
+    for (int w = 0; w < W; ++w) {
+      (*cost)[w] = sum_at(kSlotCost, w);
+      (*finite)[w] = flag_at(1, w) == 0 && std::isfinite((*cost)[w]);
+    }
+  }
+  mba_status cost64(const double* cams, const double* pts, std::vector<double>* cost, std::vector<uint8_t>* finite) {
+    MBA_TRY(cost64_async(cams, pts));
+    MBA_TRY(sync());
+    cost_result(cost, finite);
     return MBA_OK;
   }
 
@@ -1704,22 +1798,19 @@ struct Engine {
                        0, s, c->nf, d.f_lo, d.f_hi, d.partial, d.U, c->diag_c, wv, x, 1, y, done);
   }
 
-  // PCG on S x = rhs (x in d.x) for every window with active[w]; each
-  // window stops on its own tests, the launches stop when all have.
-  mba_status pcg(const std::vector<uint8_t>& active, std::vector<int>* iters, std::vector<int>* status) {
+  // PCG on S x = rhs (x in d.x) for every window still iterating (the
+  // device's active flags); each window stops on its own tests, the
+  // launches stop when all have.  The host runs at most kLook iterations
+  // ahead of the device and stops launching once the device has written
+  // "all stopped" to pinned memory; launches already queued return at
+  // once.  The final states go to pinned memory: pcg_result reads them
+  // after the next sync.
+  mba_status pcg_async() {
     const int nf = c->nf;
     const int nbg = nblocks((int64_t)kGrp * nf);  // 16 lanes per F block
-    MBA_TRY(sync());
-    c->h_pcg.assign(W, PcgState());
-    for (int w = 0; w < W; ++w) {
-      PcgState& h = c->h_pcg[w];
-      std::memset(&h, 0, sizeof(h));
-      h.iter = 1;
-      h.tol_r = c->opt.pcg_r_tolerance;
-      h.done = active[w] ? 0 : 1;
-    }
-    HIP_TRY(hipMemcpyAsync(c->pcg, c->h_pcg.data(), sizeof(PcgState) * W, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemsetAsync(c->all_done, 0, sizeof(int32_t), s));
+    *c->h_done = 0;
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kBlock), 0, s, W, c->pcg, (const uint8_t*)(W > 1 ? c->wactive : nullptr),
+                       c->opt.pcg_r_tolerance, c->all_done);
     // Windowed accumulators: dot at wacc[0, W), update sums at wacc[W, 4W);
     // the scalar kernels clear what they read.
     double* dot_acc = c->W > 1 ? c->wacc : nullptr;
@@ -1730,12 +1821,17 @@ struct Engine {
     const int minit = c->opt.min_linear_solver_iterations, maxit = c->opt.max_linear_solver_iterations;
     auto update = [&](int mode) {
       hipLaunchKernelGGL(k_pcg_update_scalar<T>, dim3(nbg), dim3(kBlock), 0, s, nf, c->pcg, wu, mode, d.p, d.q, d.rhs,
-                         d.tmp, d.Minv, d.x, d.r, d.zc, c->red, c->all_done, W, eta, minit, maxit, c->counter + 1);
+                         d.tmp, d.Minv, d.x, d.r, d.zc, c->red, c->all_done, W, eta, minit, maxit, c->counter + 1,
+                         c->d_done);
     };
     const Win wv = win(nullptr, true);
     update(3);
-    int check_every = 1;
+    HIP_TRY(hipEventRecord(c->look_ev[0], s));
     for (int it = 1; it <= maxit; ++it) {
+      // Wait for iteration it - kLook (the update launched kLook rounds
+      // ago), then stop if the device has finished.
+      HIP_TRY(hipEventSynchronize(c->look_ev[it % kLook]));
+      if (*c->h_done) break;
       if (c->grouped) {
         // A group's direction feeds all its cameras' rows: update p first.
         hipLaunchKernelGGL(k_pcg_p<T>, dim3(nblocks(9 * (int64_t)nf)), dim3(kBlock), 0, s, nf, (const PcgState*)c->pcg,
@@ -1757,24 +1853,19 @@ struct Engine {
       } else {
         update(0);
       }
-      if (it % check_every == 0 || it == maxit) {
-        int32_t done = 0;
-        HIP_TRY(hipMemcpyAsync(&done, c->all_done, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        MBA_TRY(sync());
-        if (done) break;
-        check_every = std::min(8, check_every * 2);
-      }
+      HIP_TRY(hipEventRecord(c->look_ev[it % kLook], s));
     }
-    HIP_TRY(hipMemcpyAsync(c->h_pcg.data(), c->pcg, sizeof(PcgState) * W, hipMemcpyDeviceToHost, s));
-    MBA_TRY(sync());
+    HIP_TRY(hipMemcpyAsync(c->h_pcgs, c->pcg, sizeof(PcgState) * W, hipMemcpyDeviceToHost, s));
+    return MBA_OK;
+  }
+  void pcg_result(const std::vector<uint8_t>& active, std::vector<int>* iters, std::vector<int>* status) const {
     iters->assign(W, 0);
     status->assign(W, 0);
     for (int w = 0; w < W; ++w) {
       if (!active[w]) continue;
-      (*iters)[w] = c->h_pcg[w].iter;
-      (*status)[w] = c->h_pcg[w].status;
+      (*iters)[w] = c->h_pcgs[w].iter;
+      (*status)[w] = c->h_pcgs[w].status;
     }
-    return MBA_OK;
   }
 
   // In-place solves with the factor: L y = v, then L' x = y.
@@ -1891,8 +1982,9 @@ struct Engine {
                        (int64_t)9 * c->nf, d.x, d.step_c);
   }
 
-  // Model cost change -J step . (r + J step / 2) per window.
-  mba_status model_cost_change(std::vector<double>* mcc) {
+  // Model cost change -J step . (r + J step / 2) per window, negated, into
+  // sum slot kSlotModel.
+  mba_status model_cost_async() {
     cam_dir(d.step_c, (T)1, nullptr, win());
     const int nb = c->seg_blocks;
     MBA_TRY(begin_sum());
@@ -1900,13 +1992,12 @@ struct Engine {
       hipLaunchKernelGGL((c->grouped ? k_model_cost<T, 1> : k_model_cost<T, 0>), dim3(nb), dim3(kBlock), 0, s, c->seg,
                          d.pt_start, d.row_cam, d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.step_p, huber, c->red,
                          win(c->wacc));
-    MBA_TRY(end_sum(nb, mcc));
-    for (double& v : *mcc) v = -v;
-    return MBA_OK;
+    return sum_async(nb, kSlotModel);
   }
 
-  // x_trial = x + step .* scale, per window |delta| and its finiteness.
-  mba_status plus(std::vector<double>* step_norm, std::vector<uint8_t>* finite) {
+  // x_trial = x + step .* scale; per window |delta|^2 into sum slot
+  // kSlotPlus (non-finite when the trial point is).
+  mba_status plus_async() {
     const int64_t np3 = 3 * (int64_t)c->ne, nf9 = 9 * (int64_t)c->nf;
     hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(np3)), dim3(kBlock), 0, s, np3, d.step_p, c->stepd);
     hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(nf9)), dim3(kBlock), 0, s, nf9, d.step_c,
@@ -1917,18 +2008,12 @@ struct Engine {
                        c->red, win(c->wacc));
     hipLaunchKernelGGL(k_plus_cams, dim3(nb2), dim3(kBlock), 0, s, c->nf, d.f_to_cam, c->cams64,
                        c->stepd + np3, c->cscale64, c->cams64_t, c->red + nb1, win(c->wacc));
-    std::vector<double> v;
-    MBA_TRY(end_sum(nb1 + nb2, &v));
-    step_norm->assign(W, 0.0);
-    finite->assign(W, 1);
-    for (int w = 0; w < W; ++w) {
-      (*step_norm)[w] = std::sqrt(v[w]);
-      (*finite)[w] = std::isfinite(v[w]);
-    }
-    return MBA_OK;
+    return sum_async(nb1 + nb2, kSlotPlus);
   }
 
-  mba_status x_norm(std::vector<double>* out) {
+  // |x| per window (the parameters of the local parameterisation) into
+  // sum slot kSlotXnorm as |x|^2.
+  mba_status x_norm_async() {
     const int64_t np3 = 3 * (int64_t)c->ne, nf9 = 9 * (int64_t)c->nf;
     hipLaunchKernelGGL(k_gather_cams, dim3(nblocks(nf9)), dim3(kBlock), 0, s, c->nf, d.f_to_cam, c->cams64,
                        c->stepd + np3, (const uint8_t*)c->xmask);
@@ -1936,15 +2021,14 @@ struct Engine {
     const int nb = nblocks(np3 + nf9);
     MBA_TRY(begin_sum());
     hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kBlock), 0, s, np3 + nf9, np3, c->stepd, c->red, win(c->wacc));
-    MBA_TRY(end_sum(nb, out));
-    for (double& v : *out) v = std::sqrt(v);
-    return MBA_OK;
+    return sum_async(nb, kSlotXnorm);
   }
 
-  // Max |g| and |g| of the unscaled gradient, per window.
-  mba_status gradient_norms(std::vector<double>* gmax, std::vector<double>* gnorm) {
+  // Max |g| and |g|^2 of the unscaled gradient, per window, interleaved in
+  // sum slots kSlotGrad, kSlotGrad + 1; gradient_result reads them.
+  mba_status gradient_async() {
     const int64_t np3 = 3 * (int64_t)c->ne, nf9 = 9 * (int64_t)c->nf;
-    std::vector<double> v(2 * (size_t)W, 0.0);
+    double* v = c->h_sums + (size_t)kSlotGrad * W;
     if (W == 1) {
       HIP_TRY(hipMemsetAsync(c->red2, 0, 2 * sizeof(double), s));
       int nb = std::min(1024, nblocks(np3));
@@ -1957,23 +2041,25 @@ struct Engine {
                            c->red + 2048);
         hipLaunchKernelGGL(k_grad_final, dim3(1), dim3(kBlock), 0, s, c->red, c->red + 2048, nb, c->red2);
       }
-      HIP_TRY(hipMemcpyAsync(v.data(), c->red2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(v, c->red2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     } else {
       HIP_TRY(hipMemsetAsync(c->wacc2, 0, 2 * sizeof(double) * W, s));
       hipLaunchKernelGGL(k_grad_win<T>, dim3(nblocks(np3)), dim3(kBlock), 0, s, np3, 3, 0, d.gp, c->pscale64, win(),
                          c->wacc2);
       hipLaunchKernelGGL(k_grad_win<T>, dim3(nblocks(nf9)), dim3(kBlock), 0, s, nf9, 9, 1, d.gc, c->cscale64, win(),
                          c->wacc2);
-      HIP_TRY(hipMemcpyAsync(v.data(), c->wacc2, 2 * sizeof(double) * W, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(v, c->wacc2, 2 * sizeof(double) * W, hipMemcpyDeviceToHost, s));
     }
-    MBA_TRY(sync());
+    return MBA_OK;
+  }
+  void gradient_result(std::vector<double>* gmax, std::vector<double>* gnorm) const {
+    const double* v = c->h_sums + (size_t)kSlotGrad * W;
     gmax->assign(W, 0.0);
     gnorm->assign(W, 0.0);
     for (int w = 0; w < W; ++w) {
       (*gmax)[w] = v[2 * (size_t)w];
       (*gnorm)[w] = std::sqrt(v[2 * (size_t)w + 1]);
     }
-    return MBA_OK;
   }
 
   // EstimateScale (trust_region_minimizer.cc:103-109) from an unscaled,
@@ -2006,28 +2092,35 @@ struct Engine {
                          (const uint8_t*)c->fmask, d.cscale);
   }
 
-  // Solve the LM systems at the current radii; fills step_p / step_c.
-  mba_status compute_step(const std::vector<uint8_t>& active, std::vector<uint8_t>* solved, std::vector<int>* lin) {
+  // Solve the LM systems at the current radii and fill step_p / step_c
+  // (for every window still iterating: a window whose solve failed gets a
+  // step that is never used).  `factored` is false when the explicit
+  // factorisation failed; step_result reads the PCG outcome after a sync.
+  mba_status compute_step_async(bool* factored) {
+    *factored = true;
+    if (c->opt.linear_solver == MBA_IMPLICIT_SCHUR_PCG) {
+      if (c->nf > 0) MBA_TRY(pcg_async());
+      else HIP_TRY(hipMemsetAsync(d.x, 0, sizeof(T) * 9, s));
+    } else {
+      MBA_TRY(explicit_solve(factored));
+    }
+    back_substitute();
+    return MBA_OK;
+  }
+  void step_result(const std::vector<uint8_t>& active, bool factored, std::vector<uint8_t>* solved,
+                   std::vector<int>* lin) const {
     solved->assign(W, 1);
     lin->assign(W, 0);
     if (c->opt.linear_solver == MBA_IMPLICIT_SCHUR_PCG) {
       if (c->nf > 0) {
         std::vector<int> status;
-        MBA_TRY(pcg(active, lin, &status));
+        pcg_result(active, lin, &status);
         for (int w = 0; w < W; ++w)
           if (status[w] < 0) (*solved)[w] = 0;
-      } else {
-        HIP_TRY(hipMemsetAsync(d.x, 0, sizeof(T) * 9, s));
       }
-    } else {
-      bool ok = true;
-      MBA_TRY(explicit_solve(&ok));
-      if (!ok) (*solved)[0] = 0;
+    } else if (!factored) {
+      (*solved)[0] = 0;
     }
-    bool any = false;
-    for (int w = 0; w < W; ++w) any = any || (active[w] && (*solved)[w]);
-    if (any) back_substitute();
-    return MBA_OK;
   }
 
   // Make the trial point current for the windows with accept[w].
@@ -2039,9 +2132,7 @@ struct Engine {
       HIP_TRY(hipMemcpyAsync(c->cams64_t, c->cams64, sizeof(double) * 9 * c->nslot, hipMemcpyDeviceToDevice, s));
       return MBA_OK;
     }
-    MBA_TRY(sync());
-    c->h_accept = acc;
-    HIP_TRY(hipMemcpyAsync(c->accept, c->h_accept.data(), W, hipMemcpyHostToDevice, s));
+    MBA_TRY(stage_upload(c->accept, acc.data(), (size_t)W));
     hipLaunchKernelGGL(k_accept_points, dim3(nblocks(3 * (int64_t)c->ne)), dim3(kBlock), 0, s, c->ne, win(),
                        c->accept, c->pts64_t, c->pts64);
     hipLaunchKernelGGL(k_accept_cams, dim3(nblocks(9 * (int64_t)c->nf)), dim3(kBlock), 0, s, c->nf, win(),
@@ -2097,7 +2188,7 @@ struct Engine {
       return r;
     };
     double t_jac = 0.0, t_res = 0.0, t_lin = 0.0;
-    std::vector<double> cost, gmax, gnorm, xn;
+    std::vector<double> cost, gmax, gnorm;
     std::vector<uint8_t> fin;
     double t0 = now_s();
     MBA_TRY(cost64(c->cams64, c->pts64, &cost, &fin));
@@ -2124,9 +2215,11 @@ struct Engine {
       MBA_TRY(set_radii(radii()));
       MBA_TRY(clear_flags());
       jacobian_pass(1);
+      MBA_TRY(gradient_async());
+      MBA_TRY(x_norm_async());
       MBA_TRY(sync());
       t_jac += now_s() - t0;
-      MBA_TRY(gradient_norms(&gmax, &gnorm));
+      gradient_result(&gmax, &gnorm);
       for (int w = 0; w < W; ++w) {
         if (!L[w].active) continue;
         L[w].it.gradient_max_norm = gmax[w];
@@ -2135,30 +2228,46 @@ struct Engine {
         push(w);
         if (gmax[w] <= o.gradient_tolerance) finish(w, MBA_CONVERGENCE);
       }
-      MBA_TRY(x_norm(&xn));
-      for (int w = 0; w < W; ++w) L[w].x_norm = xn[w];
+      for (int w = 0; w < W; ++w) L[w].x_norm = std::sqrt(sum_at(kSlotXnorm, w));
     }
-    std::vector<uint8_t> act(W), solved, stepping(W), acc(W), sf, cf;
+    std::vector<uint8_t> act(W), solved, stepping(W), acc(W), sf(W), cf;
     std::vector<int> lin;
-    std::vector<int32_t> fl;
-    std::vector<double> mcc(W, 0.0), pgm(W), pgn(W), sn, new_cost;
+    std::vector<double> mcc(W, 0.0), pgm(W), pgn(W), sn(W), new_cost;
     while (any_active()) {
       for (int w = 0; w < W; ++w)
         if (L[w].active && L[w].it.iteration >= o.max_num_iterations) finish(w, MBA_NO_CONVERGENCE);
       if (!any_active()) break;
       for (int w = 0; w < W; ++w) act[w] = L[w].active;
       MBA_TRY(set_active(act));
-      t0 = now_s();
+      // The step, its model cost change, the trial point and its cost are
+      // queued together and read after one sync; what a window whose step
+      // turns out invalid computed is not used.
+      HIP_TRY(hipEventRecord(c->tev[0], s));
       MBA_TRY(clear_flags());
-      MBA_TRY(compute_step(act, &solved, &lin));
-      MBA_TRY(read_flags(&fl));
-      t_lin += now_s() - t0;
-      bool any_solved = false;
+      bool factored = true;
+      MBA_TRY(compute_step_async(&factored));
+      MBA_TRY(flags_async(0));
+      HIP_TRY(hipEventRecord(c->tev[1], s));
+      MBA_TRY(model_cost_async());
+      MBA_TRY(plus_async());
+      MBA_TRY(cost64_async(c->cams64_t, c->pts64_t));
+      HIP_TRY(hipEventRecord(c->tev[2], s));
+      MBA_TRY(sync());
+      t_lin += elapsed_s(c->tev[0], c->tev[1]);
+      t_res += elapsed_s(c->tev[1], c->tev[2]);
+      step_result(act, factored, &solved, &lin);
+      cost_result(&new_cost, &cf);
+      for (int w = 0; w < W; ++w) {
+        const double v = sum_at(kSlotPlus, w);
+        mcc[w] = -sum_at(kSlotModel, w);
+        sn[w] = std::sqrt(v);
+        sf[w] = std::isfinite(v);
+      }
       for (int w = 0; w < W; ++w) {
         stepping[w] = 0;
         acc[w] = 0;
         if (!act[w]) continue;
-        if (fl[w]) solved[w] = 0;
+        if (flag_at(0, w)) solved[w] = 0;
         sums[w].num_linear_solver_iterations += lin[w];
         mba_iteration& it = L[w].it;
         const int prev = it.iteration;
@@ -2167,10 +2276,7 @@ struct Engine {
         std::memset(&it, 0, sizeof(it));
         it.iteration = prev + 1;
         it.linear_solver_iterations = lin[w];
-        any_solved = any_solved || solved[w];
       }
-      if (any_solved) MBA_TRY(model_cost_change(&mcc));
-      bool any_valid = false;
       for (int w = 0; w < W; ++w) {
         if (!act[w]) continue;
         mba_iteration& it = L[w].it;
@@ -2187,14 +2293,7 @@ struct Engine {
           it.gradient_norm = pgn[w];
         } else {
           L[w].num_invalid = 0;
-          any_valid = true;
         }
-      }
-      if (any_valid) {
-        MBA_TRY(plus(&sn, &sf));
-        t0 = now_s();
-        MBA_TRY(cost64(c->cams64_t, c->pts64_t, &new_cost, &cf));
-        t_res += now_s() - t0;
       }
       bool any_acc = false, any_step = false;
       for (int w = 0; w < W; ++w) {
@@ -2239,20 +2338,21 @@ struct Engine {
       MBA_TRY(set_active(activity()));
       if (any_acc) {
         MBA_TRY(accept(acc));
-        MBA_TRY(x_norm(&xn));
-        for (int w = 0; w < W; ++w)
-          if (acc[w]) L[w].x_norm = xn[w];
-        t0 = now_s();
+        MBA_TRY(x_norm_async());
+        HIP_TRY(hipEventRecord(c->tev[0], s));
         tables(c->cams64);
         MBA_TRY(set_radii(radii()));
         // A full pass also re-damps the windows whose step was rejected
         // (their point is unchanged).
         jacobian_pass(1);
+        HIP_TRY(hipEventRecord(c->tev[1], s));
+        MBA_TRY(gradient_async());
         MBA_TRY(sync());
-        t_jac += now_s() - t0;
-        MBA_TRY(gradient_norms(&gmax, &gnorm));
+        t_jac += elapsed_s(c->tev[0], c->tev[1]);
+        gradient_result(&gmax, &gnorm);
         for (int w = 0; w < W; ++w)
           if (acc[w]) {
+            L[w].x_norm = std::sqrt(sum_at(kSlotXnorm, w));
             L[w].it.gradient_max_norm = gmax[w];
             L[w].it.gradient_norm = gnorm[w];
           }
