@@ -35,7 +35,7 @@ CONVERGENCE, NO_CONVERGENCE, FAILURE = 0, 1, 2
 LAYOUT_PACKED9 = 0
 LAYOUT_GROUPED = 1
 ORDER_AUTOMATIC, ORDER_POINTS_FIRST = 0, 1
-PRECOND_JACOBI, PRECOND_SCHUR_JACOBI = 0, 1
+PRECOND_JACOBI, PRECOND_SCHUR_JACOBI, PRECOND_CLUSTER_JACOBI, PRECOND_CLUSTER_TRIDIAGONAL = 0, 1, 2, 3
 EXPLICIT_SCHUR, IMPLICIT_SCHUR_PCG = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
@@ -43,6 +43,8 @@ PRECISION_FP64, PRECISION_MIXED = 0, 1
 SPARSE_SCHUR = EXPLICIT_SCHUR
 DENSE_SCHUR = EXPLICIT_SCHUR
 ITERATIVE_SCHUR = IMPLICIT_SCHUR_PCG
+JACOBI, SCHUR_JACOBI, CLUSTER_JACOBI, CLUSTER_TRIDIAGONAL = (PRECOND_JACOBI, PRECOND_SCHUR_JACOBI,
+                                                           PRECOND_CLUSTER_JACOBI, PRECOND_CLUSTER_TRIDIAGONAL)
 
 _TERMINATION_NAMES = {CONVERGENCE: "CONVERGENCE", NO_CONVERGENCE: "NO_CONVERGENCE", FAILURE: "FAILURE"}
 
@@ -77,7 +79,18 @@ class Options(C.Structure):
                 ("min_trust_region_radius", C.c_double), ("min_relative_decrease", C.c_double),
                 ("min_lm_diagonal", C.c_double), ("max_lm_diagonal", C.c_double), ("eta", C.c_double),
                 ("pcg_r_tolerance", C.c_double), ("huber_a", C.c_double),
-                ("refinement_iterations", C.c_int32), ("preconditioner", C.c_int32)]
+                ("refinement_iterations", C.c_int32), ("preconditioner", C.c_int32),
+                ("dump_directory", C.c_char_p), ("dump_iterations", C.POINTER(C.c_int32)),
+                ("num_dump_iterations", C.c_int32), ("reserved2", C.c_int32)]
+
+    def set_dump(self, directory, iterations):
+        """trust_region_minimizer_iterations_to_dump / trust_region_problem_dump_directory
+        (TEXTFILE).  The arrays are kept alive by this object."""
+        its = np.ascontiguousarray(list(iterations), np.int32)
+        self._dump_keep = (os.fsencode(directory) if directory is not None else None, its)
+        self.dump_directory = self._dump_keep[0]
+        self.dump_iterations = its.ctypes.data_as(C.POINTER(C.c_int32)) if its.size else None
+        self.num_dump_iterations = int(its.size)
 
 
 class Iteration(C.Structure):
@@ -172,8 +185,12 @@ def _p(a, t=_f64p):
 
 
 def default_options(**kw) -> Options:
+    """mba_default_options with fields overridden by keyword; `dump_directory`
+    and `dump_iterations` go through Options.set_dump."""
     o = Options()
     _check(load_library().mba_default_options(C.byref(o)))
+    if "dump_directory" in kw or "dump_iterations" in kw:
+        o.set_dump(kw.pop("dump_directory", None), kw.pop("dump_iterations", ()))
     for k, v in kw.items():
         if not hasattr(o, k):
             raise AttributeError(k)
@@ -519,6 +536,28 @@ def write_bal(path, problem: BAProblem):
     _check(load_library().mba_write_bal(os.fsencode(path), C.byref(s)))
 
 
+def read_trust_region_problem(base) -> dict:
+    """A TEXTFILE trust-region problem dump (<base>_A.txt, _D.txt, _b.txt,
+    _x.txt and <base>.m, the layout of Ceres's
+    DumpLinearLeastSquaresProblemToTextFile): num_rows, num_cols, A as
+    (rows, cols, values) in file order, D, b, x (whichever exist)."""
+    out = {}
+    with open(str(base) + ".m") as f:
+        for line in f:
+            for key in ("num_rows", "num_cols"):
+                if line.startswith(f"lsqp.{key} = "):
+                    out[key] = int(line.split("=")[1].strip().rstrip(";"))
+    a = np.loadtxt(str(base) + "_A.txt", ndmin=2)
+    out["rows"] = a[:, 0].astype(np.int64)
+    out["cols"] = a[:, 1].astype(np.int64)
+    out["values"] = a[:, 2].copy()
+    for key in ("D", "b", "x"):
+        path = f"{base}_{key}.txt"
+        if os.path.exists(path):
+            out[key] = np.loadtxt(path, ndmin=1)
+    return out
+
+
 # ------------------------------------------------ Ceres-shaped host API --
 class HuberLoss:
     """ceres::HuberLoss(a): threshold a pixels (include/ceres/loss_function.h:175-184)."""
@@ -577,6 +616,9 @@ class SolverOptions:
         self.pcg_r_tolerance = o.pcg_r_tolerance
         self.device = 0
         self.minimizer_progress_to_stdout = False
+        self.preconditioner_type = JACOBI
+        self.trust_region_minimizer_iterations_to_dump = []
+        self.trust_region_problem_dump_directory = "/tmp"
 
     def _to_c(self, huber_a) -> Options:
         return default_options(
@@ -585,7 +627,10 @@ class SolverOptions:
             gradient_tolerance=self.gradient_tolerance, parameter_tolerance=self.parameter_tolerance,
             initial_trust_region_radius=self.initial_trust_region_radius,
             max_linear_solver_iterations=self.max_linear_solver_iterations, eta=self.eta,
-            pcg_r_tolerance=self.pcg_r_tolerance, device=self.device, huber_a=huber_a)
+            pcg_r_tolerance=self.pcg_r_tolerance, device=self.device, huber_a=huber_a,
+            preconditioner=self.preconditioner_type,
+            dump_directory=self.trust_region_problem_dump_directory,
+            dump_iterations=self.trust_region_minimizer_iterations_to_dump)
 
 
 class Problem:

@@ -788,12 +788,25 @@ struct mba_context {
   hipEvent_t stage_ev[kStage] = {};
   hipEvent_t look_ev[kLook] = {};  // PCG look-ahead: one per in-flight iteration
   hipEvent_t tev[3] = {};          // LM phase timing
+  // Trust-region problem dumps (single problem): iterations and directory
+  // copied from the options; `src` is the problem mba_solve was given.
+  std::vector<int32_t> dump_iters;
+  std::string dump_dir;
+  const mba_problem* src = nullptr;
   // Device point order (E blocks of all windows stably grouped by lane
   // class) and the point-major launch layout.
   std::vector<int32_t> dev_e_to_pt;  // device E block -> global point
   Seg seg;
   int32_t seg_blocks = 0;
 };
+
+// LM state at a dumped step (device order, FP64): parameters, Jacobi
+// scales, clamped squared column norms, the step.
+struct LsqpState {
+  std::vector<double> cams, pts, pscale, cscale, diag_p, diag_c, step_p, step_c;
+  double radius = 0.0;
+};
+mba_status write_lsqp(mba_context* c, int k, const LsqpState& S);
 
 namespace mba {
 namespace {
@@ -1089,8 +1102,26 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   for (int32_t w = 0; w < W; ++w)
     if (probs[w].layout == MBA_LAYOUT_GROUPED && W > 1)
       return fail(MBA_ERR_UNSUPPORTED, "grouped-intrinsics problems are solved one context each");
+  if (c->opt.preconditioner == MBA_PRECOND_CLUSTER_JACOBI || c->opt.preconditioner == MBA_PRECOND_CLUSTER_TRIDIAGONAL)
+    return fail(MBA_ERR_UNSUPPORTED, std::string(c->opt.preconditioner == MBA_PRECOND_CLUSTER_JACOBI
+                                                     ? "CLUSTER_JACOBI"
+                                                     : "CLUSTER_TRIDIAGONAL") +
+                                         " requires Solver::Options::sparse_linear_algebra_library_type to be "
+                                         "SUITE_SPARSE");
   if (c->opt.preconditioner != MBA_PRECOND_JACOBI && c->opt.preconditioner != MBA_PRECOND_SCHUR_JACOBI)
     return fail(MBA_ERR_INVALID_ARGUMENT, "unknown preconditioner " + std::to_string(c->opt.preconditioner));
+  if (c->opt.num_dump_iterations < 0 || (c->opt.num_dump_iterations > 0 && !c->opt.dump_iterations))
+    return fail(MBA_ERR_INVALID_ARGUMENT, "bad dump_iterations");
+  if (c->opt.num_dump_iterations > 0) {
+    if (!c->opt.dump_directory || !c->opt.dump_directory[0])
+      return fail(MBA_ERR_INVALID_ARGUMENT, "Solver::Options::trust_region_problem_dump_directory is empty.");
+    if (W > 1) return fail(MBA_ERR_UNSUPPORTED, "trust-region problem dumps are written for a single problem");
+    c->dump_iters.assign(c->opt.dump_iterations, c->opt.dump_iterations + c->opt.num_dump_iterations);
+    c->dump_dir = c->opt.dump_directory;
+  }
+  c->opt.dump_iterations = nullptr;
+  c->opt.dump_directory = nullptr;
+  c->opt.num_dump_iterations = 0;
   const bool schur_jacobi =
       c->opt.preconditioner == MBA_PRECOND_SCHUR_JACOBI && c->opt.linear_solver == MBA_IMPLICIT_SCHUR_PCG;
   if (schur_jacobi && probs[0].layout == MBA_LAYOUT_GROUPED)
@@ -2140,6 +2171,38 @@ struct Engine {
     return MBA_OK;
   }
 
+  // Trust-region problem dump of the step just computed (single problem):
+  // the device state is read back and written out by write_lsqp.
+  mba_status dump_step(int k, double radius) {
+    if (!c->src) return fail(MBA_ERR_UNSUPPORTED, "trust-region problem dumps are written by mba_solve");
+    LsqpState S;
+    const size_t np3 = 3 * (size_t)c->ne, nf9 = 9 * (size_t)c->nf;
+    S.cams.resize(9 * (size_t)c->nslot);
+    S.pts.resize(np3);
+    S.pscale.resize(np3);
+    S.diag_p.resize(np3);
+    S.cscale.resize(nf9);
+    S.diag_c.resize(nf9);
+    std::vector<T> sp(np3), scm(nf9);
+    auto get = [&](void* dst, const void* src, size_t bytes) -> mba_status {
+      if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+      return MBA_OK;
+    };
+    MBA_TRY(get(S.cams.data(), c->cams64, 8 * S.cams.size()));
+    MBA_TRY(get(S.pts.data(), c->pts64, 8 * np3));
+    MBA_TRY(get(S.pscale.data(), c->pscale64, 8 * np3));
+    MBA_TRY(get(S.diag_p.data(), c->diag_p, 8 * np3));
+    MBA_TRY(get(S.cscale.data(), c->cscale64, 8 * nf9));
+    MBA_TRY(get(S.diag_c.data(), c->diag_c, 8 * nf9));
+    MBA_TRY(get(sp.data(), d.step_p, sizeof(T) * np3));
+    MBA_TRY(get(scm.data(), d.step_c, sizeof(T) * nf9));
+    MBA_TRY(sync());
+    S.step_p.assign(sp.begin(), sp.end());
+    S.step_c.assign(scm.begin(), scm.end());
+    S.radius = radius;
+    return write_lsqp(c, k, S);
+  }
+
   // Per-window trust-region state.
   struct WinLM {
     double radius = 0.0, decrease_factor = 2.0, cost = 0.0, x_norm = 0.0;
@@ -2257,6 +2320,10 @@ struct Engine {
       t_res += elapsed_s(c->tev[1], c->tev[2]);
       step_result(act, factored, &solved, &lin);
       cost_result(&new_cost, &cf);
+      // TrustRegionMinimizer numbers the dump by the last iteration record.
+      if (!c->dump_iters.empty() &&
+          std::find(c->dump_iters.begin(), c->dump_iters.end(), L[0].it.iteration) != c->dump_iters.end())
+        MBA_TRY(dump_step(L[0].it.iteration, L[0].radius));
       for (int w = 0; w < W; ++w) {
         const double v = sum_at(kSlotPlus, w);
         mcc[w] = -sum_at(kSlotModel, w);
@@ -2654,6 +2721,7 @@ mba_status mba_solve(const mba_options* options, mba_problem* problem, mba_summa
   mba_context* ctx = nullptr;
   mba_status s = context_create(problem, 1, options, &ctx);
   if (s != MBA_OK) return s;
+  ctx->src = problem;
   const double t_pre = now_s() - t0;
   s = mba_context_solve(ctx, summary, iterations, max_iterations);
   if (s == MBA_OK && summary->is_solution_usable) s = mba_context_download(ctx, problem);
@@ -2672,6 +2740,8 @@ mba_status mba_solve_batched(const mba_options* options, mba_problem* problems, 
   std::memset(summaries, 0, sizeof(mba_summary) * (size_t)num_problems);
   mba_options o;
   if (options) o = *options; else mba_default_options(&o);
+  if (o.num_dump_iterations > 0 && num_problems > 1)
+    return fail(MBA_ERR_UNSUPPORTED, "trust-region problem dumps are written for a single problem");
   bool any_grouped = false;
   for (int32_t w = 0; w < num_problems; ++w) any_grouped = any_grouped || problems[w].layout == MBA_LAYOUT_GROUPED;
   if (o.linear_solver != MBA_IMPLICIT_SCHUR_PCG || any_grouped) {
@@ -2831,6 +2901,166 @@ static mba_status evaluate_impl(const mba_problem* p, double huber_a, double* re
   pool.release();
   (void)hipStreamDestroy(s);
   return st;
+}
+
+// TEXTFILE dump of the linear least-squares problem of a step
+// (DumpLinearLeastSquaresProblemToTextFile, linear_least_squares_problems.cc:
+// 534-603; BlockSparseMatrix::ToTextFile, block_sparse_matrix.cc:222-243):
+// A = the reduced program's Jacobian, robustified (Corrector with rho'' <= 0:
+// rows times sqrt(rho'), corrector.cc:41-85) and Jacobi-scaled, row blocks
+// in residual order, each row's cells by column block (E blocks first),
+// local columns of each block; D = sqrt(clamped column norm^2 / radius);
+// b = robustified residuals; x = the step.
+mba_status write_lsqp(mba_context* c, int k, const LsqpState& S) {
+  const mba_problem& p0 = *c->src;
+  const Structure& st = c->wst[0];
+  const bool grouped = c->grouped;
+  const int32_t nc = p0.num_cameras, np = p0.num_points, ng = grouped ? p0.num_intrinsics : 0;
+  const int64_t n = p0.num_observations;
+  const int cw = grouped ? 6 : 9;
+  // Current parameters in the problem's layout.
+  std::vector<double> cams((size_t)cw * nc), intr(6 * (size_t)ng), pts(p0.points, p0.points + 3 * (size_t)np);
+  for (int32_t cc = 0; cc < nc; ++cc)
+    std::memcpy(cams.data() + (size_t)cw * cc, S.cams.data() + 9 * (size_t)cc, 8 * (size_t)cw);
+  for (int32_t g = 0; g < ng; ++g)
+    std::memcpy(intr.data() + 6 * (size_t)g, S.cams.data() + 9 * ((size_t)nc + g), 48);
+  for (int32_t e = 0; e < c->ne; ++e)
+    std::memcpy(pts.data() + 3 * (size_t)c->dev_e_to_pt[e], S.pts.data() + 3 * (size_t)e, 24);
+  mba_problem q = p0;
+  q.cameras = cams.data();
+  q.points = pts.data();
+  if (grouped) q.intrinsics = intr.data();
+  std::vector<double> r(2 * (size_t)n), jc((size_t)2 * cw * n), jp(6 * (size_t)n), ji(grouped ? 12 * (size_t)n : 0);
+  MBA_TRY(evaluate_impl<double>(&q, 0.0, r.data(), grouped ? ji.data() : nullptr, jc.data(), jp.data(), nullptr,
+                                c->device));
+  // Columns: E blocks (3 each) in elimination order, then the F blocks'
+  // local columns (the free entries of each slot block) in F order.
+  const int32_t ne = st.num_e, nslot = c->nslot;
+  std::vector<int32_t> pt_de(np, -1), pt_rank(np, -1), e_pt(ne);
+  for (int32_t e = 0; e < c->ne; ++e) pt_de[c->dev_e_to_pt[e]] = e;
+  for (int32_t i = 0; i < ne; ++i) {
+    const int32_t pt = st.order[i] - (grouped ? ng + nc : nc);
+    pt_rank[pt] = i;
+    e_pt[i] = pt;
+  }
+  std::vector<int32_t> slot_rank(nslot, -1), slot_col(nslot, 0), slot_bits(nslot, 0), f_slot;
+  int64_t ncols = 3 * (int64_t)ne;
+  for (size_t j = ne; j < st.order.size(); ++j) {
+    const int32_t id = st.order[j];
+    const int32_t b = grouped ? (id < ng ? nc + id : id - ng) : id;
+    int32_t bits;
+    if (grouped && b >= nc) bits = 0x3F & ~(p0.intrinsic_subset ? (int32_t)p0.intrinsic_subset[b - nc] : 0);
+    else bits = p0.camera_params ? p0.camera_params : (grouped ? 0x3F : 0x1FF);
+    slot_rank[b] = (int32_t)(j - ne);
+    slot_col[b] = (int32_t)ncols;
+    slot_bits[b] = bits;
+    f_slot.push_back(b);
+    ncols += __builtin_popcount((unsigned)bits);
+  }
+  const int64_t nres = (int64_t)st.res_order.size();
+  const int64_t nrows = 2 * nres;
+  std::string base = c->dump_dir;
+  char name[64];
+  std::snprintf(name, sizeof(name), "ceres_solver_iteration_%03d", k);
+  if (!base.empty() && base.back() != '/') base += '/';
+  base += name;
+  auto open = [&](const std::string& f, FILE** fp) -> mba_status {
+    *fp = std::fopen(f.c_str(), "w");
+    return *fp ? MBA_OK : fail(MBA_ERR_INVALID_ARGUMENT, "cannot write " + f);
+  };
+  const double a = c->opt.huber_a;
+  std::vector<double> w(nres);
+  {
+    FILE* fa = nullptr;
+    MBA_TRY(open(base + "_A.txt", &fa));
+    struct Cell {
+      int32_t id, col, size;
+      double v[18];
+    };
+    for (int64_t row = 0; row < nres; ++row) {
+      const int64_t i = st.res_order[row];
+      const double r0 = r[2 * i], r1 = r[2 * i + 1], sq = r0 * r0 + r1 * r1;
+      w[row] = (a > 0.0 && sq > a * a) ? std::sqrt(std::max(std::numeric_limits<double>::min(), a / std::sqrt(sq)))
+                                       : 1.0;
+      Cell cells[3];
+      int nce = 0;
+      const int32_t pt = p0.point_index[i];
+      if (pt_rank[pt] >= 0) {
+        Cell& ce = cells[nce++];
+        ce.id = pt_rank[pt];
+        ce.col = 3 * pt_rank[pt];
+        ce.size = 3;
+        for (int rr = 0; rr < 2; ++rr)
+          for (int q3 = 0; q3 < 3; ++q3)
+            ce.v[3 * rr + q3] = jp[6 * i + 3 * rr + q3] * S.pscale[3 * (size_t)pt_de[pt] + q3] * w[row];
+      }
+      auto add_f = [&](int32_t b, const double* J, int width) {
+        if (slot_rank[b] < 0) return;
+        const int32_t f = c->cam_to_f[b];
+        Cell& ce = cells[nce++];
+        ce.id = ne + slot_rank[b];
+        ce.col = slot_col[b];
+        ce.size = __builtin_popcount((unsigned)slot_bits[b]);
+        for (int rr = 0, m = 0; rr < 2; ++rr)
+          for (int kk = 0; kk < 9; ++kk)
+            if ((slot_bits[b] >> kk) & 1) ce.v[m++] = J[width * rr + kk] * S.cscale[9 * (size_t)f + kk] * w[row];
+      };
+      add_f(p0.camera_index[i], jc.data() + (size_t)2 * cw * i, cw);
+      if (grouped) add_f(nc + p0.intrinsic_index[i], ji.data() + 12 * (size_t)i, 6);
+      std::sort(cells, cells + nce, [](const Cell& x, const Cell& y) { return x.id < y.id; });
+      for (int t = 0; t < nce; ++t)
+        for (int rr = 0; rr < 2; ++rr)
+          for (int q = 0; q < cells[t].size; ++q)
+            std::fprintf(fa, "% 10d % 10d %17f\n", (int)(2 * row + rr), cells[t].col + q,
+                         cells[t].v[cells[t].size * rr + q]);
+    }
+    std::fclose(fa);
+  }
+  auto write_array = [&](const std::string& f, const std::vector<double>& v) -> mba_status {
+    FILE* fp = nullptr;
+    MBA_TRY(open(f, &fp));
+    for (double x : v) std::fprintf(fp, "%17f\n", x);
+    std::fclose(fp);
+    return MBA_OK;
+  };
+  std::vector<double> D, b, x;
+  D.reserve(ncols);
+  x.reserve(ncols);
+  for (int32_t i = 0; i < ne; ++i)
+    for (int q3 = 0; q3 < 3; ++q3) {
+      const size_t at = 3 * (size_t)pt_de[e_pt[i]] + q3;
+      D.push_back(std::sqrt(S.diag_p[at] / S.radius));
+      x.push_back(S.step_p[at]);
+    }
+  for (int32_t bs : f_slot) {
+    const int32_t f = c->cam_to_f[bs];
+    for (int kk = 0; kk < 9; ++kk)
+      if ((slot_bits[bs] >> kk) & 1) {
+        D.push_back(std::sqrt(S.diag_c[9 * (size_t)f + kk] / S.radius));
+        x.push_back(S.step_c[9 * (size_t)f + kk]);
+      }
+  }
+  b.reserve(nrows);
+  for (int64_t row = 0; row < nres; ++row) {
+    const int64_t i = st.res_order[row];
+    b.push_back(r[2 * i] * w[row]);
+    b.push_back(r[2 * i + 1] * w[row]);
+  }
+  MBA_TRY(write_array(base + "_D.txt", D));
+  MBA_TRY(write_array(base + "_b.txt", b));
+  MBA_TRY(write_array(base + "_x.txt", x));
+  FILE* fm = nullptr;
+  MBA_TRY(open(base + ".m", &fm));
+  std::fprintf(fm, "function lsqp = load_trust_region_problem()\n");
+  std::fprintf(fm, "lsqp.num_rows = %d;\n", (int)nrows);
+  std::fprintf(fm, "lsqp.num_cols = %d;\n", (int)ncols);
+  std::fprintf(fm, "tmp = load('%s_A.txt', '-ascii');\n", base.c_str());
+  std::fprintf(fm, "lsqp.A = sparse(tmp(:, 1) + 1, tmp(:, 2) + 1, tmp(:, 3), %d, %d);\n", (int)nrows, (int)ncols);
+  std::fprintf(fm, "lsqp.D = load('%s_D.txt', '-ascii');\n", base.c_str());
+  std::fprintf(fm, "lsqp.b = load('%s_b.txt', '-ascii');\n", base.c_str());
+  std::fprintf(fm, "lsqp.x = load('%s_x.txt', '-ascii');\n", base.c_str());
+  std::fclose(fm);
+  return MBA_OK;
 }
 
 extern "C" {

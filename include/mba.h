@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define MBA_ABI_VERSION 2
+#define MBA_ABI_VERSION 3
 
 typedef enum mba_status {
   MBA_OK = 0,
@@ -108,7 +108,14 @@ typedef enum mba_preconditioner {
   /* Exact diagonal blocks of S (schur_jacobi_preconditioner.cc:83-103):
    * U + D^2 - sum over the camera's rows of W V^-1 W'.  Packed cameras that
    * observe each point at most once. */
-  MBA_PRECOND_SCHUR_JACOBI = 1
+  MBA_PRECOND_SCHUR_JACOBI = 1,
+  /* The visibility-based preconditioners need SuiteSparse, which the
+   * reference's Ceres build leaves off (3rdparty/ceres/CMakeLists.txt:72):
+   * its Solve rejects them for every linear solver (solver.cc:135-149).
+   * So does this library: MBA_ERR_UNSUPPORTED, with the reference's
+   * message, before any device work. */
+  MBA_PRECOND_CLUSTER_JACOBI = 2,
+  MBA_PRECOND_CLUSTER_TRIDIAGONAL = 3
 } mba_preconditioner;
 
 /* Arithmetic of the residual / Jacobian / Schur / PCG kernels.  The LM cost
@@ -188,6 +195,22 @@ typedef struct mba_options {
    * MIXED) Cholesky solve of S; 0 disables. */
   int32_t refinement_iterations;              /* 2 */
   int32_t preconditioner;                     /* mba_preconditioner, JACOBI */
+  /* Trust-region problem dumps, TEXTFILE format
+   * (Solver::Options::trust_region_minimizer_iterations_to_dump and
+   * trust_region_problem_dump_directory; trust_region_minimizer.cc:287-301,
+   * linear_least_squares_problems.cc:534-603).  For every listed k, the
+   * linear least-squares problem of the step computed after iteration
+   * record k is written as <dump_directory>/ceres_solver_iteration_%03d
+   * {_A.txt, _D.txt, _b.txt, _x.txt, .m}: A the Jacobi-scaled, robustified
+   * Jacobian of the reduced program ("row col value" per entry, row blocks
+   * in residual order, E columns first), D the LM diagonal, b the
+   * robustified residuals, x the step.  A is evaluated in FP64 at the
+   * current parameters.  mba_solve only; a non-empty list needs a non-empty
+   * directory (MBA_ERR_INVALID_ARGUMENT, as solver.cc:243-247). */
+  const char* dump_directory;                 /* NULL */
+  const int32_t* dump_iterations;             /* NULL */
+  int32_t num_dump_iterations;                /* 0 */
+  int32_t reserved2;
 } mba_options;
 
 typedef struct mba_iteration {

@@ -361,6 +361,27 @@ def ref_set_schur_jacobi(on: bool):
     ref_lib().ref_set_schur_jacobi(C.c_int(1 if on else 0))
 
 
+def ref_set_preconditioner(kind: int):
+    """Solver::Options::preconditioner_type of ref_solve: 0 JACOBI, 1 SCHUR_JACOBI,
+    2 CLUSTER_JACOBI, 3 CLUSTER_TRIDIAGONAL."""
+    ref_lib().ref_set_preconditioner(C.c_int(kind))
+
+
+def ref_set_dump(directory, iterations):
+    """trust_region_minimizer_iterations_to_dump / trust_region_problem_dump_directory
+    (TEXTFILE) of ref_solve; an empty list turns dumps off."""
+    its = (C.c_int * max(1, len(iterations)))(*iterations)
+    ref_lib().ref_set_dump(None if directory is None else os.fsencode(directory), its, C.c_int(len(iterations)))
+
+
+def ref_last_message() -> str:
+    """Solver::Summary::message of the last ref_solve."""
+    n = ref_lib().ref_last_message(None, C.c_int(0))
+    buf = C.create_string_buffer(n + 1)
+    ref_lib().ref_last_message(buf, C.c_int(n + 1))
+    return buf.value.decode()
+
+
 def ref_rotation_matrix_to_angle_axis(R):
     """ceres::RotationMatrixToAngleAxis of (n, 3, 3) row-major matrices."""
     R = np.asarray(R, np.float64).reshape(-1, 3, 3)

@@ -189,8 +189,25 @@ void AddAll(const Scene& s, P* problem, ceres::LossFunction* loss,
 // layout 1 the cameras and intrinsics are moved into one buffer
 // [cameras | intrinsics] (the libs/sfm container order) while solving.
 int g_points_first = 0;
-// Preconditioner of ITERATIVE_SCHUR in ref_solve: 0 JACOBI, 1 SCHUR_JACOBI.
-int g_schur_jacobi = 0;
+// Preconditioner of ITERATIVE_SCHUR in ref_solve: 0 JACOBI, 1 SCHUR_JACOBI,
+// 2 CLUSTER_JACOBI, 3 CLUSTER_TRIDIAGONAL (the last two need SuiteSparse,
+// which the reference's Ceres build leaves off: Solve rejects the options).
+int g_precond = 0;
+// Solver::Options::trust_region_minimizer_iterations_to_dump and
+// trust_region_problem_dump_directory (TEXTFILE) of ref_solve.
+std::vector<int> g_dump_iters;
+std::string g_dump_dir;
+// Summary::message of the last ref_solve.
+std::string g_message;
+
+ceres::PreconditionerType PrecondType(int k) {
+  switch (k) {
+    case 1: return ceres::SCHUR_JACOBI;
+    case 2: return ceres::CLUSTER_JACOBI;
+    case 3: return ceres::CLUSTER_TRIDIAGONAL;
+    default: return ceres::JACOBI;
+  }
+}
 
 void AddPointsFirst(const Scene& s, const std::vector<double*>& blocks, ceres::ParameterBlockOrdering* o) {
   for (double* b : blocks) o->AddElementToGroup(b, (b >= s.pts && b < s.pts + 3L * s.np) ? 0 : 1);
@@ -221,7 +238,21 @@ struct ContiguousCamsIntr {
 extern "C" {
 
 void ref_set_points_first(int on) { g_points_first = on; }
-void ref_set_schur_jacobi(int on) { g_schur_jacobi = on; }
+void ref_set_schur_jacobi(int on) { g_precond = on ? 1 : 0; }
+void ref_set_preconditioner(int k) { g_precond = k; }
+void ref_set_dump(const char* dir, const int* iters, int n) {
+  g_dump_dir = dir ? dir : "";
+  g_dump_iters.assign(iters, iters + (n > 0 ? n : 0));
+}
+// Copies the last ref_solve's Summary::message; returns its length.
+int ref_last_message(char* buf, int cap) {
+  if (buf && cap > 0) {
+    const int n = std::min<int>(cap - 1, (int)g_message.size());
+    std::copy(g_message.begin(), g_message.begin() + n, buf);
+    buf[n] = 0;
+  }
+  return (int)g_message.size();
+}
 
 // ceres::RotationMatrixToAngleAxis on n column-major matrices.
 void ref_rotation_matrix_to_angle_axis(const double* R, double* aa, long n) {
@@ -393,7 +424,6 @@ int ref_solve(int layout, int nc, int ni, int np, long nobs, double* cams, doubl
   ceres::LossFunction* loss = huber_a > 0.0 ? new ceres::HuberLoss(huber_a) : NULL;
   AddAll(s, &problem, loss, NULL);
   ceres::Solver::Options options;
-  if (g_schur_jacobi) options.preconditioner_type = ceres::SCHUR_JACOBI;
   if (g_points_first) {
     std::vector<double*> blocks;
     problem.GetParameterBlocks(&blocks);
@@ -401,7 +431,7 @@ int ref_solve(int layout, int nc, int ni, int np, long nobs, double* cams, doubl
     AddPointsFirst(s, blocks, o);
     options.linear_solver_ordering.reset(o);
   }
-  options.preconditioner_type = g_schur_jacobi ? ceres::SCHUR_JACOBI : ceres::JACOBI;
+  options.preconditioner_type = PrecondType(g_precond);
   options.linear_solver_type = linear_solver == 1 ? ceres::ITERATIVE_SCHUR
                                : linear_solver == 2 ? ceres::DENSE_SCHUR
                                                     : ceres::SPARSE_SCHUR;
@@ -411,8 +441,12 @@ int ref_solve(int layout, int nc, int ni, int np, long nobs, double* cams, doubl
   options.max_num_iterations = max_iters;
   options.num_threads = num_threads;
   options.num_linear_solver_threads = num_threads;
+  options.trust_region_minimizer_iterations_to_dump = g_dump_iters;
+  options.trust_region_problem_dump_directory = g_dump_dir;
+  options.trust_region_problem_dump_format_type = ceres::TEXTFILE;
   ceres::Solver::Summary sum;
   ceres::Solve(options, &problem, &sum);
+  g_message = sum.message;
   summary[0] = sum.initial_cost;
   summary[1] = sum.final_cost;
   summary[2] = sum.num_successful_steps;

@@ -152,6 +152,38 @@ def write_bal_scene():
             fo.write("%.17g\n" % v)
 
 
+# Trust-region problem dumps (TEXTFILE): small scenes so that the A files
+# stay a few thousand entries.  Packed: Huber 4 px, 10% outliers, camera 2
+# constant.  Grouped: three groups with GROUP_SUBSET masks, group 1
+# constant, camera 2 constant.
+DUMP_SPEC = dict(num_cameras=6, num_points=60, track_min=2, track_max=5, outlier_frac=0.1, seed=43)
+DUMP_GROUP_SPEC = dict(num_cameras=8, num_points=50, track_min=2, track_max=5, outlier_frac=0.1, seed=44)
+DUMP_ITERATIONS = [0, 1, 3]
+DUMP_CONST_CAMERA = 2
+
+
+def dump_scene():
+    return O.generate(O.make_spec(**DUMP_SPEC))
+
+
+def dump_group_scene():
+    return O.generate_groups(O.make_spec(**DUMP_GROUP_SPEC), GROUPS)
+
+
+def dump_const(n):
+    c = np.zeros(n, np.uint8)
+    c[DUMP_CONST_CAMERA] = 1
+    return c
+
+
+def read_dump(base):
+    """The reference's TEXTFILE dump files as arrays, the .m text as is."""
+    a = np.loadtxt(base + "_A.txt", ndmin=2)
+    return dict(rows=a[:, 0].astype(np.int32), cols=a[:, 1].astype(np.int32), values=a[:, 2],
+                D=np.loadtxt(base + "_D.txt"), b=np.loadtxt(base + "_b.txt"), x=np.loadtxt(base + "_x.txt"),
+                m=open(base + ".m").read())
+
+
 def tiny_scene():
     """Hand-built: 4 cameras, 6 points; camera 3 sees only points 0 and 1."""
     spec = O.make_spec(3, 6, 2, 3, seed=5)
@@ -343,6 +375,43 @@ def main():
             out[f"{name}_cameras"] = sc.cameras
     finally:
         O.ref_set_schur_jacobi(False)
+    # 10. Trust-region problem dumps (TEXTFILE) of SPARSE_SCHUR solves, Huber
+    #     4 px: the scaled, robustified Jacobian of the reduced program, the LM
+    #     diagonal, the residuals and the step at DUMP_ITERATIONS; the .m
+    #     loader with the directory replaced by {DIR}.
+    import tempfile
+    for name, sc in [("dump", O.Scene(dump_scene(), cam_constant=dump_const(DUMP_SPEC["num_cameras"]))),
+                     ("dump_grp", O.group_scene(dump_group_scene(),
+                                                cam_constant=dump_const(DUMP_GROUP_SPEC["num_cameras"]),
+                                                intr_constant=GROUP_INTR_CONST, intr_subset=GROUP_SUBSET))]:
+        tmp = tempfile.mkdtemp()
+        O.ref_set_dump(tmp, DUMP_ITERATIONS)
+        try:
+            r = O.ref_solve(sc, huber_a=4.0, linear_solver=0, max_iters=6)
+        finally:
+            O.ref_set_dump(None, [])
+        for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
+            out[f"{name}_{k}"] = np.asarray(r[k])
+        for it in DUMP_ITERATIONS:
+            dd = read_dump(os.path.join(tmp, f"ceres_solver_iteration_{it:03d}"))
+            for k in ["rows", "cols", "values", "D", "b", "x"]:
+                out[f"{name}_{it}_{k}"] = dd[k]
+            out[f"{name}_{it}_m"] = np.asarray(dd["m"].replace(tmp, "{DIR}"))
+    # 11. CLUSTER_JACOBI / CLUSTER_TRIDIAGONAL: the reference's Ceres is built
+    #     without SuiteSparse, so Solve rejects them (solver.cc:135-149) for
+    #     either linear solver: termination type and message.
+    dc = dump_scene()
+    for kind, pname in [(2, "cluster_jacobi"), (3, "cluster_tridiagonal")]:
+        O.ref_set_preconditioner(kind)
+        try:
+            for ls in (0, 1):
+                r = O.ref_solve(O.Scene(dc), huber_a=4.0, linear_solver=ls, max_iters=5)
+                out[f"{pname}_{ls}_termination"] = np.asarray(r["termination"])
+                out[f"{pname}_{ls}_usable"] = np.asarray(r["usable"])
+                out[f"{pname}_{ls}_num_iterations"] = np.asarray(len(r["iterations"]))
+                out[f"{pname}_{ls}_message"] = np.asarray(O.ref_last_message())
+        finally:
+            O.ref_set_preconditioner(0)
     path = os.path.join(HERE, "ceres_ref.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")

@@ -23,9 +23,38 @@ def test_library_exports_every_declared_symbol(mba):
     for name in syms:
         assert hasattr(lib, name), name
     assert set(syms) == set(mba.EXPORTS)
-    assert lib.mba_abi_version() == 2
+    assert lib.mba_abi_version() == 3
     # mba_problem: 4 int32, 1 int64, 10 pointers, 2 int32 (x86-64 layout).
     assert ctypes.sizeof(mba._Problem) == 4 * 4 + 8 + 10 * 8 + 2 * 4
+
+
+def test_struct_layouts_match_the_header(mba, tmp_path):
+    """sizeof and field offsets of the ctypes mirrors against the C compiler's
+    reading of include/mba.h."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    checks = {"mba_options": (mba.Options, ["eta", "preconditioner", "dump_directory", "dump_iterations",
+                                            "num_dump_iterations"]),
+              "mba_problem": (mba._Problem, []), "mba_summary": (mba._Summary, ["num_linear_solver_iterations"]),
+              "mba_iteration": (mba.Iteration, ["cumulative_time_s"])}
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "mba.h"', "int main(void) {"]
+    for cname, (_, fields) in checks.items():
+        src.append(f'  printf("%zu\\n", sizeof({cname}));')
+        for f in fields:
+            src.append(f'  printf("%zu\\n", offsetof({cname}, {f}));')
+    src.append("  return 0;\n}")
+    (tmp_path / "l.c").write_text("\n".join(src))
+    exe = tmp_path / "l"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(tmp_path / "l.c"), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = []
+    for cname, (cls, fields) in checks.items():
+        want.append(ctypes.sizeof(cls))
+        want.extend(getattr(cls, f).offset for f in fields)
+    assert got == want
 
 
 def test_default_options_are_the_reference_defaults(mba):

@@ -6,7 +6,11 @@
     read by the library against an independent parse, and their Schur
     structure under a points-first ordering against the reference's Ceres;
   * BAL text: the reader against a plain parse of the text, the writer
-    byte for byte against the reference's BALProblem::WriteToFile (digest).
+    byte for byte against the reference's BALProblem::WriteToFile (digest);
+  * trust-region problem dumps (TEXTFILE): option checks as Ceres's
+    (solver.cc:243-247) and the reader on files of that layout;
+  * CLUSTER_JACOBI / CLUSTER_TRIDIAGONAL rejected with the reference's
+    termination and message (its Ceres has no SuiteSparse).
 """
 import hashlib
 import os
@@ -105,3 +109,55 @@ def test_bal_reader_and_writer(mba, golden, mg, tmp_path):
     assert hashlib.sha256(out.read_bytes()).hexdigest() == str(golden["bal_write_sha256"])
     with pytest.raises(mba.MbaError):
         mba.read_bal(str(tmp_path / "missing.txt"))
+
+
+@pytest.mark.parametrize("kind,name", [(2, "cluster_jacobi"), (3, "cluster_tridiagonal")])
+@pytest.mark.parametrize("solver", [0, 1])
+def test_cluster_preconditioners_rejected_as_the_reference(mba, golden, mg, kind, name, solver):
+    """The reference's Solve fails option validation (FAILURE, no iterations,
+    solution not usable); the library refuses before any device work with
+    the same message."""
+    assert int(golden[f"{name}_{solver}_termination"]) == mba.FAILURE
+    assert int(golden[f"{name}_{solver}_num_iterations"]) == 0 and not bool(golden[f"{name}_{solver}_usable"])
+    d = mg.dump_scene()
+    p = mba.BAProblem(d["cam_init"], d["pt_init"], d["obs"], d["cam_idx"], d["pt_idx"])
+    with pytest.raises(mba.MbaError) as e:
+        mba.solve(p, mba.default_options(linear_solver=solver, preconditioner=kind, huber_a=4.0))
+    assert e.value.status == mba.ERR_UNSUPPORTED
+    assert str(e.value).endswith(str(golden[f"{name}_{solver}_message"]))
+
+
+def test_dump_options_are_checked_before_device_work(mba, mg, tmp_path):
+    d = mg.dump_scene()
+    p = mba.BAProblem(d["cam_init"], d["pt_init"], d["obs"], d["cam_idx"], d["pt_idx"])
+    for directory in ("", None):
+        with pytest.raises(mba.MbaError) as e:
+            mba.solve(p, mba.default_options(dump_directory=directory, dump_iterations=[0]))
+        assert e.value.status == mba.ERR_INVALID_ARGUMENT
+        assert str(e.value).endswith("Solver::Options::trust_region_problem_dump_directory is empty.")
+    with pytest.raises(mba.MbaError) as e:
+        mba.solve_batched([p, p], mba.default_options(linear_solver=mba.IMPLICIT_SCHUR_PCG,
+                                                      dump_directory=str(tmp_path), dump_iterations=[0]))
+    assert e.value.status == mba.ERR_UNSUPPORTED
+
+
+def test_trust_region_problem_reader(mba, tmp_path):
+    """Files in the layout of DumpLinearLeastSquaresProblemToTextFile
+    (linear_least_squares_problems.cc:534-603; "% 10d % 10d %17f" entries)."""
+    base = str(tmp_path / "ceres_solver_iteration_004")
+    rows, cols, vals = [0, 0, 1, 3], [0, 2, 1, 4], [1.5, -0.25, 1e-6, 123456.789]
+    with open(base + "_A.txt", "w") as f:
+        for r, c, v in zip(rows, cols, vals):
+            f.write("% 10d % 10d %17f\n" % (r, c, v))
+    for key, arr in [("D", [0.1, 0.2, 0.3, 0.4, 0.5]), ("b", [1.0, -2.0, 3.0, -4.0]), ("x", [5, 4, 3, 2, 1])]:
+        with open(f"{base}_{key}.txt", "w") as f:
+            f.writelines("%17f\n" % v for v in arr)
+    with open(base + ".m", "w") as f:
+        f.write("function lsqp = load_trust_region_problem()\nlsqp.num_rows = 4;\nlsqp.num_cols = 5;\n")
+    t = mba.read_trust_region_problem(base)
+    assert (t["num_rows"], t["num_cols"]) == (4, 5)
+    np.testing.assert_array_equal(t["rows"], rows)
+    np.testing.assert_array_equal(t["cols"], cols)
+    np.testing.assert_allclose(t["values"], vals, atol=5e-7)
+    np.testing.assert_allclose(t["x"], [5, 4, 3, 2, 1])
+    assert t["D"].shape == (5,) and t["b"].shape == (4,)
