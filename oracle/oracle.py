@@ -339,6 +339,19 @@ def ref_solve(scene: Scene, huber_a=0.0, linear_solver=0, max_iters=50, num_thre
                 linear_solver_time=summ[7], usable=bool(usable), iterations=its[:min(n, max_rec)].copy())
 
 
+def ref_jacobian_schur_pass(scene: Scene, huber_a=0.0, num_threads=1, reps=1):
+    """One timed Jacobian + Schur pass of the reference's SPARSE_SCHUR path
+    (ProgramEvaluator::Evaluate, ScaleColumns, SchurEliminator::Eliminate) on
+    num_threads threads: (seconds per pass, set-up seconds, residual blocks)."""
+    sec = np.zeros(2)
+    f = ref_lib().ref_jacobian_schur_pass
+    f.restype = C.c_long
+    n = f(*_ref_args(scene), C.c_double(huber_a), C.c_int(num_threads), C.c_int(reps), ptr(sec))
+    if n < 0:
+        raise RuntimeError(f"ref_jacobian_schur_pass failed ({n})")
+    return float(sec[0]), float(sec[1]), int(n)
+
+
 def ref_evaluate_obs(layout, cam, pt, obs, intr=None):
     r = np.zeros(2)
     if layout == 0:

@@ -41,6 +41,9 @@
 #include "ceres/block_random_access_sparse_matrix.h"
 #undef private
 #undef protected
+#include "ceres/detect_structure.h"
+#include "ceres/schur_eliminator.h"
+#include <chrono>
 
 namespace {
 
@@ -403,6 +406,80 @@ int ref_schur_ordering(int layout, int nc, int ni, int np, long nobs, double* ca
     }
   }
   return ne;
+}
+
+// One "Jacobian + Schur" pass of the reference's SPARSE_SCHUR path, as its
+// LM iteration runs it: ProgramEvaluator::Evaluate (residuals, robustified
+// Jacobian; program_evaluator.h:138-285), BlockSparseMatrix::ScaleColumns
+// (trust_region_minimizer.cc:239-244), then SchurEliminator::Eliminate into
+// the block-sparse S and rhs (schur_complement_solver.cc:120-128), on
+// num_threads OpenMP threads.  Problem building, ordering and storage set-up
+// are outside the timing.  seconds[0] = mean seconds per pass over `reps`
+// passes after one untimed pass; seconds[1] = the set-up time.  Returns the
+// number of residual blocks, or < 0 on error.
+long ref_jacobian_schur_pass(int layout, int nc, int ni, int np, long nobs, double* cams, double* intr,
+                             double* pts, const double* obs, const int* cam_idx, const int* intr_idx,
+                             const int* pt_idx, const unsigned char* cam_const, double huber_a,
+                             int num_threads, int reps, double* seconds) {
+  typedef std::chrono::steady_clock Clock;
+  const Clock::time_point t0 = Clock::now();
+  Scene s = {layout, nc, ni, np, nobs, cams, intr, pts, obs, cam_idx, intr_idx, pt_idx, cam_const};
+  ceres::Problem::Options po;
+  ceres::internal::ProblemImpl problem(po);
+  ceres::LossFunction* loss = huber_a > 0.0 ? new ceres::HuberLoss(huber_a) : NULL;
+  AddAll(s, &problem, loss, NULL);
+  std::vector<double*> removed;
+  double fixed_cost = 0.0;
+  std::string error;
+  ceres::internal::scoped_ptr<ceres::internal::Program> reduced(
+      problem.mutable_program()->CreateReducedProgram(&removed, &fixed_cost, &error));
+  if (reduced.get() == NULL) return -1;
+  ceres::ParameterBlockOrdering ordering;
+  for (size_t i = 0; i < reduced->parameter_blocks().size(); ++i)
+    ordering.AddElementToGroup(reduced->parameter_blocks()[i]->mutable_user_state(), 0);
+  if (!ceres::internal::ReorderProgramForSchurTypeLinearSolver(
+          ceres::SPARSE_SCHUR, ceres::CX_SPARSE, problem.parameter_map(), &ordering, reduced.get(), &error))
+    return -2;
+  const int ne = ordering.group_to_elements().begin()->second.size();
+  ceres::internal::Evaluator::Options eo;
+  eo.linear_solver_type = ceres::SPARSE_SCHUR;
+  eo.num_eliminate_blocks = ne;
+  eo.num_threads = num_threads;
+  ceres::internal::scoped_ptr<ceres::internal::Evaluator> ev(
+      ceres::internal::Evaluator::Create(eo, reduced.get(), &error));
+  if (ev.get() == NULL) return -3;
+  ceres::internal::scoped_ptr<ceres::internal::SparseMatrix> jac(ev->CreateJacobian());
+  ceres::internal::BlockSparseMatrix* A = static_cast<ceres::internal::BlockSparseMatrix*>(jac.get());
+  ceres::internal::LinearSolver::Options lo;
+  lo.type = ceres::SPARSE_SCHUR;
+  lo.sparse_linear_algebra_library_type = ceres::CX_SPARSE;
+  lo.elimination_groups.push_back(ne);
+  lo.elimination_groups.push_back((int)reduced->parameter_blocks().size() - ne);
+  lo.num_threads = num_threads;
+  ceres::internal::DetectStructure(*A->block_structure(), ne, &lo.row_block_size, &lo.e_block_size,
+                                   &lo.f_block_size);
+  ceres::internal::SparseSchurComplementSolver solver(lo);
+  solver.InitStorage(A->block_structure());
+  ceres::internal::scoped_ptr<ceres::internal::SchurEliminatorBase> elim(
+      ceres::internal::SchurEliminatorBase::Create(lo));
+  elim->Init(ne, A->block_structure());
+  const int ncols = A->num_cols();
+  std::vector<double> state(reduced->NumParameters()), residuals(A->num_rows()), scale(ncols, 0.5),
+      D(ncols, 1e-2), rhs(solver.lhs()->num_rows());
+  reduced->ParameterBlocksToStateVector(state.data());
+  double cost = 0.0;
+  const Clock::time_point t1 = Clock::now();
+  double total = 0.0;
+  for (int r = 0; r <= reps; ++r) {
+    const Clock::time_point a = Clock::now();
+    if (!ev->Evaluate(state.data(), &cost, residuals.data(), NULL, A)) return -4;
+    A->ScaleColumns(scale.data());
+    elim->Eliminate(A, residuals.data(), D.data(), solver.lhs_.get(), rhs.data());
+    if (r > 0) total += std::chrono::duration<double>(Clock::now() - a).count();
+  }
+  seconds[0] = reps > 0 ? total / reps : 0.0;
+  seconds[1] = std::chrono::duration<double>(t1 - t0).count();
+  return (long)reduced->NumResidualBlocks();
 }
 
 // ceres::Solve as libs/sfm configures it (sfm_incremental_engine.cpp:1449-1472):
