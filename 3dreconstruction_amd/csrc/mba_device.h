@@ -1907,56 +1907,108 @@ __global__ __launch_bounds__(64) void k_s_split_final(
 // The triangular solves with the factor run one launch per 64-row block.
 constexpr int kNB = 64;
 
-// One wave: factor the kb x kb diagonal block at (k, k) with lane i holding
-// row i in registers (columns exchanged by shuffles), write it back, and
-// write L11^-1 (64 x 64 column-major, zero-padded) from an LDS copy.  Rows
-// and columns past kb are padded with the identity.
+// Value of lane `l` (uniform) broadcast to the wave (v_readlane).
+__device__ __forceinline__ float lane_bcast(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Four waves: factor the kb x kb diagonal block at (k, k), write it back,
+// and write L11^-1 (64 x 64 column-major, zero-padded) and its transpose
+// (kept per block for the triangular solves).  Rows and columns past kb are
+// padded with the identity.  Thread (wave w, lane r) holds row r
+// of columns c = w + 4 q (q < 16) in registers.  Factor step j: the wave
+// owning column j takes the pivot from lane j, scales the column and
+// publishes it in LDS (two buffers, so one barrier per step; stored as
+// colp[c % 4][c / 4] so that a wave's 16 columns are contiguous); every
+// wave then applies the rank-1 update to its columns c > j without
+// branches (columns c <= j get a zero multiplier; entries above the
+// diagonal are updated too and cleared when their column is scaled).
+// Inverse: the factor goes to LDS and each wave forms its 16 columns of
+// L11^-1 by column-oriented forward substitution, lane r holding row r and
+// X[j][c] broadcast from lane j times 1 / L[j][j]; step j touches only the
+// columns c <= j + 3 that can be nonzero there.
 template <typename T>
-__global__ __launch_bounds__(64) void k_potrf_diag(T* __restrict__ A, int64_t n, int k, int kb,
-                                                   T* __restrict__ Linv, int32_t* __restrict__ info) {
-  __shared__ T D[kNB][kNB + 1];
-  const int lane = threadIdx.x;
-  T a[kNB];
+__global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ A, int64_t n, int k, int kb,
+                                                    T* __restrict__ Linv, T* __restrict__ LinvT,
+                                                    int32_t* __restrict__ info) {
+  __shared__ T colp[2][4][16];
+  __shared__ T Ls[kNB][kNB + 1];
+  __shared__ T rinv[kNB];
+  __shared__ int bad_w[4];
+  const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T a[16];
 #pragma unroll
-  for (int c = 0; c < kNB; ++c)
-    a[c] = (lane < kb && c < kb) ? A[(k + lane) + (int64_t)(k + c) * n] : (lane == c ? T(1) : T(0));
-  int bad = 0;
+  for (int q = 0; q < 16; ++q) {
+    const int c = w + 4 * q;
+    a[q] = (r < kb && c < kb) ? A[(k + r) + (int64_t)(k + c) * n] : (r == c ? T(1) : T(0));
+  }
+  int bad = 0;  // first non-positive pivot among this wave's columns, + 1
 #pragma unroll
   for (int j = 0; j < kNB; ++j) {
-    T d = __shfl(a[j], j, 64);
-    if (!(d > T(0))) {
-      bad = bad ? bad : j + 1;
-      d = T(1);
+    const int ow = j & 3, oq = j >> 2;
+    T(*cb)[16] = colp[j & 1];
+    if (w == ow) {
+      T d = lane_bcast(a[oq], j);
+      if (!(d > T(0))) {
+        bad = bad ? bad : j + 1;
+        d = T(1);
+      }
+      const T sj = sqrt(d);
+      const T v = r == j ? sj : (r > j ? a[oq] / sj : T(0));
+      a[oq] = v;
+      cb[r & 3][r >> 2] = v;
     }
-    const T sj = sqrt(d);
-    if (lane == j) a[j] = sj;
-    if (lane > j) a[j] /= sj;
-    const T lij = lane > j ? a[j] : T(0);
+    __syncthreads();
+    const T lr = cb[r & 3][r >> 2];
 #pragma unroll
-    for (int c = j + 1; c < kNB; ++c) {
-      const T lcj = __shfl(a[j], c, 64);  // L[c][j]
-      if (c <= lane) a[c] -= lij * lcj;
+    for (int q = 0; q < 16; ++q) {
+      const T m = (w + 4 * q > j) ? cb[w][q] : T(0);
+      a[q] = fma(-lr, m, a[q]);
     }
   }
-  if (lane == 0 && bad && bad <= kb && *info == 0) *info = k + bad;
 #pragma unroll
-  for (int c = 0; c < kNB; ++c) {
-    D[lane][c] = c <= lane ? a[c] : T(0);
-    if (lane < kb && c < kb && c <= lane) A[(k + lane) + (int64_t)(k + c) * n] = a[c];
+  for (int q = 0; q < 16; ++q) {
+    const int c = w + 4 * q;
+    Ls[r][c] = c <= r ? a[q] : T(0);
+    if (r < kb && c < kb && c <= r) A[(k + r) + (int64_t)(k + c) * n] = a[q];
+  }
+  if (r == 0) bad_w[w] = bad;
+  __syncthreads();
+  if (w == 0) rinv[r] = T(1) / Ls[r][r];
+  if (threadIdx.x == 0) {
+    int first = 0;
+    for (int v = 0; v < 4; ++v)
+      if (bad_w[v] && (!first || bad_w[v] < first)) first = bad_w[v];
+    if (first && first <= kb && *info == 0) *info = k + first;
   }
   __syncthreads();
-  // Column `lane` of L11^-1 by forward substitution (reads broadcast).
-  const int c = lane;
-  T x[kNB];
+  T x[16];
 #pragma unroll
-  for (int r = 0; r < kNB; ++r) {
-    T v = (r == c) ? T(1) : T(0);
+  for (int q = 0; q < 16; ++q) x[q] = (r == w + 4 * q) ? T(1) : T(0);
 #pragma unroll
-    for (int q = 0; q < r; ++q) v -= D[r][q] * x[q];
-    x[r] = (r >= c) ? v / D[r][r] : T(0);
+  for (int j = 0; j < kNB; ++j) {
+    const T rj = rinv[j];
+    const T lrj = Ls[r][j];
+    // Columns c = w + 4 q > j + 3 hold zeros in rows <= j: nothing to do.
+#pragma unroll
+    for (int q = 0; q <= (j >> 2); ++q) {
+      const T xj = lane_bcast(x[q], j) * rj;
+      x[q] = r == j ? xj : (r > j ? fma(-lrj, xj, x[q]) : x[q]);
+    }
   }
 #pragma unroll
-  for (int r = 0; r < kNB; ++r) Linv[r + kNB * c] = (r < kb && c < kb) ? x[r] : T(0);
+  for (int q = 0; q < 16; ++q) {
+    const int c = w + 4 * q;
+    const T v = (r < kb && c < kb) ? x[q] : T(0);
+    Linv[r + kNB * c] = v;
+    LinvT[c + kNB * r] = v;
+  }
 }
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -2067,90 +2119,87 @@ __global__ __launch_bounds__(256) void k_syrk_mfma(T* __restrict__ A, int64_t n,
   });
 }
 
-// Forward solve L y = b in place, step k (k = -1 first): the workgroup of
-// 64-row block i > k subtracts L[i, k] y_k; the one for block k + 1 then
-// solves its diagonal block.  Launch over blocks k+1 .. nb-1.
+// Triangular solves with the factor, one launch per 64-row block, using the
+// stored inverses of the diagonal blocks (no serial chain inside a step).
+// Forward L y = b, step k: every workgroup forms y_k = L_kk^-1 b_k (b_k is
+// final after step k - 1); the workgroup of block k writes y_k, the one of
+// block i > k subtracts L_ik y_k from b_i.  b and y are distinct vectors.
+// Launch over blocks k .. nb-1.
 template <typename T>
-__global__ __launch_bounds__(64) void k_trsv_lower_step(const T* __restrict__ L, int64_t n, int k,
-                                                        T* __restrict__ b) {
-  __shared__ T xs[kNB];
+__global__ __launch_bounds__(64) void k_trsv_lower_inv(const T* __restrict__ L, const T* __restrict__ Linv,
+                                                       int64_t n, int k, T* __restrict__ b, T* __restrict__ y) {
+  __shared__ T bs[kNB], xs[kNB];
   const int lane = threadIdx.x;
-  const int i = k + 1 + blockIdx.x;
+  const int64_t k0 = (int64_t)k * kNB;
+  const int kb = (int)min((int64_t)kNB, n - k0);
+  bs[lane] = lane < kb ? b[k0 + lane] : T(0);
+  __syncthreads();
+  const T* Li = Linv + (size_t)k * kNB * kNB;
+  T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) acc[c & 3] = fma(Li[lane + kNB * c], bs[c], acc[c & 3]);
+  const T yk = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  const int i = k + blockIdx.x;
+  if (i == k) {
+    if (lane < kb) y[k0 + lane] = yk;
+    return;
+  }
+  xs[lane] = yk;
+  __syncthreads();
+  // Block k precedes block i, so it is full.
   const int64_t i0 = (int64_t)i * kNB;
   const int ib = (int)min((int64_t)kNB, n - i0);
   const bool on = lane < ib;
-  T bi = on ? b[i0 + lane] : T(0);
-  if (k >= 0) {
-    const int64_t k0 = (int64_t)k * kNB;
-    xs[lane] = b[k0 + lane];  // block k is full (it precedes block i)
-    __syncthreads();
-    const T* col = L + (on ? i0 + lane : i0) + k0 * n;
-    T acc[4] = {T(0), T(0), T(0), T(0)};
+  const T* col = L + (on ? i0 + lane : i0) + k0 * n;
+  T a2[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-    for (int c = 0; c < kNB; ++c) acc[c & 3] += col[c * n] * xs[c];
-    if (on) bi -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
-  }
-  if (i == k + 1) {
-    // Diagonal block: lane r's row into registers, then the serial solve.
-    T l[kNB];
-#pragma unroll
-    for (int j = 0; j < kNB; ++j) l[j] = (on && j < ib) ? L[(i0 + lane) + (i0 + j) * n] : T(1);
-#pragma unroll
-    for (int j = 0; j < kNB; ++j) {
-      const T xj = __shfl(bi, j, 64) / __shfl(l[j], j, 64);
-      if (lane == j) bi = xj;
-      if (lane > j) bi -= l[j] * xj;
-    }
-  }
-  if (on) b[i0 + lane] = bi;
+  for (int c = 0; c < kNB; ++c) a2[c & 3] = fma(col[c * n], xs[c], a2[c & 3]);
+  if (on) b[i0 + lane] -= (a2[0] + a2[1]) + (a2[2] + a2[3]);
 }
 
-// Backward solve L' x = y in place, step k (k = nb first): the workgroup of
-// block i < k subtracts L[k, i]' x_k (tile staged through LDS); the one for
-// block k - 1 then solves its diagonal block.  Launch over blocks 0 .. k-1.
+// Backward L' x = y, step k (k = nb - 1 first): every workgroup forms
+// x_k = L_kk^-T y_k from the stored transpose; the workgroup of block k
+// writes x_k, the one of block i < k subtracts L_ki' x_k from y_i (the
+// 64 x ib tile of L staged through LDS for coalesced reads).  Launch over
+// blocks 0 .. k.
 template <typename T>
-__global__ __launch_bounds__(64) void k_trsv_upper_step(const T* __restrict__ L, int64_t n, int k, int nb,
-                                                        int i_first, T* __restrict__ b) {
+__global__ __launch_bounds__(64) void k_trsv_upper_inv(const T* __restrict__ L, const T* __restrict__ LinvT,
+                                                       int64_t n, int k, T* __restrict__ y, T* __restrict__ x) {
   __shared__ T tile[kNB][kNB + 1];
-  __shared__ T xs[kNB];
+  __shared__ T ys[kNB], xs[kNB];
   const int lane = threadIdx.x;
-  const int i = i_first + blockIdx.x;
-  const int64_t i0 = (int64_t)i * kNB;
-  const int ib = (int)min((int64_t)kNB, n - i0);
-  T yi = lane < ib ? b[i0 + lane] : T(0);
-  // Stage the 64 x ib tile L[r0 .. r0+63, i0 .. i0+ib) (rows past n: 0,
-  // columns past ib: identity-padded) through registers into LDS.
-  auto stage = [&](int64_t r0, int rb) {
-    T v[kNB];
-    const bool on = lane < rb;
-    const T* row = L + (on ? r0 + lane : r0) + i0 * n;
+  const int64_t k0 = (int64_t)k * kNB;
+  const int kb = (int)min((int64_t)kNB, n - k0);
+  ys[lane] = lane < kb ? y[k0 + lane] : T(0);
+  __syncthreads();
+  const T* Lt = LinvT + (size_t)k * kNB * kNB;
+  T acc[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-    for (int c = 0; c < kNB; ++c) v[c] = (on && c < ib) ? row[c * n] : T(0);
+  for (int c = 0; c < kNB; ++c) acc[c & 3] = fma(Lt[lane + kNB * c], ys[c], acc[c & 3]);
+  const T xk = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  const int i = blockIdx.x;
+  if (i == k) {
+    if (lane < kb) x[k0 + lane] = xk;
+    return;
+  }
+  xs[lane] = xk;
+  // Block i precedes block k, so it is full: tile[r][c] = L[k0 + r][i0 + c]
+  // for rows r < kb (zero past them).
+  const int64_t i0 = (int64_t)i * kNB;
+  {
+    const bool on = lane < kb;
+    const T* row = L + (on ? k0 + lane : k0) + i0 * n;
+    T v[kNB];
+#pragma unroll
+    for (int c = 0; c < kNB; ++c) v[c] = on ? row[c * n] : T(0);
 #pragma unroll
     for (int c = 0; c < kNB; ++c) tile[lane][c] = v[c];
-  };
-  if (k < nb) {
-    const int64_t k0 = (int64_t)k * kNB;
-    const int kb = (int)min((int64_t)kNB, n - k0);
-    xs[lane] = lane < kb ? b[k0 + lane] : T(0);
-    stage(k0, kb);
-    __syncthreads();
-    T acc[4] = {T(0), T(0), T(0), T(0)};
+  }
+  __syncthreads();
+  T a2[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-    for (int r = 0; r < kNB; ++r) acc[r & 3] += tile[r][lane] * xs[r];
-    yi -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
-    __syncthreads();
-  }
-  if (i == k - 1) {
-    stage(i0, ib);
-    __syncthreads();
-    for (int j = ib - 1; j >= 0; --j) {
-      const T xj = __shfl(yi, j, 64) / tile[j][j];
-      if (lane == j) yi = xj;
-      if (lane < j) yi -= tile[j][lane] * xj;
-    }
-  }
-  if (lane < ib) b[i0 + lane] = yi;
+  for (int r = 0; r < kNB; ++r) a2[r & 3] = fma(tile[r][lane], xs[r], a2[r & 3]);
+  y[i0 + lane] -= (a2[0] + a2[1]) + (a2[2] + a2[3]);
 }
 
 // FP64 residual r = b - S x (S column-major, both triangles stored) in two

@@ -683,6 +683,9 @@ struct Dev {
   T *step_p = nullptr, *step_c = nullptr;
   // explicit S
   T *S = nullptr, *L = nullptr, *sv = nullptr, *WY = nullptr, *Linv = nullptr, *cross = nullptr;
+  // Inverses of the factor's 64 x 64 diagonal blocks and their transposes
+  // (one per block), and the triangular solves' two work vectors.
+  T *LinvT = nullptr, *tb = nullptr, *ty = nullptr;
   T* s_scratch = nullptr;
   // SCHUR_JACOBI preconditioner terms (IMPLICIT_SCHUR_PCG only)
   T *partial_sj = nullptr, *Tsj = nullptr;
@@ -1899,14 +1902,17 @@ struct Engine {
     }
   }
 
-  // In-place solves with the factor: L y = v, then L' x = y.
+  // In-place solve with the factor: L y = v, then L' x = y (work vectors
+  // d.tb, d.ty; one launch per 64-row block and direction).
   void trsv(T* v, int64_t n) {
     const int nb = (int)((n + kNB - 1) / kNB);
-    for (int k = -1; k < nb - 1; ++k)
-      hipLaunchKernelGGL(k_trsv_lower_step<T>, dim3(nb - 1 - k), dim3(64), 0, s, d.L, n, k, v);
-    hipLaunchKernelGGL(k_trsv_upper_step<T>, dim3(1), dim3(64), 0, s, d.L, n, nb, nb, nb - 1, v);
-    for (int k = nb - 1; k >= 1; --k)
-      hipLaunchKernelGGL(k_trsv_upper_step<T>, dim3(k), dim3(64), 0, s, d.L, n, k, nb, 0, v);
+    (void)hipMemcpyAsync(d.tb, v, sizeof(T) * n, hipMemcpyDeviceToDevice, s);
+    for (int k = 0; k < nb; ++k)
+      hipLaunchKernelGGL(k_trsv_lower_inv<T>, dim3(nb - k), dim3(64), 0, s, (const T*)d.L, (const T*)d.Linv, n, k,
+                         d.tb, d.ty);
+    for (int k = nb - 1; k >= 0; --k)
+      hipLaunchKernelGGL(k_trsv_upper_inv<T>, dim3(k + 1), dim3(64), 0, s, (const T*)d.L, (const T*)d.LinvT, n, k,
+                         d.ty, v);
   }
 
   // Explicit Schur (single problem): assemble dense S, Cholesky, solve,
@@ -1921,7 +1927,11 @@ struct Engine {
       MBA_TRY(c->pool.alloc(&d.WY, (size_t)kWY * c->nrows * (c->grouped ? 2 : 1)));
       if (c->grouped) MBA_TRY(c->pool.alloc(&d.cross, 36 * (size_t)std::max(1, c->nc)));
       MBA_TRY(c->pool.alloc(&d.s_scratch, 81 * (size_t)std::max<int64_t>(1, c->s_nslots)));
-      MBA_TRY(c->pool.alloc(&d.Linv, (size_t)kNB * kNB));
+      const size_t nbk = (size_t)((n + kNB - 1) / kNB);
+      MBA_TRY(c->pool.alloc(&d.Linv, nbk * kNB * kNB));
+      MBA_TRY(c->pool.alloc(&d.LinvT, nbk * kNB * kNB));
+      MBA_TRY(c->pool.alloc(&d.tb, (size_t)n));
+      MBA_TRY(c->pool.alloc(&d.ty, (size_t)n));
       if (!c->gemv_part) MBA_TRY(c->pool.alloc(&c->gemv_part, (size_t)kGemvSplit * n));
     }
     const double inv_r = c->h_inv_r[0];
@@ -1961,11 +1971,13 @@ struct Engine {
     HIP_TRY(hipMemsetAsync(c->info, 0, sizeof(int32_t), s));
     for (int64_t k = 0; k < n; k += kNB) {
       const int kb = (int)std::min<int64_t>(kNB, n - k);
-      hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(64), 0, s, d.L, n, (int)k, kb, d.Linv, c->info);
+      T* li = d.Linv + (size_t)(k / kNB) * kNB * kNB;
+      hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, s, d.L, n, (int)k, kb, li,
+                         d.LinvT + (size_t)(k / kNB) * kNB * kNB, c->info);
       const int64_t rest = n - k - kb;
       if (rest > 0) {
         const int nt = (int)((rest + kNB - 1) / kNB);
-        hipLaunchKernelGGL(k_panel_mfma<T>, dim3(nt), dim3(256), 0, s, d.L, n, (int)k, kb, (const T*)d.Linv);
+        hipLaunchKernelGGL(k_panel_mfma<T>, dim3(nt), dim3(256), 0, s, d.L, n, (int)k, kb, (const T*)li);
         hipLaunchKernelGGL(k_syrk_mfma<T>, dim3((unsigned)((int64_t)nt * (nt + 1) / 2)), dim3(256), 0, s, d.L, n,
                            (int)k, kb);
       }
@@ -1981,7 +1993,9 @@ struct Engine {
     // x = S^-1 rhs with FP64 iterative refinement of the residual.
     HIP_TRY(hipMemcpyAsync(d.x, d.rhs, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
     trsv(d.x, n);
-    if (c->opt.refinement_iterations > 0) {
+    // FP64 refinement of an FP32 (mixed) factorisation; an FP64 factor is
+    // used once, as the reference's sparse Cholesky solve is.
+    if (c->mixed && c->opt.refinement_iterations > 0) {
       hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, d.rhs, c->b64);
       hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, d.x, c->x64);
       for (int it = 0; it < c->opt.refinement_iterations; ++it) {

@@ -191,8 +191,9 @@ typedef struct mba_options {
    * LM strategy disables it (-1, levenberg_marquardt_strategy.cc:91-96). */
   double pcg_r_tolerance;                     /* -1 */
   double huber_a;                             /* <= 0: no loss (NULL) */
-  /* Explicit Schur: FP64 iterative-refinement sweeps after the (FP32 in
-   * MIXED) Cholesky solve of S; 0 disables. */
+  /* Explicit Schur, MIXED precision: FP64 iterative-refinement sweeps after
+   * the FP32 Cholesky solve of S; 0 disables.  An FP64 factorisation is
+   * used once, as the reference's sparse Cholesky solve is. */
   int32_t refinement_iterations;              /* 2 */
   int32_t preconditioner;                     /* mba_preconditioner, JACOBI */
   /* Trust-region problem dumps, TEXTFILE format
