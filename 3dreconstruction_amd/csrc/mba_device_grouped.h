@@ -398,6 +398,21 @@ __global__ __launch_bounds__(kBlock) void k_s_rows_g(
   }
 }
 
+// Merged group rows: row 2 nrows + m of WY = the sum of the group parts
+// (rows nrows + r) listed in m_row[m_start[m] .. m_start[m + 1]), in order.
+// One wave per merged row, lane k sums entry k.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_merge_rows(int64_t nm, const int32_t* __restrict__ m_start,
+                                                      const int32_t* __restrict__ m_row, int nrows,
+                                                      T* __restrict__ WY) {
+  const int64_t m = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int k = threadIdx.x & 63;
+  if (m >= nm || k >= kWY) return;
+  T v = T(0);
+  for (int q = m_start[m]; q < m_start[m + 1]; ++q) v += WY[(size_t)kWY * ((size_t)nrows + m_row[q]) + k];
+  WY[(size_t)kWY * (2 * (size_t)nrows + m) + k] = v;
+}
+
 // Per camera-major chunk: sum over its rows of J_e' J_i (6 x 6, unscaled).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cross_g(

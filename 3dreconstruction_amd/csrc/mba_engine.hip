@@ -762,6 +762,11 @@ struct mba_context {
   int32_t *s_split_blk = nullptr, *s_split_lo = nullptr, *s_split_hi = nullptr;
   int64_t s_nitems = 0, s_nsplit = 0, s_nslots = 0;
   int64_t s_nblk = 0, s_npairs = 0;
+  // Grouped layout: merged group rows (virtual rows 2 nrows + m), each the
+  // sum of one point's group parts (virtual rows nrows + r) whose cameras
+  // share an intrinsic group: rows s_m_row[s_m_start[m] .. s_m_start[m+1]).
+  int32_t *s_m_start = nullptr, *s_m_row = nullptr;
+  int64_t s_nmerged = 0;
   // Host copies of what was last uploaded.
   std::vector<double> h_inv_r;
   std::vector<uint8_t> h_active;
@@ -937,35 +942,71 @@ std::string window_prefix(int32_t W, int32_t w) {
 mba_status build_s_pairs(mba_context* c, const std::vector<int32_t>& pt_start,
                          const Buf<int32_t>& row_cam) {
   const int32_t nf = c->nf, ne = c->ne;
-  // Virtual rows: one per observation (packed), two in the grouped layout
-  // (its camera block at r, its group block at nrows + r).
-  const int nv = c->grouped ? 2 : 1;
-  if ((int64_t)nv * c->nrows > 0x7fffffffLL) return fail(MBA_ERR_UNSUPPORTED, "explicit Schur: too many rows");
-  std::vector<int32_t> row_f((size_t)nv * c->nrows);
+  // Virtual rows: one per observation (packed).  Grouped layout: the camera
+  // part of row r at r, its group part at nrows + r, and one merged row per
+  // (point, group F block) at 2 nrows + m summing the point's group parts
+  // in that group.  W and Y = W V^-1 are linear in the rows, so the group
+  // blocks of S pair merged rows: one pair per point for a group-group
+  // block and one per camera row for a camera-group block, instead of one
+  // per pair of the point's rows.
+  const int32_t nrows = c->nrows;
+  const bool grouped = c->grouped;
+  if ((grouped ? 3 : 1) * (int64_t)nrows > 0x7fffffffLL) return fail(MBA_ERR_UNSUPPORTED, "explicit Schur: too many rows");
+  std::vector<int32_t> row_f((size_t)(grouped ? 2 : 1) * nrows);
   parallel_for(c->nrows, [&](int64_t b, int64_t e) {
     for (int64_t r = b; r < e; ++r) {
       row_f[r] = c->cam_to_f[row_cam[r]];
-      if (nv == 2) row_f[c->nrows + r] = c->cam_gf[row_cam[r]];
+      if (grouped) row_f[nrows + r] = c->cam_gf[row_cam[r]];
     }
   });
-  const int32_t nrows = c->nrows;
+  std::vector<int32_t> e_m0, m_start, m_row;
+  if (grouped) {
+    e_m0.assign((size_t)ne + 1, 0);
+    m_start.push_back(0);
+    std::vector<int32_t> fm;  // F block of each merged row
+    for (int32_t e = 0; e < ne; ++e) {
+      const int32_t m0 = (int32_t)fm.size();
+      e_m0[e] = m0;
+      for (int32_t r = pt_start[e]; r < pt_start[e + 1]; ++r) {
+        const int32_t gf = row_f[(size_t)nrows + r];
+        if (gf < 0) continue;
+        bool seen = false;
+        for (int32_t m = m0; m < (int32_t)fm.size(); ++m) seen = seen || fm[m] == gf;
+        if (!seen) fm.push_back(gf);
+      }
+      for (int32_t m = m0; m < (int32_t)fm.size(); ++m) {
+        for (int32_t r = pt_start[e]; r < pt_start[e + 1]; ++r)
+          if (row_f[(size_t)nrows + r] == fm[m]) m_row.push_back(r);
+        m_start.push_back((int32_t)m_row.size());
+      }
+    }
+    e_m0[ne] = (int32_t)fm.size();
+    c->s_nmerged = (int64_t)fm.size();
+    row_f.insert(row_f.end(), fm.begin(), fm.end());
+    MBA_TRY(upload(c, &c->s_m_start, m_start));
+    MBA_TRY(upload(c, &c->s_m_row, m_row));
+  }
   const int nt = host_threads(ne, 4096);
   const int64_t step = ((int64_t)ne + nt - 1) / nt;
   // Pairs per fa, per thread (point ranges), then bucket offsets.
   std::vector<std::vector<int64_t>> cnt(nt, std::vector<int64_t>((size_t)nf, 0));
   auto for_pairs = [&](int t, auto&& fn) {
     const int64_t end = std::min<int64_t>(ne, (t + 1) * step);
-    for (int64_t e = t * step; e < end; ++e)
-      for (int ka = 0; ka < nv; ++ka)
-        for (int32_t ra = pt_start[e]; ra < pt_start[e + 1]; ++ra) {
-          const int32_t a = ka * nrows + ra, fa = row_f[a];
-          if (fa < 0) continue;
-          for (int kb = 0; kb < nv; ++kb)
-            for (int32_t rb = pt_start[e]; rb < pt_start[e + 1]; ++rb) {
-              const int32_t b = kb * nrows + rb, fb = row_f[b];
-              if (fb >= 0 && fb <= fa) fn(fa, a, b);
-            }
+    std::vector<int32_t> vr;
+    for (int64_t e = t * step; e < end; ++e) {
+      vr.clear();
+      for (int32_t r = pt_start[e]; r < pt_start[e + 1]; ++r)
+        if (row_f[r] >= 0) vr.push_back(r);
+      if (grouped)
+        for (int32_t m = e_m0[e]; m < e_m0[e + 1]; ++m) vr.push_back(2 * nrows + m);
+      for (const int32_t a : vr) {
+        const int32_t fa = row_f[a];
+        for (const int32_t b : vr) {
+          const int32_t fb = row_f[b];
+          if (fb <= fa) fn(fa, a, b);
         }
+      }
+    }
   };
   run_threads(nt, [&](int t) { for_pairs(t, [&](int32_t fa, int32_t, int32_t) { cnt[t][fa]++; }); });
   std::vector<int64_t> fa_start((size_t)nf + 1, 0);
@@ -1924,7 +1965,7 @@ struct Engine {
     if (!d.S) {
       MBA_TRY(c->pool.alloc(&d.S, (size_t)(n * n)));
       MBA_TRY(c->pool.alloc(&d.L, (size_t)(n * n)));
-      MBA_TRY(c->pool.alloc(&d.WY, (size_t)kWY * c->nrows * (c->grouped ? 2 : 1)));
+      MBA_TRY(c->pool.alloc(&d.WY, (size_t)kWY * ((size_t)c->nrows * (c->grouped ? 2 : 1) + c->s_nmerged)));
       if (c->grouped) MBA_TRY(c->pool.alloc(&d.cross, 36 * (size_t)std::max(1, c->nc)));
       MBA_TRY(c->pool.alloc(&d.s_scratch, 81 * (size_t)std::max<int64_t>(1, c->s_nslots)));
       const size_t nbk = (size_t)((n + kNB - 1) / kNB);
@@ -1943,6 +1984,10 @@ struct Engine {
         hipLaunchKernelGGL(k_s_rows_g<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
                            d.cam_to_f, (const int32_t*)c->d_cam_gf, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab,
                            d.Vinv, huber, c->nrows, d.WY);
+      if (c->s_nmerged > 0)
+        hipLaunchKernelGGL(k_merge_rows<T>, dim3((unsigned)((c->s_nmerged + kBlock / 64 - 1) / (kBlock / 64))),
+                           dim3(kBlock), 0, s, c->s_nmerged, (const int32_t*)c->s_m_start,
+                           (const int32_t*)c->s_m_row, c->nrows, d.WY);
       // The camera-group blocks of F'F (partials reuse the pass's scratch).
       if (c->nchunks > 0) {
         hipLaunchKernelGGL(k_cross_g<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin, d.chunk_end,
