@@ -8,6 +8,11 @@
 #include <array>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+
+#include <unistd.h>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -88,13 +93,77 @@ int host_threads(int64_t n, int64_t grain = 1 << 16) {
   return (int)std::max<int64_t>(1, std::min<int64_t>({std::min<int64_t>(16, hw), (n + grain - 1) / grain}));
 }
 
-// fn(t) for t in [0, nt) on nt threads (inline when nt == 1).
+// Persistent host workers (up to 15, started on first use, never joined):
+// run() hands fn(t), t = 1 .. nt-1, to workers, runs fn(0) on the caller
+// and returns when all have finished.  One job at a time: a call made while
+// the pool is busy (another thread's job, or a nested call), or made in a
+// child of fork() (which has no workers), returns false.
+class HostPool {
+ public:
+  static HostPool& get() {
+    static HostPool* p = new HostPool();
+    return *p;
+  }
+  bool run(int nt, const std::function<void(int)>& fn) {
+    if (nt - 1 > (int)nworkers_ || getpid() != pid_) return false;
+    std::unique_lock<std::mutex> busy(busy_, std::try_to_lock);
+    if (!busy.owns_lock()) return false;
+    {
+      std::lock_guard<std::mutex> g(m_);
+      job_ = &fn;
+      nparticipants_ = nt - 1;
+      remaining_ = nt - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    fn(0);
+    std::unique_lock<std::mutex> g(m_);
+    done_.wait(g, [&] { return remaining_ == 0; });
+    job_ = nullptr;
+    return true;
+  }
+
+ private:
+  HostPool() : pid_(getpid()) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    nworkers_ = std::min(15u, hw > 1 ? hw - 1 : 0u);
+    for (unsigned i = 0; i < nworkers_; ++i) std::thread([this, i]() { loop((int)i); }).detach();
+  }
+  void loop(int i) {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(int)>* job = nullptr;
+      {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] { return gen_ != seen; });
+        seen = gen_;
+        if (i >= nparticipants_) continue;
+        job = job_;
+      }
+      (*job)(i + 1);
+      std::lock_guard<std::mutex> g(m_);
+      if (--remaining_ == 0) done_.notify_one();
+    }
+  }
+  const pid_t pid_;
+  unsigned nworkers_ = 0;
+  std::mutex busy_, m_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* job_ = nullptr;
+  int nparticipants_ = 0, remaining_ = 0;
+  uint64_t gen_ = 0;
+};
+
+// fn(t) for t in [0, nt) on nt threads (inline when nt == 1): the
+// persistent workers when free, else threads of its own.
 template <typename F>
 void run_threads(int nt, F&& fn) {
   if (nt <= 1) {
     fn(0);
     return;
   }
+  const std::function<void(int)> job = [&fn](int t) { fn(t); };
+  if (HostPool::get().run(nt, job)) return;
   std::vector<std::thread> th;
   th.reserve(nt);
   for (int t = 0; t < nt; ++t) th.emplace_back([&fn, t]() { fn(t); });
@@ -691,6 +760,8 @@ struct Dev {
   T *partial_sj = nullptr, *Tsj = nullptr;
 };
 
+constexpr size_t kBigChunk = (size_t)8 << 20;  // pinned staging buffers for large uploads (two)
+constexpr size_t kStageMin = (size_t)1 << 20;  // smaller uploads go directly
 constexpr int kSumSlots = 8;  // per-window sums read back per LM iteration
 // Sum slots: model cost change, |step|^2, trial cost, |x|^2, gradient max
 // and |g|^2 (two slots), a slot for sums read at once.
@@ -796,6 +867,10 @@ struct mba_context {
   hipEvent_t stage_ev[kStage] = {};
   hipEvent_t look_ev[kLook] = {};  // PCG look-ahead: one per in-flight iteration
   hipEvent_t tev[3] = {};          // LM phase timing
+  // Large host -> device uploads: two pinned kBigChunk buffers in turn.
+  char* big = nullptr;
+  hipEvent_t big_ev[2] = {};
+  int big_next = 0;
   // Trust-region problem dumps (single problem): iterations and directory
   // copied from the options; `src` is the problem mba_solve was given.
   std::vector<int32_t> dump_iters;
@@ -823,12 +898,35 @@ template <typename T> Dev<T>& dev(mba_context* c);
 template <> Dev<float>& dev<float>(mba_context* c) { return c->df; }
 template <> Dev<double>& dev<double>(mba_context* c) { return c->dd; }
 
+// Host -> device copy on the context stream.  A large copy goes through the
+// two pinned staging buffers in turn: filled by a parallel memcpy, sent by
+// an async DMA, refilled once the event behind its last copy has passed.
+// The source may be released on return.
+mba_status copy_to_device(mba_context* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return MBA_OK;
+  if (bytes < kStageMin || !c->big) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return MBA_OK;
+  }
+  for (size_t off = 0; off < bytes; off += kBigChunk) {
+    const size_t n = std::min(kBigChunk, bytes - off);
+    const int i = c->big_next;
+    c->big_next ^= 1;
+    HIP_TRY(hipEventSynchronize(c->big_ev[i]));
+    char* h = c->big + (size_t)i * kBigChunk;
+    const char* from = (const char*)src + off;
+    parallel_for((int64_t)n, [&](int64_t b, int64_t e) { std::memcpy(h + b, from + b, (size_t)(e - b)); }, 1 << 20);
+    HIP_TRY(hipMemcpyAsync((char*)dst + off, h, n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipEventRecord(c->big_ev[i], c->stream));
+  }
+  return MBA_OK;
+}
+
 // Copy a host vector to a new device allocation.
 template <typename X, typename A>
 mba_status upload(mba_context* c, X** out, const std::vector<X, A>& v) {
   MBA_TRY(c->pool.alloc(out, v.size()));
-  if (!v.empty()) HIP_TRY(hipMemcpyAsync(*out, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice, c->stream));
-  return MBA_OK;
+  return copy_to_device(c, *out, v.data(), v.size() * sizeof(X));
 }
 
 template <typename T>
@@ -919,6 +1017,9 @@ void context_free(mba_context* c) {
     if (e) (void)hipEventDestroy(e);
   if (c->pin) (void)hipHostFree(c->pin);
   if (c->pin_done) (void)hipHostFree(c->pin_done);
+  for (hipEvent_t e : c->big_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->big) (void)hipHostFree(c->big);
   c->pool.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1450,6 +1551,8 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   if (c->device < 0 || c->device >= ndev) return fail(MBA_ERR_DEVICE, "no such HIP device");
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipHostMalloc((void**)&c->big, 2 * kBigChunk, hipHostMallocDefault));
+  for (hipEvent_t& e : c->big_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   // Partial rows of each F block: its chunks (packed), or its camera's
   // chunks / its cameras' group rows after the nchunks camera rows.
   std::vector<int32_t> f_lo(c->nf), f_hi(c->nf);
@@ -1490,7 +1593,8 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   }
   trace.mark("chunks + device setup");
   // FP64 state: cameras (slot blocks), then intrinsic groups (grouped).
-  std::vector<double> pts(3 * (size_t)c->ne), cams(9 * (size_t)c->nslot, 0.0);
+  Buf<double> pts(3 * (size_t)c->ne);
+  std::vector<double> cams(9 * (size_t)c->nslot, 0.0);
   parallel_for(c->ne, [&](int64_t b, int64_t end) {
     for (int64_t de = b; de < end; ++de) {
       const int32_t w = ge_win[dperm[de]];
@@ -1542,11 +1646,16 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       MBA_TRY(upload(c, &c->xmask, xm));
     }
   }
+  trace.mark("state gather");
   MBA_TRY(upload(c, &c->cams64, cams));
-  MBA_TRY(upload(c, &c->cams64_t, cams));
+  MBA_TRY(c->pool.alloc(&c->cams64_t, cams.size()));
+  HIP_TRY(hipMemcpyAsync(c->cams64_t, c->cams64, 8 * cams.size(), hipMemcpyDeviceToDevice, c->stream));
   MBA_TRY(upload(c, &c->pts64, pts));
-  MBA_TRY(upload(c, &c->pts64_t, pts));
+  MBA_TRY(c->pool.alloc(&c->pts64_t, pts.size()));
+  if (!pts.empty())
+    HIP_TRY(hipMemcpyAsync(c->pts64_t, c->pts64, 8 * pts.size(), hipMemcpyDeviceToDevice, c->stream));
   MBA_TRY(upload(c, &c->row_obs64, row_obs));
+  trace.mark("state uploads");
   MBA_TRY(c->pool.alloc(&c->ctab64, kCamTab * (size_t)std::max(1, c->nc)));
   MBA_TRY(c->pool.alloc(&c->pscale64, 3 * (size_t)c->ne));
   MBA_TRY(c->pool.alloc(&c->cscale64, 9 * (size_t)c->nf));
@@ -1573,8 +1682,11 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   c->h_active.assign(W, 1);
   MBA_TRY(upload(c, &c->wactive, c->h_active));
   if (W > 1) {
-    std::vector<int32_t> of_e(c->ne), of_f(c->nf);
-    for (int32_t de = 0; de < c->ne; ++de) of_e[de] = ge_win[dperm[de]];
+    Buf<int32_t> of_e(c->ne);
+    std::vector<int32_t> of_f(c->nf);
+    parallel_for(c->ne, [&](int64_t b, int64_t e) {
+      for (int64_t de = b; de < e; ++de) of_e[de] = ge_win[dperm[de]];
+    });
     for (int32_t w = 0; w < W; ++w)
       for (int32_t f = c->f_off[w]; f < c->f_off[w + 1]; ++f) of_f[f] = w;
     MBA_TRY(upload(c, &c->of_e, of_e));
@@ -1583,7 +1695,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   MBA_TRY(c->pool.alloc(&c->b64, 9 * (size_t)c->nf));
   MBA_TRY(c->pool.alloc(&c->x64, 9 * (size_t)c->nf));
   MBA_TRY(c->pool.alloc(&c->r64, 9 * (size_t)c->nf));
-  trace.mark("state + uploads");
+  trace.mark("scratch + window maps");
   if (c->mixed)
     MBA_TRY(alloc_precision<float>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_lo, f_hi));
   else
