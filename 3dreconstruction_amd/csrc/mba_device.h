@@ -1801,9 +1801,20 @@ __global__ __launch_bounds__(kBlock) void k_s_rows(
 // also carries the F'F coupling of their common rows, cross[36 c + 6 a + b]
 // (camera slot a, group slot b); blk_cross = 2 c (+1 if the group block is
 // fa), or -1.
+// Where the blocks of S go: one dense n x n column-major matrix (woff NULL),
+// or one per window (batched explicit solve): window w's of order
+// 9 (f_off[w+1] - f_off[w]) at S + woff[w], its F blocks numbered from
+// f_off[w]; of_f gives an F block's window.
+struct SDest {
+  int64_t n;
+  const int64_t* woff;
+  const int32_t* f_off;
+  const int32_t* of_f;
+};
+
 template <typename T>
 __device__ __forceinline__ void s_put(int blk, int fa, int fb, int i, int j, T acc, const T* __restrict__ U,
-                                      const double* __restrict__ diag_c, double inv_radius, int64_t n,
+                                      const double* __restrict__ diag_c, double inv_radius, const SDest& sd,
                                       T* __restrict__ S, const int32_t* __restrict__ blk_cross,
                                       const T* __restrict__ cross) {
   T base = T(0);
@@ -1817,9 +1828,17 @@ __device__ __forceinline__ void s_put(int blk, int fa, int fb, int i, int j, T a
       base = (xc & 1) ? x[6 * j + i] : x[6 * i + j];
     }
   }
-  const int64_t row = 9 * (int64_t)fa + i, col = 9 * (int64_t)fb + j;
-  S[row + col * n] = base - acc;
-  if (fa != fb) S[col + row * n] = base - acc;
+  int64_t off = 0, ld = sd.n, la = fa, lb = fb;
+  if (sd.woff) {
+    const int w = sd.of_f[fa], f0 = sd.f_off[w];
+    off = sd.woff[w];
+    ld = 9 * (int64_t)(sd.f_off[w + 1] - f0);
+    la = fa - f0;
+    lb = fb - f0;
+  }
+  const int64_t row = 9 * la + i, col = 9 * lb + j;
+  S[off + row + col * ld] = base - acc;
+  if (fa != fb) S[off + col + row * ld] = base - acc;
 }
 
 // One 64-lane workgroup per work item: a block (fa, fb), fa >= fb, that
@@ -1834,7 +1853,7 @@ __global__ __launch_bounds__(64) void k_s_blocks(
     const int32_t* __restrict__ item_blk, const int64_t* __restrict__ item_p0, const int64_t* __restrict__ item_p1,
     const int32_t* __restrict__ item_slot, const int32_t* __restrict__ blk_fa, const int32_t* __restrict__ blk_fb,
     const int32_t* __restrict__ pair_a, const int32_t* __restrict__ pair_b, const T* __restrict__ WY,
-    const T* __restrict__ U, const double* __restrict__ diag_c, double inv_radius, int64_t n, T* __restrict__ S,
+    const T* __restrict__ U, const double* __restrict__ diag_c, Win win, SDest sd, T* __restrict__ S,
     const int32_t* __restrict__ blk_cross, const T* __restrict__ cross, T* __restrict__ scratch) {
   __shared__ T Ys[64][kWYY];
   __shared__ T Ws[64][kWYY];
@@ -1843,6 +1862,8 @@ __global__ __launch_bounds__(64) void k_s_blocks(
   const int blk = item_blk[item];
   const int64_t p0 = item_p0[item], p1 = item_p1[item];
   const int fa = blk_fa[blk], fb = blk_fb[blk];
+  if (win_idle_f(win, fa)) return;
+  const double inv_radius = win_inv_r(win, win_of_f(win, fa));
   const int i0 = lane / 9, j0 = lane % 9;
   const int e1 = lane + 64, i1 = e1 / 9, j1 = e1 % 9;
   const bool two = e1 < 81;
@@ -1874,8 +1895,8 @@ __global__ __launch_bounds__(64) void k_s_blocks(
     if (two) scratch[81 * (size_t)slot + e1] = acc1;
     return;
   }
-  s_put(blk, fa, fb, i0, j0, acc0, U, diag_c, inv_radius, n, S, blk_cross, cross);
-  if (two) s_put(blk, fa, fb, i1, j1, acc1, U, diag_c, inv_radius, n, S, blk_cross, cross);
+  s_put(blk, fa, fb, i0, j0, acc0, U, diag_c, inv_radius, sd, S, blk_cross, cross);
+  if (two) s_put(blk, fa, fb, i1, j1, acc1, U, diag_c, inv_radius, sd, S, blk_cross, cross);
 }
 
 // Split blocks: sum slots [split_lo, split_hi) in order, then write.
@@ -1883,11 +1904,13 @@ template <typename T>
 __global__ __launch_bounds__(64) void k_s_split_final(
     const int32_t* __restrict__ split_blk, const int32_t* __restrict__ split_lo,
     const int32_t* __restrict__ split_hi, const int32_t* __restrict__ blk_fa, const int32_t* __restrict__ blk_fb,
-    const T* __restrict__ U, const double* __restrict__ diag_c, double inv_radius, int64_t n, T* __restrict__ S,
+    const T* __restrict__ U, const double* __restrict__ diag_c, Win win, SDest sd, T* __restrict__ S,
     const int32_t* __restrict__ blk_cross, const T* __restrict__ cross, const T* __restrict__ scratch) {
   const int k = blockIdx.x, lane = threadIdx.x;
   const int blk = split_blk[k];
   const int fa = blk_fa[blk], fb = blk_fb[blk];
+  if (win_idle_f(win, fa)) return;
+  const double inv_radius = win_inv_r(win, win_of_f(win, fa));
   const int e1 = lane + 64;
   const bool two = e1 < 81;
   T acc0 = T(0), acc1 = T(0);
@@ -1895,8 +1918,78 @@ __global__ __launch_bounds__(64) void k_s_split_final(
     acc0 += scratch[81 * (size_t)sl + lane];
     if (two) acc1 += scratch[81 * (size_t)sl + e1];
   }
-  s_put(blk, fa, fb, lane / 9, lane % 9, acc0, U, diag_c, inv_radius, n, S, blk_cross, cross);
-  if (two) s_put(blk, fa, fb, e1 / 9, e1 % 9, acc1, U, diag_c, inv_radius, n, S, blk_cross, cross);
+  s_put(blk, fa, fb, lane / 9, lane % 9, acc0, U, diag_c, inv_radius, sd, S, blk_cross, cross);
+  if (two) s_put(blk, fa, fb, e1 / 9, e1 % 9, acc1, U, diag_c, inv_radius, sd, S, blk_cross, cross);
+}
+
+// Batched explicit solve: one 256-thread workgroup per window factors the
+// window's S (order n = 9 nf_w <= kWinMaxN, column-major at S + woff[w],
+// lower triangle, in place) by right-looking column steps (column j scaled
+// and cached in LDS, then the trailing lower triangle updated), then solves
+// S x = rhs on the window's slice of the F-ordered vectors by forward and
+// backward substitution.  A non-positive pivot flags the window (its step
+// is then invalid) and is replaced by 1.
+constexpr int kWinMaxF = 64, kWinMaxN = 9 * kWinMaxF;
+template <typename T>
+__global__ __launch_bounds__(256) void k_win_chol_solve(const int64_t* __restrict__ woff,
+                                                        const int32_t* __restrict__ f_off, T* __restrict__ S,
+                                                        const T* __restrict__ rhs, T* __restrict__ x, Win win) {
+  __shared__ T colj[kWinMaxN];
+  __shared__ T v[kWinMaxN];
+  __shared__ T piv;
+  __shared__ int bad;
+  const int w = blockIdx.x, t = threadIdx.x;
+  if (win_idle(win, w)) return;
+  const int f0 = f_off[w];
+  const int n = 9 * (f_off[w + 1] - f0);
+  T* A = S + woff[w];
+  if (t == 0) bad = 0;
+  for (int j = 0; j < n; ++j) {
+    __syncthreads();
+    if (t == 0) {
+      T d = A[j + (int64_t)j * n];
+      if (!(d > T(0))) {
+        bad = 1;
+        d = T(1);
+      }
+      piv = sqrt(d);
+      A[j + (int64_t)j * n] = piv;
+    }
+    __syncthreads();
+    const T sj = piv;
+    for (int r = j + 1 + t; r < n; r += 256) {
+      const T l = A[r + (int64_t)j * n] / sj;
+      A[r + (int64_t)j * n] = l;
+      colj[r] = l;
+    }
+    __syncthreads();
+    for (int c = j + 1 + (t >> 5); c < n; c += 8) {
+      const T lc = colj[c];
+      T* Ac = A + (int64_t)c * n;
+      for (int r = c + (t & 31); r < n; r += 32) Ac[r] -= colj[r] * lc;
+    }
+  }
+  __syncthreads();
+  if (t == 0 && bad) win.flag[w] = 1;
+  for (int r = t; r < n; r += 256) v[r] = rhs[9 * (size_t)f0 + r];
+  // L y = rhs
+  for (int j = 0; j < n; ++j) {
+    __syncthreads();
+    if (t == 0) v[j] = v[j] / A[j + (int64_t)j * n];
+    __syncthreads();
+    const T vj = v[j];
+    for (int r = j + 1 + t; r < n; r += 256) v[r] -= A[r + (int64_t)j * n] * vj;
+  }
+  // L' x = y
+  for (int j = n - 1; j >= 0; --j) {
+    __syncthreads();
+    if (t == 0) v[j] = v[j] / A[j + (int64_t)j * n];
+    __syncthreads();
+    const T vj = v[j];
+    for (int r = t; r < j; r += 256) v[r] -= A[j + (int64_t)r * n] * vj;
+  }
+  __syncthreads();
+  for (int r = t; r < n; r += 256) x[9 * (size_t)f0 + r] = v[r];
 }
 
 // ------------------------------------------------- dense Cholesky (lower)

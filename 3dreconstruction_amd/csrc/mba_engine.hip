@@ -838,6 +838,11 @@ struct mba_context {
   // share an intrinsic group: rows s_m_row[s_m_start[m] .. s_m_start[m+1]).
   int32_t *s_m_start = nullptr, *s_m_row = nullptr;
   int64_t s_nmerged = 0;
+  // Batched explicit solve (W > 1): window w's dense S at offset d_woff[w]
+  // (s_wtotal entries in all), window F-block offsets on the device.
+  int64_t* d_woff = nullptr;
+  int32_t* d_f_off = nullptr;
+  int64_t s_wtotal = 0;
   // Host copies of what was last uploaded.
   std::vector<double> h_inv_r;
   std::vector<uint8_t> h_active;
@@ -1346,6 +1351,16 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   // columns) to cancellation in U - W Vinv W', which FP64 refinement against
   // the same S cannot restore: grouped explicit solves run in FP64.
   if (c->grouped && c->opt.linear_solver == MBA_EXPLICIT_SCHUR) c->mixed = false;
+  // Batched explicit solves (several windows) factor each window's S in
+  // FP64 by column steps; windows of more than kWinMaxF F blocks are
+  // solved one context each by the caller.
+  if (W > 1 && c->opt.linear_solver == MBA_EXPLICIT_SCHUR) {
+    c->mixed = false;
+    for (int32_t w = 0; w < W; ++w)
+      if (c->f_off[w + 1] - c->f_off[w] > kWinMaxF)
+        return fail(MBA_ERR_UNSUPPORTED, window_prefix(W, w) + "more than " + std::to_string(kWinMaxF) +
+                                             " camera blocks for a batched explicit solve");
+  }
   c->ng = c->grouped ? c->wst[0].ng : 0;
   c->nslot = c->nc + c->ng;
   // Global E blocks in window order, each window in its reference E order.
@@ -1700,7 +1715,20 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     MBA_TRY(alloc_precision<float>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_lo, f_hi));
   else
     MBA_TRY(alloc_precision<double>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_lo, f_hi));
-  if (W == 1 && c->opt.linear_solver == MBA_EXPLICIT_SCHUR) MBA_TRY(build_s_pairs(c, dev_pt_start, row_cam));
+  if (c->opt.linear_solver == MBA_EXPLICIT_SCHUR) {
+    MBA_TRY(build_s_pairs(c, dev_pt_start, row_cam));
+    if (W > 1) {
+      // One dense S per window: window w's of order 9 nf_w at woff[w].
+      std::vector<int64_t> woff((size_t)W + 1, 0);
+      for (int32_t w = 0; w < W; ++w) {
+        const int64_t nw = 9 * (int64_t)(c->f_off[w + 1] - c->f_off[w]);
+        woff[w + 1] = woff[w] + nw * nw;
+      }
+      c->s_wtotal = woff[W];
+      MBA_TRY(upload(c, &c->d_woff, woff));
+      MBA_TRY(upload(c, &c->d_f_off, c->f_off));
+    }
+  }
   trace.mark("precision layout + S pairs");
   for (int i = 0; i < 8; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
   for (hipEvent_t& e : c->stage_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -2070,9 +2098,48 @@ struct Engine {
 
   // Explicit Schur (single problem): assemble dense S, Cholesky, solve,
   // FP64 refinement.
+  // The S blocks points couple: W_r V^-1 W_r' products summed per block
+  // (all windows; idle ones skipped), written with U + D^2 into `sd`.
+  void assemble_s(const SDest& sd) {
+    const Win wv = win();
+    if (c->s_nitems > 0)
+      hipLaunchKernelGGL(k_s_blocks<T>, dim3((unsigned)c->s_nitems), dim3(64), 0, s, (const int32_t*)c->s_item_blk,
+                         (const int64_t*)c->s_item_p0, (const int64_t*)c->s_item_p1, (const int32_t*)c->s_item_slot,
+                         (const int32_t*)c->s_blk_fa, (const int32_t*)c->s_blk_fb, (const int32_t*)c->s_pair_a,
+                         (const int32_t*)c->s_pair_b, (const T*)d.WY, (const T*)d.U, (const double*)c->diag_c, wv,
+                         sd, d.S, (const int32_t*)c->s_blk_cross, (const T*)d.cross, d.s_scratch);
+    if (c->s_nsplit > 0)
+      hipLaunchKernelGGL(k_s_split_final<T>, dim3((unsigned)c->s_nsplit), dim3(64), 0, s,
+                         (const int32_t*)c->s_split_blk, (const int32_t*)c->s_split_lo,
+                         (const int32_t*)c->s_split_hi, (const int32_t*)c->s_blk_fa, (const int32_t*)c->s_blk_fb,
+                         (const T*)d.U, (const double*)c->diag_c, wv, sd, d.S, (const int32_t*)c->s_blk_cross,
+                         (const T*)d.cross, (const T*)d.s_scratch);
+  }
+
+  // Batched explicit solve (several windows, FP64): every window's dense S
+  // assembled in its own region, factored and solved by one workgroup per
+  // window; a failed factorisation flags its window.
+  mba_status batched_explicit_solve() {
+    if (c->nf == 0) return MBA_OK;
+    if (!d.S) {
+      MBA_TRY(c->pool.alloc(&d.S, (size_t)std::max<int64_t>(1, c->s_wtotal)));
+      MBA_TRY(c->pool.alloc(&d.WY, (size_t)kWY * c->nrows));
+      MBA_TRY(c->pool.alloc(&d.s_scratch, 81 * (size_t)std::max<int64_t>(1, c->s_nslots)));
+    }
+    HIP_TRY(hipMemsetAsync(d.S, 0, sizeof(T) * c->s_wtotal, s));
+    if (c->seg_blocks > 0)
+      hipLaunchKernelGGL(k_s_rows<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
+                         d.cam_to_f, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab, d.Vinv, huber, d.WY);
+    assemble_s(SDest{0, (const int64_t*)c->d_woff, (const int32_t*)c->d_f_off, (const int32_t*)c->of_f});
+    hipLaunchKernelGGL(k_win_chol_solve<T>, dim3(W), dim3(256), 0, s, (const int64_t*)c->d_woff,
+                       (const int32_t*)c->d_f_off, d.S, (const T*)d.rhs, d.x, win());
+    return MBA_OK;
+  }
+
   mba_status explicit_solve(bool* ok) {
     const int64_t n = 9 * (int64_t)c->nf;
     *ok = true;
+    if (W > 1) return batched_explicit_solve();
     if (n == 0) return MBA_OK;
     if (!d.S) {
       MBA_TRY(c->pool.alloc(&d.S, (size_t)(n * n)));
@@ -2087,7 +2154,6 @@ struct Engine {
       MBA_TRY(c->pool.alloc(&d.ty, (size_t)n));
       if (!c->gemv_part) MBA_TRY(c->pool.alloc(&c->gemv_part, (size_t)kGemvSplit * n));
     }
-    const double inv_r = c->h_inv_r[0];
     if (c->timing) (void)hipEventRecord(c->ev[4], s);
     // Blocks no point couples stay zero.
     HIP_TRY(hipMemsetAsync(d.S, 0, sizeof(T) * n * n, s));
@@ -2111,18 +2177,7 @@ struct Engine {
       hipLaunchKernelGGL(k_s_rows<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
                          d.cam_to_f, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab, d.Vinv, huber, d.WY);
     }
-    if (c->s_nitems > 0)
-      hipLaunchKernelGGL(k_s_blocks<T>, dim3((unsigned)c->s_nitems), dim3(64), 0, s, (const int32_t*)c->s_item_blk,
-                         (const int64_t*)c->s_item_p0, (const int64_t*)c->s_item_p1, (const int32_t*)c->s_item_slot,
-                         (const int32_t*)c->s_blk_fa, (const int32_t*)c->s_blk_fb, (const int32_t*)c->s_pair_a,
-                         (const int32_t*)c->s_pair_b, (const T*)d.WY, (const T*)d.U, (const double*)c->diag_c, inv_r,
-                         n, d.S, (const int32_t*)c->s_blk_cross, (const T*)d.cross, d.s_scratch);
-    if (c->s_nsplit > 0)
-      hipLaunchKernelGGL(k_s_split_final<T>, dim3((unsigned)c->s_nsplit), dim3(64), 0, s,
-                         (const int32_t*)c->s_split_blk, (const int32_t*)c->s_split_lo,
-                         (const int32_t*)c->s_split_hi, (const int32_t*)c->s_blk_fa, (const int32_t*)c->s_blk_fb,
-                         (const T*)d.U, (const double*)c->diag_c, inv_r, n, d.S, (const int32_t*)c->s_blk_cross,
-                         (const T*)d.cross, (const T*)d.s_scratch);
+    assemble_s(SDest{n, nullptr, nullptr, nullptr});
     if (c->timing) (void)hipEventRecord(c->ev[5], s);
     HIP_TRY(hipMemcpyAsync(d.L, d.S, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(c->info, 0, sizeof(int32_t), s));
@@ -2913,11 +2968,14 @@ mba_status mba_solve_batched(const mba_options* options, mba_problem* problems, 
   if (options) o = *options; else mba_default_options(&o);
   if (o.num_dump_iterations > 0 && num_problems > 1)
     return fail(MBA_ERR_UNSUPPORTED, "trust-region problem dumps are written for a single problem");
-  bool any_grouped = false;
-  for (int32_t w = 0; w < num_problems; ++w) any_grouped = any_grouped || problems[w].layout == MBA_LAYOUT_GROUPED;
-  if (o.linear_solver != MBA_IMPLICIT_SCHUR_PCG || any_grouped) {
-    // Dense reduced systems (and grouped-intrinsics problems) are solved one
-    // window at a time.
+  bool any_grouped = false, any_large = false;
+  for (int32_t w = 0; w < num_problems; ++w) {
+    any_grouped = any_grouped || problems[w].layout == MBA_LAYOUT_GROUPED;
+    any_large = any_large || problems[w].num_cameras > kWinMaxF;
+  }
+  if (any_grouped || (o.linear_solver == MBA_EXPLICIT_SCHUR && any_large)) {
+    // Grouped-intrinsics problems, and dense reduced systems larger than a
+    // batched explicit solve factors, are solved one window at a time.
     for (int32_t w = 0; w < num_problems; ++w) {
       const mba_status s = mba_solve(&o, &problems[w], &summaries[w], nullptr, 0);
       if (s != MBA_OK) return fail(s, "window " + std::to_string(w) + ": " + g_last_error);

@@ -241,6 +241,19 @@ def run(args):
                    "lm_iterations": {"min": min(its), "mean": float(np.mean(its)), "max": max(its)},
                    "all_usable": all(x.IsSolutionUsable() for x in sums)}
         log(f"  {batched['windows_per_s']:.0f} windows/s ({tot:.3f} s, preprocessing {pre:.3f} s)")
+        # The same windows with the explicit solver (the DENSE_SCHUR role): every
+        # window's dense S factored by its own workgroup in shared launches, FP64.
+        probs = [mba.BAProblem(w["cam_init"].copy(), w["pt_init"].copy(), w["obs"], w["cam_idx"], w["pt_idx"])
+                 for w in ws]
+        torch.cuda.synchronize()
+        sums = mba.solve_batched(probs, mba.default_options(linear_solver=mba.EXPLICIT_SCHUR))
+        tot, pre = sums[0].total_time_s, sums[0].preprocessor_time_s
+        its = [x.num_iterations - 1 for x in sums]
+        batched["explicit_fp64"] = {"windows_per_s": len(ws) / tot, "windows_per_s_solve_only": len(ws) / (tot - pre),
+                                    "preprocess_s": pre, "total_s": tot,
+                                    "lm_iterations": {"min": min(its), "mean": float(np.mean(its)), "max": max(its)},
+                                    "all_usable": all(x.IsSolutionUsable() for x in sums)}
+        log(f"  explicit FP64: {len(ws) / tot:.0f} windows/s ({tot:.3f} s, preprocessing {pre:.3f} s)")
         del ws, probs
 
     large = None

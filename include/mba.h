@@ -263,9 +263,13 @@ mba_status mba_solve(const mba_options* options, mba_problem* problem,
  * pair at :1168).  With MBA_IMPLICIT_SCHUR_PCG every window is solved in the
  * same device launches: each window keeps its own trust region, PCG state
  * and termination, exactly as its own mba_solve would decide them (sums are
- * accumulated in another order, so values agree to rounding).  Other linear
- * solvers solve the windows one after another.  summaries[i] is window i's;
- * its time fields are the whole batch's.  Usable windows are written back. */
+ * accumulated in another order, so values agree to rounding).  With
+ * MBA_EXPLICIT_SCHUR (the DENSE_SCHUR role of the 2-view problems,
+ * sfm_global_engine.cpp:1168) the windows also share launches: each
+ * window's dense S is assembled in its own region and factored (FP64) by
+ * one workgroup; windows of more than 64 cameras, and grouped-intrinsics
+ * problems, are solved one after another.  summaries[i] is window i's; its
+ * time fields are the whole batch's.  Usable windows are written back. */
 mba_status mba_solve_batched(const mba_options* options, mba_problem* problems,
                              int32_t num_problems, mba_summary* summaries);
 
