@@ -1756,9 +1756,9 @@ __global__ __launch_bounds__(kBlock) void k_s_rows(
     Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const int32_t* __restrict__ cam_to_f, const T* __restrict__ row_obs, const T* __restrict__ pkX,
     const T* __restrict__ pscale, const T* __restrict__ cscale, const T* __restrict__ ctab,
-    const T* __restrict__ Vinv, T huber_a, T* __restrict__ WY) {
+    const T* __restrict__ Vinv, T huber_a, T* __restrict__ WY, Win win) {
   const SegPos q = seg_locate(sg);
-  if (!q.valid) return;
+  if (!q.valid || win_idle_e(win, q.e)) return;
   const int e = q.e;
   T X[3], unused[3];
   load_pk(pkX, e, X, unused);
@@ -1922,16 +1922,16 @@ __global__ __launch_bounds__(64) void k_s_split_final(
   if (two) s_put(blk, fa, fb, e1 / 9, e1 % 9, acc1, U, diag_c, inv_radius, sd, S, blk_cross, cross);
 }
 
-// Batched explicit solve: one 256-thread workgroup per window factors the
+// Batched explicit solve: one workgroup per window factors the
 // window's S (order n = 9 nf_w <= kWinMaxN, column-major at S + woff[w],
 // lower triangle, in place) by right-looking column steps (column j scaled
 // and cached in LDS, then the trailing lower triangle updated), then solves
 // S x = rhs on the window's slice of the F-ordered vectors by forward and
 // backward substitution.  A non-positive pivot flags the window (its step
 // is then invalid) and is replaced by 1.
-constexpr int kWinMaxF = 64, kWinMaxN = 9 * kWinMaxF;
+constexpr int kWinMaxF = 64, kWinMaxN = 9 * kWinMaxF, kWinThreads = 512;
 template <typename T>
-__global__ __launch_bounds__(256) void k_win_chol_solve(const int64_t* __restrict__ woff,
+__global__ __launch_bounds__(kWinThreads) void k_win_chol_solve(const int64_t* __restrict__ woff,
                                                         const int32_t* __restrict__ f_off, T* __restrict__ S,
                                                         const T* __restrict__ rhs, T* __restrict__ x, Win win) {
   __shared__ T colj[kWinMaxN];
@@ -1957,13 +1957,13 @@ __global__ __launch_bounds__(256) void k_win_chol_solve(const int64_t* __restric
     }
     __syncthreads();
     const T sj = piv;
-    for (int r = j + 1 + t; r < n; r += 256) {
+    for (int r = j + 1 + t; r < n; r += kWinThreads) {
       const T l = A[r + (int64_t)j * n] / sj;
       A[r + (int64_t)j * n] = l;
       colj[r] = l;
     }
     __syncthreads();
-    for (int c = j + 1 + (t >> 5); c < n; c += 8) {
+    for (int c = j + 1 + (t >> 5); c < n; c += kWinThreads / 32) {
       const T lc = colj[c];
       T* Ac = A + (int64_t)c * n;
       for (int r = c + (t & 31); r < n; r += 32) Ac[r] -= colj[r] * lc;
@@ -1971,14 +1971,14 @@ __global__ __launch_bounds__(256) void k_win_chol_solve(const int64_t* __restric
   }
   __syncthreads();
   if (t == 0 && bad) win.flag[w] = 1;
-  for (int r = t; r < n; r += 256) v[r] = rhs[9 * (size_t)f0 + r];
+  for (int r = t; r < n; r += kWinThreads) v[r] = rhs[9 * (size_t)f0 + r];
   // L y = rhs
   for (int j = 0; j < n; ++j) {
     __syncthreads();
     if (t == 0) v[j] = v[j] / A[j + (int64_t)j * n];
     __syncthreads();
     const T vj = v[j];
-    for (int r = j + 1 + t; r < n; r += 256) v[r] -= A[r + (int64_t)j * n] * vj;
+    for (int r = j + 1 + t; r < n; r += kWinThreads) v[r] -= A[r + (int64_t)j * n] * vj;
   }
   // L' x = y
   for (int j = n - 1; j >= 0; --j) {
@@ -1986,10 +1986,10 @@ __global__ __launch_bounds__(256) void k_win_chol_solve(const int64_t* __restric
     if (t == 0) v[j] = v[j] / A[j + (int64_t)j * n];
     __syncthreads();
     const T vj = v[j];
-    for (int r = t; r < j; r += 256) v[r] -= A[j + (int64_t)r * n] * vj;
+    for (int r = t; r < j; r += kWinThreads) v[r] -= A[j + (int64_t)r * n] * vj;
   }
   __syncthreads();
-  for (int r = t; r < n; r += 256) x[9 * (size_t)f0 + r] = v[r];
+  for (int r = t; r < n; r += kWinThreads) x[9 * (size_t)f0 + r] = v[r];
 }
 
 // ------------------------------------------------- dense Cholesky (lower)

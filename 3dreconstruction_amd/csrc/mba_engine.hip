@@ -2129,9 +2129,9 @@ struct Engine {
     HIP_TRY(hipMemsetAsync(d.S, 0, sizeof(T) * c->s_wtotal, s));
     if (c->seg_blocks > 0)
       hipLaunchKernelGGL(k_s_rows<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
-                         d.cam_to_f, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab, d.Vinv, huber, d.WY);
+                         d.cam_to_f, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab, d.Vinv, huber, d.WY, win());
     assemble_s(SDest{0, (const int64_t*)c->d_woff, (const int32_t*)c->d_f_off, (const int32_t*)c->of_f});
-    hipLaunchKernelGGL(k_win_chol_solve<T>, dim3(W), dim3(256), 0, s, (const int64_t*)c->d_woff,
+    hipLaunchKernelGGL(k_win_chol_solve<T>, dim3(W), dim3(kWinThreads), 0, s, (const int64_t*)c->d_woff,
                        (const int32_t*)c->d_f_off, d.S, (const T*)d.rhs, d.x, win());
     return MBA_OK;
   }
@@ -2175,7 +2175,7 @@ struct Engine {
       }
     } else if (c->seg_blocks > 0) {
       hipLaunchKernelGGL(k_s_rows<T>, dim3(c->seg_blocks), dim3(kBlock), 0, s, c->seg, d.pt_start, d.row_cam,
-                         d.cam_to_f, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab, d.Vinv, huber, d.WY);
+                         d.cam_to_f, d.row_obs, d.pk, d.pscale, d.cscale, d.ctab, d.Vinv, huber, d.WY, win());
     }
     assemble_s(SDest{n, nullptr, nullptr, nullptr});
     if (c->timing) (void)hipEventRecord(c->ev[5], s);
