@@ -297,6 +297,77 @@ void user_points_first(const std::vector<int32_t>& active, int32_t first_point, 
 // Relaxed atomic add on a plain int (host threads).
 inline int32_t atomic_inc(int32_t* v) { return __atomic_fetch_add(v, 1, __ATOMIC_RELAXED); }
 
+// Residual order from each residual's bucket (0 .. ne, ne last): buckets
+// in order, each filled back to front (descending residual index), as
+// LexicographicallyOrderResidualBlocks (reorder_program.cc:295-309).  Each
+// thread owns a range of buckets: it counts them over all residuals, then
+// fills them scanning the residuals downwards.  bstart: bucket offsets
+// (ne + 2 entries).
+template <typename B, typename R>
+void order_residuals(const B& bucket, int64_t no, int32_t ne, std::vector<int64_t>* bstart_out, R* res_order) {
+  std::vector<int32_t> hist((size_t)ne + 1, 0);
+  std::vector<int64_t>& bstart = *bstart_out;
+  bstart.assign((size_t)ne + 2, 0);
+  res_order->resize(no);
+  const int nt = host_threads(no);
+  const int64_t span = ((int64_t)ne + 1 + nt - 1) / std::max(1, nt);
+  run_threads(nt, [&](int t) {
+    const int32_t b0 = (int32_t)std::min<int64_t>(ne + 1, t * span), b1 = (int32_t)std::min<int64_t>(ne + 1, (t + 1) * span);
+    // Branch-free: buckets outside the range count into a spare slot.
+    const uint32_t w = (uint32_t)(b1 - b0);
+    std::vector<int32_t> local((size_t)w + 1, 0);
+    for (int64_t i = 0; i < no; ++i) {
+      const uint32_t k = (uint32_t)(bucket[i] - b0);
+      local[k < w ? k : w]++;
+    }
+    std::copy(local.begin(), local.begin() + w, hist.begin() + b0);
+  });
+  for (int32_t e = 0; e <= ne; ++e) bstart[e + 1] = bstart[e] + hist[e];
+  run_threads(nt, [&](int t) {
+    const int32_t b0 = (int32_t)std::min<int64_t>(ne + 1, t * span), b1 = (int32_t)std::min<int64_t>(ne + 1, (t + 1) * span);
+    if (b0 >= b1) return;
+    std::vector<int64_t> pos(bstart.begin() + b0, bstart.begin() + b1);
+    for (int64_t i = no - 1; i >= 0; --i) {
+      const int32_t m = bucket[i];
+      if (m >= b0 && m < b1) (*res_order)[pos[m - b0]++] = i;
+    }
+  });
+}
+
+// Program order (problem_impl.cc:143,271): parameter blocks in order of
+// first appearance in the residuals' argument lists.  blocks(i, out)
+// lists residual i's block ids (at most kmax); a block's first position
+// k_max i + k is found by a parallel atomic minimum, and the positions
+// (distinct) are scanned in order.
+template <typename F>
+std::vector<int32_t> first_appearance(int64_t no, int32_t nb, int kmax, F&& blocks) {
+  std::vector<int64_t> first(nb, std::numeric_limits<int64_t>::max());
+  parallel_for(no, [&](int64_t b, int64_t e) {
+    int32_t rb[3];
+    for (int64_t i = b; i < e; ++i) {
+      const int n = blocks(i, rb);
+      for (int k = 0; k < n; ++k) {
+        const int64_t key = kmax * i + k;
+        int64_t cur = __atomic_load_n(&first[rb[k]], __ATOMIC_RELAXED);
+        while (key < cur &&
+               !__atomic_compare_exchange_n(&first[rb[k]], &cur, key, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+        }
+      }
+    }
+  });
+  Buf<int32_t> slot(kmax * (size_t)no);
+  parallel_for((int64_t)slot.size(), [&](int64_t b, int64_t e) { std::fill(slot.begin() + b, slot.begin() + e, -1); });
+  parallel_for(nb, [&](int64_t b, int64_t e) {
+    for (int64_t v = b; v < e; ++v)
+      if (first[v] != std::numeric_limits<int64_t>::max()) slot[first[v]] = (int32_t)v;
+  });
+  std::vector<int32_t> prog;
+  prog.reserve(nb);
+  for (int32_t v : slot)
+    if (v >= 0) prog.push_back(v);
+  return prog;
+}
+
 mba_status build_structure(const mba_problem* p, Structure* st) {
   MBA_TRY(validate(p));
   if (p->layout == MBA_LAYOUT_GROUPED) return build_structure_grouped(p, st);
@@ -310,15 +381,12 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
     return b < nc && p->camera_constant != nullptr && p->camera_constant[b] != 0;
   };
   PrepTrace tr;
-  // First appearance (program order, one pass).
-  std::vector<int32_t> prog;
-  prog.reserve(nb);
-  std::vector<uint8_t> seen(nb, 0);
-  for (int64_t i = 0; i < no; ++i) {
-    const int32_t a = p->camera_index[i], b = nc + p->point_index[i];
-    if (!seen[a]) { seen[a] = 1; prog.push_back(a); }
-    if (!seen[b]) { seen[b] = 1; prog.push_back(b); }
-  }
+  // First appearance (program order).
+  const std::vector<int32_t> prog = first_appearance(no, nb, 2, [&](int64_t i, int32_t* rb) {
+    rb[0] = p->camera_index[i];
+    rb[1] = nc + p->point_index[i];
+    return 2;
+  });
   // CSR of observations by camera: per-thread histograms over observation
   // ranges (the cameras are few), then each thread fills its own slots.
   std::vector<int64_t> cstart((size_t)nc + 1, 0);
@@ -439,7 +507,6 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
   // buckets are counted and filled in parallel, then each bucket is sorted.
   const int32_t ne = st->num_e;
   Buf<int32_t> bucket(no);
-  std::vector<int32_t> hist((size_t)ne + 1, 0);
   parallel_for(no, [&](int64_t b, int64_t e) {
     for (int64_t i = b; i < e; ++i) {
       const int32_t a = p->camera_index[i], q = nc + p->point_index[i];
@@ -449,35 +516,8 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
       bucket[i] = m;
     }
   });
-  // Each thread owns a range of buckets: it counts them over all
-  // observations, then fills them scanning the observations downwards.
-  std::vector<int64_t> bstart((size_t)ne + 2, 0);
-  st->res_order.resize(no);
-  {
-    const int nt = host_threads(no);
-    const int64_t span = ((int64_t)ne + 1 + nt - 1) / std::max(1, nt);
-    run_threads(nt, [&](int t) {
-      const int32_t b0 = (int32_t)std::min<int64_t>(ne + 1, t * span), b1 = (int32_t)std::min<int64_t>(ne + 1, (t + 1) * span);
-      // Branch-free: buckets outside the range count into a spare slot.
-      const uint32_t w = (uint32_t)(b1 - b0);
-      std::vector<int32_t> local((size_t)w + 1, 0);
-      for (int64_t i = 0; i < no; ++i) {
-        const uint32_t k = (uint32_t)(bucket[i] - b0);
-        local[k < w ? k : w]++;
-      }
-      std::copy(local.begin(), local.begin() + w, hist.begin() + b0);
-    });
-    for (int32_t e = 0; e <= ne; ++e) bstart[e + 1] = bstart[e] + hist[e];
-    run_threads(nt, [&](int t) {
-      const int32_t b0 = (int32_t)std::min<int64_t>(ne + 1, t * span), b1 = (int32_t)std::min<int64_t>(ne + 1, (t + 1) * span);
-      if (b0 >= b1) return;
-      std::vector<int64_t> pos(bstart.begin() + b0, bstart.begin() + b1);
-      for (int64_t i = no - 1; i >= 0; --i) {
-        const int32_t m = bucket[i];
-        if (m >= b0 && m < b1) st->res_order[pos[m - b0]++] = i;
-      }
-    });
-  }
+  std::vector<int64_t> bstart;
+  order_residuals(bucket, no, ne, &bstart, &st->res_order);
   tr.mark("  residual order");
   // Standard BA structure: E = every active point, F = every active camera.
   int32_t n_act_pts = 0;
@@ -500,7 +540,8 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
   st->pt_start.assign((size_t)st->num_e + 1, 0);
   for (int32_t e = 0; e <= st->num_e; ++e) st->pt_start[e] = (int32_t)bstart[e];
   st->max_track = 0;
-  for (int32_t e = 0; e < st->num_e; ++e) st->max_track = std::max(st->max_track, hist[e]);
+  for (int32_t e = 0; e < st->num_e; ++e)
+    st->max_track = std::max(st->max_track, (int32_t)(bstart[e + 1] - bstart[e]));
   tr.mark("  E/F maps");
   return MBA_OK;
 }
@@ -547,6 +588,7 @@ mba_status build_structure_grouped(const mba_problem* p, Structure* st) {
   st->nobs = no;
   if ((int64_t)ng + nc + np > 0x7fffffffLL) return fail(MBA_ERR_UNSUPPORTED, "more than 2^31-1 parameter blocks");
   const int32_t nb = ng + nc + np;
+  PrepTrace tr;
   st->cam_grp.assign(nc, -1);
   for (int64_t i = 0; i < no; ++i) {
     const int32_t c = p->camera_index[i], g = p->intrinsic_index[i];
@@ -555,88 +597,164 @@ mba_status build_structure_grouped(const mba_problem* p, Structure* st) {
       return fail(MBA_ERR_UNSUPPORTED, "camera " + std::to_string(c) + " is observed with two intrinsic groups");
   }
   auto is_const = [&](int32_t b) { return block_constant(p, b); };
-  std::vector<int32_t> prog;
-  prog.reserve(nb);
-  std::vector<uint8_t> seen(nb, 0);
+  const std::vector<int32_t> prog =
+      first_appearance(no, nb, 3, [&](int64_t i, int32_t* out) { return residual_blocks(p, i, out); });
   int32_t rb[3];
-  for (int64_t i = 0; i < no; ++i) {
-    const int n = residual_blocks(p, i, rb);
-    for (int k = 0; k < n; ++k)
-      if (!seen[rb[k]]) { seen[rb[k]] = 1; prog.push_back(rb[k]); }
-  }
-  // Distinct edges between non-constant blocks, then adjacency lists.
-  std::vector<int64_t> keys;
-  keys.reserve(3 * (size_t)no);
-  for (int64_t i = 0; i < no; ++i) {
-    const int n = residual_blocks(p, i, rb);
-    for (int a = 0; a < n; ++a) {
-      if (is_const(rb[a])) continue;
-      for (int b = a + 1; b < n; ++b) {
-        if (is_const(rb[b]) || rb[a] == rb[b]) continue;
-        const int32_t lo = std::min(rb[a], rb[b]), hi = std::max(rb[a], rb[b]);
-        keys.push_back((int64_t)lo * nb + hi);
-      }
-    }
-  }
-  std::sort(keys.begin(), keys.end());
-  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-  std::vector<int64_t> adj_start((size_t)nb + 1, 0);
-  for (int64_t k : keys) {
-    adj_start[k / nb + 1]++;
-    adj_start[k % nb + 1]++;
-  }
-  for (int32_t v = 0; v < nb; ++v) adj_start[v + 1] += adj_start[v];
-  std::vector<int32_t> adj((size_t)adj_start[nb]);
-  {
-    std::vector<int64_t> fill(adj_start.begin(), adj_start.end() - 1);
-    for (int64_t k : keys) {
-      const int32_t a = (int32_t)(k / nb), b = (int32_t)(k % nb);
-      adj[fill[a]++] = b;
-      adj[fill[b]++] = a;
-    }
-  }
   std::vector<int32_t> active;
   active.reserve(prog.size());
   for (int32_t b : prog)
     if (!is_const(b)) active.push_back(b);
-  int32_t maxdeg = 0;
-  for (int32_t b : active) maxdeg = std::max<int32_t>(maxdeg, (int32_t)(adj_start[b + 1] - adj_start[b]));
-  std::vector<int64_t> cnt((size_t)maxdeg + 2, 0);
-  for (int32_t b : active) cnt[adj_start[b + 1] - adj_start[b] + 1]++;
-  for (int32_t d = 0; d <= maxdeg; ++d) cnt[d + 1] += cnt[d];
-  std::vector<int32_t> queue(active.size());
-  for (int32_t b : active) queue[cnt[adj_start[b + 1] - adj_start[b]]++] = b;
+  tr.mark("  first appearance");
   st->order.clear();
   st->order.reserve(active.size());
   if (p->ordering == MBA_ORDER_POINTS_FIRST) {
     user_points_first(active, ng + nc, st);
   } else {
-    std::vector<uint8_t> color(nb, 0);
-    for (int32_t b : queue) {
-      if (color[b] != 0) continue;
-      st->order.push_back(b);
-      color[b] = 2;
-      for (int64_t k = adj_start[b]; k < adj_start[b + 1]; ++k) color[adj[k]] = 1;
+    // Vertex degrees of the Hessian graph without listing its edges: per
+    // point (in parallel, over its observations) the distinct non-constant
+    // cameras and groups it meets (camera-point and group-point edges),
+    // plus one camera-group edge per non-constant camera in a non-constant
+    // group.
+    std::vector<int32_t> deg(nb, 0);
+    std::vector<int64_t> pstart((size_t)np + 1, 0);
+    Buf<int32_t> pobs(no);
+    {
+      // Per-thread histograms over observation ranges, then each thread
+      // fills its own slots (observations of a point in index order).
+      const int nt = host_threads(no);
+      const int64_t step = (no + nt - 1) / std::max(1, nt);
+      std::vector<std::vector<int64_t>> h(nt, std::vector<int64_t>((size_t)np, 0));
+      run_threads(nt, [&](int t) {
+        const int64_t e = std::min<int64_t>(no, (t + 1) * step);
+        for (int64_t i = t * step; i < e; ++i) h[t][p->point_index[i]]++;
+      });
+      int64_t run = 0;
+      for (int32_t q = 0; q < np; ++q) {
+        pstart[q] = run;
+        for (int t = 0; t < nt; ++t) {
+          const int64_t k = h[t][q];
+          h[t][q] = run;
+          run += k;
+        }
+      }
+      pstart[np] = run;
+      run_threads(nt, [&](int t) {
+        const int64_t e = std::min<int64_t>(no, (t + 1) * step);
+        for (int64_t i = t * step; i < e; ++i) pobs[h[t][p->point_index[i]]++] = (int32_t)i;
+      });
     }
-    st->num_e = (int32_t)st->order.size();
-    for (int32_t b : queue)
-      if (color[b] != 2) st->order.push_back(b);
+    // Camera and group degrees are counted per thread, then added.
+    parallel_for(np, [&](int64_t b, int64_t e) {
+      std::vector<int32_t> cams, grps, dloc((size_t)ng + nc, 0);
+      for (int64_t q = b; q < e; ++q) {
+        cams.clear();
+        grps.clear();
+        for (int64_t k = pstart[q]; k < pstart[q + 1]; ++k) {
+          const int64_t i = pobs[k];
+          const int32_t cv = ng + p->camera_index[i], gv = p->intrinsic_index[i];
+          if (!is_const(cv) && std::find(cams.begin(), cams.end(), cv) == cams.end()) cams.push_back(cv);
+          if (!is_const(gv) && std::find(grps.begin(), grps.end(), gv) == grps.end()) grps.push_back(gv);
+        }
+        deg[ng + nc + q] = (int32_t)(cams.size() + grps.size());
+        for (int32_t v : cams) dloc[v]++;
+        for (int32_t v : grps) dloc[v]++;
+      }
+      for (int32_t v = 0; v < ng + nc; ++v)
+        if (dloc[v]) __atomic_fetch_add(&deg[v], dloc[v], __ATOMIC_RELAXED);
+    }, 1 << 12);
+    for (int32_t c = 0; c < nc; ++c) {
+      const int32_t g = st->cam_grp[c];
+      if (g >= 0 && !is_const(ng + c) && !is_const(g)) {
+        deg[ng + c]++;
+        deg[g]++;
+      }
+    }
+    tr.mark("  degrees");
+    int32_t maxdeg = 0, max_pt = 0, min_other = std::numeric_limits<int32_t>::max();
+    for (int32_t b : active) {
+      maxdeg = std::max(maxdeg, deg[b]);
+      if (b >= ng + nc) max_pt = std::max(max_pt, deg[b]);
+      else min_other = std::min(min_other, deg[b]);
+    }
+    std::vector<int64_t> cnt((size_t)maxdeg + 2, 0);
+    for (int32_t b : active) cnt[deg[b] + 1]++;
+    for (int32_t d = 0; d <= maxdeg; ++d) cnt[d + 1] += cnt[d];
+    std::vector<int32_t> queue(active.size());
+    for (int32_t b : active) queue[cnt[deg[b]]++] = b;
+    if (max_pt < min_other) {
+      // Every point precedes every camera and group in the queue, and only
+      // cameras and groups neighbour a point: the greedy set is the points
+      // in queue order.
+      for (int32_t b : queue)
+        if (b >= ng + nc) st->order.push_back(b);
+      st->num_e = (int32_t)st->order.size();
+      for (int32_t b : queue)
+        if (b < ng + nc) st->order.push_back(b);
+    } else {
+      // Greedy independent set over the explicit adjacency.
+      std::vector<int64_t> keys;
+      keys.reserve(3 * (size_t)no);
+      for (int64_t i = 0; i < no; ++i) {
+        const int n = residual_blocks(p, i, rb);
+        for (int a = 0; a < n; ++a) {
+          if (is_const(rb[a])) continue;
+          for (int bb = a + 1; bb < n; ++bb) {
+            if (is_const(rb[bb]) || rb[a] == rb[bb]) continue;
+            const int32_t lo = std::min(rb[a], rb[bb]), hi = std::max(rb[a], rb[bb]);
+            keys.push_back((int64_t)lo * nb + hi);
+          }
+        }
+      }
+      std::sort(keys.begin(), keys.end());
+      keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+      std::vector<int64_t> adj_start((size_t)nb + 1, 0);
+      for (int64_t k : keys) {
+        adj_start[k / nb + 1]++;
+        adj_start[k % nb + 1]++;
+      }
+      for (int32_t v = 0; v < nb; ++v) adj_start[v + 1] += adj_start[v];
+      std::vector<int32_t> adj((size_t)adj_start[nb]);
+      {
+        std::vector<int64_t> fill(adj_start.begin(), adj_start.end() - 1);
+        for (int64_t k : keys) {
+          const int32_t a = (int32_t)(k / nb), bb = (int32_t)(k % nb);
+          adj[fill[a]++] = bb;
+          adj[fill[bb]++] = a;
+        }
+      }
+      std::vector<uint8_t> color(nb, 0);
+      for (int32_t b : queue) {
+        if (color[b] != 0) continue;
+        st->order.push_back(b);
+        color[b] = 2;
+        for (int64_t k = adj_start[b]; k < adj_start[b + 1]; ++k) color[adj[k]] = 1;
+      }
+      st->num_e = (int32_t)st->order.size();
+      for (int32_t b : queue)
+        if (color[b] != 2) st->order.push_back(b);
+    }
   }
+  tr.mark("  independent set");
   std::vector<int32_t> col(nb, -1);
   for (size_t i = 0; i < st->order.size(); ++i) col[st->order[i]] = (int32_t)i;
-  std::vector<int64_t> hist((size_t)st->num_e + 1, 0);
-  std::vector<int32_t> bucket(no);
-  for (int64_t i = 0; i < no; ++i) {
-    const int n = residual_blocks(p, i, rb);
-    int32_t m = st->num_e;
-    for (int k = 0; k < n; ++k)
-      if (!is_const(rb[k])) m = std::min(m, col[rb[k]]);
-    bucket[i] = m;
-    hist[m]++;
-  }
-  for (int32_t e = 1; e <= st->num_e; ++e) hist[e] += hist[e - 1];
-  st->res_order.assign(no, 0);
-  for (int64_t i = 0; i < no; ++i) st->res_order[--hist[bucket[i]]] = i;
+  // Residual order: bucket = the smallest column block among the residual's
+  // non-constant blocks, buckets in column order, each filled back to
+  // front (as the packed layout, in parallel).
+  const int32_t ne = st->num_e;
+  Buf<int32_t> bucket(no);
+  parallel_for(no, [&](int64_t b, int64_t e) {
+    int32_t r3[3];
+    for (int64_t i = b; i < e; ++i) {
+      const int n = residual_blocks(p, i, r3);
+      int32_t m = ne;
+      for (int k = 0; k < n; ++k)
+        if (!is_const(r3[k])) m = std::min(m, col[r3[k]]);
+      bucket[i] = m;
+    }
+  });
+  std::vector<int64_t> bstart;
+  order_residuals(bucket, no, ne, &bstart, &st->res_order);
+  tr.mark("  residual order");
   int32_t n_act_pts = 0;
   for (int32_t b : active) n_act_pts += b >= ng + nc;
   bool ok = st->num_e == n_act_pts;
@@ -653,10 +771,10 @@ mba_status build_structure_grouped(const mba_problem* p, Structure* st) {
     st->f_to_cam[i] = v < ng ? nc + v : v - ng;
     st->cam_to_f[st->f_to_cam[i]] = i;
   }
+  // Every row's bucket is its point's E block (points are never constant).
   st->pt_start.assign((size_t)st->num_e + 1, 0);
-  for (int64_t r = 0; r < no; ++r) st->pt_start[bucket[st->res_order[r]] + 1]++;
-  for (int32_t e = 0; e < st->num_e; ++e) st->pt_start[e + 1] += st->pt_start[e];
   st->max_track = 0;
+  for (int32_t e = 0; e <= st->num_e; ++e) st->pt_start[e] = (int32_t)bstart[e];
   for (int32_t e = 0; e < st->num_e; ++e)
     st->max_track = std::max(st->max_track, st->pt_start[e + 1] - st->pt_start[e]);
   return MBA_OK;
