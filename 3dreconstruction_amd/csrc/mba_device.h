@@ -170,23 +170,61 @@ template <typename T> __device__ __forceinline__ T tiny();
 template <> __device__ __forceinline__ float tiny<float>() { return FLT_MIN; }
 template <> __device__ __forceinline__ double tiny<double>() { return DBL_MIN; }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// Value of lane `l` (uniform) broadcast to the wave (v_readlane).
+__device__ __forceinline__ float lane_bcast(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Block-wide sum of N accumulators (blockDim.x == kBlock); result valid in
-// out[] for threads 0..N-1 (out is LDS of size N * kBlock/64).
+// v of another lane by a DPP pattern (row-local, every source lane valid).
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes, result in every lane (all lanes active): DPP adds
+// within each row of 16 lanes (quad_perm xor 1, xor 2, half-row mirror,
+// row mirror), then the four row sums read from lanes 0, 16, 32, 48 and
+// added in a fixed order.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x140>(v);
+  return (lane_bcast(v, 0) + lane_bcast(v, 16)) + (lane_bcast(v, 32) + lane_bcast(v, 48));
+}
+
+// Block-wide sum of N accumulators (blockDim.x == kBlock): wave sums land
+// in lds[wave * N + k] (lane k % 64 of each wave keeps sum k, then writes
+// it; no branch per accumulator).
 template <typename T, int N>
 __device__ __forceinline__ void block_sum(T (&acc)[N], T* lds) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int M = (N + 63) / 64;
+  T mine[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) mine[m] = T(0);
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const T v = wave_sum(acc[k]);
-    if (lane == 0) lds[wave * N + k] = v;
+    mine[k >> 6] = lane == (k & 63) ? v : mine[k >> 6];
   }
+#pragma unroll
+  for (int m = 0; m < M; ++m)
+    if (64 * m + lane < N) lds[wave * N + 64 * m + lane] = mine[m];
   __syncthreads();
 }
 
@@ -2000,16 +2038,6 @@ __global__ __launch_bounds__(kWinThreads) void k_win_chol_solve(const int64_t* _
 // The triangular solves with the factor run one launch per 64-row block.
 constexpr int kNB = 64;
 
-// Value of lane `l` (uniform) broadcast to the wave (v_readlane).
-__device__ __forceinline__ float lane_bcast(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-__device__ __forceinline__ double lane_bcast(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
 
 // Four waves: factor the kb x kb diagonal block at (k, k), write it back,
 // and write L11^-1 (64 x 64 column-major, zero-padded) and its transpose
