@@ -84,14 +84,6 @@ __device__ __forceinline__ SegPos seg_locate(const Seg& sg) {
   return p;
 }
 
-// Sum over the `lanes` lanes of a point group (aligned, power of two).
-template <typename T>
-__device__ __forceinline__ T group_sum(T v, int lanes) {
-  for (int o = 32; o > 0; o >>= 1)
-    if (o < lanes) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 // acc += a b + c d as two chained FMAs (the sum form would round a b + c d
 // first and add it separately).
 template <typename T>
@@ -205,6 +197,21 @@ __device__ __forceinline__ T wave_sum(T v) {
   v += dpp_mov<0x141>(v);
   v += dpp_mov<0x140>(v);
   return (lane_bcast(v, 0) + lane_bcast(v, 16)) + (lane_bcast(v, 32) + lane_bcast(v, 48));
+}
+
+// Sum over the `lanes` lanes of a point group (aligned, power of two,
+// uniform in the wave; all lanes active), result in every lane of the
+// group: DPP adds over 2, 4, 8 and 16 lanes (quad_perm xor 1, xor 2,
+// half-row mirror, row mirror), shuffles beyond.
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int lanes) {
+  if (lanes > 1) v += dpp_mov<0xB1>(v);
+  if (lanes > 2) v += dpp_mov<0x4E>(v);
+  if (lanes > 4) v += dpp_mov<0x141>(v);
+  if (lanes > 8) v += dpp_mov<0x140>(v);
+  if (lanes > 16) v += __shfl_xor(v, 16, 64);
+  if (lanes > 32) v += __shfl_xor(v, 32, 64);
+  return v;
 }
 
 // Block-wide sum of N accumulators (blockDim.x == kBlock): wave sums land
@@ -1281,7 +1288,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_finalize(int nf, const int32_t* 
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_model_cost(
     Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
-    const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
+    const T* __restrict__ row_obs, const T* __restrict__ pkX, int xstride, const T* __restrict__ pscale,
     const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ step_p,
     T huber_a, double* __restrict__ partial, Win win) {
   __shared__ double lds[kBlock / 64];
@@ -1290,8 +1297,8 @@ __global__ __launch_bounds__(kBlock) void k_model_cost(
   double acc[1] = {0.0};
   if (qp.valid) {
     const int e = qp.e;
-    T X[3], unused[3];
-    load_pk(pkX, e, X, unused);
+    // Point position: a packed record (xstride kPk) or FP64 positions (3).
+    const T X[3] = {pkX[(size_t)xstride * e], pkX[(size_t)xstride * e + 1], pkX[(size_t)xstride * e + 2]};
     const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
     const T dp[3] = {step_p[3 * (size_t)e], step_p[3 * (size_t)e + 1], step_p[3 * (size_t)e + 2]};
     for (int r = pt_start[e] + qp.sub; r < pt_start[e + 1]; r += qp.lanes) {

@@ -926,6 +926,7 @@ struct mba_context {
   double *ctab64 = nullptr, *row_obs64 = nullptr, *pscale64 = nullptr, *cscale64 = nullptr;
   double *diag_p = nullptr, *diag_c = nullptr, *red = nullptr, *red2 = nullptr, *stepd = nullptr;
   double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr, *gemv_part = nullptr;
+  double* dtab64 = nullptr;  // FP64 direction table of the step (mixed: the model cost change)
   Scalars* sc = nullptr;        // device
   PcgState* pcg = nullptr;      // device, one per window
   int32_t* flag = nullptr;      // device, one per window
@@ -1790,6 +1791,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   MBA_TRY(upload(c, &c->row_obs64, row_obs));
   trace.mark("state uploads");
   MBA_TRY(c->pool.alloc(&c->ctab64, kCamTab * (size_t)std::max(1, c->nc)));
+  MBA_TRY(c->pool.alloc(&c->dtab64, (size_t)std::max(kDirTab, kDirTabG) * std::max(1, c->nc)));
   MBA_TRY(c->pool.alloc(&c->pscale64, 3 * (size_t)c->ne));
   MBA_TRY(c->pool.alloc(&c->cscale64, 9 * (size_t)c->nf));
   MBA_TRY(c->pool.alloc(&c->diag_p, 3 * (size_t)c->ne));
@@ -2358,15 +2360,43 @@ struct Engine {
   }
 
   // Model cost change -J step . (r + J step / 2) per window, negated, into
-  // sum slot kSlotModel.
+  // sum slot kSlotModel.  Always formed in FP64 (as the reference forms it):
+  // with FP32 kernels the step is converted and the Jacobian re-evaluated
+  // from the FP64 state, since FP32 residuals (pred - obs at ~500 px) carry
+  // ~3e-5 px of cancellation error that would make the gain ratio of the
+  // small steps near convergence noisy.
   mba_status model_cost_async() {
-    cam_dir(d.step_c, (T)1, nullptr, win());
     const int nb = c->seg_blocks;
-    MBA_TRY(begin_sum());
-    if (nb > 0)
-      hipLaunchKernelGGL((c->grouped ? k_model_cost<T, 1> : k_model_cost<T, 0>), dim3(nb), dim3(kBlock), 0, s, c->seg,
-                         d.pt_start, d.row_cam, d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.step_p, huber, c->red,
-                         win(c->wacc));
+    if constexpr (std::is_same<T, float>::value) {
+      const int64_t np3 = 3 * (int64_t)c->ne, nf9 = 9 * (int64_t)c->nf;
+      hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(np3)), dim3(kBlock), 0, s, np3, d.step_p, c->stepd);
+      hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(nf9)), dim3(kBlock), 0, s, nf9, d.step_c,
+                         c->stepd + np3);
+      table_into<double>(c->cams64, c->ctab64);
+      const Win wv = win();
+      if (c->grouped)
+        hipLaunchKernelGGL(k_cam_dir_g<double>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f,
+                           (const int32_t*)c->d_cam_gf, (const double*)c->cscale64, (const double*)c->ctab64,
+                           (const double*)(c->stepd + np3), 1.0, c->dtab64, (const int32_t*)nullptr);
+      else
+        hipLaunchKernelGGL(k_cam_dir<double>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f,
+                           (const double*)c->cscale64, (const double*)c->ctab64, (const double*)(c->stepd + np3),
+                           1.0, c->dtab64, (const int32_t*)nullptr, wv);
+      MBA_TRY(begin_sum());
+      if (nb > 0)
+        hipLaunchKernelGGL((c->grouped ? k_model_cost<double, 1> : k_model_cost<double, 0>), dim3(nb), dim3(kBlock),
+                           0, s, c->seg, d.pt_start, d.row_cam, (const double*)c->row_obs64,
+                           (const double*)c->pts64, 3, (const double*)c->pscale64, (const double*)c->ctab64,
+                           (const double*)c->dtab64, (const double*)c->stepd, c->opt.huber_a, c->red,
+                           win(c->wacc));
+    } else {
+      cam_dir(d.step_c, (T)1, nullptr, win());
+      MBA_TRY(begin_sum());
+      if (nb > 0)
+        hipLaunchKernelGGL((c->grouped ? k_model_cost<T, 1> : k_model_cost<T, 0>), dim3(nb), dim3(kBlock), 0, s,
+                           c->seg, d.pt_start, d.row_cam, d.row_obs, d.pk, kPk, d.pscale, d.ctab, d.dtab, d.step_p,
+                           huber, c->red, win(c->wacc));
+    }
     return sum_async(nb, kSlotModel);
   }
 
