@@ -2217,8 +2217,10 @@ __global__ __launch_bounds__(256) void k_panel_mfma(T* __restrict__ A, int64_t n
   load_cols(As, A, n, k, kb, r0);
   load_cols(Bs, Linv, kNB, 0, kNB, 0);
   __syncthreads();
-  tile_nt(As, Bs, kb, [&](int i, int j, T v) {
-    if (r0 + i < n && j < kb) A[(r0 + i) + (int64_t)(k + j) * n] = v;
+  // The transposed product (Linv A21') puts the lane index on the row of the
+  // column-major matrix: a store instruction writes whole column segments.
+  tile_nt(Bs, As, kb, [&](int i, int j, T v) {
+    if (r0 + j < n && i < kb) A[(r0 + j) + (int64_t)(k + i) * n] = v;
   });
 }
 
@@ -2241,8 +2243,9 @@ __global__ __launch_bounds__(256) void k_syrk_mfma(T* __restrict__ A, int64_t n,
   load_cols(As, A, n, k, kb, r0);
   load_cols(Bs, A, n, k, kb, c0);
   __syncthreads();
-  tile_nt(As, Bs, kb, [&](int i, int j, T v) {
-    const int64_t row = r0 + i, col = c0 + j;
+  // Transposed product (lane index on the row): coalesced read-modify-write.
+  tile_nt(Bs, As, kb, [&](int i, int j, T v) {
+    const int64_t row = r0 + j, col = c0 + i;
     if (row < n && col < n && row >= col) A[row + col * n] -= v;
   });
 }
