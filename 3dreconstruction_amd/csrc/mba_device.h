@@ -2040,8 +2040,9 @@ __global__ __launch_bounds__(kWinThreads) void k_win_chol_solve(const int64_t* _
 // ------------------------------------------------- dense Cholesky (lower)
 // Right-looking blocked LLT of a column-major SPD matrix, NB = 64.  Step k:
 //   k_potrf_diag   factors the 64 x 64 diagonal block and inverts it;
-//   k_panel_mfma   L21 = A21 L11^-T as a GEMM with the inverted block;
-//   k_syrk_mfma_*  A22 -= L21 L21'.
+//   k_panel_col0   L21 = A21 L11^-T as a GEMM with the inverted block, and
+//                  A22 -= L21 L21' on the first trailing block column;
+//   k_syrk_mfma    A22 -= L21 L21' on the other trailing tiles.
 // The triangular solves with the factor run one launch per 64-row block.
 constexpr int kNB = 64;
 
@@ -2151,28 +2152,45 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 //   FP64: v_mfma_f64_16x16x4_f64 -- A lane l = A[l % 16][k + l / 16], B lane
 //         l = B[k + l / 16][l % 16], result r = D[l / 16 + 4 r][l % 16]
 //         (both result layouts checked on gfx950 with a lane probe).
-template <typename F>
-__device__ __forceinline__ void tile_nt(const float (*As)[kNB + 1], const float (*Bs)[kNB + 1], int kb, F&& put) {
+// One wave's 32 x 32 quadrant of a 64 x 64 tile product, in MFMA registers.
+template <typename T>
+struct TileAcc;
+template <>
+struct TileAcc<float> {
+  floatx16 v;
+};
+template <>
+struct TileAcc<double> {
+  doublex4 v[2][2];
+};
+
+__device__ __forceinline__ void tile_mma(const float (*As)[kNB + 1], const float (*Bs)[kNB + 1], int kb,
+                                         TileAcc<float>& acc) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int qi = w >> 1, qj = w & 1;
-  floatx16 acc;
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int r = 0; r < 16; ++r) acc.v[r] = 0.0f;
   const int kpad = (kb + 1) & ~1;
   for (int kk = 0; kk < kpad; kk += 2) {
     const float a = As[qi * 32 + (lane & 31)][kk + (lane >> 5)];
     const float b = Bs[qj * 32 + (lane & 31)][kk + (lane >> 5)];
-    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    acc.v = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc.v, 0, 0, 0);
   }
-  for (int r = 0; r < 16; ++r) put(qi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), qj * 32 + (lane & 31), acc[r]);
 }
 template <typename F>
-__device__ __forceinline__ void tile_nt(const double (*As)[kNB + 1], const double (*Bs)[kNB + 1], int kb, F&& put) {
+__device__ __forceinline__ void tile_put(const TileAcc<float>& acc, F&& put) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int qi = w >> 1, qj = w & 1;
-  doublex4 acc[2][2];
+#pragma unroll
+  for (int r = 0; r < 16; ++r)
+    put(qi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), qj * 32 + (lane & 31), acc.v[r]);
+}
+__device__ __forceinline__ void tile_mma(const double (*As)[kNB + 1], const double (*Bs)[kNB + 1], int kb,
+                                         TileAcc<double>& acc) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int qi = w >> 1, qj = w & 1;
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
-      for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.0;
+      for (int r = 0; r < 4; ++r) acc.v[a][b][r] = 0.0;
   const int kpad = (kb + 3) & ~3;
   for (int kk = 0; kk < kpad; kk += 4) {
     double av[2], bv[2];
@@ -2181,12 +2199,61 @@ __device__ __forceinline__ void tile_nt(const double (*As)[kNB + 1], const doubl
       bv[q] = Bs[qj * 32 + q * 16 + (lane & 15)][kk + (lane >> 4)];
     }
     for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+      for (int b = 0; b < 2; ++b)
+        acc.v[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc.v[a][b], 0, 0, 0);
   }
+}
+template <typename F>
+__device__ __forceinline__ void tile_put(const TileAcc<double>& acc, F&& put) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int qi = w >> 1, qj = w & 1;
+#pragma unroll
   for (int a = 0; a < 2; ++a)
+#pragma unroll
     for (int b = 0; b < 2; ++b)
+#pragma unroll
       for (int r = 0; r < 4; ++r)
-        put(qi * 32 + a * 16 + (lane >> 4) + 4 * r, qj * 32 + b * 16 + (lane & 15), acc[a][b][r]);
+        put(qi * 32 + a * 16 + (lane >> 4) + 4 * r, qj * 32 + b * 16 + (lane & 15), acc.v[a][b][r]);
+}
+template <typename T, typename F>
+__device__ __forceinline__ void tile_nt(const T (*As)[kNB + 1], const T (*Bs)[kNB + 1], int kb, F&& put) {
+  TileAcc<T> acc;
+  tile_mma(As, Bs, kb, acc);
+  tile_put(acc, put);
+}
+
+// A[r0 + j][c0 + i] -= element (i, j) of a transposed tile product, lower
+// triangle only (row >= col).  All 16 reads of a lane are issued before the
+// first write: a read-modify-write per element would wait one memory round
+// trip per element.
+template <typename T>
+__device__ __forceinline__ void tile_sub_lower(const TileAcc<T>& acc, T* __restrict__ A, int64_t n, int64_t r0,
+                                               int64_t c0) {
+  T val[16];
+  int q = 0;
+  if (r0 + kNB <= n && r0 >= c0 + kNB) {
+    // Whole tile inside and strictly below the diagonal (uniform): no
+    // per-element guards, so the stores issue back to back.
+    tile_put(acc, [&](int i, int j, T) { val[q++] = A[(r0 + j) + (c0 + i) * n]; });
+    q = 0;
+    tile_put(acc, [&](int, int, T v) {
+      val[q] -= v;
+      ++q;
+    });
+    q = 0;
+    tile_put(acc, [&](int i, int j, T) { A[(r0 + j) + (c0 + i) * n] = val[q++]; });
+    return;
+  }
+  tile_put(acc, [&](int i, int j, T) {
+    const int64_t row = r0 + j, col = c0 + i;
+    val[q++] = (row < n && col < n && row >= col) ? A[row + col * n] : T(0);
+  });
+  q = 0;
+  tile_put(acc, [&](int i, int j, T v) {
+    const int64_t row = r0 + j, col = c0 + i;
+    if (row < n && col < n && row >= col) A[row + col * n] = val[q] - v;
+    ++q;
+  });
 }
 
 // Rows r0.. of columns k..k+kb of A into a 64 x 64 LDS tile (zero-padded).
@@ -2195,10 +2262,18 @@ __device__ __forceinline__ void load_cols(T (*Ts)[kNB + 1], const T* __restrict_
                                           int64_t r0) {
   // 256 threads: all 16 loads of a thread in flight before the LDS stores.
   T v[kNB * kNB / 256];
+  if (kb == kNB && r0 + kNB <= n) {  // whole tile in range (uniform): no per-load guards
 #pragma unroll
-  for (int q = 0; q < kNB * kNB / 256; ++q) {
-    const int idx = threadIdx.x + 256 * q, i = idx % kNB, l = idx / kNB;
-    v[q] = (l < kb && r0 + i < n) ? A[(r0 + i) + (int64_t)(k + l) * n] : T(0);
+    for (int q = 0; q < kNB * kNB / 256; ++q) {
+      const int idx = threadIdx.x + 256 * q, i = idx % kNB, l = idx / kNB;
+      v[q] = A[(r0 + i) + (int64_t)(k + l) * n];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < kNB * kNB / 256; ++q) {
+      const int idx = threadIdx.x + 256 * q, i = idx % kNB, l = idx / kNB;
+      v[q] = (l < kb && r0 + i < n) ? A[(r0 + i) + (int64_t)(k + l) * n] : T(0);
+    }
   }
 #pragma unroll
   for (int q = 0; q < kNB * kNB / 256; ++q) {
@@ -2207,21 +2282,38 @@ __device__ __forceinline__ void load_cols(T (*Ts)[kNB + 1], const T* __restrict_
   }
 }
 
-// Panel: A[k+kb:n, k:k+kb] <- A[...] L11^-T, one 64-row slab per workgroup.
+// Look-ahead form of the panel step: workgroup ti writes its 64-row slab of
+// L21 = A21 L11^-T and then applies the trailing update to tile (ti, 0), the
+// first block column after the panel (the next step's diagonal and panel
+// columns).  Every workgroup forms the first slab's L21 rows itself (same
+// operands, same MFMA sequence: the values the panel writes), so no
+// workgroup waits for another.  The remaining tiles (k_syrk_mfma with
+// tile offset 1) can then run beside the next diagonal factor.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_mfma(T* __restrict__ A, int64_t n, int k, int kb,
+__global__ __launch_bounds__(256) void k_panel_col0(T* __restrict__ A, int64_t n, int k, int kb,
                                                     const T* __restrict__ Linv) {
   __shared__ T As[kNB][kNB + 1];
   __shared__ T Bs[kNB][kNB + 1];
-  const int64_t r0 = (int64_t)k + kb + (int64_t)blockIdx.x * kNB;
+  __shared__ T Cs[kNB][kNB + 1];
+  const int64_t base = (int64_t)k + kb;
+  const int64_t r0 = base + (int64_t)blockIdx.x * kNB;
   load_cols(As, A, n, k, kb, r0);
   load_cols(Bs, Linv, kNB, 0, kNB, 0);
+  load_cols(Cs, A, n, k, kb, base);
   __syncthreads();
-  // The transposed product (Linv A21') puts the lane index on the row of the
-  // column-major matrix: a store instruction writes whole column segments.
-  tile_nt(Bs, As, kb, [&](int i, int j, T v) {
+  TileAcc<T> xi, x0;
+  tile_mma(Bs, As, kb, xi);  // element (i, j) = L21[r0 + j][k + i]
+  tile_mma(Bs, Cs, kb, x0);  // element (i, j) = L21[base + j][k + i]
+  tile_put(xi, [&](int i, int j, T v) {
     if (r0 + j < n && i < kb) A[(r0 + j) + (int64_t)(k + i) * n] = v;
   });
+  __syncthreads();
+  tile_put(xi, [&](int i, int j, T v) { As[j][i] = v; });
+  tile_put(x0, [&](int i, int j, T v) { Cs[j][i] = v; });
+  __syncthreads();
+  TileAcc<T> up;
+  tile_mma(Cs, As, kb, up);  // element (i, j) = sum_l L21[base + i][l] L21[r0 + j][l]
+  tile_sub_lower(up, A, n, r0, base);
 }
 
 __device__ __forceinline__ void syrk_tile_coords(int t, int& ti, int& tj) {
@@ -2231,23 +2323,25 @@ __device__ __forceinline__ void syrk_tile_coords(int t, int& ti, int& tj) {
   tj = t - ti * (ti + 1) / 2;
 }
 
-// Trailing update A22 -= L21 L21' on 64 x 64 lower tiles (MFMA).
+// Trailing update A22 -= L21 L21' on 64 x 64 lower tiles (MFMA), over the
+// tiles (ti, tj) with ti >= tj >= off (tile units after the panel).
 template <typename T>
-__global__ __launch_bounds__(256) void k_syrk_mfma(T* __restrict__ A, int64_t n, int k, int kb) {
+__global__ __launch_bounds__(256) void k_syrk_mfma(T* __restrict__ A, int64_t n, int k, int kb, int off) {
   __shared__ T As[kNB][kNB + 1];
   __shared__ T Bs[kNB][kNB + 1];
   int ti, tj;
   syrk_tile_coords(blockIdx.x, ti, tj);
+  ti += off;
+  tj += off;
   const int64_t base = (int64_t)k + kb;
   const int64_t r0 = base + (int64_t)ti * kNB, c0 = base + (int64_t)tj * kNB;
   load_cols(As, A, n, k, kb, r0);
   load_cols(Bs, A, n, k, kb, c0);
   __syncthreads();
   // Transposed product (lane index on the row): coalesced read-modify-write.
-  tile_nt(Bs, As, kb, [&](int i, int j, T v) {
-    const int64_t row = r0 + j, col = c0 + i;
-    if (row < n && col < n && row >= col) A[row + col * n] -= v;
-  });
+  TileAcc<T> up;
+  tile_mma(Bs, As, kb, up);
+  tile_sub_lower(up, A, n, r0, c0);
 }
 
 // Triangular solves with the factor, one launch per 64-row block, using the

@@ -991,6 +991,11 @@ struct mba_context {
   hipEvent_t stage_ev[kStage] = {};
   hipEvent_t look_ev[kLook] = {};  // PCG look-ahead: one per in-flight iteration
   hipEvent_t tev[3] = {};          // LM phase timing
+  // Dense Cholesky look-ahead: the trailing tiles past the next block column
+  // run on stream2 beside the next diagonal factor; chol_ev[0] orders them
+  // after the panel, chol_ev[1] the next panel after them (rings of 4).
+  hipStream_t stream2 = nullptr;
+  hipEvent_t chol_ev[2][4] = {};
   // Large host -> device uploads: two pinned kBigChunk buffers in turn.
   char* big = nullptr;
   hipEvent_t big_ev[2] = {};
@@ -1139,6 +1144,11 @@ void context_free(mba_context* c) {
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->tev)
     if (e) (void)hipEventDestroy(e);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  for (auto& ring : c->chol_ev)
+    for (hipEvent_t e : ring)
+      if (e) (void)hipEventDestroy(e);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->pin) (void)hipHostFree(c->pin);
   if (c->pin_done) (void)hipHostFree(c->pin_done);
   for (hipEvent_t e : c->big_ev)
@@ -1854,6 +1864,11 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   for (hipEvent_t& e : c->stage_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (hipEvent_t& e : c->look_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (hipEvent_t& e : c->tev) HIP_TRY(hipEventCreate(&e));
+  if (c->opt.linear_solver == MBA_EXPLICIT_SCHUR) {
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    for (auto& ring : c->chol_ev)
+      for (hipEvent_t& e : ring) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   {
     const size_t Wz = (size_t)W;
     const size_t o_flags = sizeof(double) * kSumSlots * Wz;
@@ -2301,6 +2316,11 @@ struct Engine {
     if (c->timing) (void)hipEventRecord(c->ev[5], s);
     HIP_TRY(hipMemcpyAsync(d.L, d.S, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(c->info, 0, sizeof(int32_t), s));
+    // Step k: diagonal factor, then panel + first trailing block column (the
+    // next step's columns) on s; the other trailing tiles on stream2, where
+    // they overlap the next diagonal factor.  Each matrix element still
+    // receives its updates in step order: the next panel waits for them.
+    int ring = 0, pending = -1;
     for (int64_t k = 0; k < n; k += kNB) {
       const int kb = (int)std::min<int64_t>(kNB, n - k);
       T* li = d.Linv + (size_t)(k / kNB) * kNB * kNB;
@@ -2309,11 +2329,21 @@ struct Engine {
       const int64_t rest = n - k - kb;
       if (rest > 0) {
         const int nt = (int)((rest + kNB - 1) / kNB);
-        hipLaunchKernelGGL(k_panel_mfma<T>, dim3(nt), dim3(256), 0, s, d.L, n, (int)k, kb, (const T*)li);
-        hipLaunchKernelGGL(k_syrk_mfma<T>, dim3((unsigned)((int64_t)nt * (nt + 1) / 2)), dim3(256), 0, s, d.L, n,
-                           (int)k, kb);
+        if (pending >= 0) HIP_TRY(hipStreamWaitEvent(s, c->chol_ev[1][pending], 0));
+        pending = -1;
+        hipLaunchKernelGGL(k_panel_col0<T>, dim3(nt), dim3(256), 0, s, d.L, n, (int)k, kb, (const T*)li);
+        if (nt > 1) {
+          HIP_TRY(hipEventRecord(c->chol_ev[0][ring], s));
+          HIP_TRY(hipStreamWaitEvent(c->stream2, c->chol_ev[0][ring], 0));
+          hipLaunchKernelGGL(k_syrk_mfma<T>, dim3((unsigned)((int64_t)(nt - 1) * nt / 2)), dim3(256), 0, c->stream2,
+                             d.L, n, (int)k, kb, 1);
+          HIP_TRY(hipEventRecord(c->chol_ev[1][ring], c->stream2));
+          pending = ring;
+          ring = (ring + 1) & 3;
+        }
       }
     }
+    if (pending >= 0) HIP_TRY(hipStreamWaitEvent(s, c->chol_ev[1][pending], 0));
     if (c->timing) (void)hipEventRecord(c->ev[6], s);
     int32_t info = 0;
     HIP_TRY(hipMemcpyAsync(&info, c->info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
