@@ -214,6 +214,96 @@ __device__ __forceinline__ T group_sum(T v, int lanes) {
   return v;
 }
 
+// Reduce-scatter across a wave.  Stage D exchanges with the lane that
+// differs in lane bit D: given lo and hi (values i and i + n/2 of the n
+// still held), each lane keeps the pair sum of the half its bit selects (lo
+// for bit clear, hi for bit set), halving n.  D = 32 and 16 use gfx950's
+// permlane swaps (v_permlane32_swap: the upper half of the first operand
+// with the lower half of the second; v_permlane16_swap: odd rows of the
+// first with even rows of the second), which place both partners' values
+// of the kept half in the two results; D <= 8 use DPP (row mirror, half-row
+// mirror, quad_perm xor 2, xor 1: each pairs lanes differing in bit D).
+// With one value left a stage is a plain butterfly add (lo = hi).
+template <int D>
+__device__ __forceinline__ float rs_pair(float lo, float hi) {
+  if constexpr (D == 32 || D == 16) {
+    const auto r = D == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(lo), __float_as_uint(hi), false, false)
+                           : __builtin_amdgcn_permlane16_swap(__float_as_uint(lo), __float_as_uint(hi), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  } else {
+    constexpr int ctrl = D == 8 ? 0x140 : D == 4 ? 0x141 : D == 2 ? 0x4E : 0xB1;
+    const bool bit = (threadIdx.x & D) != 0;
+    const float keep = bit ? hi : lo, send = bit ? lo : hi;
+    return keep + dpp_mov<ctrl>(send);
+  }
+}
+template <int D>
+__device__ __forceinline__ double rs_pair(double lo, double hi) {
+  if constexpr (D == 32 || D == 16) {
+    const unsigned long long l = (unsigned long long)__double_as_longlong(lo);
+    const unsigned long long h = (unsigned long long)__double_as_longlong(hi);
+    const auto r0 = D == 32 ? __builtin_amdgcn_permlane32_swap((unsigned)l, (unsigned)h, false, false)
+                            : __builtin_amdgcn_permlane16_swap((unsigned)l, (unsigned)h, false, false);
+    const auto r1 = D == 32 ? __builtin_amdgcn_permlane32_swap((unsigned)(l >> 32), (unsigned)(h >> 32), false, false)
+                            : __builtin_amdgcn_permlane16_swap((unsigned)(l >> 32), (unsigned)(h >> 32), false, false);
+    const double a = __longlong_as_double((long long)(((unsigned long long)r1[0] << 32) | r0[0]));
+    const double b = __longlong_as_double((long long)(((unsigned long long)r1[1] << 32) | r0[1]));
+    return a + b;
+  } else {
+    constexpr int ctrl = D == 8 ? 0x140 : D == 4 ? 0x141 : D == 2 ? 0x4E : 0xB1;
+    const bool bit = (threadIdx.x & D) != 0;
+    const double keep = bit ? hi : lo, send = bit ? lo : hi;
+    return keep + dpp_mov<ctrl>(send);
+  }
+}
+template <typename T, int n, int D>
+__device__ __forceinline__ void rs_stages(T* a) {
+  if constexpr (D >= 1) {
+    if constexpr (n > 1) {
+#pragma unroll
+      for (int i = 0; i < n / 2; ++i) a[i] = rs_pair<D>(a[i], a[i + n / 2]);
+      rs_stages<T, n / 2, D / 2>(a);
+    } else {
+      a[0] = rs_pair<D>(a[0], a[0]);
+      rs_stages<T, 1, D / 2>(a);
+    }
+  }
+}
+// log2 of the power of two >= n (n <= 64).
+constexpr int rs_bits(int n) { return n <= 1 ? 0 : 1 + rs_bits((n + 1) / 2); }
+
+// Wave sums of N <= 64 accumulators (all lanes active): lane l returns the
+// sum of accumulator l >> (6 - rs_bits(N)) over the 64 lanes.
+template <typename T, int N>
+__device__ __forceinline__ T wave_reduce_scatter(const T (&v)[N]) {
+  constexpr int P = 1 << rs_bits(N);
+  T a[P];
+#pragma unroll
+  for (int i = 0; i < P; ++i) a[i] = i < N ? v[i] : T(0);
+  rs_stages<T, P, 32>(a);
+  return a[0];
+}
+
+// Block sums of N <= 64 accumulators over B threads: thread t < N returns
+// the sum of accumulator t (one wave: straight from the reduce-scatter when
+// N > 32; otherwise through lds[N * B / 64]).
+template <typename T, int N, int B>
+__device__ __forceinline__ T block_reduce(const T (&acc)[N], T* lds) {
+  constexpr int M = rs_bits(N);
+  const T v = wave_reduce_scatter<T, N>(acc);
+  if constexpr (B == 64 && M == 6) {
+    return v;
+  } else {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, idx = lane >> (6 - M);
+    if ((lane & ((1 << (6 - M)) - 1)) == 0 && idx < N) lds[wave * N + idx] = v;
+    __syncthreads();
+    T s = T(0);
+    if (threadIdx.x < N)
+      for (int w = 0; w < B / 64; ++w) s += lds[w * N + threadIdx.x];
+    return s;
+  }
+}
+
 // Block-wide sum of N accumulators (blockDim.x == kBlock): wave sums land
 // in lds[wave * N + k] (lane k % 64 of each wave keeps sum k, then writes
 // it; no branch per accumulator).
@@ -559,7 +649,9 @@ __global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __rest
                                                    const double* __restrict__ row_obs,
                                                    const double* __restrict__ pts,
                                                    const double* __restrict__ ctab, double huber_a,
-                                                   double* __restrict__ partial, Win win) {
+                                                   double* __restrict__ partial, Win win,
+                                                   float* __restrict__ rrow, float* __restrict__ rcm,
+                                                   const int32_t* __restrict__ row_cm) {
   __shared__ double lds[kBlock / 64];
   SegPos sp = seg_locate(sg);
   if (sp.valid && win_idle_e(win, sp.e)) sp.valid = false;
@@ -576,6 +668,17 @@ __global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __rest
       const double s = o.rx * o.rx + o.ry * o.ry;
       if (!isfinite(s)) bad = 1;
       acc[0] += huber_cost(s, huber_a);
+      if (rrow) {
+        // The FP64 residual, rounded once, for the FP32 passes at this state
+        // (row order and, through row_cm, camera-major order).
+        rrow[2 * (size_t)r] = (float)o.rx;
+        rrow[2 * (size_t)r + 1] = (float)o.ry;
+        const int k = row_cm[r];
+        if (k >= 0) {
+          rcm[2 * (size_t)k] = (float)o.rx;
+          rcm[2 * (size_t)k + 1] = (float)o.ry;
+        }
+      }
     }
     if (bad) win.flag[win_of_e(win, e)] = 1;
   }
@@ -689,12 +792,16 @@ __global__ __launch_bounds__(kBlock) void k_grad_final(const double* __restrict_
 // in FP64; w = Vinv g_p.
 template <typename T, int L>
 __device__ __forceinline__ void point_row_accumulate(const T* __restrict__ ctab, int cam, T ox, T oy,
-                                                     const T X[3], const T sp[3], T huber_a, T v[6],
-                                                     T g[3]) {
+                                                     const T* __restrict__ rf, const T X[3], const T sp[3],
+                                                     T huber_a, T v[6], T g[3]) {
   CamProj<T> c;
   load_proj<T, L>(ctab + (size_t)kCamTab * cam, c);
   Proj<T> o;
   project<T, L>(c, X, ox, oy, o);
+  if (rf) {
+    o.rx = rf[0];
+    o.ry = rf[1];
+  }
   const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
   T Jp[6];
   jac_point(o, c.R, wgt, sp, Jp);
@@ -716,7 +823,7 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
     const T* __restrict__ row_obs, const double* __restrict__ pts64, const T* __restrict__ pscale,
     const T* __restrict__ ctab, T huber_a, Win win, double min_diag, double max_diag,
     double* __restrict__ diag_p, int recompute_diag, T* __restrict__ V, T* __restrict__ gp,
-    T* __restrict__ Vinv, T* __restrict__ pk) {
+    T* __restrict__ Vinv, T* __restrict__ pk, const T* __restrict__ rres) {
   SegPos q = seg_locate(sg);
   if (q.valid && win_idle_e(win, q.e)) q.valid = false;
   const int e = q.valid ? q.e : sg.e0[0];
@@ -728,16 +835,28 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
     const int ln = q.lanes;
     int r = pt_start[e] + q.sub;
     // Two rows per iteration so both rows' loads are in flight together.
+    // rres (optional): residuals evaluated in FP64 at this state, used in
+    // place of the kernel-precision pred - obs.
+    const T* ob = rres ? rres : row_obs;
     for (; r + ln < r1; r += 2 * ln) {
       const int c0 = row_cam[r], c1 = row_cam[r + ln];
-      const T o0x = row_obs[2 * (size_t)r], o0y = row_obs[2 * (size_t)r + 1];
-      const T o1x = row_obs[2 * (size_t)(r + ln)], o1y = row_obs[2 * (size_t)(r + ln) + 1];
-      point_row_accumulate<T, L>(ctab, c0, o0x, o0y, X, sp, huber_a, v, g);
-      point_row_accumulate<T, L>(ctab, c1, o1x, o1y, X, sp, huber_a, v, g);
+      const T o0x = ob[2 * (size_t)r], o0y = ob[2 * (size_t)r + 1];
+      const T o1x = ob[2 * (size_t)(r + ln)], o1y = ob[2 * (size_t)(r + ln) + 1];
+      if (rres) {
+        const T f0[2] = {o0x, o0y}, f1[2] = {o1x, o1y};
+        point_row_accumulate<T, L>(ctab, c0, T(0), T(0), f0, X, sp, huber_a, v, g);
+        point_row_accumulate<T, L>(ctab, c1, T(0), T(0), f1, X, sp, huber_a, v, g);
+      } else {
+        point_row_accumulate<T, L>(ctab, c0, o0x, o0y, nullptr, X, sp, huber_a, v, g);
+        point_row_accumulate<T, L>(ctab, c1, o1x, o1y, nullptr, X, sp, huber_a, v, g);
+      }
     }
-    if (r < r1)
-      point_row_accumulate<T, L>(ctab, row_cam[r], row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1], X, sp,
+    if (r < r1) {
+      const T ox = ob[2 * (size_t)r], oy = ob[2 * (size_t)r + 1];
+      const T f0[2] = {ox, oy};
+      point_row_accumulate<T, L>(ctab, row_cam[r], rres ? T(0) : ox, rres ? T(0) : oy, rres ? f0 : nullptr, X, sp,
                                  huber_a, v, g);
+    }
   }
 #pragma unroll
   for (int i = 0; i < 6; ++i) v[i] = group_sum(v[i], q.lanes);
@@ -815,11 +934,15 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restri
 // camera Jacobi scale left out (applied per camera in k_cam_finalize).
 template <typename T>
 __device__ __forceinline__ void cam_row_accumulate(const CamProj<T>& c, const T dR[27], const T X[3],
-                                                   const T sw[3], T ox, T oy, T huber_a, int with_h,
-                                                   T acc[63]) {
+                                                   const T sw[3], T ox, T oy, bool given, T huber_a,
+                                                   int with_h, T acc[63]) {
   const T one[9] = {T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1)};
   Proj<T> o;
-  project(c, X, ox, oy, o);
+  project(c, X, given ? T(0) : ox, given ? T(0) : oy, o);
+  if (given) {  // (ox, oy) is the residual itself
+    o.rx = ox;
+    o.ry = oy;
+  }
   const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
   T Jc[18];
   jac_cam(o, dR, X, wgt, one, Jc);
@@ -843,14 +966,25 @@ __device__ __forceinline__ void cam_row_accumulate(const CamProj<T>& c, const T 
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cam_pass(
+// Camera-major launches: workgroup slot -> chunk (-1: empty slot), or the
+// identity without a slot map.  Workgroups are dealt to the XCDs round
+// robin, so the host places the chunks of point range q in the slots
+// congruent to q mod 8: each XCD's L2 then serves gathers from one eighth
+// of the point records.
+__device__ __forceinline__ int chunk_of_slot(const int32_t* __restrict__ slot_chunk) {
+  return slot_chunk ? slot_chunk[blockIdx.x] : (int)blockIdx.x;
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(B) void k_cam_pass(
     const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
-    const T* __restrict__ ctab, T huber_a, int with_h, T* __restrict__ partial, Win win) {
-  __shared__ T lds[63 * (kBlock / 64)];
-  const int ch = blockIdx.x;
+    const T* __restrict__ ctab, T huber_a, int with_h, T* __restrict__ partial, Win win,
+    const int32_t* __restrict__ slot_chunk, const T* __restrict__ cres) {
+  __shared__ T lds[63 * (B / 64)];
+  const int ch = chunk_of_slot(slot_chunk);
+  if (ch < 0) return;
   const int f = chunk_f[ch];
   if (win_idle_f(win, f)) return;
   const T* tab = ctab + (size_t)kCamTab * f_to_cam[f];
@@ -864,29 +998,29 @@ __global__ __launch_bounds__(kBlock) void k_cam_pass(
   for (int i = 0; i < 63; ++i) acc[i] = T(0);
   const int end = chunk_end[ch];
   int i = chunk_begin[ch] + threadIdx.x;
+  // cres (optional): residuals evaluated in FP64 at this state, camera-major,
+  // used in place of the kernel-precision pred - obs.
+  const bool given = cres != nullptr;
+  const T* ob = given ? cres : cm_obs;
   // Two rows per iteration: both rows' index, observation and point-record
   // loads are issued before either row is computed.
-  for (; i + kBlock < end; i += 2 * kBlock) {
-    const int e0 = cm_e[i], e1 = cm_e[i + kBlock];
-    const T o0x = cm_obs[2 * (size_t)i], o0y = cm_obs[2 * (size_t)i + 1];
-    const T o1x = cm_obs[2 * (size_t)(i + kBlock)], o1y = cm_obs[2 * (size_t)(i + kBlock) + 1];
+  for (; i + B < end; i += 2 * B) {
+    const int e0 = cm_e[i], e1 = cm_e[i + B];
+    const T o0x = ob[2 * (size_t)i], o0y = ob[2 * (size_t)i + 1];
+    const T o1x = ob[2 * (size_t)(i + B)], o1y = ob[2 * (size_t)(i + B) + 1];
     T X0[3], s0[3], X1[3], s1[3];
     load_pk(pk, e0, X0, s0);
     load_pk(pk, e1, X1, s1);
-    cam_row_accumulate(c, dR, X0, s0, o0x, o0y, huber_a, with_h, acc);
-    cam_row_accumulate(c, dR, X1, s1, o1x, o1y, huber_a, with_h, acc);
+    cam_row_accumulate(c, dR, X0, s0, o0x, o0y, given, huber_a, with_h, acc);
+    cam_row_accumulate(c, dR, X1, s1, o1x, o1y, given, huber_a, with_h, acc);
   }
   if (i < end) {
     T X0[3], s0[3];
     load_pk(pk, cm_e[i], X0, s0);
-    cam_row_accumulate(c, dR, X0, s0, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], huber_a, with_h, acc);
+    cam_row_accumulate(c, dR, X0, s0, ob[2 * (size_t)i], ob[2 * (size_t)i + 1], given, huber_a, with_h, acc);
   }
-  block_sum<T, 63>(acc, lds);
-  if (threadIdx.x < 63) {
-    T s = T(0);
-    for (int w = 0; w < kBlock / 64; ++w) s += lds[w * 63 + threadIdx.x];
-    partial[(size_t)kPartial * ch + threadIdx.x] = s;
-  }
+  const T s = block_reduce<T, 63, B>(acc, lds);
+  if (threadIdx.x < 63) partial[(size_t)kPartial * ch + threadIdx.x] = s;
 }
 
 // One wave per F block: lanes sum the chunk partials (one value each), lane
@@ -969,15 +1103,16 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
 // columns unscaled like the camera pass, point columns scaled, Huber weight
 // applied) -- each row's share of W Vinv W' in the camera's diagonal block
 // of S when a camera observes a point once.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sj_pass(
+template <typename T, int B>
+__global__ __launch_bounds__(B) void k_sj_pass(
     const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
     const T* __restrict__ pscale, const T* __restrict__ Vinv, const T* __restrict__ ctab, T huber_a,
-    T* __restrict__ partial, Win win) {
-  __shared__ T lds[45 * (kBlock / 64)];
-  const int ch = blockIdx.x;
+    T* __restrict__ partial, Win win, const int32_t* __restrict__ slot_chunk) {
+  __shared__ T lds[45 * (B / 64)];
+  const int ch = chunk_of_slot(slot_chunk);
+  if (ch < 0) return;
   const int f = chunk_f[ch];
   if (win_idle_f(win, f)) return;
   const T* tab = ctab + (size_t)kCamTab * f_to_cam[f];
@@ -990,7 +1125,7 @@ __global__ __launch_bounds__(kBlock) void k_sj_pass(
   T acc[45];
 #pragma unroll
   for (int i = 0; i < 45; ++i) acc[i] = T(0);
-  for (int i = chunk_begin[ch] + threadIdx.x; i < chunk_end[ch]; i += kBlock) {
+  for (int i = chunk_begin[ch] + threadIdx.x; i < chunk_end[ch]; i += B) {
     const int e = cm_e[i];
     T X[3], unused[3];
     load_pk(pk, e, X, unused);
@@ -1025,12 +1160,8 @@ __global__ __launch_bounds__(kBlock) void k_sj_pass(
 #pragma unroll
       for (int b = a; b < 9; ++b) fma2(acc[k++], Jc[a], B0[b], Jc[9 + a], B1[b]);
   }
-  block_sum<T, 45>(acc, lds);
-  if (threadIdx.x < 45) {
-    T sum = T(0);
-    for (int w = 0; w < kBlock / 64; ++w) sum += lds[w * 45 + threadIdx.x];
-    partial[(size_t)kPartial * ch + threadIdx.x] = sum;
-  }
+  const T sum = block_reduce<T, 45, B>(acc, lds);
+  if (threadIdx.x < 45) partial[(size_t)kPartial * ch + threadIdx.x] = sum;
 }
 
 // Tsj_f = s_a s_b (sum of the F block's SCHUR_JACOBI partials), 45 per block.
@@ -1173,16 +1304,17 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
 }
 
 // Per chunk: partial_c = sum_rows Jc'(Jp z_e) (9 values).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mv_cam(
+template <typename T, int B>
+__global__ __launch_bounds__(B) void k_mv_cam(
     const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
     const T* __restrict__ cscale, const T* __restrict__ ctab, T huber_a, T* __restrict__ partial,
-    const int32_t* __restrict__ all_done, Win win) {
-  __shared__ T lds[9 * (kBlock / 64)];
+    const int32_t* __restrict__ all_done, Win win, const int32_t* __restrict__ slot_chunk) {
+  __shared__ T lds[9 * (B / 64)];
   if (all_done && *all_done) return;
-  const int ch = blockIdx.x;
+  const int ch = chunk_of_slot(slot_chunk);
+  if (ch < 0) return;
   const int f = chunk_f[ch];
   if (win_idle_f(win, f)) return;
   const T* tab = ctab + (size_t)kCamTab * f_to_cam[f];
@@ -1213,10 +1345,10 @@ __global__ __launch_bounds__(kBlock) void k_mv_cam(
   };
   int i = chunk_begin[ch] + threadIdx.x;
   // Two rows per iteration: both rows' point-record gathers in flight.
-  for (; i + kBlock < end; i += 2 * kBlock) {
-    const int e0 = cm_e[i], e1 = cm_e[i + kBlock];
+  for (; i + B < end; i += 2 * B) {
+    const int e0 = cm_e[i], e1 = cm_e[i + B];
     const T o0x = cm_obs[2 * (size_t)i], o0y = cm_obs[2 * (size_t)i + 1];
-    const T o1x = cm_obs[2 * (size_t)(i + kBlock)], o1y = cm_obs[2 * (size_t)(i + kBlock) + 1];
+    const T o1x = cm_obs[2 * (size_t)(i + B)], o1y = cm_obs[2 * (size_t)(i + B) + 1];
     T X0[3], s0[3], X1[3], s1[3];
     load_pk(pk, e0, X0, s0);
     load_pk(pk, e1, X1, s1);
@@ -1228,12 +1360,8 @@ __global__ __launch_bounds__(kBlock) void k_mv_cam(
     load_pk(pk, cm_e[i], X0, s0);
     row(X0, s0, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1]);
   }
-  block_sum<T, 9>(acc, lds);
-  if (threadIdx.x < 9) {
-    T s = T(0);
-    for (int w = 0; w < kBlock / 64; ++w) s += lds[w * 9 + threadIdx.x];
-    partial[(size_t)kPartial * ch + threadIdx.x] = s;
-  }
+  const T s = block_reduce<T, 9, B>(acc, lds);
+  if (threadIdx.x < 9) partial[(size_t)kPartial * ch + threadIdx.x] = s;
 }
 
 // y_f = [u_term] (U + D^2) x_f - sum partials (L = 0), or D^2 x_f + sum

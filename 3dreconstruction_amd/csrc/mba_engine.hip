@@ -909,6 +909,20 @@ struct mba_context {
   int32_t ng = 0, nslot = 0;
   std::vector<int32_t> cam_grp, cam_gf;
   int32_t *d_cam_grp = nullptr, *d_cam_gf = nullptr;
+  // Launch slot -> camera-major chunk of the packed kernels (null: slot =
+  // chunk); nslots launch slots (nchunks without the map).
+  int32_t* d_slot_chunk = nullptr;
+  int32_t nslots = 0;
+  // Mixed precision, one packed problem: residuals of the FP64 cost at the
+  // current state (slot rcur) and at the trial state (1 - rcur), rounded to
+  // FP32, in row order (rres) and camera-major order (cres, through row_cm;
+  // -1 for rows of constant cameras).  Inside the LM loop (rvalid) the FP32
+  // passes use them instead of their own pred - obs, whose rounding (~1e-5
+  // relative) would put noise into the gradient.
+  bool rfix = false, rvalid = false;
+  int rcur = 0;
+  float *rres[2] = {}, *cres[2] = {};
+  int32_t* row_cm = nullptr;
   // 9 per F block (NULL when every slot is a free parameter): fmask = 1 for
   // a slot that takes steps, xmask = 1 for a slot of the parameter block
   // (counted in |x|; a SubsetParameterization entry is held but counted).
@@ -1126,6 +1140,12 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_st
 
 // Rows per camera-major chunk: aim for >= ~2048 workgroups on large
 // problems, 64..2048 rows each.
+// XCDs of the part (workgroups are dealt to them round robin) and the
+// camera-major row count from which the chunks are split by point range.
+constexpr int kXcd = 8;
+constexpr int64_t kXcdSplitMin = 1 << 18;
+constexpr int kXcdChunkRows = 512;
+
 int chunk_rows_for(int64_t nrows) {
   int64_t c = nrows / 2048;
   c = std::max<int64_t>(64, std::min<int64_t>(2048, c));
@@ -1678,17 +1698,54 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   }
   trace.mark("camera-major sort");
   const int cr = chunk_rows_for(c->ncm);
-  std::vector<int32_t> chunk_f, chunk_begin, chunk_end, key_chunk((size_t)nkeys + 1, 0);
+  // Packed layout, large problems: each camera's rows (in row order, i.e.
+  // by device point) are split at the bounds of kXcd point ranges of equal
+  // row count, and range q's chunks get the launch slots congruent to q
+  // mod kXcd, so that each XCD gathers from one range of point records.
+  const bool xcd_split = !c->grouped && c->ncm >= kXcdSplitMin;
+  std::vector<int32_t> range_row(kXcd + 1, 0);
+  if (xcd_split) {
+    for (int q = 1; q < kXcd; ++q) {
+      const int64_t target = (int64_t)c->nrows * q / kXcd;
+      const size_t de = (size_t)(std::lower_bound(dev_pt_start.begin(), dev_pt_start.end(), target) -
+                                 dev_pt_start.begin());
+      range_row[q] = dev_pt_start[std::min(de, (size_t)c->ne)];
+    }
+    range_row[kXcd] = c->nrows;
+  }
+  const int nq = xcd_split ? kXcd : 1;
+  // Split chunks run as one-wave workgroups of kXcdChunkRows rows.
+  const int crq = xcd_split ? kXcdChunkRows : cr;
+  std::vector<int32_t> chunk_f, chunk_begin, chunk_end, chunk_q, key_chunk((size_t)nkeys + 1, 0);
   for (int32_t f = 0; f < nkeys; ++f) {
     key_chunk[f] = (int32_t)chunk_f.size();
-    for (int64_t b = fstart[f]; b < fstart[f + 1]; b += cr) {
-      chunk_f.push_back(c->grouped ? key_cam[f] : f);
-      chunk_begin.push_back((int32_t)b);
-      chunk_end.push_back((int32_t)std::min<int64_t>(b + cr, fstart[f + 1]));
+    int64_t j = fstart[f];
+    for (int q = 0; q < nq; ++q) {
+      int64_t jend = fstart[f + 1];
+      if (q + 1 < nq)
+        jend = std::lower_bound(cm_src.begin() + j, cm_src.begin() + jend, range_row[q + 1]) - cm_src.begin();
+      for (int64_t b = j; b < jend; b += crq) {
+        chunk_f.push_back(c->grouped ? key_cam[f] : f);
+        chunk_begin.push_back((int32_t)b);
+        chunk_end.push_back((int32_t)std::min<int64_t>(b + crq, jend));
+        chunk_q.push_back(q);
+      }
+      j = jend;
     }
   }
   key_chunk[nkeys] = (int32_t)chunk_f.size();
   c->nchunks = (int32_t)chunk_f.size();
+  std::vector<int32_t> slot_chunk;
+  if (xcd_split) {
+    std::vector<std::vector<int32_t>> lists(kXcd);
+    for (int32_t ch = 0; ch < c->nchunks; ++ch) lists[chunk_q[ch]].push_back(ch);
+    size_t most = 0;
+    for (const auto& l : lists) most = std::max(most, l.size());
+    slot_chunk.assign(most * kXcd, -1);
+    for (int q = 0; q < kXcd; ++q)
+      for (size_t i = 0; i < lists[q].size(); ++i) slot_chunk[i * kXcd + q] = lists[q][i];
+  }
+  c->nslots = xcd_split ? (int32_t)slot_chunk.size() : c->nchunks;
   // Host-side checks are done: the device from here on.
   int ndev = 0;
   HIP_TRY(hipGetDeviceCount(&ndev));
@@ -1697,6 +1754,22 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(hipHostMalloc((void**)&c->big, 2 * kBigChunk, hipHostMallocDefault));
   for (hipEvent_t& e : c->big_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (!slot_chunk.empty()) MBA_TRY(upload(c, &c->d_slot_chunk, slot_chunk));
+  c->rfix = c->mixed && W == 1 && !c->grouped && c->nrows > 0;
+  if (c->rfix) {
+    Buf<int32_t> row_cm((size_t)c->nrows);
+    parallel_for(c->nrows, [&](int64_t b, int64_t end) {
+      for (int64_t r = b; r < end; ++r) row_cm[r] = -1;
+    });
+    parallel_for(c->ncm, [&](int64_t b, int64_t end) {
+      for (int64_t k = b; k < end; ++k) row_cm[cm_src[k]] = (int32_t)k;
+    });
+    MBA_TRY(upload(c, &c->row_cm, row_cm));
+    for (int q = 0; q < 2; ++q) {
+      MBA_TRY(c->pool.alloc(&c->rres[q], 2 * (size_t)c->nrows));
+      MBA_TRY(c->pool.alloc(&c->cres[q], 2 * (size_t)std::max(1, c->ncm)));
+    }
+  }
   // Partial rows of each F block: its chunks (packed), or its camera's
   // chunks / its cameras' group rows after the nchunks camera rows.
   std::vector<int32_t> f_lo(c->nf), f_hi(c->nf);
@@ -2021,14 +2094,19 @@ struct Engine {
 
   // FP64 cost per window at (cams, pts) into sum slot kSlotCost and its
   // non-finite flags into flag slot 1; cost_result reads them after a sync.
-  mba_status cost64_async(const double* cams, const double* pts) {
+  // slot >= 0 (residual-fix contexts): the residuals also go to slot `slot`
+  // of rres / cres.
+  mba_status cost64_async(const double* cams, const double* pts, int slot = -1) {
     MBA_TRY(clear_flags());
     table_into<double>(cams, c->ctab64);
     const int nb = c->seg_blocks;
     MBA_TRY(begin_sum());
+    const bool wr = c->rfix && slot >= 0;
     if (nb > 0)
       hipLaunchKernelGGL((c->grouped ? k_cost64<1> : k_cost64<0>), dim3(nb), dim3(kBlock), 0, s, c->seg, d.pt_start,
-                         d.row_cam, c->row_obs64, pts, c->ctab64, c->opt.huber_a, c->red, win(c->wacc));
+                         d.row_cam, c->row_obs64, pts, c->ctab64, c->opt.huber_a, c->red, win(c->wacc),
+                         wr ? c->rres[slot] : nullptr, wr ? c->cres[slot] : nullptr,
+                         wr ? (const int32_t*)c->row_cm : nullptr);
     MBA_TRY(sum_async(nb, kSlotCost));
     return flags_async(1);
   }
@@ -2040,21 +2118,37 @@ struct Engine {
       (*finite)[w] = flag_at(1, w) == 0 && std::isfinite((*cost)[w]);
     }
   }
-  mba_status cost64(const double* cams, const double* pts, std::vector<double>* cost, std::vector<uint8_t>* finite) {
-    MBA_TRY(cost64_async(cams, pts));
+  mba_status cost64(const double* cams, const double* pts, std::vector<double>* cost, std::vector<uint8_t>* finite,
+                    int slot = -1) {
+    MBA_TRY(cost64_async(cams, pts, slot));
     MBA_TRY(sync());
     cost_result(cost, finite);
     return MBA_OK;
   }
 
   // Jacobian + Schur pass at the current x (tables and radii current).
+  // FP64-evaluated residuals of the current state for the FP32 passes
+  // (null: the passes form pred - obs themselves).
+  const T* fixed_rows() const {
+    if constexpr (std::is_same<T, float>::value)
+      return c->rvalid ? c->rres[c->rcur] : nullptr;
+    else
+      return nullptr;
+  }
+  const T* fixed_cm() const {
+    if constexpr (std::is_same<T, float>::value)
+      return c->rvalid ? c->cres[c->rcur] : nullptr;
+    else
+      return nullptr;
+  }
+
   void jacobian_pass(int recompute_diag) {
     if (c->timing) (void)hipEventRecord(c->ev[0], s);
     if (c->seg_blocks > 0)
       hipLaunchKernelGGL((c->grouped ? k_point_pass<T, 1> : k_point_pass<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0,
                          s, c->seg, d.pt_start, d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, win(),
                          c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag,
-                         d.V, d.gp, d.Vinv, d.pk);
+                         d.V, d.gp, d.Vinv, d.pk, fixed_rows());
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
     if (c->nchunks > 0) {
       if (c->grouped) {
@@ -2062,8 +2156,10 @@ struct Engine {
                            d.chunk_begin, d.chunk_end, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, d.partial);
         group_rows(63, nullptr);
       } else
-        hipLaunchKernelGGL(k_cam_pass<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
-                           d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial, win());
+        hipLaunchKernelGGL((c->d_slot_chunk ? k_cam_pass<T, 64> : k_cam_pass<T, kBlock>), dim3(c->nslots),
+                           dim3(c->d_slot_chunk ? 64 : kBlock), 0, s, d.chunk_f, d.chunk_begin,
+                           d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial, win(),
+                           (const int32_t*)c->d_slot_chunk, fixed_cm());
     }
     sj_terms();
     if (c->timing) (void)hipEventRecord(c->ev[2], s);
@@ -2079,8 +2175,10 @@ struct Engine {
   void sj_terms() {
     if (!d.Tsj || c->nf == 0) return;
     if (c->nchunks > 0)
-      hipLaunchKernelGGL(k_sj_pass<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin, d.chunk_end,
-                         d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.pscale, d.Vinv, d.ctab, huber, d.partial_sj, win());
+      hipLaunchKernelGGL((c->d_slot_chunk ? k_sj_pass<T, 64> : k_sj_pass<T, kBlock>), dim3(c->nslots),
+                         dim3(c->d_slot_chunk ? 64 : kBlock), 0, s, d.chunk_f, d.chunk_begin, d.chunk_end,
+                         d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.pscale, d.Vinv, d.ctab, huber, d.partial_sj, win(),
+                         (const int32_t*)c->d_slot_chunk);
     hipLaunchKernelGGL(k_sj_finalize<T>, dim3(nblocks(45 * (int64_t)c->nf)), dim3(kBlock), 0, s, c->nf, d.f_lo,
                        d.f_hi, (const T*)d.partial_sj, (const T*)d.cscale, win(), d.Tsj);
   }
@@ -2132,9 +2230,10 @@ struct Engine {
                          d.ctab, dtab, huber, d.partial, done);
       group_rows(9, done);
     } else
-      hipLaunchKernelGGL(k_mv_cam<T>, dim3(c->nchunks), dim3(kBlock), 0, s, d.chunk_f, d.chunk_begin,
+      hipLaunchKernelGGL((c->d_slot_chunk ? k_mv_cam<T, 64> : k_mv_cam<T, kBlock>), dim3(c->nslots),
+                         dim3(c->d_slot_chunk ? 64 : kBlock), 0, s, d.chunk_f, d.chunk_begin,
                          d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber, d.partial,
-                         done, wv ? *wv : win());
+                         done, wv ? *wv : win(), (const int32_t*)c->d_slot_chunk);
   }
 
   // y = S x (implicit), x and y F-order.  Inside PCG (`done` set) the
@@ -2563,6 +2662,7 @@ struct Engine {
     if (W == 1) {
       std::swap(c->cams64, c->cams64_t);
       std::swap(c->pts64, c->pts64_t);
+      c->rcur = 1 - c->rcur;  // the trial state's residuals are now current
       // Keep the constant blocks of the trial copy in sync.
       HIP_TRY(hipMemcpyAsync(c->cams64_t, c->cams64, sizeof(double) * 9 * c->nslot, hipMemcpyDeviceToDevice, s));
       return MBA_OK;
@@ -2624,6 +2724,12 @@ struct Engine {
   mba_status solve(mba_summary* sums, mba_iteration* its, int32_t max_its) {
     const mba_options& o = c->opt;
     const double t_start = now_s();
+    // The fixed residuals track the state only inside this loop.
+    struct ResidualGuard {
+      mba_context* c;
+      ~ResidualGuard() { c->rvalid = false; }
+    } guard{c};
+    c->rvalid = false;
     std::vector<WinLM> L(W);
     for (int w = 0; w < W; ++w) {
       std::memset(&sums[w], 0, sizeof(mba_summary));
@@ -2658,7 +2764,8 @@ struct Engine {
     std::vector<double> cost, gmax, gnorm;
     std::vector<uint8_t> fin;
     double t0 = now_s();
-    MBA_TRY(cost64(c->cams64, c->pts64, &cost, &fin));
+    MBA_TRY(cost64(c->cams64, c->pts64, &cost, &fin, c->rcur));
+    c->rvalid = c->rfix;
     t_res += now_s() - t0;
     for (int w = 0; w < W; ++w) {
       if (!fin[w]) {
@@ -2717,7 +2824,7 @@ struct Engine {
       HIP_TRY(hipEventRecord(c->tev[1], s));
       MBA_TRY(model_cost_async());
       MBA_TRY(plus_async());
-      MBA_TRY(cost64_async(c->cams64_t, c->pts64_t));
+      MBA_TRY(cost64_async(c->cams64_t, c->pts64_t, 1 - c->rcur));
       HIP_TRY(hipEventRecord(c->tev[2], s));
       MBA_TRY(sync());
       t_lin += elapsed_s(c->tev[0], c->tev[1]);
