@@ -1144,7 +1144,7 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_st
 // camera-major row count from which the chunks are split by point range.
 constexpr int kXcd = 8;
 constexpr int64_t kXcdSplitMin = 1 << 18;
-constexpr int kXcdChunkRows = 512;
+constexpr int kXcdChunkRows = 768;
 
 int chunk_rows_for(int64_t nrows) {
   int64_t c = nrows / 2048;
@@ -1724,10 +1724,13 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       int64_t jend = fstart[f + 1];
       if (q + 1 < nq)
         jend = std::lower_bound(cm_src.begin() + j, cm_src.begin() + jend, range_row[q + 1]) - cm_src.begin();
-      for (int64_t b = j; b < jend; b += crq) {
+      // ceil(rows / crq) chunks of equal size (no short tail chunk).
+      const int64_t nck = (jend - j + crq - 1) / crq;
+      const int64_t len = nck > 0 ? (jend - j + nck - 1) / nck : 0;
+      for (int64_t b = j; b < jend; b += len) {
         chunk_f.push_back(c->grouped ? key_cam[f] : f);
         chunk_begin.push_back((int32_t)b);
-        chunk_end.push_back((int32_t)std::min<int64_t>(b + crq, jend));
+        chunk_end.push_back((int32_t)std::min<int64_t>(b + len, jend));
         chunk_q.push_back(q);
       }
       j = jend;

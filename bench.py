@@ -136,7 +136,7 @@ def run(args):
     # recomputes W instead of storing it, so it moves fewer bytes than the
     # unit counts: the fraction is an effective one; `traffic` is what the
     # pass's kernels really move (counters), and `dominant_kernel` gives the
-    # camera pass against its own bytes.
+    # slower of the point and camera passes against its own bytes.
     unit_bytes = 16 + 108 + 60.0 * npt / nobs
     pass_bytes = unit_bytes * nobs
     achieved = pass_bytes / (kern_mixed[2] * 1e-3) / 1e9
