@@ -405,6 +405,14 @@ __device__ __forceinline__ void load_proj(const T* __restrict__ tab, CamProj<T>&
   }
 }
 
+// Reciprocal and reciprocal square root: FP32 kernels use the hardware
+// instructions (v_rcp_f32 / v_rsq_f32, about 1 ulp, well inside the FP32
+// path's rounding budget); FP64 keeps IEEE division and sqrt.
+__device__ __forceinline__ float kp_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double kp_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float kp_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double kp_rsqrt(double x) { return 1.0 / sqrt(x); }
+
 template <typename T>
 struct Proj {
   T rx, ry;          // residual, predicted - observed
@@ -421,7 +429,7 @@ __device__ __forceinline__ void project(const CamProj<T>& c, const T X[3], T ox,
   const T xc = c.R[0] * X[0] + c.R[1] * X[1] + c.R[2] * X[2] + c.t[0];
   const T yc = c.R[3] * X[0] + c.R[4] * X[1] + c.R[5] * X[2] + c.t[1];
   const T zc = c.R[6] * X[0] + c.R[7] * X[1] + c.R[8] * X[2] + c.t[2];
-  const T iz = T(1) / zc;
+  const T iz = kp_rcp(zc);
   const T xe = xc * iz, ye = yc * iz;
   const T r2 = xe * xe + ye * ye;
   T d, fdd;
@@ -464,7 +472,7 @@ __device__ __forceinline__ void project(const CamProj<T>& c, const T X[3], T ox,
 template <typename T>
 __device__ __forceinline__ T huber_weight(T s, T a) {
   if (a > T(0) && s > a * a) {
-    T rho1 = a / sqrt(s);
+    T rho1 = a * kp_rsqrt(s);
     rho1 = rho1 > tiny<T>() ? rho1 : tiny<T>();
     return sqrt(rho1);
   }
