@@ -240,6 +240,24 @@ def test_explicit_solve_with_tracks_longer_than_128(mba, O):
     assert abs(sm.final_cost - so.final_cost) <= 1e-6 * so.final_cost
 
 
+def test_mixed_lm_with_fp64_residuals_on_split_layout(mba, O):
+    """C2 (697k observations, Huber 4 px): the camera-major kernels run on XCD
+    point ranges, and inside the LM loop the mixed Jacobian pass takes its
+    residuals from the FP64 cost of the same state.  FP32 residual noise in
+    the gradient used to give runs of invalid steps (negative FP64 model cost
+    change) that stopped mixed solves early, well above the minimum.  The
+    mixed solve must take no invalid step and end at or below the FP64
+    solve's final cost (within 1e-6)."""
+    d = O.generate(O.preset(2))
+    res = {}
+    for prec in (0, 1):
+        o = mba.default_options(precision=prec, huber_a=4.0, linear_solver=1, max_num_iterations=60)
+        res[prec] = mba.solve(ba_problem(mba, d), o)
+    assert res[1].is_solution_usable
+    assert not any(not it.step_is_valid for it in res[1].iterations[1:])
+    assert res[1].final_cost <= res[0].final_cost * (1 + 1e-6), (res[1].final_cost, res[0].final_cost)
+
+
 @pytest.mark.parametrize("solver", [0, 1])
 def test_mixed_precision_reaches_fp64_final_cost(mba, golden, c1, solver):
     s = mba.solve(ba_problem(mba, c1), mba.default_options(max_num_iterations=20, linear_solver=solver,
