@@ -1716,26 +1716,50 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   const int nq = xcd_split ? kXcd : 1;
   // Split chunks run as one-wave workgroups of kXcdChunkRows rows.
   const int crq = xcd_split ? kXcdChunkRows : cr;
-  std::vector<int32_t> chunk_f, chunk_begin, chunk_end, chunk_q, key_chunk((size_t)nkeys + 1, 0);
-  for (int32_t f = 0; f < nkeys; ++f) {
-    key_chunk[f] = (int32_t)chunk_f.size();
-    int64_t j = fstart[f];
-    for (int q = 0; q < nq; ++q) {
-      int64_t jend = fstart[f + 1];
-      if (q + 1 < nq)
-        jend = std::lower_bound(cm_src.begin() + j, cm_src.begin() + jend, range_row[q + 1]) - cm_src.begin();
-      // ceil(rows / crq) chunks of equal size (no short tail chunk).
-      const int64_t nck = (jend - j + crq - 1) / crq;
-      const int64_t len = nck > 0 ? (jend - j + nck - 1) / nck : 0;
-      for (int64_t b = j; b < jend; b += len) {
-        chunk_f.push_back(c->grouped ? key_cam[f] : f);
-        chunk_begin.push_back((int32_t)b);
-        chunk_end.push_back((int32_t)std::min<int64_t>(b + len, jend));
-        chunk_q.push_back(q);
+  // Two parallel passes over the keys (the binary searches in cm_src are
+  // cache misses: serially they cost ~10 ms at 6.8M rows): the range split
+  // points and chunk counts of each key, then the chunks themselves.
+  std::vector<int64_t> split((size_t)nkeys * (nq + 1));
+  std::vector<int32_t> key_chunk((size_t)nkeys + 1, 0);
+  auto chunk_len = [&](int64_t rows) -> int64_t {  // ceil(rows / crq) chunks of equal size
+    const int64_t nck = (rows + crq - 1) / crq;
+    return nck > 0 ? (rows + nck - 1) / nck : 0;
+  };
+  parallel_for(nkeys, [&](int64_t b, int64_t e) {
+    for (int64_t f = b; f < e; ++f) {
+      int64_t* sp = &split[(size_t)f * (nq + 1)];
+      sp[0] = fstart[f];
+      int32_t n = 0;
+      for (int q = 0; q < nq; ++q) {
+        int64_t jend = fstart[f + 1];
+        if (q + 1 < nq)
+          jend = std::lower_bound(cm_src.begin() + sp[q], cm_src.begin() + jend, range_row[q + 1]) - cm_src.begin();
+        sp[q + 1] = jend;
+        const int64_t rows = jend - sp[q];
+        if (rows > 0) n += (int32_t)((rows + chunk_len(rows) - 1) / chunk_len(rows));
       }
-      j = jend;
+      key_chunk[f + 1] = n;
     }
-  }
+  }, 1 << 10);
+  for (int32_t f = 0; f < nkeys; ++f) key_chunk[f + 1] += key_chunk[f];
+  const size_t total_chunks = (size_t)key_chunk[nkeys];
+  std::vector<int32_t> chunk_f(total_chunks), chunk_begin(total_chunks), chunk_end(total_chunks),
+      chunk_q(total_chunks);
+  parallel_for(nkeys, [&](int64_t b, int64_t e) {
+    for (int64_t f = b; f < e; ++f) {
+      const int64_t* sp = &split[(size_t)f * (nq + 1)];
+      size_t k = (size_t)key_chunk[f];
+      for (int q = 0; q < nq; ++q) {
+        const int64_t len = chunk_len(sp[q + 1] - sp[q]);
+        for (int64_t r = sp[q]; r < sp[q + 1]; r += len, ++k) {
+          chunk_f[k] = c->grouped ? key_cam[f] : (int32_t)f;
+          chunk_begin[k] = (int32_t)r;
+          chunk_end[k] = (int32_t)std::min<int64_t>(r + len, sp[q + 1]);
+          chunk_q[k] = q;
+        }
+      }
+    }
+  }, 1 << 10);
   key_chunk[nkeys] = (int32_t)chunk_f.size();
   c->nchunks = (int32_t)chunk_f.size();
   std::vector<int32_t> slot_chunk;
@@ -1749,13 +1773,16 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       for (size_t i = 0; i < lists[q].size(); ++i) slot_chunk[i * kXcd + q] = lists[q][i];
   }
   c->nslots = xcd_split ? (int32_t)slot_chunk.size() : c->nchunks;
+  trace.mark("  chunks");
   // Host-side checks are done: the device from here on.
   int ndev = 0;
   HIP_TRY(hipGetDeviceCount(&ndev));
   if (c->device < 0 || c->device >= ndev) return fail(MBA_ERR_DEVICE, "no such HIP device");
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  trace.mark("  stream");
   HIP_TRY(hipHostMalloc((void**)&c->big, 2 * kBigChunk, hipHostMallocDefault));
+  trace.mark("  pinned staging");
   for (hipEvent_t& e : c->big_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   if (!slot_chunk.empty()) MBA_TRY(upload(c, &c->d_slot_chunk, slot_chunk));
   c->rfix = c->mixed && W == 1 && !c->grouped && c->nrows > 0;
