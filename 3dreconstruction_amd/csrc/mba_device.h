@@ -116,25 +116,6 @@ __device__ __forceinline__ bool win_idle(const Win& w, int wi) {
 __device__ __forceinline__ bool win_idle_e(const Win& w, int e) { return w.of_e && win_idle(w, w.of_e[e]); }
 __device__ __forceinline__ bool win_idle_f(const Win& w, int f) { return w.of_f && win_idle(w, w.of_f[f]); }
 
-// acc[stride * window + k] += v for every lane (wi < 0: no contribution).
-// All lanes of the wave must call it.  A wave whose contributing lanes share
-// one window adds its wave sum with a single atomic.
-__device__ __forceinline__ void win_atomic_add(double* acc, int stride, int k, int wi, double v) {
-  const unsigned long long valid = __ballot(wi >= 0);
-  if (valid == 0ull) return;
-  const int leader = __ffsll((long long)valid) - 1;
-  const int wl = __shfl(wi, leader, 64);
-  const int wv = wi < 0 ? wl : wi;
-  if (__all(wv == wl)) {
-    double s = v;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((int)(threadIdx.x & 63) == leader) atomicAdd(acc + (size_t)stride * wl + k, s);
-  } else if (wi >= 0 && v != 0.0) {
-    atomicAdd(acc + (size_t)stride * wi + k, v);
-  }
-}
-
 // Max of a non-negative double via its ordered bit pattern.
 __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
   atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
@@ -198,6 +179,27 @@ __device__ __forceinline__ T wave_sum(T v) {
   v += dpp_mov<0x140>(v);
   return (lane_bcast(v, 0) + lane_bcast(v, 16)) + (lane_bcast(v, 32) + lane_bcast(v, 48));
 }
+
+// acc[stride * window + k] += v for every lane (wi < 0: no contribution).
+// All lanes of the wave must call it.
+__device__ __forceinline__ void win_atomic_add(double* acc, int stride, int k, int wi, double v) {
+  const unsigned long long valid = __ballot(wi >= 0);
+  if (valid == 0ull) return;
+  // Windows are laid out in order, so a wave usually holds one or two: the
+  // first and last valid lanes' windows get a DPP wave sum and one atomic
+  // each; lanes of any window in between add their own value.
+  const int lane = (int)(threadIdx.x & 63);
+  const int lo = __ffsll((long long)valid) - 1, hi = 63 - __clzll((long long)valid);
+  const int w1 = __shfl(wi, lo, 64), w2 = __shfl(wi, hi, 64);
+  const double s1 = wave_sum(wi == w1 ? v : 0.0);
+  if (lane == lo) atomicAdd(acc + (size_t)stride * w1 + k, s1);
+  if (w2 != w1) {
+    const double s2 = wave_sum(wi == w2 ? v : 0.0);
+    if (lane == hi) atomicAdd(acc + (size_t)stride * w2 + k, s2);
+    if (wi >= 0 && wi != w1 && wi != w2 && v != 0.0) atomicAdd(acc + (size_t)stride * wi + k, v);
+  }
+}
+
 
 // Sum over the `lanes` lanes of a point group (aligned, power of two,
 // uniform in the wave; all lanes active), result in every lane of the
