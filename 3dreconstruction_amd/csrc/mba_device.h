@@ -407,14 +407,6 @@ __device__ __forceinline__ void load_proj(const T* __restrict__ tab, CamProj<T>&
   }
 }
 
-// Reciprocal and reciprocal square root: FP32 kernels use the hardware
-// instructions (v_rcp_f32 / v_rsq_f32, about 1 ulp, well inside the FP32
-// path's rounding budget); FP64 keeps IEEE division and sqrt.
-__device__ __forceinline__ float kp_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double kp_rcp(double x) { return 1.0 / x; }
-__device__ __forceinline__ float kp_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ double kp_rsqrt(double x) { return 1.0 / sqrt(x); }
-
 template <typename T>
 struct Proj {
   T rx, ry;          // residual, predicted - observed
@@ -431,7 +423,7 @@ __device__ __forceinline__ void project(const CamProj<T>& c, const T X[3], T ox,
   const T xc = c.R[0] * X[0] + c.R[1] * X[1] + c.R[2] * X[2] + c.t[0];
   const T yc = c.R[3] * X[0] + c.R[4] * X[1] + c.R[5] * X[2] + c.t[1];
   const T zc = c.R[6] * X[0] + c.R[7] * X[1] + c.R[8] * X[2] + c.t[2];
-  const T iz = kp_rcp(zc);
+  const T iz = T(1) / zc;
   const T xe = xc * iz, ye = yc * iz;
   const T r2 = xe * xe + ye * ye;
   T d, fdd;
@@ -474,7 +466,7 @@ __device__ __forceinline__ void project(const CamProj<T>& c, const T X[3], T ox,
 template <typename T>
 __device__ __forceinline__ T huber_weight(T s, T a) {
   if (a > T(0) && s > a * a) {
-    T rho1 = a * kp_rsqrt(s);
+    T rho1 = a / sqrt(s);
     rho1 = rho1 > tiny<T>() ? rho1 : tiny<T>();
     return sqrt(rho1);
   }
@@ -1033,6 +1025,25 @@ __global__ __launch_bounds__(B) void k_cam_pass(
   if (threadIdx.x < 63) partial[(size_t)kPartial * ch + threadIdx.x] = s;
 }
 
+// Sum of entry k over rows [lo, hi) of the chunk partials (kPartial wide),
+// accumulated in A: four independent sums, so four rows' loads are in
+// flight together instead of one dependent load per row.
+template <typename A, typename T>
+__device__ __forceinline__ A sum_partials(const T* __restrict__ partial, int lo, int hi, int k) {
+  A s0 = A(0), s1 = A(0), s2 = A(0), s3 = A(0);
+  int ch = lo;
+  for (; ch + 4 <= hi; ch += 4) {
+    const T p0 = partial[(size_t)kPartial * ch + k], p1 = partial[(size_t)kPartial * (ch + 1) + k];
+    const T p2 = partial[(size_t)kPartial * (ch + 2) + k], p3 = partial[(size_t)kPartial * (ch + 3) + k];
+    s0 += (A)p0;
+    s1 += (A)p1;
+    s2 += (A)p2;
+    s3 += (A)p3;
+  }
+  for (; ch < hi; ++ch) s0 += (A)partial[(size_t)kPartial * ch + k];
+  return (s0 + s1) + (s2 + s3);
+}
+
 // One wave per F block: lanes sum the chunk partials (one value each), lane
 // 0 forms the LM diagonal and the FP64 inverse of U + D^2 (block-Jacobi
 // preconditioner); rhs = h.
@@ -1059,7 +1070,7 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
   __syncthreads();
   double v = 0.0;
   if (k < 63) {
-    for (int ch = f_lo[f]; ch < f_hi[f]; ++ch) v += (double)partial[(size_t)kPartial * ch + k];
+    v = sum_partials<double>(partial, f_lo[f], f_hi[f], k);
     // Partials carry the unscaled camera columns: scale entry (a, b) by
     // s_a s_b, vectors by s_a.
     const int a = ij[k] >> 4, b = ij[k] & 15;
@@ -1186,8 +1197,7 @@ __global__ __launch_bounds__(kBlock) void k_sj_finalize(int nf, const int32_t* _
   int a = 0;
   while (a < 8 && k >= ut9(a + 1, a + 1)) ++a;
   const int b = a + (k - ut9(a, a));
-  double v = 0.0;
-  for (int ch = f_lo[f]; ch < f_hi[f]; ++ch) v += (double)partial[(size_t)kPartial * ch + k];
+  const double v = sum_partials<double>(partial, f_lo[f], f_hi[f], k);
   Tsj[45 * (size_t)f + k] = (T)(v * (double)cscale[9 * (size_t)f + a] * (double)cscale[9 * (size_t)f + b]);
 }
 
@@ -1801,8 +1811,7 @@ __global__ __launch_bounds__(kBlock) void k_mv_finalize_alpha(
   int wi = -1;
   if (blk && i < 9) {
     wi = win_of_f(win, f);
-    T sp = T(0);
-    for (int ch = f_lo[f]; ch < f_hi[f]; ++ch) sp += partial[(size_t)kPartial * ch + i];
+    const T sp = sum_partials<T>(partial, f_lo[f], f_hi[f], i);
     T a = (T)(diag_c[9 * (size_t)f + i] * win_inv_r(win, wi)) * xv[i];
     T yv;
     if constexpr (L == 0) {

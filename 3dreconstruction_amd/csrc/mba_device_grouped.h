@@ -467,8 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_cross_final(int nc, const int32_t* _
   if (c >= nc) return;
   const int f = cam_to_f[c], g = cam_gf[c];
   if (f < 0 || g < 0) return;
-  T s = T(0);
-  for (int ch = f_lo[f]; ch < f_hi[f]; ++ch) s += partial[(size_t)kPartial * ch + k];
+  const T s = sum_partials<T>(partial, f_lo[f], f_hi[f], k);
   cross[36 * (size_t)c + k] = s * cscale[9 * (size_t)f + k / 6] * cscale[9 * (size_t)g + k % 6];
 }
 
