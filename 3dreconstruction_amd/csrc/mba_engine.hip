@@ -2125,7 +2125,9 @@ struct Engine {
   // FP64 cost per window at (cams, pts) into sum slot kSlotCost and its
   // non-finite flags into flag slot 1; cost_result reads them after a sync.
   // slot >= 0 (residual-fix contexts): the residuals also go to slot `slot`
-  // of rres / cres.
+  // of rres / cres.  The flags are clear again when this returns (in stream
+  // order), so what the next Jacobian pass or re-damping flags (a point or
+  // camera block that is not positive definite) reaches the step's readback.
   mba_status cost64_async(const double* cams, const double* pts, int slot = -1) {
     MBA_TRY(clear_flags());
     table_into<double>(cams, c->ctab64);
@@ -2138,7 +2140,8 @@ struct Engine {
                          wr ? c->rres[slot] : nullptr, wr ? c->cres[slot] : nullptr,
                          wr ? (const int32_t*)c->row_cm : nullptr);
     MBA_TRY(sum_async(nb, kSlotCost));
-    return flags_async(1);
+    MBA_TRY(flags_async(1));
+    return clear_flags();
   }
   void cost_result(std::vector<double>* cost, std::vector<uint8_t>* finite) const {
     cost->assign(W, 0.0);
@@ -2846,8 +2849,10 @@ struct Engine {
       // The step, its model cost change, the trial point and its cost are
       // queued together and read after one sync; what a window whose step
       // turns out invalid computed is not used.
+      // Flag slot 0 = failures of the pass or re-damping that built this
+      // system (kept since the last cost evaluation) and of its solve: a
+      // flagged window's step is invalid, as a failed linear solve is.
       HIP_TRY(hipEventRecord(c->tev[0], s));
-      MBA_TRY(clear_flags());
       bool factored = true;
       MBA_TRY(compute_step_async(&factored));
       MBA_TRY(flags_async(0));
