@@ -124,7 +124,7 @@ EXPORTS = [
     "mba_context_set_timing", "mba_context_last_timing", "mba_status_string", "mba_last_error",
     "mba_abi_version", "mba_scene_generate", "mba_scene_generate_grouped",
     "mba_rotation_matrix_to_angle_axis", "mba_angle_axis_to_rotation_matrix", "mba_read_libmv",
-    "mba_read_bal", "mba_write_bal",
+    "mba_read_bal", "mba_write_bal", "mba_evaluate_noalloc",
 ]
 
 
@@ -160,6 +160,8 @@ def load_library():
     lib.mba_evaluate.argtypes = [C.POINTER(_Problem), C.c_int32, C.c_double, _f64p, _f64p, _f64p, _f64p, C.c_int32]
     lib.mba_evaluate_blocks.argtypes = [C.POINTER(_Problem), C.c_int32, C.c_double, _f64p, _f64p, _f64p, _f64p,
                                         _f64p, C.c_int32]
+    lib.mba_evaluate_noalloc.argtypes = [C.c_int32, C.c_int32, C.c_int64] + [C.c_void_p] * 5 + [
+        C.c_int32, C.c_double] + [C.c_void_p] * 4 + [C.c_int32, C.c_void_p]
     lib.mba_scene_generate.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _i64p, _f64p, _f64p, _f64p,
                                        _f64p, _f64p, _i32p, _i32p]
     lib.mba_scene_generate_grouped.argtypes = [C.c_int32, _i32p, _i32p, _i32p, _i64p, _f64p, _f64p, _f64p, _f64p,
@@ -330,6 +332,20 @@ def evaluate(problem: BAProblem, precision=PRECISION_FP64, huber_a=0.0, jacobian
     _check(lib.mba_evaluate(C.byref(s), C.c_int32(precision), C.c_double(huber_a), _p(r), _p(jc), _p(jp),
                             C.byref(cost), C.c_int32(device)))
     return r, jc, jp, cost.value
+
+
+def evaluate_noalloc(num_cameras, num_points, num_observations, cameras, points, observations, camera_index,
+                     point_index, residuals=0, jac_camera=0, jac_point=0, cost=0, precision=PRECISION_FP64,
+                     huber_a=0.0, device=0, stream=0):
+    """mba_evaluate_noalloc: arguments are device addresses (int, 0 = NULL)
+    of caller-owned device buffers; asynchronous on `stream` (a hipStream_t
+    address, 0 = the null stream).  Raises MbaError on a rejected call."""
+    lib = load_library()
+    _check(lib.mba_evaluate_noalloc(int(num_cameras), int(num_points), int(num_observations), cameras or None,
+                                    points or None, observations or None, camera_index or None,
+                                    point_index or None, int(precision), float(huber_a), residuals or None,
+                                    jac_camera or None, jac_point or None, cost or None, int(device),
+                                    stream or None))
 
 
 def evaluate_blocks(problem: BAProblem, precision=PRECISION_FP64, huber_a=0.0, device=0):

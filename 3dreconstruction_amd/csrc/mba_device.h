@@ -2676,36 +2676,68 @@ __global__ __launch_bounds__(kBlock) void k_make_scale(int ne, int nf, const T* 
 }
 
 // CostFunction::Evaluate per observation in input order (no loss, no
-// scaling): residual, row-major 2x9 camera and 2x3 point Jacobians.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_eval_obs(int64_t n, const int32_t* __restrict__ cam_idx,
+// scaling): residual, row-major 2x9 camera and 2x3 point Jacobians, in a
+// grid-stride loop.  Observations and points are read as `In` and rounded
+// once to the kernel precision T.  With cost_partial, each block writes the
+// FP64 sum of 1/2 rho(|r|^2) (HuberLoss with threshold huber_a; <= 0: none)
+// of its observations to cost_partial[blockIdx.x].  With nc > 0, an
+// observation whose camera or point index is outside [0, nc) / [0, np) is
+// never read through: its outputs are NaN.
+template <typename T, typename In = T>
+__global__ __launch_bounds__(kBlock) void k_eval_obs(int64_t n, int nc, int np,
+                                                     const int32_t* __restrict__ cam_idx,
                                                      const int32_t* __restrict__ pt_idx,
-                                                     const T* __restrict__ obs, const T* __restrict__ ctab,
-                                                     const T* __restrict__ pts, double* __restrict__ res,
-                                                     double* __restrict__ jc, double* __restrict__ jp) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const T* tab = ctab + (size_t)kCamTab * cam_idx[i];
-  CamProj<T> c;
-  load_proj(tab, c);
-  const int q = pt_idx[i];
-  const T X[3] = {pts[3 * (size_t)q], pts[3 * (size_t)q + 1], pts[3 * (size_t)q + 2]};
-  Proj<T> o;
-  project(c, X, obs[2 * i], obs[2 * i + 1], o);
-  if (res) {
-    res[2 * i] = (double)o.rx;
-    res[2 * i + 1] = (double)o.ry;
+                                                     const In* __restrict__ obs, const T* __restrict__ ctab,
+                                                     const In* __restrict__ pts, double* __restrict__ res,
+                                                     double* __restrict__ jc, double* __restrict__ jp,
+                                                     double huber_a, double* __restrict__ cost_partial) {
+  __shared__ double lds[kBlock / 64];
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int cam = cam_idx[i], q = pt_idx[i];
+    if (nc > 0 && (cam < 0 || cam >= nc || q < 0 || q >= np)) {
+      const double nan = __builtin_nan("");
+      if (res) res[2 * i] = res[2 * i + 1] = nan;
+      if (jc)
+        for (int k = 0; k < 18; ++k) jc[18 * i + k] = nan;
+      if (jp)
+        for (int k = 0; k < 6; ++k) jp[6 * i + k] = nan;
+      acc[0] += nan;
+      continue;
+    }
+    const T* tab = ctab + (size_t)kCamTab * cam;
+    CamProj<T> c;
+    load_proj(tab, c);
+    const T X[3] = {(T)pts[3 * (size_t)q], (T)pts[3 * (size_t)q + 1], (T)pts[3 * (size_t)q + 2]};
+    Proj<T> o;
+    project(c, X, (T)obs[2 * i], (T)obs[2 * i + 1], o);
+    if (res) {
+      res[2 * i] = (double)o.rx;
+      res[2 * i + 1] = (double)o.ry;
+    }
+    if (cost_partial) {
+      const double rx = (double)o.rx, ry = (double)o.ry;
+      acc[0] += huber_cost(rx * rx + ry * ry, huber_a);
+    }
+    const T one[9] = {T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1)};
+    if (jc) {
+      T J[18];
+      jac_cam(o, tab + kTabDR, X, T(1), one, J);
+      for (int k = 0; k < 18; ++k) jc[18 * i + k] = (double)J[k];
+    }
+    if (jp) {
+      T J[6];
+      jac_point(o, c.R, T(1), one, J);
+      for (int k = 0; k < 6; ++k) jp[6 * i + k] = (double)J[k];
+    }
   }
-  const T one[9] = {T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1)};
-  if (jc) {
-    T J[18];
-    jac_cam(o, tab + kTabDR, X, T(1), one, J);
-    for (int k = 0; k < 18; ++k) jc[18 * i + k] = (double)J[k];
-  }
-  if (jp) {
-    T J[6];
-    jac_point(o, c.R, T(1), one, J);
-    for (int k = 0; k < 6; ++k) jp[6 * i + k] = (double)J[k];
+  if (!cost_partial) return;
+  block_sum<double, 1>(acc, lds);
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w];
+    cost_partial[blockIdx.x] = s;
   }
 }
 

@@ -3423,8 +3423,9 @@ static mba_status evaluate_impl(const mba_problem* p, double huber_a, double* re
         hipLaunchKernelGGL(k_eval_obs_g<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ci, pi, obs, tab, pts, res, ji,
                            jc, jp);
       else
-        hipLaunchKernelGGL(k_eval_obs<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ci, pi, obs, tab, pts, res, jc,
-                           jp);
+        hipLaunchKernelGGL(k_eval_obs<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, 0, 0, (const int32_t*)ci,
+                           (const int32_t*)pi, (const T*)obs, (const T*)tab, (const T*)pts, res, jc, jp, 0.0,
+                           (double*)nullptr);
     }
     std::vector<double> hr(2 * (size_t)n);
     if (n) HIP_TRY(hipMemcpyAsync(hr.data(), res, hr.size() * 8, hipMemcpyDeviceToHost, s));
@@ -3612,7 +3613,93 @@ mba_status write_lsqp(mba_context* c, int k, const LsqpState& S) {
   return MBA_OK;
 }
 
+// ------------------------------------------ evaluation without allocation
+// Device scratch of mba_evaluate_noalloc: the camera table (kCamTab values
+// of the kernel precision per camera) followed by the block partials of the
+// cost.  Each device holds its own copy until the process ends; every call
+// on a device shares it, so calls that overlap on one device race.
+constexpr int kNoAllocMaxCameras = 16384;
+constexpr int kNoAllocBlocks = 2048;  // grid of the evaluation kernel
+constexpr size_t kNoAllocTable = (size_t)kCamTab * kNoAllocMaxCameras;  // doubles
+static __device__ __align__(256) double scratch[kNoAllocTable + kNoAllocBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device `dev` (looked up
+// once per device).  Host code uses this address, never the array's name.
+mba_status scratch_address(int dev, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(MBA_ERR_DEVICE, "no device address for the evaluation scratch");
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MBA_OK;
+}
+
+template <typename T>
+mba_status evaluate_noalloc_launch(double* scr, int32_t nc, int32_t np, int64_t n, const double* cams,
+                                   const double* pts, const double* obs, const int32_t* ci, const int32_t* pi,
+                                   double huber_a, double* res, double* jc, double* jp, double* cost,
+                                   hipStream_t s) {
+  T* tab = reinterpret_cast<T*>(scr);
+  double* partial = scr + kNoAllocTable;
+  if (nc > 0) hipLaunchKernelGGL(k_cam_table<T>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cams, tab);
+  const int nb = (int)std::min<int64_t>(kNoAllocBlocks, std::max<int64_t>(1, (n + kBlock - 1) / kBlock));
+  if (n > 0 && (res || jc || jp || cost))
+    hipLaunchKernelGGL((k_eval_obs<T, double>), dim3(nb), dim3(kBlock), 0, s, n, (int)nc, (int)np, ci, pi, obs,
+                       (const T*)tab, pts, res, jc, jp, huber_a, cost ? partial : nullptr);
+  if (cost) {
+    if (n > 0) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, (const double*)partial, nb, cost);
+    else hipLaunchKernelGGL(k_fill64, dim3(1), dim3(kBlock), 0, s, (int64_t)1, 0.0, cost);
+  }
+  HIP_TRY(hipGetLastError());
+  return MBA_OK;
+}
+}  // namespace
+
 extern "C" {
+
+mba_status mba_evaluate_noalloc(int32_t num_cameras, int32_t num_points, int64_t num_observations,
+                                const double* cameras, const double* points, const double* observations,
+                                const int32_t* camera_index, const int32_t* point_index, int32_t precision,
+                                double huber_a, double* residuals, double* jac_camera, double* jac_point,
+                                double* cost, int32_t device, void* stream) {
+  g_last_error.clear();
+  if (num_cameras < 0 || num_points < 0 || num_observations < 0)
+    return fail(MBA_ERR_INVALID_ARGUMENT, "negative problem size");
+  if (num_cameras > kNoAllocMaxCameras)
+    return fail(MBA_ERR_INVALID_ARGUMENT, "mba_evaluate_noalloc: " + std::to_string(num_cameras) +
+                                              " cameras exceed its scratch limit of " +
+                                              std::to_string(kNoAllocMaxCameras));
+  if (precision != MBA_PRECISION_FP64 && precision != MBA_PRECISION_MIXED)
+    return fail(MBA_ERR_INVALID_ARGUMENT, "unknown precision");
+  if ((num_cameras > 0 && !cameras) || (num_points > 0 && !points))
+    return fail(MBA_ERR_INVALID_ARGUMENT, "NULL camera or point array");
+  if (num_observations > 0 && (!observations || !camera_index || !point_index))
+    return fail(MBA_ERR_INVALID_ARGUMENT, "NULL observation or index array");
+  if (num_observations > 0 && (num_cameras == 0 || num_points == 0))
+    return fail(MBA_ERR_INVALID_ARGUMENT, "observations without cameras or points");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev || device >= kMaxDevices) return fail(MBA_ERR_DEVICE, "no such HIP device");
+  HIP_TRY(hipSetDevice(device));
+  double* scr = nullptr;
+  MBA_TRY(scratch_address(device, &scr));
+  const hipStream_t s = (hipStream_t)stream;
+  if (precision == MBA_PRECISION_MIXED)
+    return evaluate_noalloc_launch<float>(scr, num_cameras, num_points, num_observations, cameras, points,
+                                          observations, camera_index, point_index, huber_a, residuals, jac_camera,
+                                          jac_point, cost, s);
+  return evaluate_noalloc_launch<double>(scr, num_cameras, num_points, num_observations, cameras, points,
+                                         observations, camera_index, point_index, huber_a, residuals, jac_camera,
+                                         jac_point, cost, s);
+}
 
 mba_status mba_evaluate(const mba_problem* p, int32_t precision, double huber_a, double* residuals,
                         double* jac_camera, double* jac_point, double* cost, int32_t device) {

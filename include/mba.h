@@ -291,6 +291,29 @@ mba_status mba_evaluate_blocks(const mba_problem* problem, int32_t precision,
                                double* jac_intrinsics, double* jac_camera,
                                double* jac_point, double* cost, int32_t device);
 
+/* mba_evaluate for a PACKED9 problem whose arrays are already in device
+ * memory, with no allocation of any kind (no counterpart in the reference:
+ * for callers that cannot allocate, such as code under HIP graph capture or
+ * a tight serving loop).  Every pointer is device memory on `device`:
+ * cameras 9 per camera, points 3 per point, observations 2 per
+ * observation, camera_index / point_index per observation; residuals (2),
+ * jac_camera (row-major 2x9), jac_point (row-major 2x3) per observation,
+ * and cost (one double: the FP64 sum of 1/2 rho(|r|^2) with HuberLoss
+ * threshold huber_a, <= 0 no loss); each output may be NULL.  Values are
+ * those of mba_evaluate in the same precision.  Asynchronous on `stream`
+ * (a hipStream_t; NULL is the null stream): the outputs are valid once the
+ * stream reaches this point.  Intermediates live in a fixed device scratch
+ * array of the library (one per device, held until the process ends), so
+ * at most 16384 cameras (more: MBA_ERR_INVALID_ARGUMENT); calls that
+ * overlap on one device race on it.  An observation whose camera or point
+ * index is out of range gets NaN outputs (and makes the cost NaN); nothing
+ * is read out of bounds. */
+mba_status mba_evaluate_noalloc(int32_t num_cameras, int32_t num_points, int64_t num_observations,
+                                const double* cameras, const double* points, const double* observations,
+                                const int32_t* camera_index, const int32_t* point_index, int32_t precision,
+                                double huber_a, double* residuals, double* jac_camera, double* jac_point,
+                                double* cost, int32_t device, void* stream);
+
 /* The bit-exact structure the reference's preprocessor builds
  * (reorder_program.cc:436-531, schur_complement_solver.cc:246-313):
  *   param_order: column blocks in order (PACKED9: cameras 0..Nc-1, points
