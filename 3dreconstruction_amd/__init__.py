@@ -661,6 +661,7 @@ class Problem:
         self._cam_id, self._pt_id, self._intr_id = {}, {}, {}
         self._obs, self._ci, self._pi, self._ii = [], [], [], []
         self._loss = None
+        self._unrobust = False  # some block was added with loss_function None
         self._constant, self._intr_constant = set(), set()
         self._subset = {}
         self._layout = None
@@ -683,7 +684,17 @@ class Problem:
         if self._layout is not None and layout != self._layout:
             raise ValueError("all residual blocks of a problem use one camera layout")
         self._layout = layout
-        if loss_function is not None:
+        # The engine applies one loss to every observation (libs/sfm shares one
+        # HuberLoss instance across its blocks); a problem mixing robustified
+        # and plain blocks, or two thresholds, is refused rather than solved
+        # with a loss some blocks did not ask for.
+        if loss_function is None:
+            if self._loss is not None:
+                raise ValueError("residual blocks with and without a loss function cannot be mixed")
+            self._unrobust = True
+        else:
+            if self._unrobust:
+                raise ValueError("residual blocks with and without a loss function cannot be mixed")
             if self._loss is not None and loss_function is not self._loss and loss_function.a != self._loss.a:
                 raise ValueError("all residual blocks must share one loss function")
             self._loss = loss_function

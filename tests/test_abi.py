@@ -229,6 +229,22 @@ def test_ceres_shaped_problem_builds_the_same_arrays(mba, c1):
     np.testing.assert_array_equal(arr.observations, c1["obs"][:200])
 
 
+@pytest.mark.parametrize("first_robust", [True, False])
+def test_ceres_shaped_problem_refuses_mixed_loss(mba, c1, first_robust):
+    """The engine applies one loss to all observations: a block without a loss
+    next to one with HuberLoss (in either order) is refused, not silently
+    robustified; two blocks sharing one loss, or both plain, are fine."""
+    cams = [c1["cam_init"][i].copy() for i in range(2)]
+    pts = [c1["pt_init"][i].copy() for i in range(2)]
+    loss = mba.HuberLoss(4.0)
+    prob = mba.Problem()
+    prob.AddResidualBlock(mba.ReprojectionError(c1["obs"][0]), loss if first_robust else None, cams[0], pts[0])
+    with pytest.raises(ValueError, match="with and without a loss"):
+        prob.AddResidualBlock(mba.ReprojectionError(c1["obs"][1]), None if first_robust else loss, cams[1], pts[1])
+    prob.AddResidualBlock(mba.ReprojectionError(c1["obs"][1]), loss if first_robust else None, cams[1], pts[1])
+    assert prob.NumResidualBlocks() == 2
+
+
 def test_ceres_shaped_grouped_problem_builds_the_same_arrays(mba, group_scene):
     """The incremental engine's calls (sfm_incremental_engine.cpp:1396-1446):
     AddResidualBlock(intrinsics, extrinsics, point), SetParameterization with
