@@ -124,7 +124,7 @@ EXPORTS = [
     "mba_context_set_timing", "mba_context_last_timing", "mba_status_string", "mba_last_error",
     "mba_abi_version", "mba_scene_generate", "mba_scene_generate_grouped",
     "mba_rotation_matrix_to_angle_axis", "mba_angle_axis_to_rotation_matrix", "mba_read_libmv",
-    "mba_read_bal", "mba_write_bal", "mba_evaluate_noalloc",
+    "mba_read_bal", "mba_write_bal", "mba_evaluate_noalloc", "mba_context_lm_pass",
 ]
 
 
@@ -149,6 +149,7 @@ def load_library():
     for name in ["mba_context_destroy", "mba_context_synchronize"]:
         getattr(lib, name).argtypes = [C.c_void_p]
     lib.mba_context_jacobian_schur.argtypes = [C.c_void_p, C.c_double]
+    lib.mba_context_lm_pass.argtypes = [C.c_void_p, C.c_double, _f64p]
     lib.mba_context_set_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.mba_context_last_timing.argtypes = [C.c_void_p, _f64p, C.c_int32]
     lib.mba_context_read_schur.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p, _f64p]
@@ -418,6 +419,13 @@ class Context:
 
     def jacobian_schur(self, radius: float):
         _check(self._lib.mba_context_jacobian_schur(self.handle, C.c_double(radius)))
+
+    def lm_pass(self, radius: float) -> float:
+        """The pass as the LM loop runs it (FP64 cost first; a mixed context's
+        FP32 kernels take those residuals); returns the FP64 cost."""
+        cost = C.c_double()
+        _check(self._lib.mba_context_lm_pass(self.handle, C.c_double(radius), C.byref(cost)))
+        return cost.value
 
     def synchronize(self):
         _check(self._lib.mba_context_synchronize(self.handle))

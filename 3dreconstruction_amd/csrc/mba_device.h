@@ -793,9 +793,9 @@ __global__ __launch_bounds__(kBlock) void k_grad_final(const double* __restrict_
 // V = sum Jp'Jp, g_p = sum Jp' r; LM diagonal of points; Vinv = (V + D^2)^-1
 // in FP64; w = Vinv g_p.
 template <typename T, int L>
-__device__ __forceinline__ void point_row_accumulate(const T* __restrict__ ctab, int cam, T ox, T oy,
-                                                     const T* __restrict__ rf, const T X[3], const T sp[3],
-                                                     T huber_a, T v[6], T g[3]) {
+__device__ __forceinline__ T point_row_accumulate(const T* __restrict__ ctab, int cam, T ox, T oy,
+                                                  const T* __restrict__ rf, const T X[3], const T sp[3],
+                                                  T huber_a, T v[6], T g[3]) {
   CamProj<T> c;
   load_proj<T, L>(ctab + (size_t)kCamTab * cam, c);
   Proj<T> o;
@@ -817,6 +817,7 @@ __device__ __forceinline__ void point_row_accumulate(const T* __restrict__ ctab,
   fma2(g[0], Jp[0], rx, Jp[3], ry);
   fma2(g[1], Jp[1], rx, Jp[4], ry);
   fma2(g[2], Jp[2], rx, Jp[5], ry);
+  return wgt;
 }
 
 template <typename T, int L>
@@ -825,7 +826,7 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
     const T* __restrict__ row_obs, const double* __restrict__ pts64, const T* __restrict__ pscale,
     const T* __restrict__ ctab, T huber_a, Win win, double min_diag, double max_diag,
     double* __restrict__ diag_p, int recompute_diag, T* __restrict__ V, T* __restrict__ gp,
-    T* __restrict__ Vinv, T* __restrict__ pk, const T* __restrict__ rres) {
+    T* __restrict__ Vinv, T* __restrict__ pk, const T* __restrict__ rres, T* __restrict__ wrow) {
   SegPos q = seg_locate(sg);
   if (q.valid && win_idle_e(win, q.e)) q.valid = false;
   const int e = q.valid ? q.e : sg.e0[0];
@@ -844,20 +845,26 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
       const int c0 = row_cam[r], c1 = row_cam[r + ln];
       const T o0x = ob[2 * (size_t)r], o0y = ob[2 * (size_t)r + 1];
       const T o1x = ob[2 * (size_t)(r + ln)], o1y = ob[2 * (size_t)(r + ln) + 1];
+      T w0, w1;
       if (rres) {
         const T f0[2] = {o0x, o0y}, f1[2] = {o1x, o1y};
-        point_row_accumulate<T, L>(ctab, c0, T(0), T(0), f0, X, sp, huber_a, v, g);
-        point_row_accumulate<T, L>(ctab, c1, T(0), T(0), f1, X, sp, huber_a, v, g);
+        w0 = point_row_accumulate<T, L>(ctab, c0, T(0), T(0), f0, X, sp, huber_a, v, g);
+        w1 = point_row_accumulate<T, L>(ctab, c1, T(0), T(0), f1, X, sp, huber_a, v, g);
       } else {
-        point_row_accumulate<T, L>(ctab, c0, o0x, o0y, nullptr, X, sp, huber_a, v, g);
-        point_row_accumulate<T, L>(ctab, c1, o1x, o1y, nullptr, X, sp, huber_a, v, g);
+        w0 = point_row_accumulate<T, L>(ctab, c0, o0x, o0y, nullptr, X, sp, huber_a, v, g);
+        w1 = point_row_accumulate<T, L>(ctab, c1, o1x, o1y, nullptr, X, sp, huber_a, v, g);
+      }
+      if (wrow) {
+        wrow[r] = w0;
+        wrow[r + ln] = w1;
       }
     }
     if (r < r1) {
       const T ox = ob[2 * (size_t)r], oy = ob[2 * (size_t)r + 1];
       const T f0[2] = {ox, oy};
-      point_row_accumulate<T, L>(ctab, row_cam[r], rres ? T(0) : ox, rres ? T(0) : oy, rres ? f0 : nullptr, X, sp,
-                                 huber_a, v, g);
+      const T w0 = point_row_accumulate<T, L>(ctab, row_cam[r], rres ? T(0) : ox, rres ? T(0) : oy,
+                                              rres ? f0 : nullptr, X, sp, huber_a, v, g);
+      if (wrow) wrow[r] = w0;
     }
   }
 #pragma unroll
@@ -935,9 +942,9 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restri
 // Accumulate one camera-major row into U (45), g (9), h (9), with the
 // camera Jacobi scale left out (applied per camera in k_cam_finalize).
 template <typename T>
-__device__ __forceinline__ void cam_row_accumulate(const CamProj<T>& c, const T dR[27], const T X[3],
-                                                   const T sw[3], T ox, T oy, bool given, T huber_a,
-                                                   int with_h, T acc[63]) {
+__device__ __forceinline__ T cam_row_accumulate(const CamProj<T>& c, const T dR[27], const T X[3],
+                                                const T sw[3], T ox, T oy, bool given, T huber_a,
+                                                int with_h, T acc[63]) {
   const T one[9] = {T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1)};
   Proj<T> o;
   project(c, X, given ? T(0) : ox, given ? T(0) : oy, o);
@@ -966,6 +973,7 @@ __device__ __forceinline__ void cam_row_accumulate(const CamProj<T>& c, const T 
     fma2(acc[45 + a], Jc[a], rx, Jc[9 + a], ry);
     fma2(acc[54 + a], Jc[a], tx, Jc[9 + a], ty);
   }
+  return wgt;
 }
 
 // Camera-major launches: workgroup slot -> chunk (-1: empty slot), or the
@@ -983,7 +991,7 @@ __global__ __launch_bounds__(B) void k_cam_pass(
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
     const T* __restrict__ ctab, T huber_a, int with_h, T* __restrict__ partial, Win win,
-    const int32_t* __restrict__ slot_chunk, const T* __restrict__ cres) {
+    const int32_t* __restrict__ slot_chunk, const T* __restrict__ cres, T* __restrict__ wcm) {
   __shared__ T lds[63 * (B / 64)];
   const int ch = chunk_of_slot(slot_chunk);
   if (ch < 0) return;
@@ -1013,13 +1021,19 @@ __global__ __launch_bounds__(B) void k_cam_pass(
     T X0[3], s0[3], X1[3], s1[3];
     load_pk(pk, e0, X0, s0);
     load_pk(pk, e1, X1, s1);
-    cam_row_accumulate(c, dR, X0, s0, o0x, o0y, given, huber_a, with_h, acc);
-    cam_row_accumulate(c, dR, X1, s1, o1x, o1y, given, huber_a, with_h, acc);
+    const T w0 = cam_row_accumulate(c, dR, X0, s0, o0x, o0y, given, huber_a, with_h, acc);
+    const T w1 = cam_row_accumulate(c, dR, X1, s1, o1x, o1y, given, huber_a, with_h, acc);
+    if (wcm) {
+      wcm[i] = w0;
+      wcm[i + B] = w1;
+    }
   }
   if (i < end) {
     T X0[3], s0[3];
     load_pk(pk, cm_e[i], X0, s0);
-    cam_row_accumulate(c, dR, X0, s0, ob[2 * (size_t)i], ob[2 * (size_t)i + 1], given, huber_a, with_h, acc);
+    const T w0 =
+        cam_row_accumulate(c, dR, X0, s0, ob[2 * (size_t)i], ob[2 * (size_t)i + 1], given, huber_a, with_h, acc);
+    if (wcm) wcm[i] = w0;
   }
   const T s = block_reduce<T, 63, B>(acc, lds);
   if (threadIdx.x < 63) partial[(size_t)kPartial * ch + threadIdx.x] = s;
@@ -1053,7 +1067,7 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
     int nf, const int32_t* __restrict__ f_lo, const int32_t* __restrict__ f_hi, const T* __restrict__ partial,
     const T* __restrict__ cscale, Win win, double min_diag, double max_diag,
     double* __restrict__ diag_c, int recompute_diag, int with_h, T* __restrict__ U, T* __restrict__ gc,
-    T* __restrict__ rhs, T* __restrict__ Minv, const T* __restrict__ Tsj) {
+    T* __restrict__ rhs, T* __restrict__ Minv, const double* __restrict__ Tsj) {
   __shared__ double sum[64];
   __shared__ int ij[64];
   const int f = blockIdx.x;
@@ -1087,25 +1101,22 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
   if (k >= 9) return;
   const int wi = win_of_f(win, f);
   const double inv_radius = win_inv_r(win, wi);
+  // The block to invert: U, or with SCHUR_JACOBI the diagonal block of S
+  // (Tsj); plus the LM diagonal, taken from the Jacobian's columns (U's
+  // diagonal) in either case.
   double a[45];
 #pragma unroll
-  for (int q = 0; q < 45; ++q) a[q] = sum[q];
+  for (int q = 0; q < 45; ++q) a[q] = Tsj ? Tsj[45 * (size_t)f + q] : sum[q];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     double dg;
     if (recompute_diag) {
-      // D from the Jacobian's columns (U's diagonal), before any
-      // SCHUR_JACOBI term.
-      dg = fmin(fmax(a[ut9(i, i)], min_diag), max_diag);
+      dg = fmin(fmax(sum[ut9(i, i)], min_diag), max_diag);
       if (k == 0) diag_c[9 * (size_t)f + i] = dg;
     } else {
       dg = diag_c[9 * (size_t)f + i];
     }
     a[ut9(i, i)] += dg * inv_radius;
-  }
-  if (Tsj) {
-#pragma unroll
-    for (int q = 0; q < 45; ++q) a[q] -= (double)Tsj[45 * (size_t)f + q];
   }
   double l[45], e[9];
   if (chol9(a, l)) {
@@ -1120,17 +1131,18 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
 }
 
 // SCHUR_JACOBI (schur_jacobi_preconditioner.cc:83-103): per camera-major
-// chunk, the sum over its rows of Jc' (Jp Vinv Jp') Jc (45 values; camera
-// columns unscaled like the camera pass, point columns scaled, Huber weight
-// applied) -- each row's share of W Vinv W' in the camera's diagonal block
-// of S when a camera observes a point once.
+// chunk, the sum over its rows of Jc' (I - Jp Vinv Jp') Jc (45 values;
+// camera columns unscaled like the camera pass, point columns scaled, loss
+// weight applied) -- each row's share of U - W Vinv W', the camera's
+// diagonal block of S without its LM diagonal, when a camera observes a
+// point once.
 template <typename T, int B>
 __global__ __launch_bounds__(B) void k_sj_pass(
     const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
     const T* __restrict__ pscale, const T* __restrict__ Vinv, const T* __restrict__ ctab, T huber_a,
-    T* __restrict__ partial, Win win, const int32_t* __restrict__ slot_chunk) {
+    T* __restrict__ partial, Win win, const int32_t* __restrict__ slot_chunk, const T* __restrict__ wcm) {
   __shared__ T lds[45 * (B / 64)];
   const int ch = chunk_of_slot(slot_chunk);
   if (ch < 0) return;
@@ -1155,11 +1167,13 @@ __global__ __launch_bounds__(B) void k_sj_pass(
     const T Vi[9] = {iv[0], iv[1], iv[2], iv[1], iv[3], iv[4], iv[2], iv[4], iv[5]};
     Proj<T> o;
     project(c, X, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], o);
-    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    const T wgt = wcm ? wcm[i] : huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
     T Jc[18], Jp[6];
     jac_cam(o, dR, X, wgt, one, Jc);
     jac_point(o, c.R, wgt, sp, Jp);
-    // A = Jp Vinv Jp' (2 x 2), then B = A Jc.
+    // A = Jp Vinv Jp' (2 x 2), then B = (I - A) Jc: the row's share of the
+    // diagonal block, Jc'Jc - Jc'A Jc, formed per row so that the FP32
+    // sum carries no cancellation between two large sums.
     T G[6];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
@@ -1172,8 +1186,8 @@ __global__ __launch_bounds__(B) void k_sj_pass(
     T B0[9], B1[9];
 #pragma unroll
     for (int b = 0; b < 9; ++b) {
-      B0[b] = a00 * Jc[b] + a01 * Jc[9 + b];
-      B1[b] = a01 * Jc[b] + a11 * Jc[9 + b];
+      B0[b] = (T(1) - a00) * Jc[b] - a01 * Jc[9 + b];
+      B1[b] = (T(1) - a11) * Jc[9 + b] - a01 * Jc[b];
     }
     int k = 0;
 #pragma unroll
@@ -1185,12 +1199,14 @@ __global__ __launch_bounds__(B) void k_sj_pass(
   if (threadIdx.x < 45) partial[(size_t)kPartial * ch + threadIdx.x] = sum;
 }
 
-// Tsj_f = s_a s_b (sum of the F block's SCHUR_JACOBI partials), 45 per block.
+// Tsj_f = s_a s_b (sum of the F block's SCHUR_JACOBI partials), 45 per block,
+// kept in FP64: the block is factored in FP64 and can be ill-conditioned
+// enough that an FP32 rounding of it is no longer positive definite.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sj_finalize(int nf, const int32_t* __restrict__ f_lo,
                                                         const int32_t* __restrict__ f_hi,
                                                         const T* __restrict__ partial, const T* __restrict__ cscale,
-                                                        Win win, T* __restrict__ Tsj) {
+                                                        Win win, double* __restrict__ Tsj) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int f = (int)(t / 45), k = (int)(t % 45);
   if (f >= nf || win_idle_f(win, f)) return;
@@ -1198,21 +1214,21 @@ __global__ __launch_bounds__(kBlock) void k_sj_finalize(int nf, const int32_t* _
   while (a < 8 && k >= ut9(a + 1, a + 1)) ++a;
   const int b = a + (k - ut9(a, a));
   const double v = sum_partials<double>(partial, f_lo[f], f_hi[f], k);
-  Tsj[45 * (size_t)f + k] = (T)(v * (double)cscale[9 * (size_t)f + a] * (double)cscale[9 * (size_t)f + b]);
+  Tsj[45 * (size_t)f + k] = v * (double)cscale[9 * (size_t)f + a] * (double)cscale[9 * (size_t)f + b];
 }
 
 // Re-damp the camera preconditioner after a rejected step.
 template <typename T>
 __global__ __launch_bounds__(64) void k_cam_damp(int nf, const T* __restrict__ U,
                                                  const double* __restrict__ diag_c, Win win,
-                                                 T* __restrict__ Minv, const T* __restrict__ Tsj) {
+                                                 T* __restrict__ Minv, const double* __restrict__ Tsj) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf || win_idle_f(win, f)) return;
   const int wi = win_of_f(win, f);
   const double inv_radius = win_inv_r(win, wi);
   double a[45];
 #pragma unroll
-  for (int q = 0; q < 45; ++q) a[q] = (double)U[45 * (size_t)f + q] - (Tsj ? (double)Tsj[45 * (size_t)f + q] : 0.0);
+  for (int q = 0; q < 45; ++q) a[q] = Tsj ? Tsj[45 * (size_t)f + q] : (double)U[45 * (size_t)f + q];
 #pragma unroll
   for (int i = 0; i < 9; ++i) a[ut9(i, i)] += diag_c[9 * (size_t)f + i] * inv_radius;
   double l[45];
@@ -1268,7 +1284,7 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
     const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
     const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ Vinv, T huber_a,
     T beta, int apply_vinv, T* __restrict__ zout, T* __restrict__ pk, const int32_t* __restrict__ all_done,
-    Win win) {
+    Win win, const T* __restrict__ wrow, const T* __restrict__ rres) {
   if (all_done && *all_done) return;
   SegPos qp = seg_locate(sg);
   if (qp.valid && win_idle_e(win, qp.e)) qp.valid = false;
@@ -1277,12 +1293,18 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
   load_pk(pkX, e, X, unused);
   const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
   T q[3] = {0, 0, 0};
-  auto row = [&](int cam, T ox, T oy) {
+  // wrow (optional): the row's loss weight from the last pass; rres
+  // (optional): its residual evaluated in FP64 (beta != 0 only).
+  auto row = [&](int r, int cam, T ox, T oy) {
     CamProj<T> c;
     load_proj<T, L>(ctab + (size_t)kCamTab * cam, c);
     Proj<T> o;
     project<T, L>(c, X, ox, oy, o);
-    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    if (rres) {
+      o.rx = rres[2 * (size_t)r];
+      o.ry = rres[2 * (size_t)r + 1];
+    }
+    const T wgt = wrow ? wrow[r] : huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
     T Jp[6];
     jac_point(o, c.R, wgt, sp, Jp);
     T v[2];
@@ -1301,10 +1323,10 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
       const int c0 = row_cam[r], c1 = row_cam[r + ln];
       const T o0x = row_obs[2 * (size_t)r], o0y = row_obs[2 * (size_t)r + 1];
       const T o1x = row_obs[2 * (size_t)(r + ln)], o1y = row_obs[2 * (size_t)(r + ln) + 1];
-      row(c0, o0x, o0y);
-      row(c1, o1x, o1y);
+      row(r, c0, o0x, o0y);
+      row(r + ln, c1, o1x, o1y);
     }
-    if (r < r1) row(row_cam[r], row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1]);
+    if (r < r1) row(r, row_cam[r], row_obs[2 * (size_t)r], row_obs[2 * (size_t)r + 1]);
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i) q[i] = group_sum(q[i], qp.lanes);
@@ -1330,7 +1352,8 @@ __global__ __launch_bounds__(B) void k_mv_cam(
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
     const T* __restrict__ cscale, const T* __restrict__ ctab, T huber_a, T* __restrict__ partial,
-    const int32_t* __restrict__ all_done, Win win, const int32_t* __restrict__ slot_chunk) {
+    const int32_t* __restrict__ all_done, Win win, const int32_t* __restrict__ slot_chunk,
+    const T* __restrict__ wcm) {
   __shared__ T lds[9 * (B / 64)];
   if (all_done && *all_done) return;
   const int ch = chunk_of_slot(slot_chunk);
@@ -1350,10 +1373,11 @@ __global__ __launch_bounds__(B) void k_mv_cam(
 #pragma unroll
   for (int i = 0; i < 9; ++i) acc[i] = T(0);
   const int end = chunk_end[ch];
-  auto row = [&](const T X[3], const T sz[3], T ox, T oy) {
+  // wcm (optional): the row's loss weight from the last pass.
+  auto row = [&](int i, const T X[3], const T sz[3], T ox, T oy) {
     Proj<T> o;
     project(c, X, ox, oy, o);
-    const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+    const T wgt = wcm ? wcm[i] : huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
     T Jc[18];
     jac_cam(o, dR, X, wgt, cs, Jc);
     T u[2];
@@ -1372,13 +1396,13 @@ __global__ __launch_bounds__(B) void k_mv_cam(
     T X0[3], s0[3], X1[3], s1[3];
     load_pk(pk, e0, X0, s0);
     load_pk(pk, e1, X1, s1);
-    row(X0, s0, o0x, o0y);
-    row(X1, s1, o1x, o1y);
+    row(i, X0, s0, o0x, o0y);
+    row(i + B, X1, s1, o1x, o1y);
   }
   if (i < end) {
     T X0[3], s0[3];
     load_pk(pk, cm_e[i], X0, s0);
-    row(X0, s0, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1]);
+    row(i, X0, s0, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1]);
   }
   const T s = block_reduce<T, 9, B>(acc, lds);
   if (threadIdx.x < 9) partial[(size_t)kPartial * ch + threadIdx.x] = s;

@@ -875,7 +875,13 @@ struct Dev {
   T *LinvT = nullptr, *tb = nullptr, *ty = nullptr;
   T* s_scratch = nullptr;
   // SCHUR_JACOBI preconditioner terms (IMPLICIT_SCHUR_PCG only)
-  T *partial_sj = nullptr, *Tsj = nullptr;
+  T* partial_sj = nullptr;
+  double* Tsj = nullptr;
+  // Packed layout with a loss function: each row's loss weight sqrt(rho')
+  // as the last Jacobian pass formed it, in row order (point pass) and
+  // camera-major order (camera pass); the matvecs and the SCHUR_JACOBI pass
+  // read them instead of re-deriving them from kernel-precision residuals.
+  T *wrow = nullptr, *wcm = nullptr;
 };
 
 constexpr size_t kBigChunk = (size_t)8 << 20;  // pinned staging buffers for large uploads (two)
@@ -1134,6 +1140,10 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_st
   if (c->opt.preconditioner == MBA_PRECOND_SCHUR_JACOBI && c->opt.linear_solver == MBA_IMPLICIT_SCHUR_PCG) {
     MBA_TRY(c->pool.alloc(&d.partial_sj, (size_t)kPartial * std::max(1, c->nchunks)));
     MBA_TRY(c->pool.alloc(&d.Tsj, 45 * std::max<size_t>(1, nf)));
+  }
+  if (c->opt.huber_a > 0.0 && !c->grouped && c->nrows > 0) {
+    MBA_TRY(c->pool.alloc(&d.wrow, (size_t)c->nrows));
+    MBA_TRY(c->pool.alloc(&d.wcm, (size_t)std::max(1, c->ncm)));
   }
   return MBA_OK;
 }
@@ -2181,7 +2191,7 @@ struct Engine {
       hipLaunchKernelGGL((c->grouped ? k_point_pass<T, 1> : k_point_pass<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0,
                          s, c->seg, d.pt_start, d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, win(),
                          c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag,
-                         d.V, d.gp, d.Vinv, d.pk, fixed_rows());
+                         d.V, d.gp, d.Vinv, d.pk, fixed_rows(), c->grouped ? nullptr : d.wrow);
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
     if (c->nchunks > 0) {
       if (c->grouped) {
@@ -2192,14 +2202,14 @@ struct Engine {
         hipLaunchKernelGGL((c->d_slot_chunk ? k_cam_pass<T, 64> : k_cam_pass<T, kBlock>), dim3(c->nslots),
                            dim3(c->d_slot_chunk ? 64 : kBlock), 0, s, d.chunk_f, d.chunk_begin,
                            d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial, win(),
-                           (const int32_t*)c->d_slot_chunk, fixed_cm());
+                           (const int32_t*)c->d_slot_chunk, fixed_cm(), d.wcm);
     }
     sj_terms();
     if (c->timing) (void)hipEventRecord(c->ev[2], s);
     if (c->nf > 0)
       hipLaunchKernelGGL(k_cam_finalize<T>, dim3(c->nf), dim3(64), 0, s, c->nf, d.f_lo, d.f_hi, d.partial,
                          d.cscale, win(), c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_c,
-                         recompute_diag, 1, d.U, d.gc, d.rhs, d.Minv, (const T*)d.Tsj);
+                         recompute_diag, 1, d.U, d.gc, d.rhs, d.Minv, (const double*)d.Tsj);
     if (c->timing) (void)hipEventRecord(c->ev[3], s);
   }
 
@@ -2211,7 +2221,7 @@ struct Engine {
       hipLaunchKernelGGL((c->d_slot_chunk ? k_sj_pass<T, 64> : k_sj_pass<T, kBlock>), dim3(c->nslots),
                          dim3(c->d_slot_chunk ? 64 : kBlock), 0, s, d.chunk_f, d.chunk_begin, d.chunk_end,
                          d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.pscale, d.Vinv, d.ctab, huber, d.partial_sj, win(),
-                         (const int32_t*)c->d_slot_chunk);
+                         (const int32_t*)c->d_slot_chunk, (const T*)d.wcm);
     hipLaunchKernelGGL(k_sj_finalize<T>, dim3(nblocks(45 * (int64_t)c->nf)), dim3(kBlock), 0, s, c->nf, d.f_lo,
                        d.f_hi, (const T*)d.partial_sj, (const T*)d.cscale, win(), d.Tsj);
   }
@@ -2223,7 +2233,7 @@ struct Engine {
     if (c->nf > 0) {
       sj_terms();
       hipLaunchKernelGGL(k_cam_damp<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf, d.U, c->diag_c, win(),
-                         d.Minv, (const T*)d.Tsj);
+                         d.Minv, (const double*)d.Tsj);
       mv_cam(nullptr, nullptr);
       hipLaunchKernelGGL((c->grouped ? k_rhs_finalize<T, 1> : k_rhs_finalize<T, 0>), dim3(nblocks(c->nf)),
                          dim3(kBlock), 0, s, c->nf, d.f_lo, d.f_hi, d.partial, d.gc, d.rhs, win());
@@ -2240,12 +2250,14 @@ struct Engine {
       hipLaunchKernelGGL(k_cam_dir<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, d.cam_to_f, d.cscale,
                          d.ctab, x, sign, d.dtab, done, wv);
   }
-  // Point-major: z_e = Vinv E'(beta r + F x) (x through d.dtab).
+  // Point-major: z_e = Vinv E'(beta r + F x) (x through d.dtab); the rows'
+  // loss weights from the last pass and, for beta != 0 inside the LM loop,
+  // the FP64 cost's residuals (the ones the pass's g_p was formed from).
   void mv_point(T beta, T* pk_out, const int32_t* done, const Win& wv) {
     if (c->seg_blocks == 0) return;
     hipLaunchKernelGGL((c->grouped ? k_mv_point<T, 1> : k_mv_point<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0, s,
                        c->seg, d.pt_start, d.row_cam, d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber,
-                       beta, 1, d.z, pk_out, done, wv);
+                       beta, 1, d.z, pk_out, done, wv, (const T*)d.wrow, beta != T(0) ? fixed_rows() : nullptr);
   }
   // Camera-major partials: packed, sum Jc'(Jp z); grouped, sum F'(F x - E z)
   // (dtab: the direction table of x, or NULL for x = 0).
@@ -2266,7 +2278,7 @@ struct Engine {
       hipLaunchKernelGGL((c->d_slot_chunk ? k_mv_cam<T, 64> : k_mv_cam<T, kBlock>), dim3(c->nslots),
                          dim3(c->d_slot_chunk ? 64 : kBlock), 0, s, d.chunk_f, d.chunk_begin,
                          d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber, d.partial,
-                         done, wv ? *wv : win(), (const int32_t*)c->d_slot_chunk);
+                         done, wv ? *wv : win(), (const int32_t*)c->d_slot_chunk, (const T*)d.wcm);
   }
 
   // y = S x (implicit), x and y F-order.  Inside PCG (`done` set) the
@@ -3148,6 +3160,36 @@ mba_status mba_context_jacobian_schur(mba_context* ctx, double radius) {
   }
   Engine<double> e(ctx);
   return run(e);
+}
+
+mba_status mba_context_lm_pass(mba_context* ctx, double radius, double* cost) {
+  if (!ctx) return fail(MBA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  HIP_TRY(hipSetDevice(ctx->device));
+  auto run = [&](auto& eng) -> mba_status {
+    std::vector<double> c;
+    std::vector<uint8_t> fin;
+    // As Engine::solve starts an iteration: the FP64 cost of the state (its
+    // residuals, rounded once, kept for the FP32 kernels), then the pass.
+    MBA_TRY(eng.cost64(ctx->cams64, ctx->pts64, &c, &fin, ctx->rcur));
+    ctx->rvalid = ctx->rfix;
+    eng.tables(ctx->cams64);
+    if (!ctx->scale_ready) MBA_TRY(eng.init_scale(ctx->opt.jacobi_scaling != 0));
+    MBA_TRY(eng.set_radius(radius));
+    eng.jacobian_pass(1);
+    ctx->rvalid = false;
+    if (cost) *cost = c.empty() ? 0.0 : c[0];
+    return MBA_OK;
+  };
+  mba_status st;
+  if (ctx->mixed) {
+    Engine<float> e(ctx);
+    st = run(e);
+  } else {
+    Engine<double> e(ctx);
+    st = run(e);
+  }
+  ctx->rvalid = false;
+  return st;
 }
 
 mba_status mba_context_read_scale(mba_context* ctx, double* cam_scale, double* pt_scale);

@@ -346,6 +346,15 @@ mba_status mba_context_destroy(mba_context* ctx);
  * (radius <= 0: no damping).  Asynchronous on the context stream. */
 mba_status mba_context_jacobian_schur(mba_context* ctx, double radius);
 
+/* The Jacobian + Schur pass as the LM loop runs it at the start of an
+ * iteration: the FP64 cost of the context's current parameters first (into
+ * *cost, may be NULL), whose residuals, rounded once to FP32, are the
+ * residuals of a MIXED context's pass (gradient, rhs and Huber weights);
+ * then the pass of mba_context_jacobian_schur at trust radius `radius`.  An
+ * FP64 context computes the same blocks as mba_context_jacobian_schur.
+ * Synchronous for the cost; the pass is queued on the context stream. */
+mba_status mba_context_lm_pass(mba_context* ctx, double radius, double* cost);
+
 /* Copy the blocks of the last pass to the host, in INPUT ids: U (45 per
  * camera, upper triangle row-major), g_c (9), rhs (9), V (6 per point:
  * xx xy xz yy yz zz), g_p (3).  Any output may be NULL. */

@@ -417,5 +417,48 @@ def main():
     print(path, os.path.getsize(path), "bytes")
 
 
+# Full-size trajectories of the benchmark scenes (one thread, so the run is
+# reproducible bit for bit): the adjoint-check scene (200 cameras, 100,000
+# points, 697k observations) with SPARSE_SCHUR + CX_SPARSE, no loss, 20 LM
+# iterations (the libs/sfm solver), and the mixed-precision scene (1,000
+# cameras, 500,000 points, 4.0M observations, 5% 20-px outliers) with
+# Huber 4 px and ITERATIVE_SCHUR, JACOBI and SCHUR_JACOBI, capped at 5 LM
+# iterations (CXSparse on its dense 9,000^2 S would take hours).
+LARGE_CASES = [("c2_sparse", 2, 0.0, 0, 20, 0), ("c3_iterative", 3, 4.0, 1, 5, 0),
+               ("c3_schur_jacobi", 3, 4.0, 1, 5, 1)]
+LARGE_POINT_SAMPLE = 2000
+
+
+def main_large():
+    """Writes ceres_ref_large.npz: per case the summary, the iteration
+    records (iteration, cost, cost_change, radius, successful, valid, CG
+    iterations, relative_decrease, gradient_max_norm), the final cameras
+    and the final positions of a seeded sample of points."""
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref/libceres_ref.so is not built (make -C oracle/ref)")
+    out = {}
+    for name, preset, huber, ls, iters, sj in LARGE_CASES:
+        d = O.generate(O.preset(preset))
+        sc = O.Scene(d)
+        O.ref_set_schur_jacobi(bool(sj))
+        try:
+            r = O.ref_solve(sc, huber_a=huber, linear_solver=ls, max_iters=iters, num_threads=1)
+        finally:
+            O.ref_set_schur_jacobi(False)
+        for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
+            out[f"{name}_{k}"] = np.asarray(r[k])
+        idx = np.sort(np.random.default_rng(preset).choice(d["num_points"], LARGE_POINT_SAMPLE, replace=False))
+        out[f"{name}_cameras"] = sc.cameras
+        out[f"{name}_point_index"] = idx
+        out[f"{name}_points"] = sc.points[idx]
+        print(name, r["final_cost"], len(r["iterations"]), flush=True)
+    path = os.path.join(HERE, "ceres_ref_large.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
+
+
 if __name__ == "__main__":
-    main()
+    if "--large" in sys.argv:
+        main_large()
+    else:
+        main()

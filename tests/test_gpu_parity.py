@@ -96,18 +96,23 @@ def test_evaluate_fp64_matches_reference_autodiff(mba, O, golden):
 
 def test_adjoint_check_700k_mixed_vs_fp64_jets(mba, O):
     """200 cameras, 100k points, ~700k observations: FP32 residual + analytic
-    Jacobian vs the FP64 Jets, per-entry 1e-5 (|b| + column RMS); residuals
-    within 1e-5 of the image-plane scale (|predicted| + f): an FP32 residual
-    is the difference of two pixel coordinates of size up to f |x/z|."""
+    Jacobian against the FP64 Jets, in units of the FP32 unit roundoff
+    u = 2^-24.  Residuals: an FP32 residual is the difference of two pixel
+    coordinates of the image-plane scale |predicted| + f (the camera-frame
+    point's cancellation carries f |X_c| / z), held to 16 u of that scale.
+    Jacobian entries: per entry against |b| + column RMS, held to 128 u
+    (7.6e-6; the measured worst entries are ~57 u)."""
     d2 = O.generate(O.preset(2))
     r, jc, jp, _ = mba.evaluate(ba_problem(mba, d2), mba.PRECISION_MIXED)
     ro, jco, jpo, _ = O.evaluate(O.Scene(d2))
     f = d2["cam_init"][d2["cam_idx"], 6][:, None]
-    pred = np.abs(ro + d2["obs"]) + f
-    assert np.max(np.abs(r - ro) / pred) < 1e-5
-    e_c, e_p = rel_err(jc, jco), rel_err(jp, jpo)
-    print(f"700k adjoint check: camera J {e_c:.2e}, point J {e_p:.2e}")
-    assert e_c < 1e-5 and e_p < 1e-5
+    scale = np.abs(ro + d2["obs"]) + f
+    u = 2.0 ** -24
+    e_r = float(np.max(np.abs(r - ro) / scale)) / u
+    e_c, e_p = rel_err(jc, jco) / u, rel_err(jp, jpo) / u
+    print(f"700k adjoint check: residual {e_r:.2f} u of |pred| + f, camera J {e_c:.1f} u, point J {e_p:.1f} u")
+    assert e_r < 16
+    assert e_c < 128 and e_p < 128
 
 
 def test_jacobian_matches_central_differences(mba, O):
@@ -141,11 +146,17 @@ def test_jacobian_matches_central_differences(mba, O):
 @pytest.mark.parametrize("prec,huber,tol", [(0, 4.0, 1e-10), (1, 0.0, 1e-5), (1, 4.0, 1e-4)])
 @pytest.mark.parametrize("cfg", [1, 2])
 def test_jacobian_schur_blocks_match_oracle(mba, O, prec, huber, tol, cfg):
-    """Blocks of one Jacobian + Schur pass against the FP64 oracle.  Mixed
-    with Huber: the FP32 residual is the difference of two pixel coordinates
-    of size ~f, so its absolute error is ~1e-4 px; an outlier row's weight
-    sqrt(a/|r|) inherits that relative to |r| (a few px), which bounds the
-    blocks at 1e-4.  Without a loss function the mixed blocks meet 1e-5."""
+    """Blocks of one plain Jacobian + Schur pass (residuals formed as FP32
+    pred - obs inside the kernels) against the FP64 oracle.  Without a loss
+    function the mixed blocks meet 1e-5.  With Huber, V is held to 1e-4 by
+    a cancellation argument: the FP32 residual is the difference of two
+    pixel coordinates of size ~f, so its absolute error is up to ~1e-4 px,
+    and an outlier row's weight sqrt(a/|r|) carries it as a relative error
+    |dr| / (2 |r|) ~ 1e-5 at |r| = a = 4 px; a point seen by few rows, most
+    of them outliers, passes that straight into V (measured worst 5e-5).
+    The LM loop does not run this variant: its mixed passes take the FP64
+    cost's residuals (Context.lm_pass, held to 1e-5 with Huber in
+    test_gpu_scale.py)."""
     d = O.generate(O.preset(cfg))
     with mba.Context(ba_problem(mba, d), mba.default_options(precision=prec, huber_a=huber)) as ctx:
         ctx.jacobian_schur(1e4)
@@ -158,12 +169,14 @@ def test_jacobian_schur_blocks_match_oracle(mba, O, prec, huber, tol, cfg):
     D = np.sqrt(np.concatenate([np.clip(ref0["U"][:, ut], 1e-6, 1e32).ravel(),
                                 np.clip(ref0["V"][:, [0, 3, 5]], 1e-6, 1e32).ravel()]) / 1e4)
     ref = O.jacobian_schur(sc, huber_a=huber, scale=scale, D=D)
+    errs = {}
     for k, b, n in [("U", ref0["U"], 9), ("V", ref0["V"], 3)]:
-        e = block_err(out[k], b, n)
-        assert e < tol, (k, e)
+        errs[k] = block_err(out[k], b, n)
     ts = term_scales(O, d, huber, scale, ref)
     for k in ("gc", "gp", "rhs"):
-        e = float(np.max(np.abs(out[k] - ref[k]) / (ts[k] + 1e-300)))
+        errs[k] = float(np.max(np.abs(out[k] - ref[k]) / (ts[k] + 1e-300)))
+    print(f"blocks cfg {cfg} precision {prec} huber {huber}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    for k, e in errs.items():
         assert e < tol, (k, e)
 
 
