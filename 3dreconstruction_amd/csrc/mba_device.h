@@ -792,10 +792,14 @@ __global__ __launch_bounds__(kBlock) void k_grad_final(const double* __restrict_
 // ------------------------------------------------------------- point pass
 // V = sum Jp'Jp, g_p = sum Jp' r; LM diagonal of points; Vinv = (V + D^2)^-1
 // in FP64; w = Vinv g_p.
+// V accumulates in FP64 from the kernel-precision Jacobian rows: V is then
+// (up to FP64 rounding) exactly the sum of the rows' J_p'J_p, so V - J_p'J_p
+// stays positive semidefinite for every row and the FP64 inverse never meets
+// a block that FP32 accumulation rounded to indefinite.
 template <typename T, int L>
 __device__ __forceinline__ T point_row_accumulate(const T* __restrict__ ctab, int cam, T ox, T oy,
                                                   const T* __restrict__ rf, const T X[3], const T sp[3],
-                                                  T huber_a, T v[6], T g[3]) {
+                                                  T huber_a, double v[6], T g[3]) {
   CamProj<T> c;
   load_proj<T, L>(ctab + (size_t)kCamTab * cam, c);
   Proj<T> o;
@@ -808,12 +812,15 @@ __device__ __forceinline__ T point_row_accumulate(const T* __restrict__ ctab, in
   T Jp[6];
   jac_point(o, c.R, wgt, sp, Jp);
   const T rx = wgt * o.rx, ry = wgt * o.ry;
-  fma2(v[0], Jp[0], Jp[0], Jp[3], Jp[3]);
-  fma2(v[1], Jp[0], Jp[1], Jp[3], Jp[4]);
-  fma2(v[2], Jp[0], Jp[2], Jp[3], Jp[5]);
-  fma2(v[3], Jp[1], Jp[1], Jp[4], Jp[4]);
-  fma2(v[4], Jp[1], Jp[2], Jp[4], Jp[5]);
-  fma2(v[5], Jp[2], Jp[2], Jp[5], Jp[5]);
+  double J[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) J[i] = (double)Jp[i];
+  fma2(v[0], J[0], J[0], J[3], J[3]);
+  fma2(v[1], J[0], J[1], J[3], J[4]);
+  fma2(v[2], J[0], J[2], J[3], J[5]);
+  fma2(v[3], J[1], J[1], J[4], J[4]);
+  fma2(v[4], J[1], J[2], J[4], J[5]);
+  fma2(v[5], J[2], J[2], J[5], J[5]);
   fma2(g[0], Jp[0], rx, Jp[3], ry);
   fma2(g[1], Jp[1], rx, Jp[4], ry);
   fma2(g[2], Jp[2], rx, Jp[5], ry);
@@ -825,14 +832,16 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
     Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const double* __restrict__ pts64, const T* __restrict__ pscale,
     const T* __restrict__ ctab, T huber_a, Win win, double min_diag, double max_diag,
-    double* __restrict__ diag_p, int recompute_diag, T* __restrict__ V, T* __restrict__ gp,
-    T* __restrict__ Vinv, T* __restrict__ pk, const T* __restrict__ rres, T* __restrict__ wrow) {
+    double* __restrict__ diag_p, int recompute_diag, double* __restrict__ V, T* __restrict__ gp,
+    T* __restrict__ Vinv, T* __restrict__ pk, const T* __restrict__ rres, T* __restrict__ wrow,
+    double* __restrict__ Vinv64) {
   SegPos q = seg_locate(sg);
   if (q.valid && win_idle_e(win, q.e)) q.valid = false;
   const int e = q.valid ? q.e : sg.e0[0];
   const T X[3] = {(T)pts64[3 * (size_t)e], (T)pts64[3 * (size_t)e + 1], (T)pts64[3 * (size_t)e + 2]};
   const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
-  T v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  T g[3] = {0, 0, 0};
   if (q.valid) {
     const int r1 = pt_start[e + 1];
     const int ln = q.lanes;
@@ -881,7 +890,7 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     if (recompute_diag) {
-      dg[i] = fmin(fmax((double)v[di[i]], min_diag), max_diag);
+      dg[i] = fmin(fmax(v[di[i]], min_diag), max_diag);
       diag_p[3 * (size_t)e + i] = dg[i];
     } else {
       dg[i] = diag_p[3 * (size_t)e + i];
@@ -889,7 +898,7 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
   }
   const int wi = win_of_e(win, e);
   const double inv_radius = win_inv_r(win, wi);
-  double a[6] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3], (double)v[4], (double)v[5]};
+  double a[6] = {v[0], v[1], v[2], v[3], v[4], v[5]};
   a[0] += dg[0] * inv_radius;
   a[3] += dg[1] * inv_radius;
   a[5] += dg[2] * inv_radius;
@@ -900,6 +909,9 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
   }
 #pragma unroll
   for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
+  if (Vinv64)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) Vinv64[6 * (size_t)e + i] = iv[i];
   const double g0 = g[0], g1 = g[1], g2 = g[2];
   const T sw[3] = {(T)((double)sp[0] * (iv[0] * g0 + iv[1] * g1 + iv[2] * g2)),
                    (T)((double)sp[1] * (iv[1] * g0 + iv[3] * g1 + iv[4] * g2)),
@@ -909,17 +921,18 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
 
 // Re-damp after a rejected step: Vinv, w from stored V, g_p, diag_p.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restrict__ V,
+__global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const double* __restrict__ V,
                                                        const T* __restrict__ gp,
                                                        const double* __restrict__ diag_p, Win win,
                                                        const T* __restrict__ pscale,
-                                                       T* __restrict__ Vinv, T* __restrict__ pk) {
+                                                       T* __restrict__ Vinv, T* __restrict__ pk,
+                                                       double* __restrict__ Vinv64) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne || win_idle_e(win, e)) return;
   const int wi = win_of_e(win, e);
   const double inv_radius = win_inv_r(win, wi);
   double a[6];
-  for (int i = 0; i < 6; ++i) a[i] = (double)V[6 * (size_t)e + i];
+  for (int i = 0; i < 6; ++i) a[i] = V[6 * (size_t)e + i];
   a[0] += diag_p[3 * (size_t)e] * inv_radius;
   a[3] += diag_p[3 * (size_t)e + 1] * inv_radius;
   a[5] += diag_p[3 * (size_t)e + 2] * inv_radius;
@@ -929,6 +942,8 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const T* __restri
     for (int i = 0; i < 6; ++i) iv[i] = 0.0;
   }
   for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
+  if (Vinv64)
+    for (int i = 0; i < 6; ++i) Vinv64[6 * (size_t)e + i] = iv[i];
   const double g0 = gp[3 * (size_t)e], g1 = gp[3 * (size_t)e + 1], g2 = gp[3 * (size_t)e + 2];
   const T w[3] = {(T)(iv[0] * g0 + iv[1] * g1 + iv[2] * g2), (T)(iv[1] * g0 + iv[3] * g1 + iv[4] * g2),
                   (T)(iv[2] * g0 + iv[4] * g1 + iv[5] * g2)};
@@ -1141,8 +1156,9 @@ __global__ __launch_bounds__(B) void k_sj_pass(
     const int32_t* __restrict__ chunk_f, const int32_t* __restrict__ chunk_begin,
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
-    const T* __restrict__ pscale, const T* __restrict__ Vinv, const T* __restrict__ ctab, T huber_a,
-    T* __restrict__ partial, Win win, const int32_t* __restrict__ slot_chunk, const T* __restrict__ wcm) {
+    const T* __restrict__ pscale, const double* __restrict__ Vinv64, const T* __restrict__ ctab,
+    T huber_a, T* __restrict__ partial, Win win, const int32_t* __restrict__ slot_chunk,
+    const T* __restrict__ wcm) {
   __shared__ T lds[45 * (B / 64)];
   const int ch = chunk_of_slot(slot_chunk);
   if (ch < 0) return;
@@ -1163,8 +1179,8 @@ __global__ __launch_bounds__(B) void k_sj_pass(
     T X[3], unused[3];
     load_pk(pk, e, X, unused);
     const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
-    const T* iv = Vinv + 6 * (size_t)e;
-    const T Vi[9] = {iv[0], iv[1], iv[2], iv[1], iv[3], iv[4], iv[2], iv[4], iv[5]};
+    const double* iv = Vinv64 + 6 * (size_t)e;
+    const double Vi[9] = {iv[0], iv[1], iv[2], iv[1], iv[3], iv[4], iv[2], iv[4], iv[5]};
     Proj<T> o;
     project(c, X, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1], o);
     const T wgt = wcm ? wcm[i] : huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
@@ -1172,22 +1188,26 @@ __global__ __launch_bounds__(B) void k_sj_pass(
     jac_cam(o, dR, X, wgt, one, Jc);
     jac_point(o, c.R, wgt, sp, Jp);
     // A = Jp Vinv Jp' (2 x 2), then B = (I - A) Jc: the row's share of the
-    // diagonal block, Jc'Jc - Jc'A Jc, formed per row so that the FP32
-    // sum carries no cancellation between two large sums.
-    T G[6];
+    // diagonal block, Jc'Jc - Jc'A Jc, formed per row so that the sum
+    // carries no cancellation between two large sums.  A comes from the
+    // FP64 inverse of the FP64-accumulated V of these same Jacobian rows, so
+    // I - A is positive semidefinite up to FP64 rounding.
+    double Jd[6], G[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Jd[q] = (double)Jp[q];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        G[3 * r + j] = Jp[3 * r] * Vi[j] + Jp[3 * r + 1] * Vi[3 + j] + Jp[3 * r + 2] * Vi[6 + j];
-    const T a00 = G[0] * Jp[0] + G[1] * Jp[1] + G[2] * Jp[2];
-    const T a01 = G[0] * Jp[3] + G[1] * Jp[4] + G[2] * Jp[5];
-    const T a11 = G[3] * Jp[3] + G[4] * Jp[4] + G[5] * Jp[5];
+        G[3 * r + j] = Jd[3 * r] * Vi[j] + Jd[3 * r + 1] * Vi[3 + j] + Jd[3 * r + 2] * Vi[6 + j];
+    const T m00 = (T)(1.0 - (G[0] * Jd[0] + G[1] * Jd[1] + G[2] * Jd[2]));
+    const T m01 = (T)(-(G[0] * Jd[3] + G[1] * Jd[4] + G[2] * Jd[5]));
+    const T m11 = (T)(1.0 - (G[3] * Jd[3] + G[4] * Jd[4] + G[5] * Jd[5]));
     T B0[9], B1[9];
 #pragma unroll
     for (int b = 0; b < 9; ++b) {
-      B0[b] = (T(1) - a00) * Jc[b] - a01 * Jc[9 + b];
-      B1[b] = (T(1) - a11) * Jc[9 + b] - a01 * Jc[b];
+      B0[b] = m00 * Jc[b] + m01 * Jc[9 + b];
+      B1[b] = m01 * Jc[b] + m11 * Jc[9 + b];
     }
     int k = 0;
 #pragma unroll
@@ -1284,7 +1304,7 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
     const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
     const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ Vinv, T huber_a,
     T beta, int apply_vinv, T* __restrict__ zout, T* __restrict__ pk, const int32_t* __restrict__ all_done,
-    Win win, const T* __restrict__ wrow, const T* __restrict__ rres) {
+    Win win, const T* __restrict__ wrow, const T* __restrict__ rres, const double* __restrict__ Vinv64) {
   if (all_done && *all_done) return;
   SegPos qp = seg_locate(sg);
   if (qp.valid && win_idle_e(win, qp.e)) qp.valid = false;
@@ -1332,7 +1352,16 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
   for (int i = 0; i < 3; ++i) q[i] = group_sum(q[i], qp.lanes);
   if (!qp.valid || qp.sub != 0) return;
   T zv[3] = {q[0], q[1], q[2]};
-  if (apply_vinv) {
+  if (apply_vinv && Vinv64) {
+    // The FP64 inverse of the FP64-accumulated V: an FP32 copy of the
+    // inverse of an ill-conditioned point block is too coarse to keep the
+    // implicit Schur complement positive definite.
+    const double* iv = Vinv64 + 6 * (size_t)e;
+    const double q0 = q[0], q1 = q[1], q2 = q[2];
+    zv[0] = (T)(iv[0] * q0 + iv[1] * q1 + iv[2] * q2);
+    zv[1] = (T)(iv[1] * q0 + iv[3] * q1 + iv[4] * q2);
+    zv[2] = (T)(iv[2] * q0 + iv[4] * q1 + iv[5] * q2);
+  } else if (apply_vinv) {
     const T* iv = Vinv + 6 * (size_t)e;
     zv[0] = iv[0] * q[0] + iv[1] * q[1] + iv[2] * q[2];
     zv[1] = iv[1] * q[0] + iv[3] * q[1] + iv[4] * q[2];
@@ -2686,14 +2715,14 @@ __global__ __launch_bounds__(kBlock) void k_plus_cams(int nf, const int32_t* __r
 // Jacobi scaling 1 / (1 + |column|) from an unscaled pass
 // (TrustRegionMinimizer::EstimateScale, trust_region_minimizer.cc:103-109).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_make_scale(int ne, int nf, const T* __restrict__ V,
+__global__ __launch_bounds__(kBlock) void k_make_scale(int ne, int nf, const double* __restrict__ V,
                                                        const T* __restrict__ U,
                                                        double* __restrict__ pscale,
                                                        double* __restrict__ cscale) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < ne) {
     const int di[3] = {0, 3, 5};
-    for (int j = 0; j < 3; ++j) pscale[3 * (size_t)i + j] = 1.0 / (1.0 + sqrt((double)V[6 * (size_t)i + di[j]]));
+    for (int j = 0; j < 3; ++j) pscale[3 * (size_t)i + j] = 1.0 / (1.0 + sqrt(V[6 * (size_t)i + di[j]]));
   }
   if (i < nf)
     for (int j = 0; j < 9; ++j) cscale[9 * (size_t)i + j] = 1.0 / (1.0 + sqrt((double)U[45 * (size_t)i + ut9(j, j)]));

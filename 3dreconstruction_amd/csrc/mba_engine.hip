@@ -862,8 +862,10 @@ struct Dev {
   T* cm_obs = nullptr;
   // kernel-precision parameters
   T *ctab = nullptr, *pscale = nullptr, *cscale = nullptr;
-  // blocks
-  T *V = nullptr, *gp = nullptr, *Vinv = nullptr, *z = nullptr, *pk = nullptr;
+  // blocks (V in FP64 in either precision; Vinv64, the FP64 inverse of the
+  // damped V, for FP32 kernels: the FP64 kernels' Vinv is already FP64)
+  double *V = nullptr, *Vinv64 = nullptr;
+  T *gp = nullptr, *Vinv = nullptr, *z = nullptr, *pk = nullptr;
   T *U = nullptr, *gc = nullptr, *rhs = nullptr, *Minv = nullptr, *dtab = nullptr, *partial = nullptr;
   // PCG / step
   T *x = nullptr, *r = nullptr, *zc = nullptr, *p = nullptr, *q = nullptr, *tmp = nullptr;
@@ -1119,6 +1121,7 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_st
   MBA_TRY(c->pool.alloc(&d.V, 6 * ne));
   MBA_TRY(c->pool.alloc(&d.gp, 3 * ne));
   MBA_TRY(c->pool.alloc(&d.Vinv, 6 * ne));
+  if constexpr (std::is_same<T, float>::value) MBA_TRY(c->pool.alloc(&d.Vinv64, 6 * std::max<size_t>(1, ne)));
   MBA_TRY(c->pool.alloc(&d.z, 3 * ne));
   MBA_TRY(c->pool.alloc(&d.pk, kPk * ne));
   MBA_TRY(c->pool.alloc(&d.U, 45 * nf));
@@ -2178,6 +2181,13 @@ struct Engine {
     else
       return nullptr;
   }
+  // FP64 inverses of the damped point blocks.
+  const double* vinv64() const {
+    if constexpr (std::is_same<T, double>::value)
+      return d.Vinv;
+    else
+      return d.Vinv64;
+  }
   const T* fixed_cm() const {
     if constexpr (std::is_same<T, float>::value)
       return c->rvalid ? c->cres[c->rcur] : nullptr;
@@ -2191,7 +2201,7 @@ struct Engine {
       hipLaunchKernelGGL((c->grouped ? k_point_pass<T, 1> : k_point_pass<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0,
                          s, c->seg, d.pt_start, d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, win(),
                          c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag,
-                         d.V, d.gp, d.Vinv, d.pk, fixed_rows(), c->grouped ? nullptr : d.wrow);
+                         d.V, d.gp, d.Vinv, d.pk, fixed_rows(), c->grouped ? nullptr : d.wrow, d.Vinv64);
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
     if (c->nchunks > 0) {
       if (c->grouped) {
@@ -2220,7 +2230,8 @@ struct Engine {
     if (c->nchunks > 0)
       hipLaunchKernelGGL((c->d_slot_chunk ? k_sj_pass<T, 64> : k_sj_pass<T, kBlock>), dim3(c->nslots),
                          dim3(c->d_slot_chunk ? 64 : kBlock), 0, s, d.chunk_f, d.chunk_begin, d.chunk_end,
-                         d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.pscale, d.Vinv, d.ctab, huber, d.partial_sj, win(),
+                         d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.pscale, vinv64(), d.ctab, huber,
+                         d.partial_sj, win(),
                          (const int32_t*)c->d_slot_chunk, (const T*)d.wcm);
     hipLaunchKernelGGL(k_sj_finalize<T>, dim3(nblocks(45 * (int64_t)c->nf)), dim3(kBlock), 0, s, c->nf, d.f_lo,
                        d.f_hi, (const T*)d.partial_sj, (const T*)d.cscale, win(), d.Tsj);
@@ -2228,8 +2239,8 @@ struct Engine {
 
   // Re-damp after a rejected step (radii current) and rebuild the rhs.
   void damp() {
-    hipLaunchKernelGGL(k_point_damp<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, d.V, d.gp, c->diag_p,
-                       win(), d.pscale, d.Vinv, d.pk);
+    hipLaunchKernelGGL(k_point_damp<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, (const double*)d.V, d.gp,
+                       c->diag_p, win(), d.pscale, d.Vinv, d.pk, d.Vinv64);
     if (c->nf > 0) {
       sj_terms();
       hipLaunchKernelGGL(k_cam_damp<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf, d.U, c->diag_c, win(),
@@ -2257,7 +2268,8 @@ struct Engine {
     if (c->seg_blocks == 0) return;
     hipLaunchKernelGGL((c->grouped ? k_mv_point<T, 1> : k_mv_point<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0, s,
                        c->seg, d.pt_start, d.row_cam, d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber,
-                       beta, 1, d.z, pk_out, done, wv, (const T*)d.wrow, beta != T(0) ? fixed_rows() : nullptr);
+                       beta, 1, d.z, pk_out, done, wv, (const T*)d.wrow, beta != T(0) ? fixed_rows() : nullptr,
+                       vinv64());
   }
   // Camera-major partials: packed, sum Jc'(Jp z); grouped, sum F'(F x - E z)
   // (dtab: the direction table of x, or NULL for x = 0).
@@ -2656,7 +2668,7 @@ struct Engine {
     MBA_TRY(set_radius(0.0));
     jacobian_pass(1);
     hipLaunchKernelGGL(k_make_scale<T>, dim3(nblocks(std::max(c->ne, c->nf))), dim3(kBlock), 0, s, c->ne,
-                       c->nf, d.V, d.U, c->pscale64, c->cscale64);
+                       c->nf, (const double*)d.V, (const T*)d.U, c->pscale64, c->cscale64);
     sync_scale_T();
     c->scale_ready = true;
     return MBA_OK;
