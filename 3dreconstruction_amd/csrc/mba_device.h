@@ -461,6 +461,141 @@ __device__ __forceinline__ void project(const CamProj<T>& c, const T X[3], T ox,
   }
 }
 
+// project() that also returns y = R X, the rotated point before the
+// translation (the FP32 camera-major kernels' local rotation Jacobian).
+template <typename T>
+__device__ __forceinline__ void project_y(const CamProj<T>& c, const T X[3], T ox, T oy, Proj<T>& o, T y[3]) {
+  y[0] = c.R[0] * X[0] + c.R[1] * X[1] + c.R[2] * X[2];
+  y[1] = c.R[3] * X[0] + c.R[4] * X[1] + c.R[5] * X[2];
+  y[2] = c.R[6] * X[0] + c.R[7] * X[1] + c.R[8] * X[2];
+  const T zc = y[2] + c.t[2];
+  const T iz = T(1) / zc;
+  const T xe = (y[0] + c.t[0]) * iz, ye = (y[1] + c.t[1]) * iz;
+  const T r2 = xe * xe + ye * ye;
+  const T d = T(1) + r2 * (c.k1 + c.k2 * r2);
+  const T fdd = c.f * T(2) * (c.k1 + T(2) * c.k2 * r2);
+  const T fd = c.f * d;
+  o.rx = fd * xe - ox;
+  o.ry = fd * ye - oy;
+  const T a11 = fd + fdd * xe * xe, a12 = fdd * xe * ye, a22 = fd + fdd * ye * ye;
+  o.P[0] = a11 * iz;
+  o.P[1] = a12 * iz;
+  o.P[2] = -(a11 * xe + a12 * ye) * iz;
+  o.P[3] = a12 * iz;
+  o.P[4] = a22 * iz;
+  o.P[5] = -(a12 * xe + a22 * ye) * iz;
+  o.jf[0] = d * xe;
+  o.jf[1] = d * ye;
+  o.jk1[0] = c.f * xe * r2;
+  o.jk1[1] = c.f * ye * r2;
+  o.jk2[0] = o.jk1[0] * r2;
+  o.jk2[1] = o.jk1[1] * r2;
+}
+
+// Local rotation basis of the FP32 camera-major kernels.  With y = R X,
+// d(R X)/d aa = -[y]x G where G = J_l(aa) is the left Jacobian of SO(3), so
+// a row's camera Jacobian is J_c = Jt B, B = blockdiag(G, I, I), with
+// Jt = w [y x P_r | P_r | jf_r, jk1_r, jk2_r] per residual row r (the aa
+// columns are cross products instead of P dR_k X).  Sums over rows are
+// formed with Jt and taken to the parameter basis once per chunk.
+template <typename T>
+__device__ __forceinline__ void jac_cam_local(const Proj<T>& o, const T y[3], T wgt, T Jt[18]) {
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const T p0 = wgt * o.P[3 * r], p1 = wgt * o.P[3 * r + 1], p2 = wgt * o.P[3 * r + 2];
+    Jt[9 * r + 0] = y[1] * p2 - y[2] * p1;
+    Jt[9 * r + 1] = y[2] * p0 - y[0] * p2;
+    Jt[9 * r + 2] = y[0] * p1 - y[1] * p0;
+    Jt[9 * r + 3] = p0;
+    Jt[9 * r + 4] = p1;
+    Jt[9 * r + 5] = p2;
+  }
+  Jt[6] = wgt * o.jf[0];
+  Jt[15] = wgt * o.jf[1];
+  Jt[7] = wgt * o.jk1[0];
+  Jt[16] = wgt * o.jk1[1];
+  Jt[8] = wgt * o.jk2[0];
+  Jt[17] = wgt * o.jk2[1];
+}
+
+// G = J_l(w) = I + a [w]x + b [w]x^2, a = (1 - cos t) / t^2,
+// b = (t - sin t) / t^3 (series below t^2 = 1e-6), per camera, row-major
+// G[3 i + j], rounded to T.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cam_lj(int nc, const double* __restrict__ cams, T* __restrict__ gtab) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const double* p = cams + 9 * (size_t)c;
+  const double w0 = p[0], w1 = p[1], w2 = p[2];
+  const double t2 = w0 * w0 + w1 * w1 + w2 * w2;
+  double a, b;
+  if (t2 < 1e-6) {
+    a = 0.5 - t2 / 24.0;
+    b = 1.0 / 6.0 - t2 / 120.0;
+  } else {
+    const double t = sqrt(t2);
+    a = (1.0 - cos(t)) / t2;
+    b = (t - sin(t)) / (t2 * t);
+  }
+  const double K[9] = {0.0, -w2, w1, w2, 0.0, -w0, -w1, w0, 0.0};
+  T* o = gtab + 9 * (size_t)c;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double kk = K[3 * i] * K[j] + K[3 * i + 1] * K[3 + j] + K[3 * i + 2] * K[6 + j];
+      o[3 * i + j] = (T)((i == j ? 1.0 : 0.0) + a * K[3 * i + j] + b * kk);
+    }
+}
+
+// Chunk sums in the local basis -> parameter basis, for the 63 camera-pass
+// sums held one per lane of a wave (lane k: U upper entry k < 45, g 45..53,
+// h 54..62): U = B' Ut B, g = B' gt, h = B' ht.  Every lane of the wave
+// calls it.
+template <typename T>
+__device__ __forceinline__ T chunk_to_param_basis63(T v, int k, const T* __restrict__ G) {
+  // Upper-triangle index of (i, j), i <= j, in a row-major 9 x 9.
+  auto ut9 = [](int i, int j) { return i * 9 - i * (i - 1) / 2 + (j - i); };
+  const T u00 = __shfl(v, 0, 64), u01 = __shfl(v, 1, 64), u02 = __shfl(v, 2, 64);
+  const T u11 = __shfl(v, 9, 64), u12 = __shfl(v, 10, 64), u22 = __shfl(v, 17, 64);
+  int a, b;  // entry (a, b); b = 9 for g, 10 for h
+  if (k < 45) {
+    a = 0;
+    while (a < 8 && k >= ut9(a + 1, a + 1)) ++a;
+    b = a + (k - ut9(a, a));
+  } else if (k < 54) {
+    a = k - 45;
+    b = 9;
+  } else {
+    a = k < 63 ? k - 54 : 9;
+    b = 10;
+  }
+  auto src = [&](int i) { return b < 9 ? ut9(i, b) : (b == 9 ? 45 + i : 54 + i); };
+  const int bs = b < 3 ? 3 : b;  // aa-block columns use the u table below
+  const T c0 = __shfl(v, src(0) < 64 && bs == b ? src(0) : 0, 64);
+  const T c1 = __shfl(v, src(1) < 64 && bs == b ? src(1) : 0, 64);
+  const T c2 = __shfl(v, src(2) < 64 && bs == b ? src(2) : 0, 64);
+  if (a >= 3) return v;
+  const T g0 = G[a], g1 = G[3 + a], g2 = G[6 + a];  // column a of G
+  if (b >= 3) return g0 * c0 + g1 * c1 + g2 * c2;
+  const T h0 = G[b], h1 = G[3 + b], h2 = G[6 + b];  // column b of G
+  // Ut_aa h, then g' (Ut_aa h)
+  const T m0 = u00 * h0 + u01 * h1 + u02 * h2;
+  const T m1 = u01 * h0 + u11 * h1 + u12 * h2;
+  const T m2 = u02 * h0 + u12 * h1 + u22 * h2;
+  return g0 * m0 + g1 * m1 + g2 * m2;
+}
+
+// Chunk sums of the camera matvec (lanes 0..8) -> parameter basis and
+// camera column scale: p = diag(cs) B' pt.
+template <typename T>
+__device__ __forceinline__ T chunk_to_param_basis9(T v, int k, const T* __restrict__ G, const T* __restrict__ cs) {
+  const T p0 = __shfl(v, 0, 64), p1 = __shfl(v, 1, 64), p2 = __shfl(v, 2, 64);
+  if (k >= 9) return v;
+  if (k >= 3) return cs[k] * v;
+  return cs[k] * (G[k] * p0 + G[3 + k] * p1 + G[6 + k] * p2);
+}
+
 // HuberLoss + Corrector for rho'' <= 0 (loss_function.cc:47-61,
 // corrector.cc:41-85): residual and Jacobian scale by sqrt(rho').
 template <typename T>
@@ -576,29 +711,25 @@ __device__ __forceinline__ void jac_point_apply(const Proj<T>& o, const T R[9], 
   u[1] = o.P[3] * a0 + o.P[4] * a1 + o.P[5] * a2;
 }
 
-// FP64 inverse of an SPD 3x3 (xx xy xz yy yz zz) via LLT.
+// FP64 inverse of an SPD 3x3 (xx xy xz yy yz zz): adjugate over the
+// determinant, one division.  Positive definiteness by Sylvester's
+// criterion (a00 > 0, leading 2x2 minor > 0, det > 0); false otherwise.
 __device__ __forceinline__ bool inv3_spd(const double a[6], double o[6]) {
-  if (!(a[0] > 0.0)) return false;
-  const double l00 = sqrt(a[0]), l10 = a[1] / l00, l20 = a[2] / l00;
-  double t = a[3] - l10 * l10;
-  if (!(t > 0.0)) return false;
-  const double l11 = sqrt(t), l21 = (a[4] - l20 * l10) / l11;
-  t = a[5] - l20 * l20 - l21 * l21;
-  if (!(t > 0.0)) return false;
-  const double l22 = sqrt(t);
-  const double i00 = 1.0 / l00, i11 = 1.0 / l11, i22 = 1.0 / l22;
-  const double i10 = -l10 * i00 * i11, i21 = -l21 * i11 * i22, i20 = -(l20 * i00 + l21 * i10) * i22;
-  o[0] = i00 * i00 + i10 * i10 + i20 * i20;
-  o[1] = i10 * i11 + i20 * i21;
-  o[2] = i20 * i22;
-  o[3] = i11 * i11 + i21 * i21;
-  o[4] = i21 * i22;
-  o[5] = i22 * i22;
+  const double c00 = a[3] * a[5] - a[4] * a[4], c01 = a[2] * a[4] - a[1] * a[5], c02 = a[1] * a[4] - a[2] * a[3];
+  const double c11 = a[0] * a[5] - a[2] * a[2], c12 = a[1] * a[2] - a[0] * a[4], c22 = a[0] * a[3] - a[1] * a[1];
+  const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+  if (!(a[0] > 0.0) || !(c22 > 0.0) || !(det > 0.0)) return false;
+  const double id = 1.0 / det;
+  o[0] = c00 * id;
+  o[1] = c01 * id;
+  o[2] = c02 * id;
+  o[3] = c11 * id;
+  o[4] = c12 * id;
+  o[5] = c22 * id;
   return true;
 }
 
-// FP64 inverse of an SPD n x n (row-major, n <= 9) via LLT; false if not PD.
-
+// Packed upper-triangle index of (i, j), i <= j, in a 9 x 9 block.
 __device__ __forceinline__ int ut9(int i, int j) { return i * 9 - i * (i - 1) / 2 + (j - i); }
 __device__ __forceinline__ int lt9(int i, int j) { return i * (i + 1) / 2 + j; }  // packed lower, j <= i
 
@@ -653,7 +784,8 @@ __global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __rest
                                                    const double* __restrict__ ctab, double huber_a,
                                                    double* __restrict__ partial, Win win,
                                                    float* __restrict__ rrow, float* __restrict__ rcm,
-                                                   const int32_t* __restrict__ row_cm) {
+                                                   const int32_t* __restrict__ row_cm,
+                                                   float* __restrict__ wrow, float* __restrict__ wcm) {
   __shared__ double lds[kBlock / 64];
   SegPos sp = seg_locate(sg);
   if (sp.valid && win_idle_e(win, sp.e)) sp.valid = false;
@@ -675,10 +807,14 @@ __global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __rest
         // (row order and, through row_cm, camera-major order).
         rrow[2 * (size_t)r] = (float)o.rx;
         rrow[2 * (size_t)r + 1] = (float)o.ry;
+        // With a loss, its weight sqrt(rho') too, evaluated in FP64.
+        const float w = wrow ? (float)huber_weight(s, huber_a) : 1.0f;
+        if (wrow) wrow[r] = w;
         const int k = row_cm[r];
         if (k >= 0) {
           rcm[2 * (size_t)k] = (float)o.rx;
           rcm[2 * (size_t)k + 1] = (float)o.ry;
+          if (wcm) wcm[k] = w;
         }
       }
     }
@@ -799,7 +935,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_final(const double* __restrict_
 template <typename T, int L>
 __device__ __forceinline__ T point_row_accumulate(const T* __restrict__ ctab, int cam, T ox, T oy,
                                                   const T* __restrict__ rf, const T X[3], const T sp[3],
-                                                  T huber_a, double v[6], T g[3]) {
+                                                  T huber_a, double v[6], T g[3], const T* wg = nullptr) {
   CamProj<T> c;
   load_proj<T, L>(ctab + (size_t)kCamTab * cam, c);
   Proj<T> o;
@@ -808,7 +944,8 @@ __device__ __forceinline__ T point_row_accumulate(const T* __restrict__ ctab, in
     o.rx = rf[0];
     o.ry = rf[1];
   }
-  const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+  // wg (optional): the row's weight, given with the residual rf.
+  const T wgt = wg ? *wg : huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
   T Jp[6];
   jac_point(o, c.R, wgt, sp, Jp);
   const T rx = wgt * o.rx, ry = wgt * o.ry;
@@ -850,6 +987,10 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
     // rres (optional): residuals evaluated in FP64 at this state, used in
     // place of the kernel-precision pred - obs.
     const T* ob = rres ? rres : row_obs;
+    // With rres, wrow holds their weights (from the same FP64 evaluation);
+    // otherwise the weights are computed here and written to wrow (if any).
+    const T* wgi = rres ? wrow : nullptr;
+    T* wout = rres ? nullptr : wrow;
     for (; r + ln < r1; r += 2 * ln) {
       const int c0 = row_cam[r], c1 = row_cam[r + ln];
       const T o0x = ob[2 * (size_t)r], o0y = ob[2 * (size_t)r + 1];
@@ -857,23 +998,24 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
       T w0, w1;
       if (rres) {
         const T f0[2] = {o0x, o0y}, f1[2] = {o1x, o1y};
-        w0 = point_row_accumulate<T, L>(ctab, c0, T(0), T(0), f0, X, sp, huber_a, v, g);
-        w1 = point_row_accumulate<T, L>(ctab, c1, T(0), T(0), f1, X, sp, huber_a, v, g);
+        w0 = point_row_accumulate<T, L>(ctab, c0, T(0), T(0), f0, X, sp, huber_a, v, g, wgi ? wgi + r : nullptr);
+        w1 = point_row_accumulate<T, L>(ctab, c1, T(0), T(0), f1, X, sp, huber_a, v, g,
+                                        wgi ? wgi + r + ln : nullptr);
       } else {
         w0 = point_row_accumulate<T, L>(ctab, c0, o0x, o0y, nullptr, X, sp, huber_a, v, g);
         w1 = point_row_accumulate<T, L>(ctab, c1, o1x, o1y, nullptr, X, sp, huber_a, v, g);
       }
-      if (wrow) {
-        wrow[r] = w0;
-        wrow[r + ln] = w1;
+      if (wout) {
+        wout[r] = w0;
+        wout[r + ln] = w1;
       }
     }
     if (r < r1) {
       const T ox = ob[2 * (size_t)r], oy = ob[2 * (size_t)r + 1];
       const T f0[2] = {ox, oy};
       const T w0 = point_row_accumulate<T, L>(ctab, row_cam[r], rres ? T(0) : ox, rres ? T(0) : oy,
-                                              rres ? f0 : nullptr, X, sp, huber_a, v, g);
-      if (wrow) wrow[r] = w0;
+                                              rres ? f0 : nullptr, X, sp, huber_a, v, g, wgi ? wgi + r : nullptr);
+      if (wout) wout[r] = w0;
     }
   }
 #pragma unroll
@@ -907,8 +1049,10 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
     win.flag[wi] = 1;
     for (int i = 0; i < 6; ++i) iv[i] = 0.0;
   }
+  // Vinv (kernel precision) may be NULL when only Vinv64 is read.
+  if (Vinv)
 #pragma unroll
-  for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
+    for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
   if (Vinv64)
 #pragma unroll
     for (int i = 0; i < 6; ++i) Vinv64[6 * (size_t)e + i] = iv[i];
@@ -941,7 +1085,8 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const double* __r
     win.flag[wi] = 1;
     for (int i = 0; i < 6; ++i) iv[i] = 0.0;
   }
-  for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
+  if (Vinv)
+    for (int i = 0; i < 6; ++i) Vinv[6 * (size_t)e + i] = (T)iv[i];
   if (Vinv64)
     for (int i = 0; i < 6; ++i) Vinv64[6 * (size_t)e + i] = iv[i];
   const double g0 = gp[3 * (size_t)e], g1 = gp[3 * (size_t)e + 1], g2 = gp[3 * (size_t)e + 2];
@@ -958,18 +1103,29 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const double* __r
 // camera Jacobi scale left out (applied per camera in k_cam_finalize).
 template <typename T>
 __device__ __forceinline__ T cam_row_accumulate(const CamProj<T>& c, const T dR[27], const T X[3],
-                                                const T sw[3], T ox, T oy, bool given, T huber_a,
+                                                const T sw[3], T ox, T oy, bool given, const T* wg, T huber_a,
                                                 int with_h, T acc[63]) {
+  // FP32: Jacobian in the local rotation basis (jac_cam_local); FP64: in the
+  // parameter basis through the table's dR.
+  constexpr bool kLocal = std::is_same<T, float>::value;
   const T one[9] = {T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1), T(1)};
   Proj<T> o;
-  project(c, X, given ? T(0) : ox, given ? T(0) : oy, o);
+  T y[3];
+  if constexpr (kLocal)
+    project_y(c, X, given ? T(0) : ox, given ? T(0) : oy, o, y);
+  else
+    project(c, X, given ? T(0) : ox, given ? T(0) : oy, o);
   if (given) {  // (ox, oy) is the residual itself
     o.rx = ox;
     o.ry = oy;
   }
-  const T wgt = huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+  // wg (optional): the row's weight, given with the residual.
+  const T wgt = wg ? *wg : huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
   T Jc[18];
-  jac_cam(o, dR, X, wgt, one, Jc);
+  if constexpr (kLocal)
+    jac_cam_local(o, y, wgt, Jc);
+  else
+    jac_cam(o, dR, X, wgt, one, Jc);
   const T rx = wgt * o.rx, ry = wgt * o.ry;
   T tx = rx, ty = ry;
   if (with_h) {
@@ -1006,27 +1162,36 @@ __global__ __launch_bounds__(B) void k_cam_pass(
     const int32_t* __restrict__ chunk_end, const int32_t* __restrict__ f_to_cam,
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
     const T* __restrict__ ctab, T huber_a, int with_h, T* __restrict__ partial, Win win,
-    const int32_t* __restrict__ slot_chunk, const T* __restrict__ cres, T* __restrict__ wcm) {
+    const int32_t* __restrict__ slot_chunk, const T* __restrict__ cres, T* __restrict__ wcm,
+    const T* __restrict__ gtab) {
+  constexpr bool kLocal = std::is_same<T, float>::value;
   __shared__ T lds[63 * (B / 64)];
   const int ch = chunk_of_slot(slot_chunk);
   if (ch < 0) return;
   const int f = chunk_f[ch];
   if (win_idle_f(win, f)) return;
-  const T* tab = ctab + (size_t)kCamTab * f_to_cam[f];
+  const int cam = f_to_cam[f];
+  const T* tab = ctab + (size_t)kCamTab * cam;
   CamProj<T> c;
   load_proj(tab, c);
   T dR[27];
+  if constexpr (!kLocal) {
 #pragma unroll
-  for (int i = 0; i < 27; ++i) dR[i] = tab[kTabDR + i];
+    for (int i = 0; i < 27; ++i) dR[i] = tab[kTabDR + i];
+  }
   T acc[63];
 #pragma unroll
   for (int i = 0; i < 63; ++i) acc[i] = T(0);
   const int end = chunk_end[ch];
   int i = chunk_begin[ch] + threadIdx.x;
   // cres (optional): residuals evaluated in FP64 at this state, camera-major,
-  // used in place of the kernel-precision pred - obs.
+  // used in place of the kernel-precision pred - obs, and then wcm holds
+  // their weights; otherwise the weights are computed here and written to
+  // wcm (if any) for the matvecs.
   const bool given = cres != nullptr;
   const T* ob = given ? cres : cm_obs;
+  const T* wgi = given ? wcm : nullptr;
+  T* wout = given ? nullptr : wcm;
   // Two rows per iteration: both rows' index, observation and point-record
   // loads are issued before either row is computed.
   for (; i + B < end; i += 2 * B) {
@@ -1036,21 +1201,25 @@ __global__ __launch_bounds__(B) void k_cam_pass(
     T X0[3], s0[3], X1[3], s1[3];
     load_pk(pk, e0, X0, s0);
     load_pk(pk, e1, X1, s1);
-    const T w0 = cam_row_accumulate(c, dR, X0, s0, o0x, o0y, given, huber_a, with_h, acc);
-    const T w1 = cam_row_accumulate(c, dR, X1, s1, o1x, o1y, given, huber_a, with_h, acc);
-    if (wcm) {
-      wcm[i] = w0;
-      wcm[i + B] = w1;
+    const T w0 = cam_row_accumulate(c, dR, X0, s0, o0x, o0y, given, wgi ? wgi + i : nullptr, huber_a, with_h, acc);
+    const T w1 =
+        cam_row_accumulate(c, dR, X1, s1, o1x, o1y, given, wgi ? wgi + i + B : nullptr, huber_a, with_h, acc);
+    if (wout) {
+      wout[i] = w0;
+      wout[i + B] = w1;
     }
   }
   if (i < end) {
     T X0[3], s0[3];
     load_pk(pk, cm_e[i], X0, s0);
-    const T w0 =
-        cam_row_accumulate(c, dR, X0, s0, ob[2 * (size_t)i], ob[2 * (size_t)i + 1], given, huber_a, with_h, acc);
-    if (wcm) wcm[i] = w0;
+    const T w0 = cam_row_accumulate(c, dR, X0, s0, ob[2 * (size_t)i], ob[2 * (size_t)i + 1], given,
+                                    wgi ? wgi + i : nullptr, huber_a, with_h, acc);
+    if (wout) wout[i] = w0;
   }
-  const T s = block_reduce<T, 63, B>(acc, lds);
+  T s = block_reduce<T, 63, B>(acc, lds);
+  if constexpr (kLocal) {
+    if (threadIdx.x < 64) s = chunk_to_param_basis63(s, (int)threadIdx.x, gtab + 9 * (size_t)cam);
+  }
   if (threadIdx.x < 63) partial[(size_t)kPartial * ch + threadIdx.x] = s;
 }
 
@@ -1382,46 +1551,79 @@ __global__ __launch_bounds__(B) void k_mv_cam(
     const int32_t* __restrict__ cm_e, const T* __restrict__ cm_obs, const T* __restrict__ pk,
     const T* __restrict__ cscale, const T* __restrict__ ctab, T huber_a, T* __restrict__ partial,
     const int32_t* __restrict__ all_done, Win win, const int32_t* __restrict__ slot_chunk,
-    const T* __restrict__ wcm) {
+    const T* __restrict__ wcm, const T* __restrict__ gtab) {
+  // FP32: rows summed in the local rotation basis (jac_cam_local), taken to
+  // the parameter basis and scaled once per chunk.
+  constexpr bool kLocal = std::is_same<T, float>::value;
   __shared__ T lds[9 * (B / 64)];
   if (all_done && *all_done) return;
   const int ch = chunk_of_slot(slot_chunk);
   if (ch < 0) return;
   const int f = chunk_f[ch];
   if (win_idle_f(win, f)) return;
-  const T* tab = ctab + (size_t)kCamTab * f_to_cam[f];
+  const int cam = f_to_cam[f];
+  const T* tab = ctab + (size_t)kCamTab * cam;
   CamProj<T> c;
   load_proj(tab, c);
   T dR[27];
-#pragma unroll
-  for (int i = 0; i < 27; ++i) dR[i] = tab[kTabDR + i];
   T cs[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) cs[i] = cscale[9 * (size_t)f + i];
+  if constexpr (!kLocal) {
+#pragma unroll
+    for (int i = 0; i < 27; ++i) dR[i] = tab[kTabDR + i];
+  }
   T acc[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) acc[i] = T(0);
   const int end = chunk_end[ch];
-  // wcm (optional): the row's loss weight from the last pass.
+  // wcm (optional): the row's loss weight from the last pass.  Without it
+  // and with a loss the weight needs the residual, hence the observation.
+  const bool need_obs = !wcm && huber_a > T(0);
   auto row = [&](int i, const T X[3], const T sz[3], T ox, T oy) {
     Proj<T> o;
-    project(c, X, ox, oy, o);
-    const T wgt = wcm ? wcm[i] : huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
-    T Jc[18];
-    jac_cam(o, dR, X, wgt, cs, Jc);
-    T u[2];
-    jac_point_apply(o, c.R, sz, u);
-    u[0] *= wgt;
-    u[1] *= wgt;
+    if constexpr (kLocal) {
+      T y[3];
+      project_y(c, X, ox, oy, o, y);
+      const T wgt = wcm ? wcm[i] : huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+      T u[2];
+      jac_point_apply(o, c.R, sz, u);
+      const T w2 = wgt * wgt;
+      const T s0 = w2 * u[0], s1 = w2 * u[1];
+      const T v0 = s0 * o.P[0] + s1 * o.P[3], v1 = s0 * o.P[1] + s1 * o.P[4], v2 = s0 * o.P[2] + s1 * o.P[5];
+      acc[0] += y[1] * v2 - y[2] * v1;
+      acc[1] += y[2] * v0 - y[0] * v2;
+      acc[2] += y[0] * v1 - y[1] * v0;
+      acc[3] += v0;
+      acc[4] += v1;
+      acc[5] += v2;
+      fma2(acc[6], s0, o.jf[0], s1, o.jf[1]);
+      fma2(acc[7], s0, o.jk1[0], s1, o.jk1[1]);
+      fma2(acc[8], s0, o.jk2[0], s1, o.jk2[1]);
+    } else {
+      project(c, X, ox, oy, o);
+      const T wgt = wcm ? wcm[i] : huber_weight(o.rx * o.rx + o.ry * o.ry, huber_a);
+      T Jc[18];
+      jac_cam(o, dR, X, wgt, cs, Jc);
+      T u[2];
+      jac_point_apply(o, c.R, sz, u);
+      u[0] *= wgt;
+      u[1] *= wgt;
 #pragma unroll
-    for (int a = 0; a < 9; ++a) fma2(acc[a], Jc[a], u[0], Jc[9 + a], u[1]);
+      for (int a = 0; a < 9; ++a) fma2(acc[a], Jc[a], u[0], Jc[9 + a], u[1]);
+    }
   };
   int i = chunk_begin[ch] + threadIdx.x;
   // Two rows per iteration: both rows' point-record gathers in flight.
   for (; i + B < end; i += 2 * B) {
     const int e0 = cm_e[i], e1 = cm_e[i + B];
-    const T o0x = cm_obs[2 * (size_t)i], o0y = cm_obs[2 * (size_t)i + 1];
-    const T o1x = cm_obs[2 * (size_t)(i + B)], o1y = cm_obs[2 * (size_t)(i + B) + 1];
+    T o0x = T(0), o0y = T(0), o1x = T(0), o1y = T(0);
+    if (need_obs) {
+      o0x = cm_obs[2 * (size_t)i];
+      o0y = cm_obs[2 * (size_t)i + 1];
+      o1x = cm_obs[2 * (size_t)(i + B)];
+      o1y = cm_obs[2 * (size_t)(i + B) + 1];
+    }
     T X0[3], s0[3], X1[3], s1[3];
     load_pk(pk, e0, X0, s0);
     load_pk(pk, e1, X1, s1);
@@ -1431,9 +1633,12 @@ __global__ __launch_bounds__(B) void k_mv_cam(
   if (i < end) {
     T X0[3], s0[3];
     load_pk(pk, cm_e[i], X0, s0);
-    row(i, X0, s0, cm_obs[2 * (size_t)i], cm_obs[2 * (size_t)i + 1]);
+    row(i, X0, s0, need_obs ? cm_obs[2 * (size_t)i] : T(0), need_obs ? cm_obs[2 * (size_t)i + 1] : T(0));
   }
-  const T s = block_reduce<T, 9, B>(acc, lds);
+  T s = block_reduce<T, 9, B>(acc, lds);
+  if constexpr (kLocal) {
+    if (threadIdx.x < 64) s = chunk_to_param_basis9(s, (int)threadIdx.x, gtab + 9 * (size_t)cam, cs);
+  }
   if (threadIdx.x < 9) partial[(size_t)kPartial * ch + threadIdx.x] = s;
 }
 

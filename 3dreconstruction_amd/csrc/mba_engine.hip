@@ -28,6 +28,7 @@
 #include "mba_device.h"
 #include "mba_device_grouped.h"
 #include "mba_scene.h"
+#include "mba_tuning.h"
 
 namespace mba {
 namespace {
@@ -880,10 +881,16 @@ struct Dev {
   T* partial_sj = nullptr;
   double* Tsj = nullptr;
   // Packed layout with a loss function: each row's loss weight sqrt(rho')
-  // as the last Jacobian pass formed it, in row order (point pass) and
+  // as the last Jacobian pass used it, in row order (point pass) and
   // camera-major order (camera pass); the matvecs and the SCHUR_JACOBI pass
   // read them instead of re-deriving them from kernel-precision residuals.
-  T *wrow = nullptr, *wcm = nullptr;
+  // Mixed single problems keep a second slot (wrow1, wcm1), written with
+  // the FP64 residuals of a trial state, as rres / cres are.
+  T *wrow = nullptr, *wcm = nullptr, *wrow1 = nullptr, *wcm1 = nullptr;
+  // FP32 packed layout: per camera the left Jacobian G = J_l(aa) (9) that
+  // takes the camera-major kernels' local rotation basis to the
+  // parameters.
+  T* gtab = nullptr;
 };
 
 constexpr size_t kBigChunk = (size_t)8 << 20;  // pinned staging buffers for large uploads (two)
@@ -943,6 +950,9 @@ struct mba_context {
   hipStream_t stream = nullptr;
   Pool pool;
   int32_t nc = 0, np = 0, ne = 0, nf = 0, nrows = 0, nchunks = 0, ncm = 0;
+  // Launch shapes in use (options or the tuned defaults): rows per lane of
+  // the point-major kernels, rows per camera-major chunk (0: by size).
+  int32_t rows_per_lane = 4, chunk_rows = 0;
   // FP64 state and LM data
   double *cams64 = nullptr, *cams64_t = nullptr, *pts64 = nullptr, *pts64_t = nullptr;
   double *ctab64 = nullptr, *row_obs64 = nullptr, *pscale64 = nullptr, *cscale64 = nullptr;
@@ -1147,7 +1157,13 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_st
   if (c->opt.huber_a > 0.0 && !c->grouped && c->nrows > 0) {
     MBA_TRY(c->pool.alloc(&d.wrow, (size_t)c->nrows));
     MBA_TRY(c->pool.alloc(&d.wcm, (size_t)std::max(1, c->ncm)));
+    if (c->rfix) {
+      MBA_TRY(c->pool.alloc(&d.wrow1, (size_t)c->nrows));
+      MBA_TRY(c->pool.alloc(&d.wcm1, (size_t)std::max(1, c->ncm)));
+    }
   }
+  if constexpr (std::is_same<T, float>::value)
+    if (!c->grouped) MBA_TRY(c->pool.alloc(&d.gtab, 9 * std::max<size_t>(1, nc)));
   return MBA_OK;
 }
 
@@ -1424,6 +1440,10 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     return fail(MBA_ERR_INVALID_ARGUMENT, "unknown preconditioner " + std::to_string(c->opt.preconditioner));
   if (c->opt.num_dump_iterations < 0 || (c->opt.num_dump_iterations > 0 && !c->opt.dump_iterations))
     return fail(MBA_ERR_INVALID_ARGUMENT, "bad dump_iterations");
+  if (c->opt.point_rows_per_lane < 0 || c->opt.point_rows_per_lane > 64)
+    return fail(MBA_ERR_INVALID_ARGUMENT, "point_rows_per_lane must be 0 (tuned) or 1..64");
+  if (c->opt.camera_chunk_rows != 0 && (c->opt.camera_chunk_rows < 64 || c->opt.camera_chunk_rows > 8192))
+    return fail(MBA_ERR_INVALID_ARGUMENT, "camera_chunk_rows must be 0 (tuned) or 64..8192");
   if (c->opt.num_dump_iterations > 0) {
     if (!c->opt.dump_directory || !c->opt.dump_directory[0])
       return fail(MBA_ERR_INVALID_ARGUMENT, "Solver::Options::trust_region_problem_dump_directory is empty.");
@@ -1508,6 +1528,11 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   c->nf = c->f_off[W];
   c->nrows = (int32_t)nrows;
   c->grouped = probs[0].layout == MBA_LAYOUT_GROUPED;
+  {
+    const TunedShape ts = tuned_shape(nrows);
+    c->rows_per_lane = c->opt.point_rows_per_lane > 0 ? c->opt.point_rows_per_lane : ts.point_rows_per_lane;
+    c->chunk_rows = c->opt.camera_chunk_rows > 0 ? (c->opt.camera_chunk_rows + 63) / 64 * 64 : ts.camera_chunk_rows;
+  }
   // An explicit S assembled in FP32 loses the weakly determined directions
   // of a grouped problem (shared intrinsics with nearly collinear k1..k3
   // columns) to cancellation in U - W Vinv W', which FP64 refinement against
@@ -1537,43 +1562,55 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     const int32_t e = ge_local[ge];
     return st.pt_start[e + 1] - st.pt_start[e];
   };
-  // Device point order: global E order stably grouped by lane class, so
-  // each point-major launch segment is contiguous.
+  // Device point order: global E order stably sorted by track length, so
+  // each lane class (a point-major launch segment) is contiguous and the
+  // points sharing a wave share their rows' trip count.
   std::vector<int32_t> dperm(c->ne), dev_pt_start((size_t)c->ne + 1, 0);
   {
-    // Stable counting sort by class, per-thread counts over E ranges.
-    Buf<uint8_t> cls(c->ne);
+    // Stable counting sort by track length, per-thread counts over E ranges.
+    Buf<int32_t> tlv(c->ne);
     const int nt = host_threads(c->ne, 1 << 14);
     const int64_t step = ((int64_t)c->ne + nt - 1) / nt;
-    std::vector<std::array<int32_t, 8>> tc(nt);
+    std::vector<int32_t> tmax(nt, 0);
     run_threads(nt, [&](int t) {
-      tc[t].fill(0);
       const int64_t end = std::min<int64_t>(c->ne, (t + 1) * step);
       for (int64_t e = t * step; e < end; ++e) {
-        const int tl = track_len((int32_t)e);
-        int k = 0;
-        while (k < 6 && tl > 4 * (1 << k)) ++k;
-        cls[e] = (uint8_t)k;
-        tc[t][k]++;
+        tlv[e] = track_len((int32_t)e);
+        tmax[t] = std::max(tmax[t], tlv[e]);
       }
     });
-    int32_t ncls[8] = {0}, start[8] = {0};
-    for (int t = 0; t < nt; ++t)
-      for (int k = 0; k < 8; ++k) ncls[k] += tc[t][k];
-    for (int k = 1; k < 8; ++k) start[k] = start[k - 1] + ncls[k - 1];
-    {
-      int32_t run[8];
-      std::memcpy(run, start, sizeof(run));
-      for (int t = 0; t < nt; ++t)
-        for (int k = 0; k < 8; ++k) {
-          const int32_t n = tc[t][k];
-          tc[t][k] = run[k];
-          run[k] += n;
-        }
-    }
+    const int32_t nkey = *std::max_element(tmax.begin(), tmax.end()) + 1;
+    std::vector<int32_t> tc((size_t)nt * nkey, 0);
     run_threads(nt, [&](int t) {
       const int64_t end = std::min<int64_t>(c->ne, (t + 1) * step);
-      for (int64_t e = t * step; e < end; ++e) dperm[tc[t][cls[e]]++] = (int32_t)e;
+      int32_t* h = tc.data() + (size_t)t * nkey;
+      for (int64_t e = t * step; e < end; ++e) h[tlv[e]]++;
+    });
+    // Lane class of a track length: 1 lane up to 4 rows, 2 up to 8, ...,
+    // 64 beyond 128.
+    const int32_t rpl = c->rows_per_lane;
+    auto cls_of = [rpl](int32_t tl) {
+      int k = 0;
+      while (k < 6 && tl > rpl * (1 << k)) ++k;
+      return k;
+    };
+    int32_t ncls[8] = {0}, start[8] = {0};
+    {
+      int32_t run = 0;
+      for (int32_t key = 0; key < nkey; ++key)
+        for (int t = 0; t < nt; ++t) {
+          int32_t& v = tc[(size_t)t * nkey + key];
+          const int32_t n = v;
+          v = run;
+          run += n;
+          ncls[cls_of(key)] += n;
+        }
+    }
+    for (int k = 1; k < 8; ++k) start[k] = start[k - 1] + ncls[k - 1];
+    run_threads(nt, [&](int t) {
+      const int64_t end = std::min<int64_t>(c->ne, (t + 1) * step);
+      int32_t* h = tc.data() + (size_t)t * nkey;
+      for (int64_t e = t * step; e < end; ++e) dperm[h[tlv[e]]++] = (int32_t)e;
     });
     c->seg.n = 0;
     int32_t blk = 0;
@@ -1727,8 +1764,9 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     range_row[kXcd] = c->nrows;
   }
   const int nq = xcd_split ? kXcd : 1;
-  // Split chunks run as one-wave workgroups of kXcdChunkRows rows.
-  const int crq = xcd_split ? kXcdChunkRows : cr;
+  // Split chunks run as one-wave workgroups of (by default) kXcdChunkRows
+  // rows; an explicit or tuned chunk size applies to either layout.
+  const int crq = c->chunk_rows > 0 ? c->chunk_rows : (xcd_split ? kXcdChunkRows : cr);
   // Two parallel passes over the keys (the binary searches in cm_src are
   // cache misses: serially they cost ~10 ms at 6.8M rows): the range split
   // points and chunk counts of each key, then the chunks themselves.
@@ -2133,7 +2171,15 @@ struct Engine {
     else
       hipLaunchKernelGGL(k_cam_table<X>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, tab);
   }
-  void tables(const double* cams) { table_into<T>(cams, d.ctab); }
+  void tables(const double* cams) {
+    table_into<T>(cams, d.ctab);
+    if (d.gtab && c->nc > 0)
+      hipLaunchKernelGGL(k_cam_lj<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, d.gtab);
+  }
+  // Row weights of the current state (slot rcur for mixed single problems).
+  int wslot() const { return c->rfix ? c->rcur : 0; }
+  T* wrow_cur() const { return wslot() ? d.wrow1 : d.wrow; }
+  T* wcm_cur() const { return wslot() ? d.wcm1 : d.wcm; }
 
   // FP64 cost per window at (cams, pts) into sum slot kSlotCost and its
   // non-finite flags into flag slot 1; cost_result reads them after a sync.
@@ -2151,7 +2197,8 @@ struct Engine {
       hipLaunchKernelGGL((c->grouped ? k_cost64<1> : k_cost64<0>), dim3(nb), dim3(kBlock), 0, s, c->seg, d.pt_start,
                          d.row_cam, c->row_obs64, pts, c->ctab64, c->opt.huber_a, c->red, win(c->wacc),
                          wr ? c->rres[slot] : nullptr, wr ? c->cres[slot] : nullptr,
-                         wr ? (const int32_t*)c->row_cm : nullptr);
+                         wr ? (const int32_t*)c->row_cm : nullptr, wr ? (slot ? c->df.wrow1 : c->df.wrow) : nullptr,
+                         wr ? (slot ? c->df.wcm1 : c->df.wcm) : nullptr);
     MBA_TRY(sum_async(nb, kSlotCost));
     MBA_TRY(flags_async(1));
     return clear_flags();
@@ -2181,6 +2228,15 @@ struct Engine {
     else
       return nullptr;
   }
+  // The kernel-precision inverses of the point blocks, written only where
+  // they are read: FP64 kernels (they are the FP64 inverses) and the
+  // explicit solver's S assembly.
+  T* vinv_out() const {
+    if constexpr (std::is_same<T, float>::value)
+      return c->opt.linear_solver == MBA_EXPLICIT_SCHUR ? d.Vinv : nullptr;
+    else
+      return d.Vinv;
+  }
   // FP64 inverses of the damped point blocks.
   const double* vinv64() const {
     if constexpr (std::is_same<T, double>::value)
@@ -2201,7 +2257,7 @@ struct Engine {
       hipLaunchKernelGGL((c->grouped ? k_point_pass<T, 1> : k_point_pass<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0,
                          s, c->seg, d.pt_start, d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, win(),
                          c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag,
-                         d.V, d.gp, d.Vinv, d.pk, fixed_rows(), c->grouped ? nullptr : d.wrow, d.Vinv64);
+                         d.V, d.gp, vinv_out(), d.pk, fixed_rows(), c->grouped ? nullptr : wrow_cur(), d.Vinv64);
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
     if (c->nchunks > 0) {
       if (c->grouped) {
@@ -2212,7 +2268,7 @@ struct Engine {
         hipLaunchKernelGGL((c->d_slot_chunk ? k_cam_pass<T, 64> : k_cam_pass<T, kBlock>), dim3(c->nslots),
                            dim3(c->d_slot_chunk ? 64 : kBlock), 0, s, d.chunk_f, d.chunk_begin,
                            d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.ctab, huber, 1, d.partial, win(),
-                           (const int32_t*)c->d_slot_chunk, fixed_cm(), d.wcm);
+                           (const int32_t*)c->d_slot_chunk, fixed_cm(), wcm_cur(), (const T*)d.gtab);
     }
     sj_terms();
     if (c->timing) (void)hipEventRecord(c->ev[2], s);
@@ -2232,7 +2288,7 @@ struct Engine {
                          dim3(c->d_slot_chunk ? 64 : kBlock), 0, s, d.chunk_f, d.chunk_begin, d.chunk_end,
                          d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.pscale, vinv64(), d.ctab, huber,
                          d.partial_sj, win(),
-                         (const int32_t*)c->d_slot_chunk, (const T*)d.wcm);
+                         (const int32_t*)c->d_slot_chunk, (const T*)wcm_cur());
     hipLaunchKernelGGL(k_sj_finalize<T>, dim3(nblocks(45 * (int64_t)c->nf)), dim3(kBlock), 0, s, c->nf, d.f_lo,
                        d.f_hi, (const T*)d.partial_sj, (const T*)d.cscale, win(), d.Tsj);
   }
@@ -2240,7 +2296,7 @@ struct Engine {
   // Re-damp after a rejected step (radii current) and rebuild the rhs.
   void damp() {
     hipLaunchKernelGGL(k_point_damp<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, (const double*)d.V, d.gp,
-                       c->diag_p, win(), d.pscale, d.Vinv, d.pk, d.Vinv64);
+                       c->diag_p, win(), d.pscale, vinv_out(), d.pk, d.Vinv64);
     if (c->nf > 0) {
       sj_terms();
       hipLaunchKernelGGL(k_cam_damp<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf, d.U, c->diag_c, win(),
@@ -2268,7 +2324,7 @@ struct Engine {
     if (c->seg_blocks == 0) return;
     hipLaunchKernelGGL((c->grouped ? k_mv_point<T, 1> : k_mv_point<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0, s,
                        c->seg, d.pt_start, d.row_cam, d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber,
-                       beta, 1, d.z, pk_out, done, wv, (const T*)d.wrow, beta != T(0) ? fixed_rows() : nullptr,
+                       beta, 1, d.z, pk_out, done, wv, (const T*)wrow_cur(), beta != T(0) ? fixed_rows() : nullptr,
                        vinv64());
   }
   // Camera-major partials: packed, sum Jc'(Jp z); grouped, sum F'(F x - E z)
@@ -2290,7 +2346,8 @@ struct Engine {
       hipLaunchKernelGGL((c->d_slot_chunk ? k_mv_cam<T, 64> : k_mv_cam<T, kBlock>), dim3(c->nslots),
                          dim3(c->d_slot_chunk ? 64 : kBlock), 0, s, d.chunk_f, d.chunk_begin,
                          d.chunk_end, d.f_to_cam, d.cm_e, d.cm_obs, d.pk, d.cscale, d.ctab, huber, d.partial,
-                         done, wv ? *wv : win(), (const int32_t*)c->d_slot_chunk, (const T*)d.wcm);
+                         done, wv ? *wv : win(), (const int32_t*)c->d_slot_chunk, (const T*)wcm_cur(),
+                         (const T*)d.gtab);
   }
 
   // y = S x (implicit), x and y F-order.  Inside PCG (`done` set) the

@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define MBA_ABI_VERSION 3
+#define MBA_ABI_VERSION 4
 
 typedef enum mba_status {
   MBA_OK = 0,
@@ -211,7 +211,16 @@ typedef struct mba_options {
   const char* dump_directory;                 /* NULL */
   const int32_t* dump_iterations;             /* NULL */
   int32_t num_dump_iterations;                /* 0 */
-  int32_t reserved2;
+  /* Launch shapes (no counterpart in the reference; 0 = the library's
+   * tuned default for the problem's size, from its autotuning table).
+   * point_rows_per_lane: rows each lane of the point-major kernels takes
+   * (a point of track length t gets the smallest power-of-two lane count
+   * L <= 64 with t <= point_rows_per_lane * L); 1..64.
+   * camera_chunk_rows: rows per camera-major chunk (one workgroup's
+   * partial sums), rounded up to a multiple of 64; 64..8192. */
+  int32_t point_rows_per_lane;                /* 0 */
+  int32_t camera_chunk_rows;                  /* 0 */
+  int32_t reserved3;
 } mba_options;
 
 typedef struct mba_iteration {

@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol(mba):
     for name in syms:
         assert hasattr(lib, name), name
     assert set(syms) == set(mba.EXPORTS)
-    assert lib.mba_abi_version() == 3
+    assert lib.mba_abi_version() == 4
     # mba_problem: 4 int32, 1 int64, 10 pointers, 2 int32 (x86-64 layout).
     assert ctypes.sizeof(mba._Problem) == 4 * 4 + 8 + 10 * 8 + 2 * 4
 
@@ -37,7 +37,7 @@ def test_struct_layouts_match_the_header(mba, tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     checks = {"mba_options": (mba.Options, ["eta", "preconditioner", "dump_directory", "dump_iterations",
-                                            "num_dump_iterations"]),
+                                            "num_dump_iterations", "point_rows_per_lane", "camera_chunk_rows"]),
               "mba_problem": (mba._Problem, []), "mba_summary": (mba._Summary, ["num_linear_solver_iterations"]),
               "mba_iteration": (mba.Iteration, ["cumulative_time_s"])}
     src = ["#include <stdio.h>", "#include <stddef.h>", '#include "mba.h"', "int main(void) {"]

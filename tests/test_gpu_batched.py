@@ -82,8 +82,10 @@ def test_explicit_batch_shares_launches(mba, O, huber_scene, huber):
     30-camera scene holds one camera constant, which leaves the scale gauge
     free: S is singular along it but for the LM damping, so FP64 solvers
     that round differently (the batch factors S by column steps, a lone
-    solve by blocked MFMA steps) drift apart along it by up to ~1e-6 while
-    the cost agrees; parameters are compared to the oracle's within 2e-6."""
+    solve by blocked MFMA steps) drift apart along it by a few 1e-6 while
+    the cost agrees.  So the gauge-invariant check is on the reprojections:
+    every residual at the batch's and the lone solution equals the oracle's
+    to 1e-6 px; raw parameters are compared within 5e-6."""
     const = np.zeros(30, np.uint8)
     const[3] = 1
     scenes = [(huber_scene, const)] + [(d, None) for d in windows(mba, 3, first=7)]
@@ -102,9 +104,15 @@ def test_explicit_batch_shares_launches(mba, O, huber_scene, huber):
                (so.num_successful_steps, so.num_unsuccessful_steps, so.termination_type), w
         print(f"window {w}: cameras batch - oracle {np.max(np.abs(probs[w].cameras - sc.cameras)):.1e}, "
               f"lone - oracle {np.max(np.abs(lone.cameras - sc.cameras)):.1e}", flush=True)
-        np.testing.assert_allclose(probs[w].cameras, sc.cameras, rtol=0, atol=2e-6)
-        np.testing.assert_allclose(lone.cameras, sc.cameras, rtol=0, atol=2e-6)
-        np.testing.assert_allclose(probs[w].points, sc.points, rtol=1e-6, atol=2e-6)
+        ro = O.evaluate(O.Scene(dict(d, cam_init=sc.cameras, pt_init=sc.points), cam_constant=k),
+                        jacobians=False)[0]
+        for sol in (probs[w], lone):
+            r = O.evaluate(O.Scene(dict(d, cam_init=sol.cameras, pt_init=sol.points), cam_constant=k),
+                           jacobians=False)[0]
+            assert np.max(np.abs(r - ro)) < 1e-6, (w, np.max(np.abs(r - ro)))
+        np.testing.assert_allclose(probs[w].cameras, sc.cameras, rtol=0, atol=5e-6)
+        np.testing.assert_allclose(lone.cameras, sc.cameras, rtol=0, atol=5e-6)
+        np.testing.assert_allclose(probs[w].points, sc.points, rtol=1e-6, atol=5e-6)
 
 
 def test_explicit_batch_precision_and_large_windows(mba, O):

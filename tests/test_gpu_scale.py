@@ -255,3 +255,39 @@ def test_fp64_lm_pass_equals_plain_pass(mba, O, c1):
         np.testing.assert_array_equal(a[k], b[k])
     _, _, _, costo = O.evaluate(O.Scene(c1), huber_a=4.0)
     np.testing.assert_allclose(cost, costo, rtol=1e-12)
+
+
+# ------------------------------------------------------ launch shapes --
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-5)])
+def test_launch_shapes_give_the_same_blocks(mba, O, prec, tol):
+    """The autotuner's knobs change only how work is laid out: rows per lane
+    of the point-major kernels (1, 8) and rows per camera-major chunk (256,
+    1536) give the blocks of the default shape on the 697k-observation scene
+    with Huber 4 px, to summation-order rounding (FP64: 1e-12 relative per
+    block in Frobenius norm; mixed: 2e-5), and the same matvec."""
+    from test_gpu_parity import block_err
+    d = O.generate(O.preset(2))
+    out = []
+    for rpl, cr in ((0, 0), (1, 256), (8, 1536)):
+        o = mba.default_options(precision=prec, huber_a=4.0, linear_solver=mba.IMPLICIT_SCHUR_PCG,
+                                point_rows_per_lane=rpl, camera_chunk_rows=cr)
+        with mba.Context(ba_problem(mba, d), o) as ctx:
+            ctx.jacobian_schur(1e4)
+            blk = ctx.read_schur()
+            x = np.random.default_rng(1).standard_normal((d["cam_init"].shape[0], 9))
+            blk["Sx"] = ctx.schur_matvec(x)
+            out.append(blk)
+    for other in out[1:]:
+        errs = {"U": block_err(other["U"], out[0]["U"], 9), "V": block_err(other["V"], out[0]["V"], 3)}
+        for k in ("gc", "gp", "rhs", "Sx"):
+            errs[k] = float(np.max(np.abs(other[k] - out[0][k])) / np.max(np.abs(out[0][k])))
+        print(f"launch shapes, precision {prec}: " + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()), flush=True)
+        assert max(errs.values()) < tol, errs
+
+
+def test_launch_shape_options_are_checked(mba, c1):
+    for kw in ({"point_rows_per_lane": 65}, {"point_rows_per_lane": -1}, {"camera_chunk_rows": 32},
+               {"camera_chunk_rows": 9000}):
+        with pytest.raises(mba.MbaError) as e:
+            mba.Context(ba_problem(mba, c1), mba.default_options(**kw))
+        assert e.value.status == mba.ERR_INVALID_ARGUMENT
