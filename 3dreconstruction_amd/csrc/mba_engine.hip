@@ -993,6 +993,12 @@ struct mba_context {
   // (s_wtotal entries in all), window F-block offsets on the device.
   int64_t* d_woff = nullptr;
   int32_t* d_f_off = nullptr;
+  // Batched PCG in one persistent launch (k_pcg_window): each window's
+  // device E blocks (CSR over windows), its camera range, and each F
+  // block's camera-major rows.
+  bool pcg_window = false;
+  int32_t *d_win_e_lo = nullptr, *d_win_e = nullptr, *d_cam_off = nullptr, *d_win_order = nullptr;
+  int32_t *d_fcm_lo = nullptr, *d_fcm_hi = nullptr;
   int64_t s_wtotal = 0;
   // Host copies of what was last uploaded.
   std::vector<double> h_inv_r;
@@ -1999,6 +2005,41 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     MBA_TRY(alloc_precision<float>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_lo, f_hi));
   else
     MBA_TRY(alloc_precision<double>(c, dev_pt_start, row_cam, cm_src, chunk_f, chunk_begin, chunk_end, f_lo, f_hi));
+  // Batched PCG, packed windows of at most kPcgWinMaxCams cameras: one
+  // workgroup per window runs the window's whole PCG in one launch.
+  if (W > 1 && !c->grouped && c->opt.linear_solver == MBA_IMPLICIT_SCHUR_PCG && c->nf > 0) {
+    bool fits = true;
+    for (int32_t w = 0; w < W && fits; ++w) fits = c->cam_off[w + 1] - c->cam_off[w] <= kPcgWinMaxCams;
+    if (fits) {
+      std::vector<int32_t> cnt((size_t)W + 1, 0), list(c->ne);
+      for (int32_t de = 0; de < c->ne; ++de) cnt[ge_win[dperm[de]] + 1]++;
+      for (int32_t w = 0; w < W; ++w) cnt[w + 1] += cnt[w];
+      std::vector<int32_t> run(cnt.begin(), cnt.end() - 1);
+      for (int32_t de = 0; de < c->ne; ++de) list[run[ge_win[dperm[de]]]++] = de;
+      std::vector<int32_t> lo(c->nf, 0), hi(c->nf, 0);
+      for (int32_t f = 0; f < c->nf; ++f)
+        if (f_hi[f] > f_lo[f]) {
+          lo[f] = chunk_begin[f_lo[f]];
+          hi[f] = chunk_end[f_hi[f] - 1];
+        }
+      // Workgroups take the windows with the most rows first.
+      std::vector<int32_t> order(W);
+      std::vector<int64_t> wrows(W, 0);
+      for (int32_t w = 0; w < W; ++w) {
+        order[w] = w;
+        for (int32_t i = cnt[w]; i < cnt[w + 1]; ++i) wrows[w] += dev_pt_start[list[i] + 1] - dev_pt_start[list[i]];
+      }
+      std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return wrows[a] > wrows[b]; });
+      MBA_TRY(upload(c, &c->d_win_order, order));
+      MBA_TRY(upload(c, &c->d_win_e_lo, cnt));
+      MBA_TRY(upload(c, &c->d_win_e, list));
+      MBA_TRY(upload(c, &c->d_cam_off, c->cam_off));
+      MBA_TRY(upload(c, &c->d_fcm_lo, lo));
+      MBA_TRY(upload(c, &c->d_fcm_hi, hi));
+      if (!c->d_f_off) MBA_TRY(upload(c, &c->d_f_off, c->f_off));
+      c->pcg_window = true;
+    }
+  }
   if (c->opt.linear_solver == MBA_EXPLICIT_SCHUR) {
     MBA_TRY(build_s_pairs(c, dev_pt_start, row_cam));
     if (W > 1) {
@@ -2010,7 +2051,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       }
       c->s_wtotal = woff[W];
       MBA_TRY(upload(c, &c->d_woff, woff));
-      MBA_TRY(upload(c, &c->d_f_off, c->f_off));
+      if (!c->d_f_off) MBA_TRY(upload(c, &c->d_f_off, c->f_off));
     }
   }
   trace.mark("precision layout + S pairs");
@@ -2371,6 +2412,47 @@ struct Engine {
   mba_status pcg_async() {
     const int nf = c->nf;
     const int nbg = nblocks((int64_t)kGrp * nf);  // 16 lanes per F block
+    if (c->pcg_window) {
+      PcgWinArgs<T> a;
+      a.order = c->d_win_order;
+      a.win_e_lo = c->d_win_e_lo;
+      a.win_e = c->d_win_e;
+      a.f_off = c->d_f_off;
+      a.cam_off = c->d_cam_off;
+      a.f_to_cam = d.f_to_cam;
+      a.cam_to_f = d.cam_to_f;
+      a.fcm_lo = c->d_fcm_lo;
+      a.fcm_hi = c->d_fcm_hi;
+      a.pt_start = d.pt_start;
+      a.row_cam = d.row_cam;
+      a.row_obs = d.row_obs;
+      a.cm_e = d.cm_e;
+      a.cm_obs = d.cm_obs;
+      a.pk = d.pk;
+      a.pscale = d.pscale;
+      a.cscale = d.cscale;
+      a.ctab = d.ctab;
+      a.Vinv64 = vinv64();
+      a.wrow = wrow_cur();
+      a.wcm = wcm_cur();
+      a.U = d.U;
+      a.Minv = d.Minv;
+      a.diag_c = c->diag_c;
+      a.inv_r = c->inv_r;
+      a.b = d.rhs;
+      a.x = d.x;
+      a.st = c->pcg;
+      a.active = c->wactive;
+      a.huber_a = huber;
+      a.eta = c->opt.eta;
+      a.tol_r = c->opt.pcg_r_tolerance;
+      a.min_iters = c->opt.min_linear_solver_iterations;
+      a.max_iters = c->opt.max_linear_solver_iterations;
+      hipLaunchKernelGGL(k_pcg_window<T>, dim3(W), dim3(pcg_win_threads<T>()), 0, s, a);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(c->h_pcgs, c->pcg, sizeof(PcgState) * W, hipMemcpyDeviceToHost, s));
+      return MBA_OK;
+    }
     *c->h_done = 0;
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kBlock), 0, s, W, c->pcg, (const uint8_t*)(W > 1 ? c->wactive : nullptr),
                        c->opt.pcg_r_tolerance, c->all_done);
