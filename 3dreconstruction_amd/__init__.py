@@ -82,7 +82,7 @@ class Options(C.Structure):
                 ("refinement_iterations", C.c_int32), ("preconditioner", C.c_int32),
                 ("dump_directory", C.c_char_p), ("dump_iterations", C.POINTER(C.c_int32)),
                 ("num_dump_iterations", C.c_int32), ("point_rows_per_lane", C.c_int32),
-                ("camera_chunk_rows", C.c_int32), ("reserved3", C.c_int32)]
+                ("camera_chunk_rows", C.c_int32), ("pcg_graph_launch", C.c_int32)]
 
     def set_dump(self, directory, iterations):
         """trust_region_minimizer_iterations_to_dump / trust_region_problem_dump_directory

@@ -71,17 +71,20 @@ struct SegPos {
   bool valid;
 };
 
-__device__ __forceinline__ SegPos seg_locate(const Seg& sg) {
-  const int b = blockIdx.x;
+// Position of thread `tid` of (virtual) block b of `bdim` threads.
+__device__ __forceinline__ SegPos seg_locate_at(const Seg& sg, int b, int tid, int bdim) {
   int k = 0;
   while (k + 1 < sg.n && b >= sg.blk0[k + 1]) ++k;
   SegPos p;
   p.lanes = sg.lanes[k];
-  const int t = (b - sg.blk0[k]) * blockDim.x + threadIdx.x;
+  const int t = (b - sg.blk0[k]) * bdim + tid;
   p.e = sg.e0[k] + t / p.lanes;
   p.sub = t % p.lanes;
   p.valid = p.e < sg.e1[k];
   return p;
+}
+__device__ __forceinline__ SegPos seg_locate(const Seg& sg) {
+  return seg_locate_at(sg, blockIdx.x, threadIdx.x, blockDim.x);
 }
 
 // acc += a b + c d as two chained FMAs (the sum form would round a b + c d
@@ -99,14 +102,13 @@ __device__ __forceinline__ void fma2(T& acc, T a, T b, T c, T d) {
 struct Win {
   const int32_t* of_e;
   const int32_t* of_f;
-  const double* inv_r;      // 1 / radius per window, or NULL: inv_r0 (single problem)
-  double inv_r0;
+  const double* inv_r;      // 1 / radius per window (device memory, so captured launches stay valid)
   double* acc;
   int32_t* flag;            // per-window "non-finite / failed factorisation" flags
   const uint8_t* active;    // windows still iterating (nullptr: all)
   const PcgState* pcg;      // inside PCG: windows whose solve has stopped are idle
 };
-__device__ __forceinline__ double win_inv_r(const Win& w, int wi) { return w.inv_r ? w.inv_r[wi] : w.inv_r0; }
+__device__ __forceinline__ double win_inv_r(const Win& w, int wi) { return w.inv_r[wi]; }
 __device__ __forceinline__ int win_of_e(const Win& w, int e) { return w.of_e ? w.of_e[e] : 0; }
 __device__ __forceinline__ int win_of_f(const Win& w, int f) { return w.of_f ? w.of_f[f] : 0; }
 __device__ __forceinline__ bool win_idle(const Win& w, int wi) {
@@ -925,6 +927,71 @@ __global__ __launch_bounds__(kBlock) void k_grad_final(const double* __restrict_
   }
 }
 
+// Camera data of a point-major row: projection parameters and the
+// direction table entry, from the global tables ...
+template <typename T, int L>
+struct GlobalCams {
+  const T* __restrict__ ctab;
+  const T* __restrict__ dtab;
+  __device__ __forceinline__ void proj(int cam, CamProj<T>& c) const {
+    load_proj<T, L>(ctab + (size_t)kCamTab * cam, c);
+  }
+  __device__ __forceinline__ const T* dir(int cam) const { return dtab + (size_t)dir_stride<L>() * cam; }
+};
+
+// ... or from records staged in LDS by a persistent launch (FP32, packed
+// cameras): Q 16-byte quads per camera, [R(9) t(3) | f k1 k2 -] and, for
+// Q = 8, the direction (16), 4 Q + 4 floats apart (an odd number of bank
+// quads, so that the 16-byte reads of random cameras spread over the 64
+// banks).  A camera-table gather per row from L2 limits the point-major
+// kernels; an LDS read does not.
+constexpr int kStageThreads = 4 * kBlock;  // four virtual blocks per workgroup
+template <int Q>
+__host__ __device__ constexpr int stage_rec() { return 4 * Q + 4; }
+static_assert(kTabT == 9 && kTabF == 12 && kCamTab % 4 == 0 && kDirTab == 16,
+              "staged records copy R, t, f, k1, k2 and the direction as 16-byte quads");
+
+// Fill the LDS records of cameras [0, nc): 16-byte copies, eight in flight
+// per thread.  Quad k is table quad k (k < 4) or direction quad k - 4.
+typedef float stage_quad __attribute__((ext_vector_type(4)));
+template <int Q>
+__device__ __forceinline__ void stage_cams(float4* __restrict__ lds4, int nc, const float* __restrict__ ctab,
+                                           const float* __restrict__ dtab) {
+  const stage_quad* ct4 = reinterpret_cast<const stage_quad*>(ctab);
+  const stage_quad* dt4 = reinterpret_cast<const stage_quad*>(dtab);
+  stage_quad* lds = reinterpret_cast<stage_quad*>(lds4);
+  const int nq = Q * nc;
+  for (int i0 = 0; i0 < nq; i0 += 8 * (int)blockDim.x) {
+    stage_quad v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x + (int)threadIdx.x, nq - 1);
+      const int cam = i / Q, k = i % Q;
+      v[u] = k < 4 ? ct4[(size_t)(kCamTab / 4) * cam + k] : dt4[(size_t)(kDirTab / 4) * cam + (k - 4)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * (int)blockDim.x + (int)threadIdx.x;
+      if (i < nq) lds[(stage_rec<Q>() / 4) * (i / Q) + (i % Q)] = v[u];
+    }
+  }
+  __syncthreads();
+}
+
+template <int Q>
+struct LdsCams {
+  const float* rec;
+  __device__ __forceinline__ void proj(int cam, CamProj<float>& c) const {
+    const float4* q = reinterpret_cast<const float4*>(rec + stage_rec<Q>() * cam);
+    const float4 a = q[0], b = q[1], d = q[2], e = q[3];
+    c.R[0] = a.x; c.R[1] = a.y; c.R[2] = a.z; c.R[3] = a.w;
+    c.R[4] = b.x; c.R[5] = b.y; c.R[6] = b.z; c.R[7] = b.w;
+    c.R[8] = d.x; c.t[0] = d.y; c.t[1] = d.z; c.t[2] = d.w;
+    c.f = e.x; c.k1 = e.y; c.k2 = e.z;
+  }
+  __device__ __forceinline__ const float* dir(int cam) const { return rec + stage_rec<Q>() * cam + 16; }
+};
+
 // ------------------------------------------------------------- point pass
 // V = sum Jp'Jp, g_p = sum Jp' r; LM diagonal of points; Vinv = (V + D^2)^-1
 // in FP64; w = Vinv g_p.
@@ -932,12 +999,12 @@ __global__ __launch_bounds__(kBlock) void k_grad_final(const double* __restrict_
 // (up to FP64 rounding) exactly the sum of the rows' J_p'J_p, so V - J_p'J_p
 // stays positive semidefinite for every row and the FP64 inverse never meets
 // a block that FP32 accumulation rounded to indefinite.
-template <typename T, int L>
-__device__ __forceinline__ T point_row_accumulate(const T* __restrict__ ctab, int cam, T ox, T oy,
+template <typename T, int L, typename Cams>
+__device__ __forceinline__ T point_row_accumulate(const Cams& cams, int cam, T ox, T oy,
                                                   const T* __restrict__ rf, const T X[3], const T sp[3],
                                                   T huber_a, double v[6], T g[3], const T* wg = nullptr) {
   CamProj<T> c;
-  load_proj<T, L>(ctab + (size_t)kCamTab * cam, c);
+  cams.proj(cam, c);
   Proj<T> o;
   project<T, L>(c, X, ox, oy, o);
   if (rf) {
@@ -964,17 +1031,15 @@ __device__ __forceinline__ T point_row_accumulate(const T* __restrict__ ctab, in
   return wgt;
 }
 
-template <typename T, int L>
-__global__ __launch_bounds__(kBlock) void k_point_pass(
-    Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
-    const T* __restrict__ row_obs, const double* __restrict__ pts64, const T* __restrict__ pscale,
-    const T* __restrict__ ctab, T huber_a, Win win, double min_diag, double max_diag,
-    double* __restrict__ diag_p, int recompute_diag, double* __restrict__ V, T* __restrict__ gp,
-    T* __restrict__ Vinv, T* __restrict__ pk, const T* __restrict__ rres, T* __restrict__ wrow,
-    double* __restrict__ Vinv64) {
-  SegPos q = seg_locate(sg);
+template <typename T, int L, typename Cams>
+__device__ __forceinline__ void point_pass_block(
+    SegPos q, const Cams& cams, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const T* __restrict__ row_obs, const double* __restrict__ pts64, const T* __restrict__ pscale, T huber_a,
+    const Win& win, double min_diag, double max_diag, double* __restrict__ diag_p, int recompute_diag,
+    double* __restrict__ V, T* __restrict__ gp, T* __restrict__ Vinv, T* __restrict__ pk,
+    const T* __restrict__ rres, T* __restrict__ wrow, double* __restrict__ Vinv64, int e_fallback) {
   if (q.valid && win_idle_e(win, q.e)) q.valid = false;
-  const int e = q.valid ? q.e : sg.e0[0];
+  const int e = q.valid ? q.e : e_fallback;
   const T X[3] = {(T)pts64[3 * (size_t)e], (T)pts64[3 * (size_t)e + 1], (T)pts64[3 * (size_t)e + 2]};
   const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
   double v[6] = {0, 0, 0, 0, 0, 0};
@@ -998,12 +1063,12 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
       T w0, w1;
       if (rres) {
         const T f0[2] = {o0x, o0y}, f1[2] = {o1x, o1y};
-        w0 = point_row_accumulate<T, L>(ctab, c0, T(0), T(0), f0, X, sp, huber_a, v, g, wgi ? wgi + r : nullptr);
-        w1 = point_row_accumulate<T, L>(ctab, c1, T(0), T(0), f1, X, sp, huber_a, v, g,
+        w0 = point_row_accumulate<T, L>(cams, c0, T(0), T(0), f0, X, sp, huber_a, v, g, wgi ? wgi + r : nullptr);
+        w1 = point_row_accumulate<T, L>(cams, c1, T(0), T(0), f1, X, sp, huber_a, v, g,
                                         wgi ? wgi + r + ln : nullptr);
       } else {
-        w0 = point_row_accumulate<T, L>(ctab, c0, o0x, o0y, nullptr, X, sp, huber_a, v, g);
-        w1 = point_row_accumulate<T, L>(ctab, c1, o1x, o1y, nullptr, X, sp, huber_a, v, g);
+        w0 = point_row_accumulate<T, L>(cams, c0, o0x, o0y, nullptr, X, sp, huber_a, v, g);
+        w1 = point_row_accumulate<T, L>(cams, c1, o1x, o1y, nullptr, X, sp, huber_a, v, g);
       }
       if (wout) {
         wout[r] = w0;
@@ -1013,7 +1078,7 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
     if (r < r1) {
       const T ox = ob[2 * (size_t)r], oy = ob[2 * (size_t)r + 1];
       const T f0[2] = {ox, oy};
-      const T w0 = point_row_accumulate<T, L>(ctab, row_cam[r], rres ? T(0) : ox, rres ? T(0) : oy,
+      const T w0 = point_row_accumulate<T, L>(cams, row_cam[r], rres ? T(0) : ox, rres ? T(0) : oy,
                                               rres ? f0 : nullptr, X, sp, huber_a, v, g, wgi ? wgi + r : nullptr);
       if (wout) wout[r] = w0;
     }
@@ -1061,6 +1126,41 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
                    (T)((double)sp[1] * (iv[1] * g0 + iv[3] * g1 + iv[4] * g2)),
                    (T)((double)sp[2] * (iv[2] * g0 + iv[4] * g1 + iv[5] * g2))};
   store_pk(pk, e, X, sw);
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_point_pass(
+    Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const T* __restrict__ row_obs, const double* __restrict__ pts64, const T* __restrict__ pscale,
+    const T* __restrict__ ctab, T huber_a, Win win, double min_diag, double max_diag,
+    double* __restrict__ diag_p, int recompute_diag, double* __restrict__ V, T* __restrict__ gp,
+    T* __restrict__ Vinv, T* __restrict__ pk, const T* __restrict__ rres, T* __restrict__ wrow,
+    double* __restrict__ Vinv64) {
+  point_pass_block<T, L>(seg_locate(sg), GlobalCams<T, L>{ctab, nullptr}, pt_start, row_cam, row_obs, pts64, pscale,
+                         huber_a, win, min_diag, max_diag, diag_p, recompute_diag, V, gp, Vinv, pk, rres, wrow,
+                         Vinv64, sg.e0[0]);
+}
+
+// The same with the cameras' projection records staged in LDS (FP32,
+// packed): persistent, each workgroup fills the table once and its four
+// groups of kBlock threads take the virtual blocks blockIdx.x * 4 + j,
+// stepping by 4 * gridDim.x.
+__global__ __launch_bounds__(kStageThreads) void k_point_pass_lds(
+    Seg sg, int nc, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const float* __restrict__ row_obs, const double* __restrict__ pts64, const float* __restrict__ pscale,
+    const float* __restrict__ ctab, float huber_a, Win win, double min_diag, double max_diag,
+    double* __restrict__ diag_p, int recompute_diag, double* __restrict__ V, float* __restrict__ gp,
+    float* __restrict__ Vinv, float* __restrict__ pk, const float* __restrict__ rres, float* __restrict__ wrow,
+    double* __restrict__ Vinv64) {
+  extern __shared__ float4 stage4[];
+  stage_cams<4>(stage4, nc, ctab, nullptr);
+  const LdsCams<4> cams{reinterpret_cast<const float*>(stage4)};
+  const int j = threadIdx.x / kBlock, tid = threadIdx.x % kBlock;
+  const int nblk = sg.blk0[sg.n];
+  for (int vb = blockIdx.x * 4 + j; vb < nblk; vb += 4 * gridDim.x)
+    point_pass_block<float, 0>(seg_locate_at(sg, vb, tid, kBlock), cams, pt_start, row_cam, row_obs, pts64, pscale,
+                               huber_a, win, min_diag, max_diag, diag_p, recompute_diag, V, gp, Vinv, pk, rres, wrow,
+                               Vinv64, sg.e0[0]);
 }
 
 // Re-damp after a rejected step: Vinv, w from stored V, g_p, diag_p.
@@ -1467,17 +1567,15 @@ __global__ __launch_bounds__(kBlock) void k_cam_dir(int nc, const int32_t* __res
 
 // Per E block: q = sum_rows Jp'(beta r + Jc x) (x via dtab); z = Vinv q
 // (apply_vinv) or q; pk's v part (if pk) becomes s_p .* z.
-template <typename T, int L>
-__global__ __launch_bounds__(kBlock) void k_mv_point(
-    Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+template <typename T, int L, typename Cams>
+__device__ __forceinline__ void mv_point_block(
+    SegPos qp, const Cams& cams, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
-    const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ Vinv, T huber_a,
-    T beta, int apply_vinv, T* __restrict__ zout, T* __restrict__ pk, const int32_t* __restrict__ all_done,
-    Win win, const T* __restrict__ wrow, const T* __restrict__ rres, const double* __restrict__ Vinv64) {
-  if (all_done && *all_done) return;
-  SegPos qp = seg_locate(sg);
+    const T* __restrict__ Vinv, T huber_a, T beta, int apply_vinv, T* __restrict__ zout, T* __restrict__ pk,
+    const Win& win, const T* __restrict__ wrow, const T* __restrict__ rres, const double* __restrict__ Vinv64,
+    int e_fallback) {
   if (qp.valid && win_idle_e(win, qp.e)) qp.valid = false;
-  const int e = qp.valid ? qp.e : sg.e0[0];
+  const int e = qp.valid ? qp.e : e_fallback;
   T X[3], unused[3];
   load_pk(pkX, e, X, unused);
   const T sp[3] = {pscale[3 * (size_t)e], pscale[3 * (size_t)e + 1], pscale[3 * (size_t)e + 2]};
@@ -1486,7 +1584,7 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
   // (optional): its residual evaluated in FP64 (beta != 0 only).
   auto row = [&](int r, int cam, T ox, T oy) {
     CamProj<T> c;
-    load_proj<T, L>(ctab + (size_t)kCamTab * cam, c);
+    cams.proj(cam, c);
     Proj<T> o;
     project<T, L>(c, X, ox, oy, o);
     if (rres) {
@@ -1497,7 +1595,7 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
     T Jp[6];
     jac_point(o, c.R, wgt, sp, Jp);
     T v[2];
-    jac_cam_dir<T, L>(o, dtab + (size_t)dir_stride<L>() * cam, X, wgt, v);
+    jac_cam_dir<T, L>(o, cams.dir(cam), X, wgt, v);
     v[0] += beta * wgt * o.rx;
     v[1] += beta * wgt * o.ry;
     fma2(q[0], Jp[0], v[0], Jp[3], v[1]);
@@ -1541,6 +1639,37 @@ __global__ __launch_bounds__(kBlock) void k_mv_point(
     const T sz[3] = {sp[0] * zv[0], sp[1] * zv[1], sp[2] * zv[2]};
     store_pk_v(pk, e, sz);
   }
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_mv_point(
+    Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const T* __restrict__ row_obs, const T* __restrict__ pkX, const T* __restrict__ pscale,
+    const T* __restrict__ ctab, const T* __restrict__ dtab, const T* __restrict__ Vinv, T huber_a,
+    T beta, int apply_vinv, T* __restrict__ zout, T* __restrict__ pk, const int32_t* __restrict__ all_done,
+    Win win, const T* __restrict__ wrow, const T* __restrict__ rres, const double* __restrict__ Vinv64) {
+  if (all_done && *all_done) return;
+  mv_point_block<T, L>(seg_locate(sg), GlobalCams<T, L>{ctab, dtab}, pt_start, row_cam, row_obs, pkX, pscale,
+                       Vinv, huber_a, beta, apply_vinv, zout, pk, win, wrow, rres, Vinv64, sg.e0[0]);
+}
+
+// The same with every camera's record and direction staged in LDS (FP32,
+// packed); persistent as k_point_pass_lds.
+__global__ __launch_bounds__(kStageThreads) void k_mv_point_lds(
+    Seg sg, int nc, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
+    const float* __restrict__ row_obs, const float* __restrict__ pkX, const float* __restrict__ pscale,
+    const float* __restrict__ ctab, const float* __restrict__ dtab, const float* __restrict__ Vinv, float huber_a,
+    float beta, int apply_vinv, float* __restrict__ zout, float* __restrict__ pk, const int32_t* __restrict__ all_done,
+    Win win, const float* __restrict__ wrow, const float* __restrict__ rres, const double* __restrict__ Vinv64) {
+  extern __shared__ float4 stage4[];
+  if (all_done && *all_done) return;
+  stage_cams<8>(stage4, nc, ctab, dtab);
+  const LdsCams<8> cams{reinterpret_cast<const float*>(stage4)};
+  const int j = threadIdx.x / kBlock, tid = threadIdx.x % kBlock;
+  const int nblk = sg.blk0[sg.n];
+  for (int vb = blockIdx.x * 4 + j; vb < nblk; vb += 4 * gridDim.x)
+    mv_point_block<float, 0>(seg_locate_at(sg, vb, tid, kBlock), cams, pt_start, row_cam, row_obs, pkX, pscale,
+                             Vinv, huber_a, beta, apply_vinv, zout, pk, win, wrow, rres, Vinv64, sg.e0[0]);
 }
 
 // Per chunk: partial_c = sum_rows Jc'(Jp z_e) (9 values).

@@ -908,6 +908,13 @@ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 using namespace mba;
 
+// Grid and dynamic LDS bytes of a persistent LDS-staged launch (grid 0: not
+// used for this context).
+struct LdsLaunch {
+  int grid = 0;
+  size_t bytes = 0;
+};
+
 struct mba_context {
   mba_options opt;
   // Independent problems ("windows") held together; a single problem is
@@ -1028,6 +1035,12 @@ struct mba_context {
   int stage_next = 0;
   hipEvent_t stage_ev[kStage] = {};
   hipEvent_t look_ev[kLook] = {};  // PCG look-ahead: one per in-flight iteration
+  // Captured PCG iterations, [row-weight slot][residual reset]; made on
+  // first use.
+  hipGraphExec_t pcg_graph[2][2] = {};
+  // FP32 point-major pass and matvec with LDS-staged camera records (grid
+  // > 0): grids and dynamic LDS bytes.
+  LdsLaunch lds_pass, lds_mv;
   hipEvent_t tev[3] = {};          // LM phase timing
   // Dense Cholesky look-ahead: the trailing tiles past the next block column
   // run on stream2 beside the next diagonal factor; chol_ev[0] orders them
@@ -1197,6 +1210,9 @@ void context_free(mba_context* c) {
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->look_ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& row : c->pcg_graph)
+    for (hipGraphExec_t g : row)
+      if (g) (void)hipGraphExecDestroy(g);
   for (hipEvent_t e : c->tev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream2) (void)hipStreamSynchronize(c->stream2);
@@ -1895,6 +1911,25 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     MBA_TRY(upload(c, &c->g_hi, ghi));
     MBA_TRY(upload(c, &c->g_out, gout));
   }
+  // FP32 point-major kernels with the camera records staged in LDS (the
+  // pass: projection quads; the matvec: projection and direction quads),
+  // one or two workgroups per CU, as many as the table leaves room for.
+  if (c->mixed && !c->grouped && c->nc > 0 && c->seg_blocks > 0) {
+    int ncu = 0, lds_max = 0;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device));
+    auto plan = [&](const void* kernel, int rec_floats, LdsLaunch* l) -> mba_status {
+      const size_t bytes = sizeof(float) * rec_floats * (size_t)c->nc;
+      if (ncu <= 0 || bytes > (size_t)lds_max) return MBA_OK;
+      HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+      const int per_cu = (int)std::min<size_t>(2, (size_t)lds_max / bytes);
+      l->bytes = bytes;
+      l->grid = (int)std::min<int64_t>(((int64_t)c->seg_blocks + 3) / 4, (int64_t)ncu * per_cu);
+      return MBA_OK;
+    };
+    MBA_TRY(plan((const void*)k_point_pass_lds, stage_rec<4>(), &c->lds_pass));
+    MBA_TRY(plan((const void*)k_mv_point_lds, stage_rec<8>(), &c->lds_mv));
+  }
   trace.mark("chunks + device setup");
   // FP64 state: cameras (slot blocks), then intrinsic groups (grouped).
   Buf<double> pts(3 * (size_t)c->ne);
@@ -2097,9 +2132,7 @@ Win win_desc(mba_context* c, double* acc, bool in_pcg) {
   const bool multi = c->W > 1;
   w.of_e = multi ? c->of_e : nullptr;
   w.of_f = multi ? c->of_f : nullptr;
-  // A single problem takes 1 / radius by value (no upload, no sync).
-  w.inv_r = multi ? c->inv_r : nullptr;
-  w.inv_r0 = c->h_inv_r.empty() ? 0.0 : c->h_inv_r[0];
+  w.inv_r = c->inv_r;
   w.acc = multi ? acc : nullptr;
   w.flag = c->flag;
   w.active = multi ? c->wactive : nullptr;
@@ -2190,15 +2223,14 @@ struct Engine {
     return MBA_OK;
   }
 
-  // 1 / radius per window (0 for radius 0: no damping).  A single problem
-  // passes it by value in the window descriptor; several windows upload
-  // it when it changes.
+  // 1 / radius per window (0 for radius 0: no damping), uploaded when it
+  // changes: kernels read it from device memory, so a captured PCG
+  // iteration replays under every radius.
   mba_status set_radii(const std::vector<double>& radius) {
     std::vector<double> inv(W);
     for (int w = 0; w < W; ++w) inv[w] = radius[w] > 0.0 ? 1.0 / radius[w] : 0.0;
     if (inv == c->h_inv_r) return MBA_OK;
     c->h_inv_r = inv;
-    if (W == 1) return MBA_OK;
     return stage_upload(c->inv_r, c->h_inv_r.data(), sizeof(double) * W);
   }
   mba_status set_radius(double r) { return set_radii(std::vector<double>(W, r)); }
@@ -2294,7 +2326,17 @@ struct Engine {
 
   void jacobian_pass(int recompute_diag) {
     if (c->timing) (void)hipEventRecord(c->ev[0], s);
-    if (c->seg_blocks > 0)
+    bool staged = false;
+    if constexpr (std::is_same<T, float>::value) {
+      if (c->lds_pass.grid > 0) {
+        hipLaunchKernelGGL(k_point_pass_lds, dim3(c->lds_pass.grid), dim3(kStageThreads), c->lds_pass.bytes, s, c->seg,
+                           c->nc, d.pt_start, d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, win(),
+                           c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag, d.V, d.gp,
+                           vinv_out(), d.pk, fixed_rows(), wrow_cur(), d.Vinv64);
+        staged = true;
+      }
+    }
+    if (c->seg_blocks > 0 && !staged)
       hipLaunchKernelGGL((c->grouped ? k_point_pass<T, 1> : k_point_pass<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0,
                          s, c->seg, d.pt_start, d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, win(),
                          c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag,
@@ -2363,6 +2405,15 @@ struct Engine {
   // the FP64 cost's residuals (the ones the pass's g_p was formed from).
   void mv_point(T beta, T* pk_out, const int32_t* done, const Win& wv) {
     if (c->seg_blocks == 0) return;
+    if constexpr (std::is_same<T, float>::value) {
+      if (c->lds_mv.grid > 0) {
+        hipLaunchKernelGGL(k_mv_point_lds, dim3(c->lds_mv.grid), dim3(kStageThreads), c->lds_mv.bytes, s, c->seg, c->nc,
+                           d.pt_start, d.row_cam, d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber, beta, 1,
+                           d.z, pk_out, done, wv, (const T*)wrow_cur(), beta != T(0) ? fixed_rows() : nullptr,
+                           vinv64());
+        return;
+      }
+    }
     hipLaunchKernelGGL((c->grouped ? k_mv_point<T, 1> : k_mv_point<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0, s,
                        c->seg, d.pt_start, d.row_cam, d.row_obs, d.pk, d.pscale, d.ctab, d.dtab, d.Vinv, huber,
                        beta, 1, d.z, pk_out, done, wv, (const T*)wrow_cur(), beta != T(0) ? fixed_rows() : nullptr,
@@ -2470,13 +2521,13 @@ struct Engine {
                          c->d_done);
     };
     const Win wv = win(nullptr, true);
-    update(3);
-    HIP_TRY(hipEventRecord(c->look_ev[0], s));
-    for (int it = 1; it <= maxit; ++it) {
-      // Wait for iteration it - kLook (the update launched kLook rounds
-      // ago), then stop if the device has finished.
-      HIP_TRY(hipEventSynchronize(c->look_ev[it % kLook]));
-      if (*c->h_done) break;
+    // One CG iteration: direction, implicit matvec, alpha, and the x / r
+    // updates, or every tenth iteration the residual reset r = b - S x.
+    // Every argument is fixed for the context but the row weights' slot, so
+    // with pcg_graph_launch the iteration is captured once per (slot, reset)
+    // as a HIP graph and replayed; no memset is recorded (the scalar kernels
+    // clear what they read).
+    auto iteration = [&](int reset) {
       if (c->grouped) {
         // A group's direction feeds all its cameras' rows: update p first.
         hipLaunchKernelGGL(k_pcg_p<T>, dim3(nblocks(9 * (int64_t)nf)), dim3(kBlock), 0, s, nf, (const PcgState*)c->pcg,
@@ -2491,13 +2542,46 @@ struct Engine {
       hipLaunchKernelGGL((c->grouped ? k_mv_finalize_alpha<T, 1> : k_mv_finalize_alpha<T, 0>), dim3(nbg), dim3(kBlock),
                          0, s, nf, d.f_lo, d.f_hi, d.partial, d.U, c->diag_c, wd, d.p, d.q, c->all_done, c->pcg, W,
                          c->red, c->counter);
-      if (it % 10 == 0) {  // residual reset: r = b - S x
+      if (reset) {
         update(1);
         matvec(d.x, d.tmp, c->all_done);
         update(2);
       } else {
         update(0);
       }
+    };
+    hipGraphExec_t* graphs = c->opt.pcg_graph_launch ? c->pcg_graph[wslot()] : nullptr;
+    auto launch = [&](int reset) -> mba_status {
+      if (!graphs) {
+        iteration(reset);
+        HIP_TRY(hipGetLastError());
+        return MBA_OK;
+      }
+      hipGraphExec_t& ex = graphs[reset];
+      if (!ex) {
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        iteration(reset);
+        const hipError_t le = hipGetLastError();
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(s, &g);
+        HIP_TRY(le);
+        HIP_TRY(ce);
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) ex = nullptr;
+        HIP_TRY(ie);
+      }
+      HIP_TRY(hipGraphLaunch(ex, s));
+      return MBA_OK;
+    };
+    update(3);
+    HIP_TRY(hipEventRecord(c->look_ev[0], s));
+    for (int it = 1; it <= maxit; ++it) {
+      // Wait for iteration it - kLook (the update launched kLook rounds
+      // ago), then stop if the device has finished.
+      HIP_TRY(hipEventSynchronize(c->look_ev[it % kLook]));
+      if (*c->h_done) break;
+      MBA_TRY(launch(it % 10 == 0 ? 1 : 0));
       HIP_TRY(hipEventRecord(c->look_ev[it % kLook], s));
     }
     HIP_TRY(hipMemcpyAsync(c->h_pcgs, c->pcg, sizeof(PcgState) * W, hipMemcpyDeviceToHost, s));

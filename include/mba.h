@@ -220,7 +220,12 @@ typedef struct mba_options {
    * partial sums), rounded up to a multiple of 64; 64..8192. */
   int32_t point_rows_per_lane;                /* 0 */
   int32_t camera_chunk_rows;                  /* 0 */
-  int32_t reserved3;
+  /* Implicit-Schur PCG launches (no counterpart in the reference): 0
+   * launches each CG iteration's kernels one by one, the host queueing up to
+   * four iterations ahead; 1 replays the iteration as a HIP graph captured
+   * on first use.  Same kernels, same arguments, bit-identical results; on
+   * MI355X (ROCm 7.2) the graph replay costs 8-13 us more per iteration. */
+  int32_t pcg_graph_launch;                   /* 0 */
 } mba_options;
 
 typedef struct mba_iteration {

@@ -37,7 +37,8 @@ def test_struct_layouts_match_the_header(mba, tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     checks = {"mba_options": (mba.Options, ["eta", "preconditioner", "dump_directory", "dump_iterations",
-                                            "num_dump_iterations", "point_rows_per_lane", "camera_chunk_rows"]),
+                                            "num_dump_iterations", "point_rows_per_lane", "camera_chunk_rows",
+                                            "pcg_graph_launch"]),
               "mba_problem": (mba._Problem, []), "mba_summary": (mba._Summary, ["num_linear_solver_iterations"]),
               "mba_iteration": (mba.Iteration, ["cumulative_time_s"])}
     src = ["#include <stdio.h>", "#include <stddef.h>", '#include "mba.h"', "int main(void) {"]

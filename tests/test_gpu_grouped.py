@@ -246,3 +246,22 @@ def test_grouped_mixed_reaches_fp64_final_cost(mba, golden, group_scene):
     ref = float(golden["grp_iterative_final_cost"])
     assert s.IsSolutionUsable()
     assert abs(s.final_cost - ref) / ref < 1e-6
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_grouped_pcg_graph_replay_matches_direct_launches(mba, group_scene, prec):
+    """Grouped layout (group-direction kernel and group-row sums inside the
+    CG iteration): replaying the captured iteration equals launching its
+    kernels one by one, bit for bit."""
+    d, _ = group_scene
+    runs = []
+    for graph in (1, 0):
+        p = grouped_problem(mba, d)
+        s = mba.solve(p, mba.default_options(max_num_iterations=10, linear_solver=1, huber_a=4.0, precision=prec,
+                                             pcg_graph_launch=graph))
+        runs.append((s, p))
+    (sa, pa), (sb, pb) = runs
+    assert sa.final_cost == sb.final_cost
+    assert sa.num_linear_solver_iterations == sb.num_linear_solver_iterations
+    np.testing.assert_array_equal(pa.cameras, pb.cameras)
+    np.testing.assert_array_equal(pa.intrinsics, pb.intrinsics)

@@ -289,6 +289,34 @@ def test_solve_is_deterministic(mba, c1):
     np.testing.assert_array_equal(a.points, b.points)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("precond", ["jacobi", "schur_jacobi"])
+def test_pcg_graph_replay_matches_direct_launches(mba, c1, huber_scene, prec, precond):
+    """A CG iteration replayed from its captured HIP graph (pcg_graph_launch) runs
+    the kernels that launching them one by one runs, with the same
+    arguments: bit-identical parameters, costs and iteration counts over LM
+    runs whose radius changes every iteration (kernels read it from device
+    memory) and, in mixed precision, alternate the row-weight slot."""
+    const = np.zeros(30, np.uint8)
+    const[3] = 1
+    for d, k, huber in ((c1, None, 0.0), (huber_scene, const, 4.0)):
+        runs = []
+        for graph in (1, 0):
+            p = ba_problem(mba, d, k)
+            o = mba.default_options(max_num_iterations=15, linear_solver=mba.IMPLICIT_SCHUR_PCG, huber_a=huber,
+                                    precision=mba.PRECISION_MIXED if prec == "mixed" else mba.PRECISION_FP64,
+                                    preconditioner=mba.PRECOND_SCHUR_JACOBI if precond == "schur_jacobi"
+                                    else mba.PRECOND_JACOBI, pcg_graph_launch=graph)
+            runs.append((mba.solve(p, o), p))
+        (sa, pa), (sb, pb) = runs
+        assert sa.IsSolutionUsable()
+        assert sa.final_cost == sb.final_cost
+        assert (sa.num_successful_steps, sa.num_unsuccessful_steps, sa.num_linear_solver_iterations) == \
+               (sb.num_successful_steps, sb.num_unsuccessful_steps, sb.num_linear_solver_iterations)
+        np.testing.assert_array_equal(pa.cameras, pb.cameras)
+        np.testing.assert_array_equal(pa.points, pb.points)
+
+
 def test_nonfinite_input_is_an_error(mba, c1):
     p = ba_problem(mba, c1)
     p.points[3, 1] = np.inf
