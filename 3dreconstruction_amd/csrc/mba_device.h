@@ -2068,6 +2068,17 @@ __device__ __forceinline__ bool pcg_step_state(PcgState& s, double rz, double xb
 // per-window accumulators (windowed runs) are cleared as they are read.
 __device__ __forceinline__ void pcg_alpha(const double* __restrict__ partial, int nb, int nw,
                                           PcgState* __restrict__ st, double* __restrict__ wacc) {
+  if (!wacc && nw == 1) {
+    // Single problem: the block partials summed by the first wave (strided
+    // lanes, then the wave sum), not by one thread in turn.
+    if (threadIdx.x < 64) {
+      double v = 0.0;
+      for (int i = threadIdx.x; i < nb; i += 64) v += partial[i];
+      v = wave_sum(v);
+      if (threadIdx.x == 0 && !st[0].done) pcg_alpha_state(st[0], v);
+    }
+    return;
+  }
   for (int w = threadIdx.x; w < nw; w += blockDim.x) {
     PcgState& s = st[w];
     if (s.done) continue;
@@ -2091,6 +2102,19 @@ __device__ __forceinline__ void pcg_scalar(const double* __restrict__ partial, i
   __shared__ int any_active;
   if (threadIdx.x == 0) any_active = 0;
   __syncthreads();
+  if (!wacc && nw == 1) {
+    // Single problem: the three sums by the first wave, as in pcg_alpha.
+    if (threadIdx.x < 64) {
+      double v[3] = {0.0, 0.0, 0.0};
+      for (int i = threadIdx.x; i < nb; i += 64)
+        for (int k = 0; k < 3; ++k) v[k] += partial[3 * i + k];
+      for (int k = 0; k < 3; ++k) v[k] = wave_sum(v[k]);
+      if (threadIdx.x == 0 && !st[0].done &&
+          !pcg_step_state(st[0], v[0], v[1], v[2], first != 0, eta, min_iters, max_iters))
+        any_active = 1;
+    }
+    nw = 0;  // the loop below has nothing left to do
+  }
   for (int w = threadIdx.x; w < nw; w += blockDim.x) {
     PcgState& s = st[w];
     if (s.done) continue;

@@ -388,6 +388,7 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
     rb[1] = nc + p->point_index[i];
     return 2;
   });
+  tr.mark("  first appearance");
   // CSR of observations by camera: per-thread histograms over observation
   // ranges (the cameras are few), then each thread fills its own slots.
   std::vector<int64_t> cstart((size_t)nc + 1, 0);
@@ -415,7 +416,7 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
       for (int64_t i = t * step; i < e; ++i) cobs_pt[h[t][p->camera_index[i]]++] = p->point_index[i];
     });
   }
-  tr.mark("  first appearance + CSR");
+  tr.mark("  camera CSR");
   // Hessian-graph edges: the distinct points of every non-constant camera
   // (cameras in parallel, a point marker per thread), compacted into cadj;
   // a point's degree counts the cameras that list it.
