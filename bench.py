@@ -60,17 +60,65 @@ def log(msg):
 
 def cam_pass_bytes(nrows, ne, nf, nchunks, elem):
     """Algorithmic bytes of one camera-pass launch: per camera-major row the
-    point index (4 B) and observation (2 elem); per point block its position,
-    Jacobi scale and V^-1 g_p (3 elem each) read once; per camera its table
-    row and scale; per chunk 63 partial sums written."""
-    return (nrows * (4 + 2 * elem) + ne * 9 * elem + nf * (48 + 9) * elem + nchunks * 63 * elem)
+    point index (4 B), observation (2 elem) and loss weight (1 elem); per
+    point block its position, Jacobi scale and V^-1 g_p (3 elem each) read
+    once; per camera its table row and scale; per chunk 63 partial sums
+    written."""
+    return (nrows * (4 + 3 * elem) + ne * 9 * elem + nf * (48 + 9) * elem + nchunks * 63 * elem)
 
 
 def point_pass_bytes(nrows, ne, nc, elem):
-    """Per row camera id (4 B) + observation; per point start (4 B),
-    position, scale (read) and V, g_p, V^-1, w (write, 18 elem) plus the FP64
-    LM diagonal (24 B); camera tables once."""
-    return nrows * (4 + 2 * elem) + ne * (4 + 6 * elem + 18 * elem + 24) + nc * 48 * elem
+    """Per row camera id (4 B), observation (2 elem) and loss weight (1 elem);
+    per point start (4 B), FP64 position (24 B) and scale (3 elem) read, and
+    V and V^-1 (FP64, 48 B each), g_p (3 elem), the packed point record
+    (8 elem) and the FP64 LM diagonal (24 B) written; camera projection
+    records (16 elem) once."""
+    return nrows * (4 + 3 * elem) + ne * (4 + 24 + 3 * elem + 96 + 3 * elem + 8 * elem + 24) + nc * 16 * elem
+
+
+# Whole solves, the reference's Ceres on the host cores against the library on
+# the GPU, same scenes and options: (name, preset, Huber threshold, reference
+# linear solver (0 SPARSE_SCHUR, 1 ITERATIVE_SCHUR + JACOBI), LM iterations).
+REF_SOLVE_CASES = [("C1", 1, 0.0, 0, 20), ("C2", 2, 0.0, 0, 20), ("C3", 3, 4.0, 1, 5)]
+
+
+def reference_solves(mba, O, threads, cpu_model):
+    """Optional CPU-baseline leg (--ref-solves, ~1 min): per scene, the
+    reference Ceres solve's wall time and final cost beside the library's
+    (SPARSE_SCHUR -> explicit Schur FP64; ITERATIVE_SCHUR -> implicit-Schur PCG
+    in FP64 and mixed), each library solve the second of two calls."""
+    out = []
+    for name, preset, huber, ls, iters in REF_SOLVE_CASES:
+        d = mba.generate_scene(preset)
+        nres = int(d["obs"].shape[0])
+        log(f"bench: reference solve {name} ({nres} observations, {threads} threads)")
+        sc = O.Scene(d)
+        t0 = time.perf_counter()
+        r = O.ref_solve(sc, huber_a=huber, linear_solver=ls, max_iters=iters, num_threads=threads)
+        ref_wall = time.perf_counter() - t0
+        row = {"scene": name, "observations": nres, "lm_iterations": iters,
+               "reference": {"solver": "SPARSE_SCHUR" if ls == 0 else "ITERATIVE_SCHUR + JACOBI",
+                             "wall_s": ref_wall, "ceres_total_s": r["total_time"],
+                             "iterations": len(r["iterations"]) - 1, "final_cost": r["final_cost"],
+                             "cores": threads, "cpu": cpu_model},
+               "gpu": []}
+        runs = ([("explicit Schur FP64", mba.EXPLICIT_SCHUR, mba.PRECISION_FP64)] if ls == 0 else
+                [("implicit-Schur PCG FP64", mba.IMPLICIT_SCHUR_PCG, mba.PRECISION_FP64),
+                 ("implicit-Schur PCG mixed", mba.IMPLICIT_SCHUR_PCG, mba.PRECISION_MIXED)])
+        for label, solver, prec in runs:
+            o = mba.default_options(max_num_iterations=iters, linear_solver=solver, precision=prec, huber_a=huber)
+            for _ in range(2):  # the first call pays the device set-up
+                p = mba.BAProblem(d["cam_init"].copy(), d["pt_init"].copy(), d["obs"], d["cam_idx"], d["pt_idx"])
+                t0 = time.perf_counter()
+                s = mba.solve(p, o)
+                wall = time.perf_counter() - t0
+            row["gpu"].append({"solver": label, "wall_s": wall, "preprocess_s": s.preprocessor_time_s,
+                               "iterations": s.num_iterations - 1, "final_cost": s.final_cost})
+        log(f"  reference {ref_wall:.2f} s (cost {r['final_cost']:.10g}); GPU " +
+            ", ".join(f"{g['solver']} {g['wall_s']:.3f} s (cost {g['final_cost']:.10g})" for g in row["gpu"]))
+        out.append(row)
+        del d, sc
+    return out
 
 
 def run(args):
@@ -150,12 +198,14 @@ def run(args):
             tdata = json.load(f)
     traffic_of = tdata.get("hbm_bytes_per_launch", {})
     valu_of = tdata.get("valu_issue_frac", {})
-    pass_kernels = ["k_point_pass<float>", "k_cam_pass<float>", "k_cam_finalize<float>"]
+    # rocprofv3 names of the mixed pass's kernels at this size (1,000 cameras:
+    # the point pass runs in its LDS-staged persistent form).
+    pass_kernels = ["k_point_pass_lds", "k_cam_pass<float, 64>", "k_cam_finalize<float>"]
     traffic = (sum(traffic_of[k] for k in pass_kernels) if all(k in traffic_of for k in pass_kernels) else None)
     if kern_mixed[1] >= kern_mixed[0]:
-        name, kms, nbytes = "k_cam_pass<float>", kern_mixed[1], cam_pass_bytes(nobs, ne, nf, nchunks, 4)
+        name, kms, nbytes = "k_cam_pass<float, 64>", kern_mixed[1], cam_pass_bytes(nobs, ne, nf, nchunks, 4)
     else:
-        name, kms, nbytes = "k_point_pass<float>", kern_mixed[0], point_pass_bytes(nobs, ne, nc, 4)
+        name, kms, nbytes = "k_point_pass_lds", kern_mixed[0], point_pass_bytes(nobs, ne, nc, 4)
     dom_achieved = nbytes / (kms * 1e-3) / 1e9
     # The Jacobian+Schur unit of the metric (131.5 B/obs incl. 108 B of
     # W = Jc'Jp this engine never writes), over the whole pass: an effective
@@ -399,6 +449,8 @@ def run(args):
                              f"OpenMP {threads} threads on {cpu_model}",
                    "port": port}
         log(f"  CPU {cpu['kind']}: {cpu['value']:.3g} obs/s")
+        if args.ref_solves and O.ref_available():
+            cpu["solves"] = reference_solves(mba, O, threads, cpu_model)
 
     value = nobs / (ms_mixed * 1e-3)
     out = {
@@ -415,7 +467,9 @@ def run(args):
         "roofline": {"bound": "hbm", "kernel": name, "achieved": dom_achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom_achieved / HBM_PEAK_GBS, "traffic": traffic_of.get(name),
                      "algorithmic_bytes": nbytes, "kernel_ms": kms,
-                     "limiter": "valu" if valu_of.get(name, 0.0) > 0.5 else "memory",
+                     "limiter": ("valu" if valu_of.get(name, 0.0) > 0.6 else
+                                 "hbm" if dom_achieved / HBM_PEAK_GBS > 0.6 else
+                                 "latency (neither VALU issue nor HBM bandwidth saturated)"),
                      "valu_issue_frac": valu_of.get(name),
                      "counters_source": "profiles/counters_jacobian_schur_4M.json (rocprofv3 --pmc runs; "
                                         "traffic and VALU share are committed profile values, not measured "
@@ -448,7 +502,8 @@ def pmc_target(kind):
                                                  linear_solver=mba.EXPLICIT_SCHUR, max_num_iterations=2))
         ctx.solve()
     else:
-        ctx = mba.Context(p, mba.default_options(precision=mba.PRECISION_MIXED, huber_a=HUBER_A,
+        prec = mba.PRECISION_FP64 if kind == "pass_fp64" else mba.PRECISION_MIXED
+        ctx = mba.Context(p, mba.default_options(precision=prec, huber_a=HUBER_A,
                                                  linear_solver=mba.IMPLICIT_SCHUR_PCG))
         for _ in range(5):
             ctx.jacobian_schur(RADIUS)
@@ -466,15 +521,19 @@ def main():
     ap.add_argument("--no-lm", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cpu-ref", action="store_true", help="CPU baseline from the oracle port only")
+    ap.add_argument("--ref-solves", action="store_true",
+                    help="also time whole solves of the reference's Ceres (host cores) against the library "
+                         "on C1-C3 (cpu_baseline.solves; about a minute more)")
     ap.add_argument("--no-explicit", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--explicit-iters", type=int, default=5)
-    ap.add_argument("--pmc-target", nargs="?", const="pass", choices=["pass", "explicit"],
+    ap.add_argument("--pmc-target", nargs="?", const="pass", choices=["pass", "pass_fp64", "explicit"],
                     help="target run for rocprofv3 --pmc: a 256 MiB device copy (known bytes, the "
-                         "counter calibration) followed by five Jacobian+Schur passes ('pass') or a "
-                         "two-iteration explicit-Schur solve ('explicit'); prints nothing else")
+                         "counter calibration) followed by five mixed-precision Jacobian+Schur passes "
+                         "('pass'), five FP64 ones ('pass_fp64') or a two-iteration explicit-Schur "
+                         "solve ('explicit'); prints nothing else")
     args = ap.parse_args()
     if args.pmc_target:
         pmc_target(args.pmc_target)
