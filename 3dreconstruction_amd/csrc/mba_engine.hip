@@ -369,6 +369,194 @@ std::vector<int32_t> first_appearance(int64_t no, int32_t nb, int kmax, F&& bloc
   return prog;
 }
 
+// Exclusive prefix sum out[0..n] of v[0..n) (out[n] = total), in parallel
+// ranges: per-range totals, their scan, then each range's own.
+template <typename A, typename B>
+void exclusive_scan(const A* v, int64_t n, B* out) {
+  const int nt = host_threads(n, 1 << 18);
+  const int64_t step = (n + nt - 1) / std::max(1, nt);
+  std::vector<B> part((size_t)nt + 1, 0);
+  run_threads(nt, [&](int t) {
+    B acc = 0;
+    for (int64_t i = t * step, e = std::min(n, (t + 1) * step); i < e; ++i) acc += (B)v[i];
+    part[t + 1] = acc;
+  });
+  for (int t = 0; t < nt; ++t) part[t + 1] += part[t];
+  run_threads(nt, [&](int t) {
+    B acc = part[t];
+    for (int64_t i = t * step, e = std::min(n, (t + 1) * step); i < e; ++i) {
+      out[i] = acc;
+      acc += (B)v[i];
+    }
+  });
+  out[n] = part[nt];
+}
+
+// The ordering of the common bundle-adjustment structure, without the
+// Hessian adjacency: one counting sort of the observations by point (each
+// point's list in descending observation index) gives every point its rows,
+// its first appearance (the list's last entry), its degree (distinct
+// non-constant cameras) and every camera's degree (distinct points).  When
+// every point's degree is below every active camera's, the stable degree sort
+// queues all points before all cameras, the greedy independent set is
+// exactly the points, and each row's bucket is its point's E position -- the
+// general path's result (point-first branch of build_structure), built here
+// directly: E = points by (degree, first appearance), F = cameras by
+// (degree, first appearance), rows = the points' lists in E order.  Returns
+// false, leaving st untouched, when that condition fails.
+bool build_structure_points_first(const mba_problem* p, Structure* st, PrepTrace& tr) {
+  const int32_t nc = p->num_cameras, np = p->num_points;
+  const int64_t no = p->num_observations;
+  if (no == 0 || np == 0) return false;
+  const int32_t* cam = p->camera_index;
+  const int32_t* pt = p->point_index;
+  auto is_const = [&](int32_t c) { return p->camera_constant != nullptr && p->camera_constant[c] != 0; };
+  // Rows per point, then each point's list filled from its end (atomic
+  // cursors: any order) and sorted descending below.
+  Buf<int32_t> cnt(np);
+  parallel_for(np, [&](int64_t b, int64_t e) { std::fill(cnt.begin() + b, cnt.begin() + e, 0); }, 1 << 18);
+  parallel_for(no, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) __atomic_fetch_add(&cnt[pt[i]], 1, __ATOMIC_RELAXED);
+  });
+  Buf<int32_t> pstart((size_t)np + 1);
+  exclusive_scan(cnt.data(), np, pstart.data());
+  Buf<int32_t> cursor(np);
+  parallel_for(np, [&](int64_t b, int64_t e) {
+    for (int64_t q = b; q < e; ++q) cursor[q] = pstart[q + 1];
+  }, 1 << 18);
+  Buf<int32_t> rows(no);
+  parallel_for(no, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) rows[__atomic_sub_fetch(&cursor[pt[i]], 1, __ATOMIC_RELAXED)] = (int32_t)i;
+  });
+  tr.mark("  rows by point");
+  // Per point: sort its list descending, its degree; per camera (thread
+  // histograms): distinct points and first appearance.
+  const int nt = host_threads(np, 1 << 14);
+  const int64_t step = ((int64_t)np + nt - 1) / nt;
+  std::vector<std::vector<int32_t>> cdeg(nt, std::vector<int32_t>((size_t)nc, 0));
+  std::vector<std::vector<int32_t>> cfirst(nt, std::vector<int32_t>((size_t)nc, INT32_MAX));
+  std::vector<int32_t> tmax(nt, 0);
+  Buf<int32_t> pdeg(np);
+  run_threads(nt, [&](int t) {
+    int32_t* cd = cdeg[t].data();
+    int32_t* cf = cfirst[t].data();
+    std::vector<int32_t> seen;
+    int32_t mx = 0;
+    for (int64_t q = t * step, qe = std::min<int64_t>(np, (t + 1) * step); q < qe; ++q) {
+      int32_t* r = rows.data() + pstart[q];
+      const int32_t n = pstart[q + 1] - pstart[q];
+      if (n == 0) {
+        pdeg[q] = -1;  // not in any residual: not a program block
+        continue;
+      }
+      std::sort(r, r + n, [](int32_t a, int32_t b) { return a > b; });
+      seen.clear();
+      for (int32_t k = 0; k < n; ++k) {
+        const int32_t c = cam[r[k]];
+        cf[c] = std::min(cf[c], r[k]);
+        if (is_const(c) || std::find(seen.begin(), seen.end(), c) != seen.end()) continue;
+        seen.push_back(c);
+        cd[c]++;
+      }
+      pdeg[q] = (int32_t)seen.size();
+      mx = std::max(mx, pdeg[q]);
+    }
+    tmax[t] = mx;
+  });
+  const int32_t max_pt = *std::max_element(tmax.begin(), tmax.end());
+  std::vector<int32_t> deg((size_t)nc, 0), first((size_t)nc, INT32_MAX);
+  for (int t = 0; t < nt; ++t)
+    for (int32_t c = 0; c < nc; ++c) {
+      deg[c] += cdeg[t][c];
+      first[c] = std::min(first[c], cfirst[t][c]);
+    }
+  int32_t min_cam = INT32_MAX;
+  std::vector<int32_t> cams;
+  for (int32_t c = 0; c < nc; ++c)
+    if (first[c] != INT32_MAX && !is_const(c)) {
+      cams.push_back(c);
+      min_cam = std::min(min_cam, deg[c]);
+    }
+  tr.mark("  degrees");
+  if (!(max_pt < min_cam)) return false;
+  // E: points in first-appearance order (a point's first appearance is its
+  // list's last entry; distinct across points), then stably by degree.
+  Buf<int32_t> slot(no);
+  parallel_for(no, [&](int64_t b, int64_t e) { std::fill(slot.begin() + b, slot.begin() + e, -1); });
+  parallel_for(np, [&](int64_t b, int64_t e) {
+    for (int64_t q = b; q < e; ++q)
+      if (pdeg[q] >= 0) slot[rows[pstart[q + 1] - 1]] = (int32_t)q;
+  }, 1 << 16);
+  // Stable counting sort by degree: per-range histograms, then each range
+  // fills its slots in index order.
+  const int nt2 = host_threads(no);
+  const int64_t step2 = (no + nt2 - 1) / nt2;
+  const int32_t nd = max_pt + 1;
+  std::vector<std::vector<int64_t>> hist(nt2, std::vector<int64_t>((size_t)nd, 0));
+  run_threads(nt2, [&](int t) {
+    int64_t* h = hist[t].data();
+    for (int64_t i = t * step2, e = std::min(no, (t + 1) * step2); i < e; ++i)
+      if (slot[i] >= 0) h[pdeg[slot[i]]]++;
+  });
+  int64_t run = 0;
+  for (int32_t d = 0; d < nd; ++d)
+    for (int t = 0; t < nt2; ++t) {
+      const int64_t k = hist[t][d];
+      hist[t][d] = run;
+      run += k;
+    }
+  const int32_t ne = (int32_t)run;
+  std::vector<int32_t> epts((size_t)ne);
+  run_threads(nt2, [&](int t) {
+    int64_t* h = hist[t].data();
+    for (int64_t i = t * step2, e = std::min(no, (t + 1) * step2); i < e; ++i)
+      if (slot[i] >= 0) epts[h[pdeg[slot[i]]]++] = slot[i];
+  });
+  // F: cameras by (degree, first appearance); first appearances are distinct.
+  std::sort(cams.begin(), cams.end(), [&](int32_t a, int32_t b) {
+    return deg[a] != deg[b] ? deg[a] < deg[b] : first[a] < first[b];
+  });
+  tr.mark("  E/F order");
+  // Rows in E order.
+  st->num_e = ne;
+  st->order.resize((size_t)ne + cams.size());
+  st->e_to_pt.assign(epts.begin(), epts.end());
+  st->pt_start.resize((size_t)ne + 1);
+  {
+    Buf<int32_t> len(ne);
+    std::vector<int32_t> lmax(host_threads(ne, 1 << 14), 0);
+    const int64_t lstep = ((int64_t)ne + (int64_t)lmax.size() - 1) / (int64_t)lmax.size();
+    run_threads((int)lmax.size(), [&](int t) {
+      for (int64_t e = t * lstep, ee = std::min<int64_t>(ne, (t + 1) * lstep); e < ee; ++e) {
+        const int32_t q = epts[e];
+        len[e] = pstart[q + 1] - pstart[q];
+        st->order[e] = nc + q;
+        lmax[t] = std::max(lmax[t], len[e]);
+      }
+    });
+    exclusive_scan(len.data(), ne, st->pt_start.data());
+    st->max_track = *std::max_element(lmax.begin(), lmax.end());
+  }
+  st->res_order.resize(no);
+  parallel_for(ne, [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; ++k) {
+      const int32_t q = epts[k];
+      int64_t o = st->pt_start[k];
+      for (int32_t r = pstart[q]; r < pstart[q + 1]; ++r) st->res_order[o++] = rows[r];
+    }
+  }, 1 << 14);
+  const int32_t nf = (int32_t)cams.size();
+  st->f_to_cam.assign(cams.begin(), cams.end());
+  st->cam_to_f.assign(nc, -1);
+  for (int32_t f = 0; f < nf; ++f) {
+    st->order[(size_t)ne + f] = cams[f];
+    st->cam_to_f[cams[f]] = f;
+  }
+  st->standard = true;
+  tr.mark("  residual order");
+  return true;
+}
+
 mba_status build_structure(const mba_problem* p, Structure* st) {
   MBA_TRY(validate(p));
   if (p->layout == MBA_LAYOUT_GROUPED) return build_structure_grouped(p, st);
@@ -382,6 +570,7 @@ mba_status build_structure(const mba_problem* p, Structure* st) {
     return b < nc && p->camera_constant != nullptr && p->camera_constant[b] != 0;
   };
   PrepTrace tr;
+  if (p->ordering == MBA_ORDER_AUTOMATIC && build_structure_points_first(p, st, tr)) return MBA_OK;
   // First appearance (program order).
   const std::vector<int32_t> prog = first_appearance(no, nb, 2, [&](int64_t i, int32_t* rb) {
     rb[0] = p->camera_index[i];

@@ -146,6 +146,52 @@ def test_schur_structure_matches_oracle_on_700k_scene(mba, O):
     np.testing.assert_array_equal(pairs, O.schur_block_pairs(O.Scene(d)))
 
 
+@pytest.mark.parametrize("case", ["points_first", "camera_ties_points"])
+def test_schur_structure_matches_oracle_on_edge_structures(mba, O, case):
+    """The ordering's points-first construction (every point's degree below
+    every camera's) and the general greedy path (a camera whose degree ties
+    the longest tracks) against the oracle's restatement of Ceres's ordering,
+    on a scene with repeated (camera, point) observations, constant cameras,
+    a point seen only through constant cameras, and an unobserved camera and
+    point."""
+    rng = np.random.default_rng(11)
+    d = dict(O.generate(O.make_spec(40, 3000, 2, 8, seed=21)))
+    obs, ci, pi = d["obs"], d["cam_idx"], d["pt_idx"]
+    dup = rng.choice(len(ci), 300, replace=False)  # repeated observations, shuffled in
+    obs = np.concatenate([obs, obs[dup] + 0.5])
+    ci = np.concatenate([ci, ci[dup]])
+    pi = np.concatenate([pi, pi[dup]])
+    const = np.zeros(41, np.uint8)
+    const[[2, 17]] = 1
+    # Point 0 seen only through constant cameras; camera 40 and point 3000 unobserved.
+    rows0 = np.flatnonzero(pi == 0)
+    ci[rows0] = 2
+    cams = np.concatenate([d["cam_init"], d["cam_init"][:1]])
+    pts = np.concatenate([d["pt_init"], d["pt_init"][:1]])
+    if case == "camera_ties_points":
+        # Camera 5 keeps three points (its shortest tracks; the others move to
+        # camera 6): its degree no longer exceeds every point's, so the greedy
+        # independent set decides, and a 2-observation point ahead of it in
+        # the queue keeps it out of E.
+        rows5 = np.flatnonzero(ci == 5)
+        tl = np.bincount(pi)
+        rows5 = rows5[np.argsort(tl[pi[rows5]], kind="stable")]
+        ci[rows5[3:]] = 6
+    perm = rng.permutation(len(ci))
+    d2 = dict(d, cam_init=cams, pt_init=pts, obs=np.ascontiguousarray(obs[perm]),
+              cam_idx=np.ascontiguousarray(ci[perm], np.int32), pt_idx=np.ascontiguousarray(pi[perm], np.int32))
+    p = mba.BAProblem(cams.copy(), pts.copy(), d2["obs"], d2["cam_idx"], d2["pt_idx"], const)
+    ne2, order2, res2 = O.schur_ordering(O.Scene(d2, cam_constant=const))
+    try:
+        ne, order, res, _ = mba.schur_structure(p)
+    except mba.MbaError as e:  # a camera in E: the oracle must agree it is not the standard structure
+        assert e.status == mba.ERR_UNSUPPORTED and np.any(order2[:ne2] < 41)
+        return
+    assert ne == ne2
+    np.testing.assert_array_equal(order, order2)
+    np.testing.assert_array_equal(res, res2)
+
+
 def test_scene_generator_matches_oracle_copy(mba, O):
     for preset, window in [(1, 0), (4, 7)]:
         a = mba.generate_scene(preset, window)
