@@ -2627,6 +2627,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_scalar(
   if (threadIdx.x == 0) *counter = 0u;
 }
 
+// Inverse of the camera-major order: slot_of[src[k]] = k.
+__global__ __launch_bounds__(kBlock) void k_slot_of_row(int n, const int32_t* __restrict__ src,
+                                                        int32_t* __restrict__ slot_of) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) slot_of[src[k]] = k;
+}
+
+// A single problem's FP64 observations in row order, from the caller's
+// order uploaded as is (src: each row's observation index).
+__global__ __launch_bounds__(kBlock) void k_gather_obs(int64_t n, const int32_t* __restrict__ src,
+                                                       const double2* __restrict__ in, double2* __restrict__ out) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r < n) out[r] = in[src[r]];
+}
+
 // Per-window flagged copy of trial parameters into the current ones.
 __global__ __launch_bounds__(kBlock) void k_accept_points(int ne, Win win, const uint8_t* __restrict__ accept,
                                                           const double* __restrict__ src, double* __restrict__ dst) {

@@ -12,6 +12,7 @@
 #include <functional>
 #include <mutex>
 
+#include <sys/mman.h>
 #include <unistd.h>
 #include <cmath>
 #include <cstdio>
@@ -66,7 +67,11 @@ double now_s() {
 inline int nblocks(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
 
 // Host buffers that are fully written before they are read: default-
-// initialised (no zero fill of tens of MB on the preprocessing path).
+// initialised (no zero fill of tens of MB on the preprocessing path), and
+// from 4 MB on backed by transparent huge pages where the kernel offers them
+// (2 MB-aligned, MADV_HUGEPAGE): the first touch of a 200 MB row array by
+// 16 threads then takes ~100 page faults instead of ~50,000.
+constexpr size_t kHugePage = size_t(2) << 20, kHugeMin = size_t(4) << 20;
 template <typename T>
 struct NoInit : std::allocator<T> {
   template <typename U>
@@ -76,6 +81,22 @@ struct NoInit : std::allocator<T> {
   NoInit() = default;
   template <typename U>
   NoInit(const NoInit<U>&) noexcept {}
+  T* allocate(size_t n) {
+    const size_t bytes = n * sizeof(T);
+    if (bytes < kHugeMin) return std::allocator<T>::allocate(n);
+    const size_t sz = (bytes + kHugePage - 1) / kHugePage * kHugePage;
+    void* q = std::aligned_alloc(kHugePage, sz);
+    if (!q) throw std::bad_alloc();
+    (void)madvise(q, sz, MADV_HUGEPAGE);
+    return static_cast<T*>(q);
+  }
+  void deallocate(T* q, size_t n) {
+    if (n * sizeof(T) < kHugeMin) {
+      std::allocator<T>::deallocate(q, n);
+      return;
+    }
+    std::free(q);
+  }
   template <typename U>
   void construct(U* q) noexcept {
     ::new ((void*)q) U;
@@ -265,11 +286,21 @@ mba_status validate(const mba_problem* p) {
     return fail(MBA_ERR_INVALID_ARGUMENT, "observation arrays are NULL");
   if ((p->num_cameras > 0 && !p->cameras) || (p->num_points > 0 && !p->points))
     return fail(MBA_ERR_INVALID_ARGUMENT, "parameter arrays are NULL");
-  for (int64_t i = 0; i < p->num_observations; ++i) {
-    const int32_t c = p->camera_index[i], q = p->point_index[i];
-    if (c < 0 || c >= p->num_cameras || q < 0 || q >= p->num_points)
-      return fail(MBA_ERR_INVALID_ARGUMENT, "observation " + std::to_string(i) + " has an index out of range");
-  }
+  // Index ranges, in parallel; the first offending observation is named.
+  std::atomic<int64_t> bad{INT64_MAX};
+  parallel_for(p->num_observations, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const int32_t c = p->camera_index[i], q = p->point_index[i];
+      if (c < 0 || c >= p->num_cameras || q < 0 || q >= p->num_points) {
+        int64_t cur = bad.load();
+        while (i < cur && !bad.compare_exchange_weak(cur, i)) {
+        }
+        return;
+      }
+    }
+  }, 1 << 18);
+  if (bad.load() != INT64_MAX)
+    return fail(MBA_ERR_INVALID_ARGUMENT, "observation " + std::to_string(bad.load()) + " has an index out of range");
   return MBA_OK;
 }
 
@@ -1320,6 +1351,12 @@ mba_status alloc_precision(mba_context* c, const std::vector<int32_t>& dev_pt_st
   }
   int32_t *src = nullptr, *row_e = nullptr;
   MBA_TRY(upload(c, &src, cm_src));
+  if (c->row_cm) {
+    HIP_TRY(hipMemsetAsync(c->row_cm, 0xff, sizeof(int32_t) * (size_t)c->nrows, s));
+    if (c->ncm > 0)
+      hipLaunchKernelGGL(k_slot_of_row, dim3(nblocks(c->ncm)), dim3(kBlock), 0, s, c->ncm, (const int32_t*)src,
+                         c->row_cm);
+  }
   MBA_TRY(c->pool.alloc(&row_e, (size_t)c->nrows));
   MBA_TRY(c->pool.alloc(&d.cm_e, (size_t)c->ncm));
   MBA_TRY(c->pool.alloc(&d.cm_obs, 2 * (size_t)c->ncm));
@@ -1872,8 +1909,12 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   }
   // Rows in device order (each point's rows keep its window's reference
   // order); camera indices are global.  Points are independent.
-  Buf<int32_t> row_cam(c->nrows);
-  Buf<double> row_obs(2 * (size_t)c->nrows);
+  // A single problem's observations are uploaded in the caller's order and
+  // put in row order on the device (row_src: each row's observation), so
+  // the host reads only the camera index at random.
+  const bool obs_on_device = W == 1;
+  Buf<int32_t> row_cam(c->nrows), row_src(obs_on_device ? (size_t)c->nrows : 0);
+  Buf<double> row_obs(obs_on_device ? 0 : 2 * (size_t)c->nrows);
   parallel_for(c->ne, [&](int64_t b, int64_t end) {
     for (int64_t de = b; de < end; ++de) {
       const int32_t ge = dperm[de], w = ge_win[ge], e = ge_local[ge];
@@ -1883,8 +1924,12 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       for (int32_t rr = st.pt_start[e]; rr < st.pt_start[e + 1]; ++rr, ++r) {
         const int64_t i = st.res_order[rr];
         row_cam[r] = c->cam_off[w] + p->camera_index[i];
-        row_obs[2 * (size_t)r] = p->observations[2 * i];
-        row_obs[2 * (size_t)r + 1] = p->observations[2 * i + 1];
+        if (obs_on_device) {
+          row_src[r] = (int32_t)i;
+        } else {
+          row_obs[2 * (size_t)r] = p->observations[2 * i];
+          row_obs[2 * (size_t)r + 1] = p->observations[2 * i + 1];
+        }
       }
     }
   }, 1 << 14);
@@ -2050,14 +2095,9 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   if (!slot_chunk.empty()) MBA_TRY(upload(c, &c->d_slot_chunk, slot_chunk));
   c->rfix = c->mixed && W == 1 && !c->grouped && c->nrows > 0;
   if (c->rfix) {
-    Buf<int32_t> row_cm((size_t)c->nrows);
-    parallel_for(c->nrows, [&](int64_t b, int64_t end) {
-      for (int64_t r = b; r < end; ++r) row_cm[r] = -1;
-    });
-    parallel_for(c->ncm, [&](int64_t b, int64_t end) {
-      for (int64_t k = b; k < end; ++k) row_cm[cm_src[k]] = (int32_t)k;
-    });
-    MBA_TRY(upload(c, &c->row_cm, row_cm));
+    // row -> camera-major slot (-1: a constant camera's row), filled on the
+    // device from the camera-major order (alloc_precision).
+    MBA_TRY(c->pool.alloc(&c->row_cm, (size_t)c->nrows));
     for (int q = 0; q < 2; ++q) {
       MBA_TRY(c->pool.alloc(&c->rres[q], 2 * (size_t)c->nrows));
       MBA_TRY(c->pool.alloc(&c->cres[q], 2 * (size_t)std::max(1, c->ncm)));
@@ -2183,7 +2223,20 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   MBA_TRY(c->pool.alloc(&c->pts64_t, pts.size()));
   if (!pts.empty())
     HIP_TRY(hipMemcpyAsync(c->pts64_t, c->pts64, 8 * pts.size(), hipMemcpyDeviceToDevice, c->stream));
-  MBA_TRY(upload(c, &c->row_obs64, row_obs));
+  if (obs_on_device) {
+    double* obs_in = nullptr;
+    int32_t* src = nullptr;
+    MBA_TRY(c->pool.alloc(&obs_in, 2 * (size_t)c->nrows));
+    MBA_TRY(copy_to_device(c, obs_in, probs[0].observations, sizeof(double) * 2 * (size_t)c->nrows));
+    MBA_TRY(upload(c, &src, row_src));
+    MBA_TRY(c->pool.alloc(&c->row_obs64, 2 * (size_t)c->nrows));
+    if (c->nrows > 0)
+      hipLaunchKernelGGL(k_gather_obs, dim3(nblocks((int64_t)c->nrows)), dim3(kBlock), 0, c->stream,
+                         (int64_t)c->nrows, (const int32_t*)src, (const double2*)obs_in, (double2*)c->row_obs64);
+    HIP_TRY(hipGetLastError());
+  } else {
+    MBA_TRY(upload(c, &c->row_obs64, row_obs));
+  }
   trace.mark("state uploads");
   MBA_TRY(c->pool.alloc(&c->ctab64, kCamTab * (size_t)std::max(1, c->nc)));
   MBA_TRY(c->pool.alloc(&c->dtab64, (size_t)std::max(kDirTab, kDirTabG) * std::max(1, c->nc)));

@@ -192,6 +192,20 @@ def test_schur_structure_matches_oracle_on_edge_structures(mba, O, case):
     np.testing.assert_array_equal(res, res2)
 
 
+def test_out_of_range_index_names_the_first_observation(mba, O):
+    """The parallel index check reports the lowest offending observation."""
+    d = O.generate(O.preset(2))
+    p = ba_problem(mba, d)
+    p.point_index = p.point_index.copy()
+    p.camera_index = p.camera_index.copy()
+    p.point_index[600000] = -1
+    p.camera_index[123457] = 10 ** 6
+    p.point_index[123458] = 10 ** 7
+    with pytest.raises(mba.MbaError) as e:
+        mba.schur_structure(p)
+    assert e.value.status == mba.ERR_INVALID_ARGUMENT and "observation 123457 " in str(e.value)
+
+
 def test_scene_generator_matches_oracle_copy(mba, O):
     for preset, window in [(1, 0), (4, 7)]:
         a = mba.generate_scene(preset, window)
