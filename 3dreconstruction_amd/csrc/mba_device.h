@@ -2432,7 +2432,9 @@ __global__ __launch_bounds__(pcg_win_threads<T>()) void k_pcg_window(PcgWinArgs<
 // (active[w] == 0) marked stopped; all_done cleared.
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int nw, PcgState* __restrict__ st,
                                                      const uint8_t* __restrict__ active, double tol_r,
-                                                     int32_t* __restrict__ all_done) {
+                                                     int32_t* __restrict__ all_done,
+                                                     unsigned int* __restrict__ progress) {
+  if (threadIdx.x == 0 && progress) *progress = 0u;
   for (int w = threadIdx.x; w < nw; w += blockDim.x) {
     PcgState z;
     z.rho = z.rho_last = z.Q0 = z.alpha = z.beta = z.pq = z.norm_b = z.zeta = z.norm_r = 0.0;
@@ -2561,7 +2563,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_scalar(
     int nf, PcgState* __restrict__ st, Win win, int mode, const T* __restrict__ p, const T* __restrict__ q,
     const T* __restrict__ b, const T* __restrict__ tmp, const T* __restrict__ Minv, T* __restrict__ x,
     T* __restrict__ r, T* __restrict__ z, double* __restrict__ bpart, int32_t* __restrict__ all_done, int nw,
-    double eta, int min_iters, int max_iters, unsigned int* __restrict__ counter, int32_t* __restrict__ host_done) {
+    double eta, int min_iters, int max_iters, unsigned int* __restrict__ counter, int32_t* __restrict__ host_done,
+    unsigned int* __restrict__ progress) {
   __shared__ double lds[3 * (kBlock / 64)];
   if (*all_done) return;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -2624,7 +2627,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_scalar(
   }
   if (!last_block(counter)) return;
   pcg_scalar(bpart, gridDim.x, nw, st, win.acc, eta, min_iters, max_iters, mode == 3, all_done, host_done);
-  if (threadIdx.x == 0) *counter = 0u;
+  if (threadIdx.x == 0) {
+    *counter = 0u;
+    // Completed updates, for the host's look-ahead (host_done[1], pinned).
+    if (progress && host_done) host_done[1] = (int32_t)(++*progress);
+  }
 }
 
 // Inverse of the camera-major order: slot_of[src[k]] = k.

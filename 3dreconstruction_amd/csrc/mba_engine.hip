@@ -1255,7 +1255,6 @@ struct mba_context {
   size_t stage_bytes = 0;
   int stage_next = 0;
   hipEvent_t stage_ev[kStage] = {};
-  hipEvent_t look_ev[kLook] = {};  // PCG look-ahead: one per in-flight iteration
   // Captured PCG iterations, [row-weight slot][residual reset]; made on
   // first use.
   hipGraphExec_t pcg_graph[2][2] = {};
@@ -1434,8 +1433,6 @@ void context_free(mba_context* c) {
   for (int i = 0; i < 8; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   for (hipEvent_t e : c->stage_ev)
-    if (e) (void)hipEventDestroy(e);
-  for (hipEvent_t e : c->look_ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& row : c->pcg_graph)
     for (hipGraphExec_t g : row)
@@ -1800,12 +1797,14 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   c->ng = c->grouped ? c->wst[0].ng : 0;
   c->nslot = c->nc + c->ng;
   // Global E blocks in window order, each window in its reference E order.
-  std::vector<int32_t> ge_win(c->ne), ge_local(c->ne);
-  for (int32_t w = 0; w < W; ++w)
-    for (int32_t e = 0; e < c->wst[w].num_e; ++e) {
-      ge_win[c->e_off[w] + e] = w;
-      ge_local[c->e_off[w] + e] = e;
-    }
+  Buf<int32_t> ge_win(c->ne), ge_local(c->ne);
+  parallel_for(W, [&](int64_t b, int64_t end) {
+    for (int64_t w = b; w < end; ++w)
+      for (int32_t e = 0; e < c->wst[w].num_e; ++e) {
+        ge_win[c->e_off[w] + e] = (int32_t)w;
+        ge_local[c->e_off[w] + e] = e;
+      }
+  }, 16);
   auto track_len = [&](int32_t ge) {
     const Structure& st = c->wst[ge_win[ge]];
     const int32_t e = ge_local[ge];
@@ -1882,14 +1881,15 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     }
     c->seg_blocks = blk;
     c->dev_e_to_pt.resize(c->ne);
+    Buf<int32_t> dlen(c->ne);
     parallel_for(c->ne, [&](int64_t b, int64_t end) {
       for (int64_t de = b; de < end; ++de) {
         const int32_t ge = dperm[de], w = ge_win[ge];
         c->dev_e_to_pt[de] = c->pt_off[w] + c->wst[w].e_to_pt[ge_local[ge]];
-        dev_pt_start[de + 1] = track_len(ge);
+        dlen[de] = track_len(ge);
       }
     }, 1 << 14);
-    for (int32_t de = 0; de < c->ne; ++de) dev_pt_start[de + 1] += dev_pt_start[de];
+    exclusive_scan(dlen.data(), c->ne, dev_pt_start.data());
   }
   trace.mark("device point order");
   // F blocks: window-contiguous, each window in its reference F order.
@@ -2335,7 +2335,6 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   trace.mark("precision layout + S pairs");
   for (int i = 0; i < 8; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
   for (hipEvent_t& e : c->stage_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  for (hipEvent_t& e : c->look_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (hipEvent_t& e : c->tev) HIP_TRY(hipEventCreate(&e));
   if (c->opt.linear_solver == MBA_EXPLICIT_SCHUR) {
     HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
@@ -2696,6 +2695,21 @@ struct Engine {
                        0, s, c->nf, d.f_lo, d.f_hi, d.partial, d.U, c->diag_c, wv, x, 1, y, done);
   }
 
+  // Spin until the device's count of completed PCG updates (pinned
+  // h_done[1]) reaches `target` or it reports all stopped; a stream that
+  // has drained ends the wait too.
+  mba_status wait_progress(int target) {
+    for (int spin = 1; !*c->h_done && c->h_done[1] < target; ++spin) {
+      if (spin % 256 == 0) {
+        const hipError_t q = hipStreamQuery(s);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady) HIP_TRY(q);
+      }
+      __builtin_ia32_pause();
+    }
+    return MBA_OK;
+  }
+
   // PCG on S x = rhs (x in d.x) for every window still iterating (the
   // device's active flags); each window stops on its own tests, the
   // launches stop when all have.  The host runs at most kLook iterations
@@ -2748,8 +2762,9 @@ struct Engine {
       return MBA_OK;
     }
     *c->h_done = 0;
+    c->h_done[1] = 0;
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kBlock), 0, s, W, c->pcg, (const uint8_t*)(W > 1 ? c->wactive : nullptr),
-                       c->opt.pcg_r_tolerance, c->all_done);
+                       c->opt.pcg_r_tolerance, c->all_done, c->counter + 3);
     // Windowed accumulators: dot at wacc[0, W), update sums at wacc[W, 4W);
     // the scalar kernels clear what they read.
     double* dot_acc = c->W > 1 ? c->wacc : nullptr;
@@ -2761,7 +2776,7 @@ struct Engine {
     auto update = [&](int mode) {
       hipLaunchKernelGGL(k_pcg_update_scalar<T>, dim3(nbg), dim3(kBlock), 0, s, nf, c->pcg, wu, mode, d.p, d.q, d.rhs,
                          d.tmp, d.Minv, d.x, d.r, d.zc, c->red, c->all_done, W, eta, minit, maxit, c->counter + 1,
-                         c->d_done);
+                         c->d_done, mode == 1 ? nullptr : c->counter + 3);
     };
     const Win wv = win(nullptr, true);
     // One CG iteration: direction, implicit matvec, alpha, and the x / r
@@ -2818,14 +2833,13 @@ struct Engine {
       return MBA_OK;
     };
     update(3);
-    HIP_TRY(hipEventRecord(c->look_ev[0], s));
     for (int it = 1; it <= maxit; ++it) {
-      // Wait for iteration it - kLook (the update launched kLook rounds
-      // ago), then stop if the device has finished.
-      HIP_TRY(hipEventSynchronize(c->look_ev[it % kLook]));
+      // Wait until the device has finished iteration it - kLook (its update
+      // kernel counts completed updates into pinned memory, the first one
+      // before the loop), then stop if it has finished altogether.
+      MBA_TRY(wait_progress(it + 1 - kLook));
       if (*c->h_done) break;
       MBA_TRY(launch(it % 10 == 0 ? 1 : 0));
-      HIP_TRY(hipEventRecord(c->look_ev[it % kLook], s));
     }
     HIP_TRY(hipMemcpyAsync(c->h_pcgs, c->pcg, sizeof(PcgState) * W, hipMemcpyDeviceToHost, s));
     return MBA_OK;
