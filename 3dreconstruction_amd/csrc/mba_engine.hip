@@ -446,9 +446,16 @@ bool build_structure_points_first(const mba_problem* p, Structure* st, PrepTrace
   // cursors: any order) and sorted descending below.
   Buf<int32_t> cnt(np);
   parallel_for(np, [&](int64_t b, int64_t e) { std::fill(cnt.begin() + b, cnt.begin() + e, 0); }, 1 << 18);
-  parallel_for(no, [&](int64_t b, int64_t e) {
-    for (int64_t i = b; i < e; ++i) __atomic_fetch_add(&cnt[pt[i]], 1, __ATOMIC_RELAXED);
-  });
+  // One host thread (a small problem, or a window of a batch): plain
+  // increments; several: relaxed atomics.
+  const bool serial = host_threads(no) == 1;
+  if (serial) {
+    for (int64_t i = 0; i < no; ++i) cnt[pt[i]]++;
+  } else {
+    parallel_for(no, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; ++i) __atomic_fetch_add(&cnt[pt[i]], 1, __ATOMIC_RELAXED);
+    });
+  }
   Buf<int32_t> pstart((size_t)np + 1);
   exclusive_scan(cnt.data(), np, pstart.data());
   Buf<int32_t> cursor(np);
@@ -456,9 +463,13 @@ bool build_structure_points_first(const mba_problem* p, Structure* st, PrepTrace
     for (int64_t q = b; q < e; ++q) cursor[q] = pstart[q + 1];
   }, 1 << 18);
   Buf<int32_t> rows(no);
-  parallel_for(no, [&](int64_t b, int64_t e) {
-    for (int64_t i = b; i < e; ++i) rows[__atomic_sub_fetch(&cursor[pt[i]], 1, __ATOMIC_RELAXED)] = (int32_t)i;
-  });
+  if (serial) {
+    for (int64_t i = 0; i < no; ++i) rows[--cursor[pt[i]]] = (int32_t)i;
+  } else {
+    parallel_for(no, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; ++i) rows[__atomic_sub_fetch(&cursor[pt[i]], 1, __ATOMIC_RELAXED)] = (int32_t)i;
+    });
+  }
   tr.mark("  rows by point");
   // Per point: sort its list descending, its degree; per camera (thread
   // histograms): distinct points and first appearance.
@@ -2289,11 +2300,32 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     bool fits = true;
     for (int32_t w = 0; w < W && fits; ++w) fits = c->cam_off[w + 1] - c->cam_off[w] <= kPcgWinMaxCams;
     if (fits) {
+      // Device points by window (each window's in device order): a stable
+      // counting sort with per-range histograms.
       std::vector<int32_t> cnt((size_t)W + 1, 0), list(c->ne);
-      for (int32_t de = 0; de < c->ne; ++de) cnt[ge_win[dperm[de]] + 1]++;
-      for (int32_t w = 0; w < W; ++w) cnt[w + 1] += cnt[w];
-      std::vector<int32_t> run(cnt.begin(), cnt.end() - 1);
-      for (int32_t de = 0; de < c->ne; ++de) list[run[ge_win[dperm[de]]]++] = de;
+      {
+        const int nt = host_threads(c->ne, 1 << 15);
+        const int64_t step = ((int64_t)c->ne + nt - 1) / nt;
+        std::vector<std::vector<int32_t>> h(nt, std::vector<int32_t>((size_t)W, 0));
+        run_threads(nt, [&](int t) {
+          for (int64_t de = t * step, e = std::min<int64_t>(c->ne, (t + 1) * step); de < e; ++de)
+            h[t][ge_win[dperm[de]]]++;
+        });
+        int32_t run = 0;
+        for (int32_t w = 0; w < W; ++w) {
+          cnt[w] = run;
+          for (int t = 0; t < nt; ++t) {
+            const int32_t k = h[t][w];
+            h[t][w] = run;
+            run += k;
+          }
+        }
+        cnt[W] = run;
+        run_threads(nt, [&](int t) {
+          for (int64_t de = t * step, e = std::min<int64_t>(c->ne, (t + 1) * step); de < e; ++de)
+            list[h[t][ge_win[dperm[de]]]++] = (int32_t)de;
+        });
+      }
       std::vector<int32_t> lo(c->nf, 0), hi(c->nf, 0);
       for (int32_t f = 0; f < c->nf; ++f)
         if (f_hi[f] > f_lo[f]) {
