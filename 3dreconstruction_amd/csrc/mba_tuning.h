@@ -18,8 +18,8 @@ struct TunedShape {
 constexpr TunedShape kTunedShapes[] = {
     {int64_t(1) << 18, 4, 0},
     {int64_t(2097152), 4, 1024},  // adjoint_check_697k
-    {int64_t(8388608), 4, 768},  // mixed_precision_4.0M
-    {INT64_MAX, 8, 512},  // large_scene_25M
+    {int64_t(8388608), 8, 1536},  // mixed_precision_4.0M
+    {INT64_MAX, 4, 1536},  // large_scene_25M
 };
 
 inline TunedShape tuned_shape(int64_t rows) {

@@ -206,6 +206,27 @@ def test_schur_matvec_matches_dense_schur_complement(mba, O):
     np.testing.assert_allclose(y.ravel(), S @ x.ravel(), rtol=1e-9, atol=1e-9 * np.abs(S @ x.ravel()).max())
 
 
+@pytest.mark.parametrize("ncams", [200, 1500])
+def test_mixed_pass_and_matvec_with_staged_camera_records(mba, O, ncams):
+    """The FP32 point-major kernels read camera records staged in LDS when the
+    table fits (pass up to ~2,000 cameras, matvec up to ~1,100): 200 cameras
+    run both staged, 1,500 the staged pass beside the gathering matvec.  The
+    mixed pass's V and g_p and the mixed implicit S x against the FP64 path's
+    (which gathers): within FP32 rounding, 1e-5 of each block's scale."""
+    d = O.generate(O.make_spec(ncams, 12 * ncams, 2, 8, seed=41))
+    out = {}
+    for prec in (0, 1):
+        with mba.Context(ba_problem(mba, d), mba.default_options(precision=prec)) as ctx:
+            ctx.jacobian_schur(1e4)
+            blocks = ctx.read_schur()
+            x = np.random.default_rng(7).standard_normal((ncams, 9))
+            out[prec] = (blocks, ctx.schur_matvec(x))
+    (b64, y64), (b32, y32) = out[0], out[1]
+    assert block_err(b32["V"], b64["V"], 3) < 1e-5
+    assert float(np.max(np.abs(b32["gp"] - b64["gp"]))) < 1e-5 * float(np.max(np.abs(b64["gp"])))
+    assert float(np.max(np.abs(y32 - y64))) < 1e-5 * float(np.max(np.abs(y64)))
+
+
 # -------------------------------------------------------------- LM solves --
 @pytest.mark.parametrize("solver,name", [(0, "sparse"), (1, "iterative")])
 def test_fp64_solve_matches_reference_ceres(mba, golden, c1, solver, name):
