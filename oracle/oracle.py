@@ -380,6 +380,12 @@ def ref_set_preconditioner(kind: int):
     ref_lib().ref_set_preconditioner(C.c_int(kind))
 
 
+def ref_set_lm_tolerances(eta=0.0, function_tolerance=0.0):
+    """Solver::Options::eta and ::function_tolerance of ref_solve; 0 keeps
+    Ceres's defaults (1e-1 and 1e-6)."""
+    ref_lib().ref_set_lm_tolerances(C.c_double(eta), C.c_double(function_tolerance))
+
+
 def ref_set_dump(directory, iterations):
     """trust_region_minimizer_iterations_to_dump / trust_region_problem_dump_directory
     (TEXTFILE) of ref_solve; an empty list turns dumps off."""

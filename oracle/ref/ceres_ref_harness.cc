@@ -202,6 +202,10 @@ std::vector<int> g_dump_iters;
 std::string g_dump_dir;
 // Summary::message of the last ref_solve.
 std::string g_message;
+// Solver::Options::eta and ::function_tolerance of ref_solve; a value <= 0
+// keeps Ceres's default (eta 1e-1, function tolerance 1e-6).
+double g_eta = 0.0;
+double g_function_tolerance = 0.0;
 
 ceres::PreconditionerType PrecondType(int k) {
   switch (k) {
@@ -243,6 +247,10 @@ extern "C" {
 void ref_set_points_first(int on) { g_points_first = on; }
 void ref_set_schur_jacobi(int on) { g_precond = on ? 1 : 0; }
 void ref_set_preconditioner(int k) { g_precond = k; }
+void ref_set_lm_tolerances(double eta, double function_tolerance) {
+  g_eta = eta;
+  g_function_tolerance = function_tolerance;
+}
 void ref_set_dump(const char* dir, const int* iters, int n) {
   g_dump_dir = dir ? dir : "";
   g_dump_iters.assign(iters, iters + (n > 0 ? n : 0));
@@ -516,6 +524,8 @@ int ref_solve(int layout, int nc, int ni, int np, long nobs, double* cams, doubl
   options.minimizer_progress_to_stdout = false;
   options.logging_type = ceres::SILENT;
   options.max_num_iterations = max_iters;
+  if (g_eta > 0.0) options.eta = g_eta;
+  if (g_function_tolerance > 0.0) options.function_tolerance = g_function_tolerance;
   options.num_threads = num_threads;
   options.num_linear_solver_threads = num_threads;
   options.trust_region_minimizer_iterations_to_dump = g_dump_iters;

@@ -5,7 +5,9 @@ checkout; the fixtures it writes are plain numbers (inputs are regenerated
 from the seeded scene generator, so only seeds, indices and outputs are
 stored).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # ceres_ref.npz
+    python tests/golden/make_golden.py --large  # ceres_ref_large.npz
+    python tests/golden/make_golden.py --lm     # ceres_ref_lm.npz
 """
 from __future__ import annotations
 
@@ -457,8 +459,41 @@ def main_large():
     print(path, os.path.getsize(path), "bytes")
 
 
+# Whole LM solves to convergence at the benchmark's LM settings (PCG forcing
+# eta 1e-2, function tolerance 1e-12, at most 300 iterations) on the
+# adjoint-check scene (200 cameras, 100,000 points, 697k observations, no
+# loss) with ITERATIVE_SCHUR and JACOBI or SCHUR_JACOBI, one thread.
+LM_CASES = [("c2_lm_jacobi", 2, 0.0, 0), ("c2_lm_schur_jacobi", 2, 0.0, 1)]
+LM_ETA, LM_FUNCTION_TOLERANCE, LM_MAX_ITERATIONS = 1e-2, 1e-12, 300
+
+
+def main_lm():
+    """Writes ceres_ref_lm.npz: per case the summary and the iteration
+    records (as ceres_ref_large.npz) of the whole solve."""
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref/libceres_ref.so is not built (make -C oracle/ref)")
+    out = {}
+    for name, preset, huber, precond in LM_CASES:
+        sc = O.Scene(O.generate(O.preset(preset)))
+        O.ref_set_preconditioner(precond)
+        O.ref_set_lm_tolerances(LM_ETA, LM_FUNCTION_TOLERANCE)
+        try:
+            r = O.ref_solve(sc, huber_a=huber, linear_solver=1, max_iters=LM_MAX_ITERATIONS, num_threads=1)
+        finally:
+            O.ref_set_preconditioner(0)
+            O.ref_set_lm_tolerances()
+        for k in ["initial_cost", "final_cost", "successful", "unsuccessful", "termination", "iterations"]:
+            out[f"{name}_{k}"] = np.asarray(r[k])
+        print(name, r["final_cost"], len(r["iterations"]), r["termination"], flush=True)
+    path = os.path.join(HERE, "ceres_ref_lm.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
+
+
 if __name__ == "__main__":
     if "--large" in sys.argv:
         main_large()
+    elif "--lm" in sys.argv:
+        main_lm()
     else:
         main()
