@@ -3225,87 +3225,193 @@ __global__ __launch_bounds__(256) void k_syrk_mfma(T* __restrict__ A, int64_t n,
   tile_sub_lower(up, A, n, r0, c0);
 }
 
-// Triangular solves with the factor, one launch per 64-row block, using the
-// stored inverses of the diagonal blocks (no serial chain inside a step).
-// Forward L y = b, step k: every workgroup forms y_k = L_kk^-1 b_k (b_k is
-// final after step k - 1); the workgroup of block k writes y_k, the one of
-// block i > k subtracts L_ik y_k from b_i.  b and y are distinct vectors.
-// Launch over blocks k .. nb-1.
+// Triangular solves with the factor in one launch per direction: one
+// workgroup of kFlowThreads threads per 64-row block.  A block's solution
+// is handed to the workgroups that need it as 64-bit words (32 bits of the
+// value, the solve's epoch), each stored and polled with agent-scope
+// relaxed atomics: a word that carries the epoch is this solve's value, so
+// there is no flag, fence or barrier between the blocks.  A workgroup waits
+// only on blocks that lower-numbered workgroups own, and workgroups are
+// dispatched in order, so every block it waits on is owned by a workgroup
+// already resident or finished.  Each wait is bounded (kFlowSpinMax polls):
+// a wait that runs out stores the epoch into err, the other waits then give
+// up too, and the blocks written after that are NaN, so a failed solve
+// reaches the LM as a non-finite step instead of a hang.  Per-thread
+// partial sums over all earlier blocks are reduced once, at the end.
+constexpr int kFlowThreads = 4 * 64;
+constexpr unsigned kFlowSpinMax = 1u << 20;
+
 template <typename T>
-__global__ __launch_bounds__(64) void k_trsv_lower_inv(const T* __restrict__ L, const T* __restrict__ Linv,
-                                                       int64_t n, int k, T* __restrict__ b, T* __restrict__ y) {
-  __shared__ T bs[kNB], xs[kNB];
-  const int lane = threadIdx.x;
-  const int64_t k0 = (int64_t)k * kNB;
-  const int kb = (int)min((int64_t)kNB, n - k0);
-  bs[lane] = lane < kb ? b[k0 + lane] : T(0);
-  __syncthreads();
-  const T* Li = Linv + (size_t)k * kNB * kNB;
-  T acc[4] = {T(0), T(0), T(0), T(0)};
+__host__ __device__ constexpr int flow_words() { return (int)(sizeof(T) / 4); }
+
+template <typename T>
+__device__ __forceinline__ void flow_put(uint64_t* dst, T v, unsigned epoch) {
+  unsigned u[flow_words<T>()];
+  __builtin_memcpy(u, &v, sizeof(T));
 #pragma unroll
-  for (int c = 0; c < kNB; ++c) acc[c & 3] = fma(Li[lane + kNB * c], bs[c], acc[c & 3]);
-  const T yk = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-  const int i = k + blockIdx.x;
-  if (i == k) {
-    if (lane < kb) y[k0 + lane] = yk;
-    return;
-  }
-  xs[lane] = yk;
-  __syncthreads();
-  // Block k precedes block i, so it is full.
-  const int64_t i0 = (int64_t)i * kNB;
-  const int ib = (int)min((int64_t)kNB, n - i0);
-  const bool on = lane < ib;
-  const T* col = L + (on ? i0 + lane : i0) + k0 * n;
-  T a2[4] = {T(0), T(0), T(0), T(0)};
-#pragma unroll
-  for (int c = 0; c < kNB; ++c) a2[c & 3] = fma(col[c * n], xs[c], a2[c & 3]);
-  if (on) b[i0 + lane] -= (a2[0] + a2[1]) + (a2[2] + a2[3]);
+  for (int c = 0; c < flow_words<T>(); ++c)
+    __hip_atomic_store(dst + c, ((uint64_t)epoch << 32) | u[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Backward L' x = y, step k (k = nb - 1 first): every workgroup forms
-// x_k = L_kk^-T y_k from the stored transpose; the workgroup of block k
-// writes x_k, the one of block i < k subtracts L_ki' x_k from y_i (the
-// 64 x ib tile of L staged through LDS for coalesced reads).  Launch over
-// blocks 0 .. k.
+// Polls the N values at src (flow_words<T>() words each) until every word
+// carries epoch; false when the wait ran out or another one had (err ==
+// epoch).
+template <typename T, int N>
+__device__ __forceinline__ bool flow_get(const uint64_t* src, unsigned epoch, uint64_t* err, T out[N]) {
+  constexpr int C = flow_words<T>();
+  uint64_t w[N * C];
+  bool good = true;
+  for (unsigned spins = 0;; ++spins) {
+    bool all = true;
+#pragma unroll
+    for (int q = 0; q < N * C; ++q) {
+      w[q] = __hip_atomic_load(src + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      all = all && (unsigned)(w[q] >> 32) == epoch;
+    }
+    if (all) break;
+    if (spins >= kFlowSpinMax ||
+        ((spins & 63u) == 63u && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)) {
+      __hip_atomic_store(err, (uint64_t)epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      good = false;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    unsigned u[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) u[c] = (unsigned)w[j * C + c];
+    __builtin_memcpy(&out[j], u, sizeof(T));
+  }
+  return good;
+}
+
+// The thread's 16 entries of the block's stored 64 x 64 inverse M
+// (column-major): row lane, columns 16w..16w+15.  Loaded at the start, off
+// the chain of blocks.
 template <typename T>
-__global__ __launch_bounds__(64) void k_trsv_upper_inv(const T* __restrict__ L, const T* __restrict__ LinvT,
-                                                       int64_t n, int k, T* __restrict__ y, T* __restrict__ x) {
-  __shared__ T tile[kNB][kNB + 1];
-  __shared__ T ys[kNB], xs[kNB];
-  const int lane = threadIdx.x;
-  const int64_t k0 = (int64_t)k * kNB;
-  const int kb = (int)min((int64_t)kNB, n - k0);
-  ys[lane] = lane < kb ? y[k0 + lane] : T(0);
-  __syncthreads();
-  const T* Lt = LinvT + (size_t)k * kNB * kNB;
-  T acc[4] = {T(0), T(0), T(0), T(0)};
+__device__ __forceinline__ void flow_block_load(const T* __restrict__ M, T m[16]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-  for (int c = 0; c < kNB; ++c) acc[c & 3] = fma(Lt[lane + kNB * c], ys[c], acc[c & 3]);
-  const T xk = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  for (int j = 0; j < 16; ++j) m[j] = M[lane + kNB * (16 * w + j)];
+}
+
+// d = M s: wave w takes columns 16w..16w+15 (m from flow_block_load), the
+// four partial column sums meet in LDS.  s in LDS (zero past the block's
+// rows); every thread calls it.
+template <typename T>
+__device__ __forceinline__ T flow_block_gemv(const T m[16], const T* sv, T (*part)[kNB]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T a = T(0);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) a = fma(m[j], sv[16 * w + j], a);
+  part[w][lane] = a;
+  __syncthreads();
+  return (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+}
+
+// Wave 0 writes the block's values (rows < nrows) to out and, as epoch
+// words, to hand.
+template <typename T>
+__device__ __forceinline__ void flow_publish(T* out, uint64_t* hand, int nrows, T v, unsigned epoch) {
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x < 64 && lane < nrows) {
+    out[lane] = v;
+    flow_put(hand + (size_t)flow_words<T>() * lane, v, epoch);
+  }
+}
+
+// Forward L y = b, workgroup i = block i: y_i = L_ii^-1 (b_i - sum_{k<i}
+// L_ik y_k).  Thread (wave w, lane) accumulates row lane of columns
+// 16w..16w+15 of every L_ik (coalesced column reads), loading the tile
+// before polling for y_k.  hand: n values as epoch words.
+template <typename T>
+__global__ __launch_bounds__(kFlowThreads) void k_trsv_lower_flow(const T* __restrict__ L, const T* __restrict__ Linv,
+                                                                  int64_t n, const T* __restrict__ b,
+                                                                  T* __restrict__ y, uint64_t* hand, unsigned epoch,
+                                                                  uint64_t* err) {
+  constexpr int C = flow_words<T>();
+  __shared__ T part[4][kNB];
+  __shared__ T sv[kNB];
   const int i = blockIdx.x;
-  if (i == k) {
-    if (lane < kb) x[k0 + lane] = xk;
-    return;
-  }
-  xs[lane] = xk;
-  // Block i precedes block k, so it is full: tile[r][c] = L[k0 + r][i0 + c]
-  // for rows r < kb (zero past them).
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t i0 = (int64_t)i * kNB;
-  {
-    const bool on = lane < kb;
-    const T* row = L + (on ? k0 + lane : k0) + i0 * n;
-    T v[kNB];
+  const int ib = (int)min((int64_t)kNB, n - i0);
+  const int64_t row = i0 + (lane < ib ? lane : 0);
+  T m[16];
+  flow_block_load(Linv + (size_t)i * kNB * kNB, m);
+  const T bi = w == 0 && lane < ib ? b[i0 + lane] : T(0);
+  T acc = T(0);
+  bool good = true;
+  for (int k = 0; k < i && good; ++k) {
+    // Block k < i is full.
+    const int64_t c0 = (int64_t)k * kNB + 16 * w;
+    const T* col = L + row + c0 * n;
+    T l[16], yk[16];
 #pragma unroll
-    for (int c = 0; c < kNB; ++c) v[c] = on ? row[c * n] : T(0);
+    for (int j = 0; j < 16; ++j) l[j] = col[j * n];
+    good = flow_get<T, 16>(hand + C * c0, epoch, err, yk);
 #pragma unroll
-    for (int c = 0; c < kNB; ++c) tile[lane][c] = v[c];
+    for (int j = 0; j < 16; ++j) acc = fma(l[j], yk[j], acc);
   }
+  part[w][lane] = acc;
+  good = __syncthreads_and(good) != 0;
+  if (w == 0) sv[lane] = lane < ib ? bi - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) : T(0);
   __syncthreads();
-  T a2[4] = {T(0), T(0), T(0), T(0)};
+  T yi = flow_block_gemv(m, sv, part);
+  if (!good) yi = T(NAN);
+  flow_publish(y + i0, hand + C * i0, ib, yi, epoch);
+}
+
+// Backward L' x = y, workgroup j = block nb - 1 - j: x_i = L_ii^-T (y_i -
+// sum_{k>i} L_ki' x_k).  Thread (wave w, lane) accumulates lane's share of
+// rows 16w..16w+15 of every L_ki' (column i0 + 16w + r of L, read along
+// its rows k0 + lane: coalesced), summed over the wave once at the end.
+template <typename T>
+__global__ __launch_bounds__(kFlowThreads) void k_trsv_upper_flow(const T* __restrict__ L, const T* __restrict__ LinvT,
+                                                                  int64_t n, int nb, const T* __restrict__ y,
+                                                                  T* __restrict__ x, uint64_t* hand, unsigned epoch,
+                                                                  uint64_t* err) {
+  constexpr int C = flow_words<T>();
+  __shared__ T part[4][kNB];
+  __shared__ T sv[kNB];
+  const int i = nb - 1 - (int)blockIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t i0 = (int64_t)i * kNB;
+  const int ib = (int)min((int64_t)kNB, n - i0);
+  T m[16];
+  flow_block_load(LinvT + (size_t)i * kNB * kNB, m);
+  const T yi = lane < 16 && 16 * w + lane < ib ? y[i0 + 16 * w + lane] : T(0);
+  T acc[16];
 #pragma unroll
-  for (int r = 0; r < kNB; ++r) a2[r & 3] = fma(tile[r][lane], xs[r], a2[r & 3]);
-  y[i0 + lane] -= (a2[0] + a2[1]) + (a2[2] + a2[3]);
+  for (int r = 0; r < 16; ++r) acc[r] = T(0);
+  bool good = true;
+  for (int k = nb - 1; k > i && good; --k) {
+    // Block i < k is full; block k is the last (possibly partial) one.
+    const int64_t k0 = (int64_t)k * kNB;
+    const bool on = lane < (int)min((int64_t)kNB, n - k0);
+    const T* p = L + (k0 + (on ? lane : 0)) + (i0 + 16 * w) * n;
+    T l[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) l[r] = on ? p[r * n] : T(0);
+    T xk[1] = {T(0)};
+    if (on) good = flow_get<T, 1>(hand + C * (k0 + lane), epoch, err, xk);
+    good = __all(good);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = fma(l[r], xk[0], acc[r]);
+  }
+  good = __syncthreads_and(good) != 0;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const T v = wave_sum(acc[r]);
+    if (lane == r) acc[0] = v;  // lane r keeps row 16w + r
+  }
+  if (lane < 16) sv[16 * w + lane] = 16 * w + lane < ib ? yi - acc[0] : T(0);
+  __syncthreads();
+  T xi = flow_block_gemv(m, sv, part);
+  if (!good) xi = T(NAN);
+  flow_publish(x + i0, hand + C * i0, ib, xi, epoch);
 }
 
 // FP64 residual r = b - S x (S column-major, both triangles stored) in two

@@ -1107,7 +1107,11 @@ struct Dev {
   T *S = nullptr, *L = nullptr, *sv = nullptr, *WY = nullptr, *Linv = nullptr, *cross = nullptr;
   // Inverses of the factor's 64 x 64 diagonal blocks and their transposes
   // (one per block), and the triangular solves' two work vectors.
-  T *LinvT = nullptr, *tb = nullptr, *ty = nullptr;
+  T *LinvT = nullptr, *ty = nullptr;
+  // Triangular solves: n values as epoch words (two per FP64 value), then
+  // the failed-epoch word; epoch of the last launch (host side).
+  uint64_t* flow = nullptr;
+  unsigned flow_epoch = 0;
   T* s_scratch = nullptr;
   // SCHUR_JACOBI preconditioner terms (IMPLICIT_SCHUR_PCG only)
   T* partial_sj = nullptr;
@@ -2886,17 +2890,17 @@ struct Engine {
     }
   }
 
-  // In-place solve with the factor: L y = v, then L' x = y (work vectors
-  // d.tb, d.ty; one launch per 64-row block and direction).
+  // In-place solve with the factor: L y = v, then L' x = y (work vector
+  // d.ty; one launch per direction, blocks handed on through d.flow under a
+  // new epoch per launch).
   void trsv(T* v, int64_t n) {
     const int nb = (int)((n + kNB - 1) / kNB);
-    (void)hipMemcpyAsync(d.tb, v, sizeof(T) * n, hipMemcpyDeviceToDevice, s);
-    for (int k = 0; k < nb; ++k)
-      hipLaunchKernelGGL(k_trsv_lower_inv<T>, dim3(nb - k), dim3(64), 0, s, (const T*)d.L, (const T*)d.Linv, n, k,
-                         d.tb, d.ty);
-    for (int k = nb - 1; k >= 0; --k)
-      hipLaunchKernelGGL(k_trsv_upper_inv<T>, dim3(k + 1), dim3(64), 0, s, (const T*)d.L, (const T*)d.LinvT, n, k,
-                         d.ty, v);
+    uint64_t* err = d.flow + (size_t)flow_words<T>() * n;
+    const unsigned e1 = ++d.flow_epoch, e2 = ++d.flow_epoch;
+    hipLaunchKernelGGL(k_trsv_lower_flow<T>, dim3(nb), dim3(kFlowThreads), 0, s, (const T*)d.L, (const T*)d.Linv, n,
+                       (const T*)v, d.ty, d.flow, e1, err);
+    hipLaunchKernelGGL(k_trsv_upper_flow<T>, dim3(nb), dim3(kFlowThreads), 0, s, (const T*)d.L, (const T*)d.LinvT, n,
+                       nb, (const T*)d.ty, v, d.flow, e2, err);
   }
 
   // Explicit Schur (single problem): assemble dense S, Cholesky, solve,
@@ -2953,8 +2957,10 @@ struct Engine {
       const size_t nbk = (size_t)((n + kNB - 1) / kNB);
       MBA_TRY(c->pool.alloc(&d.Linv, nbk * kNB * kNB));
       MBA_TRY(c->pool.alloc(&d.LinvT, nbk * kNB * kNB));
-      MBA_TRY(c->pool.alloc(&d.tb, (size_t)n));
       MBA_TRY(c->pool.alloc(&d.ty, (size_t)n));
+      MBA_TRY(c->pool.alloc(&d.flow, (size_t)flow_words<T>() * n + 1));
+      HIP_TRY(hipMemsetAsync(d.flow, 0, sizeof(uint64_t) * ((size_t)flow_words<T>() * n + 1), s));
+      d.flow_epoch = 0;
       if (!c->gemv_part) MBA_TRY(c->pool.alloc(&c->gemv_part, (size_t)kGemvSplit * n));
     }
     if (c->timing) (void)hipEventRecord(c->ev[4], s);
