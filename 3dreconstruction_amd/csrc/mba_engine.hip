@@ -1136,7 +1136,8 @@ constexpr int kSumSlots = 8;  // per-window sums read back per LM iteration
 // and |g|^2 (two slots), a slot for sums read at once.
 constexpr int kSlotModel = 0, kSlotPlus = 1, kSlotCost = 2, kSlotXnorm = 3, kSlotGrad = 4, kSlotSync = 7;
 constexpr int kStage = 8;     // pinned upload slots in rotation
-constexpr int kLook = 4;      // PCG iterations the host may run ahead
+constexpr int kLook = 2;      // PCG iterations the host may run ahead (each queued
+                              // past the last costs ~22 us of early-exit launches)
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 }  // namespace

@@ -14,7 +14,8 @@ Also reported:
     until the RMS reprojection error (from the FP64-evaluated cost) is within
     1e-6 relative of the FP64 path's final RMS after its iteration cap, and
     the stricter time until the cost itself is within 1e-6; the mixed run may
-    take 1.5x the iterations;
+    take 1.5x the iterations; the same with SCHUR_JACOBI preconditioning, and
+    both preconditioners solved to convergence on the 697k-observation scene;
   * the explicit-Schur LM variant on the same scene (dense 9,000 x 9,000
     reduced camera matrix, FP32 MFMA Cholesky, FP64 refinement): per-phase
     times of one solve and the factorisation's MFMA rate;
@@ -23,8 +24,9 @@ Also reported:
   * the 10,000-camera / 4M-point / ~25M-observation scene: the pass and the
     LM solve with block-Jacobi PCG to relative residual 1e-6;
   * the dominant kernel's roofline (HIP events on the engine's stream);
-  * the CPU baseline: the FP64 oracle's Jacobian + Schur pass on the host
-    cores (OpenMP), on the same scene.
+  * the grouped-intrinsics scene (the incremental engine's residual blocks);
+  * the CPU baseline: the reference's Ceres Jacobian + Schur pass and the FP64
+    oracle's on the host cores (OpenMP), on the same scene.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -259,6 +261,54 @@ def run(args):
                                         f"final cost",
                               "time_to_tol_s": {"mixed": t_mixed, "fp64": t_fp64},
                               "speedup_mixed_vs_fp64": (t_fp64 / t_mixed) if (t_mixed and t_fp64) else None}}
+
+        # The same LM with SCHUR_JACOBI (both precisions, same caps), against
+        # the JACOBI FP64 final cost; and both preconditioners to convergence
+        # on the 697k-observation scene.
+        def lm_record(s, wall, ref, target):
+            return {"final_cost": s.final_cost, "final_cost_rel_to_jacobi_fp64": (s.final_cost - ref) / ref,
+                    "termination": s.termination_type, "lm_iterations": len(s.iterations) - 1,
+                    "steps": [s.num_successful_steps, s.num_unsuccessful_steps],
+                    "invalid_steps": sum(1 for it in s.iterations[1:] if not it.step_is_valid),
+                    "cg_iterations": s.num_linear_solver_iterations, "solve_wall_s": wall,
+                    "time_to_tol_s": time_to(s, target)}
+
+        log("bench: LM with SCHUR_JACOBI")
+        sj = {}
+        for label, prec, cap in (("fp64", mba.PRECISION_FP64, args.lm_iters),
+                                 ("mixed", mba.PRECISION_MIXED, (3 * args.lm_iters) // 2)):
+            opts = mba.default_options(precision=prec, huber_a=HUBER_A, linear_solver=mba.IMPLICIT_SCHUR_PCG,
+                                       max_num_iterations=cap, eta=LM_ETA, function_tolerance=LM_FTOL,
+                                       preconditioner=mba.PRECOND_SCHUR_JACOBI)
+            t0 = time.perf_counter()
+            s = mba.solve(problem(), opts)
+            sj[label] = lm_record(s, time.perf_counter() - t0, ref_cost, rms_target)
+            log(f"  {label}: final cost {s.final_cost:.10e} ({sj[label]['final_cost_rel_to_jacobi_fp64']:+.2e}), "
+                f"{s.num_successful_steps}+{s.num_unsuccessful_steps} steps, {s.num_linear_solver_iterations} CG, "
+                f"{sj[label]['solve_wall_s']:.3f} s")
+        lm["schur_jacobi"] = sj
+        d2 = mba.generate_scene(2)
+        c2 = {}
+        for pre_name, pre in (("jacobi", mba.PRECOND_JACOBI), ("schur_jacobi", mba.PRECOND_SCHUR_JACOBI)):
+            for label, prec, cap in (("fp64", mba.PRECISION_FP64, 300), ("mixed", mba.PRECISION_MIXED, 450)):
+                opts = mba.default_options(precision=prec, linear_solver=mba.IMPLICIT_SCHUR_PCG, preconditioner=pre,
+                                           max_num_iterations=cap, eta=LM_ETA, function_tolerance=LM_FTOL)
+                best = None
+                for _ in range(2):  # the first call pays the device set-up
+                    p2 = mba.BAProblem(d2["cam_init"].copy(), d2["pt_init"].copy(), d2["obs"], d2["cam_idx"],
+                                       d2["pt_idx"])
+                    t0 = time.perf_counter()
+                    s = mba.solve(p2, opts)
+                    best = (s, time.perf_counter() - t0)
+                c2[f"{pre_name}_{label}"] = best
+        ref2 = c2["jacobi_fp64"][0].final_cost
+        lm["c2_to_convergence"] = {
+            "scene": "200 cameras / 100,000 points / 696,955 observations, no loss",
+            "target": f"cost within {RMS_TOL} relative of the JACOBI FP64 final cost",
+            "runs": {k: lm_record(s, wall, ref2, ref2 * (1.0 + RMS_TOL)) for k, (s, wall) in c2.items()}}
+        for k, r in lm["c2_to_convergence"]["runs"].items():
+            log(f"  C2 {k}: {r['lm_iterations']} iterations, {r['cg_iterations']} CG, termination "
+                f"{r['termination']}, {r['solve_wall_s']:.3f} s, to tolerance {r['time_to_tol_s']}")
 
     explicit = None
     if not args.no_explicit:
@@ -497,6 +547,16 @@ def pmc_target(kind):
     torch.cuda.synchronize()
     d = mba.generate_scene(3)
     p = mba.BAProblem(d["cam_init"], d["pt_init"], d["obs"], d["cam_idx"], d["pt_idx"])
+    if kind == "lm_mixed":
+        # Two whole mixed LM solves with the LM leg's options, 0.3 s apart: the
+        # second (no device set-up) is the one to read in a kernel trace.
+        for _ in range(2):
+            q = mba.BAProblem(d["cam_init"].copy(), d["pt_init"].copy(), d["obs"], d["cam_idx"], d["pt_idx"])
+            mba.solve(q, mba.default_options(precision=mba.PRECISION_MIXED, huber_a=HUBER_A,
+                                             linear_solver=mba.IMPLICIT_SCHUR_PCG, max_num_iterations=450,
+                                             eta=LM_ETA, function_tolerance=LM_FTOL))
+            time.sleep(0.3)
+        return
     if kind == "explicit":
         ctx = mba.Context(p, mba.default_options(precision=mba.PRECISION_MIXED, huber_a=HUBER_A,
                                                  linear_solver=mba.EXPLICIT_SCHUR, max_num_iterations=2))
@@ -529,11 +589,12 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--explicit-iters", type=int, default=5)
-    ap.add_argument("--pmc-target", nargs="?", const="pass", choices=["pass", "pass_fp64", "explicit"],
-                    help="target run for rocprofv3 --pmc: a 256 MiB device copy (known bytes, the "
-                         "counter calibration) followed by five mixed-precision Jacobian+Schur passes "
-                         "('pass'), five FP64 ones ('pass_fp64') or a two-iteration explicit-Schur "
-                         "solve ('explicit'); prints nothing else")
+    ap.add_argument("--pmc-target", nargs="?", const="pass", choices=["pass", "pass_fp64", "explicit", "lm_mixed"],
+                    help="target run for rocprofv3 (--pmc or --kernel-trace): a 256 MiB device copy (known "
+                         "bytes, the counter calibration) followed by five mixed-precision Jacobian+Schur "
+                         "passes ('pass'), five FP64 ones ('pass_fp64'), a two-iteration explicit-Schur "
+                         "solve ('explicit') or two whole mixed LM solves with the LM leg's options "
+                         "('lm_mixed'); prints nothing else")
     args = ap.parse_args()
     if args.pmc_target:
         pmc_target(args.pmc_target)
