@@ -2159,7 +2159,8 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   }
   // FP32 point-major kernels with the camera records staged in LDS (the
   // pass: projection quads; the matvec: projection and direction quads),
-  // one or two workgroups per CU, as many as the table leaves room for.
+  // one or two workgroups per CU, as many as the table and the kernel's
+  // registers leave room for.
   if (c->mixed && !c->grouped && c->nc > 0 && c->seg_blocks > 0) {
     int ncu = 0, lds_max = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
@@ -2168,7 +2169,11 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
       const size_t bytes = sizeof(float) * rec_floats * (size_t)c->nc;
       if (ncu <= 0 || bytes > (size_t)lds_max) return MBA_OK;
       HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-      const int per_cu = (int)std::min<size_t>(2, (size_t)lds_max / bytes);
+      // Workgroups resident per CU: the table's LDS and the kernel's
+      // registers both limit it.
+      int resident = 0;
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kernel, kStageThreads, bytes));
+      const int per_cu = (int)std::max<size_t>(1, std::min<size_t>({2, (size_t)lds_max / bytes, (size_t)resident}));
       l->bytes = bytes;
       l->grid = (int)std::min<int64_t>(((int64_t)c->seg_blocks + 3) / 4, (int64_t)ncu * per_cu);
       return MBA_OK;

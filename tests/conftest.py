@@ -61,3 +61,28 @@ def grouped_problem(mba, d, const=None, intr_const=None, subset=None):
     return mba.BAProblem(d["cam_init"].copy(), d["pt_init"].copy(), d["obs"], d["cam_idx"], d["pt_idx"], const,
                          intrinsics=d["intr_init"].copy(), intrinsic_index=d["intr_idx"],
                          intrinsic_constant=intr_const, intrinsic_subset=subset)
+
+
+def axis_point_scene():
+    """Three cameras with rotation exactly zero (R = I) at x = -1, 0, 1, 40
+    points in front of them seen by all three, and one more point seen
+    only by camera 1, on its optical axis (camera-frame x = y = 0)."""
+    rng = np.random.default_rng(5)
+    cams = np.zeros((3, 9))
+    cams[:, 3] = [-1.0, 0.0, 1.0]
+    cams[:, 6] = 500.0
+    pts = np.column_stack([rng.uniform(-2, 2, 40), rng.uniform(-2, 2, 40), rng.uniform(5, 10, 40)])
+    pts = np.vstack([pts, [[0.0, 0.0, 7.0]]])
+    cam_idx, pt_idx, obs = [], [], []
+    for q in range(40):
+        for c in range(3):
+            xc = pts[q] + cams[c, 3:6]
+            cam_idx.append(c)
+            pt_idx.append(q)
+            obs.append(500.0 * xc[:2] / xc[2] + rng.normal(0.0, 0.5, 2))
+    cam_idx.append(1)
+    pt_idx.append(40)
+    obs.append([0.4, -0.3])
+    pts_init = pts + rng.normal(0.0, 0.01, pts.shape)
+    pts_init[40] = pts[40]  # stays on the axis
+    return cams, pts_init, np.array(obs), np.array(cam_idx, np.int32), np.array(pt_idx, np.int32)

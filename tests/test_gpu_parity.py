@@ -11,7 +11,7 @@ Tolerances (stated per test):
 import numpy as np
 import pytest
 
-from conftest import ba_problem
+from conftest import axis_point_scene, ba_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -344,6 +344,42 @@ def test_nonfinite_input_is_an_error(mba, c1):
     with pytest.raises(mba.MbaError) as e:
         mba.solve(p)
     assert e.value.status == mba.ERR_NOT_FINITE
+
+
+@pytest.mark.parametrize("solver", ["explicit", "pcg"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_singular_point_block_makes_the_step_invalid(mba, O, solver, prec):
+    """A point seen by one camera on that camera's optical axis, with the
+    camera's rotation zero, has V = Jp'Jp with a zero row and column along
+    the world z axis; with min_lm_diagonal = 0 its damped block V + D^2 is
+    singular.  The pass flags the failed 3x3 inverse and every step built on
+    it is invalid, as Ceres treats a failed linear solve
+    (trust_region_minimizer.cc:323-366): five in a row end the solve with
+    FAILURE (max_num_consecutive_invalid_steps) and the parameters are not
+    moved; the oracle's LM (oracle/mba_oracle.c, a non-finite or failed
+    solve is an invalid step) gives the same records.  With the default
+    diagonal floor (1e-6) the same problem solves."""
+    ls = mba.EXPLICIT_SCHUR if solver == "explicit" else mba.IMPLICIT_SCHUR_PCG
+    pr = mba.PRECISION_FP64 if prec == "fp64" else mba.PRECISION_MIXED
+    cams, pts, obs, ci, pi = axis_point_scene()
+    p = mba.BAProblem(cams.copy(), pts.copy(), obs, ci, pi)
+    s = mba.solve(p, mba.default_options(linear_solver=ls, precision=pr, min_lm_diagonal=0.0))
+    print(f"{solver} {prec}: termination {s.termination_type}, records "
+          f"{[(it.iteration, it.step_is_valid) for it in s.iterations]}", flush=True)
+    d = dict(cam_init=cams, pt_init=pts, obs=obs, cam_idx=ci, pt_idx=pi)
+    os_, oits = O.solve(O.Scene(d), O.default_options(linear_solver=0 if solver == "explicit" else 1,
+                                                      min_lm_diagonal=0.0))
+    assert (s.termination_type, len(s.iterations)) == (os_.termination_type, len(oits))
+    assert [it.step_is_valid for it in s.iterations] == [it.step_is_valid for it in oits]
+    assert s.termination_type == mba.FAILURE and not s.is_solution_usable
+    assert [it.step_is_valid for it in s.iterations[1:]] == [0] * 4  # the fifth ends the solve
+    assert all(it.cost == s.iterations[0].cost for it in s.iterations)
+    np.testing.assert_array_equal(p.cameras, cams)
+    np.testing.assert_array_equal(p.points, pts)
+    q = mba.BAProblem(cams.copy(), pts.copy(), obs, ci, pi)
+    s = mba.solve(q, mba.default_options(linear_solver=ls, precision=pr))
+    assert s.is_solution_usable and s.iterations[1].step_is_valid
+    assert s.final_cost < s.initial_cost
 
 
 def test_ceres_shaped_api_like_the_reference_unit_test(mba, O):

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 from scipy.spatial.transform import Rotation
 
-from conftest import ROOT
+from conftest import ROOT, axis_point_scene
 
 C1_ITERS = 20
 
@@ -307,3 +307,26 @@ def test_grouped_iterative_with_constant_camera_matches_reference(O, golden, gro
     assert [it.linear_solver_iterations for it in its[:5]] == [int(v) for v in ref[:5, 6]]
     np.testing.assert_allclose([it.cost for it in its], ref[:, 1], rtol=1e-6)
     np.testing.assert_allclose(s.final_cost, float(golden["grp_iterative_const_final_cost"]), rtol=1e-7)
+
+
+@pytest.mark.parametrize("linear_solver", [0, 1])
+def test_singular_point_block_is_an_invalid_step(O, linear_solver):
+    """The axis point's Jacobian has a zero third column (x = y = 0 in the
+    camera frame, R = I), so its V has a zero row and column and, with
+    min_lm_diagonal = 0, V + D^2 is singular: the Schur elimination cannot
+    invert it, the step is invalid, and five invalid steps in a row end the
+    LM with FAILURE (trust_region_minimizer.cc:323-366).  With the default
+    floor (1e-6) the LM runs normally (its steps are valid and the cost
+    falls; the axis point's depth is barely damped, so it does not converge
+    within the 50 iterations)."""
+    cams, pts, obs, ci, pi = axis_point_scene()
+    d = dict(cam_init=cams, pt_init=pts, obs=obs, cam_idx=ci, pt_idx=pi)
+    _, _, jp, _ = O.evaluate(O.Scene(d))
+    assert np.all(jp[-1][:, 2] == 0.0) and jp[-1][0, 0] != 0.0 and jp[-1][1, 1] != 0.0
+    s, its = O.solve(O.Scene(d), O.default_options(linear_solver=linear_solver, min_lm_diagonal=0.0))
+    # The fifth invalid step ends the solve before its record is kept.
+    assert s.termination_type == 2
+    assert [it.step_is_valid for it in its[1:]] == [0] * 4
+    s, its = O.solve(O.Scene(d), O.default_options(linear_solver=linear_solver))
+    assert s.termination_type != 2 and its[1].step_is_valid
+    assert s.final_cost < s.initial_cost
