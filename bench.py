@@ -29,6 +29,10 @@ Also reported:
     oracle's on the host cores (OpenMP), on the same scene.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Each timed pass loop runs exactly K passes after at least W untimed passes and
+at least 0.5 s of them (WARM_S), so that it measures the GPU at its running
+clocks rather than its ramp from idle.
 """
 from __future__ import annotations
 
@@ -48,6 +52,22 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 HUBER_A = 4.0
 RADIUS = 1e4
+# Warm-up before a timed pass loop: the requested warm-up steps and at least
+# this much wall time of passes (the GPU's clocks ramp up over the first tens
+# of milliseconds of back-to-back passes: ~6 % slower passes at the start).
+WARM_S = 0.5
+
+
+def warm_up(ctx, steps, torch):
+    """Untimed passes: at least `steps`, and for at least WARM_S seconds."""
+    t_end = time.perf_counter() + WARM_S
+    k = 0
+    while k < steps or time.perf_counter() < t_end:
+        ctx.jacobian_schur(RADIUS)
+        k += 1
+        if k % 16 == 0:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
 # LM leg: CG forcing and function tolerance tightened from the Ceres defaults
 # (0.1, 1e-6) so that both precisions run on toward the minimum instead of
 # stopping on slow progress (5% outliers under Huber converge slowly).
@@ -142,9 +162,7 @@ def run(args):
         ctx = mba.Context(problem(), mba.default_options(precision=precision, huber_a=HUBER_A,
                                                          linear_solver=mba.IMPLICIT_SCHUR_PCG))
         stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
-        for _ in range(args.warmup):
-            ctx.jacobian_schur(RADIUS)
-        torch.cuda.synchronize()
+        warm_up(ctx, args.warmup, torch)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
         e0.record(stream)
@@ -378,9 +396,7 @@ def run(args):
         ctx = mba.Context(lproblem(), mba.default_options(precision=mba.PRECISION_MIXED,
                                                           linear_solver=mba.IMPLICIT_SCHUR_PCG))
         stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
-        for _ in range(args.warmup):
-            ctx.jacobian_schur(RADIUS)
-        torch.cuda.synchronize()
+        warm_up(ctx, args.warmup, torch)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(args.steps):
@@ -421,9 +437,7 @@ def run(args):
             ctx = mba.Context(gproblem(0), mba.default_options(precision=prec, huber_a=HUBER_A,
                                                                linear_solver=mba.IMPLICIT_SCHUR_PCG))
             stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
-            for _ in range(args.warmup):
-                ctx.jacobian_schur(RADIUS)
-            torch.cuda.synchronize()
+            warm_up(ctx, args.warmup, torch)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
             for _ in range(args.steps):
