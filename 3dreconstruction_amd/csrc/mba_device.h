@@ -3219,7 +3219,8 @@ __global__ __launch_bounds__(256) void k_syrk_mfma(T* __restrict__ A, int64_t n,
 // there is no flag, fence or barrier between the blocks.  A workgroup waits
 // only on blocks that lower-numbered workgroups own, and workgroups are
 // dispatched in order, so every block it waits on is owned by a workgroup
-// already resident or finished.  Each wait is bounded (kFlowSpinMax polls):
+// already resident or finished.  Each wait is bounded (spin_max polls,
+// kFlowSpinMax by default):
 // a wait that runs out stores the epoch into err, the other waits then give
 // up too, and the blocks written after that are NaN, so a failed solve
 // reaches the LM as a non-finite step instead of a hang.  Per-thread
@@ -3240,14 +3241,17 @@ __device__ __forceinline__ void flow_put(uint64_t* dst, T v, unsigned epoch) {
 }
 
 // Polls the N values at src (flow_words<T>() words each) until every word
-// carries epoch; false when the wait ran out or another one had (err ==
-// epoch).
+// carries epoch; false when the wait ran out (spin_max polls; 0: it gives up
+// without polling) or another one had (err == epoch).
 template <typename T, int N>
-__device__ __forceinline__ bool flow_get(const uint64_t* src, unsigned epoch, uint64_t* err, T out[N]) {
+__device__ __forceinline__ bool flow_get(const uint64_t* src, unsigned epoch, uint64_t* err, unsigned spin_max,
+                                         T out[N]) {
   constexpr int C = flow_words<T>();
   uint64_t w[N * C];
-  bool good = true;
-  for (unsigned spins = 0;; ++spins) {
+#pragma unroll
+  for (int q = 0; q < N * C; ++q) w[q] = 0;
+  bool good = spin_max != 0;
+  for (unsigned spins = 0; good; ++spins) {
     bool all = true;
 #pragma unroll
     for (int q = 0; q < N * C; ++q) {
@@ -3255,14 +3259,13 @@ __device__ __forceinline__ bool flow_get(const uint64_t* src, unsigned epoch, ui
       all = all && (unsigned)(w[q] >> 32) == epoch;
     }
     if (all) break;
-    if (spins >= kFlowSpinMax ||
-        ((spins & 63u) == 63u && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)) {
-      __hip_atomic_store(err, (uint64_t)epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (spins >= spin_max ||
+        ((spins & 63u) == 63u && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch))
       good = false;
-      break;
-    }
-    __builtin_amdgcn_s_sleep(1);
+    else
+      __builtin_amdgcn_s_sleep(1);
   }
+  if (!good) __hip_atomic_store(err, (uint64_t)epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     unsigned u[C];
@@ -3316,7 +3319,7 @@ template <typename T>
 __global__ __launch_bounds__(kFlowThreads) void k_trsv_lower_flow(const T* __restrict__ L, const T* __restrict__ Linv,
                                                                   int64_t n, const T* __restrict__ b,
                                                                   T* __restrict__ y, uint64_t* hand, unsigned epoch,
-                                                                  uint64_t* err) {
+                                                                  uint64_t* err, unsigned spin_max) {
   constexpr int C = flow_words<T>();
   __shared__ T part[4][kNB];
   __shared__ T sv[kNB];
@@ -3337,7 +3340,7 @@ __global__ __launch_bounds__(kFlowThreads) void k_trsv_lower_flow(const T* __res
     T l[16], yk[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) l[j] = col[j * n];
-    good = flow_get<T, 16>(hand + C * c0, epoch, err, yk);
+    good = flow_get<T, 16>(hand + C * c0, epoch, err, spin_max, yk);
 #pragma unroll
     for (int j = 0; j < 16; ++j) acc = fma(l[j], yk[j], acc);
   }
@@ -3358,7 +3361,7 @@ template <typename T>
 __global__ __launch_bounds__(kFlowThreads) void k_trsv_upper_flow(const T* __restrict__ L, const T* __restrict__ LinvT,
                                                                   int64_t n, int nb, const T* __restrict__ y,
                                                                   T* __restrict__ x, uint64_t* hand, unsigned epoch,
-                                                                  uint64_t* err) {
+                                                                  uint64_t* err, unsigned spin_max) {
   constexpr int C = flow_words<T>();
   __shared__ T part[4][kNB];
   __shared__ T sv[kNB];
@@ -3382,7 +3385,7 @@ __global__ __launch_bounds__(kFlowThreads) void k_trsv_upper_flow(const T* __res
 #pragma unroll
     for (int r = 0; r < 16; ++r) l[r] = on ? p[r * n] : T(0);
     T xk[1] = {T(0)};
-    if (on) good = flow_get<T, 1>(hand + C * (k0 + lane), epoch, err, xk);
+    if (on) good = flow_get<T, 1>(hand + C * (k0 + lane), epoch, err, spin_max, xk);
     good = __all(good);
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = fma(l[r], xk[0], acc[r]);

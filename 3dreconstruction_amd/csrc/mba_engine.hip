@@ -1277,6 +1277,10 @@ struct mba_context {
   // FP32 point-major pass and matvec with LDS-staged camera records (grid
   // > 0): grids and dynamic LDS bytes.
   LdsLaunch lds_pass, lds_mv;
+  // Polls per cross-block wait of the triangular solves; MBA_FLOW_SPIN_LIMIT
+  // in the environment at context creation overrides it (0: every wait
+  // gives up at once, which tests the failure path without a hang).
+  unsigned flow_spin = kFlowSpinMax;
   hipEvent_t tev[3] = {};          // LM phase timing
   // Dense Cholesky look-ahead: the trailing tiles past the next block column
   // run on stream2 beside the next diagonal factor; chol_ev[0] orders them
@@ -2157,6 +2161,7 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
     MBA_TRY(upload(c, &c->g_hi, ghi));
     MBA_TRY(upload(c, &c->g_out, gout));
   }
+  if (const char* e = std::getenv("MBA_FLOW_SPIN_LIMIT")) c->flow_spin = (unsigned)std::strtoul(e, nullptr, 10);
   // FP32 point-major kernels with the camera records staged in LDS (the
   // pass: projection quads; the matvec: projection and direction quads),
   // one or two workgroups per CU, as many as the table and the kernel's
@@ -2904,9 +2909,9 @@ struct Engine {
     uint64_t* err = d.flow + (size_t)flow_words<T>() * n;
     const unsigned e1 = ++d.flow_epoch, e2 = ++d.flow_epoch;
     hipLaunchKernelGGL(k_trsv_lower_flow<T>, dim3(nb), dim3(kFlowThreads), 0, s, (const T*)d.L, (const T*)d.Linv, n,
-                       (const T*)v, d.ty, d.flow, e1, err);
+                       (const T*)v, d.ty, d.flow, e1, err, c->flow_spin);
     hipLaunchKernelGGL(k_trsv_upper_flow<T>, dim3(nb), dim3(kFlowThreads), 0, s, (const T*)d.L, (const T*)d.LinvT, n,
-                       nb, (const T*)d.ty, v, d.flow, e2, err);
+                       nb, (const T*)d.ty, v, d.flow, e2, err, c->flow_spin);
   }
 
   // Explicit Schur (single problem): assemble dense S, Cholesky, solve,

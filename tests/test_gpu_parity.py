@@ -382,6 +382,33 @@ def test_singular_point_block_makes_the_step_invalid(mba, O, solver, prec):
     assert s.final_cost < s.initial_cost
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_failed_triangular_solve_is_an_invalid_step(mba, c1, prec, monkeypatch):
+    """The explicit solver's triangular solves hand 64-row blocks between
+    workgroups through bounded waits.  With the wait limit set to zero from
+    the host (MBA_FLOW_SPIN_LIMIT=0 when the context is created) every wait
+    gives up at once and the solve comes out NaN (16 cameras: blocks 1 and 2
+    of 3 wait); the LM treats that as a failed linear solve: invalid steps,
+    FAILURE after five, parameters untouched.  A context created with the
+    default limit then solves the problem exactly as before."""
+    pr = mba.PRECISION_FP64 if prec == "fp64" else mba.PRECISION_MIXED
+    opts = mba.default_options(linear_solver=mba.EXPLICIT_SCHUR, precision=pr)
+    ref = mba.solve(ba_problem(mba, c1), opts)
+    monkeypatch.setenv("MBA_FLOW_SPIN_LIMIT", "0")
+    p = ba_problem(mba, c1)
+    s = mba.solve(p, opts)
+    print(f"{prec}: termination {s.termination_type}, records {[(it.iteration, it.step_is_valid) for it in s.iterations]}",
+          flush=True)
+    assert s.termination_type == mba.FAILURE and not s.is_solution_usable
+    assert [it.step_is_valid for it in s.iterations[1:]] == [0] * 4  # the fifth ends the solve
+    np.testing.assert_array_equal(p.cameras, c1["cam_init"])
+    np.testing.assert_array_equal(p.points, c1["pt_init"])
+    monkeypatch.delenv("MBA_FLOW_SPIN_LIMIT")
+    again = mba.solve(ba_problem(mba, c1), opts)
+    assert again.final_cost == ref.final_cost and len(again.iterations) == len(ref.iterations)
+    assert ref.termination_type == mba.CONVERGENCE
+
+
 def test_ceres_shaped_api_like_the_reference_unit_test(mba, O):
     """bundle_adjustment_unittest.cpp:19-122 shape: per-observation residual
     blocks, SPARSE_SCHUR, the solution is usable and the RMSE decreases; plus
