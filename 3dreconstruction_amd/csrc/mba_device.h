@@ -2905,12 +2905,15 @@ __global__ __launch_bounds__(kWinThreads) void k_win_chol_solve(const int64_t* _
 }
 
 // ------------------------------------------------- dense Cholesky (lower)
-// Right-looking blocked LLT of a column-major SPD matrix, NB = 64.  Step k:
-//   k_potrf_diag   factors the 64 x 64 diagonal block and inverts it;
-//   k_panel_col0   L21 = A21 L11^-T as a GEMM with the inverted block, and
-//                  A22 -= L21 L21' on the first trailing block column;
-//   k_syrk_mfma    A22 -= L21 L21' on the other trailing tiles.
-// The triangular solves with the factor run one launch per 64-row block.
+// Right-looking blocked LLT of a column-major SPD matrix, NB = 64, in outer
+// steps of two blocks:
+//   k_potrf_diag   factors a 64 x 64 diagonal block and inverts it;
+//   k_panel_col0   L21 = A21 L11^-T as a GEMM with the inverted block (for
+//                  the outer step's first block also A22 -= L21 L21' on the
+//                  second block's columns);
+//   k_syrk2_mfma   A22 -= L21a L21a' + L21b L21b' with both blocks' panels
+//                  on the trailing tiles (K = 128).
+// The triangular solves with the factor run one launch per direction.
 constexpr int kNB = 64;
 
 
@@ -2930,14 +2933,26 @@ constexpr int kNB = 64;
 // X[j][c] broadcast from lane j times 1 / L[j][j]; step j touches only the
 // columns c <= j + 3 that can be nonzero there.
 template <typename T>
+// slab (optional): a 64 x 64 column-major copy of the block's first
+// sub-diagonal slab (rows k + kb .. k + kb + 63 of its columns, zero past
+// row n), which k_panel_col0's workgroups read as their operand while its
+// first workgroup overwrites the slab in A with L21.
 __global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ A, int64_t n, int k, int kb,
                                                     T* __restrict__ Linv, T* __restrict__ LinvT,
-                                                    int32_t* __restrict__ info) {
+                                                    int32_t* __restrict__ info, T* __restrict__ slab) {
   __shared__ T colp[2][4][16];
   __shared__ T Ls[kNB][kNB + 1];
   __shared__ T rinv[kNB];
   __shared__ int bad_w[4];
   const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (slab) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = w + 4 * q;
+      const int64_t row = (int64_t)k + kb + r;
+      slab[r + kNB * c] = (row < n && c < kb) ? A[row + (int64_t)(k + c) * n] : T(0);
+    }
+  }
   T a[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
@@ -3032,10 +3047,11 @@ struct TileAcc<double> {
 };
 
 __device__ __forceinline__ void tile_mma(const float (*As)[kNB + 1], const float (*Bs)[kNB + 1], int kb,
-                                         TileAcc<float>& acc) {
+                                         TileAcc<float>& acc, bool accumulate = false) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int qi = w >> 1, qj = w & 1;
-  for (int r = 0; r < 16; ++r) acc.v[r] = 0.0f;
+  if (!accumulate)
+    for (int r = 0; r < 16; ++r) acc.v[r] = 0.0f;
   const int kpad = (kb + 1) & ~1;
   for (int kk = 0; kk < kpad; kk += 2) {
     const float a = As[qi * 32 + (lane & 31)][kk + (lane >> 5)];
@@ -3052,12 +3068,13 @@ __device__ __forceinline__ void tile_put(const TileAcc<float>& acc, F&& put) {
     put(qi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), qj * 32 + (lane & 31), acc.v[r]);
 }
 __device__ __forceinline__ void tile_mma(const double (*As)[kNB + 1], const double (*Bs)[kNB + 1], int kb,
-                                         TileAcc<double>& acc) {
+                                         TileAcc<double>& acc, bool accumulate = false) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int qi = w >> 1, qj = w & 1;
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int r = 0; r < 4; ++r) acc.v[a][b][r] = 0.0;
+  if (!accumulate)
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int r = 0; r < 4; ++r) acc.v[a][b][r] = 0.0;
   const int kpad = (kb + 3) & ~3;
   for (int kk = 0; kk < kpad; kk += 4) {
     double av[2], bv[2];
@@ -3149,16 +3166,19 @@ __device__ __forceinline__ void load_cols(T (*Ts)[kNB + 1], const T* __restrict_
   }
 }
 
-// Look-ahead form of the panel step: workgroup ti writes its 64-row slab of
-// L21 = A21 L11^-T and then applies the trailing update to tile (ti, 0), the
-// first block column after the panel (the next step's diagonal and panel
+// Panel step: workgroup ti writes its 64-row slab of L21 = A21 L11^-T and,
+// with col0, then applies the trailing update to tile (ti, 0), the first
+// block column after the panel (the next block's diagonal and panel
 // columns).  Every workgroup forms the first slab's L21 rows itself (same
 // operands, same MFMA sequence: the values the panel writes), so no
-// workgroup waits for another.  The remaining tiles (k_syrk_mfma with
-// tile offset 1) can then run beside the next diagonal factor.
+// workgroup waits for another; they read that slab's A21 rows from the
+// copy k_potrf_diag made (slab), since workgroup 0 overwrites them in A and
+// a workgroup may start after it has.  col0 = 0: the panel alone (its
+// trailing update is left to k_syrk2_mfma).
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_col0(T* __restrict__ A, int64_t n, int k, int kb,
-                                                    const T* __restrict__ Linv) {
+                                                    const T* __restrict__ Linv, int col0,
+                                                    const T* __restrict__ slab) {
   __shared__ T As[kNB][kNB + 1];
   __shared__ T Bs[kNB][kNB + 1];
   __shared__ T Cs[kNB][kNB + 1];
@@ -3166,14 +3186,15 @@ __global__ __launch_bounds__(256) void k_panel_col0(T* __restrict__ A, int64_t n
   const int64_t r0 = base + (int64_t)blockIdx.x * kNB;
   load_cols(As, A, n, k, kb, r0);
   load_cols(Bs, Linv, kNB, 0, kNB, 0);
-  load_cols(Cs, A, n, k, kb, base);
+  if (col0) load_cols(Cs, slab, kNB, 0, kb, 0);
   __syncthreads();
   TileAcc<T> xi, x0;
   tile_mma(Bs, As, kb, xi);  // element (i, j) = L21[r0 + j][k + i]
-  tile_mma(Bs, Cs, kb, x0);  // element (i, j) = L21[base + j][k + i]
+  if (col0) tile_mma(Bs, Cs, kb, x0);  // element (i, j) = L21[base + j][k + i]
   tile_put(xi, [&](int i, int j, T v) {
     if (r0 + j < n && i < kb) A[(r0 + j) + (int64_t)(k + i) * n] = v;
   });
+  if (!col0) return;
   __syncthreads();
   tile_put(xi, [&](int i, int j, T v) { As[j][i] = v; });
   tile_put(x0, [&](int i, int j, T v) { Cs[j][i] = v; });
@@ -3190,24 +3211,39 @@ __device__ __forceinline__ void syrk_tile_coords(int t, int& ti, int& tj) {
   tj = t - ti * (ti + 1) / 2;
 }
 
-// Trailing update A22 -= L21 L21' on 64 x 64 lower tiles (MFMA), over the
-// tiles (ti, tj) with ti >= tj >= off (tile units after the panel).
+// Trailing update after two 64-column steps k and k + 64 (both full):
+// A22 -= L21a L21a' + L21b L21b' for the tiles after base = k + 128, the two
+// products summed in the MFMA accumulators and the tile read and written
+// once (one read-modify-write of the trailing matrix per 128 columns).
+// cols = 1: tiles (ti, tj) of the first two block columns (tj < 2, ti >= tj;
+// nt tile rows after base), t < nt -> (t, 0), else (t - nt + 1, 1);
+// cols = 0: the lower tiles with ti >= tj >= 2.
 template <typename T>
-__global__ __launch_bounds__(256) void k_syrk_mfma(T* __restrict__ A, int64_t n, int k, int kb, int off) {
+__global__ __launch_bounds__(256) void k_syrk2_mfma(T* __restrict__ A, int64_t n, int k, int nt, int cols) {
   __shared__ T As[kNB][kNB + 1];
   __shared__ T Bs[kNB][kNB + 1];
   int ti, tj;
-  syrk_tile_coords(blockIdx.x, ti, tj);
-  ti += off;
-  tj += off;
-  const int64_t base = (int64_t)k + kb;
+  const int t = (int)blockIdx.x;
+  if (cols) {
+    tj = t < nt ? 0 : 1;
+    ti = t < nt ? t : t - nt + 1;
+  } else {
+    syrk_tile_coords(t, ti, tj);
+    ti += 2;
+    tj += 2;
+  }
+  const int64_t base = (int64_t)k + 2 * kNB;
   const int64_t r0 = base + (int64_t)ti * kNB, c0 = base + (int64_t)tj * kNB;
-  load_cols(As, A, n, k, kb, r0);
-  load_cols(Bs, A, n, k, kb, c0);
-  __syncthreads();
-  // Transposed product (lane index on the row): coalesced read-modify-write.
   TileAcc<T> up;
-  tile_mma(Bs, As, kb, up);
+  load_cols(As, A, n, k, kNB, r0);
+  load_cols(Bs, A, n, k, kNB, c0);
+  __syncthreads();
+  tile_mma(Bs, As, kNB, up);
+  __syncthreads();
+  load_cols(As, A, n, k + kNB, kNB, r0);
+  load_cols(Bs, A, n, k + kNB, kNB, c0);
+  __syncthreads();
+  tile_mma(Bs, As, kNB, up, true);
   tile_sub_lower(up, A, n, r0, c0);
 }
 

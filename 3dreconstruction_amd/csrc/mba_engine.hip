@@ -1104,7 +1104,7 @@ struct Dev {
   T *x = nullptr, *r = nullptr, *zc = nullptr, *p = nullptr, *q = nullptr, *tmp = nullptr;
   T *step_p = nullptr, *step_c = nullptr;
   // explicit S
-  T *S = nullptr, *L = nullptr, *sv = nullptr, *WY = nullptr, *Linv = nullptr, *cross = nullptr;
+  T *S = nullptr, *L = nullptr, *sv = nullptr, *WY = nullptr, *Linv = nullptr, *cross = nullptr, *slab = nullptr;
   // Inverses of the factor's 64 x 64 diagonal blocks and their transposes
   // (one per block), and the triangular solves' two work vectors.
   T *LinvT = nullptr, *ty = nullptr;
@@ -2967,6 +2967,7 @@ struct Engine {
       MBA_TRY(c->pool.alloc(&d.s_scratch, 81 * (size_t)std::max<int64_t>(1, c->s_nslots)));
       const size_t nbk = (size_t)((n + kNB - 1) / kNB);
       MBA_TRY(c->pool.alloc(&d.Linv, nbk * kNB * kNB));
+      MBA_TRY(c->pool.alloc(&d.slab, (size_t)kNB * kNB));
       MBA_TRY(c->pool.alloc(&d.LinvT, nbk * kNB * kNB));
       MBA_TRY(c->pool.alloc(&d.ty, (size_t)n));
       MBA_TRY(c->pool.alloc(&d.flow, (size_t)flow_words<T>() * n + 1));
@@ -3001,31 +3002,45 @@ struct Engine {
     if (c->timing) (void)hipEventRecord(c->ev[5], s);
     HIP_TRY(hipMemcpyAsync(d.L, d.S, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(c->info, 0, sizeof(int32_t), s));
-    // Step k: diagonal factor, then panel + first trailing block column (the
-    // next step's columns) on s; the other trailing tiles on stream2, where
-    // they overlap the next diagonal factor.  Each matrix element still
-    // receives its updates in step order: the next panel waits for them.
+    // Outer steps of two 64-column blocks c0 and k2 = c0 + 64 on s: diagonal
+    // factor of c0, its panel with the update of block column k2, diagonal
+    // factor of k2, its panel alone; then one trailing update with both
+    // panels (K = 128): the next outer step's two block columns on s, the
+    // other tiles on stream2, where they overlap the next outer step's
+    // factors and panels.  Every element still receives all its updates
+    // before it is factored or used as a panel.
     int ring = 0, pending = -1;
-    for (int64_t k = 0; k < n; k += kNB) {
-      const int kb = (int)std::min<int64_t>(kNB, n - k);
-      T* li = d.Linv + (size_t)(k / kNB) * kNB * kNB;
-      hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, s, d.L, n, (int)k, kb, li,
-                         d.LinvT + (size_t)(k / kNB) * kNB * kNB, c->info);
-      const int64_t rest = n - k - kb;
-      if (rest > 0) {
-        const int nt = (int)((rest + kNB - 1) / kNB);
-        if (pending >= 0) HIP_TRY(hipStreamWaitEvent(s, c->chol_ev[1][pending], 0));
-        pending = -1;
-        hipLaunchKernelGGL(k_panel_col0<T>, dim3(nt), dim3(256), 0, s, d.L, n, (int)k, kb, (const T*)li);
-        if (nt > 1) {
-          HIP_TRY(hipEventRecord(c->chol_ev[0][ring], s));
-          HIP_TRY(hipStreamWaitEvent(c->stream2, c->chol_ev[0][ring], 0));
-          hipLaunchKernelGGL(k_syrk_mfma<T>, dim3((unsigned)((int64_t)(nt - 1) * nt / 2)), dim3(256), 0, c->stream2,
-                             d.L, n, (int)k, kb, 1);
-          HIP_TRY(hipEventRecord(c->chol_ev[1][ring], c->stream2));
-          pending = ring;
-          ring = (ring + 1) & 3;
-        }
+    for (int64_t c0 = 0; c0 < n; c0 += 2 * kNB) {
+      const int kb1 = (int)std::min<int64_t>(kNB, n - c0);
+      hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, s, d.L, n, (int)c0, kb1,
+                         d.Linv + (size_t)(c0 / kNB) * kNB * kNB, d.LinvT + (size_t)(c0 / kNB) * kNB * kNB, c->info,
+                         d.slab);
+      const int64_t k2 = c0 + kb1;
+      if (k2 >= n) break;
+      const int kb2 = (int)std::min<int64_t>(kNB, n - k2);
+      hipLaunchKernelGGL(k_panel_col0<T>, dim3((unsigned)((n - k2 + kNB - 1) / kNB)), dim3(256), 0, s, d.L, n,
+                         (int)c0, kb1, (const T*)(d.Linv + (size_t)(c0 / kNB) * kNB * kNB), 1, (const T*)d.slab);
+      T* li2 = d.Linv + (size_t)(k2 / kNB) * kNB * kNB;
+      hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, s, d.L, n, (int)k2, kb2, li2,
+                         d.LinvT + (size_t)(k2 / kNB) * kNB * kNB, c->info, (T*)nullptr);
+      const int64_t rest = n - k2 - kb2;
+      if (rest <= 0) break;
+      const int nt = (int)((rest + kNB - 1) / kNB);
+      hipLaunchKernelGGL(k_panel_col0<T>, dim3(nt), dim3(256), 0, s, d.L, n, (int)k2, kb2, (const T*)li2, 0,
+                         (const T*)nullptr);
+      // The next outer step's block columns were also updated by the
+      // previous step's stream2 tiles.
+      if (pending >= 0) HIP_TRY(hipStreamWaitEvent(s, c->chol_ev[1][pending], 0));
+      pending = -1;
+      hipLaunchKernelGGL(k_syrk2_mfma<T>, dim3((unsigned)(2 * nt - 1)), dim3(256), 0, s, d.L, n, (int)c0, nt, 1);
+      if (nt > 2) {
+        HIP_TRY(hipEventRecord(c->chol_ev[0][ring], s));
+        HIP_TRY(hipStreamWaitEvent(c->stream2, c->chol_ev[0][ring], 0));
+        hipLaunchKernelGGL(k_syrk2_mfma<T>, dim3((unsigned)((int64_t)(nt - 2) * (nt - 1) / 2)), dim3(256), 0,
+                           c->stream2, d.L, n, (int)c0, nt, 0);
+        HIP_TRY(hipEventRecord(c->chol_ev[1][ring], c->stream2));
+        pending = ring;
+        ring = (ring + 1) & 3;
       }
     }
     if (pending >= 0) HIP_TRY(hipStreamWaitEvent(s, c->chol_ev[1][pending], 0));
