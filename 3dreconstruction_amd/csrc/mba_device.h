@@ -2970,8 +2970,18 @@ __global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ A, int64_t n
         bad = bad ? bad : j + 1;
         d = T(1);
       }
-      const T sj = sqrt(d);
-      const T v = r == j ? sj : (r > j ? a[oq] / sj : T(0));
+      // FP32: the hardware reciprocal square root (~1 ulp) instead of an
+      // IEEE square root and division on this chain (8.6 vs 9.5 ms per
+      // factorisation at n = 9,000); the FP32 factor is refined in FP64.
+      T sj, v;
+      if constexpr (std::is_same<T, float>::value) {
+        const float is = __builtin_amdgcn_rsqf(d);
+        sj = d * is;
+        v = r == j ? sj : (r > j ? a[oq] * is : T(0));
+      } else {
+        sj = sqrt(d);
+        v = r == j ? sj : (r > j ? a[oq] / sj : T(0));
+      }
       a[oq] = v;
       cb[r & 3][r >> 2] = v;
     }
