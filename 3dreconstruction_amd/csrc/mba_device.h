@@ -371,9 +371,13 @@ __device__ __forceinline__ void rodrigues_with_jacobian(const double w[3], doubl
   }
 }
 
+// gtab (optional): also the local rotation basis of each camera (cam_lj,
+// below), in the same launch.
+template <typename T>
+__device__ __forceinline__ void cam_lj(const double* p, T* __restrict__ o);
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cam_table(int nc, const double* __restrict__ cams,
-                                                      T* __restrict__ tab) {
+                                                      T* __restrict__ tab, T* __restrict__ gtab) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   const double* p = cams + 9 * (size_t)c;
@@ -386,6 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_table(int nc, const double* __re
   o[kTabK3] = (T)0;
   for (int i = 0; i < 27; ++i) o[kTabDR + i] = (T)dR[i];
   for (int i = kTabDR + 27; i < kCamTab; ++i) o[i] = (T)0;
+  if (gtab) cam_lj(p, gtab + 9 * (size_t)c);
 }
 
 template <typename T>
@@ -521,13 +526,10 @@ __device__ __forceinline__ void jac_cam_local(const Proj<T>& o, const T y[3], T 
 }
 
 // G = J_l(w) = I + a [w]x + b [w]x^2, a = (1 - cos t) / t^2,
-// b = (t - sin t) / t^3 (series below t^2 = 1e-6), per camera, row-major
-// G[3 i + j], rounded to T.
+// b = (t - sin t) / t^3 (series below t^2 = 1e-6), for the camera at p,
+// row-major G[3 i + j], rounded to T.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cam_lj(int nc, const double* __restrict__ cams, T* __restrict__ gtab) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  const double* p = cams + 9 * (size_t)c;
+__device__ __forceinline__ void cam_lj(const double* p, T* __restrict__ o) {
   const double w0 = p[0], w1 = p[1], w2 = p[2];
   const double t2 = w0 * w0 + w1 * w1 + w2 * w2;
   double a, b;
@@ -540,7 +542,6 @@ __global__ __launch_bounds__(kBlock) void k_cam_lj(int nc, const double* __restr
     b = (t - sin(t)) / (t2 * t);
   }
   const double K[9] = {0.0, -w2, w1, w2, 0.0, -w0, -w1, w0, 0.0};
-  T* o = gtab + 9 * (size_t)c;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll

@@ -2524,20 +2524,17 @@ struct Engine {
   }
   mba_status set_radius(double r) { return set_radii(std::vector<double>(W, r)); }
 
-  // Kernel-precision camera table from the FP64 state.
+  // Kernel-precision camera table from the FP64 state (and, with gtab, the
+  // local rotation basis in the same launch).
   template <typename X>
-  void table_into(const double* cams, X* tab) {
+  void table_into(const double* cams, X* tab, X* gtab = nullptr) {
     if (c->grouped)
       hipLaunchKernelGGL(k_cam_table_g<X>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams,
                          (const int32_t*)c->d_cam_grp, tab);
     else
-      hipLaunchKernelGGL(k_cam_table<X>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, tab);
+      hipLaunchKernelGGL(k_cam_table<X>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, tab, gtab);
   }
-  void tables(const double* cams) {
-    table_into<T>(cams, d.ctab);
-    if (d.gtab && c->nc > 0)
-      hipLaunchKernelGGL(k_cam_lj<T>, dim3(nblocks(c->nc)), dim3(kBlock), 0, s, c->nc, cams, d.gtab);
-  }
+  void tables(const double* cams) { table_into<T>(cams, d.ctab, c->grouped ? nullptr : d.gtab); }
   // Row weights of the current state (slot rcur for mixed single problems).
   int wslot() const { return c->rfix ? c->rcur : 0; }
   T* wrow_cur() const { return wslot() ? d.wrow1 : d.wrow; }
@@ -4014,7 +4011,7 @@ static mba_status evaluate_impl(const mba_problem* p, double huber_a, double* re
         hipLaunchKernelGGL(k_cam_table_g<T>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, (const double*)cams,
                            (const int32_t*)cg, tab);
       else
-        hipLaunchKernelGGL(k_cam_table<T>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cams, tab);
+        hipLaunchKernelGGL(k_cam_table<T>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cams, tab, (T*)nullptr);
     }
     if (n) {
       if (grouped)
@@ -4247,7 +4244,7 @@ mba_status evaluate_noalloc_launch(double* scr, int32_t nc, int32_t np, int64_t 
                                    hipStream_t s) {
   T* tab = reinterpret_cast<T*>(scr);
   double* partial = scr + kNoAllocTable;
-  if (nc > 0) hipLaunchKernelGGL(k_cam_table<T>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cams, tab);
+  if (nc > 0) hipLaunchKernelGGL(k_cam_table<T>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cams, tab, (T*)nullptr);
   const int nb = (int)std::min<int64_t>(kNoAllocBlocks, std::max<int64_t>(1, (n + kBlock - 1) / kBlock));
   if (n > 0 && (res || jc || jp || cost))
     hipLaunchKernelGGL((k_eval_obs<T, double>), dim3(nb), dim3(kBlock), 0, s, n, (int)nc, (int)np, ci, pi, obs,
