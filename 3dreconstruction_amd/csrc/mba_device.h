@@ -1036,7 +1036,7 @@ template <typename T, int L, typename Cams>
 __device__ __forceinline__ void point_pass_block(
     SegPos q, const Cams& cams, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const double* __restrict__ pts64, const T* __restrict__ pscale, T huber_a,
-    const Win& win, double min_diag, double max_diag, double* __restrict__ diag_p, int recompute_diag,
+    const Win& win, double min_diag, double max_diag,
     double* __restrict__ V, T* __restrict__ gp, T* __restrict__ Vinv, T* __restrict__ pk,
     const T* __restrict__ rres, T* __restrict__ wrow, double* __restrict__ Vinv64, int e_fallback) {
   if (q.valid && win_idle_e(win, q.e)) q.valid = false;
@@ -1093,17 +1093,12 @@ __device__ __forceinline__ void point_pass_block(
   for (int i = 0; i < 6; ++i) V[6 * (size_t)e + i] = v[i];
 #pragma unroll
   for (int i = 0; i < 3; ++i) gp[3 * (size_t)e + i] = g[i];
+  // The LM diagonal, diag(V) clamped (levenberg_marquardt_strategy.cc:
+  // 104-111); not stored: V is, and re-damping derives it again.
   double dg[3];
   const int di[3] = {0, 3, 5};
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    if (recompute_diag) {
-      dg[i] = fmin(fmax(v[di[i]], min_diag), max_diag);
-      diag_p[3 * (size_t)e + i] = dg[i];
-    } else {
-      dg[i] = diag_p[3 * (size_t)e + i];
-    }
-  }
+  for (int i = 0; i < 3; ++i) dg[i] = fmin(fmax(v[di[i]], min_diag), max_diag);
   const int wi = win_of_e(win, e);
   const double inv_radius = win_inv_r(win, wi);
   double a[6] = {v[0], v[1], v[2], v[3], v[4], v[5]};
@@ -1134,12 +1129,11 @@ __global__ __launch_bounds__(kBlock) void k_point_pass(
     Seg sg, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const T* __restrict__ row_obs, const double* __restrict__ pts64, const T* __restrict__ pscale,
     const T* __restrict__ ctab, T huber_a, Win win, double min_diag, double max_diag,
-    double* __restrict__ diag_p, int recompute_diag, double* __restrict__ V, T* __restrict__ gp,
+    double* __restrict__ V, T* __restrict__ gp,
     T* __restrict__ Vinv, T* __restrict__ pk, const T* __restrict__ rres, T* __restrict__ wrow,
     double* __restrict__ Vinv64) {
   point_pass_block<T, L>(seg_locate(sg), GlobalCams<T, L>{ctab, nullptr}, pt_start, row_cam, row_obs, pts64, pscale,
-                         huber_a, win, min_diag, max_diag, diag_p, recompute_diag, V, gp, Vinv, pk, rres, wrow,
-                         Vinv64, sg.e0[0]);
+                         huber_a, win, min_diag, max_diag, V, gp, Vinv, pk, rres, wrow, Vinv64, sg.e0[0]);
 }
 
 // The same with the cameras' projection records staged in LDS (FP32,
@@ -1150,7 +1144,7 @@ __global__ __launch_bounds__(kStageThreads) void k_point_pass_lds(
     Seg sg, int nc, const int32_t* __restrict__ pt_start, const int32_t* __restrict__ row_cam,
     const float* __restrict__ row_obs, const double* __restrict__ pts64, const float* __restrict__ pscale,
     const float* __restrict__ ctab, float huber_a, Win win, double min_diag, double max_diag,
-    double* __restrict__ diag_p, int recompute_diag, double* __restrict__ V, float* __restrict__ gp,
+    double* __restrict__ V, float* __restrict__ gp,
     float* __restrict__ Vinv, float* __restrict__ pk, const float* __restrict__ rres, float* __restrict__ wrow,
     double* __restrict__ Vinv64) {
   extern __shared__ float4 stage4[];
@@ -1160,16 +1154,15 @@ __global__ __launch_bounds__(kStageThreads) void k_point_pass_lds(
   const int nblk = sg.blk0[sg.n];
   for (int vb = blockIdx.x * 4 + j; vb < nblk; vb += 4 * gridDim.x)
     point_pass_block<float, 0>(seg_locate_at(sg, vb, tid, kBlock), cams, pt_start, row_cam, row_obs, pts64, pscale,
-                               huber_a, win, min_diag, max_diag, diag_p, recompute_diag, V, gp, Vinv, pk, rres, wrow,
-                               Vinv64, sg.e0[0]);
+                               huber_a, win, min_diag, max_diag, V, gp, Vinv, pk, rres, wrow, Vinv64, sg.e0[0]);
 }
 
-// Re-damp after a rejected step: Vinv, w from stored V, g_p, diag_p.
+// Re-damp after a rejected step: Vinv, w from stored V and g_p, with the
+// LM diagonal (diag(V) clamped, as the pass formed it) under the new radius.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const double* __restrict__ V,
-                                                       const T* __restrict__ gp,
-                                                       const double* __restrict__ diag_p, Win win,
-                                                       const T* __restrict__ pscale,
+                                                       const T* __restrict__ gp, double min_diag, double max_diag,
+                                                       Win win, const T* __restrict__ pscale,
                                                        T* __restrict__ Vinv, T* __restrict__ pk,
                                                        double* __restrict__ Vinv64) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1178,9 +1171,9 @@ __global__ __launch_bounds__(kBlock) void k_point_damp(int ne, const double* __r
   const double inv_radius = win_inv_r(win, wi);
   double a[6];
   for (int i = 0; i < 6; ++i) a[i] = V[6 * (size_t)e + i];
-  a[0] += diag_p[3 * (size_t)e] * inv_radius;
-  a[3] += diag_p[3 * (size_t)e + 1] * inv_radius;
-  a[5] += diag_p[3 * (size_t)e + 2] * inv_radius;
+  a[0] += fmin(fmax(a[0], min_diag), max_diag) * inv_radius;
+  a[3] += fmin(fmax(a[3], min_diag), max_diag) * inv_radius;
+  a[5] += fmin(fmax(a[5], min_diag), max_diag) * inv_radius;
   double iv[6];
   if (!inv3_spd(a, iv)) {
     win.flag[wi] = 1;

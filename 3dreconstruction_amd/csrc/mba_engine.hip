@@ -1200,7 +1200,7 @@ struct mba_context {
   // FP64 state and LM data
   double *cams64 = nullptr, *cams64_t = nullptr, *pts64 = nullptr, *pts64_t = nullptr;
   double *ctab64 = nullptr, *row_obs64 = nullptr, *pscale64 = nullptr, *cscale64 = nullptr;
-  double *diag_p = nullptr, *diag_c = nullptr, *red = nullptr, *red2 = nullptr, *stepd = nullptr;
+  double *diag_c = nullptr, *red = nullptr, *red2 = nullptr, *stepd = nullptr;
   double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr, *gemv_part = nullptr;
   double* dtab64 = nullptr;  // FP64 direction table of the step (mixed: the model cost change)
   Scalars* sc = nullptr;        // device
@@ -2268,7 +2268,6 @@ mba_status context_create(const mba_problem* probs, int32_t W, const mba_options
   MBA_TRY(c->pool.alloc(&c->dtab64, (size_t)std::max(kDirTab, kDirTabG) * std::max(1, c->nc)));
   MBA_TRY(c->pool.alloc(&c->pscale64, 3 * (size_t)c->ne));
   MBA_TRY(c->pool.alloc(&c->cscale64, 9 * (size_t)c->nf));
-  MBA_TRY(c->pool.alloc(&c->diag_p, 3 * (size_t)c->ne));
   MBA_TRY(c->pool.alloc(&c->diag_c, 9 * (size_t)c->nf));
   MBA_TRY(c->pool.alloc(&c->stepd, 3 * (size_t)c->ne + 9 * (size_t)c->nf));
   const int64_t maxb = nblocks(std::max<int64_t>({(int64_t)c->ne, 9 * (int64_t)c->nf, (int64_t)c->nrows, 1})) + 64;
@@ -2617,16 +2616,16 @@ struct Engine {
       if (c->lds_pass.grid > 0) {
         hipLaunchKernelGGL(k_point_pass_lds, dim3(c->lds_pass.grid), dim3(kStageThreads), c->lds_pass.bytes, s, c->seg,
                            c->nc, d.pt_start, d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, win(),
-                           c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag, d.V, d.gp,
-                           vinv_out(), d.pk, fixed_rows(), wrow_cur(), d.Vinv64);
+                           c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, d.V, d.gp, vinv_out(), d.pk,
+                           fixed_rows(), wrow_cur(), d.Vinv64);
         staged = true;
       }
     }
     if (c->seg_blocks > 0 && !staged)
       hipLaunchKernelGGL((c->grouped ? k_point_pass<T, 1> : k_point_pass<T, 0>), dim3(c->seg_blocks), dim3(kBlock), 0,
                          s, c->seg, d.pt_start, d.row_cam, d.row_obs, c->pts64, d.pscale, d.ctab, huber, win(),
-                         c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, c->diag_p, recompute_diag,
-                         d.V, d.gp, vinv_out(), d.pk, fixed_rows(), c->grouped ? nullptr : wrow_cur(), d.Vinv64);
+                         c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, d.V, d.gp, vinv_out(), d.pk, fixed_rows(),
+                         c->grouped ? nullptr : wrow_cur(), d.Vinv64);
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
     if (c->nchunks > 0) {
       if (c->grouped) {
@@ -2665,7 +2664,7 @@ struct Engine {
   // Re-damp after a rejected step (radii current) and rebuild the rhs.
   void damp() {
     hipLaunchKernelGGL(k_point_damp<T>, dim3(nblocks(c->ne)), dim3(kBlock), 0, s, c->ne, (const double*)d.V, d.gp,
-                       c->diag_p, win(), d.pscale, vinv_out(), d.pk, d.Vinv64);
+                       c->opt.min_lm_diagonal, c->opt.max_lm_diagonal, win(), d.pscale, vinv_out(), d.pk, d.Vinv64);
     if (c->nf > 0) {
       sj_terms();
       hipLaunchKernelGGL(k_cam_damp<T>, dim3(nblocks(c->nf, 64)), dim3(64), 0, s, c->nf, d.U, c->diag_c, win(),
@@ -3293,12 +3292,18 @@ struct Engine {
     MBA_TRY(get(S.cams.data(), c->cams64, 8 * S.cams.size()));
     MBA_TRY(get(S.pts.data(), c->pts64, 8 * np3));
     MBA_TRY(get(S.pscale.data(), c->pscale64, 8 * np3));
-    MBA_TRY(get(S.diag_p.data(), c->diag_p, 8 * np3));
+    // The points' LM diagonal: diag(V) clamped, as the pass forms it.
+    std::vector<double> vblk(6 * (size_t)c->ne);
+    MBA_TRY(get(vblk.data(), d.V, 8 * vblk.size()));
     MBA_TRY(get(S.cscale.data(), c->cscale64, 8 * nf9));
     MBA_TRY(get(S.diag_c.data(), c->diag_c, 8 * nf9));
     MBA_TRY(get(sp.data(), d.step_p, sizeof(T) * np3));
     MBA_TRY(get(scm.data(), d.step_c, sizeof(T) * nf9));
     MBA_TRY(sync());
+    const int di[3] = {0, 3, 5};
+    for (size_t e = 0; e < (size_t)c->ne; ++e)
+      for (int i = 0; i < 3; ++i)
+        S.diag_p[3 * e + i] = std::min(std::max(vblk[6 * e + di[i]], c->opt.min_lm_diagonal), c->opt.max_lm_diagonal);
     S.step_p.assign(sp.begin(), sp.end());
     S.step_c.assign(scm.begin(), scm.end());
     S.radius = radius;
