@@ -92,10 +92,9 @@ def cam_pass_bytes(nrows, ne, nf, nchunks, elem):
 def point_pass_bytes(nrows, ne, nc, elem):
     """Per row camera id (4 B), observation (2 elem) and loss weight (1 elem);
     per point start (4 B), FP64 position (24 B) and scale (3 elem) read, and
-    V and V^-1 (FP64, 48 B each), g_p (3 elem), the packed point record
-    (8 elem) and the FP64 LM diagonal (24 B) written; camera projection
-    records (16 elem) once."""
-    return nrows * (4 + 3 * elem) + ne * (4 + 24 + 3 * elem + 96 + 3 * elem + 8 * elem + 24) + nc * 16 * elem
+    V and V^-1 (FP64, 48 B each), g_p (3 elem) and the packed point record
+    (8 elem) written; camera projection records (16 elem) once."""
+    return nrows * (4 + 3 * elem) + ne * (4 + 24 + 3 * elem + 96 + 3 * elem + 8 * elem) + nc * 16 * elem
 
 
 # Whole solves, the reference's Ceres on the host cores against the library on
