@@ -2911,6 +2911,69 @@ __global__ __launch_bounds__(kWinThreads) void k_win_chol_solve(const int64_t* _
 constexpr int kNB = 64;
 
 
+// Scale column J of the diagonal block (this lane's entry aj, row r) by
+// its pivot and publish it in LDS (colp[J % 2], stored as [r % 4][r / 4]).
+template <typename T, int J>
+__device__ __forceinline__ void potrf_pivot(T& aj, T (*colp)[4][16], int r, int& bad) {
+  T d = lane_bcast(aj, J);
+  if (!(d > T(0))) {
+    bad = bad ? bad : J + 1;
+    d = T(1);
+  }
+  // FP32: the hardware reciprocal square root (~1 ulp) instead of an IEEE
+  // square root and division on this chain (8.6 vs 9.5 ms per factorisation
+  // at n = 9,000); the FP32 factor is refined in FP64.
+  T v;
+  if constexpr (std::is_same<T, float>::value) {
+    const float is = __builtin_amdgcn_rsqf(d);
+    v = r == J ? d * is : (r > J ? aj * is : T(0));
+  } else {
+    const T sj = sqrt(d);
+    v = r == J ? sj : (r > J ? aj / sj : T(0));
+  }
+  aj = v;
+  colp[J & 1][r & 3][r >> 2] = v;
+}
+
+// Factor step J (column J published before the barrier): the wave owning
+// column J + 1 applies step J's update to that column first and publishes
+// it, so the next pivot is ready while the other columns are updated.
+// Finished columns (c <= J) are left alone.  A template on J, force-inlined
+// into the kernel, keeps every register index a compile-time constant.
+template <typename T, int J>
+__device__ __forceinline__ void potrf_step(T (&a)[16], T (*colp)[4][16], int r, int w, int& bad) {
+  constexpr int jn = J + 1, own = jn & 3, qn = (jn >> 2) & 15;
+  __syncthreads();
+  T(*cb)[16] = colp[J & 1];
+  const T lr = cb[r & 3][r >> 2];
+  if (jn < kNB && w == own) {
+    a[qn] = fma(-lr, cb[w][qn], a[qn]);
+    potrf_pivot<T, (jn < kNB ? jn : 0)>(a[qn], colp, r, bad);
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    if (4 * q + 3 > J) {
+      const T m = (w + 4 * q > J) ? cb[w][q] : T(0);
+      if (jn < kNB && q == qn) {
+        if (w != own) a[q] = fma(-lr, m, a[q]);
+      } else {
+        a[q] = fma(-lr, m, a[q]);
+      }
+    }
+  }
+}
+template <typename T, int J>
+struct PotrfSteps {
+  static __device__ __forceinline__ void run(T (&a)[16], T (*colp)[4][16], int r, int w, int& bad) {
+    potrf_step<T, J>(a, colp, r, w, bad);
+    PotrfSteps<T, J + 1>::run(a, colp, r, w, bad);
+  }
+};
+template <typename T>
+struct PotrfSteps<T, kNB> {
+  static __device__ __forceinline__ void run(T (&)[16], T (*)[4][16], int, int, int&) {}
+};
+
 // Four waves: factor the kb x kb diagonal block at (k, k), write it back,
 // and write L11^-1 (64 x 64 column-major, zero-padded) and its transpose
 // (kept per block for the triangular solves).  Rows and columns past kb are
@@ -2919,9 +2982,11 @@ constexpr int kNB = 64;
 // owning column j takes the pivot from lane j, scales the column and
 // publishes it in LDS (two buffers, so one barrier per step; stored as
 // colp[c % 4][c / 4] so that a wave's 16 columns are contiguous); every
-// wave then applies the rank-1 update to its columns c > j without
-// branches (columns c <= j get a zero multiplier; entries above the
-// diagonal are updated too and cleared when their column is scaled).
+// wave then applies the rank-1 update to its unfinished columns (columns
+// c <= j in a group of four with one c > j get a zero multiplier; entries
+// above the diagonal are updated too and cleared when their column is
+// scaled).  Steps are pipelined (potrf_step): column j + 1 is updated and
+// published before the rest of step j's update.
 // Inverse: the factor goes to LDS and each wave forms its 16 columns of
 // L11^-1 by column-oriented forward substitution, lane r holding row r and
 // X[j][c] broadcast from lane j times 1 / L[j][j]; step j touches only the
@@ -2939,54 +3004,22 @@ __global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ A, int64_t n
   __shared__ T rinv[kNB];
   __shared__ int bad_w[4];
   const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (slab) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = w + 4 * q;
-      const int64_t row = (int64_t)k + kb + r;
-      slab[r + kNB * c] = (row < n && c < kb) ? A[row + (int64_t)(k + c) * n] : T(0);
-    }
-  }
-  T a[16];
+  // The slab's and the block's loads in one batch (one memory round trip).
+  T a[16], sv[16];
+  const int64_t srow = (int64_t)k + kb + r;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = w + 4 * q;
+    if (slab) sv[q] = (srow < n && c < kb) ? A[srow + (int64_t)(k + c) * n] : T(0);
     a[q] = (r < kb && c < kb) ? A[(k + r) + (int64_t)(k + c) * n] : (r == c ? T(1) : T(0));
   }
-  int bad = 0;  // first non-positive pivot among this wave's columns, + 1
+  if (slab) {
 #pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    const int ow = j & 3, oq = j >> 2;
-    T(*cb)[16] = colp[j & 1];
-    if (w == ow) {
-      T d = lane_bcast(a[oq], j);
-      if (!(d > T(0))) {
-        bad = bad ? bad : j + 1;
-        d = T(1);
-      }
-      // FP32: the hardware reciprocal square root (~1 ulp) instead of an
-      // IEEE square root and division on this chain (8.6 vs 9.5 ms per
-      // factorisation at n = 9,000); the FP32 factor is refined in FP64.
-      T sj, v;
-      if constexpr (std::is_same<T, float>::value) {
-        const float is = __builtin_amdgcn_rsqf(d);
-        sj = d * is;
-        v = r == j ? sj : (r > j ? a[oq] * is : T(0));
-      } else {
-        sj = sqrt(d);
-        v = r == j ? sj : (r > j ? a[oq] / sj : T(0));
-      }
-      a[oq] = v;
-      cb[r & 3][r >> 2] = v;
-    }
-    __syncthreads();
-    const T lr = cb[r & 3][r >> 2];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const T m = (w + 4 * q > j) ? cb[w][q] : T(0);
-      a[q] = fma(-lr, m, a[q]);
-    }
+    for (int q = 0; q < 16; ++q) slab[r + kNB * (w + 4 * q)] = sv[q];
   }
+  int bad = 0;  // first non-positive pivot among this wave's columns, + 1
+  if (w == 0) potrf_pivot<T, 0>(a[0], colp, r, bad);
+  PotrfSteps<T, 0>::run(a, colp, r, w, bad);
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = w + 4 * q;
@@ -3017,12 +3050,20 @@ __global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ A, int64_t n
       x[q] = r == j ? xj : (r > j ? fma(-lrj, xj, x[q]) : x[q]);
     }
   }
+  // LinvT goes through LDS so that both stores run along columns.
+  __syncthreads();
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = w + 4 * q;
     const T v = (r < kb && c < kb) ? x[q] : T(0);
     Linv[r + kNB * c] = v;
-    LinvT[c + kNB * r] = v;
+    Ls[r][c] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = w + 4 * q;
+    LinvT[r + kNB * c] = Ls[c][r];
   }
 }
 
