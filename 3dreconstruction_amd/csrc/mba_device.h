@@ -2904,7 +2904,8 @@ __global__ __launch_bounds__(kWinThreads) void k_win_chol_solve(const int64_t* _
 //   k_potrf_diag   factors a 64 x 64 diagonal block and inverts it;
 //   k_panel_col0   L21 = A21 L11^-T as a GEMM with the inverted block (for
 //                  the outer step's first block also A22 -= L21 L21' on the
-//                  second block's columns);
+//                  second block's columns, after which workgroup 0 factors
+//                  the second block);
 //   k_syrk2_mfma   A22 -= L21a L21a' + L21b L21b' with both blocks' panels
 //                  on the trailing tiles (K = 128).
 // The triangular solves with the factor run one launch per direction.
@@ -2996,13 +2997,12 @@ template <typename T>
 // sub-diagonal slab (rows k + kb .. k + kb + 63 of its columns, zero past
 // row n), which k_panel_col0's workgroups read as their operand while its
 // first workgroup overwrites the slab in A with L21.
-__global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ A, int64_t n, int k, int kb,
-                                                    T* __restrict__ Linv, T* __restrict__ LinvT,
-                                                    int32_t* __restrict__ info, T* __restrict__ slab) {
-  __shared__ T colp[2][4][16];
-  __shared__ T Ls[kNB][kNB + 1];
-  __shared__ T rinv[kNB];
-  __shared__ int bad_w[4];
+// The factor of one diagonal block by a 256-thread workgroup, with its LDS
+// arrays passed in (k_panel_col0 lends its tiles).
+__device__ __forceinline__ void potrf_block(T* __restrict__ A, int64_t n, int k, int kb, T* __restrict__ Linv,
+                                            T* __restrict__ LinvT, int32_t* __restrict__ info,
+                                            T* __restrict__ slab, T (*colp)[4][16], T (*Ls)[kNB + 1],
+                                            T* rinv, int* bad_w) {
   const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
   // The slab's and the block's loads in one batch (one memory round trip).
   T a[16], sv[16];
@@ -3065,6 +3065,16 @@ __global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ A, int64_t n
     const int c = w + 4 * q;
     LinvT[r + kNB * c] = Ls[c][r];
   }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ A, int64_t n, int k, int kb,
+                                                    T* __restrict__ Linv, T* __restrict__ LinvT,
+                                                    int32_t* __restrict__ info, T* __restrict__ slab) {
+  __shared__ T colp[2][4][16];
+  __shared__ T Ls[kNB][kNB + 1];
+  __shared__ T rinv[kNB];
+  __shared__ int bad_w[4];
+  potrf_block<T>(A, n, k, kb, Linv, LinvT, info, slab, colp, Ls, rinv, bad_w);
 }
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -3223,7 +3233,8 @@ __device__ __forceinline__ void load_cols(T (*Ts)[kNB + 1], const T* __restrict_
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_col0(T* __restrict__ A, int64_t n, int k, int kb,
                                                     const T* __restrict__ Linv, int col0,
-                                                    const T* __restrict__ slab) {
+                                                    const T* __restrict__ slab, T* __restrict__ Linv2,
+                                                    T* __restrict__ LinvT2, int32_t* __restrict__ info) {
   __shared__ T As[kNB][kNB + 1];
   __shared__ T Bs[kNB][kNB + 1];
   __shared__ T Cs[kNB][kNB + 1];
@@ -3247,6 +3258,14 @@ __global__ __launch_bounds__(256) void k_panel_col0(T* __restrict__ A, int64_t n
   TileAcc<T> up;
   tile_mma(Cs, As, kb, up);  // element (i, j) = sum_l L21[base + i][l] L21[r0 + j][l]
   tile_sub_lower(up, A, n, r0, base);
+  if (Linv2 == nullptr || blockIdx.x != 0) return;
+  // Workgroup 0 has just finished the next block's diagonal tile (no other
+  // workgroup writes it): factor that block here instead of in a launch of
+  // its own.  Its stores are complete and visible to the workgroup after
+  // the barrier; the LDS tiles are free.
+  __syncthreads();
+  potrf_block<T>(A, n, (int)base, (int)(n - base < kNB ? n - base : kNB), Linv2, LinvT2, info, nullptr,
+                 reinterpret_cast<T(*)[4][16]>(&Bs[0][0]), As, &Cs[0][0], reinterpret_cast<int*>(&Cs[2][0]));
 }
 
 __device__ __forceinline__ void syrk_tile_coords(int t, int& ti, int& tj) {

@@ -3014,16 +3014,17 @@ struct Engine {
       const int64_t k2 = c0 + kb1;
       if (k2 >= n) break;
       const int kb2 = (int)std::min<int64_t>(kNB, n - k2);
-      hipLaunchKernelGGL(k_panel_col0<T>, dim3((unsigned)((n - k2 + kNB - 1) / kNB)), dim3(256), 0, s, d.L, n,
-                         (int)c0, kb1, (const T*)(d.Linv + (size_t)(c0 / kNB) * kNB * kNB), 1, (const T*)d.slab);
+      // The panel's workgroup 0 also factors block k2 (it writes that
+      // block's last update).
       T* li2 = d.Linv + (size_t)(k2 / kNB) * kNB * kNB;
-      hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, s, d.L, n, (int)k2, kb2, li2,
-                         d.LinvT + (size_t)(k2 / kNB) * kNB * kNB, c->info, (T*)nullptr);
+      hipLaunchKernelGGL(k_panel_col0<T>, dim3((unsigned)((n - k2 + kNB - 1) / kNB)), dim3(256), 0, s, d.L, n,
+                         (int)c0, kb1, (const T*)(d.Linv + (size_t)(c0 / kNB) * kNB * kNB), 1, (const T*)d.slab,
+                         li2, d.LinvT + (size_t)(k2 / kNB) * kNB * kNB, c->info);
       const int64_t rest = n - k2 - kb2;
       if (rest <= 0) break;
       const int nt = (int)((rest + kNB - 1) / kNB);
       hipLaunchKernelGGL(k_panel_col0<T>, dim3(nt), dim3(256), 0, s, d.L, n, (int)k2, kb2, (const T*)li2, 0,
-                         (const T*)nullptr);
+                         (const T*)nullptr, (T*)nullptr, (T*)nullptr, (int32_t*)nullptr);
       // The next outer step's block columns were also updated by the
       // previous step's stream2 tiles.
       if (pending >= 0) HIP_TRY(hipStreamWaitEvent(s, c->chol_ev[1][pending], 0));
