@@ -2774,6 +2774,48 @@ __global__ __launch_bounds__(64) void k_s_blocks(
   const int e1 = lane + 64, i1 = e1 / 9, j1 = e1 % 9;
   const bool two = e1 < 81;
   T acc0 = T(0), acc1 = T(0);
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32: lane l takes pairs p0 + l, p0 + l + 64, ... and keeps all 81
+    // sums in registers (its pair's two rows as 16-byte loads: both halves
+    // of a row are 28 floats, 16-byte aligned); the wave then adds the 64
+    // lanes' sums in lane order through LDS.  No LDS traffic per pair.
+    float s[81];
+#pragma unroll
+    for (int e = 0; e < 81; ++e) s[e] = 0.0f;
+    for (int64_t q = p0 + lane; q < p1; q += 64) {
+      const float4* ya = reinterpret_cast<const float4*>(WY + (size_t)kWY * pair_a[q] + kWYY);
+      const float4* wb = reinterpret_cast<const float4*>(WY + (size_t)kWY * pair_b[q]);
+      float y[28], w[28];
+#pragma unroll
+      for (int v = 0; v < 7; ++v) {
+        const float4 a4 = ya[v], b4 = wb[v];
+        y[4 * v] = a4.x; y[4 * v + 1] = a4.y; y[4 * v + 2] = a4.z; y[4 * v + 3] = a4.w;
+        w[4 * v] = b4.x; w[4 * v + 1] = b4.y; w[4 * v + 2] = b4.z; w[4 * v + 3] = b4.w;
+      }
+#pragma unroll
+      for (int i = 0; i < 9; ++i)
+#pragma unroll
+        for (int j = 0; j < 9; ++j)
+          s[9 * i + j] += y[3 * i] * w[3 * j] + y[3 * i + 1] * w[3 * j + 1] + y[3 * i + 2] * w[3 * j + 2];
+    }
+    // Two rounds through 41 x 65 floats of LDS: entries 0..40 (lane e sums
+    // entry e), then 41..80 (lanes 41..63 entries 41..63, lanes 0..16
+    // entries 64..80 as acc1).
+    __shared__ float red[41 * 65];
+#pragma unroll
+    for (int e = 0; e < 41; ++e) red[e * 65 + lane] = s[e];
+    __syncthreads();
+    if (lane < 41)
+      for (int m = 0; m < 64; ++m) acc0 += red[lane * 65 + m];
+    __syncthreads();
+#pragma unroll
+    for (int e = 41; e < 81; ++e) red[(e - 41) * 65 + lane] = s[e];
+    __syncthreads();
+    if (lane >= 41)
+      for (int m = 0; m < 64; ++m) acc0 += red[(lane - 41) * 65 + m];
+    if (two)
+      for (int m = 0; m < 64; ++m) acc1 += red[(e1 - 41) * 65 + m];
+  } else
   for (int64_t base = p0; base < p1; base += 64) {
     const int np = (int)min((int64_t)64, p1 - base);
     if (lane < np) {
