@@ -3453,52 +3453,100 @@ __device__ __forceinline__ void flow_publish(T* out, uint64_t* hand, int nrows, 
   }
 }
 
-// Forward L y = b, workgroup i = block i: y_i = L_ii^-1 (b_i - sum_{k<i}
+// Forward L y = b, two blocks per workgroup: workgroup g solves blocks
+// a = 2g and b = 2g + 1 (if b < nb) with y_i = L_ii^-1 (b_i - sum_{k<i}
 // L_ik y_k).  Thread (wave w, lane) accumulates row lane of columns
-// 16w..16w+15 of every L_ik (coalesced column reads), loading the tile
-// before polling for y_k.  hand: n values as epoch words.
+// 16w..16w+15 of every L_ak and L_bk, k < a (coalesced column reads), loading
+// the tiles before polling for y_k; block a is then finished and published,
+// and block b adds L_ba y_a with y_a passed through LDS (one hand-off between
+// workgroups per two blocks).  The sums run over k in the same order as with
+// one block per workgroup.  hand: n values as epoch words.
 template <typename T>
 __global__ __launch_bounds__(kFlowThreads) void k_trsv_lower_flow(const T* __restrict__ L, const T* __restrict__ Linv,
-                                                                  int64_t n, const T* __restrict__ b,
+                                                                  int64_t n, int nb, const T* __restrict__ b,
                                                                   T* __restrict__ y, uint64_t* hand, unsigned epoch,
                                                                   uint64_t* err, unsigned spin_max) {
   constexpr int C = flow_words<T>();
   __shared__ T part[4][kNB];
   __shared__ T sv[kNB];
-  const int i = blockIdx.x;
+  const int ia = 2 * (int)blockIdx.x, ibb = ia + 1;
+  const bool two = ibb < nb;  // then block a is full
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t i0 = (int64_t)i * kNB;
-  const int ib = (int)min((int64_t)kNB, n - i0);
-  const int64_t row = i0 + (lane < ib ? lane : 0);
-  T m[16];
-  flow_block_load(Linv + (size_t)i * kNB * kNB, m);
-  const T bi = w == 0 && lane < ib ? b[i0 + lane] : T(0);
-  T acc = T(0);
+  const int64_t a0 = (int64_t)ia * kNB, b0 = a0 + kNB;
+  const int na = (int)min((int64_t)kNB, n - a0);
+  const int nbr = two ? (int)min((int64_t)kNB, n - b0) : 0;
+  const int64_t rowa = a0 + (lane < na ? lane : 0);
+  const int64_t rowb = two ? b0 + (lane < nbr ? lane : 0) : rowa;
+  T ma[16], mb[16];
+  flow_block_load(Linv + (size_t)ia * kNB * kNB, ma);
+  if (two) flow_block_load(Linv + (size_t)ibb * kNB * kNB, mb);
+  const T ba = w == 0 && lane < na ? b[a0 + lane] : T(0);
+  const T bb = w == 0 && lane < nbr ? b[b0 + lane] : T(0);
+  T acca = T(0), accb = T(0);
   bool good = true;
-  for (int k = 0; k < i && good; ++k) {
-    // Block k < i is full.
+  for (int k = 0; k < ia && good; ++k) {
+    // Block k < a is full.
     const int64_t c0 = (int64_t)k * kNB + 16 * w;
-    const T* col = L + row + c0 * n;
-    T l[16], yk[16];
+    const T* cola = L + rowa + c0 * n;
+    const T* colb = L + rowb + c0 * n;
+    T la[16], lb[16], yk[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) l[j] = col[j * n];
+    for (int j = 0; j < 16; ++j) {
+      la[j] = cola[j * n];
+      lb[j] = two ? colb[j * n] : T(0);
+    }
     good = flow_get<T, 16>(hand + C * c0, epoch, err, spin_max, yk);
 #pragma unroll
-    for (int j = 0; j < 16; ++j) acc = fma(l[j], yk[j], acc);
+    for (int j = 0; j < 16; ++j) {
+      acca = fma(la[j], yk[j], acca);
+      accb = fma(lb[j], yk[j], accb);
+    }
   }
-  part[w][lane] = acc;
+  // L_ba: row lane of block b, columns 16w..16w+15 of block a.
+  T lba[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) lba[j] = two ? L[rowb + (a0 + 16 * w + j) * n] : T(0);
+  part[w][lane] = acca;
   good = __syncthreads_and(good) != 0;
-  if (w == 0) sv[lane] = lane < ib ? bi - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) : T(0);
+  if (w == 0) sv[lane] = lane < na ? ba - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) : T(0);
   __syncthreads();
-  T yi = flow_block_gemv(m, sv, part);
-  if (!good) yi = T(NAN);
-  flow_publish(y + i0, hand + C * i0, ib, yi, epoch);
+  T ya = flow_block_gemv(ma, sv, part);
+  if (!good) ya = T(NAN);
+  flow_publish(y + a0, hand + C * a0, na, ya, epoch);
+  if (!two) return;
+  __syncthreads();  // the gemv's reads of sv and part are done
+  if (w == 0) sv[lane] = ya;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 16; ++j) accb = fma(lba[j], sv[16 * w + j], accb);
+  part[w][lane] = accb;
+  __syncthreads();
+  if (w == 0) sv[lane] = lane < nbr ? bb - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) : T(0);
+  __syncthreads();
+  T yb = flow_block_gemv(mb, sv, part);
+  if (!good) yb = T(NAN);
+  flow_publish(y + b0, hand + C * b0, nbr, yb, epoch);
 }
 
-// Backward L' x = y, workgroup j = block nb - 1 - j: x_i = L_ii^-T (y_i -
+// Backward L' x = y, two blocks per workgroup: workgroup g solves blocks
+// h = nb - 1 - 2g and l = h - 1 (if l >= 0) with x_i = L_ii^-T (y_i -
 // sum_{k>i} L_ki' x_k).  Thread (wave w, lane) accumulates lane's share of
-// rows 16w..16w+15 of every L_ki' (column i0 + 16w + r of L, read along
-// its rows k0 + lane: coalesced), summed over the wave once at the end.
+// rows 16w..16w+15 of every L_kh' and L_kl', k > h (columns of L read along
+// their rows k0 + lane: coalesced), summed over the wave at the end; block h
+// is finished and published first, and block l adds L_hl' x_h, each thread
+// holding x_h's row lane from the block product.
+template <typename T>
+__device__ __forceinline__ T flow_upper_finish(T acc[16], T yv, int ib, const T m[16], T* sv, T (*part)[kNB]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const T v = wave_sum(acc[r]);
+    if (lane == r) acc[0] = v;  // lane r keeps row 16w + r
+  }
+  if (lane < 16) sv[16 * w + lane] = 16 * w + lane < ib ? yv - acc[0] : T(0);
+  __syncthreads();
+  return flow_block_gemv(m, sv, part);
+}
 template <typename T>
 __global__ __launch_bounds__(kFlowThreads) void k_trsv_upper_flow(const T* __restrict__ L, const T* __restrict__ LinvT,
                                                                   int64_t n, int nb, const T* __restrict__ y,
@@ -3507,42 +3555,58 @@ __global__ __launch_bounds__(kFlowThreads) void k_trsv_upper_flow(const T* __res
   constexpr int C = flow_words<T>();
   __shared__ T part[4][kNB];
   __shared__ T sv[kNB];
-  const int i = nb - 1 - (int)blockIdx.x;
+  const int ih = nb - 1 - 2 * (int)blockIdx.x, il = ih - 1;
+  const bool two = il >= 0;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t i0 = (int64_t)i * kNB;
-  const int ib = (int)min((int64_t)kNB, n - i0);
-  T m[16];
-  flow_block_load(LinvT + (size_t)i * kNB * kNB, m);
-  const T yi = lane < 16 && 16 * w + lane < ib ? y[i0 + 16 * w + lane] : T(0);
-  T acc[16];
+  const int64_t h0 = (int64_t)ih * kNB, l0 = h0 - kNB;
+  const int nh = (int)min((int64_t)kNB, n - h0);  // block l is full
+  T mh[16], ml[16];
+  flow_block_load(LinvT + (size_t)ih * kNB * kNB, mh);
+  if (two) flow_block_load(LinvT + (size_t)il * kNB * kNB, ml);
+  const T yh = lane < 16 && 16 * w + lane < nh ? y[h0 + 16 * w + lane] : T(0);
+  const T yl = two && lane < 16 ? y[l0 + 16 * w + lane] : T(0);
+  T acch[16], accl[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = T(0);
+  for (int r = 0; r < 16; ++r) acch[r] = accl[r] = T(0);
   bool good = true;
-  for (int k = nb - 1; k > i && good; --k) {
-    // Block i < k is full; block k is the last (possibly partial) one.
+  for (int k = nb - 1; k > ih && good; --k) {
+    // Blocks h, l < k are full; block k is the last (possibly partial) one.
     const int64_t k0 = (int64_t)k * kNB;
     const bool on = lane < (int)min((int64_t)kNB, n - k0);
-    const T* p = L + (k0 + (on ? lane : 0)) + (i0 + 16 * w) * n;
-    T l[16];
+    const T* ph = L + (k0 + (on ? lane : 0)) + (h0 + 16 * w) * n;
+    const T* pl = L + (k0 + (on ? lane : 0)) + (l0 + 16 * w) * n;
+    T lh[16], ll[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) l[r] = on ? p[r * n] : T(0);
+    for (int r = 0; r < 16; ++r) {
+      lh[r] = on ? ph[r * n] : T(0);
+      ll[r] = on && two ? pl[r * n] : T(0);
+    }
     T xk[1] = {T(0)};
     if (on) good = flow_get<T, 1>(hand + C * (k0 + lane), epoch, err, spin_max, xk);
     good = __all(good);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = fma(l[r], xk[0], acc[r]);
+    for (int r = 0; r < 16; ++r) {
+      acch[r] = fma(lh[r], xk[0], acch[r]);
+      accl[r] = fma(ll[r], xk[0], accl[r]);
+    }
   }
-  good = __syncthreads_and(good) != 0;
+  // L_hl: rows h0 + lane (block h, possibly partial), columns l0 + 16w + r.
+  const bool onh = lane < nh;
+  T lhl[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const T v = wave_sum(acc[r]);
-    if (lane == r) acc[0] = v;  // lane r keeps row 16w + r
-  }
-  if (lane < 16) sv[16 * w + lane] = 16 * w + lane < ib ? yi - acc[0] : T(0);
-  __syncthreads();
-  T xi = flow_block_gemv(m, sv, part);
-  if (!good) xi = T(NAN);
-  flow_publish(x + i0, hand + C * i0, ib, xi, epoch);
+  for (int r = 0; r < 16; ++r) lhl[r] = two && onh ? L[(h0 + lane) + (l0 + 16 * w + r) * n] : T(0);
+  good = __syncthreads_and(good) != 0;
+  T xh = flow_upper_finish(acch, yh, nh, mh, sv, part);
+  if (!good) xh = T(NAN);
+  flow_publish(x + h0, hand + C * h0, nh, xh, epoch);
+  if (!two) return;
+  const T xo = onh ? xh : T(0);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) accl[r] = fma(lhl[r], xo, accl[r]);
+  __syncthreads();  // the block product's reads of sv and part are done
+  T xl = flow_upper_finish(accl, yl, kNB, ml, sv, part);
+  if (!good) xl = T(NAN);
+  flow_publish(x + l0, hand + C * l0, kNB, xl, epoch);
 }
 
 // FP64 residual r = b - S x (S column-major, both triangles stored) in two

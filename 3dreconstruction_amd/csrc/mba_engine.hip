@@ -2904,10 +2904,11 @@ struct Engine {
     const int nb = (int)((n + kNB - 1) / kNB);
     uint64_t* err = d.flow + (size_t)flow_words<T>() * n;
     const unsigned e1 = ++d.flow_epoch, e2 = ++d.flow_epoch;
-    hipLaunchKernelGGL(k_trsv_lower_flow<T>, dim3(nb), dim3(kFlowThreads), 0, s, (const T*)d.L, (const T*)d.Linv, n,
-                       (const T*)v, d.ty, d.flow, e1, err, c->flow_spin);
-    hipLaunchKernelGGL(k_trsv_upper_flow<T>, dim3(nb), dim3(kFlowThreads), 0, s, (const T*)d.L, (const T*)d.LinvT, n,
-                       nb, (const T*)d.ty, v, d.flow, e2, err, c->flow_spin);
+    // Two 64-row blocks per workgroup.
+    hipLaunchKernelGGL(k_trsv_lower_flow<T>, dim3((nb + 1) / 2), dim3(kFlowThreads), 0, s, (const T*)d.L,
+                       (const T*)d.Linv, n, nb, (const T*)v, d.ty, d.flow, e1, err, c->flow_spin);
+    hipLaunchKernelGGL(k_trsv_upper_flow<T>, dim3((nb + 1) / 2), dim3(kFlowThreads), 0, s, (const T*)d.L,
+                       (const T*)d.LinvT, n, nb, (const T*)d.ty, v, d.flow, e2, err, c->flow_spin);
   }
 
   // Explicit Schur (single problem): assemble dense S, Cholesky, solve,
