@@ -2686,18 +2686,28 @@ __global__ __launch_bounds__(kBlock) void k_s_rows(
     jac_point(o, c.R, wgt, sp, Jp);
     jac_cam(o, tab + kTabDR, X, wgt, cs, Jc);
     T* out = WY + (size_t)kWY * r;
-    T w[27];
+    // Both halves (27 values and a zero pad) as 16-byte stores.
+    T w[kWYY], y[kWYY];
 #pragma unroll
     for (int i = 0; i < 9; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) w[3 * i + j] = Jc[i] * Jp[j] + Jc[9 + i] * Jp[3 + j];
 #pragma unroll
-    for (int k = 0; k < 27; ++k) out[k] = w[k];
-#pragma unroll
     for (int i = 0; i < 9; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        out[kWYY + 3 * i + j] = w[3 * i] * Vi[j] + w[3 * i + 1] * Vi[3 + j] + w[3 * i + 2] * Vi[6 + j];
+        y[3 * i + j] = w[3 * i] * Vi[j] + w[3 * i + 1] * Vi[3 + j] + w[3 * i + 2] * Vi[6 + j];
+    w[27] = y[27] = T(0);
+    using V16 = typename std::conditional<std::is_same<T, float>::value, float4, double2>::type;
+    constexpr int nv = kWYY * (int)sizeof(T) / 16;
+#pragma unroll
+    for (int v = 0; v < nv; ++v) {
+      V16 a, b;
+      __builtin_memcpy(&a, &w[v * 16 / sizeof(T)], 16);
+      __builtin_memcpy(&b, &y[v * 16 / sizeof(T)], 16);
+      reinterpret_cast<V16*>(out)[v] = a;
+      reinterpret_cast<V16*>(out + kWYY)[v] = b;
+    }
   }
 }
 
