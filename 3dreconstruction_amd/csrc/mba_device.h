@@ -813,7 +813,7 @@ __global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __rest
         // With a loss, its weight sqrt(rho') too, evaluated in FP64.
         const float w = wrow ? (float)huber_weight(s, huber_a) : 1.0f;
         if (wrow) wrow[r] = w;
-        const int k = row_cm[r];
+        const int k = rcm ? row_cm[r] : -1;  // rcm null: camera-major copies later (k_rows_to_cm)
         if (k >= 0) {
           rcm[2 * (size_t)k] = (float)o.rx;
           rcm[2 * (size_t)k + 1] = (float)o.ry;
@@ -833,6 +833,22 @@ __global__ __launch_bounds__(kBlock) void k_cost64(Seg sg, const int32_t* __rest
     for (int w = 0; w < kBlock / 64; ++w) s += lds[w];
     partial[blockIdx.x] = s;
   }
+}
+
+// The camera-major copies of k_cost64's row-order FP32 residuals and weights
+// (rcm[2 k], rcm[2 k + 1], wcm[k] for k = row_cm[r] >= 0), made when a trial
+// state becomes current: a rejected trial state's copies are never read.
+__global__ __launch_bounds__(kBlock) void k_rows_to_cm(int64_t nrows, const int32_t* __restrict__ row_cm,
+                                                       const float* __restrict__ rrow,
+                                                       const float* __restrict__ wrow, float* __restrict__ rcm,
+                                                       float* __restrict__ wcm) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int k = row_cm[r];
+  if (k < 0) return;
+  const float2 v = reinterpret_cast<const float2*>(rrow)[r];
+  reinterpret_cast<float2*>(rcm)[k] = v;
+  if (wcm) wcm[k] = wrow ? wrow[r] : 1.0f;
 }
 
 // Deterministic single-block sum of partials.

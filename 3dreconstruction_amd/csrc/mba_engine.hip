@@ -2545,21 +2545,30 @@ struct Engine {
   // of rres / cres.  The flags are clear again when this returns (in stream
   // order), so what the next Jacobian pass or re-damping flags (a point or
   // camera block that is not positive definite) reaches the step's readback.
-  mba_status cost64_async(const double* cams, const double* pts, int slot = -1) {
+  // cm = false (a trial state): the camera-major copies are made only if the
+  // state is accepted (cm_copies).
+  mba_status cost64_async(const double* cams, const double* pts, int slot = -1, bool cm = true) {
     MBA_TRY(clear_flags());
     table_into<double>(cams, c->ctab64);
     const int nb = c->seg_blocks;
     MBA_TRY(begin_sum());
-    const bool wr = c->rfix && slot >= 0;
+    const bool wr = c->rfix && slot >= 0, wc = wr && cm;
     if (nb > 0)
       hipLaunchKernelGGL((c->grouped ? k_cost64<1> : k_cost64<0>), dim3(nb), dim3(kBlock), 0, s, c->seg, d.pt_start,
                          d.row_cam, c->row_obs64, pts, c->ctab64, c->opt.huber_a, c->red, win(c->wacc),
-                         wr ? c->rres[slot] : nullptr, wr ? c->cres[slot] : nullptr,
-                         wr ? (const int32_t*)c->row_cm : nullptr, wr ? (slot ? c->df.wrow1 : c->df.wrow) : nullptr,
-                         wr ? (slot ? c->df.wcm1 : c->df.wcm) : nullptr);
+                         wr ? c->rres[slot] : nullptr, wc ? c->cres[slot] : nullptr,
+                         wc ? (const int32_t*)c->row_cm : nullptr, wr ? (slot ? c->df.wrow1 : c->df.wrow) : nullptr,
+                         wc ? (slot ? c->df.wcm1 : c->df.wcm) : nullptr);
     MBA_TRY(sum_async(nb, kSlotCost));
     MBA_TRY(flags_async(1));
     return clear_flags();
+  }
+  // Camera-major copies of slot `slot`'s row-order residuals and weights.
+  void cm_copies(int slot) {
+    if (!c->rfix || c->nrows == 0) return;
+    hipLaunchKernelGGL(k_rows_to_cm, dim3(nblocks((int64_t)c->nrows)), dim3(kBlock), 0, s, (int64_t)c->nrows,
+                       (const int32_t*)c->row_cm, (const float*)c->rres[slot],
+                       (const float*)(slot ? c->df.wrow1 : c->df.wrow), c->cres[slot], slot ? c->df.wcm1 : c->df.wcm);
   }
   void cost_result(std::vector<double>* cost, std::vector<uint8_t>* finite) const {
     cost->assign(W, 0.0);
@@ -3262,6 +3271,7 @@ struct Engine {
       std::swap(c->cams64, c->cams64_t);
       std::swap(c->pts64, c->pts64_t);
       c->rcur = 1 - c->rcur;  // the trial state's residuals are now current
+      cm_copies(c->rcur);
       // Keep the constant blocks of the trial copy in sync.
       HIP_TRY(hipMemcpyAsync(c->cams64_t, c->cams64, sizeof(double) * 9 * c->nslot, hipMemcpyDeviceToDevice, s));
       return MBA_OK;
@@ -3431,7 +3441,7 @@ struct Engine {
       HIP_TRY(hipEventRecord(c->tev[1], s));
       MBA_TRY(model_cost_async());
       MBA_TRY(plus_async());
-      MBA_TRY(cost64_async(c->cams64_t, c->pts64_t, 1 - c->rcur));
+      MBA_TRY(cost64_async(c->cams64_t, c->pts64_t, 1 - c->rcur, false));
       HIP_TRY(hipEventRecord(c->tev[2], s));
       MBA_TRY(sync());
       t_lin += elapsed_s(c->tev[0], c->tev[1]);
