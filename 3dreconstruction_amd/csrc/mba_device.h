@@ -1350,6 +1350,7 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
     T* __restrict__ rhs, T* __restrict__ Minv, const double* __restrict__ Tsj) {
   __shared__ double sum[64];
   __shared__ int ij[64];
+  __shared__ double lf[9][9];
   const int f = blockIdx.x;
   const int k = threadIdx.x;
   if (win_idle_f(win, f)) return;
@@ -1376,17 +1377,25 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
   else if (k < 63 && with_h) rhs[9 * (size_t)f + (k - 54)] = vt;
   sum[k] = (double)vt;
   __syncthreads();
-  // Lanes 0..8: each factors U + D^2 (FP64, redundantly) and writes one
-  // column of its inverse.
-  if (k >= 9) return;
+  // Lanes 0..8 factor U + D^2 (FP64) together, lane i holding row i of the
+  // factor: the same operations in the same order as chol9, the rows'
+  // dot products and divisions in parallel, each finished column published
+  // in LDS.  Then lane c writes column c of the inverse (chol9_inv_col).
+  // Lanes 9..63 mirror row 8 without writing, so every lane reaches the
+  // barriers.
+  const int row = k < 9 ? k : 8;
   const int wi = win_of_f(win, f);
   const double inv_radius = win_inv_r(win, wi);
   // The block to invert: U, or with SCHUR_JACOBI the diagonal block of S
   // (Tsj); plus the LM diagonal, taken from the Jacobian's columns (U's
   // diagonal) in either case.
-  double a[45];
+  // Row `row` of that block (stored as its upper triangle, packed).
+  double arow[9];
 #pragma unroll
-  for (int q = 0; q < 45; ++q) a[q] = Tsj ? Tsj[45 * (size_t)f + q] : sum[q];
+  for (int j = 0; j < 9; ++j) {
+    const int idx = j < row ? ut9(j, row) : ut9(row, j);
+    arow[j] = Tsj ? Tsj[45 * (size_t)f + idx] : sum[idx];
+  }
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     double dg;
@@ -1396,10 +1405,33 @@ __global__ __launch_bounds__(64) void k_cam_finalize(
     } else {
       dg = diag_c[9 * (size_t)f + i];
     }
-    a[ut9(i, i)] += dg * inv_radius;
+    if (i == row) arow[i] += dg * inv_radius;
   }
-  double l[45], e[9];
-  if (chol9(a, l)) {
+  double lr[9];  // row `row` of the factor, columns < j so far
+#pragma unroll
+  for (int q = 0; q < 9; ++q) lr[q] = 0.0;
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    // Row `row`'s entry in column j before the division (lane j: the pivot).
+    double sj = arow[j];
+#pragma unroll
+    for (int q = 0; q < j; ++q) sj -= lr[q] * lf[j][q];
+    double d = lane_bcast(sj, j);
+    ok = ok && d > 0.0;
+    d = ok ? sqrt(d) : 1.0;
+    lr[j] = row == j ? d : sj / d;
+    if (k < 9 && row >= j) lf[row][j] = lr[j];
+    __syncthreads();
+  }
+  if (k >= 9) return;
+  double e[9];
+  if (ok) {
+    double l[45];
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+      for (int q = 0; q <= i; ++q) l[lt9(i, q)] = lf[i][q];
     chol9_inv_col(l, k, e);
   } else {
     if (k == 0) win.flag[wi] = 1;
